@@ -1,0 +1,167 @@
+#!/usr/bin/env python3
+"""Benchmark of the SPH step on one MI355X.
+
+Workload: the 3-D dam break at dp=0.0025 m (5,145,600 fluid + 934,016 DBC wall
+particles; Wendland C2, Tait EOS, artificial viscosity 0.01, Fourtakas
+delta-SPH 0.1 (DDT2Full), Symplectic, CFL 0.2), synthetic particles from the
+case generator. A step is one Symplectic time step (two interactions, two
+neighbour-list divides, predictor and corrector), dt kept on the device.
+
+Metric: pair interactions/s, counted with the reference's rule (JDsPips: real
+pairs r^2 <= (2h)^2, r^2 >= 1e-18; fluid particles against fluid and boundary,
+boundary particles against fluid; two interactions per Symplectic step),
+averaged over the first and last state of the timed window. steps/s reported
+alongside.
+
+Prints one JSON line on stdout; progress on stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+# Algorithmic FP32 work per real pair (FMA = 2, div = 1, add/mul/cmp/max = 1;
+# transcendentals not counted): fluid-fluid, fluid-boundary, boundary-fluid.
+F_FF, F_FB, F_BF = 79, 61, 39
+PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_times(lib, ctx):
+    import ctypes as C
+    lib.dsph_ctx_timing_collect(ctx)
+    out = {}
+    i = 0
+    name = C.create_string_buffer(128)
+    cnt = C.c_uint64()
+    ms = C.c_double()
+    while lib.dsph_ctx_timing_get(ctx, i, name, 128, C.byref(cnt), C.byref(ms)) == 0:
+        out[name.value.decode()] = (int(cnt.value), float(ms.value))
+        i += 1
+    return out
+
+
+def cpu_baseline(case, steps: int):
+    """The oracle (C++/OpenMP restatement of the reference CPU path) timed on
+    a bounded sample of the same workload: `steps` Symplectic steps of the same
+    particles, pair rule as above."""
+    from oracle.oracle import OracleSim
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    o = OracleSim(case)
+    tset = time.perf_counter() - t0
+    p0 = o.pips()
+    t0 = time.perf_counter()
+    o.run(steps)
+    dt = time.perf_counter() - t0
+    p1 = o.pips()
+    pairs = 2 * steps * ((p0[0] + p0[1]) + (p1[0] + p1[1])) / 2
+    return {"value": pairs / dt, "unit": "pair interactions/s", "cores": threads, "kind": "port",
+            "sample": f"{steps} Symplectic steps (+divides) of the same {case.np}-particle dam "
+                      f"break on the host cores, {dt:.1f} s (setup {tset:.1f} s not timed)",
+            "steps_per_s": steps / dt}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--dp", type=float, default=0.0025)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build runs the SPH step on one GPU only; "
+                          "multi-GPU sharding is a design note (x-slab halo exchange), "
+                          "not implemented", "n_gpus": a.gpus}))
+        sys.exit(2)
+
+    import torch
+    from dualsphysics_amd import case as casemod
+    from dualsphysics_amd import _native as N
+    from dualsphysics_amd.sim import SphGpuSingle
+
+    t0 = time.perf_counter()
+    c = casemod.dam_break(dp=a.dp)
+    log(f"case: {c.nfluid} fluid + {c.nbound} boundary particles ({time.perf_counter()-t0:.1f}s)")
+    sim = SphGpuSingle(c)
+    lib = N.load()
+    ctx = lib.dsph_sim_ctx(sim._h)
+    log("warmup ...")
+    sim.run(a.warmup)
+    torch.cuda.synchronize()
+    p0 = sim.pips()
+    if not a.no_timing:
+        lib.dsph_ctx_timing_reset(ctx)
+        lib.dsph_ctx_set_timing(ctx, 1)
+    log(f"timing {a.steps} steps ...")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    chunk = max(1, min(a.steps, 250))
+    while done < a.steps:
+        n = min(chunk, a.steps - done)
+        sim.run(n)
+        done += n
+        log(f"  {done}/{a.steps} steps")
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    kt = kernel_times(lib, ctx) if not a.no_timing else {}
+    lib.dsph_ctx_set_timing(ctx, 0)
+    p1 = sim.pips()
+    st = sim.stats()
+    pairs_step = 2 * ((p0[0] + p0[1]) + (p1[0] + p1[1])) / 2
+    value = pairs_step * a.steps / elapsed
+    ms_step = 1e3 * elapsed / a.steps
+
+    roof = None
+    if "k_forces_fluid" in kt:
+        cnt, ms = kt["k_forces_fluid"]
+        ff = ((p0[0] - p0[4]) + (p1[0] - p1[4])) / 2
+        fb = (p0[4] + p1[4]) / 2
+        flops = ff * F_FF + fb * F_FB
+        avg_s = ms / cnt / 1e3
+        ach = flops / avg_s / 1e12
+        roof = {"bound": "valu", "kernel": "k_forces_fluid", "achieved": round(ach, 3),
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
+                "traffic": None, "avg_launch_ms": round(ms / cnt, 4), "launches": cnt,
+                "flop_per_launch": flops}
+    cpu = None
+    if not a.no_cpu_baseline:
+        del sim
+        log(f"cpu baseline: {a.cpu_steps} steps of the oracle ...")
+        cpu = cpu_baseline(c, a.cpu_steps)
+    out = {
+        "metric": "pair interactions/s + steps/s at 5M particles; ensemble cases/s vs plain",
+        "value": value, "unit": "pair interactions/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_step, "steps_per_s": 1e3 / ms_step,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"dam_break_3d_dp{a.dp:g}", "n_fluid": c.nfluid,
+                   "n_bound": c.nbound, "dp": a.dp, "kernel": "wendland",
+                   "visco": "artificial 0.01", "ddt": "fourtakas_full 0.1",
+                   "step": "symplectic", "cfl": 0.2, "pairs_per_step": pairs_step,
+                   "sim_time_s": st["timestep"]},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3),
+                           "avg_ms": round(v[1] / max(v[0], 1), 5)} for k, v in kt.items()},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

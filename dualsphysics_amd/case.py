@@ -1,0 +1,185 @@
+"""Case front-end: box dam-break lattice generator and execution parameters.
+
+GenCase (the reference's closed case generator) is not available, so the
+initial particles are defined here and fed identically to every solver that
+is compared (this package, the oracle, the reference CPU build):
+
+- fluid lattice ``x=(i+1/2)dp`` filling ``[0,Lx_col]x[0,Ly]x[0,H]``;
+- fixed DBC walls: ``layers`` lattice layers outside the tank interior
+  ``[0,Lx]x[0,Ly]x[0,Lz]`` on the bottom and the four sides (open top);
+- every coordinate jittered by ``U(-1e-4 dp, +1e-4 dp)`` from SplitMix64
+  (seed 42, one draw per component in particle-id order, boundary first);
+- hydrostatic fluid density ``rho0 (1 + rho0 g (H - z)/B)^(1/gamma)``,
+  boundary ``rho0``, zero velocity.
+
+Constants follow the reference's rules (JCaseCtes.cpp:33-44, 298-302):
+``h = coefh sqrt(3 dp^2)``, ``c0 = coefsound sqrt(g hswl)``,
+``B = c0^2 rho0 / gamma``, ``m = rho0 dp^3``; the simulation domain is the
+particle bounding box widened by ``0.05 h`` (JSph.cpp:2174-2182,
+JPartsLoad4.cpp:378-399) with ``posmax.z = default + 50%``
+(ResizeMapLimits, JSph.cpp:1445-1476).
+"""
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+
+from . import _native as N
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """``count`` successive SplitMix64 outputs (Steele, Lea & Flood 2014)."""
+    with np.errstate(over="ignore"):
+        idx = np.arange(1, count + 1, dtype=np.uint64)
+        z = np.uint64(seed) + idx * np.uint64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform01(seed: int, count: int) -> np.ndarray:
+    """Doubles in [0,1) from the top 53 bits of SplitMix64."""
+    return (splitmix64(seed, count) >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+@dataclasses.dataclass
+class Parameters:
+    """The ``<execution><parameters>`` subset of the solver (JSph.cpp:663-854)."""
+    tstep: int = 2                 # StepAlgorithm 1 Verlet, 2 Symplectic
+    verletsteps: int = 40
+    tkernel: int = N.KERNEL_WENDLAND
+    tvisco: int = N.VISCO_ARTIFICIAL
+    visco: float = 0.01
+    viscoboundfactor: float = 1.0
+    tdensity: int = N.DDT_DDT2FULL
+    ddtvalue: float = 0.1
+    tboundary: int = N.BC_DBC
+    cflnumber: float = 0.2
+    dtini: float = 0.0
+    dtmin: float = 0.0
+    rhopoutmin: float = 700.0
+    rhopoutmax: float = 1300.0
+    cellmode: int = 1              # 1 Full, 2 Half
+    celldomfixed: int = 0
+    timemax: float = 1.0
+
+
+@dataclasses.dataclass
+class Case:
+    name: str
+    dp: float
+    h: float
+    rhop0: float
+    gamma: float
+    cs0: float
+    b: float
+    massfluid: float
+    massbound: float
+    gravity: tuple
+    hswl: float
+    params: Parameters
+    idp: np.ndarray        # uint32 [np], boundary ids first
+    pos: np.ndarray        # float64 [np,3]
+    velrho: np.ndarray     # float32 [np,4]
+    nbound: int
+    nfluid: int
+    maprealposmin: tuple
+    maprealposmax: tuple
+    boundnor: np.ndarray | None = None   # float32 [nbound,3] (mDBC), not doubled
+
+    @property
+    def np(self) -> int:
+        return int(self.idp.shape[0])
+
+    def to_native(self) -> N.DsphCase:
+        c = N.DsphCase()
+        p = self.params
+        c.dp, c.kernelh, c.cteb, c.gamma = self.dp, self.h, self.b, self.gamma
+        c.rhopzero, c.cs0 = self.rhop0, self.cs0
+        c.massfluid, c.massbound = self.massfluid, self.massbound
+        for k in range(3):
+            c.gravity[k] = self.gravity[k]
+            c.maprealposmin[k] = self.maprealposmin[k]
+            c.maprealposmax[k] = self.maprealposmax[k]
+        c.cflnumber, c.dtini, c.dtmin = p.cflnumber, p.dtini, p.dtmin
+        c.tstep, c.verletsteps, c.tkernel = p.tstep, p.verletsteps, p.tkernel
+        c.tvisco, c.tdensity, c.tboundary = p.tvisco, p.tdensity, p.tboundary
+        c.visco, c.viscoboundfactor, c.ddtvalue = p.visco, p.viscoboundfactor, p.ddtvalue
+        c.rhopoutmin, c.rhopoutmax = p.rhopoutmin, p.rhopoutmax
+        c.cellmode, c.celldomfixed = p.cellmode, p.celldomfixed
+        c.casenp, c.casenbound, c.casenfixed = self.np, self.nbound, self.nbound
+        c.casenmoving, c.casenfloat, c.casenfluid = 0, 0, self.nfluid
+        return c
+
+
+def _lattice(nx0, nx1, ny0, ny1, nz0, nz1, dp):
+    """Lattice points ((i+1/2)dp, (j+1/2)dp, (k+1/2)dp), x fastest then y, z."""
+    k, j, i = np.meshgrid(np.arange(nz0, nz1), np.arange(ny0, ny1), np.arange(nx0, nx1),
+                          indexing="ij")
+    pts = np.stack([(i.ravel() + 0.5) * dp, (j.ravel() + 0.5) * dp, (k.ravel() + 0.5) * dp], 1)
+    return pts, np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+
+
+def _n(length, dp):
+    return int(math.floor(length / dp + 0.5))
+
+
+def map_limits(pos: np.ndarray, h: float, zextra: float = 0.5):
+    """Map limits of the simulation domain from the particles (see module doc)."""
+    border = float(np.float32(h)) * 0.05
+    pmin = pos.min(0) - border
+    pmax = pos.max(0) + border
+    pmax = pmax.copy()
+    pmax[2] = pmax[2] + (pmax[2] - pmin[2]) * zextra
+    return tuple(float(v) for v in pmin), tuple(float(v) for v in pmax)
+
+
+def dam_break(dp: float = 0.01, tank=(1.6, 0.67, 0.4), column=(0.4, 0.67, 0.3), layers: int = 2,
+              coefh: float = 1.2, coefsound: float = 20.0, rhop0: float = 1000.0,
+              gamma: float = 7.0, g: float = 9.81, seed: int = 42, jitter: float = 1e-4,
+              params: Parameters | None = None, name: str | None = None) -> Case:
+    """3-D dam break in a box tank (the reference's CaseDambreak geometry without
+    the obstacle): column ``column`` at the x=0 end of tank ``tank``."""
+    params = params or Parameters()
+    nxt, nyt, nzt = _n(tank[0], dp), _n(tank[1], dp), _n(tank[2], dp)
+    nxf, nyf, nzf = _n(column[0], dp), _n(column[1], dp), _n(column[2], dp)
+    # Walls: every lattice point of the widened box outside the interior.
+    wpts, widx = _lattice(-layers, nxt + layers, -layers, nyt + layers, -layers, nzt, dp)
+    inside = ((widx[:, 0] >= 0) & (widx[:, 0] < nxt) & (widx[:, 1] >= 0) & (widx[:, 1] < nyt) &
+              (widx[:, 2] >= 0))
+    wpts = wpts[~inside]
+    fpts, _ = _lattice(0, nxf, 0, nyf, 0, nzf, dp)
+    nb, nf = wpts.shape[0], fpts.shape[0]
+    pos = np.concatenate([wpts, fpts], 0)
+    u = uniform01(seed, 3 * pos.shape[0]).reshape(-1, 3)
+    pos = pos + (2.0 * u - 1.0) * (jitter * dp)
+    h = coefh * math.sqrt(3.0 * dp * dp)
+    hswl = column[2]
+    cs0 = coefsound * math.sqrt(g * hswl)
+    b = cs0 * cs0 * rhop0 / gamma
+    mass = rhop0 * dp ** 3
+    velrho = np.zeros((pos.shape[0], 4), np.float32)
+    velrho[:nb, 3] = rhop0
+    zf = pos[nb:, 2]
+    velrho[nb:, 3] = (rhop0 * (1.0 + rhop0 * g * (hswl - zf) / b) ** (1.0 / gamma)).astype(np.float32)
+    idp = np.arange(pos.shape[0], dtype=np.uint32)
+    mn, mx = map_limits(pos, h)
+    return Case(name=name or f"dambreak_dp{dp:g}", dp=dp, h=h, rhop0=rhop0, gamma=gamma,
+                cs0=cs0, b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g),
+                hswl=hswl, params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb,
+                nfluid=nf, maprealposmin=mn, maprealposmax=mx)
+
+
+def dam_break_cfg0(**kw) -> Case:
+    """3-D dam break at dp=0.01 m: 80,400 fluid + 60,248 wall particles."""
+    return dam_break(dp=0.01, **kw)
+
+
+def dam_break_cfg1(**kw) -> Case:
+    """3-D dam break at dp=0.0025 m: 5,145,600 fluid + 934,016 wall particles."""
+    return dam_break(dp=0.0025, **kw)

@@ -1,0 +1,566 @@
+// Cell-linked neighbour list on gfx950: cell limits, cell keys, stable radix
+// sort by cell, BeginEndCell ranges, reorder of particle arrays, PosCell.
+//
+// Semantics follow the reference's single-GPU divide (JCellDivGpuSingle.cpp:
+// 152-236, JCellDivGpuSingle_ker.cu:41-165, JCellDivGpu_ker.cu:120-127,
+// 436-489, 533-712) with the reference's -stable ordering: equal keys keep
+// their current array order, so the permutation equals the CPU counting sort
+// (JCellDivCpuSingle.cpp:207-262).
+#include <climits>
+#include <cstring>
+#include "dsph_ctx.h"
+
+using namespace dsph;
+
+namespace {
+
+constexpr unsigned kBs = 256;
+
+// ------------------------------------------------------------- cell limits
+// Min/max cell (x,y,z) over non-excluded particles, combined with 32-bit
+// integer atomics (exact and order independent). res[0..2]=min, res[3..5]=max.
+__global__ void k_limits_init(unsigned* res) {
+  const unsigned t = threadIdx.x;
+  if (t < 3) res[t] = UINT_MAX;
+  else if (t < 6) res[t] = 0u;
+}
+
+__global__ void __launch_bounds__(kBs) k_limits(unsigned n, unsigned pini, unsigned cellcode,
+                                                const unsigned* __restrict__ dcell,
+                                                const typecode* __restrict__ code,
+                                                unsigned* __restrict__ res) {
+  __shared__ unsigned smin[3][kBs / 64], smax[3][kBs / 64];
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned mn[3] = {UINT_MAX, UINT_MAX, UINT_MAX}, mx[3] = {0u, 0u, 0u};
+  if (p < n) {
+    const unsigned pp = p + pini;
+    if (code_special(code[pp]) < kCodeOutIgnore) {
+      const unsigned c = dcell[pp];
+      const unsigned cx = dcel_cellx(cellcode, c), cy = dcel_celly(cellcode, c),
+                     cz = dcel_cellz(cellcode, c);
+      mn[0] = mx[0] = cx; mn[1] = mx[1] = cy; mn[2] = mx[2] = cz;
+    }
+  }
+  for (int k = 0; k < 3; k++) {
+    for (int off = 32; off > 0; off >>= 1) {
+      mn[k] = min(mn[k], (unsigned)__shfl_xor((int)mn[k], off));
+      mx[k] = max(mx[k], (unsigned)__shfl_xor((int)mx[k], off));
+    }
+  }
+  const unsigned w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  if (lane == 0)
+    for (int k = 0; k < 3; k++) { smin[k][w] = mn[k]; smax[k][w] = mx[k]; }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    unsigned a = UINT_MAX, b = 0;
+    for (unsigned i = 0; i < blockDim.x / 64; i++) {
+      a = min(a, smin[threadIdx.x][i]);
+      b = max(b, smax[threadIdx.x][i]);
+    }
+    if (a != UINT_MAX) atomicMin(res + threadIdx.x, a);
+    if (b != 0u) atomicMax(res + 3 + threadIdx.x, b);
+  }
+}
+
+// ------------------------------------------------------------- cell keys
+// Box layout of JCellDivGpuSingle::PrepareNct (JCellDivGpuSingle.cpp:109-128):
+// [bound cells 0..nct) | BoundIgnore=nct | fluid cells nct+1.. | BoundOut |
+// FluidOut | BoundOutIgnore | FluidOutIgnore.
+__global__ void __launch_bounds__(kBs) k_presort_full(unsigned np, unsigned cellcode,
+    const unsigned* __restrict__ dcell, const typecode* __restrict__ code,
+    uint3 cellzero, uint3 ncells, unsigned* __restrict__ cellpart,
+    unsigned* __restrict__ sortpart) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  sortpart[p] = p;
+  const unsigned nsheet = ncells.x * ncells.y;
+  const unsigned boxboundignore = nsheet * ncells.z;
+  const unsigned boxfluid = boxboundignore + 1;
+  const unsigned boxboundout = boxfluid + boxboundignore;
+  const unsigned boxfluidout = boxboundout + 1;
+  const unsigned boxboundoutignore = boxfluidout + 1;
+  const unsigned boxfluidoutignore = boxboundoutignore + 1;
+  const unsigned c = dcell[p];
+  const unsigned cx = dcel_cellx(cellcode, c) - cellzero.x;
+  const unsigned cy = dcel_celly(cellcode, c) - cellzero.y;
+  const unsigned cz = dcel_cellz(cellcode, c) - cellzero.z;
+  const unsigned key = cx + cy * ncells.x + cz * nsheet;
+  const unsigned rc = code[p];
+  const unsigned type = code_type(rc), out = code_special(rc);
+  unsigned box;
+  if (type < kCodeTypeFloating) {
+    box = (out < kCodeOutIgnore)
+              ? ((cx < ncells.x && cy < ncells.y && cz < ncells.z) ? key : boxboundignore)
+              : (out == kCodeOutIgnore ? boxboundoutignore : boxboundout);
+  } else {
+    box = (out <= kCodeOutIgnore) ? (out < kCodeOutIgnore ? boxfluid + key : boxfluidoutignore)
+                                  : (type == kCodeTypeFloating ? boxboundout : boxfluidout);
+  }
+  cellpart[p] = box;
+}
+
+__global__ void __launch_bounds__(kBs) k_presort_fluid(unsigned n, unsigned pini,
+    unsigned cellcode, const unsigned* __restrict__ dcell,
+    const typecode* __restrict__ code, uint3 cellzero, uint3 ncells,
+    unsigned* __restrict__ cellpart, unsigned* __restrict__ sortpart) {
+  unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  p += pini;
+  sortpart[p] = p;
+  const unsigned nsheet = ncells.x * ncells.y;
+  const unsigned boxfluid = nsheet * ncells.z + 1;
+  const unsigned boxfluidout = boxfluid + boxfluid;
+  const unsigned boxboundout = boxfluidout - 1;
+  const unsigned boxfluidoutignore = boxfluidout + 2;
+  const unsigned c = dcell[p];
+  const unsigned cx = dcel_cellx(cellcode, c) - cellzero.x;
+  const unsigned cy = dcel_celly(cellcode, c) - cellzero.y;
+  const unsigned cz = dcel_cellz(cellcode, c) - cellzero.z;
+  const unsigned rc = code[p];
+  const unsigned type = code_type(rc), out = code_special(rc);
+  cellpart[p] = (out <= kCodeOutIgnore)
+                    ? (out < kCodeOutIgnore ? boxfluid + cx + cy * ncells.x + cz * nsheet
+                                            : boxfluidoutignore)
+                    : (type == kCodeTypeFloating ? boxboundout : boxfluidout);
+}
+
+// ------------------------------------------------------------- radix sort
+// LSD radix sort of (key,value) pairs, 8-bit digits, tiles of 4096 keys per
+// 256-thread workgroup (4 waves x 1024 keys). Stability inside a tile comes
+// from ranking keys in tile order: each wave walks its 1024-key segment in 16
+// rounds of 64 contiguous keys, finds the lanes sharing its digit with one
+// ballot per digit bit, and keeps per-wave digit counters in LDS. A per-digit
+// scan over waves and over tiles (digit-major histogram) gives each key its
+// global slot.
+constexpr unsigned kTileItems = 16;
+constexpr unsigned kTile = kBs * kTileItems;  // 4096
+
+__global__ void __launch_bounds__(kBs) k_radix_hist(const unsigned* __restrict__ keys,
+    unsigned n, unsigned shift, unsigned mask, unsigned ntiles, unsigned* __restrict__ hist) {
+  __shared__ unsigned cnt[256];
+  const unsigned tile = blockIdx.x;
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const unsigned base = tile * kTile;
+#pragma unroll
+  for (unsigned i = 0; i < kTileItems; i++) {
+    const unsigned idx = base + i * kBs + threadIdx.x;
+    if (idx < n) atomicAdd(&cnt[(keys[idx] >> shift) & mask], 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x <= mask) hist[threadIdx.x * ntiles + tile] = cnt[threadIdx.x];
+}
+
+// Exclusive scan over tiles of one digit (one workgroup per digit); digit
+// totals go to totals[digit].
+__global__ void __launch_bounds__(kBs) k_radix_scan_digit(unsigned* __restrict__ hist,
+    unsigned ntiles, unsigned* __restrict__ totals) {
+  __shared__ unsigned wsum[kBs / 64];
+  unsigned* h = hist + size_t(blockIdx.x) * ntiles;
+  const unsigned per = (ntiles + kBs - 1) / kBs;
+  const unsigned beg = threadIdx.x * per;
+  const unsigned end = min(beg + per, ntiles);
+  unsigned s = 0;
+  for (unsigned i = beg; i < end; i++) s += h[i];
+  // Block exclusive scan of s.
+  const unsigned lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  unsigned incl = s;
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned v = __shfl_up((int)incl, off);
+    if (lane >= (unsigned)off) incl += v;
+  }
+  if (lane == 63) wsum[w] = incl;
+  __syncthreads();
+  unsigned wbase = 0;
+  for (unsigned i = 0; i < w; i++) wbase += wsum[i];
+  unsigned run = wbase + incl - s;
+  for (unsigned i = beg; i < end; i++) {
+    const unsigned v = h[i];
+    h[i] = run;
+    run += v;
+  }
+  if (threadIdx.x == kBs - 1) totals[blockIdx.x] = wbase + incl;
+}
+
+__global__ void __launch_bounds__(kBs) k_radix_scatter(const unsigned* __restrict__ kin,
+    const unsigned* __restrict__ vin, unsigned* __restrict__ kout, unsigned* __restrict__ vout,
+    unsigned n, unsigned shift, unsigned bits, unsigned ntiles,
+    const unsigned* __restrict__ hist, const unsigned* __restrict__ totals) {
+  __shared__ unsigned dbase[256];
+  __shared__ unsigned wcnt[kBs / 64][256];
+  const unsigned mask = (1u << bits) - 1u;
+  const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+  const unsigned tile = blockIdx.x;
+  // Global base of every digit for this tile: exclusive scan of totals over
+  // digits plus the tile offset within the digit.
+  {
+    const unsigned t = (tid <= mask) ? totals[tid] : 0u;
+    unsigned incl = t;
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned v = __shfl_up((int)incl, off);
+      if (lane >= (unsigned)off) incl += v;
+    }
+    __shared__ unsigned ws[kBs / 64];
+    if (lane == 63) ws[w] = incl;
+    for (unsigned d = tid; d < 256; d += kBs) wcnt[0][d] = 0;
+    __syncthreads();
+    unsigned wb = 0;
+    for (unsigned i = 0; i < w; i++) wb += ws[i];
+    if (tid <= mask) dbase[tid] = wb + incl - t + hist[tid * ntiles + tile];
+#pragma unroll
+    for (unsigned k = 1; k < kBs / 64; k++) wcnt[k][tid] = 0;
+  }
+  __syncthreads();
+  volatile unsigned* cnt = wcnt[w];
+  const unsigned seg = tile * kTile + w * (kTile / (kBs / 64));
+  unsigned key[kTileItems], val[kTileItems], rank[kTileItems];
+  const unsigned long long lanelt = (1ull << lane) - 1ull;
+#pragma unroll
+  for (unsigned r = 0; r < kTileItems; r++) {
+    const unsigned idx = seg + r * 64 + lane;
+    const bool valid = idx < n;
+    unsigned k = 0, v = 0;
+    if (valid) { k = kin[idx]; v = vin[idx]; }
+    const unsigned d = (k >> shift) & mask;
+    unsigned long long peers = __ballot(valid);
+    for (unsigned b = 0; b < bits; b++) {
+      const bool bit = (d >> b) & 1u;
+      const unsigned long long bal = __ballot(valid && bit);
+      peers &= bit ? bal : ~bal;
+    }
+    const unsigned before = (unsigned)__popcll(peers & lanelt);
+    unsigned rk = 0;
+    if (valid) rk = cnt[d] + before;
+    // The highest lane of each peer group advances the digit counter.
+    const bool leader = valid && ((peers >> lane) == 1ull);
+    if (leader) cnt[d] = rk + 1;
+    key[r] = k; val[r] = v; rank[r] = valid ? rk : 0xffffffffu;
+  }
+  __syncthreads();
+  // Exclusive scan of the per-wave digit counts across waves.
+  for (unsigned d = tid; d <= mask; d += kBs) {
+    unsigned s = 0;
+#pragma unroll
+    for (unsigned k = 0; k < kBs / 64; k++) {
+      const unsigned c = wcnt[k][d];
+      wcnt[k][d] = s;
+      s += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (unsigned r = 0; r < kTileItems; r++) {
+    if (rank[r] != 0xffffffffu) {
+      const unsigned d = (key[r] >> shift) & mask;
+      const unsigned pos = dbase[d] + wcnt[w][d] + rank[r];
+      kout[pos] = key[r];
+      vout[pos] = val[r];
+    }
+  }
+}
+
+// ------------------------------------------------------------- BeginEndCell
+__global__ void k_fill_int2(int2* __restrict__ a, unsigned n, int2 v) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) a[p] = v;
+}
+
+__global__ void __launch_bounds__(kBs) k_begin_end(unsigned n, unsigned pini,
+    const unsigned* __restrict__ cellpart, int2* __restrict__ begcell) {
+  const unsigned pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= n) return;
+  const unsigned p = pt + pini;
+  const unsigned cel = cellpart[p];
+  if (pt == 0) {
+    begcell[cel].x = int(p);
+  } else {
+    const unsigned prev = cellpart[p - 1];
+    if (cel != prev) {
+      begcell[cel].x = int(p);
+      begcell[prev].y = int(p);
+    }
+  }
+  if (pt == n - 1) begcell[cel].y = int(p + 1);
+}
+
+// ------------------------------------------------------------- gathers
+__device__ inline float4 poscell_of(double x, double y, double z, double3 posmin, float pcs) {
+  const double dx = x - posmin.x, dy = y - posmin.y, dz = z - posmin.z;
+  const unsigned cx = unsigned(dx / pcs), cy = unsigned(dy / pcs), cz = unsigned(dz / pcs);
+  const float px = float(dx - double(pcs) * cx);
+  const float py = float(dy - double(pcs) * cy);
+  const float pz = float(dz - double(pcs) * cz);
+  return make_float4(px, py, pz, __uint_as_float(pscel_code(cx, cy, cz)));
+}
+
+template <bool kPosCell>
+__global__ void __launch_bounds__(kBs) k_sort_basic(unsigned n, unsigned pini,
+    const unsigned* __restrict__ sortpart, const unsigned* __restrict__ idp,
+    const typecode* __restrict__ code, const unsigned* __restrict__ dcell,
+    const double2* __restrict__ posxy, const double* __restrict__ posz,
+    const float4* __restrict__ velrho, unsigned* __restrict__ idp2,
+    typecode* __restrict__ code2, unsigned* __restrict__ dcell2,
+    double2* __restrict__ posxy2, double* __restrict__ posz2,
+    float4* __restrict__ velrho2, double3 posmin, float pcs,
+    float4* __restrict__ poscell) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const unsigned o = (p < pini ? p : sortpart[p]);
+  idp2[p] = idp[o];
+  code2[p] = code[o];
+  dcell2[p] = dcell[o];
+  const double2 xy = posxy[o];
+  const double z = posz[o];
+  posxy2[p] = xy;
+  posz2[p] = z;
+  velrho2[p] = velrho[o];
+  if (kPosCell) poscell[p] = poscell_of(xy.x, xy.y, z, posmin, pcs);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kBs) k_sort_one(unsigned n, unsigned pini,
+    const unsigned* __restrict__ sortpart, const T* __restrict__ a, T* __restrict__ a2) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  a2[p] = a[p < pini ? p : sortpart[p]];
+}
+
+struct Sym6 { float v[6]; };
+struct F3 { float x, y, z; };
+
+__global__ void __launch_bounds__(kBs) k_sort_d2_d_f4(unsigned n, unsigned pini,
+    const unsigned* __restrict__ sortpart, const double2* __restrict__ a,
+    const double* __restrict__ b, const float4* __restrict__ c, double2* __restrict__ a2,
+    double* __restrict__ b2, float4* __restrict__ c2) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const unsigned o = (p < pini ? p : sortpart[p]);
+  a2[p] = a[o];
+  b2[p] = b[o];
+  c2[p] = c[o];
+}
+
+__global__ void __launch_bounds__(kBs) k_update_poscell(unsigned np, double3 posmin, float pcs,
+    const double2* __restrict__ posxy, const double* __restrict__ posz,
+    float4* __restrict__ poscell) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const double2 xy = posxy[p];
+  poscell[p] = poscell_of(xy.x, xy.y, posz[p], posmin, pcs);
+}
+
+}  // namespace
+
+extern "C" {
+
+int dsph_cell_limits_sync(dsph_ctx* ctx, uint32_t np, uint32_t pini, uint32_t cellcode,
+                          const uint32_t* d_dcell, const uint16_t* d_code,
+                          uint32_t h_celmin[3], uint32_t h_celmax[3]) {
+  if (!ctx || !h_celmin || !h_celmax) return DSPH_E_ARG;
+  if (!np) {
+    for (int k = 0; k < 3; k++) { h_celmin[k] = UINT_MAX; h_celmax[k] = 0; }
+    return DSPH_OK;
+  }
+  DSPH_REQUIRE(ctx, d_dcell && d_code, "dsph_cell_limits_sync: null array");
+  KTimer kt(ctx, "k_limits");
+  unsigned* res = ctx->small_dev;
+  k_limits_init<<<1, 64, 0, ctx->stream>>>(res);
+  k_limits<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(np, pini, cellcode, d_dcell, d_code, res);
+  DSPH_LAUNCHED(ctx, "k_limits");
+  unsigned* h = static_cast<unsigned*>(ctx->pinned);
+  DSPH_HIP(ctx, hipMemcpyAsync(h, res, 6 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
+  DSPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  for (int k = 0; k < 3; k++) { h_celmin[k] = h[k]; h_celmax[k] = h[3 + k]; }
+  return DSPH_OK;
+}
+
+int dsph_presort_full(dsph_ctx* ctx, uint32_t np, uint32_t cellcode, const uint32_t* d_dcell,
+                      const uint16_t* d_code, const uint32_t cellmin[3],
+                      const uint32_t ncells[3], uint32_t* d_cellpart, uint32_t* d_sortpart) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_dcell && d_code && d_cellpart && d_sortpart && cellmin && ncells,
+               "dsph_presort_full: null argument");
+  KTimer kt(ctx, "k_presort");
+  k_presort_full<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
+      np, cellcode, d_dcell, d_code, make_uint3(cellmin[0], cellmin[1], cellmin[2]),
+      make_uint3(ncells[0], ncells[1], ncells[2]), d_cellpart, d_sortpart);
+  DSPH_LAUNCHED(ctx, "k_presort_full");
+  return DSPH_OK;
+}
+
+int dsph_presort_fluid(dsph_ctx* ctx, uint32_t npf, uint32_t pini, uint32_t cellcode,
+                       const uint32_t* d_dcell, const uint16_t* d_code,
+                       const uint32_t cellmin[3], const uint32_t ncells[3],
+                       uint32_t* d_cellpart, uint32_t* d_sortpart) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!npf) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_dcell && d_code && d_cellpart && d_sortpart && cellmin && ncells,
+               "dsph_presort_fluid: null argument");
+  KTimer kt(ctx, "k_presort");
+  k_presort_fluid<<<grid_for(npf, kBs), kBs, 0, ctx->stream>>>(
+      npf, pini, cellcode, d_dcell, d_code, make_uint3(cellmin[0], cellmin[1], cellmin[2]),
+      make_uint3(ncells[0], ncells[1], ncells[2]), d_cellpart, d_sortpart);
+  DSPH_LAUNCHED(ctx, "k_presort_fluid");
+  return DSPH_OK;
+}
+
+int dsph_sort_by_cell(dsph_ctx* ctx, uint32_t* d_keys, uint32_t* d_values, uint32_t n,
+                      uint32_t key_limit) {
+  if (!ctx) return DSPH_E_ARG;
+  if (n <= 1) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_keys && d_values, "dsph_sort_by_cell: null array");
+  int rc = ensure_sort_scratch(ctx, n);
+  if (rc) return rc;
+  unsigned bits_total = 0;
+  while (bits_total < 32 && (key_limit - 1u) >> bits_total) bits_total++;
+  if (bits_total == 0) return DSPH_OK;  // all keys equal (key_limit<=1)
+  KTimer kt(ctx, "radix_sort");
+  const unsigned ntiles = (n + kTile - 1) / kTile;
+  unsigned* totals = ctx->sort_hist + size_t(256) * ntiles;
+  unsigned *ka = d_keys, *va = d_values, *kb = ctx->sort_keys_alt, *vb = ctx->sort_vals_alt;
+  unsigned shift = 0;
+  while (shift < bits_total) {
+    const unsigned bits = min(8u, bits_total - shift);
+    const unsigned mask = (1u << bits) - 1u;
+    k_radix_hist<<<ntiles, kBs, 0, ctx->stream>>>(ka, n, shift, mask, ntiles, ctx->sort_hist);
+    k_radix_scan_digit<<<mask + 1, kBs, 0, ctx->stream>>>(ctx->sort_hist, ntiles, totals);
+    k_radix_scatter<<<ntiles, kBs, 0, ctx->stream>>>(ka, va, kb, vb, n, shift, bits, ntiles,
+                                                     ctx->sort_hist, totals);
+    DSPH_LAUNCHED(ctx, "radix pass");
+    unsigned* t;
+    t = ka; ka = kb; kb = t;
+    t = va; va = vb; vb = t;
+    shift += bits;
+  }
+  if (ka != d_keys) {
+    DSPH_HIP(ctx, hipMemcpyAsync(d_keys, ka, sizeof(unsigned) * n, hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+    DSPH_HIP(ctx, hipMemcpyAsync(d_values, va, sizeof(unsigned) * n, hipMemcpyDeviceToDevice,
+                                 ctx->stream));
+  }
+  return DSPH_OK;
+}
+
+int dsph_begin_end_cell(dsph_ctx* ctx, int full, uint32_t np, uint32_t npb,
+                        uint32_t sizebegcell, uint32_t cellfluid, const uint32_t* d_cellpart,
+                        int32_t* d_begcell) {
+  if (!ctx) return DSPH_E_ARG;
+  DSPH_REQUIRE(ctx, d_cellpart && d_begcell, "dsph_begin_end_cell: null array");
+  KTimer kt(ctx, "k_begin_end");
+  int2* bc = reinterpret_cast<int2*>(d_begcell);
+  const unsigned c0 = full ? 0u : cellfluid;
+  if (sizebegcell > c0) {
+    const unsigned nc = sizebegcell - c0;
+    k_fill_int2<<<grid_for(nc, kBs), kBs, 0, ctx->stream>>>(bc + c0, nc, make_int2(0, 0));
+  }
+  const unsigned pini = full ? 0u : npb;
+  if (np > pini) {
+    const unsigned n = np - pini;
+    k_begin_end<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(n, pini, d_cellpart, bc);
+  }
+  DSPH_LAUNCHED(ctx, "k_begin_end");
+  return DSPH_OK;
+}
+
+int dsph_cell_begin_end_sync(dsph_ctx* ctx, const int32_t* d_begcell, uint32_t cellini,
+                             uint32_t ncells, int32_t* h_out) {
+  if (!ctx || !h_out) return DSPH_E_ARG;
+  if (!ncells) return DSPH_OK;
+  DSPH_REQUIRE(ctx, ncells * 8 <= 4096, "dsph_cell_begin_end_sync: too many cells");
+  DSPH_HIP(ctx, hipMemcpyAsync(ctx->pinned, d_begcell + 2 * size_t(cellini), 8 * size_t(ncells),
+                               hipMemcpyDeviceToHost, ctx->stream));
+  DSPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  std::memcpy(h_out, ctx->pinned, 8 * size_t(ncells));
+  return DSPH_OK;
+}
+
+int dsph_sort_data_basic(dsph_ctx* ctx, uint32_t np, uint32_t pini, const uint32_t* d_sortpart,
+                         const uint32_t* d_idp, const uint16_t* d_code, const uint32_t* d_dcell,
+                         const double* d_posxy, const double* d_posz, const float* d_velrho,
+                         uint32_t* d_idp2, uint16_t* d_code2, uint32_t* d_dcell2,
+                         double* d_posxy2, double* d_posz2, float* d_velrho2) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  k_sort_basic<false><<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
+      np, pini, d_sortpart, d_idp, d_code, d_dcell, (const double2*)d_posxy, d_posz,
+      (const float4*)d_velrho, d_idp2, d_code2, d_dcell2, (double2*)d_posxy2, d_posz2,
+      (float4*)d_velrho2, make_double3(0, 0, 0), 1.f, nullptr);
+  DSPH_LAUNCHED(ctx, "k_sort_basic");
+  return DSPH_OK;
+}
+
+int dsph_sort_data_basic_poscell(dsph_ctx* ctx, uint32_t np, uint32_t pini,
+                                 const uint32_t* d_sortpart, const uint32_t* d_idp,
+                                 const uint16_t* d_code, const uint32_t* d_dcell,
+                                 const double* d_posxy, const double* d_posz,
+                                 const float* d_velrho, uint32_t* d_idp2, uint16_t* d_code2,
+                                 uint32_t* d_dcell2, double* d_posxy2, double* d_posz2,
+                                 float* d_velrho2, const double posmin[3], float poscellsize,
+                                 float* d_poscell) {
+  if (!ctx || !posmin) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  KTimer kt(ctx, "k_sort_basic_poscell");
+  k_sort_basic<true><<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
+      np, pini, d_sortpart, d_idp, d_code, d_dcell, (const double2*)d_posxy, d_posz,
+      (const float4*)d_velrho, d_idp2, d_code2, d_dcell2, (double2*)d_posxy2, d_posz2,
+      (float4*)d_velrho2, make_double3(posmin[0], posmin[1], posmin[2]), poscellsize,
+      (float4*)d_poscell);
+  DSPH_LAUNCHED(ctx, "k_sort_basic_poscell");
+  return DSPH_OK;
+}
+
+int dsph_sort_data_f4(dsph_ctx* ctx, uint32_t np, uint32_t pini, const uint32_t* d_sortpart,
+                      const float* d_a, float* d_a2) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  k_sort_one<float4><<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
+      np, pini, d_sortpart, (const float4*)d_a, (float4*)d_a2);
+  DSPH_LAUNCHED(ctx, "k_sort_f4");
+  return DSPH_OK;
+}
+
+int dsph_sort_data_f3(dsph_ctx* ctx, uint32_t np, uint32_t pini, const uint32_t* d_sortpart,
+                      const float* d_a, float* d_a2) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  k_sort_one<F3><<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(np, pini, d_sortpart,
+                                                              (const F3*)d_a, (F3*)d_a2);
+  DSPH_LAUNCHED(ctx, "k_sort_f3");
+  return DSPH_OK;
+}
+
+int dsph_sort_data_sym6(dsph_ctx* ctx, uint32_t np, uint32_t pini, const uint32_t* d_sortpart,
+                        const float* d_a, float* d_a2) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  k_sort_one<Sym6><<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(np, pini, d_sortpart,
+                                                                (const Sym6*)d_a, (Sym6*)d_a2);
+  DSPH_LAUNCHED(ctx, "k_sort_sym6");
+  return DSPH_OK;
+}
+
+int dsph_sort_data_d2_d_f4(dsph_ctx* ctx, uint32_t np, uint32_t pini,
+                           const uint32_t* d_sortpart, const double* d_a, const double* d_b,
+                           const float* d_c, double* d_a2, double* d_b2, float* d_c2) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  k_sort_d2_d_f4<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
+      np, pini, d_sortpart, (const double2*)d_a, d_b, (const float4*)d_c, (double2*)d_a2, d_b2,
+      (float4*)d_c2);
+  DSPH_LAUNCHED(ctx, "k_sort_d2_d_f4");
+  return DSPH_OK;
+}
+
+int dsph_update_poscell(dsph_ctx* ctx, uint32_t np, const double posmin[3], float poscellsize,
+                        const double* d_posxy, const double* d_posz, float* d_poscell) {
+  if (!ctx || !posmin) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  k_update_poscell<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
+      np, make_double3(posmin[0], posmin[1], posmin[2]), poscellsize, (const double2*)d_posxy,
+      d_posz, (float4*)d_poscell);
+  DSPH_LAUNCHED(ctx, "k_update_poscell");
+  return DSPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,102 @@
+// Internal context of the C ABI: device, stream, constants and scratch.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <map>
+#include <string>
+#include <utility>
+#include <vector>
+#include "dsph_common.h"
+
+struct dsph_ktime {
+  unsigned long long count = 0;
+  double ms = 0;
+};
+
+struct dsph_ctx {
+  // Per-kernel timing with HIP events on the context stream (off by default).
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<std::string, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+  std::map<std::string, dsph_ktime> ktimes;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  std::string err;
+  dsph_cte cte{};
+  bool cte_set = false;
+  // Radix-sort scratch: alternate key/value buffers and digit histograms.
+  uint32_t* sort_keys_alt = nullptr;
+  uint32_t* sort_vals_alt = nullptr;
+  size_t sort_cap = 0;
+  uint32_t* sort_hist = nullptr;   // [256 digits][ntiles] + [256] totals
+  size_t hist_cap = 0;
+  // Reduction partials (float) and a small device result area.
+  float* redu = nullptr;
+  size_t redu_cap = 0;
+  uint32_t* small_dev = nullptr;   // 64 words: limits, pips counters
+  void* pinned = nullptr;          // 4 KiB pinned host buffer for readbacks
+};
+
+namespace dsph {
+
+int set_error(dsph_ctx* ctx, int code, const std::string& msg);
+
+// Checks a HIP call; on failure records the error and returns DSPH_E_LAUNCH.
+#define DSPH_HIP(ctx, call)                                                  \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess)                                                    \
+      return dsph::set_error((ctx), DSPH_E_LAUNCH,                           \
+                             std::string(#call) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// Checks the launch of the last kernel.
+#define DSPH_LAUNCHED(ctx, what)                                             \
+  do {                                                                       \
+    hipError_t e_ = hipGetLastError();                                       \
+    if (e_ != hipSuccess)                                                    \
+      return dsph::set_error((ctx), DSPH_E_LAUNCH,                           \
+                             std::string(what) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define DSPH_REQUIRE(ctx, cond, msg)                                         \
+  do {                                                                       \
+    if (!(cond)) return dsph::set_error((ctx), DSPH_E_ARG, (msg));           \
+  } while (0)
+
+inline unsigned grid_for(size_t n, unsigned bs) { return unsigned((n + bs - 1) / bs); }
+
+// Records start/stop events around the launches in its scope when the
+// context's timing is on; resolved by dsph_ctx_timing_collect.
+struct KTimer {
+  dsph_ctx* ctx;
+  const char* name;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  static hipEvent_t take(dsph_ctx* c) {
+    hipEvent_t e = nullptr;
+    if (!c->ev_pool.empty()) {
+      e = c->ev_pool.back();
+      c->ev_pool.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
+  }
+  KTimer(dsph_ctx* c, const char* n) : ctx(c), name(n) {
+    if (!ctx->timing) return;
+    e0 = take(ctx);
+    e1 = take(ctx);
+    if (e0) hipEventRecord(e0, ctx->stream);
+  }
+  ~KTimer() {
+    if (!ctx->timing || !e0 || !e1) return;
+    hipEventRecord(e1, ctx->stream);
+    ctx->ev_pending.push_back({name, {e0, e1}});
+  }
+};
+
+// Ensures ctx scratch can hold a sort of n keys with key_limit.
+int ensure_sort_scratch(dsph_ctx* ctx, uint32_t n);
+int ensure_redu_scratch(dsph_ctx* ctx, uint32_t n);
+
+}  // namespace dsph

@@ -1,0 +1,423 @@
+// Reductions, variable dt and time integration (Symplectic, Verlet, FP64
+// position update) of the SPH step on gfx950.
+//
+// References: ReduMaxFloat / ComputeVelMod / ComputeAceMod (JSphGpu_ker.cu:
+// 55-312), DtVariable (JSphGpu.cpp:1103-1147), KerComputeStepSymplecticPre/Cor
+// and KerComputeStepVerlet (JSphGpuSimple_ker.cu:132-513), KerUpdatePos /
+// KerComputeStepPos(2) (JSphGpu_ker.cu:334-392, 1476-1570). Options out of
+// scope (floatings, shifting, inlet/outlet, mDBC2, periodic) are not built.
+#include <cfloat>
+#include "dsph_ctx.h"
+
+using namespace dsph;
+
+namespace {
+
+constexpr unsigned kBs = 256;
+
+// ---------------------------------------------------------------- max reduce
+// Mode 0: plain values; 1: |float4.xyz|^2; 2: |float3|^2.
+template <int MODE>
+__device__ inline float redu_load(const float* __restrict__ d, unsigned i) {
+  if (MODE == 0) return d[i];
+  if (MODE == 1) {
+    const float4 v = reinterpret_cast<const float4*>(d)[i];
+    return v.x * v.x + v.y * v.y + v.z * v.z;
+  }
+  const float x = d[3 * size_t(i)], y = d[3 * size_t(i) + 1], z = d[3 * size_t(i) + 2];
+  return x * x + y * y + z * z;
+}
+
+__device__ inline float block_max(float v) {
+  __shared__ float sw[kBs / 64];
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  if ((threadIdx.x & 63) == 0) sw[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (unsigned i = 1; i < blockDim.x / 64; i++) v = fmaxf(v, sw[i]);
+  return v;
+}
+
+// Each block reduces 8 x 256 values into partial[blockIdx.x].
+template <int MODE>
+__global__ void __launch_bounds__(kBs) k_max_partial(unsigned n, unsigned ini,
+    const float* __restrict__ data, float* __restrict__ partial) {
+  float v = -FLT_MAX;
+  const unsigned base = blockIdx.x * kBs * 8 + threadIdx.x;
+#pragma unroll
+  for (unsigned k = 0; k < 8; k++) {
+    const unsigned i = base + k * kBs;
+    if (i < n) v = fmaxf(v, redu_load<MODE>(data, i + ini));
+  }
+  v = block_max(v);
+  if (threadIdx.x == 0) partial[blockIdx.x] = v;
+}
+
+__global__ void __launch_bounds__(kBs) k_max_final(unsigned nparts, const float* __restrict__ partial,
+                                                   float* __restrict__ out) {
+  float v = -FLT_MAX;
+  for (unsigned i = threadIdx.x; i < nparts; i += kBs) v = fmaxf(v, partial[i]);
+  v = block_max(v);
+  if (threadIdx.x == 0) *out = (nparts ? v : 0.f);
+}
+
+template <int MODE>
+int redu_max(dsph_ctx* ctx, uint32_t n, uint32_t ini, const float* d, float* d_out) {
+  if (!n) {
+    k_max_final<<<1, kBs, 0, ctx->stream>>>(0u, ctx->redu, d_out);
+    DSPH_LAUNCHED(ctx, "k_max_final");
+    return DSPH_OK;
+  }
+  KTimer kt(ctx, "k_redu_max");
+  const unsigned nparts = (n + kBs * 8 - 1) / (kBs * 8);
+  if (nparts + 1 > ctx->redu_cap)
+    return set_error(ctx, DSPH_E_NOMEM, "reduction scratch too small (dsph_ctx_reserve)");
+  k_max_partial<MODE><<<nparts, kBs, 0, ctx->stream>>>(n, ini, d, ctx->redu);
+  k_max_final<<<1, kBs, 0, ctx->stream>>>(nparts, ctx->redu, d_out);
+  DSPH_LAUNCHED(ctx, "k_max");
+  return DSPH_OK;
+}
+
+// ---------------------------------------------------------------- dt
+// DtVariable (JSphGpu.cpp:1103-1147) on one lane, FP64. red = {max|v|^2,
+// ViscDtMax, max|ace|^2}.
+__global__ void k_dt_variable(const float* __restrict__ red, double kernelh, double cs0,
+                              double cfl, double dtmin, double* __restrict__ dtout,
+                              unsigned slot, unsigned* __restrict__ flags) {
+  if (threadIdx.x != 0) return;
+  const double velmax = double(sqrtf(red[0]));
+  const float viscdtmax = red[1];
+  const double acemax = sqrt(double(red[2]));
+  const double dt1 = (acemax ? sqrt(kernelh / acemax) : DBL_MAX);
+  const double dt2 = kernelh / (fmax(cs0, velmax * 10.) + kernelh * double(viscdtmax));
+  double dt = cfl * fmin(dt1, dt2);
+  if (dt != dt || isinf(dt)) {
+    if (flags) atomicOr(flags, 1u);
+  }
+  if (dt < dtmin) {
+    dt = dtmin;
+    if (flags) atomicOr(flags, 2u);
+  }
+  dtout[slot] = dt;
+}
+
+__global__ void k_dt_advance(double* __restrict__ d) {
+  if (threadIdx.x != 0) return;
+  const double dt = d[2];
+  d[3] += dt;
+  d[4] = dt;
+  d[2] = fmin(d[0], d[1]);
+}
+
+// ---------------------------------------------------------------- Symplectic
+__device__ inline double dev_dt(double dtm_host, const double* d_dt, unsigned slot, double scale) {
+  return d_dt ? d_dt[slot] * scale : dtm_host;
+}
+
+__global__ void __launch_bounds__(kBs) k_sym_pre(unsigned n, unsigned npb,
+    const float4* __restrict__ velrhopre, const float* __restrict__ ar,
+    const float* __restrict__ ace, double dtm_h, const double* __restrict__ d_dt, unsigned slot,
+    float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
+    typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
+    float4* __restrict__ velrho) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double dtm = dev_dt(dtm_h, d_dt, slot, 0.5);
+  if (p < npb) {
+    float4 vr = velrhopre[p];
+    vr.w = float(double(vr.w) + dtm * ar[p]);
+    vr.w = (vr.w < rhopzero ? rhopzero : vr.w);
+    velrho[p] = vr;
+    return;
+  }
+  const unsigned rcode = code[p];
+  const float4 pre = velrhopre[p];
+  float4 nw = pre;
+  nw.w = float(double(pre.w) + dtm * ar[p]);
+  const double dx = double(pre.x) * dtm, dy = double(pre.y) * dtm, dz = double(pre.z) * dtm;
+  const bool outrho = (nw.w < rhopoutmin || nw.w > rhopoutmax);
+  const float ax = ace[3 * size_t(p)], ay = ace[3 * size_t(p) + 1], az = ace[3 * size_t(p) + 2];
+  nw.x = float(double(pre.x) + (double(ax) + gravity.x) * dtm);
+  nw.y = float(double(pre.y) + (double(ay) + gravity.y) * dtm);
+  nw.z = float(double(pre.z) + (double(az) + gravity.z) * dtm);
+  movxy[p] = make_double2(dx, dy);
+  movz[p] = dz;
+  if (outrho && code_is_normal(rcode)) code[p] = typecode(code_set(rcode, kCodeOutRho));
+  velrho[p] = nw;
+}
+
+__global__ void __launch_bounds__(kBs) k_sym_cor(unsigned n, unsigned npb,
+    const float4* __restrict__ velrhopre, const float* __restrict__ ar,
+    const float* __restrict__ ace, double dtm_h, double dt_h, const double* __restrict__ d_dt,
+    unsigned slot, float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
+    typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
+    float4* __restrict__ velrho) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const double dt = dev_dt(dt_h, d_dt, slot, 1.0);
+  const double dtm = dev_dt(dtm_h, d_dt, slot, 0.5);
+  if (p < npb) {
+    const double eps = (-double(ar[p]) / double(velrho[p].w)) * dt;
+    float rrho = float(double(velrhopre[p].w) * (2. - eps) / (2. + eps));
+    rrho = (rrho < rhopzero ? rhopzero : rrho);
+    velrho[p] = make_float4(0, 0, 0, rrho);
+    return;
+  }
+  const unsigned rcode = code[p];
+  const double eps = (-double(ar[p]) / double(velrho[p].w)) * dt;
+  const float4 pre = velrhopre[p];
+  float4 nw = pre;
+  nw.w = float(double(pre.w) * (2. - eps) / (2. + eps));
+  const float ax = ace[3 * size_t(p)], ay = ace[3 * size_t(p) + 1], az = ace[3 * size_t(p) + 2];
+  nw.x = float(double(pre.x) + (double(ax) + gravity.x) * dt);
+  nw.y = float(double(pre.y) + (double(ay) + gravity.y) * dt);
+  nw.z = float(double(pre.z) + (double(az) + gravity.z) * dt);
+  const double dx = (double(pre.x) + double(nw.x)) * dtm;
+  const double dy = (double(pre.y) + double(nw.y)) * dtm;
+  const double dz = (double(pre.z) + double(nw.z)) * dtm;
+  const bool outrho = (nw.w < rhopoutmin || nw.w > rhopoutmax);
+  movxy[p] = make_double2(dx, dy);
+  movz[p] = dz;
+  if (outrho && code_is_normal(rcode)) code[p] = typecode(code_set(rcode, kCodeOutRho));
+  velrho[p] = nw;
+}
+
+__global__ void __launch_bounds__(kBs) k_verlet(unsigned n, unsigned npb,
+    const float4* __restrict__ velrho1, const float4* __restrict__ velrho2,
+    const float* __restrict__ ar, const float* __restrict__ ace, double dt, double dt205,
+    double dt2, float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
+    typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
+    float4* __restrict__ velrhonew) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  if (p < npb) {
+    float rrho = float(double(velrho2[p].w) + dt2 * ar[p]);
+    rrho = (rrho < rhopzero ? rhopzero : rrho);
+    velrhonew[p] = make_float4(0, 0, 0, rrho);
+    return;
+  }
+  const unsigned rcode = code[p];
+  const float4 v2 = velrho2[p];
+  const float rhonew = float(double(v2.w) + dt2 * ar[p]);
+  const float4 v1 = velrho1[p];
+  const double ax = double(ace[3 * size_t(p)]) + gravity.x;
+  const double ay = double(ace[3 * size_t(p) + 1]) + gravity.y;
+  const double az = double(ace[3 * size_t(p) + 2]) + gravity.z;
+  const double dx = double(v1.x) * dt + ax * dt205;
+  const double dy = double(v1.y) * dt + ay * dt205;
+  const double dz = double(v1.z) * dt + az * dt205;
+  const bool outrho = (rhonew < rhopoutmin || rhonew > rhopoutmax);
+  const float4 nw = make_float4(float(double(v2.x) + ax * dt2), float(double(v2.y) + ay * dt2),
+                                float(double(v2.z) + az * dt2), rhonew);
+  movxy[p] = make_double2(dx, dy);
+  movz[p] = dz;
+  if (outrho && code_is_normal(rcode)) code[p] = typecode(code_set(rcode, kCodeOutRho));
+  velrhonew[p] = nw;
+}
+
+// ---------------------------------------------------------------- positions
+struct PosCte {
+  float movlimit;
+  double mapx, mapy, mapz, sizex, sizey, sizez;
+  float scell;
+  unsigned cellcode;
+};
+
+// KerUpdatePos (JSphGpu_ker.cu:334-392) without periodic boundaries.
+__device__ inline void update_pos(double2 rxy, double rz, double mx, double my, double mz,
+                                  bool outrho, unsigned p, const PosCte& c,
+                                  double2* __restrict__ posxy, double* __restrict__ posz,
+                                  unsigned* __restrict__ dcell, typecode* __restrict__ code) {
+  const bool outmov =
+      (fmaxf(fabsf(float(mx)), fmaxf(fabsf(float(my)), fabsf(float(mz)))) > c.movlimit);
+  const double x = rxy.x + mx, y = rxy.y + my, z = rz + mz;
+  const double dx = x - c.mapx, dy = y - c.mapy, dz = z - c.mapz;
+  const bool out = (dx != dx || dy != dy || dz != dz || dx < 0 || dy < 0 || dz < 0 ||
+                    dx >= c.sizex || dy >= c.sizey || dz >= c.sizez);
+  posxy[p] = make_double2(x, y);
+  posz[p] = z;
+  if (outrho || outmov || out) {
+    unsigned rc = code[p];
+    if (out) rc = code_set(rc, kCodeOutPos);
+    else if (outrho) rc = code_set(rc, kCodeOutRho);
+    else rc = code_set(rc, kCodeOutMov);
+    code[p] = typecode(rc);
+    dcell[p] = kDcelMapOut;
+  } else {
+    const unsigned cx = unsigned(dx / c.scell), cy = unsigned(dy / c.scell),
+                   cz = unsigned(dz / c.scell);
+    dcell[p] = dcel_cell(c.cellcode, cx, cy, cz);
+  }
+}
+
+__global__ void __launch_bounds__(kBs) k_step_pos2(unsigned n, unsigned pini, PosCte c,
+    const double2* __restrict__ posxypre, const double* __restrict__ poszpre,
+    const double2* __restrict__ movxy, const double* __restrict__ movz,
+    double2* __restrict__ posxy, double* __restrict__ posz, unsigned* __restrict__ dcell,
+    typecode* __restrict__ code) {
+  const unsigned pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= n) return;
+  const unsigned p = pt + pini;
+  const bool outrho = code_is_outrho(code[p]);
+  const double2 m = movxy[p];
+  update_pos(posxypre[p], poszpre[p], m.x, m.y, movz[p], outrho, p, c, posxy, posz, dcell, code);
+}
+
+__global__ void __launch_bounds__(kBs) k_step_pos(unsigned n, unsigned pini, PosCte c,
+    const double2* __restrict__ movxy, const double* __restrict__ movz,
+    double2* __restrict__ posxy, double* __restrict__ posz, unsigned* __restrict__ dcell,
+    typecode* __restrict__ code) {
+  const unsigned pt = blockIdx.x * blockDim.x + threadIdx.x;
+  if (pt >= n) return;
+  const unsigned p = pt + pini;
+  const bool outrho = code_is_outrho(code[p]);
+  const double2 m = movxy[p];
+  update_pos(posxy[p], posz[p], m.x, m.y, movz[p], outrho, p, c, posxy, posz, dcell, code);
+}
+
+PosCte make_posc(const dsph_cte& c) {
+  PosCte r;
+  r.movlimit = c.movlimit;
+  r.mapx = c.maprealposminx; r.mapy = c.maprealposminy; r.mapz = c.maprealposminz;
+  r.sizex = c.maprealsizex; r.sizey = c.maprealsizey; r.sizez = c.maprealsizez;
+  r.scell = c.scell;
+  r.cellcode = c.cellcode;
+  return r;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dsph_velmax2(dsph_ctx* ctx, uint32_t n, uint32_t pini, const float* d_velrho, float* d_out) {
+  if (!ctx || !d_out) return DSPH_E_ARG;
+  return redu_max<1>(ctx, n, pini, d_velrho, d_out);
+}
+
+int dsph_acemax2(dsph_ctx* ctx, uint32_t n, uint32_t pini, const float* d_ace, float* d_out) {
+  if (!ctx || !d_out) return DSPH_E_ARG;
+  return redu_max<2>(ctx, n, pini, d_ace, d_out);
+}
+
+int dsph_redu_max_f32(dsph_ctx* ctx, uint32_t n, uint32_t ini, const float* d_data,
+                      float* d_out) {
+  if (!ctx || !d_out) return DSPH_E_ARG;
+  return redu_max<0>(ctx, n, ini, d_data, d_out);
+}
+
+int dsph_redu_max_f32_sync(dsph_ctx* ctx, uint32_t n, uint32_t ini, const float* d_data,
+                           float* h_out) {
+  if (!ctx || !h_out) return DSPH_E_ARG;
+  float* d_out = reinterpret_cast<float*>(ctx->small_dev) + 32;
+  int rc = redu_max<0>(ctx, n, ini, d_data, d_out);
+  if (rc) return rc;
+  DSPH_HIP(ctx, hipMemcpyAsync(ctx->pinned, d_out, sizeof(float), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  DSPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *h_out = *static_cast<float*>(ctx->pinned);
+  return DSPH_OK;
+}
+
+int dsph_dt_variable(dsph_ctx* ctx, const float* d_red, double kernelh, double cs0, double cfl,
+                     double dtmin, double* d_dt, uint32_t slot, uint32_t* d_flags) {
+  if (!ctx || !d_red || !d_dt) return DSPH_E_ARG;
+  k_dt_variable<<<1, 64, 0, ctx->stream>>>(d_red, kernelh, cs0, cfl, dtmin, d_dt, slot, d_flags);
+  DSPH_LAUNCHED(ctx, "k_dt_variable");
+  return DSPH_OK;
+}
+
+int dsph_dt_advance(dsph_ctx* ctx, double* d_dt) {
+  if (!ctx || !d_dt) return DSPH_E_ARG;
+  k_dt_advance<<<1, 64, 0, ctx->stream>>>(d_dt);
+  DSPH_LAUNCHED(ctx, "k_dt_advance");
+  return DSPH_OK;
+}
+
+int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_velrhopre,
+                             const float* d_ar, const float* d_ace, double dtm,
+                             const double* d_dt, uint32_t dt_slot, float rhopzero,
+                             float rhopoutmin, float rhopoutmax, const float gravity[3],
+                             uint16_t* d_code, double* d_movxy, double* d_movz,
+                             float* d_velrho) {
+  if (!ctx || !gravity) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_velrhopre && d_ar && d_ace && d_code && d_movxy && d_movz && d_velrho,
+               "dsph_step_symplectic_pre: null array");
+  KTimer kt(ctx, "k_sym_pre");
+  k_sym_pre<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
+      np, npb, (const float4*)d_velrhopre, d_ar, d_ace, dtm, d_dt, dt_slot, rhopzero, rhopoutmin,
+      rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code, (double2*)d_movxy,
+      d_movz, (float4*)d_velrho);
+  DSPH_LAUNCHED(ctx, "k_sym_pre");
+  return DSPH_OK;
+}
+
+int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_velrhopre,
+                             const float* d_ar, const float* d_ace, double dtm, double dt,
+                             const double* d_dt, uint32_t dt_slot, float rhopzero,
+                             float rhopoutmin, float rhopoutmax, const float gravity[3],
+                             uint16_t* d_code, double* d_movxy, double* d_movz,
+                             float* d_velrho) {
+  if (!ctx || !gravity) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_velrhopre && d_ar && d_ace && d_code && d_movxy && d_movz && d_velrho,
+               "dsph_step_symplectic_cor: null array");
+  KTimer kt(ctx, "k_sym_cor");
+  k_sym_cor<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
+      np, npb, (const float4*)d_velrhopre, d_ar, d_ace, dtm, dt, d_dt, dt_slot, rhopzero,
+      rhopoutmin, rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code,
+      (double2*)d_movxy, d_movz, (float4*)d_velrho);
+  DSPH_LAUNCHED(ctx, "k_sym_cor");
+  return DSPH_OK;
+}
+
+int dsph_step_verlet(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_velrho1,
+                     const float* d_velrho2, const float* d_ar, const float* d_ace, double dt,
+                     double dt2, float rhopzero, float rhopoutmin, float rhopoutmax,
+                     const float gravity[3], uint16_t* d_code, double* d_movxy, double* d_movz,
+                     float* d_velrhonew) {
+  if (!ctx || !gravity) return DSPH_E_ARG;
+  if (!np) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_velrho1 && d_velrho2 && d_ar && d_ace && d_code && d_movxy && d_movz &&
+                        d_velrhonew,
+               "dsph_step_verlet: null array");
+  const double dt205 = 0.5 * dt * dt;
+  KTimer kt(ctx, "k_verlet");
+  k_verlet<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
+      np, npb, (const float4*)d_velrho1, (const float4*)d_velrho2, d_ar, d_ace, dt, dt205, dt2,
+      rhopzero, rhopoutmin, rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code,
+      (double2*)d_movxy, d_movz, (float4*)d_velrhonew);
+  DSPH_LAUNCHED(ctx, "k_verlet");
+  return DSPH_OK;
+}
+
+int dsph_step_pos2(dsph_ctx* ctx, uint32_t np, uint32_t npb, const double* d_posxypre,
+                   const double* d_poszpre, const double* d_movxy, const double* d_movz,
+                   double* d_posxy, double* d_posz, uint32_t* d_dcell, uint16_t* d_code) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  if (np <= npb) return DSPH_OK;
+  const unsigned n = np - npb;
+  KTimer kt(ctx, "k_step_pos2");
+  k_step_pos2<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(
+      n, npb, make_posc(ctx->cte), (const double2*)d_posxypre, d_poszpre, (const double2*)d_movxy,
+      d_movz, (double2*)d_posxy, d_posz, d_dcell, d_code);
+  DSPH_LAUNCHED(ctx, "k_step_pos2");
+  return DSPH_OK;
+}
+
+int dsph_step_pos(dsph_ctx* ctx, uint32_t np, uint32_t npb, const double* d_movxy,
+                  const double* d_movz, double* d_posxy, double* d_posz, uint32_t* d_dcell,
+                  uint16_t* d_code) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  if (np <= npb) return DSPH_OK;
+  const unsigned n = np - npb;
+  k_step_pos<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(
+      n, npb, make_posc(ctx->cte), (const double2*)d_movxy, d_movz, (double2*)d_posxy, d_posz,
+      d_dcell, d_code);
+  DSPH_LAUNCHED(ctx, "k_step_pos");
+  return DSPH_OK;
+}
+
+}  // extern "C"

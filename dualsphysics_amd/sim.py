@@ -1,0 +1,110 @@
+"""Python face of the C++ host driver (``dsph_sim_*``, mirror of JSphGpuSingle).
+
+Example::
+
+    from dualsphysics_amd import case, sim
+    c = case.dam_break_cfg0()
+    s = sim.SphGpuSingle(c)
+    s.run(10)                 # 10 Symplectic steps + divides, dt kept on device
+    st = s.stats()
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+class DsphError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[dsph error {code}] {msg}")
+        self.code = code
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class SphGpuSingle:
+    """Single-GPU SPH simulation of a :class:`dualsphysics_amd.case.Case`."""
+
+    def __init__(self, case, device: int = 0):
+        self.lib = N.load()
+        self.case = case
+        self._h = C.c_void_p()
+        idp = np.ascontiguousarray(case.idp, np.uint32)
+        pos = np.ascontiguousarray(case.pos, np.float64)
+        vr = np.ascontiguousarray(case.velrho, np.float32)
+        bn = None if case.boundnor is None else np.ascontiguousarray(case.boundnor, np.float32)
+        self._native_case = case.to_native()
+        rc = self.lib.dsph_sim_create(device, C.byref(self._native_case), case.np, _ptr(idp),
+                                      _ptr(pos), _ptr(vr), None if bn is None else _ptr(bn),
+                                      C.byref(self._h))
+        if rc:
+            msg = self._err()
+            self.close()
+            raise DsphError(rc, msg)
+
+    def _err(self) -> str:
+        if not self._h:
+            return "simulation not created"
+        return self.lib.dsph_sim_last_error(self._h).decode()
+
+    def _check(self, rc):
+        if rc:
+            raise DsphError(rc, self._err())
+
+    def close(self):
+        if self._h:
+            self.lib.dsph_sim_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, nsteps: int, sync_dt: bool = False):
+        """Runs ``nsteps`` time steps. ``sync_dt=True`` follows the reference's
+        host-side dt (one readback per interaction); otherwise dt stays on the
+        device and the step has no host round trip besides the divide's counts."""
+        self._check(self.lib.dsph_sim_run(self._h, nsteps, 1 if sync_dt else 0))
+
+    def stats(self) -> dict:
+        st = N.DsphStats()
+        self._check(self.lib.dsph_sim_stats(self._h, C.byref(st)))
+        return {f: getattr(st, f) for f, _ in N.DsphStats._fields_}
+
+    def download(self):
+        """Current particles in sorted order: dict of idp, pos, velrho, code."""
+        n = self.stats()["np"]
+        idp = np.empty(n, np.uint32)
+        pos = np.empty((n, 3), np.float64)
+        vr = np.empty((n, 4), np.float32)
+        code = np.empty(n, np.uint16)
+        cnt = C.c_uint32()
+        self._check(self.lib.dsph_sim_download(self._h, C.byref(cnt), idp.ctypes.data_as(N.u32p),
+                                               _ptr(pos), _ptr(vr), _ptr(code)))
+        return {"idp": idp, "pos": pos, "velrho": vr, "code": code}
+
+    def forces(self):
+        """ace (float3) and ar of the last interaction, sorted order."""
+        n = self.stats()["np"]
+        ace = np.empty((n, 3), np.float32)
+        ar = np.empty(n, np.float32)
+        self._check(self.lib.dsph_sim_download_forces(self._h, _ptr(ace), _ptr(ar)))
+        return ace, ar
+
+    def interact(self):
+        """Forces of the current state (no time advance); read with forces()."""
+        self._check(self.lib.dsph_sim_interact(self._h))
+
+    def pips(self):
+        """Pair counts {real fluid, real bound, checked fluid, checked bound,
+        real fluid-boundary} of the current state (JDsPips rule)."""
+        res = (C.c_uint64 * 5)()
+        self._check(self.lib.dsph_sim_pips(self._h, res))
+        return [int(v) for v in res]

@@ -1,0 +1,440 @@
+/*
+ * dsph_abi.h -- C ABI of the MI355X-native DualSPHysics SPH step.
+ *
+ * This is the drop-in boundary of the GPU SPH step. Every entry point replaces
+ * one host-callable operator of the reference's CUDA layer (namespaces cusph,
+ * cusphs, cudiv; reference v5.4.355, paths relative to src/source/), keeping the
+ * argument meaning, the particle-array layout and the call order of the
+ * reference host driver (JSphGpuSingle.cpp / JSphGpu.cpp / JCellDivGpu*.cpp).
+ *
+ * Differences from the reference interface, all deliberate:
+ *  - constants live in a per-context struct (dsph_cte) instead of the global
+ *    __constant__ CTE (JSphGpu_ker.cu:39), so several independent cases can
+ *    share one device (batched ensembles);
+ *  - every call returns a status code (DSPH_OK, ...) instead of relying on
+ *    Check_CudaErroor + JException (RunExceptionGpuDef.h:19-35); the message of
+ *    the last failure is in dsph_last_error();
+ *  - every call is asynchronous on the context's stream, except the ones whose
+ *    name ends in _sync or that return values to the host (documented per call);
+ *  - no entry point allocates device memory except dsph_malloc and
+ *    dsph_ctx_reserve; scratch (sort buffers, reduction partials) belongs to
+ *    the context and is sized by dsph_ctx_reserve.
+ *
+ * Layout contract (identical to the reference, JSphGpu.h:103-209):
+ *  SoA arrays indexed [0,Npb) boundary, [Npb,Np) fluid, sorted by cell after
+ *  every divide. Positions are FP64 (double2 posxy + double posz); poscell is
+ *  float4{x,y,z, bits(PSCEL code)} relative to a cell of size poscellsize;
+ *  velrho is float4{vx,vy,vz,rho}; code is u16 with the bit fields of
+ *  DualSphDef.h:150-241; dcell is u32 with the DCEL coding of JDsDcellDef.h.
+ *  Symmetric tensors (tsymatrix3f) are 6 floats {xx,xy,xz,yy,yz,zz}.
+ *
+ * All pointers named d_* are device pointers; h_* are host pointers.
+ */
+#ifndef DSPH_ABI_H
+#define DSPH_ABI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DSPH_ABI_VERSION 1
+
+/* Status codes. */
+#define DSPH_OK        0
+#define DSPH_E_ARG     1   /* invalid argument (null pointer, size, mode) */
+#define DSPH_E_LAUNCH  2   /* kernel launch or runtime error */
+#define DSPH_E_NOMEM   3   /* device allocation failed or scratch too small */
+#define DSPH_E_STATE   4   /* call not valid in the current context state */
+#define DSPH_E_NUMERIC 5   /* NaN/Inf dt, boundary particle excluded, ... */
+
+/* Option enums: values equal the reference's (DualSphDef.h). */
+#define DSPH_KERNEL_CUBIC     1   /* TpKernel KERNEL_Cubic (not built) */
+#define DSPH_KERNEL_WENDLAND  2   /* TpKernel KERNEL_Wendland */
+#define DSPH_VISCO_ARTIFICIAL 1   /* TpVisco */
+#define DSPH_VISCO_LAMINARSPS 2
+#define DSPH_VISCO_LAMINAR    3
+#define DSPH_DDT_NONE         0   /* TpDensity */
+#define DSPH_DDT_DDT          1   /* Molteni & Colagrossi */
+#define DSPH_DDT_DDT2         2   /* Fourtakas (inner particles) */
+#define DSPH_DDT_DDT2FULL     3   /* Fourtakas (full) */
+#define DSPH_BC_DBC           1   /* TpBoundary */
+#define DSPH_BC_MDBC          2
+
+typedef struct dsph_ctx dsph_ctx;
+
+/*
+ * Constants of the interaction, mirror of StCteInteraction
+ * (JSphGpu_cte.h:27-62), filled as JSphGpu::GetConstantData (JSphGpu.cpp:564-588).
+ */
+typedef struct dsph_cte {
+  float massb, massf;            /* reference masses [kg] */
+  float kernelh;                 /* smoothing length h [m] */
+  float kernelsize2;             /* (2h)^2 */
+  float poscellsize;             /* size of PosCell cells (usually 2h) */
+  float awen, bwenh;             /* Wendland constants (FunSphKernel.h:198-207) */
+  float cs0;                     /* speed of sound at rho0 */
+  float eta2;                    /* (0.1h)^2 */
+  float ddtkh;                   /* DDTValue*KernelSize */
+  float ddtgz;                   /* rho0*|g.z|/B */
+  float scell;                   /* cell size of the neighbour search */
+  float kernelsize;              /* 2h */
+  float dp;                      /* inter-particle spacing */
+  float cteb;                    /* B of Tait's EOS [Pa] */
+  float gamma;                   /* polytropic constant */
+  float rhopzero;                /* rho0 */
+  float ovrhopzero;              /* 1/rho0 */
+  float movlimit;                /* 0.9*Scell, max displacement per step */
+  uint32_t tboundary;            /* DSPH_BC_* */
+  uint32_t cellcode;             /* DomCellCode (JSph.cpp:1936) */
+  double domposminx, domposminy, domposminz;
+  double maprealposminx, maprealposminy, maprealposminz;
+  double maprealsizex, maprealsizey, maprealsizez;
+} dsph_cte;
+
+/*
+ * Cell-division data for the neighbour search, mirror of StDivDataGpu
+ * (JCellDivDataGpu.h:27-79) for single-GPU (axis Z).
+ */
+typedef struct dsph_divdata {
+  int32_t scelldiv;              /* 1 (Full cells) or 2 (Half cells) */
+  int32_t ncx, ncy, ncz, ncw;    /* cells of the current domain, ncw=ncx*ncy */
+  uint32_t cellfluid;            /* first fluid cell = ncw*ncz+1 */
+  int32_t cellzerox, cellzeroy, cellzeroz; /* CellDomainMin */
+  const int32_t* d_beginendcell; /* int2[SizeBeginEndCell] {begin,end} */
+  float scell;
+  uint32_t domcellcode;
+  double domposminx, domposminy, domposminz;
+  float kernelsize2;
+  float poscellsize;
+} dsph_divdata;
+
+/*
+ * Parameters of one force interaction, mirror of the subset of StInterParmsg
+ * (JSphGpu_ker.h:55-203) used by the in-scope options (no floatings, no
+ * shifting, no periodic, no inlet/outlet).
+ */
+typedef struct dsph_inter_parms {
+  int32_t tkernel;               /* DSPH_KERNEL_WENDLAND */
+  int32_t tvisco;                /* DSPH_VISCO_* */
+  int32_t tdensity;              /* DSPH_DDT_* */
+  float viscob, viscof;          /* Visco*ViscoBoundFactor, Visco */
+  uint32_t np, npb, npbok;       /* Np, Npb, NpbOk */
+  dsph_divdata divdata;
+  const uint32_t* d_dcell;
+  const float*    d_poscell;     /* float4[np] */
+  const float*    d_velrho;      /* float4[np] */
+  const uint16_t* d_code;
+  const float*    d_spstaurho2;  /* sym6[np] (VISCO_LAMINARSPS) else NULL */
+  float* d_viscdt;               /* float[np]   (in/out, max-accumulated) */
+  float* d_ar;                   /* float[np]   (in/out, summed) */
+  float* d_ace;                  /* float3[np]  (in/out, summed) */
+  float* d_sps2strain;           /* sym6[np]    (in/out, VISCO_LAMINARSPS) */
+  float* d_velgrad;              /* float[9*np] full velocity gradient (learned
+                                    closure features) or NULL */
+} dsph_inter_parms;
+
+/* ---------------------------------------------------------------- context */
+
+/* Creates a context on HIP device `device` with its own stream. */
+int dsph_ctx_create(int device, dsph_ctx** out);
+int dsph_ctx_destroy(dsph_ctx* ctx);
+/* Sizes context scratch for up to np_max particles and nctt_max cells
+ * (sort double-buffers, digit histograms, reduction partials). Allocates. */
+int dsph_ctx_reserve(dsph_ctx* ctx, uint32_t np_max, uint32_t nctt_max);
+/* Replaces the context stream (hipStream_t as void*); NULL = own stream. */
+int dsph_ctx_set_stream(dsph_ctx* ctx, void* stream);
+void* dsph_ctx_get_stream(dsph_ctx* ctx);
+int dsph_synchronize(dsph_ctx* ctx);
+const char* dsph_last_error(dsph_ctx* ctx);
+/* Device memory (mirror of the JArraysGpu pool's allocations). */
+int dsph_malloc(dsph_ctx* ctx, size_t bytes, void** d_out);
+int dsph_free(dsph_ctx* ctx, void* d_ptr);
+int dsph_memset_async(dsph_ctx* ctx, void* d_ptr, int value, size_t bytes);
+int dsph_memcpy_h2d(dsph_ctx* ctx, void* d_dst, const void* h_src, size_t bytes);
+int dsph_memcpy_d2h(dsph_ctx* ctx, void* h_dst, const void* d_src, size_t bytes);
+int dsph_memcpy_d2d_async(dsph_ctx* ctx, void* d_dst, const void* d_src, size_t bytes);
+
+/* Per-kernel timing with HIP events recorded on the context stream around
+ * every kernel launch of the entry points (mirror of the reference's stage
+ * timers, JDsTimersGpu.h / JTimerCuda.h). Off by default; collect resolves
+ * the pending events (synchronizes) into per-kernel totals; get iterates the
+ * totals (name, launches, total ms); reset clears them. */
+int dsph_ctx_set_timing(dsph_ctx* ctx, int enabled);
+int dsph_ctx_timing_collect(dsph_ctx* ctx);
+int dsph_ctx_timing_get(dsph_ctx* ctx, uint32_t idx, char* name, size_t name_len,
+                        uint64_t* count, double* total_ms);
+int dsph_ctx_timing_reset(dsph_ctx* ctx);
+
+/* Replaces cusph::CteInteractionUp (JSphGpu_ker.h:284). Copies into ctx. */
+int dsph_set_constants(dsph_ctx* ctx, const dsph_cte* cte);
+
+/* ----------------------------------------------------- neighbour list (cudiv) */
+
+/* Replaces cudiv::LimitsCell (JCellDivGpu_ker.h:51): min/max cell (x,y,z) of
+ * the non-excluded particles [pini,pini+np). Synchronous (host results).
+ * With no valid particle, celmin > celmax (as the reference). */
+int dsph_cell_limits_sync(dsph_ctx* ctx, uint32_t np, uint32_t pini,
+                          uint32_t cellcode, const uint32_t* d_dcell,
+                          const uint16_t* d_code,
+                          uint32_t h_celmin[3], uint32_t h_celmax[3]);
+
+/* Replaces cudiv::PreSortFull (JCellDivGpuSingle_ker.h:28): cell key of every
+ * particle (boxes of JCellDivGpuSingle.cpp:109-128) and sortpart[p]=p. */
+int dsph_presort_full(dsph_ctx* ctx, uint32_t np, uint32_t cellcode,
+                      const uint32_t* d_dcell, const uint16_t* d_code,
+                      const uint32_t cellmin[3], const uint32_t ncells[3],
+                      uint32_t* d_cellpart, uint32_t* d_sortpart);
+
+/* Replaces cudiv::PreSortFluid (JCellDivGpuSingle_ker.h:31): fluid particles
+ * [pini,pini+npf) only. */
+int dsph_presort_fluid(dsph_ctx* ctx, uint32_t npf, uint32_t pini,
+                       uint32_t cellcode, const uint32_t* d_dcell,
+                       const uint16_t* d_code, const uint32_t cellmin[3],
+                       const uint32_t ncells[3], uint32_t* d_cellpart,
+                       uint32_t* d_sortpart);
+
+/* Replaces cudiv::Sort (JCellDivGpu_ker.h:40): in-place sort of (keys,values)
+ * by key, keys < key_limit. Always stable (the reference's -stable mode):
+ * LSD radix sort, ceil(log2(key_limit)/8) passes. Uses context scratch. */
+int dsph_sort_by_cell(dsph_ctx* ctx, uint32_t* d_keys, uint32_t* d_values,
+                      uint32_t n, uint32_t key_limit);
+
+/* Replaces cudiv::CalcBeginEndCell (JCellDivGpu_ker.h:53): clears
+ * begcell[0..sizebegcell) (full) or [cellfluid..sizebegcell) and writes
+ * {first,last+1} of every non-empty cell from the sorted keys. */
+int dsph_begin_end_cell(dsph_ctx* ctx, int full, uint32_t np, uint32_t npb,
+                        uint32_t sizebegcell, uint32_t cellfluid,
+                        const uint32_t* d_cellpart, int32_t* d_begcell);
+
+/* Replaces the CellBeginEnd readback (JCellDivGpuSingle.cpp:218-226): copies
+ * ncells int2 entries from cell `cellini` to the host. Synchronous. */
+int dsph_cell_begin_end_sync(dsph_ctx* ctx, const int32_t* d_begcell,
+                             uint32_t cellini, uint32_t ncells,
+                             int32_t* h_out);
+
+/* Replaces cudiv::SortDataParticles (basic overload, JCellDivGpu_ker.h:56;
+ * called from JCellDivGpu::SortBasicArrays, JCellDivGpu.cpp:329-337):
+ * gathers [pini,np) of idp, code, dcell, posxy, posz, velrho by sortpart. */
+int dsph_sort_data_basic(dsph_ctx* ctx, uint32_t np, uint32_t pini,
+                         const uint32_t* d_sortpart,
+                         const uint32_t* d_idp, const uint16_t* d_code,
+                         const uint32_t* d_dcell, const double* d_posxy,
+                         const double* d_posz, const float* d_velrho,
+                         uint32_t* d_idp2, uint16_t* d_code2, uint32_t* d_dcell2,
+                         double* d_posxy2, double* d_posz2, float* d_velrho2);
+/* Same gather fused with UpdatePosCell of the gathered positions
+ * (JSphGpuSingle.cpp:396-406 followed by :476-477). poscell covers [0,np):
+ * entries below pini are recomputed from the (unmoved) posxy/posz. */
+int dsph_sort_data_basic_poscell(dsph_ctx* ctx, uint32_t np, uint32_t pini,
+                         const uint32_t* d_sortpart,
+                         const uint32_t* d_idp, const uint16_t* d_code,
+                         const uint32_t* d_dcell, const double* d_posxy,
+                         const double* d_posz, const float* d_velrho,
+                         uint32_t* d_idp2, uint16_t* d_code2, uint32_t* d_dcell2,
+                         double* d_posxy2, double* d_posz2, float* d_velrho2,
+                         const double posmin[3], float poscellsize,
+                         float* d_poscell);
+/* Other SortDataParticles overloads (JCellDivGpu_ker.h:58-76). */
+int dsph_sort_data_f4(dsph_ctx* ctx, uint32_t np, uint32_t pini,
+                      const uint32_t* d_sortpart, const float* d_a, float* d_a2);
+int dsph_sort_data_f3(dsph_ctx* ctx, uint32_t np, uint32_t pini,
+                      const uint32_t* d_sortpart, const float* d_a, float* d_a2);
+int dsph_sort_data_sym6(dsph_ctx* ctx, uint32_t np, uint32_t pini,
+                        const uint32_t* d_sortpart, const float* d_a, float* d_a2);
+int dsph_sort_data_d2_d_f4(dsph_ctx* ctx, uint32_t np, uint32_t pini,
+                           const uint32_t* d_sortpart,
+                           const double* d_a, const double* d_b, const float* d_c,
+                           double* d_a2, double* d_b2, float* d_c2);
+
+/* Replaces cusphs::UpdatePosCell (JSphGpuSimple_ker.h:34). */
+int dsph_update_poscell(dsph_ctx* ctx, uint32_t np, const double posmin[3],
+                        float poscellsize, const double* d_posxy,
+                        const double* d_posz, float* d_poscell);
+
+/* ------------------------------------------------------ interaction (cusph) */
+
+/* Replaces cusph::Interaction_Forces (JSphGpu_ker.h:293): fluid-fluid and
+ * fluid-boundary pass over fluid particles [npb,np) and boundary-fluid pass
+ * over [0,npbok). Outputs are accumulated (ace,ar summed; viscdt max), as in
+ * KerInteractionForcesFluid / KerInteractionForcesBound. */
+int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t);
+
+/* Replaces cusph::ComputeSpsTau (JSphGpu_ker.h:304). */
+int dsph_sps_tau(dsph_ctx* ctx, uint32_t np, uint32_t npb, float smag,
+                 float blin, const float* d_velrho, const float* d_sps2strain,
+                 float* d_tau);
+
+/* Replaces cusph::Interaction_MdbcCorrection (JSphGpu_mdbc_iker.h:32),
+ * SlipMode SLIP_Vel0 (mDBC with vel=0): density of boundary particles
+ * [0,n) extrapolated from their ghost node; writes velrho[p].w. */
+int dsph_mdbc_correction(dsph_ctx* ctx, uint32_t n, const dsph_divdata* dvd,
+                         const double mapposmin[3], const double* d_posxy,
+                         const double* d_posz, const float* d_poscell,
+                         const uint16_t* d_code, const float* d_boundnor,
+                         float* d_velrho);
+
+/* --------------------------------------------------- reductions and dt */
+
+/* Replaces cusph::ComputeVelMod + ReduMaxFloat (JSphGpu_ker.h:281,290):
+ * max over [pini,pini+n) of |v|^2 written to *d_out (device float). */
+int dsph_velmax2(dsph_ctx* ctx, uint32_t n, uint32_t pini,
+                 const float* d_velrho, float* d_out);
+/* Replaces cusph::ComputeAceMod + ReduMaxFloat (JSphGpu_ker.h:287): max of
+ * |ace|^2 over [pini,pini+n). */
+int dsph_acemax2(dsph_ctx* ctx, uint32_t n, uint32_t pini,
+                 const float* d_ace, float* d_out);
+/* Replaces cusph::ReduMaxFloat (JSphGpu_ker.h:281): max of data[ini..ini+n)
+ * (0 when n==0) to *d_out. */
+int dsph_redu_max_f32(dsph_ctx* ctx, uint32_t n, uint32_t ini,
+                      const float* d_data, float* d_out);
+/* Synchronous variant returning the value to the host, as ReduMaxFloat. */
+int dsph_redu_max_f32_sync(dsph_ctx* ctx, uint32_t n, uint32_t ini,
+                           const float* d_data, float* h_out);
+
+/* Replaces JSphGpu::DtVariable (JSphGpu.cpp:1103-1147) on the device so the
+ * step needs no host round trip. Reads d_red[0]=max|v|^2, d_red[1]=ViscDtMax,
+ * d_red[2]=max|ace|^2 and writes d_dt[slot] (FP64):
+ *   dt1=sqrt(h/AceMax), dt2=h/(max(cs0,10*VelMax)+h*ViscDtMax),
+ *   dt=max(cfl*min(dt1,dt2), dtmin); NaN/Inf sets d_flags bit 0. */
+int dsph_dt_variable(dsph_ctx* ctx, const float* d_red, double kernelh,
+                     double cs0, double cfl, double dtmin, double* d_dt,
+                     uint32_t slot, uint32_t* d_flags);
+
+/* End of a Symplectic step on the device (JSphGpuSingle.cpp:819), with
+ * d_dt = {dt_p, dt_c, dtpre, time, dtlast}: time += dtpre; dtlast = dtpre;
+ * dtpre = min(dt_p, dt_c). */
+int dsph_dt_advance(dsph_ctx* ctx, double* d_dt);
+
+/* ------------------------------------------------- integration (cusphs) */
+
+/* Replaces cusphs::ComputeStepSymplecticPre (JSphGpuSimple_ker.h:51) for
+ * DBC/mDBC without floatings/shifting/inout. dtm = dt/2. When d_dt is not
+ * NULL, dtm is read on the device as 0.5*d_dt[dt_slot] instead. */
+int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb,
+                             const float* d_velrhopre, const float* d_ar,
+                             const float* d_ace, double dtm,
+                             const double* d_dt, uint32_t dt_slot,
+                             float rhopzero, float rhopoutmin, float rhopoutmax,
+                             const float gravity[3], uint16_t* d_code,
+                             double* d_movxy, double* d_movz, float* d_velrho);
+/* Replaces cusphs::ComputeStepSymplecticCor (JSphGpuSimple_ker.h:57).
+ * dtm = dt/2; device dt as above (dt = d_dt[dt_slot]). */
+int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb,
+                             const float* d_velrhopre, const float* d_ar,
+                             const float* d_ace, double dtm, double dt,
+                             const double* d_dt, uint32_t dt_slot,
+                             float rhopzero, float rhopoutmin, float rhopoutmax,
+                             const float gravity[3], uint16_t* d_code,
+                             double* d_movxy, double* d_movz, float* d_velrho);
+/* Replaces cusphs::ComputeStepVerlet (JSphGpuSimple_ker.h:46). */
+int dsph_step_verlet(dsph_ctx* ctx, uint32_t np, uint32_t npb,
+                     const float* d_velrho1, const float* d_velrho2,
+                     const float* d_ar, const float* d_ace, double dt, double dt2,
+                     float rhopzero, float rhopoutmin, float rhopoutmax,
+                     const float gravity[3], uint16_t* d_code,
+                     double* d_movxy, double* d_movz, float* d_velrhonew);
+/* Replaces cusph::ComputeStepPos2 (JSphGpu_ker.h:315): x=xpre+mov in FP64
+ * for fluid [npb,np), OUTMOV/OUTPOS/NaN checks and new dcell. */
+int dsph_step_pos2(dsph_ctx* ctx, uint32_t np, uint32_t npb,
+                   const double* d_posxypre, const double* d_poszpre,
+                   const double* d_movxy, const double* d_movz,
+                   double* d_posxy, double* d_posz, uint32_t* d_dcell,
+                   uint16_t* d_code);
+/* Replaces cusph::ComputeStepPos (JSphGpu_ker.h:312): in-place update. */
+int dsph_step_pos(dsph_ctx* ctx, uint32_t np, uint32_t npb,
+                  const double* d_movxy, const double* d_movz,
+                  double* d_posxy, double* d_posz, uint32_t* d_dcell,
+                  uint16_t* d_code);
+
+/* ------------------------------------------------------------ measurement */
+
+/* Replaces cusph::InteractionNg (JDsPips_ker.cu:54-124) + reductions: counts
+ * real pairs (rr2<=(2h)^2 and rr2>=1e-18) and checked candidates, as
+ * {real fluid, real bound, checked fluid, checked bound}. Synchronous. */
+int dsph_pips_count_sync(dsph_ctx* ctx, uint32_t nb, uint32_t pinitb,
+                         uint32_t nf, uint32_t pinitf, const dsph_divdata* dvd,
+                         const uint32_t* d_dcell, const float* d_poscell,
+                         uint64_t h_res[4]);
+/* Same count with the fluid real pairs also split: h_res = {real fluid,
+ * real bound, checked fluid, checked bound, real fluid-boundary subset}
+ * (the algorithmic-work model prices fluid-fluid and fluid-boundary pairs
+ * differently). Synchronous. */
+int dsph_pips_count_split_sync(dsph_ctx* ctx, uint32_t nb, uint32_t pinitb,
+                               uint32_t nf, uint32_t pinitf, const dsph_divdata* dvd,
+                               const uint32_t* d_dcell, const float* d_poscell,
+                               uint64_t h_res[5]);
+
+/* ----------------------------------------------------- whole-step driver */
+
+/*
+ * C++ host driver (mirror of JSphGpuSingle) exposed for bindings. A
+ * simulation owns its particle arrays and calls only the entry points above.
+ */
+typedef struct dsph_sim dsph_sim;
+
+/* Case description: mirror of the <execution><constants> and <parameters>
+ * blocks (JCaseCtes.cpp:208-222, JSph.cpp:663-854) plus the map limits. */
+typedef struct dsph_case {
+  double dp, kernelh, cteb, gamma, rhopzero, cs0;
+  double massfluid, massbound;
+  double gravity[3];
+  double cflnumber;
+  double dtini, dtmin;           /* 0 = derive from h/cs0 (JSph.cpp:1535-1538) */
+  int32_t tstep;                 /* 1 Verlet, 2 Symplectic */
+  int32_t verletsteps;
+  int32_t tkernel, tvisco, tdensity, tboundary;
+  double visco, viscoboundfactor, ddtvalue;
+  double rhopoutmin, rhopoutmax;
+  int32_t cellmode;              /* 1 Full, 2 Half */
+  int32_t celldomfixed;          /* 0: adapt to particles (reference default) */
+  double maprealposmin[3], maprealposmax[3];
+  uint32_t casenp, casenbound, casenfixed, casenmoving, casenfloat, casenfluid;
+} dsph_case;
+
+/* Per-step statistics, mirror of the Run.out/Run.csv key numbers. */
+typedef struct dsph_stats {
+  uint32_t np, npb, npbok, nct;
+  uint32_t nstep;
+  uint32_t npf_out_total;        /* fluid particles excluded so far */
+  double timestep;               /* simulated time */
+  double dt_last;                /* dt of the last step */
+  double velmax, acemax, viscdtmax;
+  uint32_t ndiv, ndivfull;
+} dsph_stats;
+
+/* Creates a simulation from particles sorted by idp, boundary first
+ * (JPartsLoad4 order). h_pos is double3[np], h_velrho float4[np],
+ * h_boundnor float3[casenbound] or NULL. */
+int dsph_sim_create(int device, const dsph_case* cs, uint32_t np,
+                    const uint32_t* h_idp, const double* h_pos,
+                    const float* h_velrho, const float* h_boundnor,
+                    dsph_sim** out);
+int dsph_sim_destroy(dsph_sim* sim);
+const char* dsph_sim_last_error(dsph_sim* sim);
+/* Runs nsteps full time steps (ComputeStep + RunCellDivide). With
+ * sync_dt=0 the step keeps dt on the device (no host round trip). */
+int dsph_sim_run(dsph_sim* sim, uint32_t nsteps, int sync_dt);
+int dsph_sim_stats(dsph_sim* sim, dsph_stats* out);
+/* Downloads current particles (sorted order): idp, pos (double3), velrho,
+ * code. Any pointer may be NULL. Returns the count in *np. */
+int dsph_sim_download(dsph_sim* sim, uint32_t* np, uint32_t* h_idp,
+                      double* h_pos, float* h_velrho, uint16_t* h_code);
+/* Downloads ace (float3), ar of the last interaction (sorted order). */
+int dsph_sim_download_forces(dsph_sim* sim, float* h_ace, float* h_ar);
+/* Counts pairs of the current state (PIPS rule), as
+ * dsph_pips_count_split_sync: {real fluid, real bound, checked fluid,
+ * checked bound, real fluid-boundary}. */
+int dsph_sim_pips(dsph_sim* sim, uint64_t h_res[5]);
+/* Evaluates PreInteraction_Forces + Interaction_Forces on the current state
+ * without advancing it (forces then readable with dsph_sim_download_forces). */
+int dsph_sim_interact(dsph_sim* sim);
+/* Access to the context of a simulation (streams, events). */
+dsph_ctx* dsph_sim_ctx(dsph_sim* sim);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DSPH_ABI_H */

@@ -1,0 +1,709 @@
+// Oracle for the SPH step: a CPU restatement (C++17/OpenMP) of the
+// reference's algorithm (DualSPHysics v5.4.355, paths relative to
+// src/source/).
+//
+// TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and the
+// cpu_baseline leg of bench.py may load it, as the checker or as the timed CPU
+// baseline; the product path (dualsphysics_amd) never calls it.
+//
+// Each function cites the reference code it restates. Pair arithmetic is
+// templated on the real type: float reproduces the reference's FP32 pair
+// arithmetic, double is the FP64 yardstick every parity result is measured
+// against. Three distance/EOS/accumulation semantics are provided:
+//   SEM_GPU : distances from cell-relative FP32 PosCell, EOS rho*(1/rho0),
+//             fluid and boundary neighbours in one accumulator set
+//             (KerInteractionForcesFluid, JSphGpu_ker.cu:786-990);
+//   SEM_CPU : distances float(FP64 difference), EOS rho/rho0, fluid and
+//             boundary neighbours in two passes (JSphCpu.cpp:733-1141,
+//             1315-1372);
+//   SEM_F64 : distances in FP64 from FP64 positions (yardstick).
+//
+// Pinning of each part (what checks it, in tests/): cell code, keys, sort,
+// BeginEndCell and gathers against brute force and the reference CPU build's
+// own outputs; kernels and EOS against closed forms; the interaction against
+// O(N^2) brute-force FP64 sums and against the reference CPU build
+// (oracle/_ref) on identical particles; integrators against the reference
+// CPU build's particle states. See tests/test_oracle_*.py.
+#include <omp.h>
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+typedef uint16_t typecode;
+
+namespace {
+
+// ---- code and cell bits (DualSphDef.h:150-241, JDsDcellDef.h:28-45, DualSphDef.h:661-700)
+constexpr unsigned kSpecial = 0xe000u, kOutIgnore = 0x4000u, kOutMov = 0x6000u,
+                   kOutPos = 0x8000u, kOutRho = 0xa000u, kTypeMask = 0x1800u,
+                   kTypeFloating = 0x1000u, kTypeFluid = 0x1800u;
+inline unsigned cx_of(unsigned dcc, unsigned c) { return c >> ((dcc >> 10) & 31u); }
+inline unsigned cy_of(unsigned dcc, unsigned c) { return (c << (dcc >> 25)) >> ((dcc >> 5) & 31u); }
+inline unsigned cz_of(unsigned dcc, unsigned c) { return (c << (dcc & 31u)) >> (dcc & 31u); }
+inline unsigned mkcell(unsigned dcc, unsigned x, unsigned y, unsigned z) {
+  return (x << ((dcc >> 10) & 31u)) | (y << ((dcc >> 15) & 31u)) | z;
+}
+inline unsigned fbits(float f) { unsigned u; std::memcpy(&u, &f, 4); return u; }
+inline float bitsf(unsigned u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+}  // namespace
+
+extern "C" {
+
+// Constants of one case (JSph members / StCteInteraction, JSphGpu_cte.h:27-62).
+struct orc_cte {
+  float massb, massf, kernelh, kernelsize2, poscellsize, awen, bwenh, cs0, eta2, ddtkh, ddtgz;
+  float scell, kernelsize, dp, cteb, gamma, rhopzero, ovrhopzero, movlimit;
+  unsigned cellcode;
+  double domposmin[3], maprealposmin[3], maprealsize[3];
+  double mapposmin[3];
+};
+
+// Neighbour-search data (StDivDataCpu / StDivDataGpu).
+struct orc_div {
+  int scelldiv, ncx, ncy, ncz, ncw;
+  unsigned cellfluid;
+  int czx, czy, czz;
+  const int* begincell;  // int2 {begin,end} per cell (canonical form)
+};
+
+// JDsDcell::CalcCellCode (JDsDcell.cpp:30-67).
+unsigned orc_cell_code(unsigned ncx, unsigned ncy, unsigned ncz) {
+  auto bits = [](unsigned v) { unsigned n = 2; for (; v >> n; n++) {} return n; };
+  unsigned sx = bits(ncx), sy = bits(ncy), sz = bits(ncz);
+  if (sx + sy + sz > 31) return 0;
+  unsigned rest = 31 - (sx + sy + sz);
+  while (rest) {
+    if (rest) { sx++; rest--; }
+    if (rest) { sy++; rest--; }
+    if (rest) { sz++; rest--; }
+  }
+  return ((sx + 1) << 25) | (sy << 20) | (sz << 15) | ((sy + sz) << 10) | ((sx + 1 + sz) << 5) |
+         (sx + 1 + sy);
+}
+
+// Cell of each particle from FP64 positions (JSph::LoadDcellParticles,
+// JSph.cpp:1794-1816; JSphCpu::UpdatePos, JSphCpu.cpp:1507-1515).
+void orc_dcell(unsigned n, const double* pos, const double domposmin[3], float scell,
+               unsigned dcc, unsigned* dcell) {
+  for (unsigned p = 0; p < n; p++) {
+    const double* x = pos + 3 * size_t(p);
+    dcell[p] = mkcell(dcc, unsigned((x[0] - domposmin[0]) / scell),
+                      unsigned((x[1] - domposmin[1]) / scell), unsigned((x[2] - domposmin[2]) / scell));
+  }
+}
+
+// Min/max cell of the non-excluded particles [pini,pini+n)
+// (JCellDivCpu::LimitsCellBound/Fluid, JCellDivCpu.cpp:269-377).
+void orc_limits(unsigned n, unsigned pini, unsigned dcc, const unsigned* dcell,
+                const typecode* code, unsigned cmin[3], unsigned cmax[3]) {
+  cmin[0] = cmin[1] = cmin[2] = UINT_MAX;
+  cmax[0] = cmax[1] = cmax[2] = 0;
+  for (unsigned p = pini; p < pini + n; p++) {
+    if ((code[p] & kSpecial) >= kOutIgnore) continue;
+    const unsigned c = dcell[p];
+    const unsigned v[3] = {cx_of(dcc, c), cy_of(dcc, c), cz_of(dcc, c)};
+    for (int k = 0; k < 3; k++) { cmin[k] = std::min(cmin[k], v[k]); cmax[k] = std::max(cmax[k], v[k]); }
+  }
+}
+
+// Cell keys (JCellDivCpuSingle::PreSortFull / PreSortFluid,
+// JCellDivCpuSingle.cpp:137-205). full!=0: all [0,np); else fluid [pini,np).
+void orc_presort(int full, unsigned np, unsigned pini, unsigned dcc, const unsigned* dcell,
+                 const typecode* code, const unsigned cellmin[3], const unsigned nc[3],
+                 unsigned* cellpart) {
+  const unsigned nsheet = nc[0] * nc[1], nct = nsheet * nc[2];
+  const unsigned bbi = nct, bf = nct + 1, bbo = bf + nct, bfo = bbo + 1, bboi = bfo + 1,
+                 bfoi = bboi + 1;
+  for (unsigned p = (full ? 0 : pini); p < np; p++) {
+    const unsigned c = dcell[p];
+    const unsigned x = cx_of(dcc, c) - cellmin[0], y = cy_of(dcc, c) - cellmin[1],
+                   z = cz_of(dcc, c) - cellmin[2];
+    const unsigned key = x + y * nc[0] + z * nsheet;
+    const unsigned type = code[p] & kTypeMask, out = code[p] & kSpecial;
+    unsigned box;
+    if (full && type < kTypeFloating)
+      box = out < kOutIgnore ? ((x < nc[0] && y < nc[1] && z < nc[2]) ? key : bbi)
+                             : (out == kOutIgnore ? bboi : bbo);
+    else
+      box = out <= kOutIgnore ? (out < kOutIgnore ? bf + key : bfoi)
+                              : (type == kTypeFloating ? bbo : bfo);
+    cellpart[p] = box;
+  }
+}
+
+// Stable counting sort by key over [pini,np) (JCellDivCpuSingle::MakeSortFull /
+// MakeSortFluid, JCellDivCpuSingle.cpp:207-262): sortpart[i] is the index of the
+// particle that goes to slot i; keys sorted in place; begin/end of every key in
+// [key0, nkeys) written as int2 {begin,end} ((0,0) when empty, as the GPU).
+void orc_sort_cells(unsigned np, unsigned pini, unsigned nkeys, unsigned key0, unsigned* cellpart,
+                    unsigned* sortpart, int* beginend) {
+  std::vector<unsigned> cnt(nkeys + 1, 0), beg(nkeys + 1, 0);
+  for (unsigned p = pini; p < np; p++) cnt[cellpart[p]]++;
+  beg[key0] = pini;
+  for (unsigned k = key0; k < nkeys; k++) beg[k + 1] = beg[k] + cnt[k];
+  std::vector<unsigned> fill(beg.begin(), beg.end());
+  std::vector<unsigned> sortedkeys(np);
+  for (unsigned p = 0; p < pini; p++) sortpart[p] = p;
+  for (unsigned p = pini; p < np; p++) {
+    const unsigned k = cellpart[p];
+    sortpart[fill[k]] = p;
+    sortedkeys[fill[k]] = k;
+    fill[k]++;
+  }
+  for (unsigned p = pini; p < np; p++) cellpart[p] = sortedkeys[p];
+  if (beginend)
+    for (unsigned k = key0; k < nkeys; k++) {
+      beginend[2 * size_t(k)] = cnt[k] ? int(beg[k]) : 0;
+      beginend[2 * size_t(k) + 1] = cnt[k] ? int(beg[k + 1]) : 0;
+    }
+}
+
+// PosCell of FP64 positions (KerUpdatePosCell, JSphGpuSimple_ker.cu:41-59).
+void orc_poscell(unsigned n, const double* pos, const double posmin[3], float pcs, float* poscell) {
+  for (unsigned p = 0; p < n; p++) {
+    const double* x = pos + 3 * size_t(p);
+    const double dx = x[0] - posmin[0], dy = x[1] - posmin[1], dz = x[2] - posmin[2];
+    const unsigned cx = unsigned(dx / pcs), cy = unsigned(dy / pcs), cz = unsigned(dz / pcs);
+    float* o = poscell + 4 * size_t(p);
+    o[0] = float(dx - double(pcs) * cx);
+    o[1] = float(dy - double(pcs) * cy);
+    o[2] = float(dz - double(pcs) * cz);
+    o[3] = bitsf((cx << 19) | (cy << 9) | cz);
+  }
+}
+
+}  // extern "C"
+
+namespace {
+
+enum { SEM_GPU = 0, SEM_CPU = 1, SEM_F64 = 2 };
+
+// Wendland C2 (FunSphKernel.h:198-244) in precision R.
+template <typename R>
+inline R wend_fac(R rr2, R h, R bwenh) {
+  const R rad = std::sqrt(rr2);
+  const R qq = rad / h;
+  const R w1 = R(1) - R(0.5) * qq;
+  return bwenh * w1 * w1 * w1;
+}
+template <typename R>
+inline R wend_wab(R rr2, R h, R awen) {
+  const R rad = std::sqrt(rr2);
+  const R qq = rad / h;
+  const R wqq = qq + qq + R(1);
+  const R w1 = R(1) - R(0.5) * qq;
+  const R w2 = w1 * w1;
+  return awen * wqq * w2 * w2;
+}
+
+struct Opts {
+  int sem, tvisco, tddt, velgrad;
+  float viscob, viscof;
+};
+
+// Relative position p1-p2 in the chosen semantics.
+template <typename R>
+inline void rel(int sem, const orc_cte& c, const float* ps, const double* pos, unsigned p1,
+                unsigned p2, R& dx, R& dy, R& dz) {
+  if (sem == SEM_GPU) {
+    const float* a = ps + 4 * size_t(p1);
+    const float* b = ps + 4 * size_t(p2);
+    const unsigned ua = fbits(a[3]), ub = fbits(b[3]);
+    const int ax = int(ua >> 19), ay = int((ua & 0x0007fe00u) >> 9), az = int(ua & 0x1ffu);
+    const int bx = int(ub >> 19), by = int((ub & 0x0007fe00u) >> 9), bz = int(ub & 0x1ffu);
+    dx = R(a[0] - b[0] + c.poscellsize * float(ax - bx));
+    dy = R(a[1] - b[1] + c.poscellsize * float(ay - by));
+    dz = R(a[2] - b[2] + c.poscellsize * float(az - bz));
+  } else if (sem == SEM_CPU) {
+    dx = R(float(pos[3 * size_t(p1)] - pos[3 * size_t(p2)]));
+    dy = R(float(pos[3 * size_t(p1) + 1] - pos[3 * size_t(p2) + 1]));
+    dz = R(float(pos[3 * size_t(p1) + 2] - pos[3 * size_t(p2) + 2]));
+  } else {
+    dx = R(pos[3 * size_t(p1)] - pos[3 * size_t(p2)]);
+    dy = R(pos[3 * size_t(p1) + 1] - pos[3 * size_t(p2) + 1]);
+    dz = R(pos[3 * size_t(p1) + 2] - pos[3 * size_t(p2) + 2]);
+  }
+}
+
+struct Row { unsigned a, b; };
+
+// Cell rows around cell (cx,cy,cz) of the current domain (nsearch::Init +
+// ParticleRange, JCellSearch_inline.h:33-85; GPU form JCellSearch_iker.h:34-166).
+inline int rows_at(const orc_div& d, int cx, int cy, int cz, bool fluidcells, Row* out) {
+  const int s = d.scelldiv;
+  const int x0 = cx - std::min(cx, s), x1 = cx + std::min(d.ncx - cx - 1, s) + 1;
+  const int y0 = cy - std::min(cy, s), y1 = cy + std::min(d.ncy - cy - 1, s) + 1;
+  const int z0 = cz - std::min(cz, s), z1 = cz + std::min(d.ncz - cz - 1, s) + 1;
+  const int base = fluidcells ? int(d.cellfluid) : 0;
+  int nr = 0;
+  for (int z = z0; z < z1; z++)
+    for (int y = y0; y < y1; y++) {
+      unsigned a = 0, b = 0;
+      const int rb = base + z * d.ncw + y * d.ncx;
+      for (int x = x0; x < x1; x++) {
+        const int* be = d.begincell + 2 * size_t(rb + x);
+        if (be[1]) {
+          if (!b) a = unsigned(be[0]);
+          b = unsigned(be[1]);
+        }
+      }
+      out[nr++] = Row{a, b};
+    }
+  return nr;
+}
+
+// Fluid particle p1 against the rows of one kind (JSphCpu.cpp:733-1141 /
+// KerInteractionForcesFluidBox JSphGpu_ker.cu:539-777).
+template <typename R>
+struct FAcc {
+  R ax = 0, ay = 0, az = 0, ar = 0, visc = 0, delta = 0;
+  R s[6] = {0, 0, 0, 0, 0, 0};
+  R g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  bool delta_off = false;
+};
+
+template <typename R>
+void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, int nr,
+                unsigned p1, const float* ps, const double* pos, const float* velrho,
+                const R* press, const float* tau, FAcc<R>& a) {
+  const R massp2 = R(boundp2 ? c.massb : c.massf);
+  const R visco = R(boundp2 ? o.viscob : o.viscof);
+  const float* v1 = velrho + 4 * size_t(p1);
+  const R v1x = R(v1[0]), v1y = R(v1[1]), v1z = R(v1[2]), rho1 = R(v1[3]);
+  const R press1 = press[p1];
+  const R ks2 = R(c.kernelsize2), eta2 = R(c.eta2), h = R(c.kernelh), bwenh = R(c.bwenh);
+  const R cbar = R(c.cs0), ddtkh = R(c.ddtkh), ddtgz = R(c.ddtgz), rho0 = R(c.rhopzero);
+  const R invgamma = R(1) / R(c.gamma);
+  for (int r = 0; r < nr; r++)
+    for (unsigned p2 = rows[r].a; p2 < rows[r].b; p2++) {
+      R dx, dy, dz;
+      rel<R>(o.sem, c, ps, pos, p1, p2, dx, dy, dz);
+      const R rr2 = dx * dx + dy * dy + dz * dz;
+      if (!(rr2 <= ks2 && rr2 >= R(1e-18f))) continue;
+      const R fac = wend_fac<R>(rr2, h, bwenh);
+      const R frx = fac * dx, fry = fac * dy, frz = fac * dz;
+      const float* v2 = velrho + 4 * size_t(p2);
+      const R rho2 = R(v2[3]);
+      {
+        const R prs = (press1 + press[p2]) / (rho1 * rho2);
+        const R p_vpm = -prs * massp2;
+        a.ax += p_vpm * frx; a.ay += p_vpm * fry; a.az += p_vpm * frz;
+      }
+      const R dvx = v1x - R(v2[0]), dvy = v1y - R(v2[1]), dvz = v1z - R(v2[2]);
+      a.ar += massp2 * (dvx * frx + dvy * fry + dvz * frz) * (rho1 / rho2);
+      const R dot3 = dx * frx + dy * fry + dz * frz;
+      if (o.tddt == 1 && !a.delta_off) {
+        const R visc_densi = ddtkh * cbar * (rho1 / rho2 - R(1)) / (rr2 + eta2);
+        if (boundp2) a.delta_off = true;
+        else a.delta += visc_densi * dot3 * massp2;
+      }
+      if ((o.tddt == 2 || (o.tddt == 3 && !boundp2)) && !a.delta_off) {
+        const R rh = R(1) + ddtgz * dz;
+        const R drho = rho0 * std::pow(rh, invgamma) - rho0;
+        const R visc_densi = ddtkh * cbar * ((rho2 - rho1) - drho) / (rr2 + eta2);
+        const R delta = visc_densi * dot3 * massp2 / rho2;
+        if (boundp2) a.delta_off = true;
+        else a.delta -= delta;
+      }
+      const R dot = dx * dvx + dy * dvy + dz * dvz;
+      const R dot_rr2 = dot / (rr2 + eta2);
+      a.visc = std::max(dot_rr2, a.visc);
+      if (o.tvisco == 1) {
+        if (dot < 0) {
+          const R amubar = h * dot_rr2;
+          const R robar = (rho1 + rho2) * R(0.5);
+          const R pi_visc = (-visco * cbar * amubar / robar) * massp2;
+          a.ax -= pi_visc * frx; a.ay -= pi_visc * fry; a.az -= pi_visc * frz;
+        }
+      } else {
+        const R robar2 = rho1 + rho2;
+        const R temp = R(4) * visco / ((rr2 + eta2) * robar2);
+        const R vtemp = massp2 * temp * dot3;
+        a.ax += vtemp * dvx; a.ay += vtemp * dvy; a.az += vtemp * dvz;
+      }
+      if (o.tvisco == 2) {
+        const float* t1 = tau + 6 * size_t(p1);
+        R t[6] = {R(t1[0]), R(t1[1]), R(t1[2]), R(t1[3]), R(t1[4]), R(t1[5])};
+        if (!boundp2) {
+          const float* t2 = tau + 6 * size_t(p2);
+          for (int k = 0; k < 6; k++) t[k] += R(t2[k]);
+        }
+        a.ax += massp2 * (t[0] * frx + t[1] * fry + t[2] * frz);
+        a.ay += massp2 * (t[1] * frx + t[3] * fry + t[4] * frz);
+        a.az += massp2 * (t[2] * frx + t[4] * fry + t[5] * frz);
+        const R volp2 = -massp2 / rho2;
+        R dv = dvx * volp2; a.s[0] += dv * frx; a.s[1] += dv * fry; a.s[2] += dv * frz;
+        dv = dvy * volp2;   a.s[1] += dv * frx; a.s[3] += dv * fry; a.s[4] += dv * frz;
+        dv = dvz * volp2;   a.s[2] += dv * frx; a.s[4] += dv * fry; a.s[5] += dv * frz;
+      }
+      if (o.velgrad) {
+        const R volp2 = -massp2 / rho2;
+        const R ux = dvx * volp2, uy = dvy * volp2, uz = dvz * volp2;
+        a.g[0] += ux * frx; a.g[1] += ux * fry; a.g[2] += ux * frz;
+        a.g[3] += uy * frx; a.g[4] += uy * fry; a.g[5] += uy * frz;
+        a.g[6] += uz * frx; a.g[7] += uz * fry; a.g[8] += uz * frz;
+      }
+    }
+}
+
+template <typename R>
+R eos(int sem, const orc_cte& c, float rho) {
+  if (sem == SEM_GPU)
+    return R(c.cteb) * (std::pow(R(rho) * R(c.ovrhopzero), R(c.gamma)) - R(1));
+  return R(c.cteb) * (std::pow(R(rho) / R(c.rhopzero), R(c.gamma)) - R(1));
+}
+
+template <typename R>
+void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np, unsigned npb,
+                 unsigned npbok, const unsigned* dcell, const float* ps, const double* pos,
+                 const float* velrho, const float* tau, float* ace, float* ar, float* viscdt,
+                 float* strain, float* velgrad) {
+  std::vector<R> press(np);
+#pragma omp parallel for schedule(static)
+  for (long p = 0; p < long(np); p++) press[p] = eos<R>(o.sem, c, velrho[4 * size_t(p) + 3]);
+  // Fluid particles.
+#pragma omp parallel for schedule(guided)
+  for (long pp = long(npb); pp < long(np); pp++) {
+    const unsigned p1 = unsigned(pp);
+    Row rf[25], rb[25];
+    const unsigned cel = dcell[p1];
+    const int cx = int(cx_of(c.cellcode, cel)) - d.czx, cy = int(cy_of(c.cellcode, cel)) - d.czy,
+              cz = int(cz_of(c.cellcode, cel)) - d.czz;
+    const int nf = rows_at(d, cx, cy, cz, true, rf);
+    const int nb = rows_at(d, cx, cy, cz, false, rb);
+    R oax, oay, oaz, oar, ovisc;
+    R os[6], og[9];
+    if (o.sem == SEM_CPU) {
+      // Two passes, each stored separately (JSphCpu.cpp:1331-1349).
+      FAcc<R> a1, a2;
+      fluid_rows<R>(o, c, false, rf, nf, p1, ps, pos, velrho, press.data(), tau, a1);
+      fluid_rows<R>(o, c, true, rb, nb, p1, ps, pos, velrho, press.data(), tau, a2);
+      R ar1 = a1.ar + (o.tddt && !a1.delta_off ? a1.delta : R(0));
+      R ar2 = a2.ar + (o.tddt && !a2.delta_off ? a2.delta : R(0));
+      oax = R(float(a1.ax)) + a2.ax; oay = R(float(a1.ay)) + a2.ay; oaz = R(float(a1.az)) + a2.az;
+      oar = R(float(ar1)) + ar2;
+      ovisc = std::max(a1.visc, a2.visc);
+      for (int k = 0; k < 6; k++) os[k] = a1.s[k] + a2.s[k];
+      for (int k = 0; k < 9; k++) og[k] = a1.g[k] + a2.g[k];
+    } else {
+      FAcc<R> a;
+      fluid_rows<R>(o, c, false, rf, nf, p1, ps, pos, velrho, press.data(), tau, a);
+      fluid_rows<R>(o, c, true, rb, nb, p1, ps, pos, velrho, press.data(), tau, a);
+      oax = a.ax; oay = a.ay; oaz = a.az;
+      oar = a.ar + (o.tddt && !a.delta_off ? a.delta : R(0));
+      ovisc = a.visc;
+      for (int k = 0; k < 6; k++) os[k] = a.s[k];
+      for (int k = 0; k < 9; k++) og[k] = a.g[k];
+    }
+    ace[3 * size_t(p1)] += float(oax);
+    ace[3 * size_t(p1) + 1] += float(oay);
+    ace[3 * size_t(p1) + 2] += float(oaz);
+    ar[p1] += float(oar);
+    if (float(ovisc) > viscdt[p1]) viscdt[p1] = float(ovisc);
+    if (o.tvisco == 2)
+      for (int k = 0; k < 6; k++) strain[6 * size_t(p1) + k] += float(os[k]);
+    if (o.velgrad && velgrad)
+      for (int k = 0; k < 9; k++) velgrad[9 * size_t(p1) + k] = float(og[k]);
+  }
+  // Boundary particles (JSphCpu.cpp:633-703 / KerInteractionForcesBound).
+  const R ks2 = R(c.kernelsize2), eta2 = R(c.eta2), h = R(c.kernelh), bwenh = R(c.bwenh);
+  const R massf = R(c.massf);
+#pragma omp parallel for schedule(guided)
+  for (long pp = 0; pp < long(npbok); pp++) {
+    const unsigned p1 = unsigned(pp);
+    Row rf[25];
+    const unsigned cel = dcell[p1];
+    const int cx = int(cx_of(c.cellcode, cel)) - d.czx, cy = int(cy_of(c.cellcode, cel)) - d.czy,
+              cz = int(cz_of(c.cellcode, cel)) - d.czz;
+    const int nf = rows_at(d, cx, cy, cz, true, rf);
+    const float* v1 = velrho + 4 * size_t(p1);
+    R arp = 0, visc = 0;
+    for (int r = 0; r < nf; r++)
+      for (unsigned p2 = rf[r].a; p2 < rf[r].b; p2++) {
+        R dx, dy, dz;
+        rel<R>(o.sem, c, ps, pos, p1, p2, dx, dy, dz);
+        const R rr2 = dx * dx + dy * dy + dz * dz;
+        if (!(rr2 <= ks2 && rr2 >= R(1e-18f))) continue;
+        const R fac = wend_fac<R>(rr2, h, bwenh);
+        const R frx = fac * dx, fry = fac * dy, frz = fac * dz;
+        const float* v2 = velrho + 4 * size_t(p2);
+        const R dvx = R(v1[0]) - R(v2[0]), dvy = R(v1[1]) - R(v2[1]), dvz = R(v1[2]) - R(v2[2]);
+        arp += massf * (dvx * frx + dvy * fry + dvz * frz) * (R(v1[3]) / R(v2[3]));
+        const R dot = dx * dvx + dy * dvy + dz * dvz;
+        visc = std::max(dot / (rr2 + eta2), visc);
+      }
+    ar[p1] += float(arp);
+    if (float(visc) > viscdt[p1]) viscdt[p1] = float(visc);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
+// Force interaction (see header). prec: 0 float, 1 double. sem: SEM_*.
+// ace/ar/viscdt (and strain) accumulate as in the reference.
+void orc_interaction(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
+                     const orc_cte* c, const orc_div* d, unsigned np, unsigned npb, unsigned npbok,
+                     const unsigned* dcell, const float* poscell, const double* pos,
+                     const float* velrho, const float* tau, float* ace, float* ar,
+                     float* viscdt, float* strain, float* velgrad) {
+  Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof};
+  if (prec == 0)
+    interaction<float>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
+                       viscdt, strain, velgrad);
+  else
+    interaction<double>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
+                        viscdt, strain, velgrad);
+}
+
+// SPS stress tau/rho^2 (JSphCpu.cpp:1279-1309).
+void orc_sps_tau(unsigned np, unsigned npb, float smag, float blin, const float* velrho,
+                 const float* s2, float* tau) {
+  for (unsigned p = npb; p < np; p++) {
+    const float* s = s2 + 6 * size_t(p);
+    const float xx = s[0], xy = s[1], xz = s[2], yy = s[3], yz = s[4], zz = s[5];
+    const float pow1 = xx * xx + yy * yy + zz * zz;
+    const float prr = xy * xy + xz * xz + yz * yz + pow1 + pow1;
+    const float visc_sps = smag * std::sqrt(prr);
+    const float div_u = xx + yy + zz;
+    const float sps_k = visc_sps * div_u;
+    const float sps_blin = blin * prr;
+    const float sumsps = -(sps_k + sps_blin);
+    const float two = visc_sps + visc_sps;
+    const float one_rho = 1.0f / velrho[4 * size_t(p) + 3];
+    float* t = tau + 6 * size_t(p);
+    t[0] = one_rho * (two * xx + sumsps);
+    t[1] = one_rho * (visc_sps * xy);
+    t[2] = one_rho * (visc_sps * xz);
+    t[3] = one_rho * (two * yy + sumsps);
+    t[4] = one_rho * (visc_sps * yz);
+    t[5] = one_rho * (two * zz + sumsps);
+  }
+}
+
+// Symplectic predictor (JSphCpu.cpp:1687-1844; KerComputeStepSymplecticPre,
+// JSphGpuSimple_ker.cu:265-353) for DBC, no floatings/shifting.
+void orc_sym_pre(unsigned np, unsigned npb, const float* vrpre, const float* ar, const float* ace,
+                 double dtm, float rho0, float rhomin, float rhomax, const float g[3],
+                 typecode* code, double* mov, float* velrho) {
+  for (unsigned p = 0; p < np; p++) {
+    const float* pre = vrpre + 4 * size_t(p);
+    float* nw = velrho + 4 * size_t(p);
+    if (p < npb) {
+      float w = float(double(pre[3]) + dtm * ar[p]);
+      w = (w < rho0 ? rho0 : w);
+      nw[0] = pre[0]; nw[1] = pre[1]; nw[2] = pre[2]; nw[3] = w;
+      continue;
+    }
+    const float rhonew = float(double(pre[3]) + dtm * ar[p]);
+    mov[3 * size_t(p)] = double(pre[0]) * dtm;
+    mov[3 * size_t(p) + 1] = double(pre[1]) * dtm;
+    mov[3 * size_t(p) + 2] = double(pre[2]) * dtm;
+    for (int k = 0; k < 3; k++)
+      nw[k] = float(double(pre[k]) + (double(ace[3 * size_t(p) + k]) + g[k]) * dtm);
+    nw[3] = rhonew;
+    if ((rhonew < rhomin || rhonew > rhomax) && (code[p] & kSpecial) == 0)
+      code[p] = typecode((code[p] & ~kSpecial) | kOutRho);
+  }
+}
+
+// Symplectic corrector (JSphCpu.cpp:1850-2029; KerComputeStepSymplecticCor).
+void orc_sym_cor(unsigned np, unsigned npb, const float* vrpre, const float* ar, const float* ace,
+                 double dtm, double dt, float rho0, float rhomin, float rhomax, const float g[3],
+                 typecode* code, double* mov, float* velrho) {
+  for (unsigned p = 0; p < np; p++) {
+    const float* pre = vrpre + 4 * size_t(p);
+    float* nw = velrho + 4 * size_t(p);
+    const double eps = (-double(ar[p]) / double(nw[3])) * dt;
+    const float rhonew = float(double(pre[3]) * (2. - eps) / (2. + eps));
+    if (p < npb) {
+      nw[0] = nw[1] = nw[2] = 0;
+      nw[3] = (rhonew < rho0 ? rho0 : rhonew);
+      continue;
+    }
+    float v[3];
+    for (int k = 0; k < 3; k++)
+      v[k] = float(double(pre[k]) + (double(ace[3 * size_t(p) + k]) + g[k]) * dt);
+    for (int k = 0; k < 3; k++) mov[3 * size_t(p) + k] = (double(pre[k]) + double(v[k])) * dtm;
+    nw[0] = v[0]; nw[1] = v[1]; nw[2] = v[2]; nw[3] = rhonew;
+    if ((rhonew < rhomin || rhonew > rhomax) && (code[p] & kSpecial) == 0)
+      code[p] = typecode((code[p] & ~kSpecial) | kOutRho);
+  }
+}
+
+// Verlet (JSphCpu.cpp:1523-1680; KerComputeStepVerlet JSphGpuSimple_ker.cu:132-225).
+void orc_verlet(unsigned np, unsigned npb, const float* vr1, const float* vr2, const float* ar,
+                const float* ace, double dt, double dt2, float rho0, float rhomin, float rhomax,
+                const float g[3], typecode* code, double* mov, float* vrnew) {
+  const double dt205 = 0.5 * dt * dt;
+  for (unsigned p = 0; p < np; p++) {
+    float* nw = vrnew + 4 * size_t(p);
+    if (p < npb) {
+      float w = float(double(vr2[4 * size_t(p) + 3]) + dt2 * ar[p]);
+      nw[0] = nw[1] = nw[2] = 0;
+      nw[3] = (w < rho0 ? rho0 : w);
+      continue;
+    }
+    const float rhonew = float(double(vr2[4 * size_t(p) + 3]) + dt2 * ar[p]);
+    double acg[3];
+    for (int k = 0; k < 3; k++) acg[k] = double(ace[3 * size_t(p) + k]) + g[k];
+    float v[3];
+    for (int k = 0; k < 3; k++) {
+      mov[3 * size_t(p) + k] = double(vr1[4 * size_t(p) + k]) * dt + acg[k] * dt205;
+      v[k] = float(double(vr2[4 * size_t(p) + k]) + acg[k] * dt2);
+    }
+    nw[0] = v[0]; nw[1] = v[1]; nw[2] = v[2]; nw[3] = rhonew;
+    if ((rhonew < rhomin || rhonew > rhomax) && (code[p] & kSpecial) == 0)
+      code[p] = typecode((code[p] & ~kSpecial) | kOutRho);
+  }
+}
+
+// Position update of fluid [npb,np): x = xpre + mov in FP64, OUTMOV/OUTPOS
+// checks and new cell (JSphCpu::UpdatePos, JSphCpu.cpp:1464-1516).
+void orc_step_pos(unsigned np, unsigned npb, const orc_cte* c, const double* pospre,
+                  const double* mov, double* pos, unsigned* dcell, typecode* code) {
+  for (unsigned p = npb; p < np; p++) {
+    const bool outrho = (code[p] & kSpecial) == kOutRho;
+    const double* m = mov + 3 * size_t(p);
+    const bool outmov = (std::fabs(float(m[0])) > c->movlimit || std::fabs(float(m[1])) > c->movlimit ||
+                         std::fabs(float(m[2])) > c->movlimit);
+    double x[3];
+    for (int k = 0; k < 3; k++) x[k] = pospre[3 * size_t(p) + k] + m[k];
+    double dd[3];
+    bool out = false;
+    for (int k = 0; k < 3; k++) {
+      dd[k] = x[k] - c->maprealposmin[k];
+      if (dd[k] != dd[k] || dd[k] < 0 || dd[k] >= c->maprealsize[k]) out = true;
+    }
+    for (int k = 0; k < 3; k++) pos[3 * size_t(p) + k] = x[k];
+    if (outrho || outmov || out) {
+      unsigned rc = code[p] & ~kSpecial;
+      rc |= out ? kOutPos : (outrho ? kOutRho : kOutMov);
+      code[p] = typecode(rc);
+      dcell[p] = 0xffffffffu;
+    } else {
+      dcell[p] = mkcell(c->cellcode, unsigned(dd[0] / c->scell), unsigned(dd[1] / c->scell),
+                        unsigned(dd[2] / c->scell));
+    }
+  }
+}
+
+// DtVariable (JSphGpu.cpp:1103-1147; JSphCpu.cpp:2035-2078).
+double orc_dt_variable(double velmax, double acemax, float viscdtmax, float kernelh, double cs0,
+                       double cfl, double dtmin) {
+  const double dt1 = (acemax ? std::sqrt(double(kernelh) / acemax) : DBL_MAX);
+  const double dt2 = double(kernelh) / (std::max(cs0, velmax * 10.) + double(kernelh) * viscdtmax);
+  double dt = cfl * std::min(dt1, dt2);
+  if (dt < dtmin) dt = dtmin;
+  return dt;
+}
+
+// Real and checked neighbour counts (JDsPips.cpp ComputeCpu; JDsPips_ker.cu:54-124):
+// res = {real fluid, real bound, checked fluid, checked bound}.
+void orc_pips(const orc_cte* c, const orc_div* d, unsigned np, unsigned npb, unsigned npbok,
+              const unsigned* dcell, const float* poscell, unsigned long long res[4]) {
+  unsigned long long rf = 0, rb = 0, cf = 0, cb = 0;
+#pragma omp parallel for schedule(guided) reduction(+ : rf, rb, cf, cb)
+  for (long pp = 0; pp < long(np); pp++) {
+    const unsigned p1 = unsigned(pp);
+    const bool fluid = p1 >= npb;
+    if (!fluid && p1 >= npbok) continue;
+    Row rows[50];
+    const unsigned cel = dcell[p1];
+    const int cx = int(cx_of(c->cellcode, cel)) - d->czx, cy = int(cy_of(c->cellcode, cel)) - d->czy,
+              cz = int(cz_of(c->cellcode, cel)) - d->czz;
+    int nr = rows_at(*d, cx, cy, cz, true, rows);
+    if (fluid) nr += rows_at(*d, cx, cy, cz, false, rows + nr);
+    unsigned long long nreal = 0, nchk = 0;
+    for (int r = 0; r < nr; r++) {
+      for (unsigned p2 = rows[r].a; p2 < rows[r].b; p2++) {
+        float dx, dy, dz;
+        rel<float>(SEM_GPU, *c, poscell, nullptr, p1, p2, dx, dy, dz);
+        const float rr2 = dx * dx + dy * dy + dz * dz;
+        if (rr2 <= c->kernelsize2 && rr2 >= 1e-18f) nreal++;
+      }
+      nchk += rows[r].b - rows[r].a;
+    }
+    if (fluid) { rf += nreal; cf += nchk; } else { rb += nreal; cb += nchk; }
+  }
+  res[0] = rf; res[1] = rb; res[2] = cf; res[3] = cb;
+}
+
+// Brute-force O(N^2) neighbour sums in FP64 from FP64 positions: the
+// interaction without any cell list (checks the cell-list traversal).
+void orc_bruteforce_fp64(const orc_cte* c, int tvisco, int tddt, float viscob, float viscof,
+                         unsigned np, unsigned npb, const double* pos, const float* velrho,
+                         double* ace, double* ar, double* viscdt) {
+  const double h = c->kernelh, ks2 = c->kernelsize2, eta2 = c->eta2, cbar = c->cs0;
+  std::vector<double> press(np);
+  for (unsigned p = 0; p < np; p++)
+    press[p] = double(c->cteb) * (std::pow(double(velrho[4 * size_t(p) + 3]) * double(c->ovrhopzero),
+                                           double(c->gamma)) - 1.0);
+#pragma omp parallel for schedule(dynamic, 16)
+  for (long pp = 0; pp < long(np); pp++) {
+    const unsigned p1 = unsigned(pp);
+    const bool fl1 = p1 >= npb;
+    double a[3] = {0, 0, 0}, arp = 0, visc = 0, delta = 0;
+    bool delta_off = false;
+    const float* v1 = velrho + 4 * size_t(p1);
+    // Same neighbour order class as the reference: fluid first, then boundary.
+    for (int pass = 0; pass < 2; pass++) {
+      const bool boundp2 = pass == 1;
+      if (!fl1 && boundp2) break;
+      const unsigned q0 = boundp2 ? 0 : npb, q1 = boundp2 ? npb : np;
+      for (unsigned p2 = q0; p2 < q1; p2++) {
+        const double dx = pos[3 * size_t(p1)] - pos[3 * size_t(p2)];
+        const double dy = pos[3 * size_t(p1) + 1] - pos[3 * size_t(p2) + 1];
+        const double dz = pos[3 * size_t(p1) + 2] - pos[3 * size_t(p2) + 2];
+        const double rr2 = dx * dx + dy * dy + dz * dz;
+        if (!(rr2 <= ks2 && rr2 >= 1e-18)) continue;
+        const double rad = std::sqrt(rr2), qq = rad / h, w1 = 1 - 0.5 * qq;
+        const double fac = double(c->bwenh) * w1 * w1 * w1;
+        const double fr[3] = {fac * dx, fac * dy, fac * dz};
+        const float* v2 = velrho + 4 * size_t(p2);
+        const double rho1 = v1[3], rho2 = v2[3];
+        const double dv[3] = {double(v1[0]) - v2[0], double(v1[1]) - v2[1], double(v1[2]) - v2[2]};
+        const double massp2 = boundp2 ? c->massb : c->massf;
+        const double dvfr = dv[0] * fr[0] + dv[1] * fr[1] + dv[2] * fr[2];
+        const double dot = dx * dv[0] + dy * dv[1] + dz * dv[2];
+        visc = std::max(dot / (rr2 + eta2), visc);
+        if (!fl1) {
+          arp += double(c->massf) * dvfr * (rho1 / rho2);
+          continue;
+        }
+        const double prs = (press[p1] + press[p2]) / (rho1 * rho2);
+        for (int k = 0; k < 3; k++) a[k] += -prs * massp2 * fr[k];
+        arp += massp2 * dvfr * (rho1 / rho2);
+        const double dot3 = dx * fr[0] + dy * fr[1] + dz * fr[2];
+        if ((tddt == 2 || (tddt == 3 && !boundp2)) && !delta_off) {
+          const double rh = 1 + double(c->ddtgz) * dz;
+          const double drho = double(c->rhopzero) * std::pow(rh, 1.0 / double(c->gamma)) - c->rhopzero;
+          const double vd = double(c->ddtkh) * cbar * ((rho2 - rho1) - drho) / (rr2 + eta2);
+          if (boundp2) delta_off = true;
+          else delta -= vd * dot3 * massp2 / rho2;
+        }
+        const double visco = boundp2 ? viscob : viscof;
+        if (tvisco == 1) {
+          if (dot < 0) {
+            const double pi = (-visco * cbar * (h * dot / (rr2 + eta2)) / ((rho1 + rho2) * 0.5)) * massp2;
+            for (int k = 0; k < 3; k++) a[k] -= pi * fr[k];
+          }
+        } else {
+          const double vt = massp2 * (4 * visco / ((rr2 + eta2) * (rho1 + rho2))) * dot3;
+          for (int k = 0; k < 3; k++) a[k] += vt * dv[k];
+        }
+      }
+    }
+    for (int k = 0; k < 3; k++) ace[3 * size_t(p1) + k] = a[k];
+    ar[p1] = arp + (tddt && !delta_off ? delta : 0);
+    viscdt[p1] = visc;
+  }
+}
+
+}  // extern "C"

@@ -1,0 +1,423 @@
+// Oracle whole-step driver: the reference CPU solver flow (JSphCpuSingle::
+// ComputeStep_Sym / ComputeStep_Ver + RunCellDivide, JSphCpuSingle.cpp:
+// 510-587, 685-932; JCellDivCpuSingle::Divide, JCellDivCpuSingle.cpp:280-350)
+// over the oracle functions of dsph_oracle.cpp.
+//
+// TEST INFRASTRUCTURE ONLY (see dsph_oracle.cpp). Used as the end-to-end
+// parity checker and, timed on the host cores, as the CPU baseline.
+//
+// Pair semantics follow the GPU path (SEM_GPU, FP32) unless the simulation is
+// created with prec=1 (FP64 yardstick with FP64 distances).
+#include <omp.h>
+#include <algorithm>
+#include <cfloat>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "dsph_abi.h"
+
+typedef uint16_t typecode;
+
+extern "C" {
+struct orc_cte {
+  float massb, massf, kernelh, kernelsize2, poscellsize, awen, bwenh, cs0, eta2, ddtkh, ddtgz;
+  float scell, kernelsize, dp, cteb, gamma, rhopzero, ovrhopzero, movlimit;
+  unsigned cellcode;
+  double domposmin[3], maprealposmin[3], maprealsize[3];
+  double mapposmin[3];
+};
+struct orc_div {
+  int scelldiv, ncx, ncy, ncz, ncw;
+  unsigned cellfluid;
+  int czx, czy, czz;
+  const int* begincell;
+};
+unsigned orc_cell_code(unsigned, unsigned, unsigned);
+void orc_limits(unsigned, unsigned, unsigned, const unsigned*, const typecode*, unsigned*, unsigned*);
+void orc_presort(int, unsigned, unsigned, unsigned, const unsigned*, const typecode*,
+                 const unsigned*, const unsigned*, unsigned*);
+void orc_sort_cells(unsigned, unsigned, unsigned, unsigned, unsigned*, unsigned*, int*);
+void orc_poscell(unsigned, const double*, const double*, float, float*);
+void orc_interaction(int, int, int, int, float, float, const orc_cte*, const orc_div*, unsigned,
+                     unsigned, unsigned, const unsigned*, const float*, const double*,
+                     const float*, const float*, float*, float*, float*, float*, float*);
+void orc_sps_tau(unsigned, unsigned, float, float, const float*, const float*, float*);
+void orc_sym_pre(unsigned, unsigned, const float*, const float*, const float*, double, float, float,
+                 float, const float*, typecode*, double*, float*);
+void orc_sym_cor(unsigned, unsigned, const float*, const float*, const float*, double, double,
+                 float, float, float, const float*, typecode*, double*, float*);
+void orc_verlet(unsigned, unsigned, const float*, const float*, const float*, const float*, double,
+                double, float, float, float, const float*, typecode*, double*, float*);
+void orc_step_pos(unsigned, unsigned, const orc_cte*, const double*, const double*, double*,
+                  unsigned*, typecode*);
+void orc_pips(const orc_cte*, const orc_div*, unsigned, unsigned, unsigned, const unsigned*,
+              const float*, unsigned long long*);
+}
+
+namespace {
+
+struct Sim {
+  dsph_case cs{};
+  int prec = 0;
+  std::string err;
+  orc_cte c{};
+  float Scell = 0, KernelH = 0, SpsSmag = 0, SpsBlin = 0, Visco = 0, ViscoB = 0;
+  float RhopOutMin = 0, RhopOutMax = 0, g[3] = {0, 0, 0};
+  double Cs0 = 0, DtIni = 0, DtMin = 0, CFL = 0.2;
+  int ScellDiv = 1, TStep = 2, TVisco = 1, TDDT = 0, VerletSteps = 40;
+  unsigned DomCells[3] = {0, 0, 0};
+  unsigned CaseNbound = 0;
+  // particles
+  unsigned Np = 0, Npb = 0, NpbOk = 0;
+  std::vector<unsigned> idp, dcell;
+  std::vector<typecode> code;
+  std::vector<double> pos, pospre, mov;
+  std::vector<float> velrho, velrhopre, velrhom1, poscell, ace, ar, viscdt, strain, tau;
+  bool pre_active = false;
+  // cell division
+  std::vector<unsigned> cellpart, sortpart;
+  std::vector<int> begincell;
+  unsigned cmin[3] = {0, 0, 0}, cmax[3] = {0, 0, 0}, Ncx = 0, Ncy = 0, Ncz = 0, Nct = 0;
+  bool BoundLimitOk = false, BoundDivideOk = false, DivideFull = false, BoundChanged = true;
+  unsigned blmin[3], blmax[3], bdmin[3], bdmax[3];
+  unsigned NpfOutTotal = 0, Nstep = 0, VerletStep = 0;
+  double SymplecticDtPre = 0, TimeStep = 0, DtLast = 0, VelMax = 0, AceMax = 0;
+  float ViscDtMax = 0;
+
+  void config(const dsph_case& s) {
+    cs = s;
+    KernelH = float(s.kernelh);
+    const double h = KernelH;
+    const float ks = float(h * 2.0);
+    c.kernelh = KernelH;
+    c.kernelsize = ks;
+    c.kernelsize2 = ks * ks;
+    c.awen = float(0.41778 / (h * h * h));
+    c.bwenh = float(-2.08891 / (h * h * h * h * h));
+    c.cteb = float(s.cteb); c.gamma = float(s.gamma); c.rhopzero = float(s.rhopzero);
+    c.ovrhopzero = 1.f / c.rhopzero;
+    Cs0 = std::sqrt(double(c.gamma) * double(c.cteb) / double(c.rhopzero));
+    c.cs0 = float(Cs0);
+    c.eta2 = float((h * 0.1) * (h * 0.1));
+    c.massf = float(s.massfluid); c.massb = float(s.massbound); c.dp = float(s.dp);
+    for (int k = 0; k < 3; k++) g[k] = float(s.gravity[k]);
+    c.ddtkh = ks * float(s.ddtvalue);
+    c.ddtgz = float(double(c.rhopzero) * double(std::fabs(g[2])) / double(c.cteb));
+    TStep = s.tstep; TVisco = s.tvisco; TDDT = s.tdensity;
+    VerletSteps = s.verletsteps ? s.verletsteps : 40;
+    Visco = float(s.visco); ViscoB = Visco * float(s.viscoboundfactor);
+    RhopOutMin = float(s.rhopoutmin); RhopOutMax = float(s.rhopoutmax);
+    CFL = s.cflnumber;
+    DtIni = s.dtini ? s.dtini : double(KernelH) / Cs0;
+    DtMin = s.dtmin ? s.dtmin : (double(KernelH) / Cs0) * double(0.05f);
+    if (TVisco == 2) {
+      const double dps = std::sqrt(double(c.dp) * double(c.dp) * 3.) / 3.;
+      SpsSmag = float((2. / 3.) * std::pow(0.12 * dps, 2));
+      SpsBlin = float((2. / 3.) * 0.0066 * dps * dps);
+    }
+    ScellDiv = s.cellmode == 2 ? 2 : 1;
+    double msize[3];
+    for (int k = 0; k < 3; k++) {
+      c.maprealposmin[k] = s.maprealposmin[k];
+      c.maprealsize[k] = s.maprealposmax[k] - s.maprealposmin[k];
+      c.mapposmin[k] = s.maprealposmin[k];
+      c.domposmin[k] = s.maprealposmin[k];
+      msize[k] = c.maprealsize[k];
+    }
+    c.poscellsize = ks;
+    {
+      int nks = 1;
+      auto fits = [&](double pcs) {
+        return unsigned(std::ceil(msize[0] / pcs)) <= 8192 && unsigned(std::ceil(msize[1] / pcs)) <= 1024 &&
+               unsigned(std::ceil(msize[2] / pcs)) <= 512;
+      };
+      while (!fits(c.poscellsize)) { nks = (nks == 1 ? 2 : nks + 2); c.poscellsize = ks * nks; }
+    }
+    Scell = ks / ScellDiv;
+    c.scell = Scell;
+    c.movlimit = Scell * 0.9f;
+    for (int k = 0; k < 3; k++) DomCells[k] = unsigned(std::ceil(msize[k] / Scell));
+    c.cellcode = orc_cell_code(DomCells[0] + 1, DomCells[1] + 1, DomCells[2] + 1);
+    CaseNbound = s.casenbound;
+  }
+
+  void load(unsigned np, const unsigned* hidp, const double* hpos, const float* hvr) {
+    Np = np; Npb = cs.casenfixed + cs.casenmoving; NpbOk = Npb;
+    idp.assign(hidp, hidp + np);
+    pos.assign(hpos, hpos + 3 * size_t(np));
+    velrho.assign(hvr, hvr + 4 * size_t(np));
+    code.resize(np);
+    dcell.resize(np);
+    for (unsigned p = 0; p < np; p++) {
+      code[p] = typecode(idp[p] < CaseNbound ? 0x0000 : 0x1800);
+      const double* x = &pos[3 * size_t(p)];
+      const unsigned cx = unsigned((x[0] - c.domposmin[0]) / Scell);
+      const unsigned cy = unsigned((x[1] - c.domposmin[1]) / Scell);
+      const unsigned cz = unsigned((x[2] - c.domposmin[2]) / Scell);
+      const unsigned dcc = c.cellcode;
+      dcell[p] = (cx << ((dcc >> 10) & 31u)) | (cy << ((dcc >> 15) & 31u)) | cz;
+    }
+    if (TStep == 1) velrhom1 = velrho;
+    if (TVisco == 2) { tau.assign(6 * size_t(np), 0.f); strain.assign(6 * size_t(np), 0.f); }
+    SymplecticDtPre = DtIni;
+    BoundChanged = true;
+    cell_divide();
+  }
+
+  orc_div divdata() const {
+    orc_div d;
+    d.scelldiv = ScellDiv;
+    d.ncx = int(Ncx); d.ncy = int(Ncy); d.ncz = int(Ncz); d.ncw = int(Ncx * Ncy);
+    d.cellfluid = Ncx * Ncy * Ncz + 1;
+    d.czx = int(cmin[0]); d.czy = int(cmin[1]); d.czz = int(cmin[2]);
+    d.begincell = begincell.data();
+    return d;
+  }
+
+  template <typename T>
+  void gather(std::vector<T>& a, unsigned w, unsigned n, unsigned pini) {
+    std::vector<T> b(a.size());
+    for (unsigned p = 0; p < n; p++) {
+      const unsigned o = p < pini ? p : sortpart[p];
+      for (unsigned k = 0; k < w; k++) b[size_t(p) * w + k] = a[size_t(o) * w + k];
+    }
+    for (size_t k = size_t(n) * w; k < a.size(); k++) b[k] = a[k];
+    a.swap(b);
+  }
+
+  // JCellDivCpuSingle::Divide + JSphCpuSingle::RunCellDivide.
+  void cell_divide() {
+    const unsigned npb1 = Npb, npf1 = Np - Npb, nptot = Np;
+    if (BoundChanged) { BoundLimitOk = BoundDivideOk = false; }
+    if (cs.celldomfixed) {
+      for (int k = 0; k < 3; k++) { cmin[k] = 0; cmax[k] = DomCells[k] - 1; }
+    } else {
+      unsigned bmin[3], bmax[3], fmin[3], fmax[3];
+      if (!BoundLimitOk) {
+        orc_limits(npb1, 0, c.cellcode, dcell.data(), code.data(), bmin, bmax);
+        if (bmin[0] > bmax[0]) for (int k = 0; k < 3; k++) { bmin[k] = DomCells[k]; bmax[k] = 0; }
+        BoundLimitOk = true;
+        std::memcpy(blmin, bmin, 12); std::memcpy(blmax, bmax, 12);
+      } else { std::memcpy(bmin, blmin, 12); std::memcpy(bmax, blmax, 12); }
+      orc_limits(npf1, npb1, c.cellcode, dcell.data(), code.data(), fmin, fmax);
+      if (fmin[0] > fmax[0]) for (int k = 0; k < 3; k++) { fmin[k] = DomCells[k]; fmax[k] = 0; }
+      const unsigned s = unsigned(ScellDiv);
+      for (int k = 0; k < 3; k++) {
+        cmin[k] = std::max(std::min(bmin[k], fmin[k]), fmin[k] >= s ? fmin[k] - s : 0u);
+        cmax[k] = std::min(std::max(bmax[k], fmax[k]), fmax[k] + s);
+        if (cmax[k] >= DomCells[k]) cmax[k] = DomCells[k] - 1;
+      }
+      if (cmin[0] > cmax[0] || cmin[1] > cmax[1] || cmin[2] > cmax[2])
+        for (int k = 0; k < 3; k++) cmin[k] = cmax[k] = 0;
+    }
+    Ncx = cmax[0] - cmin[0] + 1; Ncy = cmax[1] - cmin[1] + 1; Ncz = cmax[2] - cmin[2] + 1;
+    Nct = Ncx * Ncy * Ncz;
+    const unsigned nctt = 2 * Nct + 5;
+    if (!BoundDivideOk || std::memcmp(bdmin, cmin, 12) || std::memcmp(bdmax, cmax, 12)) {
+      DivideFull = true; BoundDivideOk = true;
+      std::memcpy(bdmin, cmin, 12); std::memcpy(bdmax, cmax, 12);
+    } else DivideFull = false;
+    const unsigned nc[3] = {Ncx, Ncy, Ncz};
+    cellpart.resize(nptot);
+    sortpart.resize(nptot);
+    begincell.resize(2 * size_t(nctt));
+    orc_presort(DivideFull, nptot, npb1, c.cellcode, dcell.data(), code.data(), cmin, nc,
+                cellpart.data());
+    const unsigned key0 = DivideFull ? 0 : Nct + 1;
+    const unsigned pini = DivideFull ? 0 : npb1;
+    orc_sort_cells(nptot, pini, nctt, key0, cellpart.data(), sortpart.data(), begincell.data());
+    auto csize = [&](unsigned box) { return unsigned(begincell[2 * box + 1] - begincell[2 * box]); };
+    const unsigned npbignore = csize(Nct);
+    const unsigned bbo = 2 * Nct + 1;
+    const unsigned npbout = csize(bbo), npfout = csize(bbo + 1), npboi = csize(bbo + 2),
+                   npfoi = csize(bbo + 3);
+    const unsigned npfinal = nptot - npbout - npfout - npboi - npfoi;
+    unsigned npbfinal = npb1 - npboi;
+    if (npbout && DivideFull) npbfinal = UINT_MAX;
+    const unsigned gpini = DivideFull ? 0 : npbfinal;
+    gather(idp, 1, nptot, gpini); gather(code, 1, nptot, gpini); gather(dcell, 1, nptot, gpini);
+    gather(pos, 3, nptot, gpini); gather(velrho, 4, nptot, gpini);
+    if (TStep == 1) gather(velrhom1, 4, nptot, gpini);
+    else if (pre_active) { gather(pospre, 3, nptot, gpini); gather(velrhopre, 4, nptot, gpini); }
+    if (TVisco == 2) gather(tau, 6, nptot, gpini);
+    if (npbout) throw std::runtime_error("boundary particles excluded");
+    Np = npfinal; Npb = npbfinal; NpbOk = Npb - npbignore;
+    NpfOutTotal += npfout;
+    poscell.resize(4 * size_t(Np));
+    orc_poscell(Np, pos.data(), c.mapposmin, c.poscellsize, poscell.data());
+    BoundChanged = false;
+  }
+
+  void interaction() {
+    ace.assign(3 * size_t(Np), 0.f);
+    ar.assign(Np, 0.f);
+    viscdt.assign(Np, 0.f);
+    if (TVisco == 2) std::fill(strain.begin() + 6 * size_t(Npb), strain.begin() + 6 * size_t(Np), 0.f);
+    float v2 = 0;
+    for (unsigned p = Npb; p < Np; p++) {
+      const float* v = &velrho[4 * size_t(p)];
+      v2 = std::max(v2, v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    }
+    VelMax = double(std::sqrt(v2));
+    const orc_div d = divdata();
+    orc_interaction(prec, prec ? 2 : 0, TVisco, TDDT, ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
+                    dcell.data(), poscell.data(), pos.data(), velrho.data(),
+                    TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(), viscdt.data(),
+                    TVisco == 2 ? strain.data() : nullptr, nullptr);
+    if (TVisco == 2) orc_sps_tau(Np, Npb, SpsSmag, SpsBlin, velrho.data(), strain.data(), tau.data());
+    float vd = 0, a2 = 0;
+    for (unsigned p = 0; p < Np; p++) vd = std::max(vd, viscdt[p]);
+    for (unsigned p = Npb; p < Np; p++) {
+      const float* a = &ace[3 * size_t(p)];
+      a2 = std::max(a2, a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+    }
+    ViscDtMax = vd;
+    AceMax = std::sqrt(double(a2));
+  }
+
+  double dt_variable() const {
+    const double dt1 = (AceMax ? std::sqrt(double(KernelH) / AceMax) : DBL_MAX);
+    const double dt2 = double(KernelH) / (std::max(Cs0, VelMax * 10.) + double(KernelH) * ViscDtMax);
+    double dt = CFL * std::min(dt1, dt2);
+    if (std::isnan(dt) || std::isinf(dt)) throw std::runtime_error("dt is NaN or infinity");
+    if (dt < DtMin) dt = DtMin;
+    return dt;
+  }
+
+  void step_sym() {
+    const double dt = SymplecticDtPre;
+    interaction();
+    const double dt_p = dt_variable();
+    pospre = pos; velrhopre = velrho; pre_active = true;
+    mov.assign(3 * size_t(Np), 0.0);
+    orc_sym_pre(Np, Npb, velrhopre.data(), ar.data(), ace.data(), dt * .5, c.rhopzero, RhopOutMin,
+                RhopOutMax, g, code.data(), mov.data(), velrho.data());
+    orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
+    cell_divide();
+    interaction();
+    const double dt_c = dt_variable();
+    mov.assign(3 * size_t(Np), 0.0);
+    orc_sym_cor(Np, Npb, velrhopre.data(), ar.data(), ace.data(), dt * .5, dt, c.rhopzero,
+                RhopOutMin, RhopOutMax, g, code.data(), mov.data(), velrho.data());
+    orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
+    pre_active = false;
+    SymplecticDtPre = std::min(dt_p, dt_c);
+    TimeStep += dt;
+    DtLast = dt;
+  }
+
+  void step_ver() {
+    interaction();
+    const double dt = dt_variable();
+    VerletStep++;
+    std::vector<float> nw(velrho.size());
+    mov.assign(3 * size_t(Np), 0.0);
+    if (VerletStep < unsigned(VerletSteps))
+      orc_verlet(Np, Npb, velrho.data(), velrhom1.data(), ar.data(), ace.data(), dt, dt + dt,
+                 c.rhopzero, RhopOutMin, RhopOutMax, g, code.data(), mov.data(), nw.data());
+    else {
+      orc_verlet(Np, Npb, velrho.data(), velrho.data(), ar.data(), ace.data(), dt, dt, c.rhopzero,
+                 RhopOutMin, RhopOutMax, g, code.data(), mov.data(), nw.data());
+      VerletStep = 0;
+    }
+    std::vector<double> p0 = pos;
+    orc_step_pos(Np, Npb, &c, p0.data(), mov.data(), pos.data(), dcell.data(), code.data());
+    velrhom1 = velrho;
+    velrho = nw;
+    TimeStep += dt;
+    DtLast = dt;
+  }
+
+  void run(unsigned n) {
+    for (unsigned s = 0; s < n; s++) {
+      if (TStep == 2) step_sym(); else step_ver();
+      cell_divide();
+      Nstep++;
+    }
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+void* orc_sim_create(const dsph_case* cs, int prec, unsigned np, const unsigned* idp,
+                     const double* pos, const float* velrho) {
+  Sim* s = new Sim();
+  s->prec = prec;
+  try {
+    s->config(*cs);
+    s->load(np, idp, pos, velrho);
+  } catch (const std::exception& e) {
+    s->err = e.what();
+  }
+  return s;
+}
+void orc_sim_destroy(void* h) { delete static_cast<Sim*>(h); }
+const char* orc_sim_error(void* h) { return static_cast<Sim*>(h)->err.c_str(); }
+int orc_sim_run(void* h, unsigned nsteps) {
+  Sim* s = static_cast<Sim*>(h);
+  if (!s->err.empty()) return 1;
+  try {
+    s->run(nsteps);
+  } catch (const std::exception& e) {
+    s->err = e.what();
+    return 1;
+  }
+  return 0;
+}
+// Downloads state (sorted order): np, idp, pos(double3), velrho, code.
+unsigned orc_sim_download(void* h, unsigned* idp, double* pos, float* velrho, uint16_t* code) {
+  Sim* s = static_cast<Sim*>(h);
+  const size_t n = s->Np;
+  if (idp) std::memcpy(idp, s->idp.data(), 4 * n);
+  if (pos) std::memcpy(pos, s->pos.data(), 24 * n);
+  if (velrho) std::memcpy(velrho, s->velrho.data(), 16 * n);
+  if (code) std::memcpy(code, s->code.data(), 2 * n);
+  return s->Np;
+}
+void orc_sim_forces(void* h, float* ace, float* ar) {
+  Sim* s = static_cast<Sim*>(h);
+  if (ace) std::memcpy(ace, s->ace.data(), 12 * size_t(s->Np));
+  if (ar) std::memcpy(ar, s->ar.data(), 4 * size_t(s->Np));
+}
+// {np, npb, npbok, nct, nstep, npfouttotal}, {timestep, dtlast, symdtpre, velmax, acemax, viscdtmax}
+void orc_sim_stats(void* h, unsigned* u6, double* d6) {
+  Sim* s = static_cast<Sim*>(h);
+  if (u6) {
+    u6[0] = s->Np; u6[1] = s->Npb; u6[2] = s->NpbOk; u6[3] = s->Nct; u6[4] = s->Nstep;
+    u6[5] = s->NpfOutTotal;
+  }
+  if (d6) {
+    d6[0] = s->TimeStep; d6[1] = s->DtLast; d6[2] = s->SymplecticDtPre; d6[3] = s->VelMax;
+    d6[4] = s->AceMax; d6[5] = s->ViscDtMax;
+  }
+}
+// Cell-division view: cellmin/cellmax (6 u32), full flag; begincell copy of
+// 2*nctt ints when `be` is not NULL. Returns nctt.
+unsigned orc_sim_celldiv(void* h, unsigned* c6, int* be) {
+  Sim* s = static_cast<Sim*>(h);
+  if (c6) for (int k = 0; k < 3; k++) { c6[k] = s->cmin[k]; c6[3 + k] = s->cmax[k]; }
+  if (be) std::memcpy(be, s->begincell.data(), 4 * s->begincell.size());
+  return unsigned(s->begincell.size() / 2);
+}
+void orc_sim_pips(void* h, unsigned long long res[4]) {
+  Sim* s = static_cast<Sim*>(h);
+  const orc_div d = s->divdata();
+  orc_pips(&s->c, &d, s->Np, s->Npb, s->NpbOk, s->dcell.data(), s->poscell.data(), res);
+}
+// Re-runs the interaction on the current state (forces of the current state).
+void orc_sim_interact(void* h) { static_cast<Sim*>(h)->interaction(); }
+// Neighbour-list arrays of the current state: dcell, poscell (float4), and
+// the constants used (orc_cte).
+void orc_sim_arrays(void* h, unsigned* dcell, float* poscell, orc_cte* cte) {
+  Sim* s = static_cast<Sim*>(h);
+  if (dcell) std::memcpy(dcell, s->dcell.data(), 4 * size_t(s->Np));
+  if (poscell) std::memcpy(poscell, s->poscell.data(), 16 * size_t(s->Np));
+  if (cte) *cte = s->c;
+}
+
+}  // extern "C"

@@ -1,0 +1,137 @@
+"""Shared helpers for the parity tests: device buffers over the C ABI and the
+oracle-prepared (sorted) particle states."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from dualsphysics_amd import _native as N
+
+
+_LIVE: list = []
+
+
+def release_all():
+    """Frees the device buffers of the finished test (see Dev)."""
+    while _LIVE:
+        _LIVE.pop().free()
+
+
+class Dev:
+    """A device buffer holding a copy of a numpy array (through dsph_malloc).
+
+    Buffers stay alive until release_all() (called after every test), so a
+    temporary like ``Dev(lib, ctx, a).ptr`` passed inline to a call is not
+    freed before the call runs."""
+
+    def __init__(self, lib, ctx, arr: np.ndarray | None = None, nbytes: int | None = None,
+                 dtype=None, shape=None):
+        _LIVE.append(self)
+        self.lib, self.ctx = lib, ctx
+        if arr is not None:
+            arr = np.ascontiguousarray(arr)
+            self.dtype, self.shape, self.nbytes = arr.dtype, arr.shape, arr.nbytes
+        else:
+            self.dtype, self.shape, self.nbytes = np.dtype(dtype), shape, nbytes
+        self.ptr = C.c_void_p()
+        rc = lib.dsph_malloc(ctx, max(self.nbytes, 4), C.byref(self.ptr))
+        assert rc == 0, lib.dsph_last_error(ctx)
+        if arr is not None and self.nbytes:
+            rc = lib.dsph_memcpy_h2d(ctx, self.ptr, arr.ctypes.data_as(C.c_void_p), self.nbytes)
+            assert rc == 0, lib.dsph_last_error(ctx)
+
+    @classmethod
+    def zeros(cls, lib, ctx, shape, dtype):
+        return cls(lib, ctx, np.zeros(shape, dtype))
+
+    def get(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        if self.nbytes:
+            rc = self.lib.dsph_memcpy_d2h(self.ctx, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                          self.nbytes)
+            assert rc == 0, self.lib.dsph_last_error(self.ctx)
+        return out
+
+    def off(self, nelem_bytes: int) -> C.c_void_p:
+        return C.c_void_p(self.ptr.value + nelem_bytes)
+
+    def free(self):
+        if self.ptr:
+            self.lib.dsph_free(self.ctx, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def u3(vals):
+    return (C.c_uint32 * 3)(*[int(v) for v in vals])
+
+
+def perturbed_case(case, seed=7, vamp=0.5, ramp=0.01):
+    """A copy of ``case`` with random fluid velocities (|v|<~vamp m/s) and
+    density perturbations (relative ~ramp), so every interaction term is active."""
+    rng = np.random.default_rng(seed)
+    vr = case.velrho.copy()
+    nb = case.nbound
+    vr[nb:, :3] = (rng.standard_normal((case.nfluid, 3)) * vamp).astype(np.float32)
+    vr[nb:, 3] *= (1 + ramp * rng.standard_normal(case.nfluid)).astype(np.float32)
+    vr[:nb, 3] *= (1 + 0.5 * ramp * rng.random(nb)).astype(np.float32)
+    import dataclasses
+    return dataclasses.replace(case, velrho=vr)
+
+
+def to_cte(orc_cte, tboundary=N.BC_DBC) -> N.DsphCte:
+    """ABI constants from the oracle's constants of the same case."""
+    c = N.DsphCte()
+    for f in ("massb", "massf", "kernelh", "kernelsize2", "poscellsize", "awen", "bwenh", "cs0",
+              "eta2", "ddtkh", "ddtgz", "scell", "kernelsize", "dp", "cteb", "gamma", "rhopzero",
+              "ovrhopzero", "movlimit", "cellcode"):
+        setattr(c, f, getattr(orc_cte, f))
+    c.tboundary = tboundary
+    c.domposminx, c.domposminy, c.domposminz = orc_cte.domposmin
+    c.maprealposminx, c.maprealposminy, c.maprealposminz = orc_cte.maprealposmin
+    c.maprealsizex, c.maprealsizey, c.maprealsizez = orc_cte.maprealsize
+    return c
+
+
+def divdata(celldiv, begincell_dev, cte) -> N.DsphDivData:
+    c6 = celldiv
+    d = N.DsphDivData()
+    d.scelldiv = 1
+    d.ncx, d.ncy, d.ncz = c6[3] - c6[0] + 1, c6[4] - c6[1] + 1, c6[5] - c6[2] + 1
+    d.ncw = d.ncx * d.ncy
+    d.cellfluid = d.ncw * d.ncz + 1
+    d.cellzerox, d.cellzeroy, d.cellzeroz = c6[0], c6[1], c6[2]
+    d.d_beginendcell = begincell_dev.ptr
+    d.scell = cte.scell
+    d.domcellcode = cte.cellcode
+    d.domposminx, d.domposminy, d.domposminz = cte.domposmin
+    d.kernelsize2 = cte.kernelsize2
+    d.poscellsize = cte.poscellsize
+    return d
+
+
+def rel_err(a, b, axis=None):
+    """Norm-wise relative error max|a-b| / max|b| (the parity metric)."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    den = np.max(np.abs(b))
+    return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+def per_particle_rel(a, b, floor_frac=1e-3):
+    """Per-particle relative error |a-b|/(|b| + floor) with floor = floor_frac *
+    RMS(|b|), vectors compared by Euclidean norm."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    if a.ndim == 1:
+        a, b = a[:, None], b[:, None]
+    d = np.linalg.norm(a - b, axis=1)
+    nb = np.linalg.norm(b, axis=1)
+    floor = floor_frac * np.sqrt(np.mean(nb ** 2)) + 1e-30
+    return d / (nb + floor)

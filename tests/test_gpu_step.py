@@ -1,0 +1,181 @@
+"""Integration, position update, reductions and dt on the GPU vs the oracle
+(reference rows a18-a23)."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from tests.helpers import Dev, rel_err
+
+pytestmark = pytest.mark.gpu
+
+G = (C.c_float * 3)(0.0, 0.0, -9.81)
+
+
+def _arrays(n, npb, seed=1):
+    rng = np.random.default_rng(seed)
+    vr = np.zeros((n, 4), np.float32)
+    vr[:, :3] = rng.standard_normal((n, 3)).astype(np.float32)
+    vr[:, 3] = (1000 + 5 * rng.standard_normal(n)).astype(np.float32)
+    vr[:npb, :3] = 0
+    ar = (rng.standard_normal(n) * 50).astype(np.float32)
+    ace = (rng.standard_normal((n, 3)) * 20).astype(np.float32)
+    code = np.full(n, 0x1800, np.uint16)
+    code[:npb] = 0
+    # Densities that leave [700,1300] must be flagged OUTRHO.
+    vr[npb + 3, 3] = 1299.99
+    ar[npb + 3] = 1e6
+    return vr, ar, ace, code
+
+
+@pytest.mark.parametrize("kind", ["pre", "cor"])
+def test_symplectic(lib, ctx, kind):
+    import oracle.oracle as orc
+    n, npb = 20000, 3000
+    vr, ar, ace, code = _arrays(n, npb)
+    vpre = vr.copy()
+    vpre[:, 3] -= 0.1
+    dt = 2.3e-4
+    d_vpre, d_ar, d_ace = Dev(lib, ctx, vpre), Dev(lib, ctx, ar), Dev(lib, ctx, ace)
+    d_code = Dev(lib, ctx, code)
+    d_movxy = Dev.zeros(lib, ctx, (n, 2), np.float64)
+    d_movz = Dev.zeros(lib, ctx, n, np.float64)
+    d_vr = Dev(lib, ctx, vr)
+    if kind == "pre":
+        rc = lib.dsph_step_symplectic_pre(ctx, n, npb, d_vpre.ptr, d_ar.ptr, d_ace.ptr, dt * .5,
+                                          None, 0, 1000., 700., 1300., G, d_code.ptr,
+                                          d_movxy.ptr, d_movz.ptr, d_vr.ptr)
+    else:
+        rc = lib.dsph_step_symplectic_cor(ctx, n, npb, d_vpre.ptr, d_ar.ptr, d_ace.ptr, dt * .5,
+                                          dt, None, 0, 1000., 700., 1300., G, d_code.ptr,
+                                          d_movxy.ptr, d_movz.ptr, d_vr.ptr)
+    assert rc == 0
+    ocode = code.copy()
+    omov = np.zeros((n, 3), np.float64)
+    ovr = vr.copy()
+    olib = orc.load()
+    if kind == "pre":
+        olib.orc_sym_pre(n, npb, vpre.ctypes.data, ar.ctypes.data, ace.ctypes.data, dt * .5, 1000.,
+                         700., 1300., G, ocode.ctypes.data, omov.ctypes.data, ovr.ctypes.data)
+    else:
+        olib.orc_sym_cor(n, npb, vpre.ctypes.data, ar.ctypes.data, ace.ctypes.data, dt * .5, dt,
+                         1000., 700., 1300., G, ocode.ctypes.data, omov.ctypes.data,
+                         ovr.ctypes.data)
+    gvr = d_vr.get()
+    # FP64 intermediate, FP32 result: identical up to one float ulp.
+    np.testing.assert_allclose(gvr, ovr, rtol=2e-7, atol=1e-30)
+    np.testing.assert_array_equal(d_code.get(), ocode)
+    assert ocode[npb + 3] == 0x1800 | 0xa000
+    gm = np.concatenate([d_movxy.get(), d_movz.get()[:, None]], 1)
+    np.testing.assert_allclose(gm[npb:], omov[npb:], rtol=1e-14, atol=1e-300)
+
+
+def test_verlet(lib, ctx):
+    import oracle.oracle as orc
+    n, npb = 10000, 1000
+    vr, ar, ace, code = _arrays(n, npb, 4)
+    vm1 = vr.copy()
+    vm1[:, :3] *= 0.9
+    dt = 1.7e-4
+    d_new = Dev.zeros(lib, ctx, (n, 4), np.float32)
+    d_code = Dev(lib, ctx, code)
+    d_mxy = Dev.zeros(lib, ctx, (n, 2), np.float64)
+    d_mz = Dev.zeros(lib, ctx, n, np.float64)
+    rc = lib.dsph_step_verlet(ctx, n, npb, Dev(lib, ctx, vr).ptr, Dev(lib, ctx, vm1).ptr,
+                              Dev(lib, ctx, ar).ptr, Dev(lib, ctx, ace).ptr, dt, 2 * dt, 1000.,
+                              700., 1300., G, d_code.ptr, d_mxy.ptr, d_mz.ptr, d_new.ptr)
+    assert rc == 0
+    onew = np.zeros((n, 4), np.float32)
+    ocode = code.copy()
+    omov = np.zeros((n, 3))
+    orc.load().orc_verlet(n, npb, vr.ctypes.data, vm1.ctypes.data, ar.ctypes.data,
+                          ace.ctypes.data, dt, 2 * dt, 1000., 700., 1300., G, ocode.ctypes.data,
+                          omov.ctypes.data, onew.ctypes.data)
+    np.testing.assert_allclose(d_new.get(), onew, rtol=2e-7, atol=1e-30)
+    np.testing.assert_array_equal(d_code.get(), ocode)
+
+
+def test_step_pos2_flags_and_cells(lib, ctx):
+    import oracle.oracle as orc
+    from oracle.oracle import OracleSim
+    from dualsphysics_amd import case as casemod
+    from tests.helpers import to_cte
+    c = casemod.dam_break(dp=0.02)
+    o = OracleSim(c)
+    d = o.download()
+    dcell, _, cte = o.arrays()
+    n, npb = d["idp"].size, c.nbound
+    rng = np.random.default_rng(3)
+    mov = rng.standard_normal((n, 3)) * 1e-3
+    mov[npb + 1] = [cte.movlimit * 1.01, 0, 0]           # OUTMOV
+    mov[npb + 2] = [0, 0, -5.0]                          # OUTPOS
+    mov[npb + 4] = [np.nan, 0, 0]                        # NaN -> OUTPOS
+    code = d["code"].copy()
+    code[npb + 3] = 0x1800 | 0xa000                      # already OUTRHO
+    acte = to_cte(cte)
+    assert lib.dsph_set_constants(ctx, C.byref(acte)) == 0
+    pos = d["pos"]
+    d_pxy = Dev(lib, ctx, np.ascontiguousarray(pos[:, :2]))
+    d_pz = Dev(lib, ctx, np.ascontiguousarray(pos[:, 2]))
+    o_xy = Dev.zeros(lib, ctx, (n, 2), np.float64)
+    o_z = Dev.zeros(lib, ctx, n, np.float64)
+    d_dc = Dev(lib, ctx, dcell)
+    d_code = Dev(lib, ctx, code)
+    rc = lib.dsph_step_pos2(ctx, n, npb, d_pxy.ptr, d_pz.ptr,
+                            Dev(lib, ctx, np.ascontiguousarray(mov[:, :2])).ptr,
+                            Dev(lib, ctx, np.ascontiguousarray(mov[:, 2])).ptr, o_xy.ptr, o_z.ptr,
+                            d_dc.ptr, d_code.ptr)
+    assert rc == 0
+    opos = pos.copy()
+    odc = dcell.copy()
+    ocode = code.copy()
+    orc.load().orc_step_pos(n, npb, C.byref(cte), np.ascontiguousarray(pos).ctypes.data,
+                            np.ascontiguousarray(mov).ctypes.data, opos.ctypes.data,
+                            odc.ctypes.data, ocode.ctypes.data)
+    gpos = np.concatenate([o_xy.get(), o_z.get()[:, None]], 1)
+    np.testing.assert_array_equal(gpos[npb:], opos[npb:])
+    np.testing.assert_array_equal(d_dc.get()[npb:], odc[npb:])
+    np.testing.assert_array_equal(d_code.get(), ocode)
+    assert (ocode[npb + 1] & 0xe000) == 0x6000
+    assert (ocode[npb + 2] & 0xe000) == 0x8000
+    assert (ocode[npb + 3] & 0xe000) == 0xa000
+    assert (ocode[npb + 4] & 0xe000) == 0x8000
+
+
+@pytest.mark.parametrize("n", [0, 1, 255, 2049, 1000003])
+def test_reductions(lib, ctx, n):
+    rng = np.random.default_rng(n)
+    data = (rng.standard_normal(max(n, 1)) * 3).astype(np.float32)[:n]
+    vr = rng.standard_normal((max(n, 1), 4)).astype(np.float32)[:n]
+    ace = rng.standard_normal((max(n, 1), 3)).astype(np.float32)[:n]
+    out = Dev.zeros(lib, ctx, 4, np.float32)
+    h = C.c_float()
+    assert lib.dsph_redu_max_f32_sync(ctx, n, 0, Dev(lib, ctx, data).ptr, C.byref(h)) == 0
+    assert h.value == (data.max() if n else 0.0)
+    assert lib.dsph_velmax2(ctx, n, 0, Dev(lib, ctx, vr).ptr, out.ptr) == 0
+    assert lib.dsph_acemax2(ctx, n, 0, Dev(lib, ctx, ace).ptr, out.off(4)) == 0
+    r = out.get()
+    if n:
+        v2 = (vr[:, 0] * vr[:, 0] + vr[:, 1] * vr[:, 1]) + vr[:, 2] * vr[:, 2]
+        a2 = (ace[:, 0] * ace[:, 0] + ace[:, 1] * ace[:, 1]) + ace[:, 2] * ace[:, 2]
+        assert abs(r[0] - v2.max()) <= 2e-7 * v2.max()
+        assert abs(r[1] - a2.max()) <= 2e-7 * a2.max()
+    else:
+        assert r[0] == 0 and r[1] == 0
+
+
+def test_dt_variable_on_device(lib, ctx):
+    import oracle.oracle as orc
+    olib = orc.load()
+    red = np.array([0.8 ** 2, 3.5, 45.0 ** 2, 0], np.float32)
+    d_dt = Dev.zeros(lib, ctx, 8, np.float64)
+    fl = Dev.zeros(lib, ctx, 4, np.uint32)
+    h = np.float32(0.0207846)
+    cs0, cfl, dtmin = 34.31, 0.2, 3e-5
+    assert lib.dsph_dt_variable(ctx, Dev(lib, ctx, red).ptr, float(h), cs0, cfl, dtmin, d_dt.ptr, 1,
+                                fl.ptr) == 0
+    got = d_dt.get()[1]
+    exp = olib.orc_dt_variable(float(np.sqrt(red[0])), float(np.sqrt(np.float64(red[2]))),
+                               float(red[1]), float(h), cs0, cfl, dtmin)
+    assert abs(got - exp) <= 1e-15 * exp
+    assert fl.get()[0] == 0
