@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
+    ap.add_argument("--forces-kernel", choices=["tile", "rows"], default="tile")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs the SPH step on one GPU only; "
@@ -100,6 +101,8 @@ def main():
     sim = SphGpuSingle(c)
     lib = N.load()
     ctx = lib.dsph_sim_ctx(sim._h)
+    lib.dsph_ctx_set_option(ctx, N.OPT_FORCES_KERNEL,
+                            N.FORCES_TILE if a.forces_kernel == "tile" else N.FORCES_ROWS)
     log("warmup ...")
     sim.run(a.warmup)
     torch.cuda.synchronize()
@@ -151,7 +154,7 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"dam_break_3d_dp{a.dp:g}", "n_fluid": c.nfluid,
-                   "n_bound": c.nbound, "dp": a.dp, "kernel": "wendland",
+                   "n_bound": c.nbound, "dp": a.dp, "kernel": "wendland", "forces_kernel": a.forces_kernel,
                    "visco": "artificial 0.01", "ddt": "fourtakas_full 0.1",
                    "step": "symplectic", "cfl": 0.2, "pairs_per_step": pairs_step,
                    "sim_time_s": st["timestep"]},

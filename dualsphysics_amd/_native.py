@@ -15,6 +15,8 @@ KERNEL_WENDLAND = 2
 VISCO_ARTIFICIAL, VISCO_LAMINARSPS, VISCO_LAMINAR = 1, 2, 3
 DDT_NONE, DDT_DDT, DDT_DDT2, DDT_DDT2FULL = 0, 1, 2, 3
 BC_DBC, BC_MDBC = 1, 2
+OPT_FORCES_KERNEL = 1
+FORCES_TILE, FORCES_ROWS = 0, 1
 
 u32p = C.POINTER(C.c_uint32)
 
@@ -88,6 +90,7 @@ SIGNATURES = {
     "dsph_memcpy_d2h": [_vp, _vp, _vp, _sz],
     "dsph_memcpy_d2d_async": [_vp, _vp, _vp, _sz],
     "dsph_set_constants": [_vp, C.POINTER(DsphCte)],
+    "dsph_ctx_set_option": [_vp, _i, _i],
     "dsph_cell_limits_sync": [_vp, _u, _u, _u, _vp, _vp, u32p, u32p],
     "dsph_presort_full": [_vp, _u, _u, _vp, _vp, u32p, u32p, _vp, _vp],
     "dsph_presort_fluid": [_vp, _u, _u, _u, _vp, _vp, u32p, u32p, _vp, _vp],

@@ -23,6 +23,14 @@ int ensure_sort_scratch(dsph_ctx* ctx, uint32_t n) {
   return DSPH_OK;
 }
 
+int ensure_prep_scratch(dsph_ctx* ctx, uint32_t n) {
+  if (size_t(n) > ctx->prep_cap)
+    return set_error(ctx, DSPH_E_NOMEM,
+                     "interaction scratch too small: call dsph_ctx_reserve with np_max >= " +
+                         std::to_string(n));
+  return DSPH_OK;
+}
+
 int ensure_redu_scratch(dsph_ctx* ctx, uint32_t n) {
   const size_t need = (size_t(n) + 255) / 256 + 256;
   if (need > ctx->redu_cap)
@@ -74,6 +82,8 @@ int dsph_ctx_destroy(dsph_ctx* ctx) {
   hipFree(ctx->sort_vals_alt);
   hipFree(ctx->sort_hist);
   hipFree(ctx->redu);
+  hipFree(ctx->prep_press);
+  hipFree(ctx->prep_posf);
   hipFree(ctx->small_dev);
   if (ctx->pinned) hipHostFree(ctx->pinned);
   if (ctx->own_stream) hipStreamDestroy(ctx->own_stream);
@@ -99,6 +109,14 @@ int dsph_ctx_reserve(dsph_ctx* ctx, uint32_t np_max, uint32_t nctt_max) {
     hipFree(ctx->sort_hist); ctx->sort_hist = nullptr; ctx->hist_cap = 0;
     DSPH_HIP(ctx, hipMalloc(&ctx->sort_hist, sizeof(uint32_t) * hw));
     ctx->hist_cap = hw;
+  }
+  if (np_max > ctx->prep_cap) {
+    hipFree(ctx->prep_press); ctx->prep_press = nullptr;
+    hipFree(ctx->prep_posf); ctx->prep_posf = nullptr;
+    ctx->prep_cap = 0;
+    DSPH_HIP(ctx, hipMalloc(&ctx->prep_press, sizeof(float) * size_t(np_max)));
+    DSPH_HIP(ctx, hipMalloc(&ctx->prep_posf, 4 * sizeof(float) * size_t(np_max)));
+    ctx->prep_cap = np_max;
   }
   const size_t rw = (size_t(np_max) + 255) / 256 + 256;
   if (rw > ctx->redu_cap) {

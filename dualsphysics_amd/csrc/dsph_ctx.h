@@ -35,6 +35,11 @@ struct dsph_ctx {
   size_t redu_cap = 0;
   uint32_t* small_dev = nullptr;   // 64 words: limits, pips counters
   void* pinned = nullptr;          // 4 KiB pinned host buffer for readbacks
+  // Per-interaction preparation: pressure and map-frame float positions.
+  float* prep_press = nullptr;
+  float* prep_posf = nullptr;      // float4 per particle
+  size_t prep_cap = 0;
+  int forces_kernel = 0;           // DSPH_OPT_FORCES_KERNEL: 0 queue, 1 rows
 };
 
 namespace dsph {
@@ -98,5 +103,6 @@ struct KTimer {
 // Ensures ctx scratch can hold a sort of n keys with key_limit.
 int ensure_sort_scratch(dsph_ctx* ctx, uint32_t n);
 int ensure_redu_scratch(dsph_ctx* ctx, uint32_t n);
+int ensure_prep_scratch(dsph_ctx* ctx, uint32_t n);
 
 }  // namespace dsph

@@ -1,0 +1,611 @@
+// Force interaction of the SPH step on gfx950 (FP32 pair arithmetic).
+//
+// Physics follow the reference GPU kernels KerInteractionForcesFluid /
+// KerInteractionForcesFluidBox / KerInteractionForcesBound (JSphGpu_ker.cu:
+// 450-990): for every fluid particle the neighbour cell rows are visited
+// z-outer, y-inner, fluid cells first and then boundary cells, the particles of
+// a row in ascending sorted index, all contributions in one accumulator set;
+// distances from the cell-relative FP32 PosCell.
+//
+// Two fluid kernels produce the same sums in the same order:
+//  - k_forces_fluid_rows: one lane per particle looping over its rows (the
+//    reference's structure). A wave executes the pair body whenever any lane
+//    has a neighbour, so most lanes idle (about a quarter of them do work on a
+//    jittered lattice).
+//  - k_forces_fluid_tile (default): a wave walks the rows shared by its lanes
+//    once, stages each row in LDS with coalesced loads, lets every lane run a
+//    cheap conservative distance test against the staged candidates and queue
+//    its hits in LDS, then evaluates the exact pair body for the queued pairs
+//    in wave-uniform batches (every lane busy), so the expensive body runs
+//    only for real pairs. Queue order is the scan order, so the summation
+//    order is unchanged.
+#include <cfloat>
+#include <cstring>
+#include "sph_math.h"
+
+using namespace dsph;
+
+namespace {
+
+constexpr unsigned kBsForces = 128;
+constexpr unsigned kBs = 256;
+
+// Per-interaction preparation: pressure (EOS once per particle; each pair sees
+// the same value the reference computes per pair) and float positions in the
+// map frame for the conservative neighbour test.
+__global__ void __launch_bounds__(kBs) k_prep(unsigned n, const float4* __restrict__ poscell,
+    const float4* __restrict__ velrho, float* __restrict__ press, float4* __restrict__ posf,
+    Cte c) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  press[p] = eos_press(velrho[p].w, c);
+  if (posf) {
+    const float4 ps = poscell[p];
+    const double pcs = c.pcs;
+    posf[p] = make_float4(float(double(pscel_fx(ps.w)) * pcs + double(ps.x)),
+                          float(double(pscel_fy(ps.w)) * pcs + double(ps.y)),
+                          float(double(pscel_fz(ps.w)) * pcs + double(ps.z)), 0.f);
+  }
+}
+
+struct Acc {
+  float ax, ay, az, ar, visc, delta;
+  float s[6];   // two_strain {xx,xy,xz,yy,yz,zz}
+  float g[9];   // full velocity gradient (learned closure features)
+};
+
+__device__ inline void acc_init(Acc& a) {
+  a.ax = a.ay = a.az = a.ar = a.visc = a.delta = 0;
+#pragma unroll
+  for (int k = 0; k < 6; k++) a.s[k] = 0;
+#pragma unroll
+  for (int k = 0; k < 9; k++) a.g[k] = 0;
+}
+
+struct P1 {
+  float4 ps, vr;
+  float press;
+  float tau[6];
+};
+
+// Exact pair body for fluid p1 and neighbour p2 (KerInteractionForcesFluidBox,
+// JSphGpu_ker.cu:539-777, in-scope options). boundp2 selects the boundary
+// variant (mass, viscosity, no SPS tau of p2, delta-SPH rules).
+// pair_eval takes the neighbour's PosCell, velrho and pressure already loaded
+// and applies the exact neighbour test rr2 <= (2h)^2, rr2 >= ALMOSTZERO.
+template <int TVISCO, int TDDT, bool VELGRAD>
+__device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float4& ps2,
+    const float4& vr2, float press2, const P1& q, const float* __restrict__ tau, float viscob,
+    float viscof, const Cte& c, Acc& a) {
+  const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
+  const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
+  if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
+  const float massp2 = boundp2 ? c.massb : c.massf;
+  const float visco = boundp2 ? viscob : viscof;
+  const float fac = wendland_fac(rr2, c);
+  const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
+  const float4& vr1 = q.vr;
+  {
+    const float prs = fast_div(q.press + press2, vr1.w * vr2.w);
+    const float p_vpm = -prs * massp2;
+    a.ax += p_vpm * frx; a.ay += p_vpm * fry; a.az += p_vpm * frz;
+  }
+  const float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
+  const float rrho2 = fast_rcp(vr2.w);
+  a.ar += massp2 * (dvx * frx + dvy * fry + dvz * frz) * (vr1.w * rrho2);
+  const float cbar = c.cs0;
+  const float dot3 = dr.x * frx + dr.y * fry + dr.z * frz;
+  const float rr2eta = fast_rcp(rr2 + c.eta2);
+  if (TDDT == DSPH_DDT_DDT && a.delta != FLT_MAX) {
+    const float rhop1over2 = vr1.w * rrho2;
+    const float visc_densi = c.ddtkh * cbar * (rhop1over2 - 1.f) * rr2eta;
+    const float delta = visc_densi * dot3 * massp2;
+    a.delta = (boundp2 ? FLT_MAX : a.delta + delta);
+  }
+  if ((TDDT == DSPH_DDT_DDT2 && a.delta != FLT_MAX) || (TDDT == DSPH_DDT_DDT2FULL && !boundp2)) {
+    const float rh = 1.f + c.ddtgz * dr.z;
+    const float drho = c.rhopzero * pow_invgamma(rh, c) - c.rhopzero;
+    const float visc_densi = c.ddtkh * cbar * ((vr2.w - vr1.w) - drho) * rr2eta;
+    const float delta = visc_densi * dot3 * massp2 * rrho2;
+    a.delta = (boundp2 ? FLT_MAX : a.delta - delta);
+  }
+  const float dot = dr.x * dvx + dr.y * dvy + dr.z * dvz;
+  const float dot_rr2 = dot * rr2eta;
+  a.visc = fmaxf(dot_rr2, a.visc);
+  if (TVISCO == DSPH_VISCO_ARTIFICIAL) {
+    if (dot < 0) {
+      const float amubar = c.h * dot_rr2;
+      const float robar = (vr1.w + vr2.w) * 0.5f;
+      const float pi_visc = fast_div(-visco * cbar * amubar, robar) * massp2;
+      a.ax -= pi_visc * frx; a.ay -= pi_visc * fry; a.az -= pi_visc * frz;
+    }
+  } else {
+    const float robar2 = (vr1.w + vr2.w);
+    const float temp = fast_div(4.f * visco, (rr2 + c.eta2) * robar2);
+    const float vtemp = massp2 * temp * dot3;
+    a.ax += vtemp * dvx; a.ay += vtemp * dvy; a.az += vtemp * dvz;
+  }
+  if (TVISCO == DSPH_VISCO_LAMINARSPS) {
+    float sxx = q.tau[0], sxy = q.tau[1], sxz = q.tau[2], syy = q.tau[3], syz = q.tau[4],
+          szz = q.tau[5];
+    if (!boundp2) {
+      const float* t2 = tau + 6 * size_t(p2);
+      sxx += t2[0]; sxy += t2[1]; sxz += t2[2]; syy += t2[3]; syz += t2[4]; szz += t2[5];
+    }
+    a.ax += massp2 * (sxx * frx + sxy * fry + sxz * frz);
+    a.ay += massp2 * (sxy * frx + syy * fry + syz * frz);
+    a.az += massp2 * (sxz * frx + syz * fry + szz * frz);
+    const float volp2 = -massp2 * rrho2;
+    float dv = dvx * volp2; a.s[0] += dv * frx; a.s[1] += dv * fry; a.s[2] += dv * frz;
+    dv = dvy * volp2;       a.s[1] += dv * frx; a.s[3] += dv * fry; a.s[4] += dv * frz;
+    dv = dvz * volp2;       a.s[2] += dv * frx; a.s[4] += dv * fry; a.s[5] += dv * frz;
+  }
+  if (VELGRAD) {
+    // Full gradient G_ij = sum_b -m_b/rho_b (v_a-v_b)_i dW/dx_j.
+    const float volp2 = -massp2 * rrho2;
+    const float ux = dvx * volp2, uy = dvy * volp2, uz = dvz * volp2;
+    a.g[0] += ux * frx; a.g[1] += ux * fry; a.g[2] += ux * frz;
+    a.g[3] += uy * frx; a.g[4] += uy * fry; a.g[5] += uy * frz;
+    a.g[6] += uz * frx; a.g[7] += uz * fry; a.g[8] += uz * frz;
+  }
+}
+
+// Reference-structured pair: loads PosCell first and the rest only for a
+// real neighbour.
+template <int TVISCO, int TDDT, bool VELGRAD>
+__device__ __forceinline__ void pair_body(bool boundp2, unsigned p2, const P1& q,
+    const float4* __restrict__ poscell, const float4* __restrict__ velrho,
+    const float* __restrict__ press, const float* __restrict__ tau, float viscob, float viscof,
+    const Cte& c, Acc& a) {
+  const float4 ps2 = poscell[p2];
+  const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
+  const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
+  if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
+  pair_eval<TVISCO, TDDT, VELGRAD>(boundp2, p2, ps2, velrho[p2], press[p2], q, tau, viscob,
+                                   viscof, c, a);
+}
+
+template <int TVISCO>
+__device__ inline void load_p1(unsigned p1, const float4* __restrict__ poscell,
+                               const float4* __restrict__ velrho, const float* __restrict__ press,
+                               const float* __restrict__ tau, P1& q) {
+  q.ps = poscell[p1];
+  q.vr = velrho[p1];
+  q.press = press[p1];
+#pragma unroll
+  for (int k = 0; k < 6; k++)
+    q.tau[k] = (TVISCO == DSPH_VISCO_LAMINARSPS) ? tau[6 * size_t(p1) + k] : 0.f;
+}
+
+template <int TVISCO, int TDDT, bool VELGRAD>
+__device__ inline void store_results(unsigned p1, const Acc& a, float* __restrict__ viscdt,
+                                     float* __restrict__ ar, float* __restrict__ ace,
+                                     float* __restrict__ strain, float* __restrict__ velgrad) {
+  if (a.ar != 0 || a.ax != 0 || a.ay != 0 || a.az != 0 || a.visc != 0) {
+    float arp1 = a.ar;
+    if (TDDT != DSPH_DDT_NONE && a.delta != FLT_MAX) arp1 += a.delta;
+    ar[p1] += arp1;
+    float* ap = ace + 3 * size_t(p1);
+    ap[0] += a.ax; ap[1] += a.ay; ap[2] += a.az;
+    if (a.visc > viscdt[p1]) viscdt[p1] = a.visc;
+    if (TVISCO == DSPH_VISCO_LAMINARSPS) {
+      float* sp = strain + 6 * size_t(p1);
+#pragma unroll
+      for (int k = 0; k < 6; k++) sp[k] += a.s[k];
+    }
+  }
+  if (VELGRAD) {
+    float* gp = velgrad + 9 * size_t(p1);
+#pragma unroll
+    for (int k = 0; k < 9; k++) gp[k] = a.g[k];
+  }
+}
+
+__device__ inline NgLimits p1_limits(unsigned p1, const unsigned* __restrict__ dcell,
+                                     const DivData& dvd) {
+  const unsigned cel = dcell[p1];
+  const int cx = int(dcel_cellx(dvd.domcellcode, cel)) - dvd.czx;
+  const int cy = int(dcel_celly(dvd.domcellcode, cel)) - dvd.czy;
+  const int cz = int(dcel_cellz(dvd.domcellcode, cel)) - dvd.czz;
+  return ng_limits(cx, cy, cz, dvd);
+}
+
+// ------------------------------------------------------------ reference-order rows kernel
+template <int TVISCO, int TDDT, bool VELGRAD>
+__global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
+    unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ velrho, const float* __restrict__ press,
+    const float* __restrict__ tau, float* __restrict__ viscdt, float* __restrict__ ar,
+    float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const unsigned p1 = p + pinit;
+  Acc a;
+  acc_init(a);
+  P1 q;
+  load_p1<TVISCO>(p1, poscell, velrho, press, tau, q);
+  const NgLimits l = p1_limits(p1, dcell, dvd);
+  for (int kind = 0; kind < 2; kind++) {
+    const int base = kind == 0 ? int(dvd.cellfluid) : 0;
+    for (int z = l.z0; z < l.z1; z++)
+      for (int y = l.y0; y < l.y1; y++) {
+        unsigned pini, pfin;
+        row_range(dvd.begincell, base + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1, pini, pfin);
+        for (unsigned p2 = pini; p2 < pfin; p2++)
+          pair_body<TVISCO, TDDT, VELGRAD>(kind == 1, p2, q, poscell, velrho, press, tau, viscob,
+                                           viscof, c, a);
+      }
+  }
+  store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
+}
+
+// ------------------------------------------------------------ cell-row tile kernel
+constexpr unsigned kTile = 64;   // candidates staged in LDS per tile (one bit each)
+constexpr unsigned kMinBatch = 16;
+constexpr unsigned kRingTiles = 32;  // 16 KB of hit masks per wave
+
+__device__ inline unsigned wave_min_u32(unsigned v) {
+  for (int off = 32; off > 0; off >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, off));
+  return v;
+}
+__device__ inline unsigned wave_max_u32(unsigned v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, (unsigned)__shfl_xor((int)v, off));
+  return v;
+}
+__device__ inline int wave_max_i32(int v) {
+  for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ inline float wave_max_f32(float v) {
+  for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off));
+  return v;
+}
+__device__ inline int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline float uniform_f(float v) {
+  return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+// Bits [k0, k1) of a 64-bit mask, 0 <= k0 <= k1 <= 64.
+__device__ inline unsigned long long bit_range(unsigned k0, unsigned k1) {
+  const unsigned long long hi = k1 >= 64 ? ~0ull : ((1ull << k1) - 1ull);
+  const unsigned long long lo = k0 >= 64 ? ~0ull : ((1ull << k0) - 1ull);
+  return hi & ~lo;
+}
+
+// One wave = 64 consecutive fluid particles (sorted order). Lanes in the same
+// cell row (cy, cz) form a group (normally the whole wave); the group walks
+// the union of its lanes' neighbour rows once, fluid cells then boundary
+// cells, z-outer / y-inner (JSphGpu_ker.cu:867-896). Each row is a contiguous
+// range of sorted particles, staged in LDS 64 at a time with coalesced loads
+// (positions relative to the group origin and their squared norm). Every lane
+// tests the staged candidates (LDS broadcast) with a conservative distance
+// bound, keeps the hits inside its own index window of the row (cells
+// cx-s..cx+s) as a 64-bit mask per tile, and the masks of the last kRing
+// tiles stay in LDS. The exact pair body then runs for the queued pairs in
+// wave-uniform batches (every lane busy), each lane popping its hits in tile
+// and bit order, i.e. in the reference's summation order: the results equal
+// k_forces_fluid_rows bit for bit. Loads of the next pair are issued before
+// the arithmetic of the current one.
+template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing>
+__global__ void __launch_bounds__(64) k_forces_fluid_tile(
+    unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c, float ks2scan,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ posf, const float4* __restrict__ velrho,
+    const float* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
+    float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
+    float* __restrict__ velgrad) {
+  __shared__ float4 tile[kTile];
+  __shared__ unsigned long long masks[kRing][64];
+  __shared__ unsigned tbase[kRing];
+  const unsigned lane = threadIdx.x;
+  const unsigned p = blockIdx.x * 64 + lane;
+  const bool valid = p < n;
+  const unsigned p1 = (valid ? p : 0) + pinit;
+  Acc a;
+  acc_init(a);
+  P1 q;
+  load_p1<TVISCO>(p1, poscell, velrho, press, tau, q);
+  const float4 pf1 = posf[p1];
+  int cx, cy, cz;
+  {
+    const unsigned cel = dcell[p1];
+    cx = int(dcel_cellx(dvd.domcellcode, cel)) - dvd.czx;
+    cy = int(dcel_celly(dvd.domcellcode, cel)) - dvd.czy;
+    cz = int(dcel_cellz(dvd.domcellcode, cel)) - dvd.czz;
+  }
+  const int s = dvd.scelldiv;
+  const int lx0 = cx - min(cx, s), lx1 = cx + min(dvd.ncx - cx - 1, s) + 1;
+
+  // Queue state. wt: tiles scanned (uniform). Per lane: rt = tile being
+  // consumed, cm = its remaining hits, tb = its base index | kind << 31,
+  // len = queued hits. Invariant: len > 0 => cm != 0; len == 0 => rt == wt.
+  unsigned wt = 0, rt = 0, tb = 0, len = 0;
+  unsigned long long cm = 0;
+  auto pop = [&](unsigned& j) {
+    const unsigned bit = unsigned(__ffsll((long long)cm) - 1);
+    j = tb + bit;  // kind bit 31 carried along
+    cm &= cm - 1ull;
+    len--;
+    if (!cm) {
+      if (len) {
+        do {
+          rt++;
+          cm = masks[rt % kRing][lane];
+        } while (!cm);
+        tb = tbase[rt % kRing];
+      } else {
+        rt = wt;
+      }
+    }
+  };
+  // k wave-uniform iterations; each lane pops up to k of its queued pairs.
+  auto process = [&](unsigned k) {
+    unsigned jn = 0;
+    float4 psn = make_float4(0, 0, 0, 0), vrn = psn;
+    float prn = 0;
+    bool hn = len != 0;
+    if (hn) {
+      pop(jn);
+      const unsigned jj = jn & 0x7fffffffu;
+      psn = poscell[jj]; vrn = velrho[jj]; prn = press[jj];
+    }
+    for (unsigned i = 0; i < k; i++) {
+      const bool h = hn;
+      const unsigned j = jn;
+      const float4 ps2 = psn, vr2 = vrn;
+      const float pr2 = prn;
+      hn = (i + 1 < k) && len != 0;
+      if (hn) {
+        pop(jn);
+        const unsigned jj = jn & 0x7fffffffu;
+        psn = poscell[jj]; vrn = velrho[jj]; prn = press[jj];
+      }
+      if (h)
+        pair_eval<TVISCO, TDDT, VELGRAD>((j >> 31) != 0, j & 0x7fffffffu, ps2, vr2, pr2, q, tau,
+                                         viscob, viscof, c, a);
+    }
+  };
+
+  bool pending = valid;
+  for (;;) {
+    const unsigned long long pend = __ballot(pending);
+    if (!pend) break;
+    const int leader = __ffsll((long long)pend) - 1;
+    const int gy = uniform_i(__shfl(cy, leader)), gz = uniform_i(__shfl(cz, leader));
+    const bool member = pending && cy == gy && cz == gz;
+    pending = pending && !member;
+    const int gxmin = uniform_i(__shfl(cx, leader));
+    const int gxmax = uniform_i(wave_max_i32(member ? cx : -1));
+    const int X0 = gxmin - min(gxmin, s), X1 = gxmax + min(dvd.ncx - gxmax - 1, s) + 1;
+    const int Y0 = gy - min(gy, s), Y1 = gy + min(dvd.ncy - gy - 1, s) + 1;
+    const int Z0 = gz - min(gz, s), Z1 = gz + min(dvd.ncz - gz - 1, s) + 1;
+    // Group origin and the per-lane threshold of the expanded distance test
+    // |q1|^2 + |q2|^2 - 2 q1.q2 <= ks2scan (q relative to the origin). For a
+    // real neighbour |q|,|q2| <= Q = max|q1| + 2h, and the rounding of the
+    // expanded form stays below 8 * 2^-24 * (2Q)^2.
+    const float ox = uniform_f(__shfl(pf1.x, leader)), oy = uniform_f(__shfl(pf1.y, leader)),
+                oz = uniform_f(__shfl(pf1.z, leader));
+    const float q1x = pf1.x - ox, q1y = pf1.y - oy, q1z = pf1.z - oz;
+    const float s1 = q1x * q1x + q1y * q1y + q1z * q1z;
+    const float qm = sqrtf(uniform_f(wave_max_f32(member ? s1 : 0.f))) + 1.01f * sqrtf(ks2scan);
+    const float thr = ks2scan - s1 + 2.0e-6f * qm * qm;
+    const float m2x = -2.f * q1x, m2y = -2.f * q1y, m2z = -2.f * q1z;
+    for (int kind = 0; kind < 2; kind++) {
+      const int base = kind == 0 ? int(dvd.cellfluid) : 0;
+      const unsigned kbit = unsigned(kind) << 31;
+      for (int z = Z0; z < Z1; z++)
+        for (int y = Y0; y < Y1; y++) {
+          const int rowbase = base + z * dvd.ncw + y * dvd.ncx;
+          unsigned gini, gfin;
+          row_range(dvd.begincell, rowbase, X0, X1, gini, gfin);
+          gini = unsigned(uniform_i(int(gini)));
+          gfin = unsigned(uniform_i(int(gfin)));
+          if (gfin <= gini) continue;
+          unsigned lo = 0, hi = 0;
+          if (member) row_range(dvd.begincell, rowbase, lx0, lx1, lo, hi);
+          for (unsigned t = gini; t < gfin; t += kTile) {
+            // The ring slot of tile wt must have been consumed by every lane.
+            while (wt >= kRing && wave_min_u32(rt) <= wt - kRing) process(kMinBatch);
+            __syncthreads();
+            if (t + lane < gfin) {
+              const float4 f = posf[t + lane];
+              const float x = f.x - ox, y2 = f.y - oy, z2 = f.z - oz;
+              tile[lane] = make_float4(x, y2, z2, x * x + y2 * y2 + z2 * z2);
+            }
+            if (lane == 0) tbase[wt % kRing] = t | kbit;
+            __syncthreads();
+            // Hits of this lane among the staged candidates.
+            unsigned long long hm = 0;
+            if (member) {
+              const unsigned k0 = lo > t ? min(lo - t, kTile) : 0u;
+              const unsigned k1 = hi > t ? min(hi - t, kTile) : 0u;
+              const unsigned long long win = k1 > k0 ? bit_range(k0, k1) : 0ull;
+              if (win) {
+#pragma unroll 1
+                for (unsigned k0 = 0; k0 < kTile; k0 += 16) {
+                  unsigned b16 = 0;
+#pragma unroll
+                  for (unsigned k = 0; k < 16; k++) {
+                    const float4 f = tile[k0 + k];
+                    float d = fmaf(m2x, f.x, f.w);
+                    d = fmaf(m2y, f.y, d);
+                    d = fmaf(m2z, f.z, d);
+                    b16 |= (d <= thr) ? (1u << k) : 0u;
+                  }
+                  hm |= (unsigned long long)b16 << k0;
+                }
+                hm &= win;
+              }
+            }
+            masks[wt % kRing][lane] = hm;
+            if (len == 0) {
+              // Caught up: consume the new tile directly.
+              cm = hm;
+              tb = t | kbit;
+              rt = hm ? wt : wt + 1;
+            }
+            len += unsigned(__popcll(hm));
+            wt++;
+            const unsigned mn = wave_min_u32(member ? len : 0xffffffffu);
+            if (mn >= kMinBatch) process(mn);
+          }
+        }
+    }
+  }
+  process(wave_max_u32(len));
+  if (valid) store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
+}
+
+// Boundary particle p1 against fluid neighbours (KerInteractionForcesBound,
+// JSphGpu_ker.cu:450-533).
+__global__ void __launch_bounds__(kBsForces) k_forces_bound(unsigned n, DivData dvd, Cte c,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar) {
+  const unsigned p1 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p1 >= n) return;
+  float visc = 0, arp1 = 0;
+  const float4 ps1 = poscell[p1];
+  const float4 vr1 = velrho[p1];
+  const NgLimits l = p1_limits(p1, dcell, dvd);
+  for (int z = l.z0; z < l.z1; z++)
+    for (int y = l.y0; y < l.y1; y++) {
+      unsigned pini, pfin;
+      row_range(dvd.begincell, int(dvd.cellfluid) + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1,
+                pini, pfin);
+      for (unsigned p2 = pini; p2 < pfin; p2++) {
+        const float4 ps2 = poscell[p2];
+        const float3 dr = pscel_dr(ps1, ps2, c.pcs);
+        const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
+        if (rr2 <= c.ks2 && rr2 >= kAlmostZero) {
+          const float fac = wendland_fac(rr2, c);
+          const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
+          const float4 vr2 = velrho[p2];
+          const float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
+          arp1 += c.massf * (dvx * frx + dvy * fry + dvz * frz) * fast_div(vr1.w, vr2.w);
+          const float dot = dr.x * dvx + dr.y * dvy + dr.z * dvz;
+          const float dot_rr2 = fast_div(dot, rr2 + c.eta2);
+          visc = fmaxf(dot_rr2, visc);
+        }
+      }
+    }
+  if (arp1 != 0 || visc != 0) {
+    ar[p1] += arp1;
+    if (visc > viscdt[p1]) viscdt[p1] = visc;
+  }
+}
+
+// Radius^2 of the conservative scan test: (2h + slack)^2 where the slack
+// bounds the rounding of map-frame float positions (each coordinate within
+// half an ulp of the largest map extent, PosCell within half an ulp of pcs).
+float scan_radius2(const dsph_ctx* ctx, const Cte& c) {
+  const double lmax = std::max(std::max(ctx->cte.maprealsizex, ctx->cte.maprealsizey),
+                               ctx->cte.maprealsizez) + 2.0 * c.pcs;
+  const double ulp = lmax * 1.1920928955078125e-7;
+  const double r = std::sqrt(double(c.ks2)) + 4.0 * ulp + 1e-6 * std::sqrt(double(c.ks2));
+  return float(r * r);
+}
+
+template <int TVISCO, int TDDT, bool VELGRAD>
+void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, const Cte& c,
+                  const float* press, const float4* posf) {
+  KTimer kt(ctx, "k_forces_fluid");
+  const unsigned npf = t->np - t->npb;
+  if (ctx->forces_kernel == 0) {
+    k_forces_fluid_tile<TVISCO, TDDT, VELGRAD, kRingTiles><<<grid_for(npf, 64), 64, 0, ctx->stream>>>(
+        npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
+        (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press, t->d_spstaurho2,
+        t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, t->d_velgrad);
+  } else {
+    k_forces_fluid_rows<TVISCO, TDDT, VELGRAD><<<grid_for(npf, kBsForces), kBsForces, 0,
+                                                 ctx->stream>>>(
+        npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
+        (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
+        t->d_sps2strain, t->d_velgrad);
+  }
+}
+
+template <int TVISCO, bool VELGRAD>
+int dispatch_ddt(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, const Cte& c,
+                 const float* press, const float4* posf) {
+  switch (t->tdensity) {
+    case DSPH_DDT_NONE: launch_fluid<TVISCO, DSPH_DDT_NONE, VELGRAD>(ctx, t, dvd, c, press, posf); break;
+    case DSPH_DDT_DDT: launch_fluid<TVISCO, DSPH_DDT_DDT, VELGRAD>(ctx, t, dvd, c, press, posf); break;
+    case DSPH_DDT_DDT2: launch_fluid<TVISCO, DSPH_DDT_DDT2, VELGRAD>(ctx, t, dvd, c, press, posf); break;
+    case DSPH_DDT_DDT2FULL:
+      launch_fluid<TVISCO, DSPH_DDT_DDT2FULL, VELGRAD>(ctx, t, dvd, c, press, posf);
+      break;
+    default: return set_error(ctx, DSPH_E_ARG, "dsph_interaction_forces: invalid tdensity");
+  }
+  return DSPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int dsph_ctx_set_option(dsph_ctx* ctx, int key, int value) {
+  if (!ctx) return DSPH_E_ARG;
+  switch (key) {
+    case DSPH_OPT_FORCES_KERNEL:
+      DSPH_REQUIRE(ctx, value == 0 || value == 1, "forces kernel must be 0 (tile) or 1 (rows)");
+      ctx->forces_kernel = value;
+      return DSPH_OK;
+    default: return set_error(ctx, DSPH_E_ARG, "unknown option");
+  }
+}
+
+int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
+  if (!ctx || !t) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  DSPH_REQUIRE(ctx, t->tkernel == DSPH_KERNEL_WENDLAND, "only the Wendland kernel is built");
+  DSPH_REQUIRE(ctx, t->np >= t->npb && t->npb >= t->npbok, "invalid particle counts");
+  DSPH_REQUIRE(ctx, t->d_dcell && t->d_poscell && t->d_velrho && t->d_viscdt && t->d_ar &&
+                        t->d_ace && t->divdata.d_beginendcell,
+               "dsph_interaction_forces: null array");
+  const bool sps = t->tvisco == DSPH_VISCO_LAMINARSPS;
+  DSPH_REQUIRE(ctx, !sps || (t->d_spstaurho2 && t->d_sps2strain),
+               "Laminar+SPS needs spstaurho2 and sps2strain");
+  const Cte c = make_cte(ctx->cte);
+  const DivData dvd = make_divdata(t->divdata);
+  const unsigned npf = t->np - t->npb;
+  if (npf) {
+    int rc = ensure_prep_scratch(ctx, t->np);
+    if (rc) return rc;
+    float* press = ctx->prep_press;
+    float4* posf = reinterpret_cast<float4*>(ctx->prep_posf);
+    {
+      KTimer kt(ctx, "k_prep");
+      k_prep<<<grid_for(t->np, kBs), kBs, 0, ctx->stream>>>(
+          t->np, (const float4*)t->d_poscell, (const float4*)t->d_velrho, press,
+          ctx->forces_kernel == 0 ? posf : nullptr, c);
+    }
+    const bool vg = t->d_velgrad != nullptr;
+    switch (t->tvisco) {
+      case DSPH_VISCO_ARTIFICIAL:
+        rc = vg ? dispatch_ddt<DSPH_VISCO_ARTIFICIAL, true>(ctx, t, dvd, c, press, posf)
+                : dispatch_ddt<DSPH_VISCO_ARTIFICIAL, false>(ctx, t, dvd, c, press, posf);
+        break;
+      case DSPH_VISCO_LAMINAR:
+        rc = vg ? dispatch_ddt<DSPH_VISCO_LAMINAR, true>(ctx, t, dvd, c, press, posf)
+                : dispatch_ddt<DSPH_VISCO_LAMINAR, false>(ctx, t, dvd, c, press, posf);
+        break;
+      case DSPH_VISCO_LAMINARSPS:
+        rc = vg ? dispatch_ddt<DSPH_VISCO_LAMINARSPS, true>(ctx, t, dvd, c, press, posf)
+                : dispatch_ddt<DSPH_VISCO_LAMINARSPS, false>(ctx, t, dvd, c, press, posf);
+        break;
+      default: return set_error(ctx, DSPH_E_ARG, "dsph_interaction_forces: invalid tvisco");
+    }
+    if (rc) return rc;
+    DSPH_LAUNCHED(ctx, "k_forces_fluid");
+  }
+  if (t->npbok) {
+    KTimer kt(ctx, "k_forces_bound");
+    k_forces_bound<<<grid_for(t->npbok, kBsForces), kBsForces, 0, ctx->stream>>>(
+        t->npbok, dvd, c, t->d_dcell, (const float4*)t->d_poscell, (const float4*)t->d_velrho,
+        t->d_viscdt, t->d_ar);
+    DSPH_LAUNCHED(ctx, "k_forces_bound");
+  }
+  return DSPH_OK;
+}
+
+}  // extern "C"

@@ -1,278 +1,16 @@
-// Particle interaction kernels of the SPH step on gfx950 (FP32 pair
-// arithmetic): fluid-fluid + fluid-boundary forces, boundary-fluid continuity,
-// Smagorinsky SPS stress, mDBC ghost-node density, and the PIPS pair counter.
-//
-// Physics and summation order follow the reference GPU kernels
-// (KerInteractionForcesFluid / KerInteractionForcesFluidBox /
-// KerInteractionForcesBound, JSphGpu_ker.cu:450-990): for every particle the
-// neighbour cell rows are visited z-outer, y-inner, fluid cells first and then
-// boundary cells, particles of a row in ascending sorted index, and all
-// contributions go into one set of accumulators. Distances come from the
-// cell-relative FP32 PosCell. The pressure of every particle is evaluated once
-// per interaction (same function and input as the reference's per-pair call,
-// so each pair sees the identical value).
+// Interaction kernels beside the force interaction (forces.hip): Smagorinsky
+// SPS stress, mDBC ghost-node density and the PIPS pair counter (FP32 pair
+// arithmetic, reference kernels cited at each kernel).
 #include <cfloat>
 #include <cstring>
-#include "dsph_ctx.h"
+#include "sph_math.h"
 
 using namespace dsph;
 
 namespace {
 
-struct Cte {
-  float massb, massf, h, ks2, pcs, awen, bwenh, cs0, eta2, ddtkh, ddtgz, rhopzero,
-      invgamma, cteb, gamma, ovrhopzero, dp;
-  unsigned cellcode;
-};
-
-Cte make_cte(const dsph_cte& c) {
-  Cte r;
-  r.massb = c.massb; r.massf = c.massf; r.h = c.kernelh; r.ks2 = c.kernelsize2;
-  r.pcs = c.poscellsize; r.awen = c.awen; r.bwenh = c.bwenh; r.cs0 = c.cs0; r.eta2 = c.eta2;
-  r.ddtkh = c.ddtkh; r.ddtgz = c.ddtgz; r.rhopzero = c.rhopzero; r.invgamma = 1.f / c.gamma;
-  r.cteb = c.cteb; r.gamma = c.gamma; r.ovrhopzero = c.ovrhopzero; r.dp = c.dp;
-  r.cellcode = c.cellcode;
-  return r;
-}
-
-// Tait EOS of FunSphEos_iker.h:37-59 (B*((rho/rho0)^gamma-1), rho/rho0 as
-// rho*(1/rho0)).
-__device__ inline float eos_press(float rho, const Cte& c) {
-  return c.cteb * (powf(rho * c.ovrhopzero, c.gamma) - 1.0f);
-}
-
-// Wendland C2 gradient factor (FunSphKernel_iker.h:117-124).
-__device__ inline float wendland_fac(float rr2, const Cte& c) {
-  const float rad = sqrtf(rr2);
-  const float qq = rad / c.h;
-  const float wqq1 = 1.f - 0.5f * qq;
-  return c.bwenh * wqq1 * wqq1 * wqq1;
-}
-// Wendland C2 kernel value and gradient factor (FunSphKernel_iker.h:106-140).
-__device__ inline float wendland_wab_fac(float rr2, const Cte& c, float& fac) {
-  const float rad = sqrtf(rr2);
-  const float qq = rad / c.h;
-  const float wqq1 = 1.f - 0.5f * qq;
-  const float wqq2 = wqq1 * wqq1;
-  fac = c.bwenh * wqq2 * wqq1;
-  const float wqq = qq + qq + 1.f;
-  return c.awen * wqq * wqq2 * wqq2;
-}
-
 constexpr unsigned kBsForces = 128;
 constexpr unsigned kBs = 256;
-
-__global__ void __launch_bounds__(kBs) k_press(unsigned n, const float4* __restrict__ velrho,
-                                               float* __restrict__ press, Cte c) {
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p < n) press[p] = eos_press(velrho[p].w, c);
-}
-
-struct Acc {
-  float3 ace;
-  float ar, visc, delta;
-  float s[6];   // two_strain {xx,xy,xz,yy,yz,zz}
-  float g[9];   // full velocity gradient (learned closure features)
-};
-
-// One fluid particle p1 against the particles [pini,pfin) of a cell row
-// (KerInteractionForcesFluidBox, JSphGpu_ker.cu:539-777, in-scope options).
-template <int TVISCO, int TDDT, bool BOUNDP2, bool VELGRAD>
-__device__ inline void fluid_row(unsigned pini, unsigned pfin, float visco, float massp2,
-                                 const float4& ps1, const float4& vr1, float press1,
-                                 const float* tau1, const float4* __restrict__ poscell,
-                                 const float4* __restrict__ velrho,
-                                 const float* __restrict__ press,
-                                 const float* __restrict__ tau, const Cte& c, Acc& a) {
-  for (unsigned p2 = pini; p2 < pfin; p2++) {
-    const float4 ps2 = poscell[p2];
-    const float3 dr = pscel_dr(ps1, ps2, c.pcs);
-    const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
-    if (rr2 <= c.ks2 && rr2 >= kAlmostZero) {
-      const float fac = wendland_fac(rr2, c);
-      const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
-      const float4 vr2 = velrho[p2];
-      // Momentum: pressure term.
-      {
-        const float press2 = press[p2];
-        const float prs = (press1 + press2) / (vr1.w * vr2.w);
-        const float p_vpm = -prs * massp2;
-        a.ace.x += p_vpm * frx; a.ace.y += p_vpm * fry; a.ace.z += p_vpm * frz;
-      }
-      // Continuity.
-      const float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
-      a.ar += massp2 * (dvx * frx + dvy * fry + dvz * frz) * (vr1.w / vr2.w);
-      const float cbar = c.cs0;
-      const float dot3 = (TDDT != DSPH_DDT_NONE) ? dr.x * frx + dr.y * fry + dr.z * frz : 0.f;
-      // Density diffusion (Molteni & Colagrossi 2009), DBC boundaries.
-      if (TDDT == DSPH_DDT_DDT && a.delta != FLT_MAX) {
-        const float rhop1over2 = vr1.w / vr2.w;
-        const float visc_densi = c.ddtkh * cbar * (rhop1over2 - 1.f) / (rr2 + c.eta2);
-        const float delta = visc_densi * dot3 * massp2;
-        a.delta = (BOUNDP2 ? FLT_MAX : a.delta + delta);
-      }
-      // Density diffusion (Fourtakas et al. 2019).
-      if ((TDDT == DSPH_DDT_DDT2 || (TDDT == DSPH_DDT_DDT2FULL && !BOUNDP2)) &&
-          a.delta != FLT_MAX) {
-        const float rh = 1.f + c.ddtgz * dr.z;
-        const float drho = c.rhopzero * powf(rh, c.invgamma) - c.rhopzero;
-        const float visc_densi = c.ddtkh * cbar * ((vr2.w - vr1.w) - drho) / (rr2 + c.eta2);
-        const float delta = visc_densi * dot3 * massp2 / vr2.w;
-        a.delta = (BOUNDP2 ? FLT_MAX : a.delta - delta);
-      }
-      // Viscosity.
-      const float dot = dr.x * dvx + dr.y * dvy + dr.z * dvz;
-      const float dot_rr2 = dot / (rr2 + c.eta2);
-      a.visc = fmaxf(dot_rr2, a.visc);
-      if (TVISCO == DSPH_VISCO_ARTIFICIAL) {
-        if (dot < 0) {
-          const float amubar = c.h * dot_rr2;
-          const float robar = (vr1.w + vr2.w) * 0.5f;
-          const float pi_visc = (-visco * cbar * amubar / robar) * massp2;
-          a.ace.x -= pi_visc * frx; a.ace.y -= pi_visc * fry; a.ace.z -= pi_visc * frz;
-        }
-      } else {
-        const float robar2 = (vr1.w + vr2.w);
-        const float temp = 4.f * visco / ((rr2 + c.eta2) * robar2);
-        const float vtemp = massp2 * temp * (dr.x * frx + dr.y * fry + dr.z * frz);
-        a.ace.x += vtemp * dvx; a.ace.y += vtemp * dvy; a.ace.z += vtemp * dvz;
-      }
-      if (TVISCO == DSPH_VISCO_LAMINARSPS) {
-        float sxx = tau1[0], sxy = tau1[1], sxz = tau1[2], syy = tau1[3], syz = tau1[4],
-              szz = tau1[5];
-        if (!BOUNDP2) {
-          const float* t2 = tau + 6 * size_t(p2);
-          sxx += t2[0]; sxy += t2[1]; sxz += t2[2]; syy += t2[3]; syz += t2[4]; szz += t2[5];
-        }
-        a.ace.x += massp2 * (sxx * frx + sxy * fry + sxz * frz);
-        a.ace.y += massp2 * (sxy * frx + syy * fry + syz * frz);
-        a.ace.z += massp2 * (sxz * frx + syz * fry + szz * frz);
-        const float volp2 = -massp2 / vr2.w;
-        float dv = dvx * volp2; a.s[0] += dv * frx; a.s[1] += dv * fry; a.s[2] += dv * frz;
-        dv = dvy * volp2;       a.s[1] += dv * frx; a.s[3] += dv * fry; a.s[4] += dv * frz;
-        dv = dvz * volp2;       a.s[2] += dv * frx; a.s[4] += dv * fry; a.s[5] += dv * frz;
-      }
-      if (VELGRAD) {
-        // Full gradient G_ij = sum_b -m_b/rho_b (v_a-v_b)_i dW/dx_j.
-        const float volp2 = -massp2 / vr2.w;
-        const float ux = dvx * volp2, uy = dvy * volp2, uz = dvz * volp2;
-        a.g[0] += ux * frx; a.g[1] += ux * fry; a.g[2] += ux * frz;
-        a.g[3] += uy * frx; a.g[4] += uy * fry; a.g[5] += uy * frz;
-        a.g[6] += uz * frx; a.g[7] += uz * fry; a.g[8] += uz * frz;
-      }
-    }
-  }
-}
-
-template <int TVISCO, int TDDT, bool VELGRAD>
-__global__ void __launch_bounds__(kBsForces) k_forces_fluid(
-    unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c,
-    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
-    const float4* __restrict__ velrho, const float* __restrict__ press,
-    const float* __restrict__ tau, float* __restrict__ viscdt, float* __restrict__ ar,
-    float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad) {
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const unsigned p1 = p + pinit;
-  Acc a;
-  a.ace = make_float3(0, 0, 0); a.ar = 0; a.visc = 0; a.delta = 0;
-#pragma unroll
-  for (int k = 0; k < 6; k++) a.s[k] = 0;
-#pragma unroll
-  for (int k = 0; k < 9; k++) a.g[k] = 0;
-  const float4 ps1 = poscell[p1];
-  const float4 vr1 = velrho[p1];
-  const float press1 = press[p1];
-  float tau1[6] = {0, 0, 0, 0, 0, 0};
-  if (TVISCO == DSPH_VISCO_LAMINARSPS) {
-#pragma unroll
-    for (int k = 0; k < 6; k++) tau1[k] = tau[6 * size_t(p1) + k];
-  }
-  const unsigned cel = dcell[p1];
-  const int cx = int(dcel_cellx(dvd.domcellcode, cel)) - dvd.czx;
-  const int cy = int(dcel_celly(dvd.domcellcode, cel)) - dvd.czy;
-  const int cz = int(dcel_cellz(dvd.domcellcode, cel)) - dvd.czz;
-  const NgLimits l = ng_limits(cx, cy, cz, dvd);
-  // Fluid neighbours.
-  for (int z = l.z0; z < l.z1; z++)
-    for (int y = l.y0; y < l.y1; y++) {
-      unsigned pini, pfin;
-      row_range(dvd.begincell, int(dvd.cellfluid) + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1,
-                pini, pfin);
-      if (pfin)
-        fluid_row<TVISCO, TDDT, false, VELGRAD>(pini, pfin, viscof, c.massf, ps1, vr1, press1,
-                                                tau1, poscell, velrho, press, tau, c, a);
-    }
-  // Boundary neighbours.
-  for (int z = l.z0; z < l.z1; z++)
-    for (int y = l.y0; y < l.y1; y++) {
-      unsigned pini, pfin;
-      row_range(dvd.begincell, z * dvd.ncw + y * dvd.ncx, l.x0, l.x1, pini, pfin);
-      if (pfin)
-        fluid_row<TVISCO, TDDT, true, VELGRAD>(pini, pfin, viscob, c.massb, ps1, vr1, press1,
-                                               tau1, poscell, velrho, press, tau, c, a);
-    }
-  if (a.ar != 0 || a.ace.x != 0 || a.ace.y != 0 || a.ace.z != 0 || a.visc != 0) {
-    float arp1 = a.ar;
-    if (TDDT != DSPH_DDT_NONE && a.delta != FLT_MAX) arp1 += a.delta;
-    ar[p1] += arp1;
-    float* ap = ace + 3 * size_t(p1);
-    ap[0] += a.ace.x; ap[1] += a.ace.y; ap[2] += a.ace.z;
-    if (a.visc > viscdt[p1]) viscdt[p1] = a.visc;
-    if (TVISCO == DSPH_VISCO_LAMINARSPS) {
-      float* sp = strain + 6 * size_t(p1);
-#pragma unroll
-      for (int k = 0; k < 6; k++) sp[k] += a.s[k];
-    }
-  }
-  if (VELGRAD) {
-    float* gp = velgrad + 9 * size_t(p1);
-#pragma unroll
-    for (int k = 0; k < 9; k++) gp[k] = a.g[k];
-  }
-}
-
-// Boundary particle p1 against fluid neighbours (KerInteractionForcesBound,
-// JSphGpu_ker.cu:450-533).
-__global__ void __launch_bounds__(kBsForces) k_forces_bound(unsigned n, DivData dvd, Cte c,
-    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
-    const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar) {
-  const unsigned p1 = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p1 >= n) return;
-  float visc = 0, arp1 = 0;
-  const float4 ps1 = poscell[p1];
-  const float4 vr1 = velrho[p1];
-  const unsigned cel = dcell[p1];
-  const int cx = int(dcel_cellx(dvd.domcellcode, cel)) - dvd.czx;
-  const int cy = int(dcel_celly(dvd.domcellcode, cel)) - dvd.czy;
-  const int cz = int(dcel_cellz(dvd.domcellcode, cel)) - dvd.czz;
-  const NgLimits l = ng_limits(cx, cy, cz, dvd);
-  for (int z = l.z0; z < l.z1; z++)
-    for (int y = l.y0; y < l.y1; y++) {
-      unsigned pini, pfin;
-      row_range(dvd.begincell, int(dvd.cellfluid) + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1,
-                pini, pfin);
-      for (unsigned p2 = pini; p2 < pfin; p2++) {
-        const float4 ps2 = poscell[p2];
-        const float3 dr = pscel_dr(ps1, ps2, c.pcs);
-        const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
-        if (rr2 <= c.ks2 && rr2 >= kAlmostZero) {
-          const float fac = wendland_fac(rr2, c);
-          const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
-          const float4 vr2 = velrho[p2];
-          const float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
-          arp1 += c.massf * (dvx * frx + dvy * fry + dvz * frz) * (vr1.w / vr2.w);
-          const float dot = dr.x * dvx + dr.y * dvy + dr.z * dvz;
-          const float dot_rr2 = dot / (rr2 + c.eta2);
-          visc = fmaxf(dot_rr2, visc);
-        }
-      }
-    }
-  if (arp1 != 0 || visc != 0) {
-    ar[p1] += arp1;
-    if (visc > viscdt[p1]) viscdt[p1] = visc;
-  }
-}
 
 // Smagorinsky + Blin SPS stress tau/rho^2 (KerComputeSpsTau, JSphGpu_ker.cu:1387-1420).
 __global__ void __launch_bounds__(kBs) k_sps_tau(unsigned n, unsigned pini, float smag,
@@ -479,91 +217,9 @@ __global__ void k_zero_u64(unsigned long long* a, unsigned n) {
   if (threadIdx.x < n) a[threadIdx.x] = 0ull;
 }
 
-template <int TVISCO, int TDDT, bool VELGRAD>
-void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, const Cte& c,
-                  const float* press) {
-  KTimer kt(ctx, "k_forces_fluid");
-  const unsigned npf = t->np - t->npb;
-  k_forces_fluid<TVISCO, TDDT, VELGRAD><<<grid_for(npf, kBsForces), kBsForces, 0, ctx->stream>>>(
-      npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
-      (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
-      t->d_sps2strain, t->d_velgrad);
-}
-
-template <int TVISCO, bool VELGRAD>
-int dispatch_ddt(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, const Cte& c,
-                 const float* press) {
-  switch (t->tdensity) {
-    case DSPH_DDT_NONE: launch_fluid<TVISCO, DSPH_DDT_NONE, VELGRAD>(ctx, t, dvd, c, press); break;
-    case DSPH_DDT_DDT: launch_fluid<TVISCO, DSPH_DDT_DDT, VELGRAD>(ctx, t, dvd, c, press); break;
-    case DSPH_DDT_DDT2: launch_fluid<TVISCO, DSPH_DDT_DDT2, VELGRAD>(ctx, t, dvd, c, press); break;
-    case DSPH_DDT_DDT2FULL:
-      launch_fluid<TVISCO, DSPH_DDT_DDT2FULL, VELGRAD>(ctx, t, dvd, c, press);
-      break;
-    default: return set_error(ctx, DSPH_E_ARG, "dsph_interaction_forces: invalid tdensity");
-  }
-  return DSPH_OK;
-}
-
 }  // namespace
 
 extern "C" {
-
-int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
-  if (!ctx || !t) return DSPH_E_ARG;
-  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
-  DSPH_REQUIRE(ctx, t->tkernel == DSPH_KERNEL_WENDLAND, "only the Wendland kernel is built");
-  DSPH_REQUIRE(ctx, t->np >= t->npb && t->npb >= t->npbok, "invalid particle counts");
-  DSPH_REQUIRE(ctx, t->d_dcell && t->d_poscell && t->d_velrho && t->d_viscdt && t->d_ar &&
-                        t->d_ace && t->divdata.d_beginendcell,
-               "dsph_interaction_forces: null array");
-  const bool sps = t->tvisco == DSPH_VISCO_LAMINARSPS;
-  DSPH_REQUIRE(ctx, !sps || (t->d_spstaurho2 && t->d_sps2strain),
-               "Laminar+SPS needs spstaurho2 and sps2strain");
-  const Cte c = make_cte(ctx->cte);
-  const DivData dvd = make_divdata(t->divdata);
-  const unsigned npf = t->np - t->npb;
-  if (npf) {
-    int rc = ensure_redu_scratch(ctx, 0);
-    if (rc) return rc;
-    // Pressure of every particle (fluid neighbours read it; boundary too).
-    float* press = nullptr;
-    rc = ensure_sort_scratch(ctx, t->np);
-    if (rc) return rc;
-    press = reinterpret_cast<float*>(ctx->sort_keys_alt);  // sort scratch is free here
-    {
-      KTimer kt(ctx, "k_press");
-      k_press<<<grid_for(t->np, kBs), kBs, 0, ctx->stream>>>(t->np, (const float4*)t->d_velrho,
-                                                              press, c);
-    }
-    const bool vg = t->d_velgrad != nullptr;
-    switch (t->tvisco) {
-      case DSPH_VISCO_ARTIFICIAL:
-        rc = vg ? dispatch_ddt<DSPH_VISCO_ARTIFICIAL, true>(ctx, t, dvd, c, press)
-                : dispatch_ddt<DSPH_VISCO_ARTIFICIAL, false>(ctx, t, dvd, c, press);
-        break;
-      case DSPH_VISCO_LAMINAR:
-        rc = vg ? dispatch_ddt<DSPH_VISCO_LAMINAR, true>(ctx, t, dvd, c, press)
-                : dispatch_ddt<DSPH_VISCO_LAMINAR, false>(ctx, t, dvd, c, press);
-        break;
-      case DSPH_VISCO_LAMINARSPS:
-        rc = vg ? dispatch_ddt<DSPH_VISCO_LAMINARSPS, true>(ctx, t, dvd, c, press)
-                : dispatch_ddt<DSPH_VISCO_LAMINARSPS, false>(ctx, t, dvd, c, press);
-        break;
-      default: return set_error(ctx, DSPH_E_ARG, "dsph_interaction_forces: invalid tvisco");
-    }
-    if (rc) return rc;
-    DSPH_LAUNCHED(ctx, "k_forces_fluid");
-  }
-  if (t->npbok) {
-    KTimer kt(ctx, "k_forces_bound");
-    k_forces_bound<<<grid_for(t->npbok, kBsForces), kBsForces, 0, ctx->stream>>>(
-        t->npbok, dvd, c, t->d_dcell, (const float4*)t->d_poscell, (const float4*)t->d_velrho,
-        t->d_viscdt, t->d_ar);
-    DSPH_LAUNCHED(ctx, "k_forces_bound");
-  }
-  return DSPH_OK;
-}
 
 int dsph_sps_tau(dsph_ctx* ctx, uint32_t np, uint32_t npb, float smag, float blin,
                  const float* d_velrho, const float* d_sps2strain, float* d_tau) {

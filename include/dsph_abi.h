@@ -168,6 +168,14 @@ int dsph_ctx_timing_get(dsph_ctx* ctx, uint32_t idx, char* name, size_t name_len
                         uint64_t* count, double* total_ms);
 int dsph_ctx_timing_reset(dsph_ctx* ctx);
 
+/* Context options (no counterpart in the reference: implementation choices
+ * with identical results up to FP32 rounding). */
+#define DSPH_OPT_FORCES_KERNEL 1  /* fluid interaction kernel: 0 = cell rows
+                                     staged in LDS per wave with per-lane pair
+                                     queues (default), 1 = one lane per
+                                     particle over its cell rows */
+int dsph_ctx_set_option(dsph_ctx* ctx, int key, int value);
+
 /* Replaces cusph::CteInteractionUp (JSphGpu_ker.h:284). Copies into ctx. */
 int dsph_set_constants(dsph_ctx* ctx, const dsph_cte* cte);
 

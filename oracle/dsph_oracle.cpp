@@ -183,6 +183,14 @@ namespace {
 
 enum { SEM_GPU = 0, SEM_CPU = 1, SEM_F64 = 2 };
 
+// Power rounded once to R: for R=float the correctly rounded FP32 power (the
+// FP64 power rounded to float), the common target of every FP32 path since
+// the reference's own powf is platform dependent.
+template <typename R>
+inline R crpow(R x, R y) {
+  return R(std::pow(double(x), double(y)));
+}
+
 // Wendland C2 (FunSphKernel.h:198-244) in precision R.
 template <typename R>
 inline R wend_fac(R rr2, R h, R bwenh) {
@@ -304,7 +312,7 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
       }
       if ((o.tddt == 2 || (o.tddt == 3 && !boundp2)) && !a.delta_off) {
         const R rh = R(1) + ddtgz * dz;
-        const R drho = rho0 * std::pow(rh, invgamma) - rho0;
+        const R drho = rho0 * crpow<R>(rh, invgamma) - rho0;
         const R visc_densi = ddtkh * cbar * ((rho2 - rho1) - drho) / (rr2 + eta2);
         const R delta = visc_densi * dot3 * massp2 / rho2;
         if (boundp2) a.delta_off = true;
@@ -351,11 +359,12 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
     }
 }
 
+// Tait EOS (FunSphEos.h:38-66 / FunSphEos_iker.h:37-59).
 template <typename R>
 R eos(int sem, const orc_cte& c, float rho) {
   if (sem == SEM_GPU)
-    return R(c.cteb) * (std::pow(R(rho) * R(c.ovrhopzero), R(c.gamma)) - R(1));
-  return R(c.cteb) * (std::pow(R(rho) / R(c.rhopzero), R(c.gamma)) - R(1));
+    return R(c.cteb) * (crpow<R>(R(rho) * R(c.ovrhopzero), R(c.gamma)) - R(1));
+  return R(c.cteb) * (crpow<R>(R(rho) / R(c.rhopzero), R(c.gamma)) - R(1));
 }
 
 template <typename R>

@@ -29,8 +29,9 @@ def _prepared(c):
     return o, d, dcell, poscell, cte, celldiv, be
 
 
-def _run_both(c, tvisco, tddt, visco, viscob, lib, ctx, with_tau=False):
+def _run_both(c, tvisco, tddt, visco, viscob, lib, ctx, with_tau=False, kernel=N.FORCES_TILE):
     import oracle.oracle as orc
+    assert lib.dsph_ctx_set_option(ctx, N.OPT_FORCES_KERNEL, kernel) == 0
     o, d, dcell, poscell, cte, celldiv, be = _prepared(c)
     st = o.stats()
     n, npb, npbok = st["np"], st["npb"], st["npbok"]
@@ -91,12 +92,14 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("kernel", [N.FORCES_TILE, N.FORCES_ROWS], ids=["tile", "rows"])
 @pytest.mark.parametrize("name,pert,tvisco,tddt,visco,viscob", CASES, ids=[x[0] for x in CASES])
-def test_interaction_parity(lib, ctx, name, pert, tvisco, tddt, visco, viscob):
+def test_interaction_parity(lib, ctx, name, pert, tvisco, tddt, visco, viscob, kernel):
     c = casemod.dam_break(dp=0.02)
     if pert:
         c = perturbed_case(c)
-    g, res, npb = _run_both(c, tvisco, tddt, visco, viscob, lib, ctx, with_tau=(tvisco == 2))
+    g, res, npb = _run_both(c, tvisco, tddt, visco, viscob, lib, ctx, with_tau=(tvisco == 2),
+                            kernel=kernel)
     o32, o64 = res[0], res[1]
     fl = slice(npb, None)
     e_ace = rel_err(g["ace"][fl], o32["ace"][fl])
@@ -123,6 +126,16 @@ def test_interaction_parity(lib, ctx, name, pert, tvisco, tddt, visco, viscob):
     assert np.max(pp) < TOL_PP, np.max(pp)
     if tvisco == 2:
         assert rel_err(g["strain"][fl], o32["strain"][fl]) < TOL_FP32
+
+
+def test_tile_kernel_equals_rows_kernel_bitwise(lib, ctx):
+    """Both fluid kernels sum the same pairs in the same order: identical bits."""
+    c = perturbed_case(casemod.dam_break(dp=0.01), seed=3)
+    gq, _, npb = _run_both(c, 1, 3, 0.01, 0.01, lib, ctx, kernel=N.FORCES_TILE)
+    gr, _, _ = _run_both(c, 1, 3, 0.01, 0.01, lib, ctx, kernel=N.FORCES_ROWS)
+    lib.dsph_ctx_set_option(ctx, N.OPT_FORCES_KERNEL, N.FORCES_TILE)
+    for k in ("ace", "ar", "viscdt"):
+        np.testing.assert_array_equal(gq[k].view(np.uint32), gr[k].view(np.uint32))
 
 
 def test_pips_counts_exact(lib, ctx):
