@@ -8,7 +8,9 @@ from __future__ import annotations
 import ctypes as C
 from pathlib import Path
 
-LIB_PATH = Path(__file__).resolve().parent / "libdsph.so"
+import os
+
+LIB_PATH = Path(os.environ.get("DSPH_LIB_PATH", Path(__file__).resolve().parent / "libdsph.so"))
 
 DSPH_OK, DSPH_E_ARG, DSPH_E_LAUNCH, DSPH_E_NOMEM, DSPH_E_STATE, DSPH_E_NUMERIC = range(6)
 KERNEL_WENDLAND = 2

@@ -114,7 +114,7 @@ int dsph_ctx_reserve(dsph_ctx* ctx, uint32_t np_max, uint32_t nctt_max) {
     hipFree(ctx->prep_press); ctx->prep_press = nullptr;
     hipFree(ctx->prep_posf); ctx->prep_posf = nullptr;
     ctx->prep_cap = 0;
-    DSPH_HIP(ctx, hipMalloc(&ctx->prep_press, sizeof(float) * size_t(np_max)));
+    DSPH_HIP(ctx, hipMalloc(&ctx->prep_press, 2 * sizeof(float) * size_t(np_max)));
     DSPH_HIP(ctx, hipMalloc(&ctx->prep_posf, 4 * sizeof(float) * size_t(np_max)));
     ctx->prep_cap = np_max;
   }

@@ -36,7 +36,7 @@ struct dsph_ctx {
   uint32_t* small_dev = nullptr;   // 64 words: limits, pips counters
   void* pinned = nullptr;          // 4 KiB pinned host buffer for readbacks
   // Per-interaction preparation: pressure and map-frame float positions.
-  float* prep_press = nullptr;
+  float* prep_press = nullptr;     // float2 {press, 1/rho} per particle
   float* prep_posf = nullptr;      // float4 per particle
   size_t prep_cap = 0;
   int forces_kernel = 0;           // DSPH_OPT_FORCES_KERNEL: 0 queue, 1 rows

@@ -31,14 +31,16 @@ constexpr unsigned kBsForces = 128;
 constexpr unsigned kBs = 256;
 
 // Per-interaction preparation: pressure (EOS once per particle; each pair sees
-// the same value the reference computes per pair) and float positions in the
-// map frame for the conservative neighbour test.
+// the same value the reference computes per pair), the reciprocal density
+// (v_rcp_f32, as a fast-math quotient by rho computes it), and float positions
+// in the map frame for the conservative neighbour test.
 __global__ void __launch_bounds__(kBs) k_prep(unsigned n, const float4* __restrict__ poscell,
-    const float4* __restrict__ velrho, float* __restrict__ press, float4* __restrict__ posf,
+    const float4* __restrict__ velrho, float2* __restrict__ press, float4* __restrict__ posf,
     Cte c) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  press[p] = eos_press(velrho[p].w, c);
+  const float rho = velrho[p].w;
+  press[p] = make_float2(eos_press(rho, c), fast_rcp(rho));
   if (posf) {
     const float4 ps = poscell[p];
     const double pcs = c.pcs;
@@ -64,18 +66,18 @@ __device__ inline void acc_init(Acc& a) {
 
 struct P1 {
   float4 ps, vr;
-  float press;
+  float press, rrho;
   float tau[6];
 };
 
 // Exact pair body for fluid p1 and neighbour p2 (KerInteractionForcesFluidBox,
 // JSphGpu_ker.cu:539-777, in-scope options). boundp2 selects the boundary
 // variant (mass, viscosity, no SPS tau of p2, delta-SPH rules).
-// pair_eval takes the neighbour's PosCell, velrho and pressure already loaded
+// pair_eval takes the neighbour's PosCell, velrho and {pressure, 1/rho} already loaded
 // and applies the exact neighbour test rr2 <= (2h)^2, rr2 >= ALMOSTZERO.
 template <int TVISCO, int TDDT, bool VELGRAD>
 __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float4& ps2,
-    const float4& vr2, float press2, const P1& q, const float* __restrict__ tau, float viscob,
+    const float4& vr2, float2 pr2, const P1& q, const float* __restrict__ tau, float viscob,
     float viscof, const Cte& c, Acc& a) {
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
@@ -86,12 +88,12 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
   const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
   const float4& vr1 = q.vr;
   {
-    const float prs = fast_div(q.press + press2, vr1.w * vr2.w);
+    const float prs = (q.press + pr2.x) * (q.rrho * pr2.y);
     const float p_vpm = -prs * massp2;
     a.ax += p_vpm * frx; a.ay += p_vpm * fry; a.az += p_vpm * frz;
   }
   const float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
-  const float rrho2 = fast_rcp(vr2.w);
+  const float rrho2 = pr2.y;
   a.ar += massp2 * (dvx * frx + dvy * fry + dvz * frz) * (vr1.w * rrho2);
   const float cbar = c.cs0;
   const float dot3 = dr.x * frx + dr.y * fry + dr.z * frz;
@@ -155,7 +157,7 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
 template <int TVISCO, int TDDT, bool VELGRAD>
 __device__ __forceinline__ void pair_body(bool boundp2, unsigned p2, const P1& q,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
-    const float* __restrict__ press, const float* __restrict__ tau, float viscob, float viscof,
+    const float2* __restrict__ press, const float* __restrict__ tau, float viscob, float viscof,
     const Cte& c, Acc& a) {
   const float4 ps2 = poscell[p2];
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
@@ -167,11 +169,13 @@ __device__ __forceinline__ void pair_body(bool boundp2, unsigned p2, const P1& q
 
 template <int TVISCO>
 __device__ inline void load_p1(unsigned p1, const float4* __restrict__ poscell,
-                               const float4* __restrict__ velrho, const float* __restrict__ press,
+                               const float4* __restrict__ velrho, const float2* __restrict__ press,
                                const float* __restrict__ tau, P1& q) {
   q.ps = poscell[p1];
   q.vr = velrho[p1];
-  q.press = press[p1];
+  const float2 pr1 = press[p1];
+  q.press = pr1.x;
+  q.rrho = pr1.y;
 #pragma unroll
   for (int k = 0; k < 6; k++)
     q.tau[k] = (TVISCO == DSPH_VISCO_LAMINARSPS) ? tau[6 * size_t(p1) + k] : 0.f;
@@ -215,7 +219,7 @@ template <int TVISCO, int TDDT, bool VELGRAD>
 __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
-    const float4* __restrict__ velrho, const float* __restrict__ press,
+    const float4* __restrict__ velrho, const float2* __restrict__ press,
     const float* __restrict__ tau, float* __restrict__ viscdt, float* __restrict__ ar,
     float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -243,7 +247,11 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
 // ------------------------------------------------------------ cell-row tile kernel
 constexpr unsigned kTile = 64;   // candidates staged in LDS per tile (one bit each)
 constexpr unsigned kMinBatch = 16;
-constexpr unsigned kRingTiles = 32;  // 16 KB of hit masks per wave
+#ifndef DSPH_RING_TILES
+#define DSPH_RING_TILES 32
+#endif
+// Tiles of hit masks kept per wave (8 bytes per lane and tile).
+constexpr unsigned kRingTiles = DSPH_RING_TILES;
 
 __device__ inline unsigned wave_min_u32(unsigned v) {
   for (int off = 32; off > 0; off >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, off));
@@ -291,7 +299,7 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c, float ks2scan,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ posf, const float4* __restrict__ velrho,
-    const float* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
+    const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
     float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
     float* __restrict__ velgrad) {
   __shared__ float4 tile[kTile];
@@ -342,7 +350,7 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
   auto process = [&](unsigned k) {
     unsigned jn = 0;
     float4 psn = make_float4(0, 0, 0, 0), vrn = psn;
-    float prn = 0;
+    float2 prn = make_float2(0, 0);
     bool hn = len != 0;
     if (hn) {
       pop(jn);
@@ -353,7 +361,7 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
       const bool h = hn;
       const unsigned j = jn;
       const float4 ps2 = psn, vr2 = vrn;
-      const float pr2 = prn;
+      const float2 pr2 = prn;
       hn = (i + 1 < k) && len != 0;
       if (hn) {
         pop(jn);
@@ -507,7 +515,7 @@ float scan_radius2(const dsph_ctx* ctx, const Cte& c) {
 
 template <int TVISCO, int TDDT, bool VELGRAD>
 void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, const Cte& c,
-                  const float* press, const float4* posf) {
+                  const float2* press, const float4* posf) {
   KTimer kt(ctx, "k_forces_fluid");
   const unsigned npf = t->np - t->npb;
   if (ctx->forces_kernel == 0) {
@@ -526,7 +534,7 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
 
 template <int TVISCO, bool VELGRAD>
 int dispatch_ddt(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, const Cte& c,
-                 const float* press, const float4* posf) {
+                 const float2* press, const float4* posf) {
   switch (t->tdensity) {
     case DSPH_DDT_NONE: launch_fluid<TVISCO, DSPH_DDT_NONE, VELGRAD>(ctx, t, dvd, c, press, posf); break;
     case DSPH_DDT_DDT: launch_fluid<TVISCO, DSPH_DDT_DDT, VELGRAD>(ctx, t, dvd, c, press, posf); break;
@@ -571,7 +579,7 @@ int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
   if (npf) {
     int rc = ensure_prep_scratch(ctx, t->np);
     if (rc) return rc;
-    float* press = ctx->prep_press;
+    float2* press = reinterpret_cast<float2*>(ctx->prep_press);
     float4* posf = reinterpret_cast<float4*>(ctx->prep_posf);
     {
       KTimer kt(ctx, "k_prep");

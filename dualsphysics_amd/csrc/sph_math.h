@@ -43,14 +43,14 @@ __device__ inline float eos_press(float rho, const Cte& c) {
 }
 
 // (1+u)^(1/gamma) for the Fourtakas term rh = 1 + ddtgz*dz (JSphGpu_ker.cu:668-674),
-// |u| <= ddtgz*2h ~ 1e-3: binomial series to u^5 (truncation < 1e-16 for
-// |u| < 1e-3), rounded once at 1+s, which is the correctly rounded power except
-// when the exact value lies within ~1e-10 of a rounding boundary. Outside
-// |u| < 1e-2 the library powf is used.
+// |u| <= ddtgz*2h ~ 1e-3: binomial series to u^4 (truncation < 1e-11 for
+// |u| < 1e-2, below the rounding of the FP32 sum), rounded once at 1+s, which
+// is the correctly rounded power except when the exact value lies within
+// ~1e-10 of a rounding boundary. Outside |u| < 1e-2 the library powf is used.
 __device__ inline float pow_invgamma(float rh, const Cte& c) {
   const float u = rh - 1.f;
   if (fabsf(u) < 1e-2f) {
-    const float s = u * (c.pc1 + u * (c.pc2 + u * (c.pc3 + u * (c.pc4 + u * c.pc5))));
+    const float s = u * (c.pc1 + u * (c.pc2 + u * (c.pc3 + u * c.pc4)));
     return 1.f + s;
   }
   return powf(rh, c.invgamma);

@@ -652,7 +652,7 @@ void orc_bruteforce_fp64(const orc_cte* c, int tvisco, int tddt, float viscob, f
   const double h = c->kernelh, ks2 = c->kernelsize2, eta2 = c->eta2, cbar = c->cs0;
   std::vector<double> press(np);
   for (unsigned p = 0; p < np; p++)
-    press[p] = double(c->cteb) * (std::pow(double(velrho[4 * size_t(p) + 3]) * double(c->ovrhopzero),
+    press[p] = double(c->cteb) * (std::pow(double(velrho[4 * size_t(p) + 3]) / double(c->rhopzero),
                                            double(c->gamma)) - 1.0);
 #pragma omp parallel for schedule(dynamic, 16)
   for (long pp = 0; pp < long(np); pp++) {
@@ -713,6 +713,22 @@ void orc_bruteforce_fp64(const orc_cte* c, int tvisco, int tddt, float viscob, f
     ar[p1] = arp + (tddt && !delta_off ? delta : 0);
     viscdt[p1] = visc;
   }
+}
+
+// Kernel and EOS of one value in precision prec (0 = FP32, 1 = FP64), the
+// same helpers the interaction uses (FunSphKernel.h:198-244, FunSphEos.h:38-66).
+void orc_kernel_eval(int prec, float rr2, float h, float awen, float bwenh, double* wab,
+                     double* fac) {
+  if (prec == 0) {
+    *wab = wend_wab<float>(rr2, h, awen);
+    *fac = wend_fac<float>(rr2, h, bwenh);
+  } else {
+    *wab = wend_wab<double>(rr2, h, awen);
+    *fac = wend_fac<double>(rr2, h, bwenh);
+  }
+}
+double orc_eos(int prec, int sem, const orc_cte* c, float rho) {
+  return prec == 0 ? double(eos<float>(sem, *c, rho)) : eos<double>(sem, *c, rho);
 }
 
 }  // extern "C"

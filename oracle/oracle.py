@@ -68,6 +68,8 @@ def load() -> C.CDLL:
             "orc_sim_pips": ([vp, vp], None),
             "orc_sim_interact": ([vp], None),
             "orc_sim_arrays": ([vp, vp, vp, C.POINTER(OrcCte)], None),
+            "orc_kernel_eval": ([i, f, f, f, f, C.POINTER(d), C.POINTER(d)], None),
+            "orc_eos": ([i, i, C.POINTER(OrcCte), f], d),
         }
         for name, (args, res) in sig.items():
             fn = getattr(lib, name)

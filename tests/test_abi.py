@@ -1,0 +1,88 @@
+"""The C-ABI boundary without a GPU: the library loads, exports every entry
+point that include/dsph_abi.h declares (and nothing the bindings do not
+know), and the product path refuses to run without it (no CPU fallback)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from dualsphysics_amd import _native as N
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "dsph_abi.h"
+
+
+def declared_functions() -> set[str]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    return set(re.findall(r"\b(dsph_[a-z0-9_]+)\s*\(", text))
+
+
+def test_header_and_bindings_agree():
+    decl = declared_functions()
+    assert decl, "no declarations parsed"
+    assert decl == set(N.SIGNATURES), (sorted(decl - set(N.SIGNATURES)),
+                                       sorted(set(N.SIGNATURES) - decl))
+
+
+def test_library_exports_every_declared_symbol():
+    if not N.LIB_PATH.exists():
+        pytest.fail(f"{N.LIB_PATH} not built (python -m dualsphysics_amd.build)")
+    lib = C.CDLL(str(N.LIB_PATH))
+    missing = [name for name in sorted(declared_functions()) if not hasattr(lib, name)]
+    assert not missing, missing
+
+
+def test_bindings_load_with_signatures():
+    lib = N.load()
+    for name, args in N.SIGNATURES.items():
+        fn = getattr(lib, name)
+        assert list(fn.argtypes) == list(args), name
+
+
+STRUCTS = {"dsph_cte": N.DsphCte, "dsph_divdata": N.DsphDivData,
+           "dsph_inter_parms": N.DsphInterParms, "dsph_case": N.DsphCase,
+           "dsph_stats": N.DsphStats}
+
+
+def test_struct_layouts_match_the_c_compiler(tmp_path):
+    """sizeof/offsetof of every ABI struct as gcc lays it out from the header
+    equal the ctypes mirrors used by the bindings."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no C compiler")
+    lines = ["#include <stdio.h>", "#include <stddef.h>", '#include "dsph_abi.h"',
+             "int main(void) {"]
+    for sname, cls in STRUCTS.items():
+        lines.append(f'  printf("{sname} size %zu\\n", sizeof({sname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{sname} {fname} %zu\\n", offsetof({sname}, {fname}));')
+    lines += ["  return 0;", "}"]
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    r = subprocess.run([cc, "-std=c99", f"-I{ROOT / 'include'}", str(src), "-o", str(exe)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n")
+    got = {tuple(l.split()[:2]): int(l.split()[2]) for l in out if l.strip()}
+    for sname, cls in STRUCTS.items():
+        assert got[(sname, "size")] == C.sizeof(cls), sname
+        for fname, _ in cls._fields_:
+            assert got[(sname, fname)] == getattr(cls, fname).offset, (sname, fname)
+
+
+def test_product_path_fails_loudly_without_library(monkeypatch, tmp_path):
+    monkeypatch.setattr(N, "_lib", None)
+    monkeypatch.setattr(N, "LIB_PATH", tmp_path / "missing.so")
+    with pytest.raises(RuntimeError, match="no non-HIP fallback"):
+        N.load()
+    from dualsphysics_amd import sim
+    from dualsphysics_amd import case as casemod
+    c = casemod.dam_break(dp=0.05)
+    with pytest.raises(RuntimeError):
+        sim.SphGpuSingle(c)
