@@ -14,7 +14,8 @@
 //             fluid and boundary neighbours in one accumulator set
 //             (KerInteractionForcesFluid, JSphGpu_ker.cu:786-990);
 //   SEM_CPU : distances float(FP64 difference), EOS rho/rho0, fluid and
-//             boundary neighbours in two passes (JSphCpu.cpp:733-1141,
+//             boundary neighbours in two passes, density diffusion through
+//             the delta array (JSphCpu.cpp:733-1141,
 //             1315-1372);
 //   SEM_F64 : distances in FP64 from FP64 positions (yardstick).
 //
@@ -388,14 +389,19 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
     R oax, oay, oaz, oar, ovisc;
     R os[6], og[9];
     if (o.sem == SEM_CPU) {
-      // Two passes, each stored separately (JSphCpu.cpp:1331-1349).
+      // Two passes, each stored separately (JSphCpu.cpp:1331-1349). The
+      // density-diffusion term goes to the per-particle delta array of the CPU
+      // build (DDTArray, JSph.cpp:928; JSphCpu.cpp:1086): a boundary neighbour
+      // in either pass drops the whole term, as on the GPU; otherwise the two
+      // partial terms are summed and added to ar after both passes
+      // (AddDelta, JSphCpuSingle.cpp:726).
       FAcc<R> a1, a2;
       fluid_rows<R>(o, c, false, rf, nf, p1, ps, pos, velrho, press.data(), tau, a1);
       fluid_rows<R>(o, c, true, rb, nb, p1, ps, pos, velrho, press.data(), tau, a2);
-      R ar1 = a1.ar + (o.tddt && !a1.delta_off ? a1.delta : R(0));
-      R ar2 = a2.ar + (o.tddt && !a2.delta_off ? a2.delta : R(0));
       oax = R(float(a1.ax)) + a2.ax; oay = R(float(a1.ay)) + a2.ay; oaz = R(float(a1.az)) + a2.az;
-      oar = R(float(ar1)) + ar2;
+      oar = R(float(R(float(a1.ar)) + a2.ar));
+      if (o.tddt && !a1.delta_off && !a2.delta_off)
+        oar = oar + R(float(R(float(a1.delta)) + a2.delta));
       ovisc = std::max(a1.visc, a2.visc);
       for (int k = 0; k < 6; k++) os[k] = a1.s[k] + a2.s[k];
       for (int k = 0; k < 9; k++) og[k] = a1.g[k] + a2.g[k];

@@ -7,7 +7,8 @@
 // parity checker and, timed on the host cores, as the CPU baseline.
 //
 // Pair semantics follow the GPU path (SEM_GPU, FP32) unless the simulation is
-// created with prec=1 (FP64 yardstick with FP64 distances).
+// created with prec=1 (FP64 yardstick with FP64 distances) or prec=2 (FP32
+// with the reference CPU build's semantics, SEM_CPU).
 #include <omp.h>
 #include <algorithm>
 #include <cfloat>
@@ -263,7 +264,9 @@ struct Sim {
     }
     VelMax = double(std::sqrt(v2));
     const orc_div d = divdata();
-    orc_interaction(prec, prec ? 2 : 0, TVisco, TDDT, ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
+    // prec 0: FP32 GPU semantics, 1: FP64 yardstick, 2: FP32 CPU semantics.
+    orc_interaction(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT, ViscoB,
+                    Visco, &c, &d, Np, Npb, NpbOk,
                     dcell.data(), poscell.data(), pos.data(), velrho.data(),
                     TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(), viscdt.data(),
                     TVisco == 2 ? strain.data() : nullptr, nullptr);

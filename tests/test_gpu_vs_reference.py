@@ -1,0 +1,28 @@
+"""The HIP path against the reference CPU build on identical initial
+particles: the whole step (divide, interaction, integration, dt) for the
+first 10 steps, from the reference-run fixtures of tests/golden/ (bounds in
+tests/helpers.REF_TOL, the FP32 rounding spread between the reference and
+the oracle)."""
+import numpy as np
+import pytest
+
+from tests.helpers import compare_with_reference, load_reference_fixture
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("name", ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar"])
+def test_gpu_steps_follow_reference_cpu_build(name):
+    from dualsphysics_amd.sim import SphGpuSingle
+    fx, case = load_reference_fixture(name)
+    sim = SphGpuSingle(case)
+    done = 0
+    for k, step in enumerate(fx["steps"]):
+        sim.run(int(step) - done, sync_dt=True)
+        done = int(step)
+        d = sim.download()
+        idx = np.argsort(d["idp"], kind="stable")
+        e = compare_with_reference(fx, k, sim.stats()["timestep"], case.pos, d["pos"][idx],
+                                   d["velrho"][idx, :3], d["velrho"][idx, 3])
+        print(name, int(step), e)
+    sim.close()

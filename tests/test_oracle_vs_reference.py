@@ -1,0 +1,50 @@
+"""The oracle against the reference CPU build on identical initial particles
+(fixtures written by tests/golden/make_reference_fixtures.py from
+oracle/_ref, the reference's own sources compiled with IEEE arithmetic).
+
+The oracle runs with the reference CPU build's pair semantics (prec=2: FP32,
+float(FP64 difference) distances, two passes), with the GPU semantics
+(prec=0) the HIP kernels follow, and in FP64 (prec=1); all three stay within
+FP32 rounding of the reference over the first 10 steps (bounds in
+tests/helpers.REF_TOL). Cases: Symplectic + artificial viscosity + DDT2Full
+(the benchmark configuration) and Verlet + laminar viscosity + DDT2.
+"""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleSim
+from tests.helpers import compare_with_reference, load_reference_fixture
+
+FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar"]
+
+
+@pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
+                                                 "fp64"])
+@pytest.mark.parametrize("name", FIXTURES)
+def test_oracle_follows_reference_cpu_build(name, prec):
+    fx, case = load_reference_fixture(name)
+    o = OracleSim(case, prec=prec)
+    done = 0
+    for k, step in enumerate(fx["steps"]):
+        o.run(int(step) - done)
+        done = int(step)
+        d = o.download()
+        idx = np.argsort(d["idp"], kind="stable")
+        e = compare_with_reference(fx, k, o.stats()["timestep"], case.pos, d["pos"][idx],
+                                   d["velrho"][idx, :3], d["velrho"][idx, 3])
+        print(name, prec, int(step), e)
+
+
+def test_fixture_detects_a_planted_error():
+    """A deliberately wrong oracle input (gravity 1% off) leaves the bounds:
+    the fixture comparison is sensitive to O(1e-2) physics errors."""
+    import dataclasses
+    fx, case = load_reference_fixture(FIXTURES[0])
+    bad = dataclasses.replace(case, gravity=(0.0, 0.0, -9.81 * 1.01))
+    o = OracleSim(bad, prec=1)
+    o.run(int(fx["steps"][-1]))
+    d = o.download()
+    idx = np.argsort(d["idp"], kind="stable")
+    with pytest.raises(AssertionError):
+        compare_with_reference(fx, len(fx["steps"]) - 1, o.stats()["timestep"], case.pos,
+                               d["pos"][idx], d["velrho"][idx, :3], d["velrho"][idx, 3])
