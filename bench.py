@@ -83,7 +83,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
     ap.add_argument("--forces-kernel", choices=["tile", "rows"], default="tile")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a device-to-device copy of BYTES (3 times): the known-size "
+                         "kernel that calibrates the HBM byte counters of a profiler")
     a = ap.parse_args()
+    if a.copy_probe:
+        import torch
+        n = a.copy_probe // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
+                          "launches": 3}), flush=True)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs the SPH step on one GPU only; "
                           "multi-GPU sharding is a design note (x-slab halo exchange), "

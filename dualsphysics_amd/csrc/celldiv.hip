@@ -6,6 +6,7 @@
 // 436-489, 533-712) with the reference's -stable ordering: equal keys keep
 // their current array order, so the permutation equals the CPU counting sort
 // (JCellDivCpuSingle.cpp:207-262).
+#include <algorithm>
 #include <climits>
 #include <cstring>
 #include "dsph_ctx.h"
@@ -30,15 +31,16 @@ __global__ void __launch_bounds__(kBs) k_limits(unsigned n, unsigned pini, unsig
                                                 const typecode* __restrict__ code,
                                                 unsigned* __restrict__ res) {
   __shared__ unsigned smin[3][kBs / 64], smax[3][kBs / 64];
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned mn[3] = {UINT_MAX, UINT_MAX, UINT_MAX}, mx[3] = {0u, 0u, 0u};
-  if (p < n) {
+  // Grid-stride: a bounded grid keeps the number of global atomics small.
+  for (unsigned p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
     const unsigned pp = p + pini;
     if (code_special(code[pp]) < kCodeOutIgnore) {
       const unsigned c = dcell[pp];
       const unsigned cx = dcel_cellx(cellcode, c), cy = dcel_celly(cellcode, c),
                      cz = dcel_cellz(cellcode, c);
-      mn[0] = mx[0] = cx; mn[1] = mx[1] = cy; mn[2] = mx[2] = cz;
+      mn[0] = min(mn[0], cx); mn[1] = min(mn[1], cy); mn[2] = min(mn[2], cz);
+      mx[0] = max(mx[0], cx); mx[1] = max(mx[1], cy); mx[2] = max(mx[2], cz);
     }
   }
   for (int k = 0; k < 3; k++) {
@@ -362,11 +364,14 @@ int dsph_cell_limits_sync(dsph_ctx* ctx, uint32_t np, uint32_t pini, uint32_t ce
     return DSPH_OK;
   }
   DSPH_REQUIRE(ctx, d_dcell && d_code, "dsph_cell_limits_sync: null array");
-  KTimer kt(ctx, "k_limits");
   unsigned* res = ctx->small_dev;
-  k_limits_init<<<1, 64, 0, ctx->stream>>>(res);
-  k_limits<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(np, pini, cellcode, d_dcell, d_code, res);
-  DSPH_LAUNCHED(ctx, "k_limits");
+  {
+    KTimer kt(ctx, "k_limits");
+    k_limits_init<<<1, 64, 0, ctx->stream>>>(res);
+    k_limits<<<std::min(grid_for(np, kBs), 2048u), kBs, 0, ctx->stream>>>(np, pini, cellcode,
+                                                                         d_dcell, d_code, res);
+    DSPH_LAUNCHED(ctx, "k_limits");
+  }
   unsigned* h = static_cast<unsigned*>(ctx->pinned);
   DSPH_HIP(ctx, hipMemcpyAsync(h, res, 6 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
   DSPH_HIP(ctx, hipStreamSynchronize(ctx->stream));

@@ -50,6 +50,17 @@ __global__ void __launch_bounds__(kBs) k_prep(unsigned n, const float4* __restri
   }
 }
 
+// XCD-aware block order. The dispatcher sends block i to XCD i % 8; each
+// XCD has its own L2. Renumbering so that XCD x runs one contiguous range of
+// logical blocks keeps neighbouring particles (which read the same neighbour
+// data) on the same L2.
+constexpr unsigned kNumXcd = 8;
+__device__ inline unsigned xcd_block(unsigned i, unsigned nblocks) {
+  const unsigned per = nblocks / kNumXcd, rem = nblocks % kNumXcd;
+  const unsigned x = i % kNumXcd, k = i / kNumXcd;
+  return x * per + min(x, rem) + k;
+}
+
 struct Acc {
   float ax, ay, az, ar, visc, delta;
   float s[6];   // two_strain {xx,xy,xz,yy,yz,zz}
@@ -222,7 +233,7 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
     const float4* __restrict__ velrho, const float2* __restrict__ press,
     const float* __restrict__ tau, float* __restrict__ viscdt, float* __restrict__ ar,
     float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad) {
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   const unsigned p1 = p + pinit;
   Acc a;
@@ -248,7 +259,7 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
 constexpr unsigned kTile = 64;   // candidates staged in LDS per tile (one bit each)
 constexpr unsigned kMinBatch = 16;
 #ifndef DSPH_RING_TILES
-#define DSPH_RING_TILES 32
+#define DSPH_RING_TILES 16
 #endif
 // Tiles of hit masks kept per wave (8 bytes per lane and tile).
 constexpr unsigned kRingTiles = DSPH_RING_TILES;
@@ -306,7 +317,7 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
   __shared__ unsigned long long masks[kRing][64];
   __shared__ unsigned tbase[kRing];
   const unsigned lane = threadIdx.x;
-  const unsigned p = blockIdx.x * 64 + lane;
+  const unsigned p = xcd_block(blockIdx.x, gridDim.x) * 64 + lane;
   const bool valid = p < n;
   const unsigned p1 = (valid ? p : 0) + pinit;
   Acc a;
@@ -469,7 +480,7 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
 __global__ void __launch_bounds__(kBsForces) k_forces_bound(unsigned n, DivData dvd, Cte c,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar) {
-  const unsigned p1 = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned p1 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p1 >= n) return;
   float visc = 0, arp1 = 0;
   const float4 ps1 = poscell[p1];

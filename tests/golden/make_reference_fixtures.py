@@ -13,8 +13,11 @@ Fixture ref_dambreak_dp0.03.npz (data only, no code of the reference):
   vel       fluid velocity [k, nf, 3]; rho: density of all particles [k, np].
 Particles are in id order.
 
+ref_observables_dp0.01.npz (--observables, ~1 h on 8 cores): front position
+and probe pressure every 0.01 s over 1 s of the dp=0.01 dam break.
+
 Run in the development container (the reference is not available on the
-GPU hosts):  python tests/golden/make_reference_fixtures.py
+GPU hosts):  python tests/golden/make_reference_fixtures.py [--observables]
 """
 from __future__ import annotations
 
@@ -53,9 +56,40 @@ def run_reference(case, nsteps: int, workdir: Path) -> list[dict]:
             for k in range(nsteps + 1)]
 
 
+def observables_fixture(dp: float = 0.01, tmax: float = 1.0, tout: float = 0.01):
+    """ref_observables_dp<dp>.npz: front position and probe pressure every
+    ``tout`` s over ``tmax`` s of the dam break (cfg0 at dp=0.01), from the
+    reference's PART files with dualsphysics_amd.observables."""
+    from dualsphysics_amd import observables as obs
+    case = cm.dam_break(dp=dp)
+    with tempfile.TemporaryDirectory() as td:
+        w = Path(td)
+        bi4.write_case(case, w / "in", "case", timemax=tmax, timeout=tout)
+        cmd = [str(REF_EXE), str(w / "in" / "case"), str(w / "out"), "-cpu", "-ompthreads:8",
+               "-saveposdouble:1", "-stable:1", "-svres:0"]
+        r = subprocess.run(cmd, cwd=w, capture_output=True, text=True, timeout=6 * 3600)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout[-4000:] + r.stderr[-2000:])
+        files = sorted((w / "out" / "data").glob("Part_*.bi4"))
+        times, front, press = [], [], []
+        for f in files:
+            pt = bi4.read_part(f)
+            fluid = pt["idp"] >= case.nbound
+            o = obs.sample(case, pt["pos"], pt["rho"], fluid)
+            times.append(pt["time"]); front.append(o["front"]); press.append(o["pressure"])
+    name = f"ref_observables_dp{dp:g}"
+    np.savez_compressed(OUT / f"{name}.npz", dp=dp, seed=42, tmax=tmax, tout=tout,
+                        time=np.array(times), front=np.array(front), pressure=np.array(press),
+                        probes=obs.dam_break_probes(dp=dp))
+    print(f"wrote {name}.npz ({len(times)} samples)")
+
+
 def main():
     if not REF_EXE.exists():
         sys.exit(f"{REF_EXE} missing: make -f oracle/ref.mk (needs /root/reference)")
+    if "--observables" in sys.argv:
+        observables_fixture()
+        return
     for name, (dp, nsteps, saved, over) in CASES.items():
         params = cm.Parameters(**over)
         case = cm.dam_break(dp=dp, params=params)
