@@ -152,10 +152,17 @@ def main():
         flops = ff * F_FF + fb * F_FB
         avg_s = ms / cnt / 1e3
         ach = flops / avg_s / 1e12
+        traffic = None
+        tfile = ROOT / "profiles" / f"k_forces_fluid_traffic_dam_break_dp{a.dp:g}.json"
+        if tfile.exists() and a.forces_kernel == "tile":
+            traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_launch")
         roof = {"bound": "valu", "kernel": "k_forces_fluid", "achieved": round(ach, 3),
                 "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
-                "traffic": None, "avg_launch_ms": round(ms / cnt, 4), "launches": cnt,
-                "flop_per_launch": flops}
+                "traffic": traffic, "avg_launch_ms": round(ms / cnt, 4), "launches": cnt,
+                "flop_per_launch": flops,
+                "note": "VALU-bound FP32 kernel; peak = MI355X FP32 vector peak (packed FP32); "
+                        "traffic = rocprofv3 FETCH_SIZE/WRITE_SIZE per launch scaled by a 1 GiB "
+                        "copy calibration (profiles/)"}
     cpu = None
     if not a.no_cpu_baseline:
         del sim
