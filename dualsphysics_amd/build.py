@@ -34,6 +34,9 @@ HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-fPIC", "-std=c++17",
     # Packed FP32 instructions off for every HIP translation unit.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Contract a*b+c into FMA only within one source expression, so two kernels
+    # that evaluate the same pair expressions produce the same bits.
+    "-ffp-contract=on",
     "-Wno-unused-result",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]

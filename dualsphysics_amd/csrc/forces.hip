@@ -258,8 +258,12 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
 // ------------------------------------------------------------ cell-row tile kernel
 constexpr unsigned kTile = 64;   // candidates staged in LDS per tile (one bit each)
 constexpr unsigned kMinBatch = 16;
+#ifndef DSPH_SCAN_UNROLL
+#define DSPH_SCAN_UNROLL 8
+#endif
+constexpr unsigned kScanUnroll = DSPH_SCAN_UNROLL;  // staged candidates tested per loop trip
 #ifndef DSPH_RING_TILES
-#define DSPH_RING_TILES 16
+#define DSPH_RING_TILES 12
 #endif
 // Tiles of hit masks kept per wave (8 bytes per lane and tile).
 constexpr unsigned kRingTiles = DSPH_RING_TILES;
@@ -358,30 +362,35 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
     }
   };
   // k wave-uniform iterations; each lane pops up to k of its queued pairs.
+  // Pops the next queued pair and issues its loads (32-bit offsets from the
+  // array bases).
+  auto issue = [&](unsigned& j, float4& ps2, float4& vr2, float2& pr2) {
+    pop(j);
+    const unsigned jj = j & 0x7fffffffu;
+    ps2 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(poscell) + (jj << 4));
+    vr2 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(velrho) + (jj << 4));
+    pr2 = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(press) + (jj << 3));
+  };
+  auto eval = [&](unsigned j, const float4& ps2, const float4& vr2, const float2& pr2) {
+    pair_eval<TVISCO, TDDT, VELGRAD>((j >> 31) != 0, j & 0x7fffffffu, ps2, vr2, pr2, q, tau,
+                                     viscob, viscof, c, a);
+  };
+  // k wave-uniform iterations; each lane pops up to k of its queued pairs.
+  // Two register sets alternate so the loads of the next pair are in flight
+  // during the arithmetic of the current one.
   auto process = [&](unsigned k) {
-    unsigned jn = 0;
-    float4 psn = make_float4(0, 0, 0, 0), vrn = psn;
-    float2 prn = make_float2(0, 0);
-    bool hn = len != 0;
-    if (hn) {
-      pop(jn);
-      const unsigned jj = jn & 0x7fffffffu;
-      psn = poscell[jj]; vrn = velrho[jj]; prn = press[jj];
-    }
-    for (unsigned i = 0; i < k; i++) {
-      const bool h = hn;
-      const unsigned j = jn;
-      const float4 ps2 = psn, vr2 = vrn;
-      const float2 pr2 = prn;
-      hn = (i + 1 < k) && len != 0;
-      if (hn) {
-        pop(jn);
-        const unsigned jj = jn & 0x7fffffffu;
-        psn = poscell[jj]; vrn = velrho[jj]; prn = press[jj];
-      }
-      if (h)
-        pair_eval<TVISCO, TDDT, VELGRAD>((j >> 31) != 0, j & 0x7fffffffu, ps2, vr2, pr2, q, tau,
-                                         viscob, viscof, c, a);
+    unsigned ja = 0, jb = 0;
+    float4 psa = make_float4(0, 0, 0, 0), vra = psa, psb = psa, vrb = psa;
+    float2 pra = make_float2(0, 0), prb = pra;
+    bool ha = k != 0 && len != 0;
+    if (ha) issue(ja, psa, vra, pra);
+    for (unsigned i = 0; i < k; i += 2) {
+      const bool hb = i + 1 < k && len != 0;
+      if (hb) issue(jb, psb, vrb, prb);
+      if (ha) eval(ja, psa, vra, pra);
+      ha = i + 2 < k && len != 0;
+      if (ha) issue(ja, psa, vra, pra);
+      if (hb) eval(jb, psb, vrb, prb);
     }
   };
 
@@ -441,17 +450,17 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
               const unsigned long long win = k1 > k0 ? bit_range(k0, k1) : 0ull;
               if (win) {
 #pragma unroll 1
-                for (unsigned k0 = 0; k0 < kTile; k0 += 16) {
-                  unsigned b16 = 0;
+                for (unsigned k0 = 0; k0 < kTile; k0 += kScanUnroll) {
+                  unsigned bits = 0;
 #pragma unroll
-                  for (unsigned k = 0; k < 16; k++) {
+                  for (unsigned k = 0; k < kScanUnroll; k++) {
                     const float4 f = tile[k0 + k];
                     float d = fmaf(m2x, f.x, f.w);
                     d = fmaf(m2y, f.y, d);
                     d = fmaf(m2z, f.z, d);
-                    b16 |= (d <= thr) ? (1u << k) : 0u;
+                    bits |= (d <= thr) ? (1u << k) : 0u;
                   }
-                  hm |= (unsigned long long)b16 << k0;
+                  hm |= (unsigned long long)bits << k0;
                 }
                 hm &= win;
               }
