@@ -54,6 +54,12 @@ class DsphInterParms(C.Structure):
                 ("d_velgrad", C.c_void_p)]
 
 
+class DsphMotion(C.Structure):
+    _fields_ = [("type", C.c_int32), ("idbegin", C.c_uint32), ("count", C.c_uint32),
+                ("freq", C.c_double * 3), ("ampl", C.c_double * 3), ("phase", C.c_double * 3),
+                ("start", C.c_double), ("finish", C.c_double)]
+
+
 class DsphCase(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "dp", "kernelh", "cteb", "gamma", "rhopzero", "cs0", "massfluid", "massbound")] + [
@@ -65,7 +71,7 @@ class DsphCase(C.Structure):
         ("cellmode", C.c_int32), ("celldomfixed", C.c_int32),
         ("maprealposmin", C.c_double * 3), ("maprealposmax", C.c_double * 3)] + [
         (n, C.c_uint32) for n in ("casenp", "casenbound", "casenfixed", "casenmoving",
-                                  "casenfloat", "casenfluid")]
+                                  "casenfloat", "casenfluid")] + [("motion", DsphMotion)]
 
 
 class DsphStats(C.Structure):
@@ -123,6 +129,8 @@ SIGNATURES = {
                          _vp, _vp, _vp, _vp],
     "dsph_step_pos2": [_vp, _u, _u] + [_vp] * 8,
     "dsph_step_pos": [_vp, _u, _u] + [_vp] * 6,
+    "dsph_move_lin_bound": [_vp, _u, _u, _u, C.POINTER(_d), C.POINTER(_f)] + [_vp] * 6,
+    "dsph_move_motion_device": [_vp, _u, C.POINTER(DsphMotion), _d] + [_vp] * 7,
     "dsph_pips_count_sync": [_vp, _u, _u, _u, _u, C.POINTER(DsphDivData), _vp, _vp,
                              C.POINTER(C.c_uint64)],
     "dsph_pips_count_split_sync": [_vp, _u, _u, _u, _u, C.POINTER(DsphDivData), _vp, _vp,

@@ -91,6 +91,8 @@ class Case:
     maprealposmin: tuple
     maprealposmax: tuple
     boundnor: np.ndarray | None = None   # float32 [nbound,3] (mDBC), not doubled
+    nmoving: int = 0                     # moving boundary ids [nbound-nmoving, nbound)
+    motion: dict | None = None           # rectilinear sinusoid: freq, ampl, phase (3 each), start, finish
 
     @property
     def np(self) -> int:
@@ -112,8 +114,18 @@ class Case:
         c.visco, c.viscoboundfactor, c.ddtvalue = p.visco, p.viscoboundfactor, p.ddtvalue
         c.rhopoutmin, c.rhopoutmax = p.rhopoutmin, p.rhopoutmax
         c.cellmode, c.celldomfixed = p.cellmode, p.celldomfixed
-        c.casenp, c.casenbound, c.casenfixed = self.np, self.nbound, self.nbound
-        c.casenmoving, c.casenfloat, c.casenfluid = 0, 0, self.nfluid
+        c.casenp, c.casenbound = self.np, self.nbound
+        c.casenfixed = self.nbound - self.nmoving
+        c.casenmoving, c.casenfloat, c.casenfluid = self.nmoving, 0, self.nfluid
+        if self.nmoving and self.motion:
+            m = self.motion
+            c.motion.type = 1
+            c.motion.idbegin, c.motion.count = self.nbound - self.nmoving, self.nmoving
+            for k in range(3):
+                c.motion.freq[k] = m["freq"][k]
+                c.motion.ampl[k] = m["ampl"][k]
+                c.motion.phase[k] = m["phase"][k]
+            c.motion.start, c.motion.finish = m.get("start", 0.0), m.get("finish", 1e30)
         return c
 
 
@@ -183,3 +195,57 @@ def dam_break_cfg0(**kw) -> Case:
 def dam_break_cfg1(**kw) -> Case:
     """3-D dam break at dp=0.0025 m: 5,145,600 fluid + 934,016 wall particles."""
     return dam_break(dp=0.0025, **kw)
+
+
+def flume(dp: float = 0.01, length: float = 0.6, width: float = 0.1, height: float = 0.3,
+          depth: float = 0.15, ampl: float = 0.02, period: float = 0.8, phase: float = 0.0,
+          layers: int = 2, coefh: float = 1.2, coefsound: float = 20.0, rhop0: float = 1000.0,
+          gamma: float = 7.0, g: float = 9.81, seed: int = 42, jitter: float = 1e-4,
+          params: Parameters | None = None, name: str | None = None) -> Case:
+    """Wave flume with a piston wavemaker (the flume configuration scaled by
+    its arguments): fluid ``[0,length]x[0,width]x[0,depth]``; fixed walls
+    (bottom, two sides, far end, ``layers`` lattice layers, open top) extended
+    behind the piston's stroke; a moving wall of ``layers`` layers at x<0 over
+    the tank height driven by x(t) = ampl sin(2 pi t/period + phase).
+    Ids: fixed walls, then the piston, then the fluid. ``boundnor`` (mDBC
+    normals) is the vector from each wall particle to the nearest point of
+    the fluid region (the piston face for piston particles)."""
+    params = params or Parameters()
+    nx, ny, nz = _n(length, dp), _n(width, dp), _n(height, dp)
+    nback = int(math.ceil(ampl / dp)) + layers      # stroke behind x=0 plus the piston
+    # Fixed walls: lattice of the widened box minus the interior (x from -nback).
+    wpts, widx = _lattice(-nback - layers, nx + layers, -layers, ny + layers, -layers, nz, dp)
+    inside = ((widx[:, 0] >= -nback - layers) & (widx[:, 0] < nx) & (widx[:, 1] >= 0) &
+              (widx[:, 1] < ny) & (widx[:, 2] >= 0))
+    inside &= widx[:, 0] >= -nback                      # x-end wall behind the stroke stays
+    fixed = wpts[~inside]
+    ppts, _ = _lattice(-layers, 0, 0, ny, 0, nz, dp)    # piston layers
+    fpts, _ = _lattice(0, nx, 0, ny, 0, _n(depth, dp), dp)
+    nfix, nmov, nf = fixed.shape[0], ppts.shape[0], fpts.shape[0]
+    pos = np.concatenate([fixed, ppts, fpts], 0)
+    u = uniform01(seed, 3 * pos.shape[0]).reshape(-1, 3)
+    pos = pos + (2.0 * u - 1.0) * (jitter * dp)
+    h = coefh * math.sqrt(3.0 * dp * dp)
+    cs0 = coefsound * math.sqrt(g * depth)
+    b = cs0 * cs0 * rhop0 / gamma
+    mass = rhop0 * dp ** 3
+    velrho = np.zeros((pos.shape[0], 4), np.float32)
+    nb = nfix + nmov
+    velrho[:nb, 3] = rhop0
+    zf = pos[nb:, 2]
+    velrho[nb:, 3] = (rhop0 * (1.0 + rhop0 * g * (depth - zf) / b) ** (1.0 / gamma)).astype(np.float32)
+    # Normals: to the nearest point of the fluid region [xlo,length]x[0,width]x[0,inf).
+    bp = pos[:nb]
+    xlo = np.where(np.arange(nb) >= nfix, 0.0, -nback * dp)
+    q = np.stack([np.clip(bp[:, 0], xlo, length), np.clip(bp[:, 1], 0.0, width),
+                  np.maximum(bp[:, 2], 0.0)], 1)
+    boundnor = (q - bp).astype(np.float32)
+    idp = np.arange(pos.shape[0], dtype=np.uint32)
+    mn, mx = map_limits(pos, h)
+    motion = {"freq": (1.0 / period, 0.0, 0.0), "ampl": (ampl, 0.0, 0.0),
+              "phase": (phase, 0.0, 0.0), "start": 0.0, "finish": 1e30}
+    return Case(name=name or f"flume_dp{dp:g}", dp=dp, h=h, rhop0=rhop0, gamma=gamma, cs0=cs0,
+                b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g), hswl=depth,
+                params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb, nfluid=nf,
+                maprealposmin=mn, maprealposmax=mx, boundnor=boundnor, nmoving=nmov,
+                motion=motion)

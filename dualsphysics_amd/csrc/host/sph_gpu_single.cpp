@@ -58,6 +58,10 @@ struct DevArr {
 
 }  // namespace
 
+namespace dsph {
+void rect_sinu_mov_host(const dsph_motion* m, double t, double dt, double mv[3]);
+}
+
 struct dsph_sim {
   dsph_ctx* ctx = nullptr;
   dsph_case cs{};
@@ -242,7 +246,9 @@ struct dsph_sim {
     for (unsigned p = 0; p < np; p++) {
       const unsigned id = hidp[p];
       // LoadCodeParticles: fixed boundary (mk 0) and fluid (mk 0).
-      hcode[p] = uint16_t(id < CaseNbound ? 0x0000 : 0x1800);
+      // Fixed boundary 0x0000, moving boundary CODE_TYPE_MOVING 0x0800 (mk 1),
+      // fluid 0x1800.
+      hcode[p] = uint16_t(id < cs.casenfixed ? 0x0000 : (id < CaseNbound ? 0x0801 : 0x1800));
       const double x = hpos[3 * size_t(p)], y = hpos[3 * size_t(p) + 1], z = hpos[3 * size_t(p) + 2];
       hxy[2 * size_t(p)] = x; hxy[2 * size_t(p) + 1] = y; hz[p] = z;
       // LoadDcellParticles (JSph.cpp:1794-1816).
@@ -552,6 +558,29 @@ struct dsph_sim {
     pre_active = false;
   }
 
+  // ---- JSphGpu::RunMotion (JSphGpu.cpp:1173-1200) for one rectilinear
+  // sinusoid: after the step, the moving boundary is displaced over
+  // [t, t+dt] and takes the velocity displacement/dt; the next divide is
+  // full (BoundChanged).
+  void run_motion(double dt) {
+    const dsph_motion& m = cs.motion;
+    if (!cs.casenmoving || m.type == 0) return;
+    if (device_dt) {
+      chk(dsph_move_motion_device(ctx, Npb, &m, TimeStep, d_dt, idp, posxy, posz, dcell, velrho,
+                                  code),
+          "motion");
+    } else {
+      if (!(TimeStep >= m.start && TimeStep < m.finish)) return;
+      double mv[3];
+      dsph::rect_sinu_mov_host(&m, TimeStep, dt, mv);
+      const float vel[3] = {float(mv[0] / dt), float(mv[1] / dt), float(mv[2] / dt)};
+      chk(dsph_move_lin_bound(ctx, Npb, m.idbegin, m.count, mv, vel, idp, posxy, posz, dcell,
+                              velrho, code),
+          "motion");
+    }
+    BoundChanged = true;
+  }
+
   // ---- JSphGpuSingle::ComputeStep_Sym (JSphGpuSingle.cpp:785-821)
   void step_sym() {
     const double dt = SymplecticDtPre;
@@ -566,6 +595,7 @@ struct dsph_sim {
     interaction();
     const double dt_c = dt_variable(1);
     symplectic_cor(dt);
+    run_motion(dt);
     if (device_dt) {
       chk(dsph_dt_variable_min_advance(), "dt advance");
     } else {
@@ -596,6 +626,7 @@ struct dsph_sim {
     }
     chk(dsph_step_pos(ctx, Np, Npb, movxy, movz, posxy, posz, dcell, code), "pos");
     swp(velrho, velrhom1);
+    run_motion(dt);
     TimeStep += dt;
     DtLast = dt;
   }

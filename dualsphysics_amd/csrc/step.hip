@@ -250,6 +250,46 @@ __device__ inline void update_pos(double2 rxy, double rz, double mx, double my, 
   }
 }
 
+// ---------------------------------------------------------------- moving boundaries
+// Displacement of a rectilinear sinusoid over [t, t+dt] (JMotionObj.cpp:558-575).
+__host__ __device__ inline void rect_sinu_mov(const dsph_motion& m, double t, double dt,
+                                              double mv[3]) {
+  const double twopi = 6.283185307179586476925286766559;
+  for (int k = 0; k < 3; k++) {
+    mv[k] = 0;
+    if (m.ampl[k] != 0)
+      mv[k] = m.ampl[k] * sin(m.phase[k] + twopi * m.freq[k] * (t + dt)) -
+              m.ampl[k] * sin(m.phase[k] + twopi * m.freq[k] * t);
+  }
+}
+
+// KerMoveLinBound (JSphGpu_ker.cu:1671-1685): boundary particles of the moving
+// object (selected by id) are displaced with KerUpdatePos and take the
+// imposed velocity. With a device dt slot array the displacement of the step
+// is computed here from the motion law.
+__global__ void __launch_bounds__(kBs) k_move_lin_bound(unsigned npb, unsigned idbegin,
+    unsigned count, PosCte c, double3 mvh, float3 velh, dsph_motion m, double t0,
+    const double* __restrict__ d_dt, const unsigned* __restrict__ idp,
+    double2* __restrict__ posxy, double* __restrict__ posz, unsigned* __restrict__ dcell,
+    float4* __restrict__ velrho, typecode* __restrict__ code) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npb) return;
+  if (idp[p] - idbegin >= count) return;
+  double3 mv = mvh;
+  float3 vel = velh;
+  if (d_dt) {
+    const double dt = d_dt[2], t = t0 + d_dt[3];
+    if (!(t >= m.start && t < m.finish) || dt <= 0) return;
+    double d[3];
+    rect_sinu_mov(m, t, dt, d);
+    mv = make_double3(d[0], d[1], d[2]);
+    vel = make_float3(float(d[0] / dt), float(d[1] / dt), float(d[2] / dt));
+  }
+  update_pos(posxy[p], posz[p], mv.x, mv.y, mv.z, false, p, c, posxy, posz, dcell, code);
+  const float4 vr = velrho[p];
+  velrho[p] = make_float4(vel.x, vel.y, vel.z, vr.w);
+}
+
 __global__ void __launch_bounds__(kBs) k_step_pos2(unsigned n, unsigned pini, PosCte c,
     const double2* __restrict__ posxypre, const double* __restrict__ poszpre,
     const double2* __restrict__ movxy, const double* __restrict__ movz,
@@ -420,4 +460,51 @@ int dsph_step_pos(dsph_ctx* ctx, uint32_t np, uint32_t npb, const double* d_movx
   return DSPH_OK;
 }
 
+int dsph_move_lin_bound(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin, uint32_t count,
+                        const double mvpos[3], const float mvvel[3], const uint32_t* d_idp,
+                        double* d_posxy, double* d_posz, uint32_t* d_dcell, float* d_velrho,
+                        uint16_t* d_code) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  if (!npb || !count) return DSPH_OK;
+  DSPH_REQUIRE(ctx, mvpos && mvvel && d_idp && d_posxy && d_posz && d_dcell && d_velrho && d_code,
+               "dsph_move_lin_bound: null argument");
+  dsph_motion m = {};
+  KTimer kt(ctx, "k_move_lin_bound");
+  k_move_lin_bound<<<grid_for(npb, kBs), kBs, 0, ctx->stream>>>(
+      npb, idbegin, count, make_posc(ctx->cte), make_double3(mvpos[0], mvpos[1], mvpos[2]),
+      make_float3(mvvel[0], mvvel[1], mvvel[2]), m, 0.0, nullptr, d_idp, (double2*)d_posxy,
+      d_posz, d_dcell, (float4*)d_velrho, d_code);
+  DSPH_LAUNCHED(ctx, "k_move_lin_bound");
+  return DSPH_OK;
+}
+
+int dsph_move_motion_device(dsph_ctx* ctx, uint32_t npb, const dsph_motion* m, double t0,
+                            const double* d_dt, const uint32_t* d_idp, double* d_posxy,
+                            double* d_posz, uint32_t* d_dcell, float* d_velrho,
+                            uint16_t* d_code) {
+  if (!ctx || !m) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  if (!npb || !m->count || m->type == 0) return DSPH_OK;
+  DSPH_REQUIRE(ctx, m->type == 1, "dsph_move_motion_device: only rectilinear sinusoids (type 1)");
+  DSPH_REQUIRE(ctx, d_dt && d_idp && d_posxy && d_posz && d_dcell && d_velrho && d_code,
+               "dsph_move_motion_device: null argument");
+  KTimer kt(ctx, "k_move_lin_bound");
+  k_move_lin_bound<<<grid_for(npb, kBs), kBs, 0, ctx->stream>>>(
+      npb, m->idbegin, m->count, make_posc(ctx->cte), make_double3(0, 0, 0),
+      make_float3(0, 0, 0), *m, t0, d_dt, d_idp, (double2*)d_posxy, d_posz, d_dcell,
+      (float4*)d_velrho, d_code);
+  DSPH_LAUNCHED(ctx, "k_move_lin_bound");
+  return DSPH_OK;
+}
+
 }  // extern "C"
+
+// Host-side displacement of the same law (the reference's path: motion on the
+// host, JDsMotion::ProcesTime), for the host driver.
+namespace dsph {
+__attribute__((visibility("hidden"))) void rect_sinu_mov_host(const dsph_motion* m, double t,
+                                                               double dt, double mv[3]) {
+  rect_sinu_mov(*m, t, dt, mv);
+}
+}  // namespace dsph

@@ -73,6 +73,22 @@ class SphGpuSingle:
         device and the step has no host round trip besides the divide's counts."""
         self._check(self.lib.dsph_sim_run(self._h, nsteps, 1 if sync_dt else 0))
 
+    def run_until(self, t: float, max_steps: int = 10_000_000) -> int:
+        """Steps until the simulated time reaches ``t`` (the step that crosses
+        ``t`` included, as the reference's output times); returns the number
+        of steps run. Runs batches sized from the last dt, then single steps."""
+        n = 0
+        while n < max_steps:
+            st = self.stats()
+            if st["timestep"] >= t:
+                break
+            dt = st["dt_last"] or 1e-9
+            k = int((t - st["timestep"]) / dt) - 2
+            k = max(1, min(k, max_steps - n))
+            self.run(k)
+            n += k
+        return n
+
     def stats(self) -> dict:
         st = N.DsphStats()
         self._check(self.lib.dsph_sim_stats(self._h, C.byref(st)))

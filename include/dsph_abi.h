@@ -357,6 +357,39 @@ int dsph_step_pos(dsph_ctx* ctx, uint32_t np, uint32_t npb,
                   double* d_posxy, double* d_posz, uint32_t* d_dcell,
                   uint16_t* d_code);
 
+/* Imposed motion of one moving boundary object: rectilinear sinusoidal
+ * motion (JMotionMovRectSinu, JMotionMov.h:237-254; displacement law
+ * JMotionObj.cpp:558-575): over [t, t+dt] each axis with a non-zero
+ * amplitude moves by ampl*(sin(phase+2*pi*freq*(t+dt)) - sin(phase+2*pi*freq*t)),
+ * velocity = displacement/dt (JMotionList.cpp:43-50); active for
+ * start <= t < finish. */
+typedef struct dsph_motion {
+  int32_t type;                  /* 0 none, 1 rectilinear sinusoid */
+  uint32_t idbegin, count;       /* particle ids of the object */
+  double freq[3], ampl[3], phase[3];  /* 1/s, m, rad */
+  double start, finish;          /* s */
+} dsph_motion;
+
+/* Replaces cusph::MoveLinBound (JSphGpu_ker.h:326): displaces the moving
+ * boundary particles with id in [idbegin, idbegin+count) by mvpos (FP64,
+ * KerUpdatePos rules) and sets their velocity to mvvel. The particles are
+ * found by scanning the boundary block [0,npb) by id instead of the
+ * reference's id->position map (RidpMot). */
+int dsph_move_lin_bound(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin,
+                        uint32_t count, const double mvpos[3],
+                        const float mvvel[3], const uint32_t* d_idp,
+                        double* d_posxy, double* d_posz, uint32_t* d_dcell,
+                        float* d_velrho, uint16_t* d_code);
+/* The same with the displacement of motion m computed on the device from
+ * the device time slots (dt = d_dt[2], t = t0 + d_dt[3], the layout of
+ * dsph_dt_variable): no host round trip per step. No counterpart in the
+ * reference (its motion is computed on the host every step). */
+int dsph_move_motion_device(dsph_ctx* ctx, uint32_t npb, const dsph_motion* m,
+                            double t0, const double* d_dt,
+                            const uint32_t* d_idp, double* d_posxy,
+                            double* d_posz, uint32_t* d_dcell, float* d_velrho,
+                            uint16_t* d_code);
+
 /* ------------------------------------------------------------ measurement */
 
 /* Replaces cusph::InteractionNg (JDsPips_ker.cu:54-124) + reductions: counts
@@ -400,6 +433,7 @@ typedef struct dsph_case {
   int32_t celldomfixed;          /* 0: adapt to particles (reference default) */
   double maprealposmin[3], maprealposmax[3];
   uint32_t casenp, casenbound, casenfixed, casenmoving, casenfloat, casenfluid;
+  dsph_motion motion;            /* the moving object (ids casenfixed.. +casenmoving) */
 } dsph_case;
 
 /* Per-step statistics, mirror of the Run.out/Run.csv key numbers. */

@@ -212,8 +212,10 @@ def write_case(case, directory, name: str, timemax: float = 1.0, timeout: float 
     for k in ("PeriXinc", "PeriYinc", "PeriZinc"):
         root.set(k, T_DOUBLE3, (0.0, 0.0, 0.0))
     root.set("AxisDiv", T_INT, 0)
-    root.set("CaseNp", T_ULLONG, n).set("CaseNfixed", T_ULLONG, nb)
-    root.set("CaseNmoving", T_ULLONG, 0).set("CaseNfloat", T_ULLONG, 0)
+    nmov = getattr(case, "nmoving", 0)
+    nfix = nb - nmov
+    root.set("CaseNp", T_ULLONG, n).set("CaseNfixed", T_ULLONG, nfix)
+    root.set("CaseNmoving", T_ULLONG, nmov).set("CaseNfloat", T_ULLONG, 0)
     root.set("CaseNfluid", T_ULLONG, nf)
     root.set("CasePosMin", T_DOUBLE3, tuple(map(float, pmin)))
     root.set("CasePosMax", T_DOUBLE3, tuple(map(float, pmax)))
@@ -231,6 +233,10 @@ def write_case(case, directory, name: str, timemax: float = 1.0, timeout: float 
     part.array("Posd", T_DOUBLE3, case.pos)
     part.array("Vel", T_FLOAT3, case.velrho[:, :3])
     part.array("Rhop", T_FLOAT, case.velrho[:, 3])
+    if p.tboundary == 2:
+        if case.boundnor is None:
+            raise ValueError("mDBC needs boundary normals")
+        part.array("BoundNor", T_FLOAT3, case.boundnor)
     write_file(d / f"{name}.bi4", root)
 
     r = repr
@@ -245,6 +251,23 @@ def write_case(case, directory, name: str, timemax: float = 1.0, timeout: float 
         "RhopOutMin": p.rhopoutmin, "RhopOutMax": p.rhopoutmax,
         "Boundary": p.tboundary,
     }
+    moving, motion = "", ""
+    if nmov:
+        m = case.motion
+        moving = (f'            <moving mkbound="1" mk="12" begin="{nfix}" count="{nmov}" '
+                  f'refmotion="0" />\n')
+        v3 = lambda tag, x: f'<{tag} x="{r(x[0])}" y="{r(x[1])}" z="{r(x[2])}" />'
+        motion = f"""        <motion>
+            <objreal ref="0">
+                <begin mov="1" start="{r(m.get('start', 0.0))}" />
+                <mvrectsinu id="1" duration="{r(min(m.get('finish', 1e30), 1e6))}" anglesunits="radians">
+                    {v3('freq', m['freq'])}
+                    {v3('ampl', m['ampl'])}
+                    {v3('phase', m['phase'])}
+                </mvrectsinu>
+            </objreal>
+        </motion>
+"""
     plines = "\n".join(f'            <parameter key="{k}" value="{r(v)}" />'
                        for k, v in params.items())
     xml = f"""<?xml version="1.0" encoding="UTF-8" ?>
@@ -262,10 +285,11 @@ def write_case(case, directory, name: str, timemax: float = 1.0, timeout: float 
             <massbound value="{r(case.massbound)}" />
             <massfluid value="{r(case.massfluid)}" />
         </constants>
-        <particles np="{n}" nb="{nb}" nbf="{nb}" mkboundfirst="11" mkfluidfirst="1">
-            <fixed mkbound="0" mk="11" begin="0" count="{nb}" />
-            <fluid mkfluid="0" mk="1" begin="{nb}" count="{nf}" />
+        <particles np="{n}" nb="{nb}" nbf="{nfix}" mkboundfirst="11" mkfluidfirst="1">
+            <fixed mkbound="0" mk="11" begin="0" count="{nfix}" />
+{moving}            <fluid mkfluid="0" mk="1" begin="{nb}" count="{nf}" />
         </particles>
+{motion}
         <parameters>
 {plines}
             <simulationdomain>

@@ -477,6 +477,127 @@ void orc_interaction(int prec, int sem, int tvisco, int tddt, float viscob, floa
                         viscdt, strain, velgrad);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Determinant and inverse of a 4x4 matrix by cofactors (FP64).
+inline double inverse4(const double a[4][4], double inv[4][4]) {
+  double m[16], r[16];
+  for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) m[4 * i + j] = a[i][j];
+  r[0] = m[5] * m[10] * m[15] - m[5] * m[11] * m[14] - m[9] * m[6] * m[15] + m[9] * m[7] * m[14] + m[13] * m[6] * m[11] - m[13] * m[7] * m[10];
+  r[4] = -m[4] * m[10] * m[15] + m[4] * m[11] * m[14] + m[8] * m[6] * m[15] - m[8] * m[7] * m[14] - m[12] * m[6] * m[11] + m[12] * m[7] * m[10];
+  r[8] = m[4] * m[9] * m[15] - m[4] * m[11] * m[13] - m[8] * m[5] * m[15] + m[8] * m[7] * m[13] + m[12] * m[5] * m[11] - m[12] * m[7] * m[9];
+  r[12] = -m[4] * m[9] * m[14] + m[4] * m[10] * m[13] + m[8] * m[5] * m[14] - m[8] * m[6] * m[13] - m[12] * m[5] * m[10] + m[12] * m[6] * m[9];
+  r[1] = -m[1] * m[10] * m[15] + m[1] * m[11] * m[14] + m[9] * m[2] * m[15] - m[9] * m[3] * m[14] - m[13] * m[2] * m[11] + m[13] * m[3] * m[10];
+  r[5] = m[0] * m[10] * m[15] - m[0] * m[11] * m[14] - m[8] * m[2] * m[15] + m[8] * m[3] * m[14] + m[12] * m[2] * m[11] - m[12] * m[3] * m[10];
+  r[9] = -m[0] * m[9] * m[15] + m[0] * m[11] * m[13] + m[8] * m[1] * m[15] - m[8] * m[3] * m[13] - m[12] * m[1] * m[11] + m[12] * m[3] * m[9];
+  r[13] = m[0] * m[9] * m[14] - m[0] * m[10] * m[13] - m[8] * m[1] * m[14] + m[8] * m[2] * m[13] + m[12] * m[1] * m[10] - m[12] * m[2] * m[9];
+  r[2] = m[1] * m[6] * m[15] - m[1] * m[7] * m[14] - m[5] * m[2] * m[15] + m[5] * m[3] * m[14] + m[13] * m[2] * m[7] - m[13] * m[3] * m[6];
+  r[6] = -m[0] * m[6] * m[15] + m[0] * m[7] * m[14] + m[4] * m[2] * m[15] - m[4] * m[3] * m[14] - m[12] * m[2] * m[7] + m[12] * m[3] * m[6];
+  r[10] = m[0] * m[5] * m[15] - m[0] * m[7] * m[13] - m[4] * m[1] * m[15] + m[4] * m[3] * m[13] + m[12] * m[1] * m[7] - m[12] * m[3] * m[5];
+  r[14] = -m[0] * m[5] * m[14] + m[0] * m[6] * m[13] + m[4] * m[1] * m[14] - m[4] * m[2] * m[13] - m[12] * m[1] * m[6] + m[12] * m[2] * m[5];
+  r[3] = -m[1] * m[6] * m[11] + m[1] * m[7] * m[10] + m[5] * m[2] * m[11] - m[5] * m[3] * m[10] - m[9] * m[2] * m[7] + m[9] * m[3] * m[6];
+  r[7] = m[0] * m[6] * m[11] - m[0] * m[7] * m[10] - m[4] * m[2] * m[11] + m[4] * m[3] * m[10] + m[8] * m[2] * m[7] - m[8] * m[3] * m[6];
+  r[11] = -m[0] * m[5] * m[11] + m[0] * m[7] * m[9] + m[4] * m[1] * m[11] - m[4] * m[3] * m[9] - m[8] * m[1] * m[7] + m[8] * m[3] * m[5];
+  r[15] = m[0] * m[5] * m[10] - m[0] * m[6] * m[9] - m[4] * m[1] * m[10] + m[4] * m[2] * m[9] + m[8] * m[1] * m[6] - m[8] * m[2] * m[5];
+  const double det = m[0] * r[0] + m[1] * r[4] + m[2] * r[8] + m[3] * r[12];
+  if (det != 0)
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) inv[i][j] = r[4 * i + j] / det;
+  return det;
+}
+
+// mDBC density of boundary particles, SlipMode SLIP_Vel0: first-order
+// (4x4) corrected Shepard density and gradient at the ghost node
+// g = x_b + boundnor (normals already doubled), extrapolated back to the
+// particle. CPU build: JSphCpu_mdbc.cpp:36-146 (FP64 positions, matrix
+// accumulated in FP64); GPU build: KerInteractionMdbcCorrection_Fast,
+// JSphGpu_mdbc_iker.cu:34-150 (distances from the ghost node's PosCell, FP32
+// matrix). R = FP32 or FP64 (yardstick).
+template <typename R, typename M>
+void mdbc_correction(int sem, const orc_cte& c, const orc_div& d, unsigned n, const double* pos,
+                     const float* ps, const typecode* code, const float* boundnor,
+                     float* velrho) {
+  const R ks2 = R(c.kernelsize2), h = R(c.kernelh), awen = R(c.awen), bwenh = R(c.bwenh);
+  const R massf = R(c.massf);
+#pragma omp parallel for schedule(guided)
+  for (long pp = 0; pp < long(n); pp++) {
+    const unsigned p1 = unsigned(pp);
+    const float* bn = boundnor + 3 * size_t(p1);
+    if (bn[0] == 0 && bn[1] == 0 && bn[2] == 0) continue;
+    const double g[3] = {pos[3 * size_t(p1)] + bn[0], pos[3 * size_t(p1) + 1] + bn[1],
+                         pos[3 * size_t(p1) + 2] + bn[2]};
+    // PosCell of the ghost node (KerComputePosCell).
+    float gps[4];
+    {
+      const double dx = g[0] - c.mapposmin[0], dy = g[1] - c.mapposmin[1], dz = g[2] - c.mapposmin[2];
+      const double pcs = c.poscellsize;
+      const unsigned cx = unsigned(dx / pcs), cy = unsigned(dy / pcs), cz = unsigned(dz / pcs);
+      gps[0] = float(dx - pcs * cx); gps[1] = float(dy - pcs * cy); gps[2] = float(dz - pcs * cz);
+      gps[3] = bitsf((cx << 19) | (cy << 9) | cz);
+    }
+    const int cx = int((g[0] - c.domposmin[0]) / c.scell) - d.czx;
+    const int cy = int((g[1] - c.domposmin[1]) / c.scell) - d.czy;
+    const int cz = int((g[2] - c.domposmin[2]) / c.scell) - d.czz;
+    Row rows[25];
+    const int nr = rows_at(d, cx, cy, cz, true, rows);
+    R rhop1 = 0, gr[3] = {0, 0, 0};
+    M a[4][4];
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) a[i][j] = 0;
+    const unsigned ug = fbits(gps[3]);
+    const int gx = int(ug >> 19), gy = int((ug & 0x0007fe00u) >> 9), gz = int(ug & 0x1ffu);
+    for (int r = 0; r < nr; r++)
+      for (unsigned p2 = rows[r].a; p2 < rows[r].b; p2++) {
+        R dr[3];
+        if (sem == SEM_GPU) {
+          const float* b = ps + 4 * size_t(p2);
+          const unsigned ub = fbits(b[3]);
+          dr[0] = R(gps[0] - b[0] + c.poscellsize * float(gx - int(ub >> 19)));
+          dr[1] = R(gps[1] - b[1] + c.poscellsize * float(gy - int((ub & 0x0007fe00u) >> 9)));
+          dr[2] = R(gps[2] - b[2] + c.poscellsize * float(gz - int(ub & 0x1ffu)));
+        } else if (sem == SEM_CPU) {
+          for (int k = 0; k < 3; k++) dr[k] = R(float(g[k] - pos[3 * size_t(p2) + k]));
+        } else {
+          for (int k = 0; k < 3; k++) dr[k] = R(g[k] - pos[3 * size_t(p2) + k]);
+        }
+        const R rr2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        if (!(rr2 <= ks2 && (code[p2] & kTypeMask) == kTypeFluid)) continue;
+        const R wab = wend_wab<R>(rr2, h, awen);
+        const R fac = wend_fac<R>(rr2, h, bwenh);
+        const R fr[3] = {fac * dr[0], fac * dr[1], fac * dr[2]};
+        const R volp2 = massf / R(velrho[4 * size_t(p2) + 3]);
+        rhop1 += massf * wab;
+        for (int k = 0; k < 3; k++) gr[k] += massf * fr[k];
+        const R v[4] = {wab * volp2, fr[0] * volp2, fr[1] * volp2, fr[2] * volp2};
+        for (int i = 0; i < 4; i++) {
+          a[i][0] += v[i];
+          for (int k = 0; k < 3; k++) a[i][k + 1] += dr[k] * v[i];
+        }
+      }
+    float rhofinal = FLT_MAX;
+    double ad[4][4], inv[4][4];
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) ad[i][j] = double(a[i][j]);
+    const double det = inverse4(ad, inv);
+    if (std::fabs(det) >= 1e-3) {
+      // The GPU build rounds the inverse to FP32 (tmatrix4f); the CPU build keeps FP64.
+      M im[4][4];
+      for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) im[i][j] = M(inv[i][j]);
+      const R b[4] = {rhop1, gr[0], gr[1], gr[2]};
+      R o[4];
+      for (int i = 0; i < 4; i++)
+        o[i] = R(im[i][0] * b[0] + im[i][1] * b[1] + im[i][2] * b[2] + im[i][3] * b[3]);
+      rhofinal = float(o[0] - o[1] * R(-bn[0]) - o[2] * R(-bn[1]) - o[3] * R(-bn[2]));
+    } else if (a[0][0] > 0) {
+      rhofinal = float(rhop1 / R(a[0][0]));
+    }
+    velrho[4 * size_t(p1) + 3] = (rhofinal != FLT_MAX ? rhofinal : c.rhopzero);
+  }
+}
+
+}  // namespace
+
+extern "C" {
+
 // SPS stress tau/rho^2 (JSphCpu.cpp:1279-1309).
 void orc_sps_tau(unsigned np, unsigned npb, float smag, float blin, const float* velrho,
                  const float* s2, float* tau) {
@@ -732,6 +853,15 @@ void orc_kernel_eval(int prec, float rr2, float h, float awen, float bwenh, doub
     *wab = wend_wab<double>(rr2, h, awen);
     *fac = wend_fac<double>(rr2, h, bwenh);
   }
+}
+// mDBC correction of boundary particles [0,n) (see mdbc_correction).
+// prec: 0 FP32, 1 FP64; sem: SEM_GPU (FP32 matrix) or SEM_CPU (FP64 matrix).
+void orc_mdbc(int prec, int sem, const orc_cte* c, const orc_div* d, unsigned n,
+              const double* pos, const float* poscell, const typecode* code,
+              const float* boundnor2, float* velrho) {
+  if (prec == 1) mdbc_correction<double, double>(SEM_F64, *c, *d, n, pos, poscell, code, boundnor2, velrho);
+  else if (sem == SEM_CPU) mdbc_correction<float, double>(sem, *c, *d, n, pos, poscell, code, boundnor2, velrho);
+  else mdbc_correction<float, float>(sem, *c, *d, n, pos, poscell, code, boundnor2, velrho);
 }
 double orc_eos(int prec, int sem, const orc_cte* c, float rho) {
   return prec == 0 ? double(eos<float>(sem, *c, rho)) : eos<double>(sem, *c, rho);

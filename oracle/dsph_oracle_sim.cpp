@@ -54,6 +54,8 @@ void orc_verlet(unsigned, unsigned, const float*, const float*, const float*, co
                 double, float, float, float, const float*, typecode*, double*, float*);
 void orc_step_pos(unsigned, unsigned, const orc_cte*, const double*, const double*, double*,
                   unsigned*, typecode*);
+void orc_mdbc(int, int, const orc_cte*, const orc_div*, unsigned, const double*, const float*,
+              const typecode*, const float*, float*);
 void orc_pips(const orc_cte*, const orc_div*, unsigned, unsigned, unsigned, const unsigned*,
               const float*, unsigned long long*);
 }
@@ -77,6 +79,8 @@ struct Sim {
   std::vector<typecode> code;
   std::vector<double> pos, pospre, mov;
   std::vector<float> velrho, velrhopre, velrhom1, poscell, ace, ar, viscdt, strain, tau;
+  std::vector<float> boundnor;  // doubled normals (ConfigBoundNormals), mDBC only
+  int TBoundary = 1;
   bool pre_active = false;
   // cell division
   std::vector<unsigned> cellpart, sortpart;
@@ -107,7 +111,7 @@ struct Sim {
     for (int k = 0; k < 3; k++) g[k] = float(s.gravity[k]);
     c.ddtkh = ks * float(s.ddtvalue);
     c.ddtgz = float(double(c.rhopzero) * double(std::fabs(g[2])) / double(c.cteb));
-    TStep = s.tstep; TVisco = s.tvisco; TDDT = s.tdensity;
+    TStep = s.tstep; TVisco = s.tvisco; TDDT = s.tdensity; TBoundary = s.tboundary;
     VerletSteps = s.verletsteps ? s.verletsteps : 40;
     Visco = float(s.visco); ViscoB = Visco * float(s.viscoboundfactor);
     RhopOutMin = float(s.rhopoutmin); RhopOutMax = float(s.rhopoutmax);
@@ -153,7 +157,7 @@ struct Sim {
     code.resize(np);
     dcell.resize(np);
     for (unsigned p = 0; p < np; p++) {
-      code[p] = typecode(idp[p] < CaseNbound ? 0x0000 : 0x1800);
+      code[p] = typecode(idp[p] < cs.casenfixed ? 0x0000 : (idp[p] < CaseNbound ? 0x0801 : 0x1800));
       const double* x = &pos[3 * size_t(p)];
       const unsigned cx = unsigned((x[0] - c.domposmin[0]) / Scell);
       const unsigned cy = unsigned((x[1] - c.domposmin[1]) / Scell);
@@ -244,6 +248,7 @@ struct Sim {
     if (TStep == 1) gather(velrhom1, 4, nptot, gpini);
     else if (pre_active) { gather(pospre, 3, nptot, gpini); gather(velrhopre, 4, nptot, gpini); }
     if (TVisco == 2) gather(tau, 6, nptot, gpini);
+    if (TBoundary == 2) gather(boundnor, 3, nptot, gpini);
     if (npbout) throw std::runtime_error("boundary particles excluded");
     Np = npfinal; Npb = npbfinal; NpbOk = Npb - npbignore;
     NpfOutTotal += npfout;
@@ -290,8 +295,56 @@ struct Sim {
     return dt;
   }
 
+  // JSphCpu::RunMotion + MoveLinBound for one rectilinear sinusoid
+  // (JMotionObj.cpp:558-575, JMotionList.cpp:43-50, JSphCpu.cpp MoveLinBound):
+  // displacement over [t, t+dt], velocity displacement/dt, applied after the
+  // step; the next divide is full.
+  unsigned mkcell_host(unsigned cx, unsigned cy, unsigned cz) const {
+    const unsigned dcc = c.cellcode;
+    return (cx << ((dcc >> 10) & 31u)) | (cy << ((dcc >> 15) & 31u)) | cz;
+  }
+  void run_motion(double dt) {
+    const dsph_motion& m = cs.motion;
+    if (!cs.casenmoving || m.type != 1) return;
+    if (!(TimeStep >= m.start && TimeStep < m.finish)) return;
+    const double twopi = 6.283185307179586476925286766559;
+    double mv[3];
+    for (int k = 0; k < 3; k++)
+      mv[k] = m.ampl[k] != 0 ? m.ampl[k] * std::sin(m.phase[k] + twopi * m.freq[k] * (TimeStep + dt)) -
+                                   m.ampl[k] * std::sin(m.phase[k] + twopi * m.freq[k] * TimeStep)
+                             : 0.0;
+    for (unsigned p = 0; p < Npb; p++) {
+      if (idp[p] - m.idbegin >= m.count) continue;
+      double* x = &pos[3 * size_t(p)];
+      for (int k = 0; k < 3; k++) x[k] += mv[k];
+      double dd[3];
+      bool out = false;
+      for (int k = 0; k < 3; k++) {
+        dd[k] = x[k] - c.maprealposmin[k];
+        if (dd[k] != dd[k] || dd[k] < 0 || dd[k] >= c.maprealsize[k]) out = true;
+      }
+      if (out) throw std::runtime_error("moving boundary particle left the domain");
+      dcell[p] = mkcell_host(unsigned(dd[0] / c.scell), unsigned(dd[1] / c.scell),
+                             unsigned(dd[2] / c.scell));
+      float* v = &velrho[4 * size_t(p)];
+      for (int k = 0; k < 3; k++) v[k] = float(mv[k] / dt);
+    }
+    BoundChanged = true;
+  }
+
+  // JSphCpuSingle::MdbcBoundCorrection: mDBC (SLIP_Vel0) before the
+  // predictor (Symplectic) or the Verlet interaction; not before the
+  // corrector (MdbcCorrector=false, the default).
+  void mdbc() {
+    if (TBoundary != 2) return;
+    const orc_div d = divdata();
+    orc_mdbc(prec == 1 ? 1 : 0, prec == 2 ? 1 : 0, &c, &d, NpbOk, pos.data(), poscell.data(),
+             code.data(), boundnor.data(), velrho.data());
+  }
+
   void step_sym() {
     const double dt = SymplecticDtPre;
+    mdbc();
     interaction();
     const double dt_p = dt_variable();
     pospre = pos; velrhopre = velrho; pre_active = true;
@@ -307,12 +360,14 @@ struct Sim {
                 RhopOutMin, RhopOutMax, g, code.data(), mov.data(), velrho.data());
     orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
     pre_active = false;
+    run_motion(dt);
     SymplecticDtPre = std::min(dt_p, dt_c);
     TimeStep += dt;
     DtLast = dt;
   }
 
   void step_ver() {
+    mdbc();
     interaction();
     const double dt = dt_variable();
     VerletStep++;
@@ -330,6 +385,7 @@ struct Sim {
     orc_step_pos(Np, Npb, &c, p0.data(), mov.data(), pos.data(), dcell.data(), code.data());
     velrhom1 = velrho;
     velrho = nw;
+    run_motion(dt);
     TimeStep += dt;
     DtLast = dt;
   }
@@ -348,11 +404,20 @@ struct Sim {
 extern "C" {
 
 void* orc_sim_create(const dsph_case* cs, int prec, unsigned np, const unsigned* idp,
-                     const double* pos, const float* velrho) {
+                     const double* pos, const float* velrho, const float* boundnor) {
   Sim* s = new Sim();
   s->prec = prec;
   try {
     s->config(*cs);
+    if (s->TBoundary == 2) {
+      // Normals of the boundary particles (id order), stored doubled
+      // (JSph::ConfigBoundNormals, JSph.cpp:1385-1434).
+      if (!boundnor) throw std::runtime_error("mDBC needs boundary normals");
+      s->boundnor.assign(3 * size_t(np), 0.f);
+      for (unsigned p = 0; p < np; p++)
+        if (idp[p] < cs->casenbound)
+          for (int k = 0; k < 3; k++) s->boundnor[3 * size_t(p) + k] = boundnor[3 * size_t(idp[p]) + k] * 2.f;
+    }
     s->load(np, idp, pos, velrho);
   } catch (const std::exception& e) {
     s->err = e.what();

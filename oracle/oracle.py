@@ -57,7 +57,8 @@ def load() -> C.CDLL:
             "orc_pips": ([C.POINTER(OrcCte), C.POINTER(OrcDiv), u, u, u, vp, vp, vp], None),
             "orc_bruteforce_fp64": ([C.POINTER(OrcCte), i, i, f, f, u, u, vp, vp, vp, vp, vp],
                                     None),
-            "orc_sim_create": ([vp, i, u, vp, vp, vp], vp),
+            "orc_sim_create": ([vp, i, u, vp, vp, vp, vp], vp),
+            "orc_mdbc": ([i, i, C.POINTER(OrcCte), C.POINTER(OrcDiv), u, vp, vp, vp, vp, vp], None),
             "orc_sim_destroy": ([vp], None),
             "orc_sim_error": ([vp], C.c_char_p),
             "orc_sim_run": ([vp, u], i),
@@ -93,8 +94,11 @@ class OracleSim:
         idp = np.ascontiguousarray(case.idp, np.uint32)
         pos = np.ascontiguousarray(case.pos, np.float64)
         vr = np.ascontiguousarray(case.velrho, np.float32)
+        bn = (None if getattr(case, "boundnor", None) is None
+              else np.ascontiguousarray(case.boundnor, np.float32))
+        self._bn = bn
         self._h = self.lib.orc_sim_create(C.cast(C.pointer(self._nc), C.c_void_p), prec,
-                                          case.np, _p(idp), _p(pos), _p(vr))
+                                          case.np, _p(idp), _p(pos), _p(vr), _p(bn))
         err = self.lib.orc_sim_error(self._h).decode()
         if err:
             raise RuntimeError("oracle: " + err)

@@ -37,10 +37,15 @@ from oracle import bi4  # noqa: E402
 REF_EXE = ROOT / "oracle" / "_ref" / "DualSPHysics5.4CPU_ieee"
 OUT = Path(__file__).resolve().parent
 CASES = {
-    # name: (dp, nsteps, saved steps, Parameters overrides)
-    "ref_dambreak_dp0.03": (0.03, 10, (1, 2, 5, 10), {}),
-    "ref_dambreak_dp0.03_verlet_laminar": (0.03, 10, (1, 5, 10),
+    # name: (generator, dp, nsteps, saved steps, Parameters overrides)
+    "ref_dambreak_dp0.03": ("dam_break", 0.03, 10, (1, 2, 5, 10), {}),
+    "ref_dambreak_dp0.03_verlet_laminar": ("dam_break", 0.03, 10, (1, 5, 10),
                                            dict(tstep=1, tvisco=3, visco=1e-4, tdensity=2)),
+    # Flume with a piston wavemaker (moving boundary): DBC, and mDBC with
+    # laminar+SPS viscosity (the flume configuration's options).
+    "ref_flume_dp0.01_piston": ("flume", 0.01, 10, (1, 5, 10), {}),
+    "ref_flume_dp0.01_piston_mdbc_sps": ("flume", 0.01, 10, (1, 5, 10),
+                                         dict(tboundary=2, tvisco=2, visco=1e-6)),
 }
 
 
@@ -90,9 +95,9 @@ def main():
     if "--observables" in sys.argv:
         observables_fixture()
         return
-    for name, (dp, nsteps, saved, over) in CASES.items():
+    for name, (gen, dp, nsteps, saved, over) in CASES.items():
         params = cm.Parameters(**over)
-        case = cm.dam_break(dp=dp, params=params)
+        case = getattr(cm, gen)(dp=dp, params=params)
         with tempfile.TemporaryDirectory() as td:
             parts = run_reference(case, nsteps, Path(td))
         p0 = parts[0]
@@ -102,10 +107,15 @@ def main():
         fl = slice(case.nbound, None)
         np.savez_compressed(
             OUT / f"{name}.npz",
-            dp=dp, seed=42, params=np.array(repr(sorted(over.items()))),
+            generator=np.array(gen), dp=dp, seed=42, params=np.array(repr(sorted(over.items()))),
+            nmoving=case.nmoving,
             nbound=case.nbound, nfluid=case.nfluid, steps=np.array(saved),
             time=np.array([parts[k]["time"] for k in saved]),
             disp=np.stack([(parts[k]["pos"] - p0["pos"])[fl].astype(np.float32) for k in saved]),
+            disp_moving=np.stack([(parts[k]["pos"] - p0["pos"])[case.nbound - case.nmoving:case.nbound]
+                                  for k in saved]),
+            vel_moving=np.stack([parts[k]["vel"][case.nbound - case.nmoving:case.nbound]
+                                 for k in saved]),
             vel=np.stack([parts[k]["vel"][fl] for k in saved]),
             rho=np.stack([parts[k]["rho"] for k in saved]))
         print(f"wrote {name}.npz ({case.np} particles, steps {saved})")

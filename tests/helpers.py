@@ -149,7 +149,8 @@ def load_reference_fixture(name: str):
     from dualsphysics_amd import case as cm
     fx = dict(np.load(GOLDEN / f"{name}.npz", allow_pickle=False))
     over = dict(ast.literal_eval(str(fx["params"])))
-    case = cm.dam_break(dp=float(fx["dp"]), seed=int(fx["seed"]), params=cm.Parameters(**over))
+    gen = str(fx["generator"]) if "generator" in fx else "dam_break"
+    case = getattr(cm, gen)(dp=float(fx["dp"]), seed=int(fx["seed"]), params=cm.Parameters(**over))
     assert case.nbound == int(fx["nbound"]) and case.nfluid == int(fx["nfluid"])
     return fx, case
 
@@ -160,12 +161,25 @@ def load_reference_fixture(name: str):
 # measured between the reference and the oracle run in FP64 (time 2e-9,
 # displacement 2e-5, velocity 3e-5, density 2 ulp).
 REF_TOL = {"time": 1e-6, "disp": 2e-4, "vel": 2e-4, "rho": 10 * 6.1035e-05}
+# mDBC: the boundary density comes from inverting a 4x4 moment matrix that is
+# close to singular near walls, which amplifies FP32 rounding: the reference
+# itself differs from the FP64 oracle by 1e-3 (velocity, norm-wise) and 3e-3
+# (density) over the first steps.
+REF_TOL_MDBC = {"time": 1e-5, "disp": 3e-3, "vel": 3e-3, "rho": 1e-2}
 
 
 def compare_with_reference(fx, k, time, pos0, pos, vel, rho):
     """Errors of one saved step (arrays in id order); returns a dict and
-    asserts the bounds of REF_TOL."""
+    asserts the bounds of REF_TOL (REF_TOL_MDBC for mDBC cases). The moving
+    boundary (if any) must follow the reference to 1e-9 m and 1e-6 m/s."""
     nb = int(fx["nbound"])
+    tol = REF_TOL_MDBC if "tboundary', 2" in str(fx["params"]) else REF_TOL
+    nmov = int(fx["nmoving"]) if "nmoving" in fx else 0
+    if nmov:
+        mv = slice(nb - nmov, nb)
+        dm = float(np.abs((pos[mv] - pos0[mv]) - fx["disp_moving"][k]).max())
+        vm = float(np.abs(vel[mv] - fx["vel_moving"][k]).max())
+        assert dm < 1e-9 and vm < 1e-6, (int(fx["steps"][k]), "moving boundary", dm, vm)
     disp_ref = fx["disp"][k].astype(np.float64)
     disp = pos[nb:] - pos0[nb:]
     e = {
@@ -174,6 +188,6 @@ def compare_with_reference(fx, k, time, pos0, pos, vel, rho):
         "vel": float(np.abs(vel[nb:] - fx["vel"][k]).max() / np.abs(fx["vel"][k]).max()),
         "rho": float(np.abs(rho.astype(np.float64) - fx["rho"][k]).max()),
     }
-    for key, tol in REF_TOL.items():
-        assert e[key] < tol, (int(fx["steps"][k]), key, e)
+    for key, t in tol.items():
+        assert e[key] < t, (int(fx["steps"][k]), key, e)
     return e

@@ -11,7 +11,9 @@ from tests.helpers import compare_with_reference, load_reference_fixture
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("name", ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar"])
+@pytest.mark.parametrize("name", ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
+                                  "ref_flume_dp0.01_piston",
+                                  "ref_flume_dp0.01_piston_mdbc_sps"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)

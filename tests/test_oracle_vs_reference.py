@@ -6,8 +6,10 @@ The oracle runs with the reference CPU build's pair semantics (prec=2: FP32,
 float(FP64 difference) distances, two passes), with the GPU semantics
 (prec=0) the HIP kernels follow, and in FP64 (prec=1); all three stay within
 FP32 rounding of the reference over the first 10 steps (bounds in
-tests/helpers.REF_TOL). Cases: Symplectic + artificial viscosity + DDT2Full
-(the benchmark configuration) and Verlet + laminar viscosity + DDT2.
+tests/helpers.REF_TOL). Cases: dam break with Symplectic + artificial
+viscosity + DDT2Full (the benchmark configuration) and Verlet + laminar
+viscosity + DDT2; wave flume with a piston wavemaker (moving boundary,
+rectilinear sinusoid) with DBC, and with mDBC + laminar+SPS viscosity.
 """
 import numpy as np
 import pytest
@@ -15,7 +17,8 @@ import pytest
 from oracle.oracle import OracleSim
 from tests.helpers import compare_with_reference, load_reference_fixture
 
-FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar"]
+FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
+            "ref_flume_dp0.01_piston", "ref_flume_dp0.01_piston_mdbc_sps"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
