@@ -74,6 +74,12 @@ class DsphCase(C.Structure):
                                   "casenfloat", "casenfluid")] + [("motion", DsphMotion)]
 
 
+class DsphMlp(C.Structure):
+    _fields_ = [("d_params", C.c_void_p), ("d_wbf16", C.c_void_p), ("d_adam_m", C.c_void_p),
+                ("d_adam_v", C.c_void_p), ("feat_mean", C.c_float * 11),
+                ("feat_scale", C.c_float * 11), ("out_scale", C.c_float)]
+
+
 class DsphStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("np", "npb", "npbok", "nct", "nstep",
                                           "npf_out_total")] + [
@@ -149,6 +155,17 @@ SIGNATURES = {
     "dsph_sim_pips": [_vp, C.POINTER(C.c_uint64)],
     "dsph_sim_interact": [_vp],
     "dsph_sim_ctx": [_vp],
+    "dsph_mlp_param_count": [],
+    "dsph_mlp_bf16_count": [],
+    "dsph_mlp_sync": [_vp, C.POINTER(DsphMlp)],
+    "dsph_mlp_forward": [_vp, C.POINTER(DsphMlp), _u, _vp, _vp],
+    "dsph_mlp_grad": [_vp, C.POINTER(DsphMlp), _u, _vp, _vp, _vp, _sz, _vp, _vp],
+    "dsph_mlp_adam": [_vp, C.POINTER(DsphMlp), _vp, _f, _f, _f, _f, _u],
+    "dsph_sps_tau_learned": [_vp, C.POINTER(DsphMlp), _u, _u, _f, _f] + [_vp] * 5,
+    "dsph_filter_probes": [_vp, _u, _vp, _d, C.POINTER(DsphDivData), _f] + [_vp] * 5,
+    "dsph_sim_set_closure": [_vp, C.POINTER(DsphMlp)],
+    "dsph_sim_closure_nu": [_vp, _vp, _vp],
+    "dsph_sim_filter_probes": [_vp, _u, _vp, _d, _vp],
 }
 RESTYPES = {"dsph_last_error": C.c_char_p, "dsph_sim_last_error": C.c_char_p,
             "dsph_ctx_get_stream": _vp, "dsph_sim_ctx": _vp}

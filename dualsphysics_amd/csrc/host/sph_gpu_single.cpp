@@ -112,6 +112,11 @@ struct dsph_sim {
   u3 BoundLimitCellMin{}, BoundLimitCellMax{}, BoundDivideCellMin{}, BoundDivideCellMax{};
   unsigned NpFinal = 0, NpbFinal = 0, NpbIgnore = 0, NpbOut = 0, NpfOut = 0;
   bool BoundChanged = true;
+  // Learned SPS closure (optional): network and the velocity-gradient buffer.
+  dsph_mlp Closure{};
+  bool UseClosure = false;
+  float* velgrad = nullptr;
+  float* closure_nu = nullptr;
 
   // ---- run state
   double SymplecticDtPre = 0, TimeStep = 0, DtLast = 0;
@@ -137,7 +142,7 @@ struct dsph_sim {
                     velrho2, poscell, posxypre, poszpre, posxypre2, poszpre2, velrhopre,
                     velrhopre2, velrhom1, velrhom1b, ace, ar, viscdt, movxy, movz, sps2strain,
                     spstau, spstau2, boundnor, boundnor2, d_red, d_dt, d_flags, cellpart,
-                    sortpart, begincell};
+                    sortpart, begincell, velgrad, closure_nu};
     for (void* p : ptrs) dsph_free(ctx, p);
     dsph_ctx_destroy(ctx);
     ctx = nullptr;
@@ -506,9 +511,16 @@ struct dsph_sim {
     t.d_dcell = dcell; t.d_poscell = poscell; t.d_velrho = velrho; t.d_code = code;
     t.d_spstaurho2 = spstau;
     t.d_viscdt = viscdt; t.d_ar = ar; t.d_ace = ace; t.d_sps2strain = sps2strain;
+    t.d_velgrad = UseClosure ? velgrad : nullptr;
     chk(dsph_interaction_forces(ctx, &t), "interaction");
-    if (TVisco == DSPH_VISCO_LAMINARSPS)
-      chk(dsph_sps_tau(ctx, Np, Npb, SpsSmag, SpsBlin, velrho, sps2strain, spstau), "sps tau");
+    if (TVisco == DSPH_VISCO_LAMINARSPS) {
+      if (UseClosure)
+        chk(dsph_sps_tau_learned(ctx, &Closure, Np, Npb, SpsBlin, float(Dp), velrho, velgrad,
+                                 sps2strain, spstau, closure_nu),
+            "learned sps tau");
+      else
+        chk(dsph_sps_tau(ctx, Np, Npb, SpsSmag, SpsBlin, velrho, sps2strain, spstau), "sps tau");
+    }
     chk(dsph_redu_max_f32(ctx, Np, 0, viscdt, d_red + 1), "viscdtmax");
     chk(dsph_acemax2(ctx, Np - Npb, Npb, ace, d_red + 2), "acemax");
   }
@@ -741,6 +753,64 @@ int dsph_sim_download_forces(dsph_sim* sim, float* h_ace, float* h_ar) {
     const size_t n = sim->Np;
     if (h_ace) sim->chk(dsph_memcpy_d2h(sim->ctx, h_ace, sim->ace, 12 * n), "download ace");
     if (h_ar) sim->chk(dsph_memcpy_d2h(sim->ctx, h_ar, sim->ar, 4 * n), "download ar");
+  } catch (const Err& e) {
+    sim->err = e.what();
+    return e.code;
+  }
+  return DSPH_OK;
+}
+
+int dsph_sim_set_closure(dsph_sim* sim, const dsph_mlp* m) {
+  if (!sim) return DSPH_E_ARG;
+  try {
+    if (!m) {
+      sim->UseClosure = false;
+      return DSPH_OK;
+    }
+    if (sim->TVisco != DSPH_VISCO_LAMINARSPS)
+      throw Err(DSPH_E_STATE, "the learned closure needs Laminar+SPS viscosity");
+    sim->Closure = *m;
+    if (!sim->velgrad) {
+      sim->velgrad = sim->alloc<float>(9 * size_t(sim->NpCap));
+      sim->closure_nu = sim->alloc<float>(size_t(sim->NpCap));
+      sim->chk(dsph_memset_async(sim->ctx, sim->closure_nu, 0, 4 * size_t(sim->NpCap)), "nu");
+    }
+    sim->UseClosure = true;
+  } catch (const Err& e) {
+    sim->err = e.what();
+    return e.code;
+  }
+  return DSPH_OK;
+}
+
+int dsph_sim_closure_nu(dsph_sim* sim, float* h_nu, float* h_velgrad) {
+  if (!sim || !h_nu) return DSPH_E_ARG;
+  if (!sim->closure_nu) {
+    sim->err = "dsph_sim_closure_nu: the learned closure was never enabled";
+    return DSPH_E_STATE;
+  }
+  int rc = dsph_memcpy_d2h(sim->ctx, h_nu, sim->closure_nu, 4 * size_t(sim->Np));
+  if (!rc && h_velgrad)
+    rc = dsph_memcpy_d2h(sim->ctx, h_velgrad, sim->velgrad, 36 * size_t(sim->Np));
+  if (rc) sim->err = dsph_last_error(sim->ctx);
+  return rc;
+}
+
+int dsph_sim_filter_probes(dsph_sim* sim, uint32_t nprobe, const double* h_probes, double hf,
+                           double* h_out) {
+  if (!sim || !h_probes || !h_out) return DSPH_E_ARG;
+  if (!nprobe) return DSPH_OK;
+  try {
+    double* dp = sim->alloc<double>(3 * size_t(nprobe));
+    double* dout = sim->alloc<double>(23 * size_t(nprobe));
+    sim->chk(dsph_memcpy_h2d(sim->ctx, dp, h_probes, 24 * size_t(nprobe)), "probes");
+    const dsph_divdata d = sim->divdata();
+    sim->chk(dsph_filter_probes(sim->ctx, nprobe, dp, hf, &d, sim->MassFluid, sim->posxy,
+                                sim->posz, sim->velrho, sim->code, dout),
+             "filter probes");
+    sim->chk(dsph_memcpy_d2h(sim->ctx, h_out, dout, 8 * 23 * size_t(nprobe)), "probe moments");
+    dsph_free(sim->ctx, dp);
+    dsph_free(sim->ctx, dout);
   } catch (const Err& e) {
     sim->err = e.what();
     return e.code;

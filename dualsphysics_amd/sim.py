@@ -76,15 +76,18 @@ class SphGpuSingle:
     def run_until(self, t: float, max_steps: int = 10_000_000) -> int:
         """Steps until the simulated time reaches ``t`` (the step that crosses
         ``t`` included, as the reference's output times); returns the number
-        of steps run. Runs batches sized from the last dt, then single steps."""
+        of steps run. Runs batches of 90 % of the steps the last dt predicts
+        (dt may grow within a batch), then single steps near ``t``."""
         n = 0
         while n < max_steps:
             st = self.stats()
             if st["timestep"] >= t:
                 break
-            dt = st["dt_last"] or 1e-9
-            k = int((t - st["timestep"]) / dt) - 2
-            k = max(1, min(k, max_steps - n))
+            if st["dt_last"] > 0:
+                k = int(0.9 * (t - st["timestep"]) / st["dt_last"]) - 1
+                k = max(1, min(k, max_steps - n))
+            else:
+                k = 1  # dt not known before the first step
             self.run(k)
             n += k
         return n
@@ -124,3 +127,32 @@ class SphGpuSingle:
         res = (C.c_uint64 * 5)()
         self._check(self.lib.dsph_sim_pips(self._h, res))
         return [int(v) for v in res]
+
+    def set_closure(self, mlp):
+        """Uses the learned eddy viscosity of ``mlp`` (a
+        :class:`dualsphysics_amd.closure.Mlp` on this simulation's device) in
+        place of the Smagorinsky viscosity (Laminar+SPS runs); None restores
+        the Smagorinsky model. The network buffers must outlive the run."""
+        self._closure = mlp
+        if mlp is not None:
+            mlp.synchronize()
+        self._check(self.lib.dsph_sim_set_closure(self._h, None if mlp is None else C.byref(mlp.m)))
+
+    def closure_nu(self, with_velgrad: bool = False):
+        """Eddy viscosity of the last interaction (sorted order; 0 for
+        boundary) and, with ``with_velgrad``, the velocity gradient [np,9] it
+        was evaluated on."""
+        n = self.stats()["np"]
+        nu = np.empty(n, np.float32)
+        g = np.empty((n, 9), np.float32) if with_velgrad else None
+        self._check(self.lib.dsph_sim_closure_nu(self._h, _ptr(nu), None if g is None else _ptr(g)))
+        return (nu, g) if with_velgrad else nu
+
+    def filter_probes(self, probes: np.ndarray, hf: float) -> np.ndarray:
+        """Filter moments [n,23] of the current fluid state at ``probes``
+        (Wendland kernel of smoothing length ``hf``), see dsph_filter_probes."""
+        probes = np.ascontiguousarray(probes, np.float64)
+        out = np.empty((len(probes), 23), np.float64)
+        self._check(self.lib.dsph_sim_filter_probes(self._h, len(probes), _ptr(probes), hf,
+                                                    _ptr(out)))
+        return out

@@ -275,6 +275,59 @@ int dsph_sps_tau(dsph_ctx* ctx, uint32_t np, uint32_t npb, float smag,
                  float blin, const float* d_velrho, const float* d_sps2strain,
                  float* d_tau);
 
+/* ---------------------------------------------------------------- learned closure
+ * A per-particle MLP (11 features -> 64 -> 64 -> 1, SiLU, softplus output)
+ * predicting the eddy viscosity of the SPS stress from the velocity gradient
+ * (9, row-major d v_a / d x_b), density and dp. No counterpart in the
+ * reference: it replaces the Smagorinsky viscosity smag*sqrt(prr) of
+ * cusph::ComputeSpsTau (JSphGpu_ker.h:304) when enabled. Products run on the
+ * matrix cores in BF16 with FP32 accumulation; parameters are FP32 masters
+ * with BF16 copies. Buffers are owned by the caller. */
+typedef struct dsph_mlp {
+  float* d_params;               /* FP32 parameters [dsph_mlp_param_count()] */
+  uint16_t* d_wbf16;             /* BF16 weight copies [dsph_mlp_bf16_count()] */
+  float* d_adam_m;               /* Adam moments [param_count] (training only) */
+  float* d_adam_v;
+  float feat_mean[11], feat_scale[11];  /* x' = (x - mean) * scale */
+  float out_scale;               /* nu = out_scale * softplus(output) [m^2/s] */
+} dsph_mlp;
+/* Parameter block layout: W1[64][16] (input columns 11..15 unused), b1[64],
+ * W2[64][64], b2[64], w3[64], b3. */
+int dsph_mlp_param_count(void);
+int dsph_mlp_bf16_count(void);
+/* Refreshes the BF16 copies from the FP32 parameters. */
+int dsph_mlp_sync(dsph_ctx* ctx, const dsph_mlp* m);
+/* d_out[i] = nu(features[i]), features FP32 [n][11]. */
+int dsph_mlp_forward(dsph_ctx* ctx, const dsph_mlp* m, uint32_t n,
+                     const float* d_features, float* d_out);
+/* Gradient of L = mean_i (softplus(o_i) - t_i/out_scale)^2 over the n
+ * samples into d_grad [param_count] and L into d_loss (may be NULL).
+ * d_scratch holds per-block partials: at least 21256 bytes, more gives more
+ * parallelism (up to 1024 blocks). Deterministic (fixed reduction order). */
+int dsph_mlp_grad(dsph_ctx* ctx, const dsph_mlp* m, uint32_t n,
+                  const float* d_features, const float* d_targets,
+                  float* d_scratch, size_t scratch_bytes, float* d_grad,
+                  float* d_loss);
+/* One Adam step (step = 1, 2, ...) on the FP32 parameters, then the BF16
+ * copies. */
+int dsph_mlp_adam(dsph_ctx* ctx, const dsph_mlp* m, const float* d_grad,
+                  float lr, float beta1, float beta2, float eps, uint32_t step);
+/* SPS tau of the fluid particles [npb,np) with the learned eddy viscosity
+ * (features from d_velgrad [9*np], rho, dp); the blin term as
+ * cusph::ComputeSpsTau. d_nu [np] receives nu_t (may be NULL). */
+int dsph_sps_tau_learned(dsph_ctx* ctx, const dsph_mlp* m, uint32_t np,
+                         uint32_t npb, float blin, float dp,
+                         const float* d_velrho, const float* d_velgrad,
+                         const float* d_sps2strain, float* d_tau, float* d_nu);
+/* Training data from a fine-resolution state: Shepard/Wendland filter with
+ * smoothing length hf at nprobe probe points (FP64 [nprobe][3]) over the
+ * fluid particles; 23 FP64 moments per probe (see learned.hip). */
+int dsph_filter_probes(dsph_ctx* ctx, uint32_t nprobe, const double* d_probes,
+                       double hf, const dsph_divdata* dvd, float massf,
+                       const double* d_posxy, const double* d_posz,
+                       const float* d_velrho, const uint16_t* d_code,
+                       double* d_out);
+
 /* Replaces cusph::Interaction_MdbcCorrection (JSphGpu_mdbc_iker.h:32),
  * SlipMode SLIP_Vel0 (mDBC with vel=0): density of boundary particles
  * [0,n) extrapolated from their ghost node; writes velrho[p].w. */
@@ -470,6 +523,19 @@ int dsph_sim_download_forces(dsph_sim* sim, float* h_ace, float* h_ar);
  * dsph_pips_count_split_sync: {real fluid, real bound, checked fluid,
  * checked bound, real fluid-boundary}. */
 int dsph_sim_pips(dsph_sim* sim, uint64_t h_res[5]);
+/* Enables the learned SPS closure (Laminar+SPS only; m = NULL disables):
+ * the interaction also computes the velocity gradient and the SPS tau uses
+ * the network's eddy viscosity (dsph_sps_tau_learned). The network buffers
+ * stay owned by the caller and must outlive the simulation's use of them. */
+int dsph_sim_set_closure(dsph_sim* sim, const dsph_mlp* m);
+/* Eddy viscosity [np] of the last learned-closure evaluation and, when
+ * h_velgrad is not NULL, the velocity gradient [np][9] (dv_a/dx_b, row-major)
+ * it was evaluated on (sorted order, as dsph_sim_download). */
+int dsph_sim_closure_nu(dsph_sim* sim, float* h_nu, float* h_velgrad);
+/* Training data of the closure: dsph_filter_probes over the current state at
+ * nprobe host probe points; h_out receives 23 FP64 moments per probe. */
+int dsph_sim_filter_probes(dsph_sim* sim, uint32_t nprobe, const double* h_probes,
+                           double hf, double* h_out);
 /* Evaluates PreInteraction_Forces + Interaction_Forces on the current state
  * without advancing it (forces then readable with dsph_sim_download_forces). */
 int dsph_sim_interact(dsph_sim* sim);

@@ -35,6 +35,32 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def declared_parameters() -> dict[str, list[str]]:
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    text = re.sub(r"//[^\n]*", "", text)
+    out = {}
+    for m in re.finditer(r"\b(dsph_[a-z0-9_]+)\s*\(([^()]*)\)\s*;", text):
+        params = [p.strip() for p in m.group(2).split(",")]
+        out[m.group(1)] = [] if params in ([""], ["void"]) else params
+    return out
+
+
+def test_binding_arity_and_pointer_kinds_match_header():
+    """Each binding has the header's parameter count, and pointers where the
+    header has pointers (a missing or extra argument would shift every
+    argument after it)."""
+    decl = declared_parameters()
+    assert set(decl) == set(N.SIGNATURES)
+    for name, params in decl.items():
+        args = N.SIGNATURES[name]
+        assert len(args) == len(params), (name, len(args), len(params))
+        for p, a in zip(params, args):
+            is_ptr = "*" in p or "[" in p
+            a_ptr = a in (C.c_void_p, C.c_char_p) or issubclass(a, C._Pointer)
+            assert is_ptr == a_ptr, (name, p, a)
+
+
 def test_bindings_load_with_signatures():
     lib = N.load()
     for name, args in N.SIGNATURES.items():
@@ -44,7 +70,8 @@ def test_bindings_load_with_signatures():
 
 STRUCTS = {"dsph_cte": N.DsphCte, "dsph_divdata": N.DsphDivData,
            "dsph_inter_parms": N.DsphInterParms, "dsph_case": N.DsphCase,
-           "dsph_stats": N.DsphStats, "dsph_motion": N.DsphMotion}
+           "dsph_stats": N.DsphStats, "dsph_motion": N.DsphMotion,
+           "dsph_mlp": N.DsphMlp}
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):
