@@ -73,6 +73,62 @@ def cpu_baseline(case, steps: int):
             "steps_per_s": steps / dt}
 
 
+def ensemble_bench(n: int, tend: float, sample: int):
+    """Batched ensemble of ``n`` uneven dam breaks (seeded draws), every case
+    run from t=0 to ``tend``: cases/s of the ensemble (one graph launch per 3
+    steps for all cases, autotuned fluid kernel per size bucket) against the
+    plain single-case path (one SphGpuSingle per case, default parameters)
+    timed on a size-stratified sample of ``sample`` cases. Set-up (case
+    generation, upload, first divide, graph capture) is not timed for either."""
+    import numpy as np
+    import torch
+    from dualsphysics_amd.ensemble import Ensemble, dam_break_ensemble
+    from dualsphysics_amd.sim import SphGpuSingle
+    t0 = time.perf_counter()
+    cases = dam_break_ensemble(n)
+    sizes = np.array([c.np for c in cases])
+    log(f"ensemble: {n} cases, {sizes.sum()} particles ({sizes.min()}-{sizes.max()}), "
+        f"generated in {time.perf_counter() - t0:.1f}s")
+    ens = Ensemble(cases, tend=tend)
+    chosen, tuned = ens.autotune(reps=2)
+    log(f"  autotuner: {chosen}")
+    ens.run(max_steps=0)  # graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    steps = ens.run(check_every=4)
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    states = [ens.state(k) for k in range(n)]
+    info = ens.info()
+    ens.close()
+    stopped = sum(1 for s in states if not s["active"] and s["time"] >= tend)
+    part_steps = sum(float(s["np0"]) * s["nstep"] for s in states)
+    log(f"  ensemble: {steps} steps in {el:.2f}s, {stopped}/{n} cases reached t={tend}")
+    order = np.argsort(sizes, kind="stable")
+    pick = sorted(set(order[np.linspace(0, n - 1, min(sample, n)).round().astype(int)].tolist()))
+    times = []
+    for k in pick:
+        sim = SphGpuSingle(cases[k])
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        sim.run_until(tend)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t1)
+        sim.close()
+    plain_total = float(np.mean(times)) * n
+    log(f"  plain: {len(pick)} sampled cases, mean {np.mean(times)*1e3:.1f} ms/case, "
+        f"estimated {plain_total:.1f}s for all {n}")
+    return {"cases": n, "particles": int(sizes.sum()), "np_min": int(sizes.min()),
+            "np_max": int(sizes.max()), "tend_s": tend, "steps": steps, "seconds": el,
+            "cases_per_s": n / el, "plain_cases_per_s": n / plain_total,
+            "speedup": plain_total / el, "particle_steps_per_s": part_steps / el,
+            "cases_completed": stopped, "plain_sample": len(pick),
+            "plain_seconds_estimated": plain_total, "autotune": chosen,
+            "steps_per_graph": info["steps_per_graph"],
+            "note": "plain = one SphGpuSingle run per case to the same end time "
+                    "(size-stratified sample, mean time x cases)"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +139,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
     ap.add_argument("--forces-kernel", choices=["tile", "rows"], default="tile")
+    ap.add_argument("--ensemble", type=int, default=512, metavar="N",
+                    help="cases of the batched-ensemble measurement (0: skip)")
+    ap.add_argument("--ens-tend", type=float, default=0.01,
+                    help="physical end time of every ensemble case [s]")
+    ap.add_argument("--ens-plain-sample", type=int, default=32,
+                    help="cases timed on the plain single-case path")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (3 times): the known-size "
                          "kernel that calibrates the HBM byte counters of a profiler")
@@ -163,9 +225,13 @@ def main():
                 "note": "VALU-bound FP32 kernel; peak = MI355X FP32 vector peak (packed FP32); "
                         "traffic = rocprofv3 FETCH_SIZE/WRITE_SIZE per launch scaled by a 1 GiB "
                         "copy calibration (profiles/)"}
+    sim.close()
+    del sim
+    ens = None
+    if a.ensemble > 0:
+        ens = ensemble_bench(a.ensemble, a.ens_tend, a.ens_plain_sample)
     cpu = None
     if not a.no_cpu_baseline:
-        del sim
         log(f"cpu baseline: {a.cpu_steps} steps of the oracle ...")
         cpu = cpu_baseline(c, a.cpu_steps)
     out = {
@@ -181,6 +247,7 @@ def main():
                    "sim_time_s": st["timestep"]},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "ensemble": ens,
         "kernels_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3),
                            "avg_ms": round(v[1] / max(v[0], 1), 5)} for k, v in kt.items()},
     }

@@ -80,6 +80,12 @@ class DsphMlp(C.Structure):
                 ("feat_scale", C.c_float * 11), ("out_scale", C.c_float)]
 
 
+class DsphEnsState(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("np", "npbok", "active", "nstep", "flags", "p0", "np0",
+                                          "pad")] + [
+        (n, C.c_double) for n in ("time", "dt_last", "dt_next")]
+
+
 class DsphStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("np", "npb", "npbok", "nct", "nstep",
                                           "npf_out_total")] + [
@@ -166,9 +172,22 @@ SIGNATURES = {
     "dsph_sim_set_closure": [_vp, C.POINTER(DsphMlp)],
     "dsph_sim_closure_nu": [_vp, _vp, _vp],
     "dsph_sim_filter_probes": [_vp, _u, _vp, _d, _vp],
+    "dsph_ens_create": [_i, _u, C.POINTER(DsphCase), _vp, _vp, _vp, _vp, _vp, C.POINTER(_vp)],
+    "dsph_ens_destroy": [_vp],
+    "dsph_ens_last_error": [_vp],
+    "dsph_ens_run": [_vp, _u, _u, _vp],
+    "dsph_ens_set_graph": [_vp, _i],
+    "dsph_ens_case_state": [_vp, _u, C.POINTER(DsphEnsState)],
+    "dsph_ens_download": [_vp, _u, _vp, _vp, _vp, _vp, _vp],
+    "dsph_ens_autotune": [_vp, _u, _vp, _vp],
+    "dsph_ens_set_variant": [_vp, _u, C.c_int32],
+    "dsph_ens_info": [_vp, _vp],
+    "dsph_ens_set_timing": [_vp, _i],
+    "dsph_ens_context": [_vp],
 }
 RESTYPES = {"dsph_last_error": C.c_char_p, "dsph_sim_last_error": C.c_char_p,
-            "dsph_ctx_get_stream": _vp, "dsph_sim_ctx": _vp}
+            "dsph_ctx_get_stream": _vp, "dsph_sim_ctx": _vp, "dsph_ens_last_error": C.c_char_p,
+            "dsph_ens_context": _vp}
 
 _lib = None
 

@@ -66,8 +66,8 @@ def build_native(jobs: int = 8, verbose: bool = False) -> Path:
         if src.suffix == ".hip":
             cmds.append([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
         else:
-            cmds.append([HIPCC, "-O3", "-fPIC", "-std=c++17", f"-I{INCLUDE}", "-c", str(src),
-                         "-o", str(obj)])
+            # Host sources include the kernels' headers: same flags as the HIP files.
+            cmds.append([HIPCC, *HIP_FLAGS, "-c", str(src), "-o", str(obj)])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         for r in ex.map(_run, cmds):
             if verbose and r.stderr:

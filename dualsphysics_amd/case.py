@@ -137,6 +137,25 @@ def _lattice(nx0, nx1, ny0, ny1, nz0, nz1, dp):
     return pts, np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
 
 
+def _box_walls(nxt: int, nyt: int, nzt: int, layers: int, dp: float) -> np.ndarray:
+    """Wall lattice of an open-top box with interior [0,nxt)x[0,nyt)x[0,inf):
+    every point of [-layers, nxt+layers) x [-layers, nyt+layers) x
+    [-layers, nzt) outside the interior, in lattice order (x fastest, then y,
+    then z); built per z-plane (full plane below the floor, a ring above)."""
+    j, i = np.meshgrid(np.arange(-layers, nyt + layers), np.arange(-layers, nxt + layers),
+                       indexing="ij")
+    i, j = i.ravel(), j.ravel()
+    ring = ~((i >= 0) & (i < nxt) & (j >= 0) & (j < nyt))
+    planes_i = [i] * layers + [i[ring]] * nzt
+    planes_j = [j] * layers + [j[ring]] * nzt
+    planes_k = [np.full(len(i), k) for k in range(-layers, 0)] + \
+               [np.full(int(ring.sum()), k) for k in range(nzt)]
+    ii = np.concatenate(planes_i)
+    jj = np.concatenate(planes_j)
+    kk = np.concatenate(planes_k)
+    return np.stack([(ii + 0.5) * dp, (jj + 0.5) * dp, (kk + 0.5) * dp], 1)
+
+
 def _n(length, dp):
     return int(math.floor(length / dp + 0.5))
 
@@ -161,10 +180,7 @@ def dam_break(dp: float = 0.01, tank=(1.6, 0.67, 0.4), column=(0.4, 0.67, 0.3), 
     nxt, nyt, nzt = _n(tank[0], dp), _n(tank[1], dp), _n(tank[2], dp)
     nxf, nyf, nzf = _n(column[0], dp), _n(column[1], dp), _n(column[2], dp)
     # Walls: every lattice point of the widened box outside the interior.
-    wpts, widx = _lattice(-layers, nxt + layers, -layers, nyt + layers, -layers, nzt, dp)
-    inside = ((widx[:, 0] >= 0) & (widx[:, 0] < nxt) & (widx[:, 1] >= 0) & (widx[:, 1] < nyt) &
-              (widx[:, 2] >= 0))
-    wpts = wpts[~inside]
+    wpts = _box_walls(nxt, nyt, nzt, layers, dp)
     fpts, _ = _lattice(0, nxf, 0, nyf, 0, nzf, dp)
     nb, nf = wpts.shape[0], fpts.shape[0]
     pos = np.concatenate([wpts, fpts], 0)

@@ -10,6 +10,7 @@
 #include <climits>
 #include <cstring>
 #include "dsph_ctx.h"
+#include "ensemble.h"
 
 using namespace dsph;
 
@@ -68,13 +69,9 @@ __global__ void __launch_bounds__(kBs) k_limits(unsigned n, unsigned pini, unsig
 // Box layout of JCellDivGpuSingle::PrepareNct (JCellDivGpuSingle.cpp:109-128):
 // [bound cells 0..nct) | BoundIgnore=nct | fluid cells nct+1.. | BoundOut |
 // FluidOut | BoundOutIgnore | FluidOutIgnore.
-__global__ void __launch_bounds__(kBs) k_presort_full(unsigned np, unsigned cellcode,
-    const unsigned* __restrict__ dcell, const typecode* __restrict__ code,
-    uint3 cellzero, uint3 ncells, unsigned* __restrict__ cellpart,
-    unsigned* __restrict__ sortpart) {
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= np) return;
-  sortpart[p] = p;
+__device__ inline unsigned presort_full_box(unsigned p, unsigned cellcode,
+    const unsigned* __restrict__ dcell, const typecode* __restrict__ code, uint3 cellzero,
+    uint3 ncells) {
   const unsigned nsheet = ncells.x * ncells.y;
   const unsigned boxboundignore = nsheet * ncells.z;
   const unsigned boxfluid = boxboundignore + 1;
@@ -98,7 +95,17 @@ __global__ void __launch_bounds__(kBs) k_presort_full(unsigned np, unsigned cell
     box = (out <= kCodeOutIgnore) ? (out < kCodeOutIgnore ? boxfluid + key : boxfluidoutignore)
                                   : (type == kCodeTypeFloating ? boxboundout : boxfluidout);
   }
-  cellpart[p] = box;
+  return box;
+}
+
+__global__ void __launch_bounds__(kBs) k_presort_full(unsigned np, unsigned cellcode,
+    const unsigned* __restrict__ dcell, const typecode* __restrict__ code,
+    uint3 cellzero, uint3 ncells, unsigned* __restrict__ cellpart,
+    unsigned* __restrict__ sortpart) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  sortpart[p] = p;
+  cellpart[p] = presort_full_box(p, cellcode, dcell, code, cellzero, ncells);
 }
 
 __global__ void __launch_bounds__(kBs) k_presort_fluid(unsigned n, unsigned pini,
@@ -351,7 +358,108 @@ __global__ void __launch_bounds__(kBs) k_update_poscell(unsigned np, double3 pos
   poscell[p] = poscell_of(xy.x, xy.y, posz[p], posmin, pcs);
 }
 
+// ---------------------------------------------------------------- ensembles
+// Keys of every particle of every case: the case's box key (full divide over
+// its fixed grid) plus the case's key base.
+__global__ void __launch_bounds__(kBs) k_ens_presort(const EnsCase* __restrict__ cases,
+    const EnsBlock* __restrict__ pmap, const unsigned* __restrict__ dcell,
+    const typecode* __restrict__ code, unsigned* __restrict__ cellpart,
+    unsigned* __restrict__ sortpart) {
+  const EnsBlock e = pmap[blockIdx.x];
+  const EnsCase& ec = cases[e.x];
+  const unsigned l = e.y + threadIdx.x;
+  if (l >= ec.np0) return;
+  const unsigned p = ec.p0 + l;
+  sortpart[p] = p;
+  cellpart[p] = ec.key0 + presort_full_box(p, ec.c.cellcode, dcell, code, make_uint3(0, 0, 0),
+                                           make_uint3(ec.ncx, ec.ncy, ec.ncz));
+}
+
+// NpFinal and NpbOk of each case from the sizes of its out boxes
+// (JCellDivGpuSingle.cpp:218-236).
+__global__ void k_ens_counts(const EnsCase* __restrict__ cases, EnsState* __restrict__ st,
+                             unsigned ncases, const int2* __restrict__ bc) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncases) return;
+  const EnsCase& ec = cases[k];
+  const int2* b = bc + ec.key0;
+  auto size = [&](unsigned box) { return unsigned(b[box].y - b[box].x); };
+  const unsigned boundignore = ec.nct, boundout = 2 * ec.nct + 1;
+  const unsigned nbout = size(boundout), nfout = size(boundout + 1);
+  const unsigned nboutign = size(boundout + 2), nfoutign = size(boundout + 3);
+  EnsState& s = st[k];
+  s.np = ec.np0 - nbout - nfout - nboutign - nfoutign;
+  s.npbok = ec.npb - nboutign - size(boundignore);
+  if (nbout || nboutign) {
+    s.flags |= 4u;
+    s.active = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBs) k_ens_sort_basic(const EnsCase* __restrict__ cases,
+    const EnsBlock* __restrict__ pmap, const unsigned* __restrict__ sortpart,
+    const unsigned* __restrict__ idp, const typecode* __restrict__ code,
+    const unsigned* __restrict__ dcell, const double2* __restrict__ posxy,
+    const double* __restrict__ posz, const float4* __restrict__ velrho,
+    unsigned* __restrict__ idp2, typecode* __restrict__ code2, unsigned* __restrict__ dcell2,
+    double2* __restrict__ posxy2, double* __restrict__ posz2, float4* __restrict__ velrho2,
+    float4* __restrict__ poscell) {
+  const EnsBlock e = pmap[blockIdx.x];
+  const EnsCase& ec = cases[e.x];
+  const unsigned l = e.y + threadIdx.x;
+  if (l >= ec.np0) return;
+  const unsigned p = ec.p0 + l;
+  const unsigned o = sortpart[p];
+  idp2[p] = idp[o];
+  code2[p] = code[o];
+  dcell2[p] = dcell[o];
+  const double2 xy = posxy[o];
+  const double z = posz[o];
+  posxy2[p] = xy;
+  posz2[p] = z;
+  velrho2[p] = velrho[o];
+  poscell[p] = poscell_of(xy.x, xy.y, z, make_double3(ec.posminx, ec.posminy, ec.posminz),
+                          ec.c.pcs);
+}
+
 }  // namespace
+
+namespace dsph {
+
+int ens_presort(dsph_ctx* ctx, const EnsCase* cases, const EnsBlock* pmap, unsigned npblocks,
+                const unsigned* dcell, const uint16_t* code, unsigned* cellpart,
+                unsigned* sortpart) {
+  if (!npblocks) return DSPH_OK;
+  KTimer kt(ctx, "k_presort");
+  k_ens_presort<<<npblocks, kBs, 0, ctx->stream>>>(cases, pmap, dcell, code, cellpart, sortpart);
+  DSPH_LAUNCHED(ctx, "k_ens_presort");
+  return DSPH_OK;
+}
+
+int ens_counts(dsph_ctx* ctx, const EnsCase* cases, EnsState* st, unsigned ncases,
+               const int2* begincell) {
+  if (!ncases) return DSPH_OK;
+  k_ens_counts<<<grid_for(ncases, 64), 64, 0, ctx->stream>>>(cases, st, ncases, begincell);
+  DSPH_LAUNCHED(ctx, "k_ens_counts");
+  return DSPH_OK;
+}
+
+int ens_sort_basic_poscell(dsph_ctx* ctx, const EnsCase* cases, const EnsBlock* pmap,
+                           unsigned npblocks, const unsigned* sortpart, const unsigned* idp,
+                           const uint16_t* code, const unsigned* dcell, const double2* posxy,
+                           const double* posz, const float4* velrho, unsigned* idp2,
+                           uint16_t* code2, unsigned* dcell2, double2* posxy2, double* posz2,
+                           float4* velrho2, float4* poscell) {
+  if (!npblocks) return DSPH_OK;
+  KTimer kt(ctx, "k_sort_basic_poscell");
+  k_ens_sort_basic<<<npblocks, kBs, 0, ctx->stream>>>(cases, pmap, sortpart, idp, code, dcell,
+                                                      posxy, posz, velrho, idp2, code2, dcell2,
+                                                      posxy2, posz2, velrho2, poscell);
+  DSPH_LAUNCHED(ctx, "k_ens_sort_basic");
+  return DSPH_OK;
+}
+
+}  // namespace dsph
 
 extern "C" {
 

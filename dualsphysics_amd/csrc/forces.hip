@@ -21,6 +21,7 @@
 //    order is unchanged.
 #include <cfloat>
 #include <cstring>
+#include "ensemble.h"
 #include "sph_math.h"
 
 using namespace dsph;
@@ -34,11 +35,8 @@ constexpr unsigned kBs = 256;
 // the same value the reference computes per pair), the reciprocal density
 // (v_rcp_f32, as a fast-math quotient by rho computes it), and float positions
 // in the map frame for the conservative neighbour test.
-__global__ void __launch_bounds__(kBs) k_prep(unsigned n, const float4* __restrict__ poscell,
-    const float4* __restrict__ velrho, float2* __restrict__ press, float4* __restrict__ posf,
-    Cte c) {
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
+__device__ inline void prep_one(unsigned p, const Cte& c, const float4* __restrict__ poscell,
+    const float4* __restrict__ velrho, float2* __restrict__ press, float4* __restrict__ posf) {
   const float rho = velrho[p].w;
   press[p] = make_float2(eos_press(rho, c), fast_rcp(rho));
   if (posf) {
@@ -48,6 +46,14 @@ __global__ void __launch_bounds__(kBs) k_prep(unsigned n, const float4* __restri
                           float(double(pscel_fy(ps.w)) * pcs + double(ps.y)),
                           float(double(pscel_fz(ps.w)) * pcs + double(ps.z)), 0.f);
   }
+}
+
+__global__ void __launch_bounds__(kBs) k_prep(unsigned n, const float4* __restrict__ poscell,
+    const float4* __restrict__ velrho, float2* __restrict__ press, float4* __restrict__ posf,
+    Cte c) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  prep_one(p, c, poscell, velrho, press, posf);
 }
 
 // XCD-aware block order. The dispatcher sends block i to XCD i % 8; each
@@ -227,15 +233,12 @@ __device__ inline NgLimits p1_limits(unsigned p1, const unsigned* __restrict__ d
 
 // ------------------------------------------------------------ reference-order rows kernel
 template <int TVISCO, int TDDT, bool VELGRAD>
-__global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
-    unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c,
-    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
-    const float4* __restrict__ velrho, const float2* __restrict__ press,
-    const float* __restrict__ tau, float* __restrict__ viscdt, float* __restrict__ ar,
-    float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad) {
-  const unsigned p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const unsigned p1 = p + pinit;
+__device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float viscof,
+    const DivData& dvd, const Cte& c, const unsigned* __restrict__ dcell,
+    const float4* __restrict__ poscell, const float4* __restrict__ velrho,
+    const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
+    float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
+    float* __restrict__ velgrad) {
   Acc a;
   acc_init(a);
   P1 q;
@@ -253,6 +256,19 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
       }
   }
   store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
+}
+
+template <int TVISCO, int TDDT, bool VELGRAD>
+__global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
+    unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ velrho, const float2* __restrict__ press,
+    const float* __restrict__ tau, float* __restrict__ viscdt, float* __restrict__ ar,
+    float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad) {
+  const unsigned p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  fluid_rows_body<TVISCO, TDDT, VELGRAD>(p + pinit, viscob, viscof, dvd, c, dcell, poscell,
+                                         velrho, press, tau, viscdt, ar, ace, strain, velgrad);
 }
 
 // ------------------------------------------------------------ cell-row tile kernel
@@ -309,20 +325,26 @@ __device__ inline unsigned long long bit_range(unsigned k0, unsigned k1) {
 // and bit order, i.e. in the reference's summation order: the results equal
 // k_forces_fluid_rows bit for bit. Loads of the next pair are issued before
 // the arithmetic of the current one.
+// LDS of one wave of the tile kernel.
+template <unsigned kRing>
+struct TileLds {
+  float4 tile[kTile];
+  unsigned long long masks[kRing][64];
+  unsigned tbase[kRing];
+};
+
 template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing>
-__global__ void __launch_bounds__(64) k_forces_fluid_tile(
-    unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c, float ks2scan,
+__device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned pinit,
+    float viscob, float viscof, const DivData& dvd, const Cte& c, float ks2scan,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ posf, const float4* __restrict__ velrho,
     const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
     float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
-    float* __restrict__ velgrad) {
-  __shared__ float4 tile[kTile];
-  __shared__ unsigned long long masks[kRing][64];
-  __shared__ unsigned tbase[kRing];
+    float* __restrict__ velgrad, TileLds<kRing>& lds) {
+  float4* tile = lds.tile;
+  auto& masks = lds.masks;
+  unsigned* tbase = lds.tbase;
   const unsigned lane = threadIdx.x;
-  const unsigned p = xcd_block(blockIdx.x, gridDim.x) * 64 + lane;
-  const bool valid = p < n;
   const unsigned p1 = (valid ? p : 0) + pinit;
   Acc a;
   acc_init(a);
@@ -484,13 +506,26 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
   if (valid) store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
 }
 
+template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing>
+__global__ void __launch_bounds__(64) k_forces_fluid_tile(
+    unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c, float ks2scan,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ posf, const float4* __restrict__ velrho,
+    const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
+    float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
+    float* __restrict__ velgrad) {
+  __shared__ TileLds<kRing> lds;
+  const unsigned p = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
+  fluid_tile_body<TVISCO, TDDT, VELGRAD, kRing>(p, p < n, pinit, viscob, viscof, dvd, c, ks2scan,
+                                                dcell, poscell, posf, velrho, press, tau, viscdt,
+                                                ar, ace, strain, velgrad, lds);
+}
+
 // Boundary particle p1 against fluid neighbours (KerInteractionForcesBound,
 // JSphGpu_ker.cu:450-533).
-__global__ void __launch_bounds__(kBsForces) k_forces_bound(unsigned n, DivData dvd, Cte c,
+__device__ __forceinline__ void bound_body(unsigned p1, const DivData& dvd, const Cte& c,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar) {
-  const unsigned p1 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (p1 >= n) return;
   float visc = 0, arp1 = 0;
   const float4 ps1 = poscell[p1];
   const float4 vr1 = velrho[p1];
@@ -522,16 +557,18 @@ __global__ void __launch_bounds__(kBsForces) k_forces_bound(unsigned n, DivData 
   }
 }
 
+__global__ void __launch_bounds__(kBsForces) k_forces_bound(unsigned n, DivData dvd, Cte c,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar) {
+  const unsigned p1 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (p1 >= n) return;
+  bound_body(p1, dvd, c, dcell, poscell, velrho, viscdt, ar);
+}
+
 // Radius^2 of the conservative scan test: (2h + slack)^2 where the slack
 // bounds the rounding of map-frame float positions (each coordinate within
 // half an ulp of the largest map extent, PosCell within half an ulp of pcs).
-float scan_radius2(const dsph_ctx* ctx, const Cte& c) {
-  const double lmax = std::max(std::max(ctx->cte.maprealsizex, ctx->cte.maprealsizey),
-                               ctx->cte.maprealsizez) + 2.0 * c.pcs;
-  const double ulp = lmax * 1.1920928955078125e-7;
-  const double r = std::sqrt(double(c.ks2)) + 4.0 * ulp + 1e-6 * std::sqrt(double(c.ks2));
-  return float(r * r);
-}
+float scan_radius2(const dsph_ctx* ctx, const Cte&) { return dsph::ens_scan_radius2(ctx->cte); }
 
 template <int TVISCO, int TDDT, bool VELGRAD>
 void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, const Cte& c,
@@ -567,7 +604,158 @@ int dispatch_ddt(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, c
   return DSPH_OK;
 }
 
+// ---------------------------------------------------------------- ensembles
+// The same bodies over the cases of an ensemble; the block map gives the case
+// and the first case-local index of each workgroup.
+__global__ void __launch_bounds__(kBs) k_ens_prep(const EnsCase* __restrict__ cases,
+    const EnsState* __restrict__ st, const EnsBlock* __restrict__ pmap,
+    const float4* __restrict__ poscell, const float4* __restrict__ velrho,
+    float2* __restrict__ press, float4* __restrict__ posf) {
+  const EnsBlock e = pmap[blockIdx.x];
+  if (!st[e.x].active) return;
+  const unsigned l = e.y + threadIdx.x;
+  if (l >= st[e.x].np) return;
+  const EnsCase& ec = cases[e.x];
+  prep_one(ec.p0 + l, ec.c, poscell, velrho, press, posf);
+}
+
+template <int TVISCO, int TDDT, unsigned kRing>
+__global__ void __launch_bounds__(64) k_ens_fluid_tile(const EnsCase* __restrict__ cases,
+    const EnsState* __restrict__ st, const EnsBlock* __restrict__ wmap,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ posf, const float4* __restrict__ velrho,
+    const float2* __restrict__ press, float* __restrict__ viscdt, float* __restrict__ ar,
+    float* __restrict__ ace) {
+  __shared__ TileLds<kRing> lds;
+  const EnsBlock e = wmap[xcd_block(blockIdx.x, gridDim.x)];
+  if (!st[e.x].active) return;
+  const EnsCase& ec = cases[e.x];
+  const unsigned npf = st[e.x].np - ec.npb;
+  if (e.y >= npf) return;
+  const unsigned p = e.y + threadIdx.x;
+  fluid_tile_body<TVISCO, TDDT, false, kRing>(p, p < npf, ec.p0 + ec.npb, ec.viscob, ec.viscof,
+                                              ec.dvd, ec.c, ec.ks2scan, dcell, poscell, posf,
+                                              velrho, press, nullptr, viscdt, ar, ace, nullptr,
+                                              nullptr, lds);
+}
+
+template <int TVISCO, int TDDT>
+__global__ void __launch_bounds__(64) k_ens_fluid_rows(const EnsCase* __restrict__ cases,
+    const EnsState* __restrict__ st, const EnsBlock* __restrict__ wmap,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ velrho, const float2* __restrict__ press,
+    float* __restrict__ viscdt, float* __restrict__ ar, float* __restrict__ ace) {
+  const EnsBlock e = wmap[xcd_block(blockIdx.x, gridDim.x)];
+  if (!st[e.x].active) return;
+  const EnsCase& ec = cases[e.x];
+  const unsigned l = e.y + threadIdx.x;
+  if (l >= st[e.x].np - ec.npb) return;
+  fluid_rows_body<TVISCO, TDDT, false>(ec.p0 + ec.npb + l, ec.viscob, ec.viscof, ec.dvd, ec.c,
+                                       dcell, poscell, velrho, press, nullptr, viscdt, ar, ace,
+                                       nullptr, nullptr);
+}
+
+__global__ void __launch_bounds__(kBsForces) k_ens_bound(const EnsCase* __restrict__ cases,
+    const EnsState* __restrict__ st, const EnsBlock* __restrict__ bmap,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar) {
+  const EnsBlock e = bmap[xcd_block(blockIdx.x, gridDim.x)];
+  if (!st[e.x].active) return;
+  const unsigned l = e.y + threadIdx.x;
+  if (l >= st[e.x].npbok) return;
+  const EnsCase& ec = cases[e.x];
+  bound_body(ec.p0 + l, ec.dvd, ec.c, dcell, poscell, velrho, viscdt, ar);
+}
+
+template <int TVISCO, int TDDT>
+void launch_ens_fluid(dsph_ctx* ctx, int variant, const EnsCase* cases, const EnsState* st,
+                      const EnsBlock* wmap, unsigned nb, const unsigned* dcell,
+                      const float4* poscell, const float4* posf, const float4* velrho,
+                      const float2* press, float* viscdt, float* ar, float* ace) {
+  switch (variant) {
+    case kEnsTileRing8:
+      k_ens_fluid_tile<TVISCO, TDDT, 8><<<nb, 64, 0, ctx->stream>>>(
+          cases, st, wmap, dcell, poscell, posf, velrho, press, viscdt, ar, ace);
+      break;
+    case kEnsTileRing12:
+      k_ens_fluid_tile<TVISCO, TDDT, 12><<<nb, 64, 0, ctx->stream>>>(
+          cases, st, wmap, dcell, poscell, posf, velrho, press, viscdt, ar, ace);
+      break;
+    case kEnsTileRing16:
+      k_ens_fluid_tile<TVISCO, TDDT, 16><<<nb, 64, 0, ctx->stream>>>(
+          cases, st, wmap, dcell, poscell, posf, velrho, press, viscdt, ar, ace);
+      break;
+    default:
+      k_ens_fluid_rows<TVISCO, TDDT><<<nb, 64, 0, ctx->stream>>>(
+          cases, st, wmap, dcell, poscell, velrho, press, viscdt, ar, ace);
+      break;
+  }
+}
+
 }  // namespace
+
+namespace dsph {
+
+int ens_prep(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st, const EnsBlock* pmap,
+             unsigned npblocks, const float4* poscell, const float4* velrho, float2* press,
+             float4* posf) {
+  if (!npblocks) return DSPH_OK;
+  KTimer kt(ctx, "k_prep");
+  k_ens_prep<<<npblocks, kBs, 0, ctx->stream>>>(cases, st, pmap, poscell, velrho, press, posf);
+  DSPH_LAUNCHED(ctx, "k_ens_prep");
+  return DSPH_OK;
+}
+
+int ens_forces_fluid(dsph_ctx* ctx, int variant, int tvisco, int tddt, const EnsCase* cases,
+                     const EnsState* st, const EnsBlock* wmap, unsigned nblocks,
+                     const unsigned* dcell, const float4* poscell, const float4* posf,
+                     const float4* velrho, const float2* press, float* viscdt, float* ar,
+                     float* ace) {
+  if (!nblocks) return DSPH_OK;
+  DSPH_REQUIRE(ctx, variant >= 0 && variant < kEnsNumVariants, "ensemble: invalid kernel variant");
+  KTimer kt(ctx, "k_forces_fluid");
+#define DSPH_ENS_FLUID(V, D)                                                              \
+  launch_ens_fluid<V, D>(ctx, variant, cases, st, wmap, nblocks, dcell, poscell, posf, velrho, \
+                         press, viscdt, ar, ace)
+  if (tvisco == DSPH_VISCO_ARTIFICIAL && tddt == DSPH_DDT_DDT2FULL)
+    DSPH_ENS_FLUID(DSPH_VISCO_ARTIFICIAL, DSPH_DDT_DDT2FULL);
+  else if (tvisco == DSPH_VISCO_ARTIFICIAL && tddt == DSPH_DDT_NONE)
+    DSPH_ENS_FLUID(DSPH_VISCO_ARTIFICIAL, DSPH_DDT_NONE);
+  else if (tvisco == DSPH_VISCO_LAMINAR && tddt == DSPH_DDT_DDT2FULL)
+    DSPH_ENS_FLUID(DSPH_VISCO_LAMINAR, DSPH_DDT_DDT2FULL);
+  else if (tvisco == DSPH_VISCO_LAMINAR && tddt == DSPH_DDT_NONE)
+    DSPH_ENS_FLUID(DSPH_VISCO_LAMINAR, DSPH_DDT_NONE);
+  else
+    return set_error(ctx, DSPH_E_ARG,
+                     "ensembles are built for artificial or laminar viscosity with "
+                     "DDT2Full or no density diffusion");
+#undef DSPH_ENS_FLUID
+  DSPH_LAUNCHED(ctx, "k_ens_fluid");
+  return DSPH_OK;
+}
+
+int ens_forces_bound(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st,
+                     const EnsBlock* bmap, unsigned nblocks, const unsigned* dcell,
+                     const float4* poscell, const float4* velrho, float* viscdt, float* ar) {
+  if (!nblocks) return DSPH_OK;
+  KTimer kt(ctx, "k_forces_bound");
+  k_ens_bound<<<nblocks, kBsForces, 0, ctx->stream>>>(cases, st, bmap, dcell, poscell, velrho,
+                                                     viscdt, ar);
+  DSPH_LAUNCHED(ctx, "k_ens_bound");
+  return DSPH_OK;
+}
+
+// Radius^2 of the conservative scan test for a case (see scan_radius2).
+float ens_scan_radius2(const dsph_cte& cte) {
+  const Cte c = make_cte(cte);
+  const double lmax = std::max(std::max(cte.maprealsizex, cte.maprealsizey), cte.maprealsizez) +
+                      2.0 * c.pcs;
+  const double ulp = lmax * 1.1920928955078125e-7;
+  const double r = std::sqrt(double(c.ks2)) + 4.0 * ulp + 1e-6 * std::sqrt(double(c.ks2));
+  return float(r * r);
+}
+
+}  // namespace dsph
 
 extern "C" {
 

@@ -6,666 +6,7 @@
 // ComputeStep_Sym / ComputeStep_Ver + RunCellDivide in the reference order.
 //
 // This file talks to the device only through dsph_* entry points.
-#include <algorithm>
-#include <cfloat>
-#include <climits>
-#include <cmath>
-#include <cstring>
-#include <stdexcept>
-#include <string>
-#include <vector>
-#include "dsph_abi.h"
-
-namespace {
-
-struct u3 { unsigned x, y, z; };
-inline bool operator!=(const u3& a, const u3& b) { return a.x != b.x || a.y != b.y || a.z != b.z; }
-
-// DCEL helpers (JDsDcellDef.h:28-45; JDsDcell.cpp:30-67).
-unsigned calc_bits(unsigned v, unsigned minbits) {
-  unsigned n = (minbits < 1 || minbits > 30) ? 1 : minbits;
-  for (; v >> n; n++) {}
-  return n;
-}
-unsigned calc_cell_code(u3 nc) {
-  unsigned sx = calc_bits(nc.x, 2), sy = calc_bits(nc.y, 2), sz = calc_bits(nc.z, 2);
-  const unsigned smin = sx + sy + sz;
-  const unsigned maxbits = 31;
-  if (smin > maxbits) return 0;
-  unsigned rest = maxbits - smin;
-  while (rest) {
-    if (rest) { sx++; rest--; }
-    if (rest) { sy++; rest--; }
-    if (rest) { sz++; rest--; }
-  }
-  return ((sx + 1) << 25) | (sy << 20) | (sz << 15) | ((sy + sz) << 10) | ((sx + 1 + sz) << 5) |
-         (sx + 1 + sy);
-}
-inline unsigned dcel_cell(unsigned dcc, unsigned cx, unsigned cy, unsigned cz) {
-  return (cx << ((dcc >> 10) & 31u)) | (cy << ((dcc >> 15) & 31u)) | cz;
-}
-
-struct Err : std::runtime_error {
-  int code;
-  Err(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-template <typename T>
-struct DevArr {
-  T* p = nullptr;
-  size_t n = 0;
-};
-
-}  // namespace
-
-namespace dsph {
-void rect_sinu_mov_host(const dsph_motion* m, double t, double dt, double mv[3]);
-}
-
-struct dsph_sim {
-  dsph_ctx* ctx = nullptr;
-  dsph_case cs{};
-  std::string err;
-
-  // ---- constants (JSph members)
-  float KernelH = 0, KernelSize = 0, KernelSize2 = 0, Scell = 0, MovLimit = 0, PosCellSize = 0;
-  float Eta2 = 0, awen = 0, bwenh = 0, DDTkh = 0, DDTgz = 0, CteB = 0, Gamma = 0, RhopZero = 0;
-  float MassFluid = 0, MassBound = 0, Dp = 0, Visco = 0, ViscoBoundFactor = 1;
-  float RhopOutMin = 700, RhopOutMax = 1300, SpsSmag = 0, SpsBlin = 0;
-  float Gravity[3] = {0, 0, 0};
-  double Cs0 = 0, DtIni = 0, DtMin = 0, CFLnumber = 0.2;
-  int TStep = 2, VerletSteps = 40, TVisco = 1, TDensity = 0, TBoundary = 1, ScellDiv = 1;
-  bool CellDomFixed = false;
-  double MapRealPosMin[3], MapRealPosMax[3], MapRealSize[3];
-  double Map_PosMin[3], Map_PosMax[3], Map_Size[3];
-  u3 Map_Cells{}, DomCells{};
-  double DomPosMin[3];
-  unsigned DomCellCode = 0;
-  unsigned CaseNbound = 0, CaseNpb = 0;
-
-  // ---- particle state
-  unsigned Np = 0, Npb = 0, NpbOk = 0, NpCap = 0;
-  unsigned *idp = nullptr, *dcell = nullptr, *idp2 = nullptr, *dcell2 = nullptr;
-  uint16_t *code = nullptr, *code2 = nullptr;
-  double *posxy = nullptr, *posz = nullptr, *posxy2 = nullptr, *posz2 = nullptr;
-  float *velrho = nullptr, *velrho2 = nullptr, *poscell = nullptr;
-  double *posxypre = nullptr, *poszpre = nullptr, *posxypre2 = nullptr, *poszpre2 = nullptr;
-  float *velrhopre = nullptr, *velrhopre2 = nullptr, *velrhom1 = nullptr, *velrhom1b = nullptr;
-  float *ace = nullptr, *ar = nullptr, *viscdt = nullptr;
-  double *movxy = nullptr, *movz = nullptr;
-  float *sps2strain = nullptr, *spstau = nullptr, *spstau2 = nullptr;
-  float *boundnor = nullptr, *boundnor2 = nullptr;
-  bool pre_active = false;
-  // device scalars: red[0..2] (velmax^2, viscdtmax, acemax^2), dt slots, flags
-  float* d_red = nullptr;
-  double* d_dt = nullptr;   // [0]=dt_p [1]=dt_c [2]=SymplecticDtPre [3]=time
-  unsigned* d_flags = nullptr;
-
-  // ---- cell division (JCellDivGpuSingle state)
-  unsigned *cellpart = nullptr, *sortpart = nullptr;
-  int* begincell = nullptr;
-  size_t begincell_cap = 0;
-  u3 CellDomainMin{}, CellDomainMax{};
-  unsigned Ncx = 0, Ncy = 0, Ncz = 0, Nsheet = 0, Nct = 0, Nctt = 0;
-  unsigned BoxBoundIgnore = 0, BoxFluid = 0, BoxBoundOut = 0;
-  bool BoundLimitOk = false, BoundDivideOk = false, DivideFull = false;
-  u3 BoundLimitCellMin{}, BoundLimitCellMax{}, BoundDivideCellMin{}, BoundDivideCellMax{};
-  unsigned NpFinal = 0, NpbFinal = 0, NpbIgnore = 0, NpbOut = 0, NpfOut = 0;
-  bool BoundChanged = true;
-  // Learned SPS closure (optional): network and the velocity-gradient buffer.
-  dsph_mlp Closure{};
-  bool UseClosure = false;
-  float* velgrad = nullptr;
-  float* closure_nu = nullptr;
-
-  // ---- run state
-  double SymplecticDtPre = 0, TimeStep = 0, DtLast = 0;
-  double VelMax = 0, AceMax = 0;
-  float ViscDtMax = 0;
-  unsigned Nstep = 0, VerletStep = 0, Ndiv = 0, NdivFull = 0, NpfOutTotal = 0;
-  bool device_dt = false;
-
-  ~dsph_sim() { release(); }
-
-  void chk(int rc, const char* what) {
-    if (rc != DSPH_OK) throw Err(rc, std::string(what) + ": " + dsph_last_error(ctx));
-  }
-  template <typename T>
-  T* alloc(size_t n) {
-    void* p = nullptr;
-    chk(dsph_malloc(ctx, sizeof(T) * std::max<size_t>(n, 1), &p), "dsph_malloc");
-    return static_cast<T*>(p);
-  }
-  void release() {
-    if (!ctx) return;
-    void* ptrs[] = {idp, dcell, idp2, dcell2, code, code2, posxy, posz, posxy2, posz2, velrho,
-                    velrho2, poscell, posxypre, poszpre, posxypre2, poszpre2, velrhopre,
-                    velrhopre2, velrhom1, velrhom1b, ace, ar, viscdt, movxy, movz, sps2strain,
-                    spstau, spstau2, boundnor, boundnor2, d_red, d_dt, d_flags, cellpart,
-                    sortpart, begincell, velgrad, closure_nu};
-    for (void* p : ptrs) dsph_free(ctx, p);
-    dsph_ctx_destroy(ctx);
-    ctx = nullptr;
-  }
-
-  // ---- configuration (JSph::ConfigConstants1/2, JSph.cpp:1481-1546)
-  void config(const dsph_case& c) {
-    cs = c;
-    Dp = float(c.dp);
-    KernelH = float(c.kernelh);
-    CteB = float(c.cteb);
-    Gamma = float(c.gamma);
-    RhopZero = float(c.rhopzero);
-    MassFluid = float(c.massfluid);
-    MassBound = float(c.massbound);
-    for (int k = 0; k < 3; k++) Gravity[k] = float(c.gravity[k]);
-    CFLnumber = c.cflnumber;
-    TStep = c.tstep;
-    VerletSteps = c.verletsteps ? c.verletsteps : 40;
-    TVisco = c.tvisco;
-    TDensity = c.tdensity;
-    TBoundary = c.tboundary;
-    Visco = float(c.visco);
-    ViscoBoundFactor = float(c.viscoboundfactor);
-    RhopOutMin = float(c.rhopoutmin);
-    RhopOutMax = float(c.rhopoutmax);
-    ScellDiv = (c.cellmode == 2 ? 2 : 1);
-    CellDomFixed = c.celldomfixed != 0;
-    if (c.tkernel != DSPH_KERNEL_WENDLAND) throw Err(DSPH_E_ARG, "only Wendland kernel is built");
-    const double h = KernelH;
-    KernelSize = float(h * 2.0);
-    KernelSize2 = KernelSize * KernelSize;
-    awen = float(0.41778 / (h * h * h));
-    bwenh = float(-2.08891 / (h * h * h * h * h));
-    Cs0 = std::sqrt(double(Gamma) * double(CteB) / double(RhopZero));
-    Eta2 = float((h * 0.1) * (h * 0.1));
-    if (TVisco == DSPH_VISCO_LAMINARSPS) {
-      const double dp_sps = std::sqrt(double(Dp) * double(Dp) * 3.) / 3.;
-      SpsSmag = float((2. / 3.) * std::pow(0.12 * dp_sps, 2));
-      SpsBlin = float((2. / 3.) * 0.0066 * dp_sps * dp_sps);
-    }
-    DDTkh = KernelSize * float(c.ddtvalue);
-    DDTgz = float(double(RhopZero) * double(std::fabs(Gravity[2])) / double(CteB));
-    DtIni = c.dtini ? c.dtini : double(KernelH) / Cs0;
-    DtMin = c.dtmin ? c.dtmin : (double(KernelH) / Cs0) * double(0.05f);
-    CaseNbound = c.casenbound;
-    CaseNpb = c.casenfixed + c.casenmoving;
-    for (int k = 0; k < 3; k++) {
-      MapRealPosMin[k] = c.maprealposmin[k];
-      MapRealPosMax[k] = c.maprealposmax[k];
-      MapRealSize[k] = MapRealPosMax[k] - MapRealPosMin[k];
-      Map_PosMin[k] = MapRealPosMin[k];
-      Map_PosMax[k] = MapRealPosMax[k];
-      Map_Size[k] = Map_PosMax[k] - Map_PosMin[k];
-    }
-    // ConfigPosCellGpu (JSph.cpp:1949-2014): 13/10/9 bits for PosCell.
-    PosCellSize = KernelSize;
-    {
-      int nks = 1;
-      auto ncell = [&](double s) {
-        return u3{unsigned(std::ceil(Map_Size[0] / s)), unsigned(std::ceil(Map_Size[1] / s)),
-                  unsigned(std::ceil(Map_Size[2] / s))};
-      };
-      u3 n = ncell(PosCellSize);
-      while (n.x > 8192 || n.y > 1024 || n.z > 512) {
-        nks = (nks == 1 ? 2 : nks + 2);
-        PosCellSize = KernelSize * nks;
-        n = ncell(PosCellSize);
-      }
-    }
-    // ConfigCellDivision (JSph.cpp:1877-1895).
-    Scell = KernelSize / ScellDiv;
-    MovLimit = Scell * 0.9f;
-    Map_Cells = u3{unsigned(std::ceil(Map_Size[0] / Scell)), unsigned(std::ceil(Map_Size[1] / Scell)),
-                   unsigned(std::ceil(Map_Size[2] / Scell))};
-    // SelecDomain(0, Map_Cells) (JSph.cpp:1901-1943).
-    DomCells = Map_Cells;
-    for (int k = 0; k < 3; k++) DomPosMin[k] = Map_PosMin[k];
-    DomCellCode = calc_cell_code(u3{DomCells.x + 1, DomCells.y + 1, DomCells.z + 1});
-    if (!DomCellCode) throw Err(DSPH_E_ARG, "no valid cell code for the domain");
-  }
-
-  dsph_cte make_cte() const {
-    dsph_cte c{};
-    c.massb = MassBound; c.massf = MassFluid;
-    c.kernelh = KernelH; c.kernelsize2 = KernelSize2; c.poscellsize = PosCellSize;
-    c.awen = awen; c.bwenh = bwenh; c.cs0 = float(Cs0); c.eta2 = Eta2;
-    c.ddtkh = DDTkh; c.ddtgz = DDTgz; c.scell = Scell; c.kernelsize = KernelSize; c.dp = Dp;
-    c.cteb = CteB; c.gamma = Gamma; c.rhopzero = RhopZero; c.ovrhopzero = 1.f / RhopZero;
-    c.movlimit = MovLimit; c.tboundary = unsigned(TBoundary); c.cellcode = DomCellCode;
-    c.domposminx = DomPosMin[0]; c.domposminy = DomPosMin[1]; c.domposminz = DomPosMin[2];
-    c.maprealposminx = MapRealPosMin[0]; c.maprealposminy = MapRealPosMin[1];
-    c.maprealposminz = MapRealPosMin[2];
-    c.maprealsizex = MapRealSize[0]; c.maprealsizey = MapRealSize[1]; c.maprealsizez = MapRealSize[2];
-    return c;
-  }
-
-  // ---- particle upload (JSphGpuSingle::ConfigDomain, JSphGpuSingle.cpp:151-232)
-  void load(unsigned np, const unsigned* hidp, const double* hpos, const float* hvr,
-            const float* hbn) {
-    Np = np; Npb = CaseNpb; NpbOk = Npb;
-    NpCap = np;
-    std::vector<uint16_t> hcode(np);
-    std::vector<unsigned> hdcell(np);
-    std::vector<double> hxy(2 * size_t(np)), hz(np);
-    for (unsigned p = 0; p < np; p++) {
-      const unsigned id = hidp[p];
-      // LoadCodeParticles: fixed boundary (mk 0) and fluid (mk 0).
-      // Fixed boundary 0x0000, moving boundary CODE_TYPE_MOVING 0x0800 (mk 1),
-      // fluid 0x1800.
-      hcode[p] = uint16_t(id < cs.casenfixed ? 0x0000 : (id < CaseNbound ? 0x0801 : 0x1800));
-      const double x = hpos[3 * size_t(p)], y = hpos[3 * size_t(p) + 1], z = hpos[3 * size_t(p) + 2];
-      hxy[2 * size_t(p)] = x; hxy[2 * size_t(p) + 1] = y; hz[p] = z;
-      // LoadDcellParticles (JSph.cpp:1794-1816).
-      const bool in = x >= MapRealPosMin[0] && y >= MapRealPosMin[1] && z >= MapRealPosMin[2] &&
-                      x < MapRealPosMax[0] && y < MapRealPosMax[1] && z < MapRealPosMax[2];
-      if (!in) throw Err(DSPH_E_ARG, "Found new particles out.");
-      hdcell[p] = dcel_cell(DomCellCode, unsigned((x - DomPosMin[0]) / Scell),
-                            unsigned((y - DomPosMin[1]) / Scell), unsigned((z - DomPosMin[2]) / Scell));
-    }
-    const size_t n = NpCap;
-    idp = alloc<unsigned>(n); idp2 = alloc<unsigned>(n);
-    dcell = alloc<unsigned>(n); dcell2 = alloc<unsigned>(n);
-    code = alloc<uint16_t>(n); code2 = alloc<uint16_t>(n);
-    posxy = alloc<double>(2 * n); posxy2 = alloc<double>(2 * n);
-    posz = alloc<double>(n); posz2 = alloc<double>(n);
-    velrho = alloc<float>(4 * n); velrho2 = alloc<float>(4 * n);
-    poscell = alloc<float>(4 * n);
-    ace = alloc<float>(3 * n); ar = alloc<float>(n); viscdt = alloc<float>(n);
-    movxy = alloc<double>(2 * n); movz = alloc<double>(n);
-    cellpart = alloc<unsigned>(n); sortpart = alloc<unsigned>(n);
-    if (TStep == 2) {
-      posxypre = alloc<double>(2 * n); posxypre2 = alloc<double>(2 * n);
-      poszpre = alloc<double>(n); poszpre2 = alloc<double>(n);
-      velrhopre = alloc<float>(4 * n); velrhopre2 = alloc<float>(4 * n);
-    } else {
-      velrhom1 = alloc<float>(4 * n); velrhom1b = alloc<float>(4 * n);
-    }
-    if (TVisco == DSPH_VISCO_LAMINARSPS) {
-      sps2strain = alloc<float>(6 * n);
-      spstau = alloc<float>(6 * n); spstau2 = alloc<float>(6 * n);
-    }
-    if (TBoundary == DSPH_BC_MDBC) {
-      boundnor = alloc<float>(3 * n); boundnor2 = alloc<float>(3 * n);
-    }
-    d_red = alloc<float>(4);
-    d_dt = alloc<double>(8);
-    d_flags = alloc<unsigned>(4);
-    begincell_cap = 2ull * size_t(DomCells.x) * DomCells.y * DomCells.z + 5;
-    begincell = alloc<int>(2 * begincell_cap);
-    chk(dsph_ctx_reserve(ctx, NpCap, unsigned(begincell_cap)), "dsph_ctx_reserve");
-    chk(dsph_memcpy_h2d(ctx, idp, hidp, 4 * n), "upload idp");
-    chk(dsph_memcpy_h2d(ctx, code, hcode.data(), 2 * n), "upload code");
-    chk(dsph_memcpy_h2d(ctx, dcell, hdcell.data(), 4 * n), "upload dcell");
-    chk(dsph_memcpy_h2d(ctx, posxy, hxy.data(), 16 * n), "upload posxy");
-    chk(dsph_memcpy_h2d(ctx, posz, hz.data(), 8 * n), "upload posz");
-    chk(dsph_memcpy_h2d(ctx, velrho, hvr, 16 * n), "upload velrho");
-    if (velrhom1) chk(dsph_memcpy_h2d(ctx, velrhom1, hvr, 16 * n), "upload velrhom1");
-    if (spstau) chk(dsph_memset_async(ctx, spstau, 0, 24 * n), "memset tau");
-    if (boundnor) {
-      // ConfigBoundNormals: normals are stored doubled (JSph.cpp:1385-1434).
-      std::vector<float> bn(3 * n, 0.f);
-      if (hbn)
-        for (size_t k = 0; k < 3 * size_t(CaseNbound); k++) bn[k] = hbn[k] * 2.f;
-      chk(dsph_memcpy_h2d(ctx, boundnor, bn.data(), 12 * n), "upload boundnor");
-    }
-    std::vector<double> dt0(8, 0.0);
-    dt0[2] = DtIni;
-    chk(dsph_memcpy_h2d(ctx, d_dt, dt0.data(), 8 * 8), "upload dt");
-    unsigned fl[4] = {0, 0, 0, 0};
-    chk(dsph_memcpy_h2d(ctx, d_flags, fl, 16), "upload flags");
-    const dsph_cte cte = make_cte();
-    chk(dsph_set_constants(ctx, &cte), "dsph_set_constants");
-    SymplecticDtPre = DtIni;
-    BoundChanged = true;
-    run_cell_divide();
-  }
-
-  // ---- JCellDivGpuSingle::CalcCellDomain / MergeMapCellBoundFluid
-  void limits(unsigned n, unsigned pini, u3& cmin, u3& cmax) {
-    unsigned mn[3], mx[3];
-    chk(dsph_cell_limits_sync(ctx, n, pini, DomCellCode, dcell, code, mn, mx), "cell limits");
-    cmin = u3{mn[0], mn[1], mn[2]};
-    cmax = u3{mx[0], mx[1], mx[2]};
-  }
-  void calc_cell_domain(unsigned npb1, unsigned npf1) {
-    if (CellDomFixed) {
-      CellDomainMin = u3{0, 0, 0};
-      CellDomainMax = u3{DomCells.x - 1, DomCells.y - 1, DomCells.z - 1};
-      return;
-    }
-    u3 cbmin, cbmax;
-    if (!BoundLimitOk) {
-      u3 mn, mx;
-      limits(npb1, 0, mn, mx);
-      const bool none = mn.x > mx.x;
-      cbmin = none ? DomCells : mn;
-      cbmax = none ? u3{0, 0, 0} : mx;
-      BoundLimitOk = true; BoundLimitCellMin = cbmin; BoundLimitCellMax = cbmax;
-    } else {
-      cbmin = BoundLimitCellMin; cbmax = BoundLimitCellMax;
-    }
-    u3 cfmin, cfmax;
-    {
-      u3 mn, mx;
-      limits(npf1, npb1, mn, mx);
-      const bool none = mn.x > mx.x;
-      cfmin = none ? DomCells : mn;
-      cfmax = none ? u3{0, 0, 0} : mx;
-    }
-    const unsigned s = unsigned(ScellDiv);
-    auto lo = [&](unsigned b, unsigned f) { return std::max(std::min(b, f), f >= s ? f - s : 0u); };
-    auto hi = [&](unsigned b, unsigned f) { return std::min(std::max(b, f), f + s); };
-    CellDomainMin = u3{lo(cbmin.x, cfmin.x), lo(cbmin.y, cfmin.y), lo(cbmin.z, cfmin.z)};
-    CellDomainMax = u3{hi(cbmax.x, cfmax.x), hi(cbmax.y, cfmax.y), hi(cbmax.z, cfmax.z)};
-    if (CellDomainMax.x >= DomCells.x) CellDomainMax.x = DomCells.x - 1;
-    if (CellDomainMax.y >= DomCells.y) CellDomainMax.y = DomCells.y - 1;
-    if (CellDomainMax.z >= DomCells.z) CellDomainMax.z = DomCells.z - 1;
-    if (CellDomainMin.x > CellDomainMax.x || CellDomainMin.y > CellDomainMax.y ||
-        CellDomainMin.z > CellDomainMax.z)
-      CellDomainMin = CellDomainMax = u3{0, 0, 0};
-  }
-
-  unsigned cell_size(unsigned box) {
-    int be[2];
-    chk(dsph_cell_begin_end_sync(ctx, begincell, box, 1, be), "cell size");
-    return unsigned(be[1] - be[0]);
-  }
-
-  // ---- JCellDivGpuSingle::Divide (JCellDivGpuSingle.cpp:152-236)
-  void divide(unsigned npb1, unsigned npf1, bool boundchanged) {
-    const unsigned nptot = npb1 + npf1;
-    if (boundchanged) {
-      BoundLimitOk = BoundDivideOk = false;
-      BoundLimitCellMin = BoundLimitCellMax = u3{0, 0, 0};
-      BoundDivideCellMin = BoundDivideCellMax = u3{0, 0, 0};
-    }
-    calc_cell_domain(npb1, npf1);
-    Ncx = CellDomainMax.x - CellDomainMin.x + 1;
-    Ncy = CellDomainMax.y - CellDomainMin.y + 1;
-    Ncz = CellDomainMax.z - CellDomainMin.z + 1;
-    Nsheet = Ncx * Ncy;
-    Nct = Nsheet * Ncz;
-    Nctt = 2 * Nct + 5;
-    if (size_t(Nctt) > begincell_cap) throw Err(DSPH_E_NOMEM, "begincell capacity exceeded");
-    BoxBoundIgnore = Nct;
-    BoxFluid = BoxBoundIgnore + 1;
-    BoxBoundOut = BoxFluid + Nct;
-    if (!BoundDivideOk || BoundDivideCellMin != CellDomainMin || BoundDivideCellMax != CellDomainMax) {
-      DivideFull = true;
-      BoundDivideOk = true; BoundDivideCellMin = CellDomainMin; BoundDivideCellMax = CellDomainMax;
-      NdivFull++;
-    } else {
-      DivideFull = false;
-    }
-    const unsigned cmin[3] = {CellDomainMin.x, CellDomainMin.y, CellDomainMin.z};
-    const unsigned ncel[3] = {Ncx, Ncy, Ncz};
-    if (DivideFull)
-      chk(dsph_presort_full(ctx, nptot, DomCellCode, dcell, code, cmin, ncel, cellpart, sortpart),
-          "presort full");
-    else
-      chk(dsph_presort_fluid(ctx, nptot - npb1, npb1, DomCellCode, dcell, code, cmin, ncel,
-                             cellpart, sortpart),
-          "presort fluid");
-    if (DivideFull)
-      chk(dsph_sort_by_cell(ctx, cellpart, sortpart, nptot, Nctt), "sort");
-    else
-      chk(dsph_sort_by_cell(ctx, cellpart + npb1, sortpart + npb1, nptot - npb1, Nctt), "sort");
-    chk(dsph_begin_end_cell(ctx, DivideFull ? 1 : 0, nptot, npb1, Nctt, BoxFluid, cellpart, begincell),
-        "begin end cell");
-    NpbIgnore = cell_size(BoxBoundIgnore);
-    int be[8];
-    chk(dsph_cell_begin_end_sync(ctx, begincell, BoxBoundOut, 4, be), "out boxes");
-    NpbOut = unsigned(be[1] - be[0]);
-    NpfOut = unsigned(be[3] - be[2]);
-    const unsigned npbOutIgnore = unsigned(be[5] - be[4]);
-    const unsigned npfOutIgnore = unsigned(be[7] - be[6]);
-    NpFinal = nptot - NpbOut - NpfOut - npbOutIgnore - npfOutIgnore;
-    NpbFinal = npb1 - npbOutIgnore;
-    if (NpbOut != 0 && DivideFull) NpbFinal = UINT_MAX;
-    Ndiv++;
-  }
-
-  dsph_divdata divdata() const {
-    dsph_divdata d{};
-    d.scelldiv = ScellDiv;
-    d.ncx = int(Ncx); d.ncy = int(Ncy); d.ncz = int(Ncz); d.ncw = int(Ncx * Ncy);
-    d.cellfluid = Ncx * Ncy * Ncz + 1;
-    d.cellzerox = int(CellDomainMin.x); d.cellzeroy = int(CellDomainMin.y); d.cellzeroz = int(CellDomainMin.z);
-    d.d_beginendcell = begincell;
-    d.scell = Scell;
-    d.domcellcode = DomCellCode;
-    d.domposminx = DomPosMin[0]; d.domposminy = DomPosMin[1]; d.domposminz = DomPosMin[2];
-    d.kernelsize2 = KernelSize2;
-    d.poscellsize = PosCellSize;
-    return d;
-  }
-
-  template <typename T>
-  static void swp(T*& a, T*& b) { std::swap(a, b); }
-
-  // ---- JSphGpuSingle::RunCellDivide (JSphGpuSingle.cpp:372-499)
-  void run_cell_divide() {
-    const unsigned nptot = Np;
-    divide(Npb, Np - Npb, BoundChanged);
-    const unsigned pini = DivideFull ? 0u : NpbFinal;
-    chk(dsph_sort_data_basic_poscell(ctx, nptot, pini, sortpart, idp, code, dcell, posxy, posz,
-                                     velrho, idp2, code2, dcell2, posxy2, posz2, velrho2,
-                                     Map_PosMin, PosCellSize, poscell),
-        "sort basic");
-    swp(idp, idp2); swp(code, code2); swp(dcell, dcell2); swp(posxy, posxy2); swp(posz, posz2);
-    swp(velrho, velrho2);
-    if (TStep == 1) {
-      chk(dsph_sort_data_f4(ctx, nptot, pini, sortpart, velrhom1, velrhom1b), "sort velrhom1");
-      swp(velrhom1, velrhom1b);
-    } else if (pre_active) {
-      chk(dsph_sort_data_d2_d_f4(ctx, nptot, pini, sortpart, posxypre, poszpre, velrhopre,
-                                 posxypre2, poszpre2, velrhopre2),
-          "sort pre");
-      swp(posxypre, posxypre2); swp(poszpre, poszpre2); swp(velrhopre, velrhopre2);
-    }
-    if (TVisco == DSPH_VISCO_LAMINARSPS) {
-      chk(dsph_sort_data_sym6(ctx, nptot, pini, sortpart, spstau, spstau2), "sort tau");
-      swp(spstau, spstau2);
-    }
-    if (TBoundary == DSPH_BC_MDBC) {
-      chk(dsph_sort_data_f3(ctx, nptot, pini, sortpart, boundnor, boundnor2), "sort boundnor");
-      swp(boundnor, boundnor2);
-    }
-    if (NpbOut) throw Err(DSPH_E_NUMERIC, "boundary particles excluded (AbortBoundOut)");
-    Np = NpFinal;
-    Npb = NpbFinal;
-    NpbOk = Npb - NpbIgnore;
-    NpfOutTotal += NpfOut;
-    BoundChanged = false;
-  }
-
-  // ---- JSphGpu::PreInteraction_Forces (JSphGpu.cpp:846-913)
-  void pre_interaction() {
-    chk(dsph_memset_async(ctx, viscdt, 0, 4 * size_t(Np)), "memset viscdt");
-    chk(dsph_memset_async(ctx, ar, 0, 4 * size_t(Np)), "memset ar");
-    chk(dsph_memset_async(ctx, ace, 0, 12 * size_t(Np)), "memset ace");
-    if (sps2strain)
-      chk(dsph_memset_async(ctx, sps2strain + 6 * size_t(Npb), 0, 24 * size_t(Np - Npb)), "memset strain");
-    chk(dsph_velmax2(ctx, Np - Npb, Npb, velrho, d_red + 0), "velmax");
-  }
-
-  // ---- JSphGpuSingle::MdbcBoundCorrection (JSphGpuSingle.cpp:686-712)
-  void mdbc(bool corrector) {
-    if (TBoundary != DSPH_BC_MDBC || corrector) return;  // MdbcCorrector=false default
-    const dsph_divdata d = divdata();
-    chk(dsph_mdbc_correction(ctx, NpbOk, &d, Map_PosMin, posxy, posz, poscell, code, boundnor, velrho),
-        "mdbc");
-  }
-
-  // ---- JSphGpuSingle::Interaction_Forces (JSphGpuSingle.cpp:590-680)
-  void interaction() {
-    dsph_inter_parms t{};
-    t.tkernel = DSPH_KERNEL_WENDLAND;
-    t.tvisco = TVisco;
-    t.tdensity = TDensity;
-    t.viscob = Visco * ViscoBoundFactor;
-    t.viscof = Visco;
-    t.np = Np; t.npb = Npb; t.npbok = NpbOk;
-    t.divdata = divdata();
-    t.d_dcell = dcell; t.d_poscell = poscell; t.d_velrho = velrho; t.d_code = code;
-    t.d_spstaurho2 = spstau;
-    t.d_viscdt = viscdt; t.d_ar = ar; t.d_ace = ace; t.d_sps2strain = sps2strain;
-    t.d_velgrad = UseClosure ? velgrad : nullptr;
-    chk(dsph_interaction_forces(ctx, &t), "interaction");
-    if (TVisco == DSPH_VISCO_LAMINARSPS) {
-      if (UseClosure)
-        chk(dsph_sps_tau_learned(ctx, &Closure, Np, Npb, SpsBlin, float(Dp), velrho, velgrad,
-                                 sps2strain, spstau, closure_nu),
-            "learned sps tau");
-      else
-        chk(dsph_sps_tau(ctx, Np, Npb, SpsSmag, SpsBlin, velrho, sps2strain, spstau), "sps tau");
-    }
-    chk(dsph_redu_max_f32(ctx, Np, 0, viscdt, d_red + 1), "viscdtmax");
-    chk(dsph_acemax2(ctx, Np - Npb, Npb, ace, d_red + 2), "acemax");
-  }
-
-  // ---- JSphGpu::DtVariable (JSphGpu.cpp:1103-1147)
-  double dt_variable(unsigned slot) {
-    if (device_dt) {
-      chk(dsph_dt_variable(ctx, d_red, double(KernelH), Cs0, CFLnumber, DtMin, d_dt, slot, d_flags),
-          "dt");
-      return 0;
-    }
-    float red[4];
-    chk(dsph_memcpy_d2h(ctx, red, d_red, 12), "dt readback");
-    VelMax = double(std::sqrt(red[0]));
-    ViscDtMax = red[1];
-    AceMax = std::sqrt(double(red[2]));
-    const double dt1 = (AceMax ? std::sqrt(double(KernelH) / AceMax) : DBL_MAX);
-    const double dt2 = double(KernelH) / (std::max(Cs0, VelMax * 10.) + double(KernelH) * ViscDtMax);
-    double dt = CFLnumber * std::min(dt1, dt2);
-    if (std::isnan(dt) || std::isinf(dt)) throw Err(DSPH_E_NUMERIC, "dt is NaN or infinity");
-    if (dt < DtMin) dt = DtMin;
-    return dt;
-  }
-
-  void set_host_dt_slot(unsigned slot, double v) {
-    chk(dsph_memcpy_h2d(ctx, d_dt + slot, &v, 8), "dt upload");
-  }
-
-  // ---- JSphGpu::ComputeSymplecticPre / Corr (JSphGpu.cpp:996-1097)
-  void symplectic_pre(double dt) {
-    swp(posxypre, posxy); swp(poszpre, posz); swp(velrhopre, velrho);
-    pre_active = true;
-    chk(dsph_step_symplectic_pre(ctx, Np, Npb, velrhopre, ar, ace, dt * .5,
-                                 device_dt ? d_dt : nullptr, 2, RhopZero, RhopOutMin, RhopOutMax,
-                                 Gravity, code, movxy, movz, velrho),
-        "symplectic pre");
-    chk(dsph_step_pos2(ctx, Np, Npb, posxypre, poszpre, movxy, movz, posxy, posz, dcell, code), "pos2");
-    chk(dsph_memcpy_d2d_async(ctx, posxy, posxypre, 16 * size_t(Npb)), "copy bound xy");
-    chk(dsph_memcpy_d2d_async(ctx, posz, poszpre, 8 * size_t(Npb)), "copy bound z");
-  }
-  void symplectic_cor(double dt) {
-    chk(dsph_step_symplectic_cor(ctx, Np, Npb, velrhopre, ar, ace, dt * .5, dt,
-                                 device_dt ? d_dt : nullptr, 2, RhopZero, RhopOutMin, RhopOutMax,
-                                 Gravity, code, movxy, movz, velrho),
-        "symplectic cor");
-    chk(dsph_step_pos2(ctx, Np, Npb, posxypre, poszpre, movxy, movz, posxy, posz, dcell, code), "pos2");
-    pre_active = false;
-  }
-
-  // ---- JSphGpu::RunMotion (JSphGpu.cpp:1173-1200) for one rectilinear
-  // sinusoid: after the step, the moving boundary is displaced over
-  // [t, t+dt] and takes the velocity displacement/dt; the next divide is
-  // full (BoundChanged).
-  void run_motion(double dt) {
-    const dsph_motion& m = cs.motion;
-    if (!cs.casenmoving || m.type == 0) return;
-    if (device_dt) {
-      chk(dsph_move_motion_device(ctx, Npb, &m, TimeStep, d_dt, idp, posxy, posz, dcell, velrho,
-                                  code),
-          "motion");
-    } else {
-      if (!(TimeStep >= m.start && TimeStep < m.finish)) return;
-      double mv[3];
-      dsph::rect_sinu_mov_host(&m, TimeStep, dt, mv);
-      const float vel[3] = {float(mv[0] / dt), float(mv[1] / dt), float(mv[2] / dt)};
-      chk(dsph_move_lin_bound(ctx, Npb, m.idbegin, m.count, mv, vel, idp, posxy, posz, dcell,
-                              velrho, code),
-          "motion");
-    }
-    BoundChanged = true;
-  }
-
-  // ---- JSphGpuSingle::ComputeStep_Sym (JSphGpuSingle.cpp:785-821)
-  void step_sym() {
-    const double dt = SymplecticDtPre;
-    mdbc(false);
-    pre_interaction();
-    interaction();
-    const double dt_p = dt_variable(0);
-    symplectic_pre(dt);
-    run_cell_divide();
-    mdbc(true);
-    pre_interaction();
-    interaction();
-    const double dt_c = dt_variable(1);
-    symplectic_cor(dt);
-    run_motion(dt);
-    if (device_dt) {
-      chk(dsph_dt_variable_min_advance(), "dt advance");
-    } else {
-      SymplecticDtPre = std::min(dt_p, dt_c);
-      TimeStep += dt;
-      DtLast = dt;
-    }
-  }
-  int dsph_dt_variable_min_advance();
-
-  // ---- JSphGpuSingle::ComputeStep_Ver (JSphGpuSingle.cpp:761-781) + JSphGpu::ComputeVerlet
-  void step_ver() {
-    mdbc(false);
-    pre_interaction();
-    interaction();
-    device_dt = false;
-    const double dt = dt_variable(0);
-    VerletStep++;
-    if (VerletStep < unsigned(VerletSteps)) {
-      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrhom1, ar, ace, dt, dt + dt, RhopZero,
-                           RhopOutMin, RhopOutMax, Gravity, code, movxy, movz, velrhom1),
-          "verlet");
-    } else {
-      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrho, ar, ace, dt, dt, RhopZero, RhopOutMin,
-                           RhopOutMax, Gravity, code, movxy, movz, velrhom1),
-          "verlet euler");
-      VerletStep = 0;
-    }
-    chk(dsph_step_pos(ctx, Np, Npb, movxy, movz, posxy, posz, dcell, code), "pos");
-    swp(velrho, velrhom1);
-    run_motion(dt);
-    TimeStep += dt;
-    DtLast = dt;
-  }
-
-  void run(unsigned nsteps, bool sync_dt) {
-    device_dt = (!sync_dt && TStep == 2);
-    if (device_dt) set_host_dt_slot(2, SymplecticDtPre);
-    for (unsigned s = 0; s < nsteps; s++) {
-      if (TStep == 2) step_sym(); else step_ver();
-      run_cell_divide();
-      Nstep++;
-    }
-    if (device_dt) {
-      double h[8];
-      chk(dsph_memcpy_d2h(ctx, h, d_dt, 64), "dt readback");
-      unsigned fl[4];
-      chk(dsph_memcpy_d2h(ctx, fl, d_flags, 16), "flags readback");
-      if (fl[0] & 1u) throw Err(DSPH_E_NUMERIC, "dt is NaN or infinity");
-      SymplecticDtPre = h[2];
-      TimeStep += h[3];
-      DtLast = h[4];
-      const double z[2] = {0.0, 0.0};
-      chk(dsph_memcpy_h2d(ctx, d_dt + 3, z, 16), "dt reset");
-    }
-    chk(dsph_synchronize(ctx), "synchronize");
-  }
-};
+#include "sph_gpu_single.h"
 
 // SymplecticDtPre=min(dt_p,dt_c), time+=dt and the dt used, on the device.
 int dsph_sim::dsph_dt_variable_min_advance() { return dsph_dt_advance(ctx, d_dt); }

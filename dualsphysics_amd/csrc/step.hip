@@ -6,8 +6,10 @@
 // and KerComputeStepVerlet (JSphGpuSimple_ker.cu:132-513), KerUpdatePos /
 // KerComputeStepPos(2) (JSphGpu_ker.cu:334-392, 1476-1570). Options out of
 // scope (floatings, shifting, inlet/outlet, mDBC2, periodic) are not built.
+#include <algorithm>
 #include <cfloat>
 #include "dsph_ctx.h"
+#include "ensemble.h"
 
 using namespace dsph;
 
@@ -81,23 +83,31 @@ int redu_max(dsph_ctx* ctx, uint32_t n, uint32_t ini, const float* d, float* d_o
 // ---------------------------------------------------------------- dt
 // DtVariable (JSphGpu.cpp:1103-1147) on one lane, FP64. red = {max|v|^2,
 // ViscDtMax, max|ace|^2}.
+// Returns dt; sets bit 0 of *fl for NaN/inf and bit 1 when clamped to dtmin.
+__device__ inline double dt_of(float red0, float red1, float red2, double kernelh, double cs0,
+                               double cfl, double dtmin, unsigned& fl) {
+  const double velmax = double(sqrtf(red0));
+  const float viscdtmax = red1;
+  const double acemax = sqrt(double(red2));
+  const double dt1 = (acemax ? sqrt(kernelh / acemax) : DBL_MAX);
+  const double dt2 = kernelh / (fmax(cs0, velmax * 10.) + kernelh * double(viscdtmax));
+  double dt = cfl * fmin(dt1, dt2);
+  fl = 0;
+  if (dt != dt || isinf(dt)) fl |= 1u;
+  if (dt < dtmin) {
+    dt = dtmin;
+    fl |= 2u;
+  }
+  return dt;
+}
+
 __global__ void k_dt_variable(const float* __restrict__ red, double kernelh, double cs0,
                               double cfl, double dtmin, double* __restrict__ dtout,
                               unsigned slot, unsigned* __restrict__ flags) {
   if (threadIdx.x != 0) return;
-  const double velmax = double(sqrtf(red[0]));
-  const float viscdtmax = red[1];
-  const double acemax = sqrt(double(red[2]));
-  const double dt1 = (acemax ? sqrt(kernelh / acemax) : DBL_MAX);
-  const double dt2 = kernelh / (fmax(cs0, velmax * 10.) + kernelh * double(viscdtmax));
-  double dt = cfl * fmin(dt1, dt2);
-  if (dt != dt || isinf(dt)) {
-    if (flags) atomicOr(flags, 1u);
-  }
-  if (dt < dtmin) {
-    dt = dtmin;
-    if (flags) atomicOr(flags, 2u);
-  }
+  unsigned fl;
+  const double dt = dt_of(red[0], red[1], red[2], kernelh, cs0, cfl, dtmin, fl);
+  if (fl && flags) atomicOr(flags, fl);
   dtout[slot] = dt;
 }
 
@@ -114,16 +124,12 @@ __device__ inline double dev_dt(double dtm_host, const double* d_dt, unsigned sl
   return d_dt ? d_dt[slot] * scale : dtm_host;
 }
 
-__global__ void __launch_bounds__(kBs) k_sym_pre(unsigned n, unsigned npb,
+__device__ inline void sym_pre_one(unsigned p, bool bound, double dtm,
     const float4* __restrict__ velrhopre, const float* __restrict__ ar,
-    const float* __restrict__ ace, double dtm_h, const double* __restrict__ d_dt, unsigned slot,
-    float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
-    typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
-    float4* __restrict__ velrho) {
-  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  const double dtm = dev_dt(dtm_h, d_dt, slot, 0.5);
-  if (p < npb) {
+    const float* __restrict__ ace, float rhopzero, float rhopoutmin, float rhopoutmax,
+    float3 gravity, typecode* __restrict__ code, double2* __restrict__ movxy,
+    double* __restrict__ movz, float4* __restrict__ velrho) {
+  if (bound) {
     float4 vr = velrhopre[p];
     vr.w = float(double(vr.w) + dtm * ar[p]);
     vr.w = (vr.w < rhopzero ? rhopzero : vr.w);
@@ -146,17 +152,24 @@ __global__ void __launch_bounds__(kBs) k_sym_pre(unsigned n, unsigned npb,
   velrho[p] = nw;
 }
 
-__global__ void __launch_bounds__(kBs) k_sym_cor(unsigned n, unsigned npb,
+__global__ void __launch_bounds__(kBs) k_sym_pre(unsigned n, unsigned npb,
     const float4* __restrict__ velrhopre, const float* __restrict__ ar,
-    const float* __restrict__ ace, double dtm_h, double dt_h, const double* __restrict__ d_dt,
-    unsigned slot, float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
+    const float* __restrict__ ace, double dtm_h, const double* __restrict__ d_dt, unsigned slot,
+    float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
     typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
     float4* __restrict__ velrho) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  const double dt = dev_dt(dt_h, d_dt, slot, 1.0);
-  const double dtm = dev_dt(dtm_h, d_dt, slot, 0.5);
-  if (p < npb) {
+  sym_pre_one(p, p < npb, dev_dt(dtm_h, d_dt, slot, 0.5), velrhopre, ar, ace, rhopzero,
+              rhopoutmin, rhopoutmax, gravity, code, movxy, movz, velrho);
+}
+
+__device__ inline void sym_cor_one(unsigned p, bool bound, double dt, double dtm,
+    const float4* __restrict__ velrhopre, const float* __restrict__ ar,
+    const float* __restrict__ ace, float rhopzero, float rhopoutmin, float rhopoutmax,
+    float3 gravity, typecode* __restrict__ code, double2* __restrict__ movxy,
+    double* __restrict__ movz, float4* __restrict__ velrho) {
+  if (bound) {
     const double eps = (-double(ar[p]) / double(velrho[p].w)) * dt;
     float rrho = float(double(velrhopre[p].w) * (2. - eps) / (2. + eps));
     rrho = (rrho < rhopzero ? rhopzero : rrho);
@@ -180,6 +193,19 @@ __global__ void __launch_bounds__(kBs) k_sym_cor(unsigned n, unsigned npb,
   movz[p] = dz;
   if (outrho && code_is_normal(rcode)) code[p] = typecode(code_set(rcode, kCodeOutRho));
   velrho[p] = nw;
+}
+
+__global__ void __launch_bounds__(kBs) k_sym_cor(unsigned n, unsigned npb,
+    const float4* __restrict__ velrhopre, const float* __restrict__ ar,
+    const float* __restrict__ ace, double dtm_h, double dt_h, const double* __restrict__ d_dt,
+    unsigned slot, float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
+    typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
+    float4* __restrict__ velrho) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  sym_cor_one(p, p < npb, dev_dt(dt_h, d_dt, slot, 1.0), dev_dt(dtm_h, d_dt, slot, 0.5),
+              velrhopre, ar, ace, rhopzero, rhopoutmin, rhopoutmax, gravity, code, movxy, movz,
+              velrho);
 }
 
 __global__ void __launch_bounds__(kBs) k_verlet(unsigned n, unsigned npb,
@@ -216,12 +242,6 @@ __global__ void __launch_bounds__(kBs) k_verlet(unsigned n, unsigned npb,
 }
 
 // ---------------------------------------------------------------- positions
-struct PosCte {
-  float movlimit;
-  double mapx, mapy, mapz, sizex, sizey, sizez;
-  float scell;
-  unsigned cellcode;
-};
 
 // KerUpdatePos (JSphGpu_ker.cu:334-392) without periodic boundaries.
 __device__ inline void update_pos(double2 rxy, double rz, double mx, double my, double mz,
@@ -325,7 +345,208 @@ PosCte make_posc(const dsph_cte& c) {
   return r;
 }
 
+// ---------------------------------------------------------------- ensembles
+__global__ void __launch_bounds__(kBs) k_ens_zero(unsigned n, float* __restrict__ ace,
+                                                  float* __restrict__ ar,
+                                                  float* __restrict__ viscdt) {
+  for (unsigned p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    ace[3 * size_t(p)] = 0.f;
+    ace[3 * size_t(p) + 1] = 0.f;
+    ace[3 * size_t(p) + 2] = 0.f;
+    ar[p] = 0.f;
+    viscdt[p] = 0.f;
+  }
+}
+
+// WHICH 0: max |v|^2 over fluid; 1: max viscdt over all; 2: max |ace|^2 over fluid.
+template <int WHICH>
+__global__ void __launch_bounds__(kBs) k_ens_reduce(const EnsCase* __restrict__ cases,
+    EnsState* __restrict__ st, const EnsBlock* __restrict__ pmap,
+    const float4* __restrict__ velrho, const float* __restrict__ viscdt,
+    const float* __restrict__ ace) {
+  const EnsBlock e = pmap[blockIdx.x];
+  if (!st[e.x].active) return;
+  const EnsCase& ec = cases[e.x];
+  const unsigned l = e.y + threadIdx.x;
+  const unsigned lo = (WHICH == 1 ? 0u : ec.npb);
+  const bool in = l >= lo && l < st[e.x].np;
+  float v = -FLT_MAX;
+  if (in) {
+    const unsigned p = ec.p0 + l;
+    if (WHICH == 0) {
+      const float4 vr = velrho[p];
+      v = vr.x * vr.x + vr.y * vr.y + vr.z * vr.z;
+    } else if (WHICH == 1) {
+      v = viscdt[p];
+    } else {
+      const float x = ace[3 * size_t(p)], y = ace[3 * size_t(p) + 1], z = ace[3 * size_t(p) + 2];
+      v = x * x + y * y + z * z;
+    }
+  }
+  v = block_max(v);
+  // The values are >= 0, so their FP32 bits order like the values.
+  if (threadIdx.x == 0 && v >= 0.f) atomicMax(&st[e.x].red[WHICH], __float_as_uint(v));
+}
+
+__global__ void k_ens_dt(const EnsCase* __restrict__ cases, EnsState* __restrict__ st,
+                         unsigned ncases, unsigned slot) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncases) return;
+  EnsState& s = st[k];
+  if (!s.active) return;
+  const EnsCase& ec = cases[k];
+  unsigned fl;
+  s.dt[slot] = dt_of(__uint_as_float(s.red[0]), __uint_as_float(s.red[1]),
+                     __uint_as_float(s.red[2]), ec.kernelh, ec.cs0, ec.cfl, ec.dtmin, fl);
+  s.red[0] = s.red[1] = s.red[2] = 0u;
+  if (fl) s.flags |= fl;
+  if (fl & 1u) s.active = 0;
+}
+
+__global__ void __launch_bounds__(kBs) k_ens_sym(const EnsCase* __restrict__ cases,
+    const EnsState* __restrict__ st, const EnsBlock* __restrict__ pmap, bool corrector,
+    const float4* __restrict__ velrhopre, const float* __restrict__ ar,
+    const float* __restrict__ ace, typecode* __restrict__ code, double2* __restrict__ movxy,
+    double* __restrict__ movz, float4* __restrict__ velrho) {
+  const EnsBlock e = pmap[blockIdx.x];
+  const unsigned l = e.y + threadIdx.x;
+  const EnsCase& ec = cases[e.x];
+  if (!st[e.x].active) {
+    // The predictor swapped the state arrays: a stopped case keeps its state.
+    if (!corrector && l < ec.np0) velrho[ec.p0 + l] = velrhopre[ec.p0 + l];
+    return;
+  }
+  if (l >= st[e.x].np) return;
+  const double dt = st[e.x].dtpre;
+  const float3 g = make_float3(ec.gx, ec.gy, ec.gz);
+  if (corrector)
+    sym_cor_one(ec.p0 + l, l < ec.npb, dt * 1.0, dt * 0.5, velrhopre, ar, ace, ec.c.rhopzero,
+                ec.rhopoutmin, ec.rhopoutmax, g, code, movxy, movz, velrho);
+  else
+    sym_pre_one(ec.p0 + l, l < ec.npb, dt * 0.5, velrhopre, ar, ace, ec.c.rhopzero,
+                ec.rhopoutmin, ec.rhopoutmax, g, code, movxy, movz, velrho);
+}
+
+__global__ void __launch_bounds__(kBs) k_ens_pos2(const EnsCase* __restrict__ cases,
+    const EnsState* __restrict__ st, const EnsBlock* __restrict__ pmap, bool copy_bound,
+    const double2* __restrict__ posxypre, const double* __restrict__ poszpre,
+    const double2* __restrict__ movxy, const double* __restrict__ movz,
+    double2* __restrict__ posxy, double* __restrict__ posz, unsigned* __restrict__ dcell,
+    typecode* __restrict__ code) {
+  const EnsBlock e = pmap[blockIdx.x];
+  const unsigned l = e.y + threadIdx.x;
+  const EnsCase& ec = cases[e.x];
+  const unsigned p = ec.p0 + l;
+  if (!st[e.x].active) {
+    // The predictor swapped the position arrays: a stopped case keeps them.
+    if (copy_bound && l < ec.np0) {
+      posxy[p] = posxypre[p];
+      posz[p] = poszpre[p];
+    }
+    return;
+  }
+  if (l >= st[e.x].np) return;
+  if (l < ec.npb) {
+    if (copy_bound) {
+      posxy[p] = posxypre[p];
+      posz[p] = poszpre[p];
+    }
+    return;
+  }
+  const bool outrho = code_is_outrho(code[p]);
+  const double2 m = movxy[p];
+  update_pos(posxypre[p], poszpre[p], m.x, m.y, movz[p], outrho, p, ec.pc, posxy, posz, dcell,
+             code);
+}
+
+__global__ void k_ens_advance(const EnsCase* __restrict__ cases, EnsState* __restrict__ st,
+                              unsigned ncases) {
+  const unsigned k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= ncases) return;
+  EnsState& s = st[k];
+  if (!s.active) return;
+  const double dt = s.dtpre;
+  s.time += dt;
+  s.dtlast = dt;
+  s.dtpre = fmin(s.dt[0], s.dt[1]);
+  s.nstep++;
+  if (s.time >= cases[k].tend) s.active = 0;
+}
+
 }  // namespace
+
+namespace dsph {
+
+int ens_zero_forces(dsph_ctx* ctx, unsigned n, float* ace, float* ar, float* viscdt) {
+  if (!n) return DSPH_OK;
+  k_ens_zero<<<std::min(grid_for(n, kBs), 4096u), kBs, 0, ctx->stream>>>(n, ace, ar, viscdt);
+  DSPH_LAUNCHED(ctx, "k_ens_zero");
+  return DSPH_OK;
+}
+
+int ens_reduce(dsph_ctx* ctx, const EnsCase* cases, EnsState* st, const EnsBlock* pmap,
+               unsigned npblocks, int which, const float4* velrho, const float* viscdt,
+               const float* ace) {
+  if (!npblocks) return DSPH_OK;
+  KTimer kt(ctx, "k_redu_max");
+  switch (which) {
+    case 0: k_ens_reduce<0><<<npblocks, kBs, 0, ctx->stream>>>(cases, st, pmap, velrho, viscdt, ace); break;
+    case 1: k_ens_reduce<1><<<npblocks, kBs, 0, ctx->stream>>>(cases, st, pmap, velrho, viscdt, ace); break;
+    default: k_ens_reduce<2><<<npblocks, kBs, 0, ctx->stream>>>(cases, st, pmap, velrho, viscdt, ace); break;
+  }
+  DSPH_LAUNCHED(ctx, "k_ens_reduce");
+  return DSPH_OK;
+}
+
+int ens_dt(dsph_ctx* ctx, const EnsCase* cases, EnsState* st, unsigned ncases, unsigned slot) {
+  if (!ncases) return DSPH_OK;
+  k_ens_dt<<<grid_for(ncases, 64), 64, 0, ctx->stream>>>(cases, st, ncases, slot);
+  DSPH_LAUNCHED(ctx, "k_ens_dt");
+  return DSPH_OK;
+}
+
+int ens_sym_pre(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st, const EnsBlock* pmap,
+                unsigned npblocks, const float4* velrhopre, const float* ar, const float* ace,
+                uint16_t* code, double2* movxy, double* movz, float4* velrho) {
+  if (!npblocks) return DSPH_OK;
+  KTimer kt(ctx, "k_sym_pre");
+  k_ens_sym<<<npblocks, kBs, 0, ctx->stream>>>(cases, st, pmap, false, velrhopre, ar, ace, code,
+                                               movxy, movz, velrho);
+  DSPH_LAUNCHED(ctx, "k_ens_sym_pre");
+  return DSPH_OK;
+}
+
+int ens_sym_cor(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st, const EnsBlock* pmap,
+                unsigned npblocks, const float4* velrhopre, const float* ar, const float* ace,
+                uint16_t* code, double2* movxy, double* movz, float4* velrho) {
+  if (!npblocks) return DSPH_OK;
+  KTimer kt(ctx, "k_sym_cor");
+  k_ens_sym<<<npblocks, kBs, 0, ctx->stream>>>(cases, st, pmap, true, velrhopre, ar, ace, code,
+                                               movxy, movz, velrho);
+  DSPH_LAUNCHED(ctx, "k_ens_sym_cor");
+  return DSPH_OK;
+}
+
+int ens_pos2(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st, const EnsBlock* pmap,
+             unsigned npblocks, bool copy_bound, const double2* posxypre, const double* poszpre,
+             const double2* movxy, const double* movz, double2* posxy, double* posz,
+             unsigned* dcell, uint16_t* code) {
+  if (!npblocks) return DSPH_OK;
+  KTimer kt(ctx, "k_step_pos2");
+  k_ens_pos2<<<npblocks, kBs, 0, ctx->stream>>>(cases, st, pmap, copy_bound, posxypre, poszpre,
+                                                movxy, movz, posxy, posz, dcell, code);
+  DSPH_LAUNCHED(ctx, "k_ens_pos2");
+  return DSPH_OK;
+}
+
+int ens_advance(dsph_ctx* ctx, EnsState* st, const EnsCase* cases, unsigned ncases) {
+  if (!ncases) return DSPH_OK;
+  k_ens_advance<<<grid_for(ncases, 64), 64, 0, ctx->stream>>>(cases, st, ncases);
+  DSPH_LAUNCHED(ctx, "k_ens_advance");
+  return DSPH_OK;
+}
+
+}  // namespace dsph
 
 extern "C" {
 

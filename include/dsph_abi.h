@@ -542,6 +542,58 @@ int dsph_sim_interact(dsph_sim* sim);
 /* Access to the context of a simulation (streams, events). */
 dsph_ctx* dsph_sim_ctx(dsph_sim* sim);
 
+/*
+ * Batched ensembles (no counterpart in the reference, which runs one case
+ * per process): many independent cases advanced together on one GPU, each
+ * stage of the step one launch for all cases, 3 steps per HIP graph. Every
+ * case runs JSphGpuSingle's Symplectic step with DBC (artificial or laminar
+ * viscosity, DDT2Full or no density diffusion, the same options for all
+ * cases), a cell domain fixed to its map and its own adaptive dt, and stops
+ * at its end time. A case's results equal a dsph_sim run of that case with
+ * celldomfixed=1 and device-side dt, bit for bit.
+ */
+typedef struct dsph_ens dsph_ens;
+
+typedef struct dsph_ens_state {
+  uint32_t np, npbok;            /* current particles (excluded ones not counted) */
+  uint32_t active;               /* 0 once time >= tend or after an error */
+  uint32_t nstep;                /* steps run by this case */
+  uint32_t flags;                /* 1 dt NaN/inf, 2 dt clamped to DtMin, 4 boundary excluded */
+  uint32_t p0, np0;              /* range of the case in the concatenated arrays */
+  uint32_t pad;
+  double time, dt_last, dt_next;
+} dsph_ens_state;
+
+/* Creates an ensemble of ncases cases. Particles of all cases are
+ * concatenated in case order (each case sorted by idp, boundary first, as for
+ * dsph_sim_create): np[k] particles of case k, idp local to the case;
+ * tend[k] = end time of case k. */
+int dsph_ens_create(int device, uint32_t ncases, const dsph_case* cases,
+                    const uint32_t* np, const uint32_t* h_idp, const double* h_pos,
+                    const float* h_velrho, const double* tend, dsph_ens** out);
+int dsph_ens_destroy(dsph_ens* e);
+const char* dsph_ens_last_error(dsph_ens* e);
+/* Runs groups of 3 steps (one graph launch each, or eager launches with the
+ * graph disabled) until every case has stopped or max_steps steps ran;
+ * the stop flags are read back every check_every groups. h_steps = steps run. */
+int dsph_ens_run(dsph_ens* e, uint32_t max_steps, uint32_t check_every, uint32_t* h_steps);
+int dsph_ens_set_graph(dsph_ens* e, int enable);
+int dsph_ens_case_state(dsph_ens* e, uint32_t icase, dsph_ens_state* out);
+/* Current particles of case icase in sorted order (as dsph_sim_download). */
+int dsph_ens_download(dsph_ens* e, uint32_t icase, uint32_t* h_count, uint32_t* h_idp,
+                      double* h_pos, float* h_velrho, uint16_t* h_code);
+/* Autotuner: times the fluid-kernel variants (LDS tile kernel with 8, 12 or
+ * 16 mask tiles per wave, one-lane-per-particle kernel) on each case-size
+ * bucket (fluid particles < 16384, < 131072, larger) over the current state,
+ * reps runs each, and keeps the fastest per bucket. h_variant[3] (-1 for an
+ * empty bucket) and h_ms[3*4] (best time of each variant) may be NULL. */
+int dsph_ens_autotune(dsph_ens* e, uint32_t reps, int32_t* h_variant, float* h_ms);
+int dsph_ens_set_variant(dsph_ens* e, uint32_t bucket, int32_t variant);
+/* {cases, particles, sort keys, cases per bucket [3], steps per graph, steps run} */
+int dsph_ens_info(dsph_ens* e, uint32_t* h_info);
+int dsph_ens_set_timing(dsph_ens* e, int enable);
+dsph_ctx* dsph_ens_context(dsph_ens* e);
+
 #ifdef __cplusplus
 }
 #endif

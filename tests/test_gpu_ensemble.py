@@ -1,0 +1,126 @@
+"""Batched ensembles on the GPU: every case of an ensemble equals, bit for
+bit, a single-case run of the same particles (fixed cell domain, device-side
+dt); graph replays equal eager launches; the autotuner changes the speed,
+never the results; cases stop at their own end times."""
+import numpy as np
+import pytest
+
+from dualsphysics_amd import _native as N
+from dualsphysics_amd import case as casemod
+from dualsphysics_amd.ensemble import Ensemble, EnsembleError
+from dualsphysics_amd.sim import SphGpuSingle
+
+pytestmark = pytest.mark.gpu
+
+
+def _cases(tvisco=N.VISCO_ARTIFICIAL, tddt=N.DDT_DDT2FULL):
+    prm = casemod.Parameters(celldomfixed=1, tvisco=tvisco, tdensity=tddt,
+                             visco=0.01 if tvisco == N.VISCO_ARTIFICIAL else 1e-4)
+    shapes = [  # dp, tank, column
+        (0.04, (1.6, 0.67, 0.4), (0.4, 0.67, 0.3)),
+        (0.03, (0.8, 0.4, 0.3), (0.2, 0.4, 0.2)),
+        (0.05, (2.0, 0.6, 0.5), (0.5, 0.6, 0.4)),
+        (0.035, (0.6, 0.5, 0.25), (0.3, 0.5, 0.12)),
+        (0.045, (1.2, 0.67, 0.3), (0.3, 0.67, 0.25)),
+    ]
+    return [casemod.dam_break(dp=dp, tank=t, column=c, seed=11 + k, params=prm)
+            for k, (dp, t, c) in enumerate(shapes)]
+
+
+def _single(c, nsteps):
+    s = SphGpuSingle(c)
+    try:
+        s.run(nsteps)
+        return s.download(), s.stats()
+    finally:
+        s.close()
+
+
+def _assert_same(a, b):
+    for k in ("idp", "pos", "velrho", "code"):
+        assert a[k].shape == b[k].shape, k
+        np.testing.assert_array_equal(a[k], b[k], err_msg=k)
+
+
+@pytest.mark.parametrize("tvisco,tddt", [(N.VISCO_ARTIFICIAL, N.DDT_DDT2FULL),
+                                         (N.VISCO_LAMINAR, N.DDT_NONE)])
+def test_ensemble_equals_single_runs_bitwise(tvisco, tddt):
+    cases = _cases(tvisco, tddt)
+    nsteps = 12
+    ens = Ensemble(cases, tend=10.0)
+    try:
+        assert ens.run(max_steps=nsteps) == nsteps
+        for k, c in enumerate(cases):
+            ref, st = _single(c, nsteps)
+            got = ens.download(k)
+            _assert_same(got, ref)
+            s = ens.state(k)
+            assert s["nstep"] == nsteps and s["active"] == 1
+            assert s["time"] == st["timestep"] and s["dt_last"] == st["dt_last"]
+            assert s["np"] == st["np"]
+    finally:
+        ens.close()
+
+
+def test_graph_replay_equals_eager_launches():
+    cases = _cases()
+    a = Ensemble(cases, tend=10.0)
+    b = Ensemble(cases, tend=10.0)
+    try:
+        a.run(max_steps=9, graph=True)
+        b.run(max_steps=9, graph=False)
+        a.run(max_steps=6, graph=True)   # replays of the same graph
+        b.run(max_steps=6, graph=False)
+        for k in range(len(cases)):
+            _assert_same(a.download(k), b.download(k))
+            assert a.state(k)["time"] == b.state(k)["time"]
+    finally:
+        a.close()
+        b.close()
+
+
+def test_autotuner_changes_speed_not_results():
+    cases = _cases()
+    a = Ensemble(cases, tend=10.0)
+    b = Ensemble(cases, tend=10.0)
+    try:
+        chosen, ms = a.autotune(reps=2)
+        assert chosen and np.all(ms[ms > 0] < 1e4)
+        for bucket in range(3):
+            b.set_variant(bucket, 3)   # the one-lane-per-particle kernel everywhere
+        a.run(max_steps=6)
+        b.run(max_steps=6)
+        for k in range(len(cases)):
+            _assert_same(a.download(k), b.download(k))
+    finally:
+        a.close()
+        b.close()
+
+
+def test_cases_stop_at_their_end_times():
+    cases = _cases()
+    tend = np.array([0.002, 0.004, 0.001, 0.003, 0.0025])
+    ens = Ensemble(cases, tend=tend)
+    try:
+        steps = ens.run(check_every=1)
+        assert 0 < steps < 1000
+        for k in range(len(cases)):
+            s = ens.state(k)
+            assert s["active"] == 0 and s["flags"] & 1 == 0
+            assert s["time"] >= tend[k] and s["time"] - s["dt_last"] < tend[k]
+        # a stopped case no longer changes
+        before = [ens.download(k) for k in range(len(cases))]
+        ens.run(max_steps=3)
+        for k in range(len(cases)):
+            _assert_same(ens.download(k), before[k])
+    finally:
+        ens.close()
+
+
+def test_unsupported_options_are_errors():
+    c = casemod.dam_break(dp=0.05, params=casemod.Parameters(tstep=1, celldomfixed=1))
+    with pytest.raises(EnsembleError, match="Symplectic"):
+        Ensemble([c], tend=1.0)
+    c2 = casemod.dam_break(dp=0.05, params=casemod.Parameters(tdensity=N.DDT_DDT2))
+    with pytest.raises(EnsembleError, match="DDT2Full"):
+        Ensemble([c2], tend=1.0)
