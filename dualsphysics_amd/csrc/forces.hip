@@ -21,6 +21,7 @@
 //    order is unchanged.
 #include <cfloat>
 #include <cstring>
+#include <type_traits>
 #include "ensemble.h"
 #include "sph_math.h"
 
@@ -367,7 +368,7 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
   unsigned wt = 0, rt = 0, tb = 0, len = 0;
   unsigned long long cm = 0;
   auto pop = [&](unsigned& j) {
-    const unsigned bit = unsigned(__ffsll((long long)cm) - 1);
+    const unsigned bit = unsigned(__builtin_ctzll(cm));  // cm != 0 while len > 0
     j = tb + bit;  // kind bit 31 carried along
     cm &= cm - 1ull;
     len--;
@@ -399,22 +400,26 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
   };
   // k wave-uniform iterations; each lane pops up to k of its queued pairs.
   // Two register sets alternate so the loads of the next pair are in flight
-  // during the arithmetic of the current one.
-  auto process = [&](unsigned k) {
+  // during the arithmetic of the current one. With kFull every lane has at
+  // least k queued pairs, so no lane test is needed.
+  auto process = [&](unsigned k, auto full) {
+    constexpr bool kFull = decltype(full)::value;
     unsigned ja = 0, jb = 0;
     float4 psa = make_float4(0, 0, 0, 0), vra = psa, psb = psa, vrb = psa;
     float2 pra = make_float2(0, 0), prb = pra;
-    bool ha = k != 0 && len != 0;
+    bool ha = k != 0 && (kFull || len != 0);
     if (ha) issue(ja, psa, vra, pra);
     for (unsigned i = 0; i < k; i += 2) {
-      const bool hb = i + 1 < k && len != 0;
+      const bool hb = i + 1 < k && (kFull || len != 0);
       if (hb) issue(jb, psb, vrb, prb);
       if (ha) eval(ja, psa, vra, pra);
-      ha = i + 2 < k && len != 0;
+      ha = i + 2 < k && (kFull || len != 0);
       if (ha) issue(ja, psa, vra, pra);
       if (hb) eval(jb, psb, vrb, prb);
     }
   };
+  using Full = std::true_type;
+  using Some = std::false_type;
 
   bool pending = valid;
   for (;;) {
@@ -455,7 +460,7 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
           if (member) row_range(dvd.begincell, rowbase, lx0, lx1, lo, hi);
           for (unsigned t = gini; t < gfin; t += kTile) {
             // The ring slot of tile wt must have been consumed by every lane.
-            while (wt >= kRing && wave_min_u32(rt) <= wt - kRing) process(kMinBatch);
+            while (wt >= kRing && wave_min_u32(rt) <= wt - kRing) process(kMinBatch, Some());
             __syncthreads();
             if (t + lane < gfin) {
               const float4 f = posf[t + lane];
@@ -497,12 +502,17 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
             len += unsigned(__popcll(hm));
             wt++;
             const unsigned mn = wave_min_u32(member ? len : 0xffffffffu);
-            if (mn >= kMinBatch) process(mn);
+            if (mn >= kMinBatch) {
+              // All 64 lanes have mn pairs queued (the usual case: one group,
+              // all lanes valid): batches without per-lane tests.
+              if (wave_min_u32(len) >= mn) process(mn, Full());
+              else process(mn, Some());
+            }
           }
         }
     }
   }
-  process(wave_max_u32(len));
+  process(wave_max_u32(len), Some());
   if (valid) store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
 }
 
