@@ -73,6 +73,46 @@ def cpu_baseline(case, steps: int):
             "steps_per_s": steps / dt}
 
 
+REF_FAST = ROOT / "oracle" / "_ref" / "DualSPHysics5.4CPU_fast"
+
+
+def cpu_baseline_reference(case, steps: int, pairs_step: float):
+    """The reference CPU solver itself (built from its own sources by
+    oracle/ref.mk with its release flags: -O3 -fopenmp -ffast-math) on the
+    same particles (written as its case XML + .bi4 by oracle/bi4.py), run for
+    `steps` Symplectic steps on the host cores; time = its own "Simulation
+    Runtime" (initialisation excluded). Pairs per step are those of the same
+    initial state counted with the same rule on the GPU."""
+    import re
+    import subprocess
+    import tempfile
+    from oracle import bi4
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    with tempfile.TemporaryDirectory() as td:
+        td = Path(td)
+        bi4.write_case(case, td / "in", "case", timemax=100.0, timeout=100.0)
+        cmd = [str(REF_FAST), str(td / "in" / "case"), str(td / "out"), "-cpu",
+               f"-ompthreads:{threads}", f"-nsteps:{steps}", "-sv:none", "-svres:0"]
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=1800)
+    twall = time.perf_counter() - t0
+    if r.returncode != 0:
+        raise RuntimeError("reference CPU run failed: " + r.stdout[-2000:] + r.stderr[-1000:])
+    m = re.search(r"Simulation Runtime\.*:\s*([0-9.eE+-]+)\s*sec", r.stdout)
+    ms = re.search(r"Steps of simulation\.*:\s*([0-9,]+)", r.stdout)
+    tsim = float(m.group(1))
+    nstep = int(ms.group(1).replace(",", "")) if ms else steps
+    per = tsim / nstep
+    return {"value": pairs_step / per, "unit": "pair interactions/s", "cores": threads,
+            "kind": "reference",
+            "sample": f"{nstep} Symplectic steps (+divides) of the same {case.np}-particle dam "
+                      f"break by the reference CPU solver (DualSPHysics v5.4 built from its "
+                      f"sources, -O3 -fopenmp -ffast-math -march=x86-64-v3), {threads} OpenMP "
+                      f"threads: Simulation Runtime {tsim:.1f} s ({twall:.0f} s with case "
+                      f"writing and initialisation)",
+            "steps_per_s": nstep / tsim}
+
+
 def ensemble_bench(n: int, tend: float, sample: int):
     """Batched ensemble of ``n`` uneven dam breaks (seeded draws), every case
     run from t=0 to ``tend``: cases/s of the ensemble (one graph launch per 3
@@ -179,6 +219,7 @@ def main():
     ctx = lib.dsph_sim_ctx(sim._h)
     lib.dsph_ctx_set_option(ctx, N.OPT_FORCES_KERNEL,
                             N.FORCES_TILE if a.forces_kernel == "tile" else N.FORCES_ROWS)
+    p_init = sim.pips()
     log("warmup ...")
     sim.run(a.warmup)
     torch.cuda.synchronize()
@@ -232,8 +273,12 @@ def main():
         ens = ensemble_bench(a.ensemble, a.ens_tend, a.ens_plain_sample)
     cpu = None
     if not a.no_cpu_baseline:
-        log(f"cpu baseline: {a.cpu_steps} steps of the oracle ...")
-        cpu = cpu_baseline(c, a.cpu_steps)
+        if REF_FAST.exists():
+            log(f"cpu baseline: {a.cpu_steps} steps of the reference CPU solver ...")
+            cpu = cpu_baseline_reference(c, a.cpu_steps, 2.0 * (p_init[0] + p_init[1]))
+        else:
+            log(f"cpu baseline: {a.cpu_steps} steps of the oracle ...")
+            cpu = cpu_baseline(c, a.cpu_steps)
     out = {
         "metric": "pair interactions/s + steps/s at 5M particles; ensemble cases/s vs plain",
         "value": value, "unit": "pair interactions/s", "n_gpus": 1, "steps": a.steps,

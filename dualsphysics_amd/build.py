@@ -95,7 +95,7 @@ def build_reference(jobs: int = 8) -> Path | None:
     """Builds the reference CPU solver from its sources (this container only)."""
     if not REFERENCE.is_dir():
         return None
-    r = subprocess.run(["make", "-f", "oracle/ref.mk", f"-j{jobs}"], cwd=ROOT,
+    r = subprocess.run(["make", "-f", "oracle/ref.mk", f"-j{jobs}", "all", "fast"], cwd=ROOT,
                        capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("reference build failed:\n" + r.stdout[-4000:] + r.stderr[-4000:])

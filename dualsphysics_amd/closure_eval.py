@@ -58,14 +58,19 @@ def run_observed(c, times, closure_net=None, filter_at=(), dp_coarse=None, nprob
     try:
         if closure_net is not None:
             sim.set_closure(closure_net)
-        want = sorted(set([float(t) for t in times]) | set(float(t) for t in filter_at))
-        for t in want:
+        times = [round(float(t), 9) for t in times]
+        filter_at = [round(float(t), 9) for t in filter_at]
+        want = sorted(set(times) | set(filter_at))
+        for i, t in enumerate(want):
             sim.run_until(t)
-            if any(abs(t - x) < 1e-12 for x in times):
+            if i % 10 == 0:
+                log(f"    {c.name}: t={t:.2f} s, {sim.stats()['nstep']} steps, "
+                    f"{time.perf_counter() - t0:.1f}s")
+            if t in times:
                 f, p = _sample(c, sim)
                 fronts.append(f)
                 press.append(p)
-            if any(abs(t - x) < 1e-12 for x in filter_at):
+            if t in filter_at:
                 d = sim.download()
                 pos = d["pos"][d["idp"] >= c.nbound]
                 sel = rng.choice(len(pos), min(nprobe, len(pos)), replace=False)
@@ -87,12 +92,22 @@ def run_observed(c, times, closure_net=None, filter_at=(), dp_coarse=None, nprob
 
 
 def evaluate(fine_dp=0.004, coarse_dp=0.02, tmax=1.0, tout=0.01,
-             train_times=(0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9), steps=2000, log=print):
+             train_times=(0.2, 0.3, 0.4, 0.5, 0.6, 0.7, 0.8, 0.9), steps=2000, log=print,
+             fine_file=None):
+    """fine_file: npz of a previous fine run (observables and training data)
+    to reuse; written after the fine run when it does not exist."""
+    import os
     import dualsphysics_amd._native as Nn
-    times = [k * tout for k in range(int(round(tmax / tout)) + 1)]
+    times = [round(k * tout, 9) for k in range(int(round(tmax / tout)) + 1)]
     fine = casemod.dam_break(dp=fine_dp, params=_params())
-    log(f"fine run dp={fine_dp}")
-    rf = run_observed(fine, times, filter_at=train_times, dp_coarse=coarse_dp, log=log)
+    if fine_file and os.path.exists(fine_file):
+        rf = dict(np.load(fine_file, allow_pickle=False))
+        log(f"fine run dp={fine_dp} from {fine_file}")
+    else:
+        log(f"fine run dp={fine_dp}")
+        rf = run_observed(fine, times, filter_at=train_times, dp_coarse=coarse_dp, log=log)
+        if fine_file:
+            np.savez(fine_file, **rf)
     x, y = rf["features"], rf["targets"]
     mean, scale, os_ = closure.normalisation(x, y)
     lib = Nn.load()
@@ -124,7 +139,8 @@ def evaluate(fine_dp=0.004, coarse_dp=0.02, tmax=1.0, tout=0.01,
                 "seconds": r["seconds"], "steps": int(r["nstep"])}
 
     res = {"fine_dp": fine_dp, "coarse_dp": coarse_dp, "tmax": tmax,
-           "fine": {"seconds": rf["seconds"], "steps": int(rf["nstep"]), "particles": fine.np},
+           "fine": {"seconds": float(rf["seconds"]), "steps": int(rf["nstep"]),
+                    "particles": fine.np},
            "train_samples": int(len(y)),
            "smagorinsky": metrics(rs), "learned": metrics(rl)}
     return res
@@ -136,9 +152,12 @@ def main(argv=None):
     ap.add_argument("--coarse-dp", type=float, default=0.02)
     ap.add_argument("--tmax", type=float, default=1.0)
     ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--fine-file", default=None,
+                    help="npz cache of the fine run (read if present, else written)")
     a = ap.parse_args(argv)
     log = lambda *m: print(*m, file=sys.stderr, flush=True)
-    res = evaluate(a.fine_dp, a.coarse_dp, a.tmax, steps=a.steps, log=log)
+    res = evaluate(a.fine_dp, a.coarse_dp, a.tmax, steps=a.steps, log=log,
+                   fine_file=a.fine_file)
     print(json.dumps(res), flush=True)
 
 

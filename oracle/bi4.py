@@ -304,7 +304,8 @@ def write_case(case, directory, name: str, timemax: float = 1.0, timeout: float 
 
 
 def read_part(path) -> dict:
-    """Particles of a PART_xxxx file, sorted by id."""
+    """Particles of a PART_xxxx file, sorted by id; ``idp_saved`` keeps the
+    ids in the order the solver saved them (its particle order)."""
     root = read_file(path)
     part = next(it for it in root.items if it.name.startswith("PART_"))
     idp = part.get_array("Idp")
@@ -314,5 +315,5 @@ def read_part(path) -> dict:
     vel = part.get_array("Vel")
     rho = part.get_array("Rhop")
     o = np.argsort(idp, kind="stable")
-    return {"idp": idp[o], "pos": pos[o], "vel": vel[o], "rho": rho[o],
+    return {"idp": idp[o], "pos": pos[o], "vel": vel[o], "rho": rho[o], "idp_saved": idp,
             "time": part.get("TimeStep"), "step": part.get("Step"), "npok": part.get("Npok")}

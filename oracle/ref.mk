@@ -1,9 +1,14 @@
 # Builds the DualSPHysics v5.4 CPU solver from its own sources, in place under
-# /root/reference/src/source, into oracle/_ref/ (objects and executable only).
-# Used only to pin the oracle: never shipped, never part of the product path.
-# IEEE arithmetic (no -ffast-math) so FP64 cell hashing and FP32 pair sums are
-# reproducible; closed libraries (Chrono, WaveGen, MoorDynPlus) compiled out.
-#   make -f oracle/ref.mk -j8
+# /root/reference/src/source, into oracle/_ref/ (objects and executables only).
+# Used only to pin the oracle and as the timed CPU baseline: never shipped,
+# never part of the product path. Closed libraries (Chrono, WaveGen,
+# MoorDynPlus) compiled out.
+#  - DualSPHysics5.4CPU_ieee: IEEE arithmetic (no -ffast-math) so FP64 cell
+#    hashing and FP32 pair sums are reproducible (fixtures);
+#  - DualSPHysics5.4CPU_fast: the reference's own release flags (Makefile_cpu:
+#    -O3 -fopenmp -ffast-math) for timing, with a portable AVX2/FMA target
+#    instead of -march=native (the timing host is not this machine).
+#   make -f oracle/ref.mk -j8 [fast]
 REFSRC ?= /root/reference/src/source
 OUT    ?= oracle/_ref
 OBJDIR := $(OUT)/obj
@@ -34,11 +39,24 @@ OBJS := JXml tinystr tinyxml tinyxmlerror tinyxmlparser \
   JSphFlexStruc
 OBJFILES := $(addprefix $(OBJDIR)/,$(addsuffix .o,$(OBJS)))
 
+OBJDIR_FAST := $(OUT)/obj_fast
+EXE_FAST := $(OUT)/DualSPHysics5.4CPU_fast
+CXXFLAGS_FAST := -c -O3 -fopenmp -ffast-math -march=x86-64-v3 -std=c++0x -D_WITHMR \
+            -DDISABLE_CHRONO -DDISABLE_WAVEGEN -DDISABLE_MOORDYNPLUS -w
+OBJFILES_FAST := $(addprefix $(OBJDIR_FAST)/,$(addsuffix .o,$(OBJS)))
+
 all: $(EXE)
+fast: $(EXE_FAST)
 $(EXE): $(OBJFILES)
 	$(CXX) $(OBJFILES) -fopenmp -lgomp -o $@
+$(EXE_FAST): $(OBJFILES_FAST)
+	$(CXX) $(OBJFILES_FAST) -fopenmp -lgomp -o $@
 $(OBJDIR)/%.o: $(REFSRC)/%.cpp | $(OBJDIR)
 	$(CXX) $(CXXFLAGS) $< -o $@
+$(OBJDIR_FAST)/%.o: $(REFSRC)/%.cpp | $(OBJDIR_FAST)
+	$(CXX) $(CXXFLAGS_FAST) $< -o $@
 $(OBJDIR):
 	mkdir -p $(OBJDIR)
-.PHONY: all
+$(OBJDIR_FAST):
+	mkdir -p $(OBJDIR_FAST)
+.PHONY: all fast

@@ -10,8 +10,10 @@ Fixture ref_dambreak_dp0.03.npz (data only, no code of the reference):
   steps     the saved steps;
   time      simulated time after each saved step (FP64);
   disp      fluid displacement from the initial position, FP32 [k, nf, 3];
-  vel       fluid velocity [k, nf, 3]; rho: density of all particles [k, np].
-Particles are in id order.
+  vel       fluid velocity [k, nf, 3]; rho: density of all particles [k, np];
+  order     the ids in the reference's own particle order (sorted by cell)
+            after each saved step [k, np].
+Other arrays are in id order.
 
 ref_observables_dp0.01.npz (--observables, ~1 h on 8 cores): front position
 and probe pressure every 0.01 s over 1 s of the dp=0.01 dam break.
@@ -117,6 +119,7 @@ def main():
             vel_moving=np.stack([parts[k]["vel"][case.nbound - case.nmoving:case.nbound]
                                  for k in saved]),
             vel=np.stack([parts[k]["vel"][fl] for k in saved]),
+            order=np.stack([parts[k]["idp_saved"] for k in saved]),
             rho=np.stack([parts[k]["rho"] for k in saved]))
         print(f"wrote {name}.npz ({case.np} particles, steps {saved})")
 

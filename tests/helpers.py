@@ -168,11 +168,17 @@ REF_TOL = {"time": 1e-6, "disp": 2e-4, "vel": 2e-4, "rho": 10 * 6.1035e-05}
 REF_TOL_MDBC = {"time": 1e-5, "disp": 3e-3, "vel": 3e-3, "rho": 1e-2}
 
 
-def compare_with_reference(fx, k, time, pos0, pos, vel, rho):
+def compare_with_reference(fx, k, time, pos0, pos, vel, rho, order=None):
     """Errors of one saved step (arrays in id order); returns a dict and
     asserts the bounds of REF_TOL (REF_TOL_MDBC for mDBC cases). The moving
-    boundary (if any) must follow the reference to 1e-9 m and 1e-6 m/s."""
+    boundary (if any) must follow the reference to 1e-9 m and 1e-6 m/s.
+    ``order`` (the ids in the solver's sorted particle order) must equal the
+    reference's own order bit for bit."""
     nb = int(fx["nbound"])
+    if order is not None and "order" in fx:
+        ref = fx["order"][k]
+        nd = int(np.count_nonzero(order != ref)) if order.shape == ref.shape else -1
+        assert nd == 0, (int(fx["steps"][k]), "particle order differs", nd)
     tol = REF_TOL_MDBC if "tboundary', 2" in str(fx["params"]) else REF_TOL
     nmov = int(fx["nmoving"]) if "nmoving" in fx else 0
     if nmov:

@@ -25,6 +25,7 @@ def test_gpu_steps_follow_reference_cpu_build(name):
         d = sim.download()
         idx = np.argsort(d["idp"], kind="stable")
         e = compare_with_reference(fx, k, sim.stats()["timestep"], case.pos, d["pos"][idx],
-                                   d["velrho"][idx, :3], d["velrho"][idx, 3])
+                                   d["velrho"][idx, :3], d["velrho"][idx, 3],
+                                   order=d["idp"])
         print(name, int(step), e)
     sim.close()

@@ -34,7 +34,8 @@ def test_oracle_follows_reference_cpu_build(name, prec):
         d = o.download()
         idx = np.argsort(d["idp"], kind="stable")
         e = compare_with_reference(fx, k, o.stats()["timestep"], case.pos, d["pos"][idx],
-                                   d["velrho"][idx, :3], d["velrho"][idx, 3])
+                                   d["velrho"][idx, :3], d["velrho"][idx, 3],
+                                   order=d["idp"])
         print(name, prec, int(step), e)
 
 
