@@ -119,3 +119,70 @@ def test_pips_matches_oracle():
     g = s.pips()
     assert g[:4] == o.pips()
     assert 0 < g[4] < g[0]
+
+
+@pytest.mark.parametrize("nfluid", [1, 0])
+def test_tiny_and_empty_fluid(nfluid):
+    """Edge cases of the step: a fluid of one particle resting on the floor of
+    an empty tank (matches the oracle), and the same tank without fluid (runs,
+    nothing moves)."""
+    from dualsphysics_amd.sim import SphGpuSingle
+    from oracle.oracle import OracleSim
+    c = casemod.dam_break(dp=0.02, tank=(0.2, 0.2, 0.3), column=(0.02, 0.02, 0.02))
+    assert c.nfluid == 1
+    if nfluid == 0:
+        n = c.nbound
+        c = dataclasses.replace(c, idp=c.idp[:n], pos=c.pos[:n], velrho=c.velrho[:n], nfluid=0)
+    s = SphGpuSingle(c)
+    try:
+        s.run(5, sync_dt=True)
+        st = s.stats()
+        assert st["np"] == c.np and st["nstep"] == 5 and st["timestep"] > 0
+        g = s.download()
+        if nfluid:
+            o = OracleSim(c)
+            o.run(5)
+            od = o.download()
+            np.testing.assert_array_equal(g["idp"], od["idp"])
+            fl = g["idp"] >= c.nbound
+            # FP32 velocity rounding integrated into FP64 positions: ~1e-9 relative
+            assert rel_err(g["pos"][fl], od["pos"][fl]) < 1e-7
+            assert rel_err(g["velrho"][fl], od["velrho"][fl]) < 1e-5
+        else:
+            order = np.argsort(g["idp"])
+            np.testing.assert_array_equal(g["pos"][order], c.pos)
+    finally:
+        s.close()
+
+
+def test_half_cells_match_oracle_and_kernels_agree():
+    """CELLMODE_Half (cells of size h, 5x5 rows of 5 cells): GPU forces on a
+    stepped state equal the FP32 oracle within 1e-5 (norm-wise), the neighbour
+    order is identical, and the tile and rows fluid kernels give the same bits."""
+    from dualsphysics_amd import _native as N
+    from dualsphysics_amd.sim import SphGpuSingle
+    from oracle.oracle import OracleSim
+    c = casemod.dam_break(dp=0.02, params=casemod.Parameters(cellmode=2))
+    s = SphGpuSingle(c)
+    try:
+        s.run(3, sync_dt=True)
+        ctx = s.lib.dsph_sim_ctx(s._h)
+        out = {}
+        for k in (N.FORCES_TILE, N.FORCES_ROWS):
+            assert s.lib.dsph_ctx_set_option(ctx, N.OPT_FORCES_KERNEL, k) == 0
+            s.interact()
+            out[k] = s.forces()
+        s.lib.dsph_ctx_set_option(ctx, N.OPT_FORCES_KERNEL, N.FORCES_TILE)
+        np.testing.assert_array_equal(out[N.FORCES_TILE][0], out[N.FORCES_ROWS][0])
+        np.testing.assert_array_equal(out[N.FORCES_TILE][1], out[N.FORCES_ROWS][1])
+        g = s.download()
+        gace, gar = out[N.FORCES_TILE]
+        o = OracleSim(_state_case(c, g))
+        np.testing.assert_array_equal(o.download()["idp"], g["idp"])
+        o.interact()
+        oace, oar = o.forces()
+        npb = s.stats()["npb"]
+        assert rel_err(gace[npb:], oace[npb:]) < 1e-5
+        assert rel_err(gar, oar) < 1e-5
+    finally:
+        s.close()
