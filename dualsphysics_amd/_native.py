@@ -17,6 +17,8 @@ KERNEL_WENDLAND = 2
 VISCO_ARTIFICIAL, VISCO_LAMINARSPS, VISCO_LAMINAR = 1, 2, 3
 DDT_NONE, DDT_DDT, DDT_DDT2, DDT_DDT2FULL = 0, 1, 2, 3
 BC_DBC, BC_MDBC = 1, 2
+SLIP_VEL0, SLIP_NOSLIP = 1, 2
+BMODE_DBC, BMODE_MDBC2, BMODE_MDBC2OFF = 0, 1, 2
 OPT_FORCES_KERNEL = 1
 FORCES_TILE, FORCES_ROWS = 0, 1
 
@@ -51,7 +53,8 @@ class DsphInterParms(C.Structure):
                 ("d_dcell", C.c_void_p), ("d_poscell", C.c_void_p), ("d_velrho", C.c_void_p),
                 ("d_code", C.c_void_p), ("d_spstaurho2", C.c_void_p), ("d_viscdt", C.c_void_p),
                 ("d_ar", C.c_void_p), ("d_ace", C.c_void_p), ("d_sps2strain", C.c_void_p),
-                ("d_velgrad", C.c_void_p)]
+                ("d_velgrad", C.c_void_p), ("d_boundmode", C.c_void_p),
+                ("d_tangenvel", C.c_void_p)]
 
 
 class DsphMotion(C.Structure):
@@ -71,7 +74,8 @@ class DsphCase(C.Structure):
         ("cellmode", C.c_int32), ("celldomfixed", C.c_int32),
         ("maprealposmin", C.c_double * 3), ("maprealposmax", C.c_double * 3)] + [
         (n, C.c_uint32) for n in ("casenp", "casenbound", "casenfixed", "casenmoving",
-                                  "casenfloat", "casenfluid")] + [("motion", DsphMotion)]
+                                  "casenfloat", "casenfluid")] + [("motion", DsphMotion),
+                                                                  ("slipmode", C.c_int32)]
 
 
 class DsphMlp(C.Structure):
@@ -127,17 +131,19 @@ SIGNATURES = {
     "dsph_interaction_forces": [_vp, C.POINTER(DsphInterParms)],
     "dsph_sps_tau": [_vp, _u, _u, _f, _f, _vp, _vp, _vp],
     "dsph_mdbc_correction": [_vp, _u, C.POINTER(DsphDivData), C.POINTER(_d)] + [_vp] * 6,
+    "dsph_mdbc2_correction": [_vp, _u, C.POINTER(_f), C.POINTER(DsphDivData),
+                              C.POINTER(_d)] + [_vp] * 10,
     "dsph_velmax2": [_vp, _u, _u, _vp, _vp],
     "dsph_acemax2": [_vp, _u, _u, _vp, _vp],
     "dsph_redu_max_f32": [_vp, _u, _u, _vp, _vp],
     "dsph_redu_max_f32_sync": [_vp, _u, _u, _vp, C.POINTER(_f)],
     "dsph_dt_variable": [_vp, _vp, _d, _d, _d, _d, _vp, _u, _vp],
     "dsph_dt_advance": [_vp, _vp],
-    "dsph_step_symplectic_pre": [_vp, _u, _u, _vp, _vp, _vp, _d, _vp, _u, _f, _f, _f,
+    "dsph_step_symplectic_pre": [_vp, _u, _u, _vp, _vp, _vp, _vp, _d, _vp, _u, _f, _f, _f,
                                  C.POINTER(_f), _vp, _vp, _vp, _vp],
-    "dsph_step_symplectic_cor": [_vp, _u, _u, _vp, _vp, _vp, _d, _d, _vp, _u, _f, _f, _f,
+    "dsph_step_symplectic_cor": [_vp, _u, _u, _vp, _vp, _vp, _vp, _d, _d, _vp, _u, _f, _f, _f,
                                  C.POINTER(_f), _vp, _vp, _vp, _vp],
-    "dsph_step_verlet": [_vp, _u, _u, _vp, _vp, _vp, _vp, _d, _d, _f, _f, _f, C.POINTER(_f),
+    "dsph_step_verlet": [_vp, _u, _u, _vp, _vp, _vp, _vp, _vp, _d, _d, _f, _f, _f, C.POINTER(_f),
                          _vp, _vp, _vp, _vp],
     "dsph_step_pos2": [_vp, _u, _u] + [_vp] * 8,
     "dsph_step_pos": [_vp, _u, _u] + [_vp] * 6,

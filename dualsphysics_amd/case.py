@@ -59,6 +59,7 @@ class Parameters:
     tdensity: int = N.DDT_DDT2FULL
     ddtvalue: float = 0.1
     tboundary: int = N.BC_DBC
+    slipmode: int = N.SLIP_VEL0     # mDBC SlipMode 1 vel=0, 2 no-slip (mDBC2)
     cflnumber: float = 0.2
     dtini: float = 0.0
     dtmin: float = 0.0
@@ -114,6 +115,7 @@ class Case:
         c.visco, c.viscoboundfactor, c.ddtvalue = p.visco, p.viscoboundfactor, p.ddtvalue
         c.rhopoutmin, c.rhopoutmax = p.rhopoutmin, p.rhopoutmax
         c.cellmode, c.celldomfixed = p.cellmode, p.celldomfixed
+        c.slipmode = p.slipmode
         c.casenp, c.casenbound = self.np, self.nbound
         c.casenfixed = self.nbound - self.nmoving
         c.casenmoving, c.casenfloat, c.casenfluid = self.nmoving, 0, self.nfluid
@@ -175,7 +177,9 @@ def dam_break(dp: float = 0.01, tank=(1.6, 0.67, 0.4), column=(0.4, 0.67, 0.3), 
               gamma: float = 7.0, g: float = 9.81, seed: int = 42, jitter: float = 1e-4,
               params: Parameters | None = None, name: str | None = None) -> Case:
     """3-D dam break in a box tank (the reference's CaseDambreak geometry without
-    the obstacle): column ``column`` at the x=0 end of tank ``tank``."""
+    the obstacle): column ``column`` at the x=0 end of tank ``tank``.
+    ``boundnor`` (mDBC normals) is the vector from each wall particle to the
+    nearest point of the tank interior [0,L]x[0,W]x[0,inf)."""
     params = params or Parameters()
     nxt, nyt, nzt = _n(tank[0], dp), _n(tank[1], dp), _n(tank[2], dp)
     nxf, nyf, nzf = _n(column[0], dp), _n(column[1], dp), _n(column[2], dp)
@@ -195,12 +199,16 @@ def dam_break(dp: float = 0.01, tank=(1.6, 0.67, 0.4), column=(0.4, 0.67, 0.3), 
     velrho[:nb, 3] = rhop0
     zf = pos[nb:, 2]
     velrho[nb:, 3] = (rhop0 * (1.0 + rhop0 * g * (hswl - zf) / b) ** (1.0 / gamma)).astype(np.float32)
+    bp = pos[:nb]
+    q = np.stack([np.clip(bp[:, 0], 0.0, nxt * dp), np.clip(bp[:, 1], 0.0, nyt * dp),
+                  np.maximum(bp[:, 2], 0.0)], 1)
+    boundnor = (q - bp).astype(np.float32)
     idp = np.arange(pos.shape[0], dtype=np.uint32)
     mn, mx = map_limits(pos, h)
     return Case(name=name or f"dambreak_dp{dp:g}", dp=dp, h=h, rhop0=rhop0, gamma=gamma,
                 cs0=cs0, b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g),
                 hswl=hswl, params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb,
-                nfluid=nf, maprealposmin=mn, maprealposmax=mx)
+                nfluid=nf, maprealposmin=mn, maprealposmax=mx, boundnor=boundnor)
 
 
 def dam_break_cfg0(**kw) -> Case:

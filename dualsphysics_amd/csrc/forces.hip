@@ -93,14 +93,20 @@ struct P1 {
 // variant (mass, viscosity, no SPS tau of p2, delta-SPH rules).
 // pair_eval takes the neighbour's PosCell, velrho and {pressure, 1/rho} already loaded
 // and applies the exact neighbour test rr2 <= (2h)^2, rr2 >= ALMOSTZERO.
-template <int TVISCO, int TDDT, bool VELGRAD>
+// MDBC2: boundary neighbours in mDBC2 mode (boundmode, tangential velocity):
+// no mass when their ghost node is dry, viscous terms against the tangential
+// velocity (JSphGpu_ker.cu:585-588, 725-730; velocities kept for continuity,
+// MDBC2_KEEPVEL).
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false>
 __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float4& ps2,
     const float4& vr2, float2 pr2, const P1& q, const float* __restrict__ tau, float viscob,
-    float viscof, const Cte& c, Acc& a) {
+    float viscof, const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
+    const float* __restrict__ tangenvel = nullptr) {
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
-  const float massp2 = boundp2 ? c.massb : c.massf;
+  float massp2 = boundp2 ? c.massb : c.massf;
+  if (MDBC2 && boundp2 && bmode[p2] == DSPH_BMODE_MDBC2OFF) massp2 = 0.f;
   const float visco = boundp2 ? viscob : viscof;
   const float fac = wendland_fac(rr2, c);
   const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
@@ -110,7 +116,7 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
     const float p_vpm = -prs * massp2;
     a.ax += p_vpm * frx; a.ay += p_vpm * fry; a.az += p_vpm * frz;
   }
-  const float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
+  float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
   const float rrho2 = pr2.y;
   a.ar += massp2 * (dvx * frx + dvy * fry + dvz * frz) * (vr1.w * rrho2);
   const float cbar = c.cs0;
@@ -128,6 +134,10 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
     const float visc_densi = c.ddtkh * cbar * ((vr2.w - vr1.w) - drho) * rr2eta;
     const float delta = visc_densi * dot3 * massp2 * rrho2;
     a.delta = (boundp2 ? FLT_MAX : a.delta - delta);
+  }
+  if (MDBC2 && boundp2) {
+    const float* tv = tangenvel + 3 * size_t(p2);
+    dvx = vr1.x - tv[0]; dvy = vr1.y - tv[1]; dvz = vr1.z - tv[2];
   }
   const float dot = dr.x * dvx + dr.y * dvy + dr.z * dvz;
   const float dot_rr2 = dot * rr2eta;
@@ -172,17 +182,18 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
 
 // Reference-structured pair: loads PosCell first and the rest only for a
 // real neighbour.
-template <int TVISCO, int TDDT, bool VELGRAD>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false>
 __device__ __forceinline__ void pair_body(bool boundp2, unsigned p2, const P1& q,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
     const float2* __restrict__ press, const float* __restrict__ tau, float viscob, float viscof,
-    const Cte& c, Acc& a) {
+    const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
+    const float* __restrict__ tangenvel = nullptr) {
   const float4 ps2 = poscell[p2];
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
-  pair_eval<TVISCO, TDDT, VELGRAD>(boundp2, p2, ps2, velrho[p2], press[p2], q, tau, viscob,
-                                   viscof, c, a);
+  pair_eval<TVISCO, TDDT, VELGRAD, MDBC2>(boundp2, p2, ps2, velrho[p2], press[p2], q, tau,
+                                          viscob, viscof, c, a, bmode, tangenvel);
 }
 
 template <int TVISCO>
@@ -233,13 +244,14 @@ __device__ inline NgLimits p1_limits(unsigned p1, const unsigned* __restrict__ d
 }
 
 // ------------------------------------------------------------ reference-order rows kernel
-template <int TVISCO, int TDDT, bool VELGRAD>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false>
 __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float viscof,
     const DivData& dvd, const Cte& c, const unsigned* __restrict__ dcell,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
     const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
     float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
-    float* __restrict__ velgrad) {
+    float* __restrict__ velgrad, const uint8_t* __restrict__ bmode = nullptr,
+    const float* __restrict__ tangenvel = nullptr) {
   Acc a;
   acc_init(a);
   P1 q;
@@ -252,24 +264,26 @@ __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float
         unsigned pini, pfin;
         row_range(dvd.begincell, base + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1, pini, pfin);
         for (unsigned p2 = pini; p2 < pfin; p2++)
-          pair_body<TVISCO, TDDT, VELGRAD>(kind == 1, p2, q, poscell, velrho, press, tau, viscob,
-                                           viscof, c, a);
+          pair_body<TVISCO, TDDT, VELGRAD, MDBC2>(kind == 1, p2, q, poscell, velrho, press, tau,
+                                                  viscob, viscof, c, a, bmode, tangenvel);
       }
   }
   store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
 }
 
-template <int TVISCO, int TDDT, bool VELGRAD>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2>
 __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ velrho, const float2* __restrict__ press,
     const float* __restrict__ tau, float* __restrict__ viscdt, float* __restrict__ ar,
-    float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad) {
+    float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad,
+    const uint8_t* __restrict__ bmode, const float* __restrict__ tangenvel) {
   const unsigned p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  fluid_rows_body<TVISCO, TDDT, VELGRAD>(p + pinit, viscob, viscof, dvd, c, dcell, poscell,
-                                         velrho, press, tau, viscdt, ar, ace, strain, velgrad);
+  fluid_rows_body<TVISCO, TDDT, VELGRAD, MDBC2>(p + pinit, viscob, viscof, dvd, c, dcell,
+                                                poscell, velrho, press, tau, viscdt, ar, ace,
+                                                strain, velgrad, bmode, tangenvel);
 }
 
 // ------------------------------------------------------------ cell-row tile kernel
@@ -585,17 +599,24 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
                   const float2* press, const float4* posf) {
   KTimer kt(ctx, "k_forces_fluid");
   const unsigned npf = t->np - t->npb;
-  if (ctx->forces_kernel == 0) {
+  if (t->d_boundmode) {
+    // mDBC2 boundaries: the one-lane-per-particle kernel.
+    k_forces_fluid_rows<TVISCO, TDDT, VELGRAD, true><<<grid_for(npf, kBsForces), kBsForces, 0,
+                                                       ctx->stream>>>(
+        npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
+        (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
+        t->d_sps2strain, t->d_velgrad, t->d_boundmode, t->d_tangenvel);
+  } else if (ctx->forces_kernel == 0) {
     k_forces_fluid_tile<TVISCO, TDDT, VELGRAD, kRingTiles><<<grid_for(npf, 64), 64, 0, ctx->stream>>>(
         npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
         (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press, t->d_spstaurho2,
         t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, t->d_velgrad);
   } else {
-    k_forces_fluid_rows<TVISCO, TDDT, VELGRAD><<<grid_for(npf, kBsForces), kBsForces, 0,
-                                                 ctx->stream>>>(
+    k_forces_fluid_rows<TVISCO, TDDT, VELGRAD, false><<<grid_for(npf, kBsForces), kBsForces, 0,
+                                                        ctx->stream>>>(
         npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
         (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
-        t->d_sps2strain, t->d_velgrad);
+        t->d_sps2strain, t->d_velgrad, nullptr, nullptr);
   }
 }
 
@@ -788,6 +809,7 @@ int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
   DSPH_REQUIRE(ctx, t->d_dcell && t->d_poscell && t->d_velrho && t->d_viscdt && t->d_ar &&
                         t->d_ace && t->divdata.d_beginendcell,
                "dsph_interaction_forces: null array");
+  DSPH_REQUIRE(ctx, !t->d_boundmode || t->d_tangenvel, "mDBC2 needs boundmode and tangenvel");
   const bool sps = t->tvisco == DSPH_VISCO_LAMINARSPS;
   DSPH_REQUIRE(ctx, !sps || (t->d_spstaurho2 && t->d_sps2strain),
                "Laminar+SPS needs spstaurho2 and sps2strain");

@@ -74,6 +74,7 @@ struct dsph_sim {
   float Gravity[3] = {0, 0, 0};
   double Cs0 = 0, DtIni = 0, DtMin = 0, CFLnumber = 0.2;
   int TStep = 2, VerletSteps = 40, TVisco = 1, TDensity = 0, TBoundary = 1, ScellDiv = 1;
+  int SlipMode = DSPH_SLIP_VEL0;
   bool CellDomFixed = false;
   double MapRealPosMin[3], MapRealPosMax[3], MapRealSize[3];
   double Map_PosMin[3], Map_PosMax[3], Map_Size[3];
@@ -94,6 +95,10 @@ struct dsph_sim {
   double *movxy = nullptr, *movz = nullptr;
   float *sps2strain = nullptr, *spstau = nullptr, *spstau2 = nullptr;
   float *boundnor = nullptr, *boundnor2 = nullptr;
+  // mDBC2: boundmode (DSPH_BMODE_*) and tangential velocity of [0,Npb),
+  // rewritten by every correction (JSphGpuSingle.cpp:700-707).
+  uint8_t* boundmode = nullptr;
+  float* tangenvel = nullptr;
   bool pre_active = false;
   // device scalars: red[0..2] (velmax^2, viscdtmax, acemax^2), dt slots, flags
   float* d_red = nullptr;
@@ -140,7 +145,8 @@ struct dsph_sim {
     void* ptrs[] = {idp, dcell, idp2, dcell2, code, code2, posxy, posz, posxy2, posz2, velrho,
                     velrho2, poscell, posxypre, poszpre, posxypre2, poszpre2, velrhopre,
                     velrhopre2, velrhom1, velrhom1b, ace, ar, viscdt, movxy, movz, sps2strain,
-                    spstau, spstau2, boundnor, boundnor2, d_red, d_dt, d_flags, cellpart,
+                    spstau, spstau2, boundnor, boundnor2, boundmode, tangenvel, d_red, d_dt,
+                    d_flags, cellpart,
                     sortpart, begincell, velgrad, closure_nu};
     for (void* p : ptrs) dsph_free(ctx, p);
     dsph_ctx_destroy(ctx);
@@ -164,6 +170,11 @@ struct dsph_sim {
     TVisco = c.tvisco;
     TDensity = c.tdensity;
     TBoundary = c.tboundary;
+    SlipMode = c.slipmode ? c.slipmode : DSPH_SLIP_VEL0;
+    if (TBoundary == DSPH_BC_MDBC && SlipMode != DSPH_SLIP_VEL0 && SlipMode != DSPH_SLIP_NOSLIP)
+      throw Err(DSPH_E_ARG, "SlipMode: only 1 (vel=0) and 2 (no-slip) are built");
+    if (TBoundary == DSPH_BC_MDBC && SlipMode == DSPH_SLIP_NOSLIP && c.casenmoving)
+      throw Err(DSPH_E_ARG, "mDBC no-slip is built for fixed boundaries only");
     Visco = float(c.visco);
     ViscoBoundFactor = float(c.viscoboundfactor);
     RhopOutMin = float(c.rhopoutmin);
@@ -286,6 +297,10 @@ struct dsph_sim {
     }
     if (TBoundary == DSPH_BC_MDBC) {
       boundnor = alloc<float>(3 * n); boundnor2 = alloc<float>(3 * n);
+      if (SlipMode == DSPH_SLIP_NOSLIP) {
+        boundmode = alloc<uint8_t>(n);
+        tangenvel = alloc<float>(3 * n);
+      }
     }
     d_red = alloc<float>(4);
     d_dt = alloc<double>(8);
@@ -490,9 +505,19 @@ struct dsph_sim {
   }
 
   // ---- JSphGpuSingle::MdbcBoundCorrection (JSphGpuSingle.cpp:686-712)
+  // SLIP_Vel0 corrects before the predictor only (MdbcCorrector=false);
+  // no-slip before both interactions (MdbcCorrector=true, JSph.cpp:897).
   void mdbc(bool corrector) {
-    if (TBoundary != DSPH_BC_MDBC || corrector) return;  // MdbcCorrector=false default
+    if (TBoundary != DSPH_BC_MDBC) return;
     const dsph_divdata d = divdata();
+    if (SlipMode == DSPH_SLIP_NOSLIP) {
+      chk(dsph_memset_async(ctx, boundmode, DSPH_BMODE_DBC, Npb), "memset boundmode");
+      chk(dsph_mdbc2_correction(ctx, Npb, Gravity, &d, Map_PosMin, posxy, posz, poscell, code,
+                                boundnor, nullptr, nullptr, velrho, boundmode, tangenvel),
+          "mdbc2");
+      return;
+    }
+    if (corrector) return;
     chk(dsph_mdbc_correction(ctx, NpbOk, &d, Map_PosMin, posxy, posz, poscell, code, boundnor, velrho),
         "mdbc");
   }
@@ -511,6 +536,8 @@ struct dsph_sim {
     t.d_spstaurho2 = spstau;
     t.d_viscdt = viscdt; t.d_ar = ar; t.d_ace = ace; t.d_sps2strain = sps2strain;
     t.d_velgrad = UseClosure ? velgrad : nullptr;
+    t.d_boundmode = boundmode;
+    t.d_tangenvel = tangenvel;
     chk(dsph_interaction_forces(ctx, &t), "interaction");
     if (TVisco == DSPH_VISCO_LAMINARSPS) {
       if (UseClosure)
@@ -552,7 +579,7 @@ struct dsph_sim {
   void symplectic_pre(double dt) {
     swp(posxypre, posxy); swp(poszpre, posz); swp(velrhopre, velrho);
     pre_active = true;
-    chk(dsph_step_symplectic_pre(ctx, Np, Npb, velrhopre, ar, ace, dt * .5,
+    chk(dsph_step_symplectic_pre(ctx, Np, Npb, velrhopre, boundmode, ar, ace, dt * .5,
                                  device_dt ? d_dt : nullptr, 2, RhopZero, RhopOutMin, RhopOutMax,
                                  Gravity, code, movxy, movz, velrho),
         "symplectic pre");
@@ -561,7 +588,7 @@ struct dsph_sim {
     chk(dsph_memcpy_d2d_async(ctx, posz, poszpre, 8 * size_t(Npb)), "copy bound z");
   }
   void symplectic_cor(double dt) {
-    chk(dsph_step_symplectic_cor(ctx, Np, Npb, velrhopre, ar, ace, dt * .5, dt,
+    chk(dsph_step_symplectic_cor(ctx, Np, Npb, velrhopre, boundmode, ar, ace, dt * .5, dt,
                                  device_dt ? d_dt : nullptr, 2, RhopZero, RhopOutMin, RhopOutMax,
                                  Gravity, code, movxy, movz, velrho),
         "symplectic cor");
@@ -626,11 +653,11 @@ struct dsph_sim {
     const double dt = dt_variable(0);
     VerletStep++;
     if (VerletStep < unsigned(VerletSteps)) {
-      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrhom1, ar, ace, dt, dt + dt, RhopZero,
+      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrhom1, boundmode, ar, ace, dt, dt + dt, RhopZero,
                            RhopOutMin, RhopOutMax, Gravity, code, movxy, movz, velrhom1),
           "verlet");
     } else {
-      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrho, ar, ace, dt, dt, RhopZero, RhopOutMin,
+      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrho, boundmode, ar, ace, dt, dt, RhopZero, RhopOutMin,
                            RhopOutMax, Gravity, code, movxy, movz, velrhom1),
           "verlet euler");
       VerletStep = 0;

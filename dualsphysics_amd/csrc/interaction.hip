@@ -163,6 +163,146 @@ __global__ void __launch_bounds__(kBsForces) k_mdbc(unsigned n, DivData dvd, Cte
   velrho[p1].w = (rhofinal != FLT_MAX ? rhofinal : c.rhopzero);
 }
 
+// mDBC2, SlipMode no-slip with the boundary velocities kept (MDBC2_KEEPVEL,
+// DualSphDef.h:471); KerInteractionMdbc2Correction_Fast,
+// JSphGpu_mdbc_iker.cu:289-487 (3-D). Shepard sums, the 4x4 first-order
+// matrix (columns 1..3 with -dr) and the submerged test at the ghost node;
+// density from the corrected system when it is invertible and
+// dp^2 |A|_inf |A^-1|_inf <= 50 (determinant and inverse in FP64 from the FP32
+// sums, inverse rounded to FP32), Shepard otherwise (clamped at rho0 when the
+// kernel sum is < 0.1); pressure cloned to the particle with the hydrostatic
+// gradient (KerMdbc2PressClone); tangential part of 2 v_motion - v_ghost
+// (KerMdbc2TangenVel). A dry ghost node switches the particle's mass off.
+__device__ inline float3 mdbc2_tangential(float bx, float by, float bz, float3 v) {
+  const float sn = sqrtf(bx * bx + by * by + bz * bz);
+  const float nx = bx / sn, ny = by / sn, nz = bz / sn;
+  const float vdn = v.x * nx + v.y * ny + v.z * nz;
+  return make_float3(v.x - vdn * nx, v.y - vdn * ny, v.z - vdn * nz);
+}
+
+__global__ void __launch_bounds__(kBsForces) k_mdbc2(unsigned n, DivData dvd, Cte c,
+    float3 gravity, double3 mapposmin, const double2* __restrict__ posxy,
+    const double* __restrict__ posz, const float4* __restrict__ poscell,
+    const typecode* __restrict__ code, const float* __restrict__ boundnor,
+    const float* __restrict__ motionvel, const float* __restrict__ motionace,
+    float4* __restrict__ velrho, uint8_t* __restrict__ boundmode, float* __restrict__ tangenvel) {
+  const unsigned p1 = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p1 >= n) return;
+  const float bnx = boundnor[3 * size_t(p1)], bny = boundnor[3 * size_t(p1) + 1],
+              bnz = boundnor[3 * size_t(p1) + 2];
+  if (bnx == 0 && bny == 0 && bnz == 0) return;
+  const float3 mv = motionvel ? make_float3(motionvel[3 * size_t(p1)], motionvel[3 * size_t(p1) + 1],
+                                            motionvel[3 * size_t(p1) + 2])
+                              : make_float3(0, 0, 0);
+  const float3 ma = motionace ? make_float3(motionace[3 * size_t(p1)], motionace[3 * size_t(p1) + 1],
+                                            motionace[3 * size_t(p1) + 2])
+                              : make_float3(0, 0, 0);
+  const double gx = posxy[p1].x + bnx, gy = posxy[p1].y + bny, gz = posz[p1] + bnz;
+  float4 gps;
+  {
+    const double dx = gx - mapposmin.x, dy = gy - mapposmin.y, dz = gz - mapposmin.z;
+    const unsigned cx = unsigned(dx / c.pcs), cy = unsigned(dy / c.pcs), cz = unsigned(dz / c.pcs);
+    gps = make_float4(float(dx - double(c.pcs) * cx), float(dy - double(c.pcs) * cy),
+                      float(dz - double(c.pcs) * cz), __uint_as_float(pscel_code(cx, cy, cz)));
+  }
+  float rhop1 = 0, grx = 0, gry = 0, grz = 0, sumwab = 0, submerged = 0;
+  float vpx = 0, vpy = 0, vpz = 0;
+  float A[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) A[i][j] = 0;
+  const int cx = int((gx - dvd.domposminx) / dvd.scell) - dvd.czx;
+  const int cy = int((gy - dvd.domposminy) / dvd.scell) - dvd.czy;
+  const int cz = int((gz - dvd.domposminz) / dvd.scell) - dvd.czz;
+  const NgLimits l = ng_limits(cx, cy, cz, dvd);
+  for (int z = l.z0; z < l.z1; z++)
+    for (int y = l.y0; y < l.y1; y++) {
+      unsigned pini, pfin;
+      row_range(dvd.begincell, int(dvd.cellfluid) + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1,
+                pini, pfin);
+      for (unsigned p2 = pini; p2 < pfin; p2++) {
+        const float4 ps2 = poscell[p2];
+        const float3 dr = pscel_dr(gps, ps2, c.pcs);
+        const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
+        if (rr2 <= c.ks2 && code_is_fluid(code[p2])) {
+          float fac;
+          const float wab = wendland_wab_fac(rr2, c, fac);
+          const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
+          const float4 vr2 = velrho[p2];
+          const float volp2 = c.massf / vr2.w;
+          submerged -= volp2 * (dr.x * frx + dr.y * fry + dr.z * frz);
+          rhop1 += c.massf * wab;
+          grx += c.massf * frx; gry += c.massf * fry; grz += c.massf * frz;
+          const float vwab = wab * volp2;
+          sumwab += vwab;
+          const float vfrx = frx * volp2, vfry = fry * volp2, vfrz = frz * volp2;
+          vpx += vwab * vr2.x; vpy += vwab * vr2.y; vpz += vwab * vr2.z;
+          A[0][0] += vwab; A[0][1] -= dr.x * vwab; A[0][2] -= dr.y * vwab; A[0][3] -= dr.z * vwab;
+          A[1][0] += vfrx; A[1][1] -= dr.x * vfrx; A[1][2] -= dr.y * vfrx; A[1][3] -= dr.z * vfrx;
+          A[2][0] += vfry; A[2][1] -= dr.x * vfry; A[2][2] -= dr.y * vfry; A[2][3] -= dr.z * vfry;
+          A[3][0] += vfrz; A[3][1] -= dr.x * vfrz; A[3][2] -= dr.y * vfrz; A[3][3] -= dr.z * vfrz;
+        }
+      }
+    }
+  float* tv = tangenvel + 3 * size_t(p1);
+  if (!(submerged > 0.f)) {
+    boundmode[p1] = DSPH_BMODE_MDBC2OFF;
+    velrho[p1].w = c.rhopzero;
+    const float3 t = mdbc2_tangential(bnx, bny, bnz, mv);
+    tv[0] = t.x; tv[1] = t.y; tv[2] = t.z;
+    return;
+  }
+  boundmode[p1] = DSPH_BMODE_MDBC2;
+  float rhofinal;
+  if (sumwab < 0.1f) {
+    rhofinal = fmaxf(c.rhopzero, rhop1 / A[0][0]);
+  } else {
+    M4d Ad, inv;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) Ad.m[i][j] = double(A[i][j]);
+    const double det = det4_inv(Ad, inv);
+    if (fabs(det) >= 0.001) {
+      float fi[4][4];
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) fi[i][j] = float(inv.m[i][j]);
+      float na = 0, ni = 0;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        na = fmaxf(na, fabsf(A[i][0]) + fabsf(A[i][1]) + fabsf(A[i][2]) + fabsf(A[i][3]));
+        ni = fmaxf(ni, fabsf(fi[i][0]) + fabsf(fi[i][1]) + fabsf(fi[i][2]) + fabsf(fi[i][3]));
+      }
+      const float condinf = c.dp * c.dp * na * ni;
+      rhofinal = (condinf <= 50.f)
+                     ? fi[0][0] * rhop1 + fi[0][1] * grx + fi[0][2] * gry + fi[0][3] * grz
+                     : rhop1 / A[0][0];
+    } else {
+      rhofinal = rhop1 / A[0][0];
+    }
+  }
+  {
+    const float sn = sqrtf(bnx * bnx + bny * bny + bnz * bnz);
+    const float nx = bnx / sn, ny = bny / sn, nz = bnz / sn;
+    const float normpos = (-bnx) * nx + (-bny) * ny + (-bnz) * nz;
+    const float normforce = c.rhopzero * ((gravity.x - ma.x) * nx + (gravity.y - ma.y) * ny +
+                                          (gravity.z - ma.z) * nz);
+    const float cs2 = c.cs0 * c.cs0;
+    const float pghost = cs2 * (rhofinal - c.rhopzero);
+    const float pressfinal = pghost + normforce * normpos;
+    rhofinal = c.rhopzero + pressfinal / cs2;
+  }
+  velrho[p1].w = rhofinal;
+  const float ia = A[0][0];
+  const float3 v2 = make_float3(mv.x + mv.x - vpx / ia, mv.y + mv.y - vpy / ia,
+                                mv.z + mv.z - vpz / ia);
+  const float3 t = mdbc2_tangential(bnx, bny, bnz, v2);
+  tv[0] = t.x; tv[1] = t.y; tv[2] = t.z;
+}
+
 // ---------------------------------------------------------------- PIPS
 // Real and checked neighbour counts (KerInteractionNg, JDsPips_ker.cu:54-124):
 // boundary p1 in [pinitb,pinitb+nb) count fluid neighbours; fluid p1 in
@@ -249,6 +389,28 @@ int dsph_mdbc_correction(dsph_ctx* ctx, uint32_t n, const dsph_divdata* dvdp,
       (const double2*)d_posxy, d_posz, (const float4*)d_poscell, d_code, d_boundnor,
       (float4*)d_velrho);
   DSPH_LAUNCHED(ctx, "k_mdbc");
+  return DSPH_OK;
+}
+
+int dsph_mdbc2_correction(dsph_ctx* ctx, uint32_t n, const float gravity[3],
+                          const dsph_divdata* dvdp, const double mapposmin[3],
+                          const double* d_posxy, const double* d_posz, const float* d_poscell,
+                          const uint16_t* d_code, const float* d_boundnor,
+                          const float* d_motionvel, const float* d_motionace, float* d_velrho,
+                          uint8_t* d_boundmode, float* d_tangenvel) {
+  if (!ctx || !dvdp || !mapposmin || !gravity) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  if (!n) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_posxy && d_posz && d_poscell && d_code && d_boundnor && d_velrho &&
+                        d_boundmode && d_tangenvel,
+               "dsph_mdbc2_correction: null array");
+  const Cte c = make_cte(ctx->cte);
+  k_mdbc2<<<grid_for(n, kBsForces), kBsForces, 0, ctx->stream>>>(
+      n, make_divdata(*dvdp), c, make_float3(gravity[0], gravity[1], gravity[2]),
+      make_double3(mapposmin[0], mapposmin[1], mapposmin[2]), (const double2*)d_posxy, d_posz,
+      (const float4*)d_poscell, d_code, d_boundnor, d_motionvel, d_motionace, (float4*)d_velrho,
+      d_boundmode, d_tangenvel);
+  DSPH_LAUNCHED(ctx, "k_mdbc2");
   return DSPH_OK;
 }
 

@@ -152,14 +152,25 @@ __device__ inline void sym_pre_one(unsigned p, bool bound, double dtm,
   velrho[p] = nw;
 }
 
+// mDBC2: a boundary particle with boundmode >= 1 keeps the state the step
+// starts from (JSphGpuSimple_ker.cu:142, 276, 406-412).
+__device__ inline bool mdbc2_keep(unsigned p, unsigned npb, const uint8_t* __restrict__ boundmode) {
+  return boundmode && p < npb && boundmode[p] >= DSPH_BMODE_MDBC2;
+}
+
 __global__ void __launch_bounds__(kBs) k_sym_pre(unsigned n, unsigned npb,
-    const float4* __restrict__ velrhopre, const float* __restrict__ ar,
+    const float4* __restrict__ velrhopre, const uint8_t* __restrict__ boundmode,
+    const float* __restrict__ ar,
     const float* __restrict__ ace, double dtm_h, const double* __restrict__ d_dt, unsigned slot,
     float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
     typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
     float4* __restrict__ velrho) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  if (mdbc2_keep(p, npb, boundmode)) {
+    velrho[p] = velrhopre[p];
+    return;
+  }
   sym_pre_one(p, p < npb, dev_dt(dtm_h, d_dt, slot, 0.5), velrhopre, ar, ace, rhopzero,
               rhopoutmin, rhopoutmax, gravity, code, movxy, movz, velrho);
 }
@@ -196,13 +207,18 @@ __device__ inline void sym_cor_one(unsigned p, bool bound, double dt, double dtm
 }
 
 __global__ void __launch_bounds__(kBs) k_sym_cor(unsigned n, unsigned npb,
-    const float4* __restrict__ velrhopre, const float* __restrict__ ar,
+    const float4* __restrict__ velrhopre, const uint8_t* __restrict__ boundmode,
+    const float* __restrict__ ar,
     const float* __restrict__ ace, double dtm_h, double dt_h, const double* __restrict__ d_dt,
     unsigned slot, float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
     typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
     float4* __restrict__ velrho) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  if (mdbc2_keep(p, npb, boundmode)) {
+    velrho[p] = velrhopre[p];
+    return;
+  }
   sym_cor_one(p, p < npb, dev_dt(dt_h, d_dt, slot, 1.0), dev_dt(dtm_h, d_dt, slot, 0.5),
               velrhopre, ar, ace, rhopzero, rhopoutmin, rhopoutmax, gravity, code, movxy, movz,
               velrho);
@@ -210,12 +226,17 @@ __global__ void __launch_bounds__(kBs) k_sym_cor(unsigned n, unsigned npb,
 
 __global__ void __launch_bounds__(kBs) k_verlet(unsigned n, unsigned npb,
     const float4* __restrict__ velrho1, const float4* __restrict__ velrho2,
+    const uint8_t* __restrict__ boundmode,
     const float* __restrict__ ar, const float* __restrict__ ace, double dt, double dt205,
     double dt2, float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
     typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
     float4* __restrict__ velrhonew) {
   const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= n) return;
+  if (mdbc2_keep(p, npb, boundmode)) {
+    velrhonew[p] = velrho2[p];
+    return;
+  }
   if (p < npb) {
     float rrho = float(double(velrho2[p].w) + dt2 * ar[p]);
     rrho = (rrho < rhopzero ? rhopzero : rrho);
@@ -595,7 +616,7 @@ int dsph_dt_advance(dsph_ctx* ctx, double* d_dt) {
 }
 
 int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_velrhopre,
-                             const float* d_ar, const float* d_ace, double dtm,
+                             const uint8_t* d_boundmode, const float* d_ar, const float* d_ace, double dtm,
                              const double* d_dt, uint32_t dt_slot, float rhopzero,
                              float rhopoutmin, float rhopoutmax, const float gravity[3],
                              uint16_t* d_code, double* d_movxy, double* d_movz,
@@ -606,7 +627,8 @@ int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb, const flo
                "dsph_step_symplectic_pre: null array");
   KTimer kt(ctx, "k_sym_pre");
   k_sym_pre<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
-      np, npb, (const float4*)d_velrhopre, d_ar, d_ace, dtm, d_dt, dt_slot, rhopzero, rhopoutmin,
+      np, npb, (const float4*)d_velrhopre, d_boundmode, d_ar, d_ace, dtm, d_dt, dt_slot,
+      rhopzero, rhopoutmin,
       rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code, (double2*)d_movxy,
       d_movz, (float4*)d_velrho);
   DSPH_LAUNCHED(ctx, "k_sym_pre");
@@ -614,7 +636,7 @@ int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb, const flo
 }
 
 int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_velrhopre,
-                             const float* d_ar, const float* d_ace, double dtm, double dt,
+                             const uint8_t* d_boundmode, const float* d_ar, const float* d_ace, double dtm, double dt,
                              const double* d_dt, uint32_t dt_slot, float rhopzero,
                              float rhopoutmin, float rhopoutmax, const float gravity[3],
                              uint16_t* d_code, double* d_movxy, double* d_movz,
@@ -625,7 +647,8 @@ int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb, const flo
                "dsph_step_symplectic_cor: null array");
   KTimer kt(ctx, "k_sym_cor");
   k_sym_cor<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
-      np, npb, (const float4*)d_velrhopre, d_ar, d_ace, dtm, dt, d_dt, dt_slot, rhopzero,
+      np, npb, (const float4*)d_velrhopre, d_boundmode, d_ar, d_ace, dtm, dt, d_dt, dt_slot,
+      rhopzero,
       rhopoutmin, rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code,
       (double2*)d_movxy, d_movz, (float4*)d_velrho);
   DSPH_LAUNCHED(ctx, "k_sym_cor");
@@ -633,7 +656,7 @@ int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb, const flo
 }
 
 int dsph_step_verlet(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_velrho1,
-                     const float* d_velrho2, const float* d_ar, const float* d_ace, double dt,
+                     const float* d_velrho2, const uint8_t* d_boundmode, const float* d_ar, const float* d_ace, double dt,
                      double dt2, float rhopzero, float rhopoutmin, float rhopoutmax,
                      const float gravity[3], uint16_t* d_code, double* d_movxy, double* d_movz,
                      float* d_velrhonew) {
@@ -645,7 +668,8 @@ int dsph_step_verlet(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_ve
   const double dt205 = 0.5 * dt * dt;
   KTimer kt(ctx, "k_verlet");
   k_verlet<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
-      np, npb, (const float4*)d_velrho1, (const float4*)d_velrho2, d_ar, d_ace, dt, dt205, dt2,
+      np, npb, (const float4*)d_velrho1, (const float4*)d_velrho2, d_boundmode, d_ar, d_ace, dt,
+      dt205, dt2,
       rhopzero, rhopoutmin, rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code,
       (double2*)d_movxy, d_movz, (float4*)d_velrhonew);
   DSPH_LAUNCHED(ctx, "k_verlet");

@@ -62,6 +62,11 @@ extern "C" {
 #define DSPH_DDT_DDT2FULL     3   /* Fourtakas (full) */
 #define DSPH_BC_DBC           1   /* TpBoundary */
 #define DSPH_BC_MDBC          2
+#define DSPH_SLIP_VEL0        1   /* TpSlipMode: mDBC, boundary velocity 0 */
+#define DSPH_SLIP_NOSLIP      2   /* mDBC2 no-slip (MDBC2_Std) */
+#define DSPH_BMODE_DBC        0   /* boundmode of a boundary particle */
+#define DSPH_BMODE_MDBC2      1
+#define DSPH_BMODE_MDBC2OFF   2   /* ghost node not submerged: mass off */
 
 typedef struct dsph_ctx dsph_ctx;
 
@@ -134,6 +139,8 @@ typedef struct dsph_inter_parms {
   float* d_sps2strain;           /* sym6[np]    (in/out, VISCO_LAMINARSPS) */
   float* d_velgrad;              /* float[9*np] full velocity gradient (learned
                                     closure features) or NULL */
+  const uint8_t* d_boundmode;    /* mDBC2: DSPH_BMODE_* of [0,npb), else NULL */
+  const float*   d_tangenvel;    /* mDBC2: float3[npb] tangential velocity */
 } dsph_inter_parms;
 
 /* ---------------------------------------------------------------- context */
@@ -337,6 +344,23 @@ int dsph_mdbc_correction(dsph_ctx* ctx, uint32_t n, const dsph_divdata* dvd,
                          const uint16_t* d_code, const float* d_boundnor,
                          float* d_velrho);
 
+/* Replaces cusph::Interaction_Mdbc2Correction (JSphGpu_mdbc_iker.h:44),
+ * SlipMode SLIP_NoSlip with MDBC2_KEEPVEL (DualSphDef.h:471): for boundary
+ * particles [0,n) with a normal, density at the ghost node (4x4 first-order
+ * correction when well conditioned, else Shepard), pressure cloned to the
+ * particle, velrho[p].w written; d_boundmode[p] = DSPH_BMODE_MDBC2 or
+ * DSPH_BMODE_MDBC2OFF (dry ghost node) and d_tangenvel[p] = tangential part
+ * of 2 v_motion - v_ghost. The caller clears d_boundmode[0,n) to
+ * DSPH_BMODE_DBC first (JSphGpuSingle.cpp:700-701). d_motionvel and
+ * d_motionace (float3[n]) may be NULL for fixed boundaries. */
+int dsph_mdbc2_correction(dsph_ctx* ctx, uint32_t n, const float gravity[3],
+                          const dsph_divdata* dvd, const double mapposmin[3],
+                          const double* d_posxy, const double* d_posz,
+                          const float* d_poscell, const uint16_t* d_code,
+                          const float* d_boundnor, const float* d_motionvel,
+                          const float* d_motionace, float* d_velrho,
+                          uint8_t* d_boundmode, float* d_tangenvel);
+
 /* --------------------------------------------------- reductions and dt */
 
 /* Replaces cusph::ComputeVelMod + ReduMaxFloat (JSphGpu_ker.h:281,290):
@@ -371,11 +395,15 @@ int dsph_dt_advance(dsph_ctx* ctx, double* d_dt);
 
 /* ------------------------------------------------- integration (cusphs) */
 
+/* d_boundmode (mDBC2, else NULL): boundary particles with mode >=
+ * DSPH_BMODE_MDBC2 keep velrhopre (Verlet: velrho2) instead of the DBC
+ * density update (JSphGpuSimple_ker.cu:142,276,406-412). */
 /* Replaces cusphs::ComputeStepSymplecticPre (JSphGpuSimple_ker.h:51) for
  * DBC/mDBC without floatings/shifting/inout. dtm = dt/2. When d_dt is not
  * NULL, dtm is read on the device as 0.5*d_dt[dt_slot] instead. */
 int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb,
-                             const float* d_velrhopre, const float* d_ar,
+                             const float* d_velrhopre, const uint8_t* d_boundmode,
+                             const float* d_ar,
                              const float* d_ace, double dtm,
                              const double* d_dt, uint32_t dt_slot,
                              float rhopzero, float rhopoutmin, float rhopoutmax,
@@ -384,7 +412,8 @@ int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb,
 /* Replaces cusphs::ComputeStepSymplecticCor (JSphGpuSimple_ker.h:57).
  * dtm = dt/2; device dt as above (dt = d_dt[dt_slot]). */
 int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb,
-                             const float* d_velrhopre, const float* d_ar,
+                             const float* d_velrhopre, const uint8_t* d_boundmode,
+                             const float* d_ar,
                              const float* d_ace, double dtm, double dt,
                              const double* d_dt, uint32_t dt_slot,
                              float rhopzero, float rhopoutmin, float rhopoutmax,
@@ -393,6 +422,7 @@ int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb,
 /* Replaces cusphs::ComputeStepVerlet (JSphGpuSimple_ker.h:46). */
 int dsph_step_verlet(dsph_ctx* ctx, uint32_t np, uint32_t npb,
                      const float* d_velrho1, const float* d_velrho2,
+                     const uint8_t* d_boundmode,
                      const float* d_ar, const float* d_ace, double dt, double dt2,
                      float rhopzero, float rhopoutmin, float rhopoutmax,
                      const float gravity[3], uint16_t* d_code,
@@ -487,6 +517,8 @@ typedef struct dsph_case {
   double maprealposmin[3], maprealposmax[3];
   uint32_t casenp, casenbound, casenfixed, casenmoving, casenfloat, casenfluid;
   dsph_motion motion;            /* the moving object (ids casenfixed.. +casenmoving) */
+  int32_t slipmode;              /* mDBC SlipMode (JSph.cpp:706-716): DSPH_SLIP_VEL0 or
+                                    DSPH_SLIP_NOSLIP (fixed boundaries only) */
 } dsph_case;
 
 /* Per-step statistics, mirror of the Run.out/Run.csv key numbers. */

@@ -251,6 +251,8 @@ def write_case(case, directory, name: str, timemax: float = 1.0, timeout: float 
         "RhopOutMin": p.rhopoutmin, "RhopOutMax": p.rhopoutmax,
         "Boundary": p.tboundary,
     }
+    if p.tboundary == 2:
+        params["SlipMode"] = getattr(p, "slipmode", 1)
     moving, motion = "", ""
     if nmov:
         m = case.motion

@@ -213,6 +213,10 @@ inline R wend_wab(R rr2, R h, R awen) {
 struct Opts {
   int sem, tvisco, tddt, velgrad;
   float viscob, viscof;
+  // mDBC2 (SlipMode no-slip): mode of each boundary particle (0 DBC, 1 mDBC2,
+  // 2 mDBC2 switched off) and its tangential velocity; NULL without mDBC2.
+  const unsigned char* bmode = nullptr;
+  const float* tangenvel = nullptr;
 };
 
 // Relative position p1-p2 in the chosen semantics.
@@ -280,8 +284,9 @@ template <typename R>
 void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, int nr,
                 unsigned p1, const float* ps, const double* pos, const float* velrho,
                 const R* press, const float* tau, FAcc<R>& a) {
-  const R massp2 = R(boundp2 ? c.massb : c.massf);
+  const R massp2_kind = R(boundp2 ? c.massb : c.massf);
   const R visco = R(boundp2 ? o.viscob : o.viscof);
+  const bool mdbc2 = boundp2 && o.bmode;
   const float* v1 = velrho + 4 * size_t(p1);
   const R v1x = R(v1[0]), v1y = R(v1[1]), v1z = R(v1[2]), rho1 = R(v1[3]);
   const R press1 = press[p1];
@@ -298,12 +303,15 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
       const R frx = fac * dx, fry = fac * dy, frz = fac * dz;
       const float* v2 = velrho + 4 * size_t(p2);
       const R rho2 = R(v2[3]);
+      // mDBC2: a boundary particle whose ghost node is dry has no mass
+      // (JSphCpu.cpp:851-854; JSphGpu_ker.cu:585-588).
+      const R massp2 = (mdbc2 && o.bmode[p2] == 2) ? R(0) : massp2_kind;
       {
         const R prs = (press1 + press[p2]) / (rho1 * rho2);
         const R p_vpm = -prs * massp2;
         a.ax += p_vpm * frx; a.ay += p_vpm * fry; a.az += p_vpm * frz;
       }
-      const R dvx = v1x - R(v2[0]), dvy = v1y - R(v2[1]), dvz = v1z - R(v2[2]);
+      R dvx = v1x - R(v2[0]), dvy = v1y - R(v2[1]), dvz = v1z - R(v2[2]);
       a.ar += massp2 * (dvx * frx + dvy * fry + dvz * frz) * (rho1 / rho2);
       const R dot3 = dx * frx + dy * fry + dz * frz;
       if (o.tddt == 1 && !a.delta_off) {
@@ -318,6 +326,12 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
         const R delta = visc_densi * dot3 * massp2 / rho2;
         if (boundp2) a.delta_off = true;
         else a.delta -= delta;
+      }
+      if (mdbc2) {
+        // Viscous terms against the no-slip tangential velocity
+        // (JSphCpu.cpp:988-995; JSphGpu_ker.cu:725-730).
+        const float* tv = o.tangenvel + 3 * size_t(p2);
+        dvx = v1x - R(tv[0]); dvy = v1y - R(tv[1]); dvz = v1z - R(tv[2]);
       }
       const R dot = dx * dvx + dy * dvy + dz * dvz;
       const R dot_rr2 = dot / (rr2 + eta2);
@@ -461,6 +475,11 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
 
 extern "C" {
 
+void orc_interaction_mdbc2(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                           unsigned, unsigned, unsigned, const unsigned*, const float*,
+                           const double*, const float*, const float*, float*, float*, float*,
+                           float*, float*, const unsigned char*, const float*);
+
 // Force interaction (see header). prec: 0 float, 1 double. sem: SEM_*.
 // ace/ar/viscdt (and strain) accumulate as in the reference.
 void orc_interaction(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
@@ -468,7 +487,20 @@ void orc_interaction(int prec, int sem, int tvisco, int tddt, float viscob, floa
                      const unsigned* dcell, const float* poscell, const double* pos,
                      const float* velrho, const float* tau, float* ace, float* ar,
                      float* viscdt, float* strain, float* velgrad) {
-  Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof};
+  orc_interaction_mdbc2(prec, sem, tvisco, tddt, viscob, viscof, c, d, np, npb, npbok, dcell,
+                        poscell, pos, velrho, tau, ace, ar, viscdt, strain, velgrad, nullptr,
+                        nullptr);
+}
+
+// Force interaction with mDBC2 boundary modes and tangential velocities
+// (boundmode/tangenvel of StInterParmsc; NULL = no mDBC2).
+void orc_interaction_mdbc2(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
+                           const orc_cte* c, const orc_div* d, unsigned np, unsigned npb,
+                           unsigned npbok, const unsigned* dcell, const float* poscell,
+                           const double* pos, const float* velrho, const float* tau, float* ace,
+                           float* ar, float* viscdt, float* strain, float* velgrad,
+                           const unsigned char* bmode, const float* tangenvel) {
+  Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof, bmode, tangenvel};
   if (prec == 0)
     interaction<float>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
                        viscdt, strain, velgrad);
@@ -591,6 +623,164 @@ void mdbc_correction(int sem, const orc_cte& c, const orc_div& d, unsigned n, co
       rhofinal = float(rhop1 / R(a[0][0]));
     }
     velrho[4 * size_t(p1) + 3] = (rhofinal != FLT_MAX ? rhofinal : c.rhopzero);
+  }
+}
+
+
+// mDBC2 (SlipMode no-slip, MDBC2_Std, velocities kept: MDBC2_KEEPVEL,
+// DualSphDef.h:471) of boundary particles [0,n). At the ghost node
+// g = x_b + boundnor: Shepard sums, the 4x4 first-order correction matrix
+// (columns 1..3 with -dr), a submerged test, the corrected density when the
+// matrix is invertible and its scaled infinity-norm condition number
+// dp^2 |A| |A^-1| <= 50 (else Shepard; Shepard clamped at rho0 when the
+// kernel sum is < 0.1), pressure cloned to the particle with the hydrostatic
+// gradient, and the tangential velocity of 2 v_motion - v_ghost.
+// CPU build: JSphCpu_mdbc.cpp:246-455 (dr from FP64 positions, matrix and
+// velocity sums in FP64, FP64 Cs0 and Dp); GPU build:
+// KerInteractionMdbc2Correction_Fast, JSphGpu_mdbc_iker.cu:289-487 (dr from
+// PosCell, FP32 sums, FP64 determinant and inverse rounded to FP32, FP32
+// Cs0). R = scalar sums, M = matrix and velocity sums. Writes velrho.w,
+// bmode (1 = mDBC2, 2 = dry ghost: mass off) and tangenvel.
+template <typename R, typename M>
+void mdbc2_correction(int sem, const orc_cte& c, const orc_div& d, unsigned n, const double* pos,
+                      const float* ps, const typecode* code, const float* boundnor,
+                      const float* gravity, const float* motionvel, const float* motionace,
+                      double cs0d, double dpd, float* velrho, unsigned char* bmode,
+                      float* tangenvel) {
+  const R ks2 = R(c.kernelsize2), h = R(c.kernelh), awen = R(c.awen), bwenh = R(c.bwenh);
+  const R massf = R(c.massf);
+  const bool f64 = (sem == SEM_F64);
+  // cs0^2 as each build forms it.
+  const double cs2 = sem == SEM_GPU ? double(c.cs0 * c.cs0) : cs0d * cs0d;
+  const float rho0 = c.rhopzero;
+  auto tangential = [](const float* bn, const float v[3], float* out) {
+    const float sn = std::sqrt(bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]);
+    const float nx = bn[0] / sn, ny = bn[1] / sn, nz = bn[2] / sn;
+    const float vdn = v[0] * nx + v[1] * ny + v[2] * nz;
+    out[0] = v[0] - vdn * nx; out[1] = v[1] - vdn * ny; out[2] = v[2] - vdn * nz;
+  };
+#pragma omp parallel for schedule(guided)
+  for (long pp = 0; pp < long(n); pp++) {
+    const unsigned p1 = unsigned(pp);
+    const float* bn = boundnor + 3 * size_t(p1);
+    if (bn[0] == 0 && bn[1] == 0 && bn[2] == 0) continue;
+    const float zero3[3] = {0, 0, 0};
+    const float* mv = motionvel ? motionvel + 3 * size_t(p1) : zero3;
+    const float* ma = motionace ? motionace + 3 * size_t(p1) : zero3;
+    const double g[3] = {pos[3 * size_t(p1)] + bn[0], pos[3 * size_t(p1) + 1] + bn[1],
+                         pos[3 * size_t(p1) + 2] + bn[2]};
+    float gps[4];
+    {
+      const double dx = g[0] - c.mapposmin[0], dy = g[1] - c.mapposmin[1], dz = g[2] - c.mapposmin[2];
+      const double pcs = c.poscellsize;
+      const unsigned cx = unsigned(dx / pcs), cy = unsigned(dy / pcs), cz = unsigned(dz / pcs);
+      gps[0] = float(dx - pcs * cx); gps[1] = float(dy - pcs * cy); gps[2] = float(dz - pcs * cz);
+      gps[3] = bitsf((cx << 19) | (cy << 9) | cz);
+    }
+    const int cx = int((g[0] - c.domposmin[0]) / c.scell) - d.czx;
+    const int cy = int((g[1] - c.domposmin[1]) / c.scell) - d.czy;
+    const int cz = int((g[2] - c.domposmin[2]) / c.scell) - d.czz;
+    Row rows[25];
+    const int nr = rows_at(d, cx, cy, cz, true, rows);
+    R rhop1 = 0, gr[3] = {0, 0, 0}, sumwab = 0, submerged = 0;
+    M velp[3] = {0, 0, 0};
+    M a[4][4];
+    for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) a[i][j] = 0;
+    const unsigned ug = fbits(gps[3]);
+    const int gx = int(ug >> 19), gy = int((ug & 0x0007fe00u) >> 9), gz = int(ug & 0x1ffu);
+    for (int r = 0; r < nr; r++)
+      for (unsigned p2 = rows[r].a; p2 < rows[r].b; p2++) {
+        R dr[3];
+        if (sem == SEM_GPU) {
+          const float* b = ps + 4 * size_t(p2);
+          const unsigned ub = fbits(b[3]);
+          dr[0] = R(gps[0] - b[0] + c.poscellsize * float(gx - int(ub >> 19)));
+          dr[1] = R(gps[1] - b[1] + c.poscellsize * float(gy - int((ub & 0x0007fe00u) >> 9)));
+          dr[2] = R(gps[2] - b[2] + c.poscellsize * float(gz - int(ub & 0x1ffu)));
+        } else if (sem == SEM_CPU) {
+          for (int k = 0; k < 3; k++) dr[k] = R(float(g[k] - pos[3 * size_t(p2) + k]));
+        } else {
+          for (int k = 0; k < 3; k++) dr[k] = R(g[k] - pos[3 * size_t(p2) + k]);
+        }
+        const R rr2 = dr[0] * dr[0] + dr[1] * dr[1] + dr[2] * dr[2];
+        if (!(rr2 <= ks2 && (code[p2] & kTypeMask) == kTypeFluid)) continue;
+        const R wab = wend_wab<R>(rr2, h, awen);
+        const R fac = wend_fac<R>(rr2, h, bwenh);
+        const R fr[3] = {fac * dr[0], fac * dr[1], fac * dr[2]};
+        const float* v2 = velrho + 4 * size_t(p2);
+        const R volp2 = massf / R(v2[3]);
+        submerged -= volp2 * (dr[0] * fr[0] + dr[1] * fr[1] + dr[2] * fr[2]);
+        rhop1 += massf * wab;
+        for (int k = 0; k < 3; k++) gr[k] += massf * fr[k];
+        const R vwab = wab * volp2;
+        sumwab += vwab;
+        for (int k = 0; k < 3; k++) velp[k] += M(vwab * R(v2[k]));
+        const R v[4] = {vwab, fr[0] * volp2, fr[1] * volp2, fr[2] * volp2};
+        for (int i = 0; i < 4; i++) {
+          a[i][0] += M(v[i]);
+          for (int k = 0; k < 3; k++) a[i][k + 1] -= M(dr[k] * v[i]);
+        }
+      }
+    float* vr1 = velrho + 4 * size_t(p1);
+    float* tv = tangenvel + 3 * size_t(p1);
+    if (!(submerged > R(0))) {
+      bmode[p1] = 2;
+      vr1[3] = rho0;
+      tangential(bn, mv, tv);
+      continue;
+    }
+    bmode[p1] = 1;
+    float rhofinal;
+    if (sumwab < R(0.1f)) {
+      rhofinal = std::max(rho0, float(M(rhop1) / a[0][0]));
+    } else {
+      double ad[4][4], inv[4][4];
+      for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) ad[i][j] = double(a[i][j]);
+      const double det = inverse4(ad, inv);
+      if (std::fabs(det) >= 0.001) {
+        M im[4][4];
+        for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) im[i][j] = M(inv[i][j]);
+        float na = 0, ni = 0;
+        for (int i = 0; i < 4; i++) {
+          const float ra = float(std::fabs(a[i][0]) + std::fabs(a[i][1]) + std::fabs(a[i][2]) + std::fabs(a[i][3]));
+          const float ri = float(std::fabs(im[i][0]) + std::fabs(im[i][1]) + std::fabs(im[i][2]) + std::fabs(im[i][3]));
+          na = std::max(na, ra);
+          ni = std::max(ni, ri);
+        }
+        const float dp2 = sem == SEM_GPU ? c.dp * c.dp : float(dpd * dpd);
+        const float condinf = dp2 * na * ni;
+        if (condinf <= 50)
+          rhofinal = float(im[0][0] * M(rhop1) + im[0][1] * M(gr[0]) + im[0][2] * M(gr[1]) +
+                           im[0][3] * M(gr[2]));
+        else
+          rhofinal = float(M(rhop1) / a[0][0]);
+      } else {
+        rhofinal = float(M(rhop1) / a[0][0]);
+      }
+    }
+    // Pressure cloning (Mdbc2PressClone, JSphCpu_mdbc.cpp:185-216 /
+    // KerMdbc2PressClone, JSphGpu_mdbc_iker.cu:210-236).
+    {
+      const float sn = std::sqrt(bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]);
+      const float nx = bn[0] / sn, ny = bn[1] / sn, nz = bn[2] / sn;
+      const float normpos = (-bn[0]) * nx + (-bn[1]) * ny + (-bn[2]) * nz;
+      const float fx = gravity[0] - ma[0], fy = gravity[1] - ma[1], fz = gravity[2] - ma[2];
+      const float normforce = rho0 * (fx * nx + fy * ny + fz * nz);
+      if (f64) {
+        const double pghost = cs2 * (double(rhofinal) - double(rho0));
+        const double pf = pghost + double(normforce) * double(normpos);
+        rhofinal = float(double(rho0) + pf / cs2);
+      } else {
+        const float pghost = float(cs2 * double(rhofinal - rho0));
+        const float pf = pghost + normforce * normpos;
+        rhofinal = rho0 + float(double(pf) / cs2);
+      }
+    }
+    vr1[3] = rhofinal;
+    float vg[3], v2[3];
+    for (int k = 0; k < 3; k++) vg[k] = float(velp[k] / a[0][0]);
+    for (int k = 0; k < 3; k++) v2[k] = mv[k] + mv[k] - vg[k];
+    tangential(bn, v2, tv);
   }
 }
 
@@ -862,6 +1052,33 @@ void orc_mdbc(int prec, int sem, const orc_cte* c, const orc_div* d, unsigned n,
   if (prec == 1) mdbc_correction<double, double>(SEM_F64, *c, *d, n, pos, poscell, code, boundnor2, velrho);
   else if (sem == SEM_CPU) mdbc_correction<float, double>(sem, *c, *d, n, pos, poscell, code, boundnor2, velrho);
   else mdbc_correction<float, float>(sem, *c, *d, n, pos, poscell, code, boundnor2, velrho);
+}
+// mDBC2 correction of boundary particles [0,n) (see mdbc2_correction).
+// prec: 0 FP32, 1 FP64; sem: SEM_GPU or SEM_CPU. motionvel/motionace may be
+// NULL (fixed boundaries). cs0 and dp are the FP64 case constants.
+void orc_mdbc2(int prec, int sem, const orc_cte* c, const orc_div* d, unsigned n,
+               const double* pos, const float* poscell, const typecode* code,
+               const float* boundnor, const float* gravity, const float* motionvel,
+               const float* motionace, double cs0, double dp, float* velrho,
+               unsigned char* bmode, float* tangenvel) {
+  if (prec == 1)
+    mdbc2_correction<double, double>(SEM_F64, *c, *d, n, pos, poscell, code, boundnor, gravity,
+                                     motionvel, motionace, cs0, dp, velrho, bmode, tangenvel);
+  else if (sem == SEM_CPU)
+    mdbc2_correction<float, double>(sem, *c, *d, n, pos, poscell, code, boundnor, gravity,
+                                    motionvel, motionace, cs0, dp, velrho, bmode, tangenvel);
+  else
+    mdbc2_correction<float, float>(sem, *c, *d, n, pos, poscell, code, boundnor, gravity,
+                                   motionvel, motionace, cs0, dp, velrho, bmode, tangenvel);
+}
+// Integrator rule of mDBC2 boundaries (mode >= 1 keeps the state it starts
+// from: KerComputeStepSymplecticPre/Cor JSphGpuSimple_ker.cu:276,406-412,
+// KerComputeStepVerlet :142; JSphCpu::ComputeVelrhoBound JSphCpu.cpp:1607-1616):
+// velrho[p] = src[p] for p < npb with bmode[p] >= 1, after the DBC update.
+void orc_mdbc2_keep(unsigned npb, const unsigned char* bmode, const float* src, float* velrho) {
+  for (unsigned p = 0; p < npb; p++)
+    if (bmode[p] >= 1)
+      for (int k = 0; k < 4; k++) velrho[4 * size_t(p) + k] = src[4 * size_t(p) + k];
 }
 double orc_eos(int prec, int sem, const orc_cte* c, float rho) {
   return prec == 0 ? double(eos<float>(sem, *c, rho)) : eos<double>(sem, *c, rho);

@@ -56,6 +56,14 @@ void orc_step_pos(unsigned, unsigned, const orc_cte*, const double*, const doubl
                   unsigned*, typecode*);
 void orc_mdbc(int, int, const orc_cte*, const orc_div*, unsigned, const double*, const float*,
               const typecode*, const float*, float*);
+void orc_interaction_mdbc2(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                           unsigned, unsigned, unsigned, const unsigned*, const float*,
+                           const double*, const float*, const float*, float*, float*, float*,
+                           float*, float*, const unsigned char*, const float*);
+void orc_mdbc2(int, int, const orc_cte*, const orc_div*, unsigned, const double*, const float*,
+               const typecode*, const float*, const float*, const float*, const float*, double,
+               double, float*, unsigned char*, float*);
+void orc_mdbc2_keep(unsigned, const unsigned char*, const float*, float*);
 void orc_pips(const orc_cte*, const orc_div*, unsigned, unsigned, unsigned, const unsigned*,
               const float*, unsigned long long*);
 }
@@ -80,7 +88,9 @@ struct Sim {
   std::vector<double> pos, pospre, mov;
   std::vector<float> velrho, velrhopre, velrhom1, poscell, ace, ar, viscdt, strain, tau;
   std::vector<float> boundnor;  // doubled normals (ConfigBoundNormals), mDBC only
-  int TBoundary = 1;
+  int TBoundary = 1, SlipMode = 1;
+  std::vector<unsigned char> bmode;  // mDBC2 boundary modes of [0,Npb)
+  std::vector<float> tangenvel;      // mDBC2 tangential velocities of [0,Npb)
   bool pre_active = false;
   // cell division
   std::vector<unsigned> cellpart, sortpart;
@@ -112,6 +122,9 @@ struct Sim {
     c.ddtkh = ks * float(s.ddtvalue);
     c.ddtgz = float(double(c.rhopzero) * double(std::fabs(g[2])) / double(c.cteb));
     TStep = s.tstep; TVisco = s.tvisco; TDDT = s.tdensity; TBoundary = s.tboundary;
+    SlipMode = s.slipmode ? s.slipmode : 1;
+    if (TBoundary == 2 && SlipMode == 2 && s.casenmoving)
+      throw std::runtime_error("mDBC no-slip is built for fixed boundaries only");
     VerletSteps = s.verletsteps ? s.verletsteps : 40;
     Visco = float(s.visco); ViscoB = Visco * float(s.viscoboundfactor);
     RhopOutMin = float(s.rhopoutmin); RhopOutMax = float(s.rhopoutmax);
@@ -270,11 +283,13 @@ struct Sim {
     VelMax = double(std::sqrt(v2));
     const orc_div d = divdata();
     // prec 0: FP32 GPU semantics, 1: FP64 yardstick, 2: FP32 CPU semantics.
-    orc_interaction(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT, ViscoB,
-                    Visco, &c, &d, Np, Npb, NpbOk,
-                    dcell.data(), poscell.data(), pos.data(), velrho.data(),
-                    TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(), viscdt.data(),
-                    TVisco == 2 ? strain.data() : nullptr, nullptr);
+    const bool m2 = mdbc2_on();
+    orc_interaction_mdbc2(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT,
+                          ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
+                          dcell.data(), poscell.data(), pos.data(), velrho.data(),
+                          TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(),
+                          viscdt.data(), TVisco == 2 ? strain.data() : nullptr, nullptr,
+                          m2 ? bmode.data() : nullptr, m2 ? tangenvel.data() : nullptr);
     if (TVisco == 2) orc_sps_tau(Np, Npb, SpsSmag, SpsBlin, velrho.data(), strain.data(), tau.data());
     float vd = 0, a2 = 0;
     for (unsigned p = 0; p < Np; p++) vd = std::max(vd, viscdt[p]);
@@ -332,12 +347,23 @@ struct Sim {
     BoundChanged = true;
   }
 
-  // JSphCpuSingle::MdbcBoundCorrection: mDBC (SLIP_Vel0) before the
-  // predictor (Symplectic) or the Verlet interaction; not before the
-  // corrector (MdbcCorrector=false, the default).
-  void mdbc() {
+  // JSphCpuSingle::MdbcBoundCorrection (JSphCpuSingle.cpp:754-775): mDBC
+  // (SLIP_Vel0) before the predictor (Symplectic) or the Verlet interaction,
+  // not before the corrector (MdbcCorrector=false); no-slip (mDBC2) before
+  // every interaction (MdbcCorrector=true), modes cleared to DBC first.
+  bool mdbc2_on() const { return TBoundary == 2 && SlipMode == 2; }
+  void mdbc(bool corrector = false) {
     if (TBoundary != 2) return;
     const orc_div d = divdata();
+    if (mdbc2_on()) {
+      bmode.assign(Npb, 0);
+      tangenvel.assign(3 * size_t(Npb), 0.f);
+      orc_mdbc2(prec == 1 ? 1 : 0, prec == 2 ? 1 : 0, &c, &d, Npb, pos.data(), poscell.data(),
+                code.data(), boundnor.data(), g, nullptr, nullptr, Cs0, cs.dp, velrho.data(),
+                bmode.data(), tangenvel.data());
+      return;
+    }
+    if (corrector) return;
     orc_mdbc(prec == 1 ? 1 : 0, prec == 2 ? 1 : 0, &c, &d, NpbOk, pos.data(), poscell.data(),
              code.data(), boundnor.data(), velrho.data());
   }
@@ -351,13 +377,16 @@ struct Sim {
     mov.assign(3 * size_t(Np), 0.0);
     orc_sym_pre(Np, Npb, velrhopre.data(), ar.data(), ace.data(), dt * .5, c.rhopzero, RhopOutMin,
                 RhopOutMax, g, code.data(), mov.data(), velrho.data());
+    if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrhopre.data(), velrho.data());
     orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
     cell_divide();
+    mdbc(true);
     interaction();
     const double dt_c = dt_variable();
     mov.assign(3 * size_t(Np), 0.0);
     orc_sym_cor(Np, Npb, velrhopre.data(), ar.data(), ace.data(), dt * .5, dt, c.rhopzero,
                 RhopOutMin, RhopOutMax, g, code.data(), mov.data(), velrho.data());
+    if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrhopre.data(), velrho.data());
     orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
     pre_active = false;
     run_motion(dt);
@@ -373,12 +402,14 @@ struct Sim {
     VerletStep++;
     std::vector<float> nw(velrho.size());
     mov.assign(3 * size_t(Np), 0.0);
-    if (VerletStep < unsigned(VerletSteps))
+    if (VerletStep < unsigned(VerletSteps)) {
       orc_verlet(Np, Npb, velrho.data(), velrhom1.data(), ar.data(), ace.data(), dt, dt + dt,
                  c.rhopzero, RhopOutMin, RhopOutMax, g, code.data(), mov.data(), nw.data());
-    else {
+      if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrhom1.data(), nw.data());
+    } else {
       orc_verlet(Np, Npb, velrho.data(), velrho.data(), ar.data(), ace.data(), dt, dt, c.rhopzero,
                  RhopOutMin, RhopOutMax, g, code.data(), mov.data(), nw.data());
+      if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrho.data(), nw.data());
       VerletStep = 0;
     }
     std::vector<double> p0 = pos;

@@ -59,6 +59,12 @@ def load() -> C.CDLL:
                                     None),
             "orc_sim_create": ([vp, i, u, vp, vp, vp, vp], vp),
             "orc_mdbc": ([i, i, C.POINTER(OrcCte), C.POINTER(OrcDiv), u, vp, vp, vp, vp, vp], None),
+            "orc_mdbc2": ([i, i, C.POINTER(OrcCte), C.POINTER(OrcDiv), u, vp, vp, vp, vp, vp, vp,
+                           vp, d, d, vp, vp, vp], None),
+            "orc_mdbc2_keep": ([u, vp, vp, vp], None),
+            "orc_interaction_mdbc2": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
+                                       u, u, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
+                                      None),
             "orc_sim_destroy": ([vp], None),
             "orc_sim_error": ([vp], C.c_char_p),
             "orc_sim_run": ([vp, u], i),

@@ -19,7 +19,7 @@ ref_observables_dp0.01.npz (--observables, ~1 h on 8 cores): front position
 and probe pressure every 0.01 s over 1 s of the dp=0.01 dam break.
 
 Run in the development container (the reference is not available on the
-GPU hosts):  python tests/golden/make_reference_fixtures.py [--observables]
+GPU hosts):  python tests/golden/make_reference_fixtures.py [--observables] [names]
 """
 from __future__ import annotations
 
@@ -48,6 +48,9 @@ CASES = {
     "ref_flume_dp0.01_piston": ("flume", 0.01, 10, (1, 5, 10), {}),
     "ref_flume_dp0.01_piston_mdbc_sps": ("flume", 0.01, 10, (1, 5, 10),
                                          dict(tboundary=2, tvisco=2, visco=1e-6)),
+    # mDBC2 (SlipMode no-slip) on the dam break's fixed walls.
+    "ref_dambreak_dp0.03_mdbc2": ("dam_break", 0.03, 10, (1, 5, 10),
+                                  dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4)),
 }
 
 
@@ -97,7 +100,10 @@ def main():
     if "--observables" in sys.argv:
         observables_fixture()
         return
+    only = [a for a in sys.argv[1:] if not a.startswith("--")]
     for name, (gen, dp, nsteps, saved, over) in CASES.items():
+        if only and name not in only:
+            continue
         params = cm.Parameters(**over)
         case = getattr(cm, gen)(dp=dp, params=params)
         with tempfile.TemporaryDirectory() as td:

@@ -197,3 +197,63 @@ def compare_with_reference(fx, k, time, pos0, pos, vel, rho, order=None):
     for key, t in tol.items():
         assert e[key] < t, (int(fx["steps"][k]), key, e)
     return e
+
+
+def hydrostatic_mdbc2_case(dp=0.02, depth=0.2, u=0.3, jitter=0.0):
+    """Fluid layer of depth ``depth`` filling a 0.6 m x 0.6 m tank (mDBC
+    no-slip), density linear in z with the hydrostatic gradient of the
+    linearised EOS, rho = rho0 + rho0 g (depth - z) / cs0^2, and uniform
+    velocity (u, 0, 0). Closed forms of the mDBC2 correction on it: a floor
+    particle far from the side walls gets the same linear density at its own
+    position (first-order correction reproduces linear fields; pressure
+    cloning adds the hydrostatic gradient back) and tangential velocity
+    (-u, 0, 0) (no-slip mirror 2*0 - u)."""
+    from dualsphysics_amd import case as cm
+    from dualsphysics_amd import _native as N
+    p = cm.Parameters(tboundary=N.BC_MDBC, slipmode=N.SLIP_NOSLIP, tvisco=N.VISCO_LAMINAR,
+                      visco=1e-4)
+    c = cm.dam_break(dp=dp, tank=(0.6, 0.6, 0.3), column=(0.6, 0.6, depth), jitter=jitter,
+                     params=p)
+    nb = c.nbound
+    z = c.pos[nb:, 2]
+    c.velrho[nb:, 3] = (c.rhop0 + c.rhop0 * 9.81 * (depth - z) / c.cs0 ** 2).astype(np.float32)
+    c.velrho[nb:, 0] = u
+    return c
+
+
+def mdbc2_state(c):
+    """Sorted state of case c from the oracle's first cell division: dict with
+    the oracle handle, arrays, constants, divdata pieces and the doubled
+    normals of the boundary particles in sorted order."""
+    from oracle.oracle import OracleSim, OrcDiv
+    o = OracleSim(c, prec=1)
+    d = o.download()
+    dcell, poscell, cte = o.arrays()
+    c6, be = o.celldiv()
+    st = o.stats()
+    n, npb = st["np"], st["npb"]
+    ncx, ncy, ncz = c6[3] - c6[0] + 1, c6[4] - c6[1] + 1, c6[5] - c6[2] + 1
+    odiv = OrcDiv(1, ncx, ncy, ncz, ncx * ncy, ncx * ncy * ncz + 1, c6[0], c6[1], c6[2],
+                  be.ctypes.data)
+    bn2 = np.zeros((n, 3), np.float32)
+    bn2[:npb] = 2.0 * c.boundnor[d["idp"][:npb]]
+    return {"o": o, "d": d, "dcell": dcell, "poscell": poscell, "cte": cte, "celldiv": c6,
+            "be": be, "odiv": odiv, "n": n, "npb": npb, "npbok": st["npbok"], "bn2": bn2,
+            "pos": np.ascontiguousarray(d["pos"])}
+
+
+def oracle_mdbc2(c, s, prec, sem, velrho=None):
+    """orc_mdbc2 on state s: (velrho after the correction, boundmode, tangenvel)."""
+    import ctypes as C
+    import oracle.oracle as orc
+    vr = (s["d"]["velrho"] if velrho is None else velrho).copy()
+    npb = s["npb"]
+    bm = np.zeros(npb, np.uint8)
+    tv = np.zeros((npb, 3), np.float32)
+    g = np.array(c.gravity, np.float32)
+    orc.load().orc_mdbc2(prec, sem, C.byref(s["cte"]), C.byref(s["odiv"]), npb,
+                         s["pos"].ctypes.data, s["poscell"].ctypes.data,
+                         s["d"]["code"].ctypes.data, s["bn2"].ctypes.data, g.ctypes.data, None,
+                         None, float(c.cs0), float(c.dp), vr.ctypes.data, bm.ctypes.data,
+                         tv.ctypes.data)
+    return vr, bm, tv

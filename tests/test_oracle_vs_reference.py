@@ -9,7 +9,8 @@ FP32 rounding of the reference over the first 10 steps (bounds in
 tests/helpers.REF_TOL). Cases: dam break with Symplectic + artificial
 viscosity + DDT2Full (the benchmark configuration) and Verlet + laminar
 viscosity + DDT2; wave flume with a piston wavemaker (moving boundary,
-rectilinear sinusoid) with DBC, and with mDBC + laminar+SPS viscosity.
+rectilinear sinusoid) with DBC, and with mDBC + laminar+SPS viscosity; dam
+break with mDBC2 (SlipMode no-slip) + laminar viscosity.
 """
 import numpy as np
 import pytest
@@ -18,7 +19,8 @@ from oracle.oracle import OracleSim
 from tests.helpers import compare_with_reference, load_reference_fixture
 
 FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
-            "ref_flume_dp0.01_piston", "ref_flume_dp0.01_piston_mdbc_sps"]
+            "ref_flume_dp0.01_piston", "ref_flume_dp0.01_piston_mdbc_sps",
+            "ref_dambreak_dp0.03_mdbc2"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
