@@ -60,7 +60,9 @@ class DsphInterParms(C.Structure):
 class DsphMotion(C.Structure):
     _fields_ = [("type", C.c_int32), ("idbegin", C.c_uint32), ("count", C.c_uint32),
                 ("freq", C.c_double * 3), ("ampl", C.c_double * 3), ("phase", C.c_double * 3),
-                ("start", C.c_double), ("finish", C.c_double)]
+                ("start", C.c_double), ("finish", C.c_double), ("axisp1", C.c_double * 3),
+                ("axisp2", C.c_double * 3), ("angfreq", C.c_double), ("angampl", C.c_double),
+                ("angphase", C.c_double), ("angvel", C.c_double)]
 
 
 class DsphCase(C.Structure):
@@ -148,7 +150,9 @@ SIGNATURES = {
     "dsph_step_pos2": [_vp, _u, _u] + [_vp] * 8,
     "dsph_step_pos": [_vp, _u, _u] + [_vp] * 6,
     "dsph_move_lin_bound": [_vp, _u, _u, _u, C.POINTER(_d), C.POINTER(_f)] + [_vp] * 6,
-    "dsph_move_motion_device": [_vp, _u, C.POINTER(DsphMotion), _d] + [_vp] * 7,
+    "dsph_move_motion_device": [_vp, _u, C.POINTER(DsphMotion), _d] + [_vp] * 8,
+    "dsph_move_mat_bound": [_vp, _u, _u, _u, C.POINTER(_d), _d] + [_vp] * 7,
+    "dsph_motion_matrix": [C.POINTER(DsphMotion), _d, _d, C.POINTER(_d)],
     "dsph_pips_count_sync": [_vp, _u, _u, _u, _u, C.POINTER(DsphDivData), _vp, _vp,
                              C.POINTER(C.c_uint64)],
     "dsph_pips_count_split_sync": [_vp, _u, _u, _u, _u, C.POINTER(DsphDivData), _vp, _vp,

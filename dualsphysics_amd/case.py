@@ -121,12 +121,23 @@ class Case:
         c.casenmoving, c.casenfloat, c.casenfluid = self.nmoving, 0, self.nfluid
         if self.nmoving and self.motion:
             m = self.motion
-            c.motion.type = 1
+            kind = m.get("type", "rectsinu")
+            c.motion.type = {"rectsinu": 1, "rotsinu": 2, "rot": 3}[kind]
             c.motion.idbegin, c.motion.count = self.nbound - self.nmoving, self.nmoving
-            for k in range(3):
-                c.motion.freq[k] = m["freq"][k]
-                c.motion.ampl[k] = m["ampl"][k]
-                c.motion.phase[k] = m["phase"][k]
+            if kind == "rectsinu":
+                for k in range(3):
+                    c.motion.freq[k] = m["freq"][k]
+                    c.motion.ampl[k] = m["ampl"][k]
+                    c.motion.phase[k] = m["phase"][k]
+            else:
+                for k in range(3):
+                    c.motion.axisp1[k] = m["axisp1"][k]
+                    c.motion.axisp2[k] = m["axisp2"][k]
+                if kind == "rotsinu":
+                    c.motion.angfreq, c.motion.angampl = m["angfreq"], m["angampl"]
+                    c.motion.angphase = m["angphase"]
+                else:
+                    c.motion.angvel = m["angvel"]
             c.motion.start, c.motion.finish = m.get("start", 0.0), m.get("finish", 1e30)
         return c
 
@@ -225,7 +236,8 @@ def flume(dp: float = 0.01, length: float = 0.6, width: float = 0.1, height: flo
           depth: float = 0.15, ampl: float = 0.02, period: float = 0.8, phase: float = 0.0,
           layers: int = 2, coefh: float = 1.2, coefsound: float = 20.0, rhop0: float = 1000.0,
           gamma: float = 7.0, g: float = 9.81, seed: int = 42, jitter: float = 1e-4,
-          params: Parameters | None = None, name: str | None = None) -> Case:
+          params: Parameters | None = None, name: str | None = None,
+          wavemaker: str = "piston", flap_deg: float = 4.0, angvel: float = 10.0) -> Case:
     """Wave flume with a piston wavemaker (the flume configuration scaled by
     its arguments): fluid ``[0,length]x[0,width]x[0,depth]``; fixed walls
     (bottom, two sides, far end, ``layers`` lattice layers, open top) extended
@@ -233,7 +245,11 @@ def flume(dp: float = 0.01, length: float = 0.6, width: float = 0.1, height: flo
     the tank height driven by x(t) = ampl sin(2 pi t/period + phase).
     Ids: fixed walls, then the piston, then the fluid. ``boundnor`` (mDBC
     normals) is the vector from each wall particle to the nearest point of
-    the fluid region (the piston face for piston particles)."""
+    the fluid region (the piston face for piston particles).
+    ``wavemaker="flap"``: the same wall instead rotates about the hinge line
+    x=0, z=0 (axis along +y) by flap_deg*sin(2 pi t/period + phase) degrees
+    (rotation sinusoid); ``"flap_rot"``: at a constant angular velocity of
+    ``angvel`` degrees/s."""
     params = params or Parameters()
     nx, ny, nz = _n(length, dp), _n(width, dp), _n(height, dp)
     nback = int(math.ceil(ampl / dp)) + layers      # stroke behind x=0 plus the piston
@@ -268,6 +284,15 @@ def flume(dp: float = 0.01, length: float = 0.6, width: float = 0.1, height: flo
     mn, mx = map_limits(pos, h)
     motion = {"freq": (1.0 / period, 0.0, 0.0), "ampl": (ampl, 0.0, 0.0),
               "phase": (phase, 0.0, 0.0), "start": 0.0, "finish": 1e30}
+    if wavemaker == "flap":
+        motion = {"type": "rotsinu", "axisp1": (0.0, 0.0, 0.0), "axisp2": (0.0, 1.0, 0.0),
+                  "angfreq": 1.0 / period, "angampl": flap_deg, "angphase": phase,
+                  "start": 0.0, "finish": 1e30}
+    elif wavemaker == "flap_rot":
+        motion = {"type": "rot", "axisp1": (0.0, 0.0, 0.0), "axisp2": (0.0, 1.0, 0.0),
+                  "angvel": angvel, "start": 0.0, "finish": 1e30}
+    elif wavemaker != "piston":
+        raise ValueError(f"unknown wavemaker {wavemaker!r}")
     return Case(name=name or f"flume_dp{dp:g}", dp=dp, h=h, rhop0=rhop0, gamma=gamma, cs0=cs0,
                 b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g), hswl=depth,
                 params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb, nfluid=nf,

@@ -596,25 +596,34 @@ struct dsph_sim {
     pre_active = false;
   }
 
-  // ---- JSphGpu::RunMotion (JSphGpu.cpp:1173-1200) for one rectilinear
-  // sinusoid: after the step, the moving boundary is displaced over
-  // [t, t+dt] and takes the velocity displacement/dt; the next divide is
-  // full (BoundChanged).
+  // ---- JSphGpu::RunMotion (JSphGpu.cpp:1173-1229) for one moving object:
+  // after the step it is moved over [t, t+dt], by a displacement
+  // (rectilinear sinusoid, MoveLinBound) or a matrix (rotations,
+  // MoveMatBound, which also turns the mDBC normals), and takes the velocity
+  // displacement/dt; the next divide is full (BoundChanged).
   void run_motion(double dt) {
     const dsph_motion& m = cs.motion;
     if (!cs.casenmoving || m.type == 0) return;
     if (device_dt) {
       chk(dsph_move_motion_device(ctx, Npb, &m, TimeStep, d_dt, idp, posxy, posz, dcell, velrho,
-                                  code),
+                                  code, boundnor),
           "motion");
     } else {
       if (!(TimeStep >= m.start && TimeStep < m.finish)) return;
-      double mv[3];
-      dsph::rect_sinu_mov_host(&m, TimeStep, dt, mv);
-      const float vel[3] = {float(mv[0] / dt), float(mv[1] / dt), float(mv[2] / dt)};
-      chk(dsph_move_lin_bound(ctx, Npb, m.idbegin, m.count, mv, vel, idp, posxy, posz, dcell,
-                              velrho, code),
-          "motion");
+      if (m.type == 1) {
+        double mv[3];
+        dsph::rect_sinu_mov_host(&m, TimeStep, dt, mv);
+        const float vel[3] = {float(mv[0] / dt), float(mv[1] / dt), float(mv[2] / dt)};
+        chk(dsph_move_lin_bound(ctx, Npb, m.idbegin, m.count, mv, vel, idp, posxy, posz, dcell,
+                                velrho, code),
+            "motion");
+      } else {
+        double mat[12];
+        chk(dsph_motion_matrix(&m, TimeStep, dt, mat), "motion matrix");
+        chk(dsph_move_mat_bound(ctx, Npb, m.idbegin, m.count, mat, dt, idp, posxy, posz, dcell,
+                                velrho, code, boundnor),
+            "motion");
+      }
     }
     BoundChanged = true;
   }

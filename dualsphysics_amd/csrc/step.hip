@@ -331,6 +331,85 @@ __global__ void __launch_bounds__(kBs) k_move_lin_bound(unsigned npb, unsigned i
   velrho[p] = make_float4(vel.x, vel.y, vel.z, vr.w);
 }
 
+// Rotation by ang degrees about the axis p1->p2 as JMatrix4::MatrixRot
+// (JMatrix4.h:465-497) composes it (translate, align the axis with z,
+// rotate, undo): a rotation by -ang about the unit direction u = (p2-p1)/|p2-p1|
+// in the right-hand sense, here in Rodrigues' closed form
+// R = cos(t) I + sin(t) [u]x + (1 - cos(t)) u u^T with t = -ang; x' = R (x - p1) + p1.
+// m: row-major 3x4.
+__host__ __device__ inline void rot_matrix(double ang, const double p1[3], const double p2[3],
+                                           double m[12]) {
+  const double torad = 3.14159265358979323846 / 180.0;
+  double u[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+  const double l = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+  for (int k = 0; k < 3; k++) u[k] /= l;
+  const double t = -ang * torad;
+  const double c = cos(t), sn = sin(t), oc = 1.0 - c;
+  const double r[3][3] = {
+      {c + oc * u[0] * u[0], oc * u[0] * u[1] - sn * u[2], oc * u[0] * u[2] + sn * u[1]},
+      {oc * u[1] * u[0] + sn * u[2], c + oc * u[1] * u[1], oc * u[1] * u[2] - sn * u[0]},
+      {oc * u[2] * u[0] - sn * u[1], oc * u[2] * u[1] + sn * u[0], c + oc * u[2] * u[2]}};
+  for (int i = 0; i < 3; i++) {
+    m[4 * i] = r[i][0]; m[4 * i + 1] = r[i][1]; m[4 * i + 2] = r[i][2];
+    m[4 * i + 3] = p1[i] - (r[i][0] * p1[0] + r[i][1] * p1[1] + r[i][2] * p1[2]);
+  }
+}
+
+// Matrix of a rotation motion over [t, t+dt] (JMotionObj.cpp:518-524, 577-587).
+__host__ __device__ inline void motion_matrix(const dsph_motion& m, double t, double dt,
+                                              double mat[12]) {
+  const double twopi = 6.283185307179586476925286766559;
+  double ang = 0;
+  if (m.type == 2)
+    ang = m.angampl * sin(m.angphase + twopi * m.angfreq * (t + dt)) -
+          m.angampl * sin(m.angphase + twopi * m.angfreq * t);
+  else if (m.type == 3)
+    ang = m.angvel * dt;
+  rot_matrix(ang, m.axisp1, m.axisp2, mat);
+}
+
+struct Mat34 {
+  double a[12];
+};
+
+// KerMoveMatBound (JSphGpu_ker.cu:1705-1740) for the object's particles
+// (selected by id). With a device dt slot array the matrix of the step is
+// computed here from the motion law.
+__global__ void __launch_bounds__(kBs) k_move_mat_bound(unsigned npb, unsigned idbegin,
+    unsigned count, PosCte c, Mat34 mh, double dth, dsph_motion mo, double t0,
+    const double* __restrict__ d_dt, const unsigned* __restrict__ idp,
+    double2* __restrict__ posxy, double* __restrict__ posz, unsigned* __restrict__ dcell,
+    float4* __restrict__ velrho, typecode* __restrict__ code, float* __restrict__ boundnor) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npb) return;
+  if (idp[p] - idbegin >= count) return;
+  Mat34 m = mh;
+  double dt = dth;
+  if (d_dt) {
+    dt = d_dt[2];
+    const double t = t0 + d_dt[3];
+    if (!(t >= mo.start && t < mo.finish) || dt <= 0) return;
+    motion_matrix(mo, t, dt, m.a);
+  }
+  const double2 rxy = posxy[p];
+  const double rx = rxy.x, ry = rxy.y, rz = posz[p];
+  const double x2 = rx * m.a[0] + ry * m.a[1] + rz * m.a[2] + m.a[3];
+  const double y2 = rx * m.a[4] + ry * m.a[5] + rz * m.a[6] + m.a[7];
+  const double z2 = rx * m.a[8] + ry * m.a[9] + rz * m.a[10] + m.a[11];
+  const double dx = x2 - rx, dy = y2 - ry, dz = z2 - rz;
+  update_pos(rxy, rz, dx, dy, dz, false, p, c, posxy, posz, dcell, code);
+  const float4 vr = velrho[p];
+  velrho[p] = make_float4(float(dx / dt), float(dy / dt), float(dz / dt), vr.w);
+  if (boundnor) {
+    float* bn = boundnor + 3 * size_t(p);
+    const double gx = rx + bn[0], gy = ry + bn[1], gz = rz + bn[2];
+    const double gx2 = gx * m.a[0] + gy * m.a[1] + gz * m.a[2] + m.a[3];
+    const double gy2 = gx * m.a[4] + gy * m.a[5] + gz * m.a[6] + m.a[7];
+    const double gz2 = gx * m.a[8] + gy * m.a[9] + gz * m.a[10] + m.a[11];
+    bn[0] = float(gx2 - x2); bn[1] = float(gy2 - y2); bn[2] = float(gz2 - z2);
+  }
+}
+
 __global__ void __launch_bounds__(kBs) k_step_pos2(unsigned n, unsigned pini, PosCte c,
     const double2* __restrict__ posxypre, const double* __restrict__ poszpre,
     const double2* __restrict__ movxy, const double* __restrict__ movz,
@@ -724,16 +803,52 @@ int dsph_move_lin_bound(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin, uint32_t 
   return DSPH_OK;
 }
 
+int dsph_move_mat_bound(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin, uint32_t count,
+                        const double mat[12], double dt, const uint32_t* d_idp, double* d_posxy,
+                        double* d_posz, uint32_t* d_dcell, float* d_velrho, uint16_t* d_code,
+                        float* d_boundnor) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  if (!npb || !count) return DSPH_OK;
+  DSPH_REQUIRE(ctx, mat && d_idp && d_posxy && d_posz && d_dcell && d_velrho && d_code,
+               "dsph_move_mat_bound: null argument");
+  DSPH_REQUIRE(ctx, dt > 0, "dsph_move_mat_bound: dt must be > 0");
+  Mat34 m;
+  for (int k = 0; k < 12; k++) m.a[k] = mat[k];
+  dsph_motion mo = {};
+  KTimer kt(ctx, "k_move_mat_bound");
+  k_move_mat_bound<<<grid_for(npb, kBs), kBs, 0, ctx->stream>>>(
+      npb, idbegin, count, make_posc(ctx->cte), m, dt, mo, 0.0, nullptr, d_idp,
+      (double2*)d_posxy, d_posz, d_dcell, (float4*)d_velrho, d_code, d_boundnor);
+  DSPH_LAUNCHED(ctx, "k_move_mat_bound");
+  return DSPH_OK;
+}
+
+int dsph_motion_matrix(const dsph_motion* m, double t, double dt, double mat[12]) {
+  if (!m || !mat || (m->type != 2 && m->type != 3)) return DSPH_E_ARG;
+  motion_matrix(*m, t, dt, mat);
+  return DSPH_OK;
+}
+
 int dsph_move_motion_device(dsph_ctx* ctx, uint32_t npb, const dsph_motion* m, double t0,
                             const double* d_dt, const uint32_t* d_idp, double* d_posxy,
                             double* d_posz, uint32_t* d_dcell, float* d_velrho,
-                            uint16_t* d_code) {
+                            uint16_t* d_code, float* d_boundnor) {
   if (!ctx || !m) return DSPH_E_ARG;
   if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
   if (!npb || !m->count || m->type == 0) return DSPH_OK;
-  DSPH_REQUIRE(ctx, m->type == 1, "dsph_move_motion_device: only rectilinear sinusoids (type 1)");
+  DSPH_REQUIRE(ctx, m->type >= 1 && m->type <= 3, "dsph_move_motion_device: invalid motion type");
   DSPH_REQUIRE(ctx, d_dt && d_idp && d_posxy && d_posz && d_dcell && d_velrho && d_code,
                "dsph_move_motion_device: null argument");
+  if (m->type != 1) {
+    Mat34 mz = {};
+    KTimer kt(ctx, "k_move_mat_bound");
+    k_move_mat_bound<<<grid_for(npb, kBs), kBs, 0, ctx->stream>>>(
+        npb, m->idbegin, m->count, make_posc(ctx->cte), mz, 0.0, *m, t0, d_dt, d_idp,
+        (double2*)d_posxy, d_posz, d_dcell, (float4*)d_velrho, d_code, d_boundnor);
+    DSPH_LAUNCHED(ctx, "k_move_mat_bound");
+    return DSPH_OK;
+  }
   KTimer kt(ctx, "k_move_lin_bound");
   k_move_lin_bound<<<grid_for(npb, kBs), kBs, 0, ctx->stream>>>(
       npb, m->idbegin, m->count, make_posc(ctx->cte), make_double3(0, 0, 0),

@@ -440,17 +440,29 @@ int dsph_step_pos(dsph_ctx* ctx, uint32_t np, uint32_t npb,
                   double* d_posxy, double* d_posz, uint32_t* d_dcell,
                   uint16_t* d_code);
 
-/* Imposed motion of one moving boundary object: rectilinear sinusoidal
- * motion (JMotionMovRectSinu, JMotionMov.h:237-254; displacement law
- * JMotionObj.cpp:558-575): over [t, t+dt] each axis with a non-zero
- * amplitude moves by ampl*(sin(phase+2*pi*freq*(t+dt)) - sin(phase+2*pi*freq*t)),
- * velocity = displacement/dt (JMotionList.cpp:43-50); active for
- * start <= t < finish. */
+/* Imposed motion of one moving boundary object, active for
+ * start <= t < finish; over a step [t, t+dt]:
+ *  type 1, rectilinear sinusoid (JMotionMovRectSinu, JMotionMov.h:237-254;
+ *    law JMotionObj.cpp:558-575): each axis with a non-zero amplitude moves
+ *    by ampl*(sin(phase+2*pi*freq*(t+dt)) - sin(phase+2*pi*freq*t)), velocity
+ *    = displacement/dt (JMotionList.cpp:43-50) -> MoveLinBound;
+ *  type 2, rotation sinusoid (JMotionMovRotSinu, law JMotionObj.cpp:577-587):
+ *    rotation about the axis axisp1->axisp2 by
+ *    angampl*(sin(angphase+2*pi*angfreq*(t+dt)) - sin(angphase+2*pi*angfreq*t))
+ *    degrees -> MoveMatBound;
+ *  type 3, rotation at constant angular velocity (JMotionMovRot, law
+ *    JMotionObj.cpp:518-524): by angvel*dt degrees -> MoveMatBound.
+ * Rotations follow JMatrix4::MatrixRot (JMatrix4.h:465-497): a positive
+ * angle turns clockwise seen from axisp2 towards axisp1's side, i.e. by
+ * -angle about the direction axisp2-axisp1 (right-hand rule). */
 typedef struct dsph_motion {
-  int32_t type;                  /* 0 none, 1 rectilinear sinusoid */
+  int32_t type;                  /* 0 none, 1 rect. sinusoid, 2 rot. sinusoid, 3 rotation */
   uint32_t idbegin, count;       /* particle ids of the object */
-  double freq[3], ampl[3], phase[3];  /* 1/s, m, rad */
+  double freq[3], ampl[3], phase[3];  /* type 1: 1/s, m, rad */
   double start, finish;          /* s */
+  double axisp1[3], axisp2[3];   /* types 2, 3: rotation axis points (m) */
+  double angfreq, angampl, angphase;  /* type 2: 1/s, degrees, rad */
+  double angvel;                 /* type 3: degrees/s */
 } dsph_motion;
 
 /* Replaces cusph::MoveLinBound (JSphGpu_ker.h:326): displaces the moving
@@ -463,15 +475,30 @@ int dsph_move_lin_bound(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin,
                         const float mvvel[3], const uint32_t* d_idp,
                         double* d_posxy, double* d_posz, uint32_t* d_dcell,
                         float* d_velrho, uint16_t* d_code);
-/* The same with the displacement of motion m computed on the device from
- * the device time slots (dt = d_dt[2], t = t0 + d_dt[3], the layout of
- * dsph_dt_variable): no host round trip per step. No counterpart in the
- * reference (its motion is computed on the host every step). */
+/* Replaces cusph::MoveMatBound (JSphGpu_ker.h:328; KerMoveMatBound
+ * JSphGpu_ker.cu:1705-1740): the moving boundary particles with id in
+ * [idbegin, idbegin+count) go to x' = M x (FP64, M row-major 3x4: rotation
+ * and translation), displaced with the KerUpdatePos rules, velocity
+ * (x'-x)/dt; their mDBC normals (d_boundnor, float3, may be NULL) become
+ * M(x+n) - M(x). */
+int dsph_move_mat_bound(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin,
+                        uint32_t count, const double mat[12], double dt,
+                        const uint32_t* d_idp, double* d_posxy, double* d_posz,
+                        uint32_t* d_dcell, float* d_velrho, uint16_t* d_code,
+                        float* d_boundnor);
+/* The same with the displacement (type 1) or matrix (types 2, 3) of motion
+ * m computed on the device from the device time slots (dt = d_dt[2],
+ * t = t0 + d_dt[3], the layout of dsph_dt_variable): no host round trip per
+ * step. No counterpart in the reference (its motion is computed on the host
+ * every step). d_boundnor as in dsph_move_mat_bound. */
 int dsph_move_motion_device(dsph_ctx* ctx, uint32_t npb, const dsph_motion* m,
                             double t0, const double* d_dt,
                             const uint32_t* d_idp, double* d_posxy,
                             double* d_posz, uint32_t* d_dcell, float* d_velrho,
-                            uint16_t* d_code);
+                            uint16_t* d_code, float* d_boundnor);
+/* The 3x4 FP64 matrix of a rotation motion (types 2, 3) over [t, t+dt]
+ * (host computation, the reference's JMotionObj::ProcesTime path). */
+int dsph_motion_matrix(const dsph_motion* m, double t, double dt, double mat[12]);
 
 /* ------------------------------------------------------------ measurement */
 

@@ -259,14 +259,35 @@ def write_case(case, directory, name: str, timemax: float = 1.0, timeout: float 
         moving = (f'            <moving mkbound="1" mk="12" begin="{nfix}" count="{nmov}" '
                   f'refmotion="0" />\n')
         v3 = lambda tag, x: f'<{tag} x="{r(x[0])}" y="{r(x[1])}" z="{r(x[2])}" />'
-        motion = f"""        <motion>
-            <objreal ref="0">
-                <begin mov="1" start="{r(m.get('start', 0.0))}" />
-                <mvrectsinu id="1" duration="{r(min(m.get('finish', 1e30), 1e6))}" anglesunits="radians">
+        kind = m.get("type", "rectsinu")
+        dur = r(min(m.get('finish', 1e30), 1e6))
+        if kind == "rectsinu":
+            mov = f"""<mvrectsinu id="1" duration="{dur}" anglesunits="radians">
                     {v3('freq', m['freq'])}
                     {v3('ampl', m['ampl'])}
                     {v3('phase', m['phase'])}
-                </mvrectsinu>
+                </mvrectsinu>"""
+        elif kind == "rotsinu":
+            # Degrees: amplitude as given; the phase (radians here) is
+            # written in degrees and converted back by the reader.
+            import math
+            mov = f"""<mvrotsinu id="1" duration="{dur}" anglesunits="degrees">
+                    {v3('axisp1', m['axisp1'])}
+                    {v3('axisp2', m['axisp2'])}
+                    <freq v="{r(m['angfreq'])}" />
+                    <ampl v="{r(m['angampl'])}" />
+                    <phase v="{r(math.degrees(m['angphase']))}" />
+                </mvrotsinu>"""
+        else:
+            mov = f"""<mvrot id="1" duration="{dur}" anglesunits="degrees">
+                    <vel ang="{r(m['angvel'])}" />
+                    {v3('axisp1', m['axisp1'])}
+                    {v3('axisp2', m['axisp2'])}
+                </mvrot>"""
+        motion = f"""        <motion>
+            <objreal ref="0">
+                <begin mov="1" start="{r(m.get('start', 0.0))}" />
+                {mov}
             </objreal>
         </motion>
 """

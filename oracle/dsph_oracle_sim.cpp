@@ -310,39 +310,114 @@ struct Sim {
     return dt;
   }
 
-  // JSphCpu::RunMotion + MoveLinBound for one rectilinear sinusoid
-  // (JMotionObj.cpp:558-575, JMotionList.cpp:43-50, JSphCpu.cpp MoveLinBound):
-  // displacement over [t, t+dt], velocity displacement/dt, applied after the
-  // step; the next divide is full.
+  // JSphCpu::RunMotion for one moving object, applied after the step; the
+  // next divide is full. Rectilinear sinusoid (JMotionObj.cpp:558-575,
+  // JMotionList.cpp:43-50): MoveLinBound with displacement over [t, t+dt] and
+  // velocity displacement/dt. Rotations (sinusoidal JMotionObj.cpp:577-587,
+  // constant angular velocity :518-524): MoveMatBound (JSphCpu.cpp
+  // MoveMatBound; GPU KerMoveMatBound JSphGpu_ker.cu:1705-1740) with the
+  // matrix JMatrix4::MatrixRot builds (JMatrix4.h:465-497): translate the
+  // axis point to the origin, turn the axis onto z (about x, then about y),
+  // rotate about z, undo; x' = M x, velocity (x'-x)/dt, mDBC normals
+  // M(x+n) - M(x).
   unsigned mkcell_host(unsigned cx, unsigned cy, unsigned cz) const {
     const unsigned dcc = c.cellcode;
     return (cx << ((dcc >> 10) & 31u)) | (cy << ((dcc >> 15) & 31u)) | cz;
   }
+  static void mat_mul(const double a[16], const double b[16], double out[16]) {
+    double r[16];
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) {
+        double v = 0;
+        for (int k = 0; k < 4; k++) v += a[4 * i + k] * b[4 * k + j];
+        r[4 * i + j] = v;
+      }
+    std::memcpy(out, r, sizeof(r));
+  }
+  static void mat_rot(double ang, const double p1[3], const double p2[3], double t[16]) {
+    const double rad = ang * (3.14159265358979323846 / 180.0);
+    const double v[3] = {p2[0] - p1[0], p2[1] - p1[1], p2[2] - p1[2]};
+    const double L = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    const double L1 = std::sqrt(v[1] * v[1] + v[2] * v[2]);
+    auto ident = [](double m[16]) { for (int k = 0; k < 16; k++) m[k] = (k % 5 == 0) ? 1 : 0; };
+    double tr[16], tm1[16], rx[16], rxm1[16], ry[16], rym1[16], rz[16];
+    ident(tr); ident(tm1); ident(rx); ident(rxm1); ident(ry); ident(rym1); ident(rz);
+    for (int k = 0; k < 3; k++) { tr[4 * k + 3] = p1[k]; tm1[4 * k + 3] = -p1[k]; }
+    if (L1 == 0) {
+      rx[5] = 0; rx[6] = 1; rx[9] = -1; rx[10] = 0;
+      rxm1[5] = 0; rxm1[6] = -1; rxm1[9] = 1; rxm1[10] = 0;
+    } else {
+      rx[5] = v[2] / L1; rx[6] = v[1] / L1; rx[9] = -v[1] / L1; rx[10] = v[2] / L1;
+      rxm1[5] = v[2] / L1; rxm1[6] = -v[1] / L1; rxm1[9] = v[1] / L1; rxm1[10] = v[2] / L1;
+    }
+    ry[0] = L1 / L; ry[2] = v[0] / L; ry[8] = -v[0] / L; ry[10] = L1 / L;
+    rym1[0] = L1 / L; rym1[2] = -v[0] / L; rym1[8] = v[0] / L; rym1[10] = L1 / L;
+    const double cs = std::cos(rad), sn = std::sin(rad);
+    rz[0] = cs; rz[1] = sn; rz[4] = -sn; rz[5] = cs;
+    std::memcpy(t, tr, sizeof(tr));
+    mat_mul(t, rx, t); mat_mul(t, ry, t); mat_mul(t, rz, t);
+    mat_mul(t, rym1, t); mat_mul(t, rxm1, t); mat_mul(t, tm1, t);
+  }
+  void move_particle(unsigned p, const double d[3]) {
+    double* x = &pos[3 * size_t(p)];
+    for (int k = 0; k < 3; k++) x[k] += d[k];
+    double dd[3];
+    bool out = false;
+    for (int k = 0; k < 3; k++) {
+      dd[k] = x[k] - c.maprealposmin[k];
+      if (dd[k] != dd[k] || dd[k] < 0 || dd[k] >= c.maprealsize[k]) out = true;
+    }
+    if (out) throw std::runtime_error("moving boundary particle left the domain");
+    dcell[p] = mkcell_host(unsigned(dd[0] / c.scell), unsigned(dd[1] / c.scell),
+                           unsigned(dd[2] / c.scell));
+  }
   void run_motion(double dt) {
     const dsph_motion& m = cs.motion;
-    if (!cs.casenmoving || m.type != 1) return;
+    if (!cs.casenmoving || m.type == 0) return;
     if (!(TimeStep >= m.start && TimeStep < m.finish)) return;
     const double twopi = 6.283185307179586476925286766559;
-    double mv[3];
-    for (int k = 0; k < 3; k++)
-      mv[k] = m.ampl[k] != 0 ? m.ampl[k] * std::sin(m.phase[k] + twopi * m.freq[k] * (TimeStep + dt)) -
-                                   m.ampl[k] * std::sin(m.phase[k] + twopi * m.freq[k] * TimeStep)
-                             : 0.0;
-    for (unsigned p = 0; p < Npb; p++) {
-      if (idp[p] - m.idbegin >= m.count) continue;
-      double* x = &pos[3 * size_t(p)];
-      for (int k = 0; k < 3; k++) x[k] += mv[k];
-      double dd[3];
-      bool out = false;
-      for (int k = 0; k < 3; k++) {
-        dd[k] = x[k] - c.maprealposmin[k];
-        if (dd[k] != dd[k] || dd[k] < 0 || dd[k] >= c.maprealsize[k]) out = true;
+    if (m.type == 1) {
+      double mv[3];
+      for (int k = 0; k < 3; k++)
+        mv[k] = m.ampl[k] != 0 ? m.ampl[k] * std::sin(m.phase[k] + twopi * m.freq[k] * (TimeStep + dt)) -
+                                     m.ampl[k] * std::sin(m.phase[k] + twopi * m.freq[k] * TimeStep)
+                               : 0.0;
+      for (unsigned p = 0; p < Npb; p++) {
+        if (idp[p] - m.idbegin >= m.count) continue;
+        move_particle(p, mv);
+        float* v = &velrho[4 * size_t(p)];
+        for (int k = 0; k < 3; k++) v[k] = float(mv[k] / dt);
       }
-      if (out) throw std::runtime_error("moving boundary particle left the domain");
-      dcell[p] = mkcell_host(unsigned(dd[0] / c.scell), unsigned(dd[1] / c.scell),
-                             unsigned(dd[2] / c.scell));
-      float* v = &velrho[4 * size_t(p)];
-      for (int k = 0; k < 3; k++) v[k] = float(mv[k] / dt);
+    } else {
+      double ang = 0;
+      if (m.type == 2)
+        ang = m.angampl * std::sin(m.angphase + twopi * m.angfreq * (TimeStep + dt)) -
+              m.angampl * std::sin(m.angphase + twopi * m.angfreq * TimeStep);
+      else
+        ang = m.angvel * dt;
+      double t[16];
+      mat_rot(ang, m.axisp1, m.axisp2, t);
+      auto apply = [&](const double x[3], double y[3]) {
+        for (int i = 0; i < 3; i++)
+          y[i] = x[0] * t[4 * i] + x[1] * t[4 * i + 1] + x[2] * t[4 * i + 2] + t[4 * i + 3];
+      };
+      for (unsigned p = 0; p < Npb; p++) {
+        if (idp[p] - m.idbegin >= m.count) continue;
+        const double x[3] = {pos[3 * size_t(p)], pos[3 * size_t(p) + 1], pos[3 * size_t(p) + 2]};
+        double x2[3], d[3];
+        apply(x, x2);
+        for (int k = 0; k < 3; k++) d[k] = x2[k] - x[k];
+        move_particle(p, d);
+        float* v = &velrho[4 * size_t(p)];
+        for (int k = 0; k < 3; k++) v[k] = float(d[k] / dt);
+        if (!boundnor.empty()) {
+          float* bn = &boundnor[3 * size_t(p)];
+          const double gs[3] = {x[0] + bn[0], x[1] + bn[1], x[2] + bn[2]};
+          double gs2[3];
+          apply(gs, gs2);
+          for (int k = 0; k < 3; k++) bn[k] = float(gs2[k] - x2[k]);
+        }
+      }
     }
     BoundChanged = true;
   }
@@ -456,6 +531,10 @@ void* orc_sim_create(const dsph_case* cs, int prec, unsigned np, const unsigned*
   return s;
 }
 void orc_sim_destroy(void* h) { delete static_cast<Sim*>(h); }
+// The 4x4 matrix (row-major) of JMatrix4::MatrixRot(ang, p1, p2).
+void orc_matrix_rot(double ang, const double* p1, const double* p2, double* m16) {
+  Sim::mat_rot(ang, p1, p2, m16);
+}
 const char* orc_sim_error(void* h) { return static_cast<Sim*>(h)->err.c_str(); }
 int orc_sim_run(void* h, unsigned nsteps) {
   Sim* s = static_cast<Sim*>(h);

@@ -62,6 +62,7 @@ def load() -> C.CDLL:
             "orc_mdbc2": ([i, i, C.POINTER(OrcCte), C.POINTER(OrcDiv), u, vp, vp, vp, vp, vp, vp,
                            vp, d, d, vp, vp, vp], None),
             "orc_mdbc2_keep": ([u, vp, vp, vp], None),
+            "orc_matrix_rot": ([d, vp, vp, vp], None),
             "orc_interaction_mdbc2": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
                                        u, u, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],
                                       None),

@@ -48,10 +48,20 @@ CASES = {
     "ref_flume_dp0.01_piston": ("flume", 0.01, 10, (1, 5, 10), {}),
     "ref_flume_dp0.01_piston_mdbc_sps": ("flume", 0.01, 10, (1, 5, 10),
                                          dict(tboundary=2, tvisco=2, visco=1e-6)),
+    # Flap wavemakers (matrix motions, MoveMatBound): rotation sinusoid with
+    # DBC; constant angular velocity with mDBC (the normals turn with the flap).
+    "ref_flume_dp0.01_flap": ("flume", 0.01, 10, (1, 5, 10), dict(wavemaker="flap")),
+    "ref_flume_dp0.01_flap_rot_mdbc": ("flume", 0.01, 10, (1, 5, 10),
+                                       dict(wavemaker="flap_rot", tboundary=2, tvisco=2,
+                                            visco=1e-6)),
     # mDBC2 (SlipMode no-slip) on the dam break's fixed walls.
     "ref_dambreak_dp0.03_mdbc2": ("dam_break", 0.03, 10, (1, 5, 10),
                                   dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4)),
 }
+
+
+# Keys of a CASES entry that are arguments of the case generator, not Parameters.
+GENERATOR_ARGS = ("wavemaker",)
 
 
 def run_reference(case, nsteps: int, workdir: Path) -> list[dict]:
@@ -104,8 +114,10 @@ def main():
     for name, (gen, dp, nsteps, saved, over) in CASES.items():
         if only and name not in only:
             continue
+        over = dict(over)
+        genargs = {k: over.pop(k) for k in list(over) if k in GENERATOR_ARGS}
         params = cm.Parameters(**over)
-        case = getattr(cm, gen)(dp=dp, params=params)
+        case = getattr(cm, gen)(dp=dp, params=params, **genargs)
         with tempfile.TemporaryDirectory() as td:
             parts = run_reference(case, nsteps, Path(td))
         p0 = parts[0]
@@ -116,6 +128,7 @@ def main():
         np.savez_compressed(
             OUT / f"{name}.npz",
             generator=np.array(gen), dp=dp, seed=42, params=np.array(repr(sorted(over.items()))),
+            genargs=np.array(repr(sorted(genargs.items()))),
             nmoving=case.nmoving,
             nbound=case.nbound, nfluid=case.nfluid, steps=np.array(saved),
             time=np.array([parts[k]["time"] for k in saved]),

@@ -150,7 +150,9 @@ def load_reference_fixture(name: str):
     fx = dict(np.load(GOLDEN / f"{name}.npz", allow_pickle=False))
     over = dict(ast.literal_eval(str(fx["params"])))
     gen = str(fx["generator"]) if "generator" in fx else "dam_break"
-    case = getattr(cm, gen)(dp=float(fx["dp"]), seed=int(fx["seed"]), params=cm.Parameters(**over))
+    genargs = dict(ast.literal_eval(str(fx["genargs"]))) if "genargs" in fx else {}
+    case = getattr(cm, gen)(dp=float(fx["dp"]), seed=int(fx["seed"]), params=cm.Parameters(**over),
+                            **genargs)
     assert case.nbound == int(fx["nbound"]) and case.nfluid == int(fx["nfluid"])
     return fx, case
 

@@ -99,6 +99,31 @@ def test_device_dt_equals_host_dt():
     np.testing.assert_array_equal(ga["velrho"], gb["velrho"])
 
 
+@pytest.mark.parametrize("wavemaker,tboundary", [("piston", 1), ("flap", 1), ("flap_rot", 2)])
+def test_device_motion_equals_host_motion(wavemaker, tboundary):
+    """Moving boundaries with dt and the motion law on the device follow the
+    host path (the reference's): the same run up to the last bits of the
+    device's and the host's sin/cos."""
+    from dualsphysics_amd.sim import SphGpuSingle
+    prm = casemod.Parameters(tboundary=tboundary)
+    c = casemod.flume(dp=0.015, wavemaker=wavemaker, params=prm)
+    a, b = SphGpuSingle(c), SphGpuSingle(c)
+    a.run(12, sync_dt=True)
+    b.run(12, sync_dt=False)
+    sa, sb = a.stats(), b.stats()
+    assert abs(sa["timestep"] - sb["timestep"]) <= 1e-12 * sa["timestep"]
+    ga, gb = a.download(), b.download()
+    ia, ib = np.argsort(ga["idp"]), np.argsort(gb["idp"])
+    nb, nm = c.nbound, c.nmoving
+    mv = slice(nb - nm, nb)
+    dmov = np.abs(ga["pos"][ia][mv] - c.pos[mv]).max()
+    assert dmov > 1e-6, "the wavemaker did not move"
+    assert np.abs(ga["pos"][ia] - gb["pos"][ib]).max() < 1e-12
+    assert np.abs(ga["velrho"][ia] - gb["velrho"][ib]).max() < 1e-3
+    a.close()
+    b.close()
+
+
 def test_determinism():
     from dualsphysics_amd.sim import SphGpuSingle
     c = casemod.dam_break(dp=0.02)

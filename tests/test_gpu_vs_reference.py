@@ -14,7 +14,8 @@ pytestmark = pytest.mark.gpu
 @pytest.mark.parametrize("name", ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
                                   "ref_flume_dp0.01_piston",
                                   "ref_flume_dp0.01_piston_mdbc_sps",
-                                  "ref_dambreak_dp0.03_mdbc2"])
+                                  "ref_dambreak_dp0.03_mdbc2", "ref_flume_dp0.01_flap",
+                                  "ref_flume_dp0.01_flap_rot_mdbc"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)
