@@ -79,14 +79,14 @@ REF_FAST = ROOT / "oracle" / "_ref" / "DualSPHysics5.4CPU_fast"
 def cpu_baseline_reference(case, steps: int, pairs_step: float):
     """The reference CPU solver itself (built from its own sources by
     oracle/ref.mk with its release flags: -O3 -fopenmp -ffast-math) on the
-    same particles (written as its case XML + .bi4 by oracle/bi4.py), run for
+    same particles (written as its case XML + .bi4 by dualsphysics_amd/bi4.py), run for
     `steps` Symplectic steps on the host cores; time = its own "Simulation
     Runtime" (initialisation excluded). Pairs per step are those of the same
     initial state counted with the same rule on the GPU."""
     import re
     import subprocess
     import tempfile
-    from oracle import bi4
+    from dualsphysics_amd import bi4
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     t0 = time.perf_counter()
     with tempfile.TemporaryDirectory() as td:

@@ -96,7 +96,7 @@ class DsphStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("np", "npb", "npbok", "nct", "nstep",
                                           "npf_out_total")] + [
         (n, C.c_double) for n in ("timestep", "dt_last", "velmax", "acemax", "viscdtmax")] + [
-        ("ndiv", C.c_uint32), ("ndivfull", C.c_uint32)]
+        ("ndiv", C.c_uint32), ("ndivfull", C.c_uint32), ("symplecticdtpre", C.c_double)]
 
 
 # Every exported entry point of include/dsph_abi.h with its argument types.
@@ -163,6 +163,7 @@ SIGNATURES = {
     "dsph_ctx_timing_reset": [_vp],
     "dsph_sim_create": [_i, C.POINTER(DsphCase), _u, _vp, _vp, _vp, _vp, C.POINTER(_vp)],
     "dsph_sim_destroy": [_vp],
+    "dsph_sim_set_time": [_vp, _d, _d, _u],
     "dsph_sim_last_error": [_vp],
     "dsph_sim_run": [_vp, _u, _i],
     "dsph_sim_stats": [_vp, C.POINTER(DsphStats)],

@@ -36,6 +36,14 @@ int dsph_sim_create(int device, const dsph_case* cs, uint32_t np, const uint32_t
   return DSPH_OK;
 }
 
+int dsph_sim_set_time(dsph_sim* sim, double timestep, double symplecticdtpre, uint32_t nstep) {
+  if (!sim || !(timestep >= 0)) return DSPH_E_ARG;
+  sim->TimeStep = timestep;
+  if (symplecticdtpre > 0) sim->SymplecticDtPre = symplecticdtpre;
+  sim->Nstep = nstep;
+  return DSPH_OK;
+}
+
 int dsph_sim_destroy(dsph_sim* sim) {
   delete sim;
   return DSPH_OK;
@@ -61,6 +69,7 @@ int dsph_sim_stats(dsph_sim* sim, dsph_stats* o) {
   o->timestep = sim->TimeStep; o->dt_last = sim->DtLast;
   o->velmax = sim->VelMax; o->acemax = sim->AceMax; o->viscdtmax = sim->ViscDtMax;
   o->ndiv = sim->Ndiv; o->ndivfull = sim->NdivFull;
+  o->symplecticdtpre = sim->SymplecticDtPre;
   return DSPH_OK;
 }
 

@@ -318,9 +318,14 @@ struct dsph_sim {
     if (spstau) chk(dsph_memset_async(ctx, spstau, 0, 24 * n), "memset tau");
     if (boundnor) {
       // ConfigBoundNormals: normals are stored doubled (JSph.cpp:1385-1434).
+      // hbn is indexed by id (boundary ids 0..CaseNbound-1); the particles
+      // may come in any order with the boundary block first (a resumed PART
+      // keeps the solver's order).
       std::vector<float> bn(3 * n, 0.f);
       if (hbn)
-        for (size_t k = 0; k < 3 * size_t(CaseNbound); k++) bn[k] = hbn[k] * 2.f;
+        for (unsigned p = 0; p < np; p++)
+          if (hidp[p] < CaseNbound)
+            for (int k = 0; k < 3; k++) bn[3 * size_t(p) + k] = hbn[3 * size_t(hidp[p]) + k] * 2.f;
       chk(dsph_memcpy_h2d(ctx, boundnor, bn.data(), 12 * n), "upload boundnor");
     }
     std::vector<double> dt0(8, 0.0);
@@ -605,7 +610,7 @@ struct dsph_sim {
     const dsph_motion& m = cs.motion;
     if (!cs.casenmoving || m.type == 0) return;
     if (device_dt) {
-      chk(dsph_move_motion_device(ctx, Npb, &m, TimeStep, d_dt, idp, posxy, posz, dcell, velrho,
+      chk(dsph_move_motion_device(ctx, Npb, &m, 0.0, d_dt, idp, posxy, posz, dcell, velrho,
                                   code, boundnor),
           "motion");
     } else {
@@ -680,7 +685,12 @@ struct dsph_sim {
 
   void run(unsigned nsteps, bool sync_dt) {
     device_dt = (!sync_dt && TStep == 2);
-    if (device_dt) set_host_dt_slot(2, SymplecticDtPre);
+    if (device_dt) {
+      // The device carries the absolute time so that it is summed step by
+      // step as on the host (time += dt), whatever the batching of runs.
+      set_host_dt_slot(2, SymplecticDtPre);
+      set_host_dt_slot(3, TimeStep);
+    }
     for (unsigned s = 0; s < nsteps; s++) {
       if (TStep == 2) step_sym(); else step_ver();
       run_cell_divide();
@@ -693,10 +703,8 @@ struct dsph_sim {
       chk(dsph_memcpy_d2h(ctx, fl, d_flags, 16), "flags readback");
       if (fl[0] & 1u) throw Err(DSPH_E_NUMERIC, "dt is NaN or infinity");
       SymplecticDtPre = h[2];
-      TimeStep += h[3];
+      TimeStep = h[3];
       DtLast = h[4];
-      const double z[2] = {0.0, 0.0};
-      chk(dsph_memcpy_h2d(ctx, d_dt + 3, z, 16), "dt reset");
     }
     chk(dsph_synchronize(ctx), "synchronize");
   }

@@ -92,6 +92,46 @@ class SphGpuSingle:
             n += k
         return n
 
+    def save_part(self, directory, cpart: int, posdouble: bool = True):
+        """Saves the current state as <directory>/Part_<cpart>.bi4 in the
+        reference's format (particles in the solver's order, time, step,
+        SymplecticDtPre) and Part_Head.ibi4 when missing: a restart point the
+        reference's own -partbegin and :meth:`resume` both load."""
+        from pathlib import Path
+        from . import bi4
+        d = Path(directory)
+        d.mkdir(parents=True, exist_ok=True)
+        if not (d / "Part_Head.ibi4").exists():
+            bi4.write_part_head(d / "Part_Head.ibi4", self.case)
+        st = self.stats()
+        g = self.download()
+        bi4.write_part(d / f"Part_{cpart:04d}.bi4", self.case, g["idp"], g["pos"], g["velrho"],
+                       st["timestep"], st["nstep"], cpart,
+                       symplecticdtpre=st["symplecticdtpre"], posdouble=posdouble)
+        return d / f"Part_{cpart:04d}.bi4"
+
+    @classmethod
+    def resume(cls, case, part_file, device: int = 0) -> "SphGpuSingle":
+        """A simulation of ``case`` continued from a PART file (the reference's
+        -partbegin; JPartsLoad4.cpp:150-290, JSph.cpp:2214-2219): particles in
+        the saved order, simulated time, step count and SymplecticDtPre. The
+        continued run equals the uninterrupted one for states that live in
+        the PART (positions, velocity, density); Laminar+SPS stresses, the
+        Verlet previous state and turned mDBC normals restart as in the
+        reference (zero stresses, VelrhoM1 = Velrho, the case's normals)."""
+        import dataclasses
+        from . import bi4
+        pt = bi4.read_part(part_file, sort_by_id=False)
+        vr = np.concatenate([pt["vel"], pt["rho"][:, None]], 1).astype(np.float32)
+        nf = int(np.count_nonzero(pt["idp"] >= case.nbound))
+        c2 = dataclasses.replace(case, idp=pt["idp"].astype(np.uint32), pos=pt["pos"],
+                                 velrho=vr, nfluid=nf)
+        s = cls(c2, device)
+        dtpre = pt["symplecticdtpre"] if pt["symplecticdtpre"] is not None else 0.0
+        s._check(s.lib.dsph_sim_set_time(s._h, float(pt["time"]), float(dtpre),
+                                         int(pt["step"] or 0)))
+        return s
+
     def stats(self) -> dict:
         st = N.DsphStats()
         self._check(self.lib.dsph_sim_stats(self._h, C.byref(st)))

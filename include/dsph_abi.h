@@ -557,16 +557,23 @@ typedef struct dsph_stats {
   double dt_last;                /* dt of the last step */
   double velmax, acemax, viscdtmax;
   uint32_t ndiv, ndivfull;
+  double symplecticdtpre;        /* dt of the next Symplectic step (saved in PARTs) */
 } dsph_stats;
 
-/* Creates a simulation from particles sorted by idp, boundary first
- * (JPartsLoad4 order). h_pos is double3[np], h_velrho float4[np],
- * h_boundnor float3[casenbound] or NULL. */
+/* Creates a simulation from particles with the boundary block first: sorted
+ * by idp (JPartsLoad4 order of a new case) or in the order a PART saved them
+ * (a resumed run). h_pos is double3[np], h_velrho float4[np], h_boundnor
+ * float3[casenbound] indexed by id, or NULL. */
 int dsph_sim_create(int device, const dsph_case* cs, uint32_t np,
                     const uint32_t* h_idp, const double* h_pos,
                     const float* h_velrho, const float* h_boundnor,
                     dsph_sim** out);
 int dsph_sim_destroy(dsph_sim* sim);
+/* Time state of a run resumed from a PART file (JSph::InitRun, PartBegin
+ * branch, JSph.cpp:2214-2219; JPartsLoad4.cpp:222): simulated time, the dt
+ * of the next Symplectic step (<= 0 keeps DtIni) and the step count. */
+int dsph_sim_set_time(dsph_sim* sim, double timestep, double symplecticdtpre,
+                      uint32_t nstep);
 const char* dsph_sim_last_error(dsph_sim* sim);
 /* Runs nsteps full time steps (ComputeStep + RunCellDivide). With
  * sync_dt=0 the step keeps dt on the device (no host round trip). */

@@ -34,7 +34,7 @@ ROOT = Path(__file__).resolve().parents[2]
 sys.path.insert(0, str(ROOT))
 
 from dualsphysics_amd import case as cm  # noqa: E402
-from oracle import bi4  # noqa: E402
+from dualsphysics_amd import bi4  # noqa: E402
 
 REF_EXE = ROOT / "oracle" / "_ref" / "DualSPHysics5.4CPU_ieee"
 OUT = Path(__file__).resolve().parent
