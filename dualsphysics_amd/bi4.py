@@ -192,7 +192,8 @@ def read_file(path) -> Item:
 
 
 # ---------------------------------------------------------------- case files
-def write_case(case, directory, name: str, timemax: float = 1.0, timeout: float = 1.0):
+def write_case(case, directory, name: str, timemax: float | None = None,
+               timeout: float | None = None):
     """Writes <name>.xml and <name>.bi4 for the reference solver: the
     <execution> section GenCase would produce (constants, parameters,
     particle blocks mk 0 = fixed walls, mk 1 = fluid) and the particle file
@@ -200,6 +201,8 @@ def write_case(case, directory, name: str, timemax: float = 1.0, timeout: float 
     d = Path(directory)
     d.mkdir(parents=True, exist_ok=True)
     p = case.params
+    timemax = p.timemax if timemax is None else timemax
+    timeout = (p.timeout or timemax) if timeout is None else timeout
     n, nb, nf = case.np, case.nbound, case.nfluid
     pmin, pmax = case.pos.min(0), case.pos.max(0)
     root = Item("JPartDataBi4")
@@ -509,7 +512,7 @@ def read_case(directory, name: str):
                       if ct.find("cflnumber") is not None else gf("CFLnumber", 0.2),
                       dtini=gf("DtIni", 0.0), dtmin=gf("DtMin", 0.0),
                       rhopoutmin=gf("RhopOutMin", 700.0), rhopoutmax=gf("RhopOutMax", 1300.0),
-                      timemax=gf("TimeMax", 1.0))
+                      timemax=gf("TimeMax", 1.0), timeout=gf("TimeOut", 0.0))
     parts = ex.find("particles")
     blocks = list(parts)
     if any(bk.tag == "floating" for bk in blocks):

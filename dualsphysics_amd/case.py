@@ -68,6 +68,7 @@ class Parameters:
     cellmode: int = 1              # 1 Full, 2 Half
     celldomfixed: int = 0
     timemax: float = 1.0
+    timeout: float = 0.0           # TimeOut: time between PART files (0: none)
 
 
 @dataclasses.dataclass

@@ -348,14 +348,15 @@ struct TileLds {
   unsigned tbase[kRing];
 };
 
-template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing>
+template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false>
 __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned pinit,
     float viscob, float viscof, const DivData& dvd, const Cte& c, float ks2scan,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ posf, const float4* __restrict__ velrho,
     const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
     float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
-    float* __restrict__ velgrad, TileLds<kRing>& lds) {
+    float* __restrict__ velgrad, TileLds<kRing>& lds, const uint8_t* __restrict__ bmode = nullptr,
+    const float* __restrict__ tangenvel = nullptr) {
   float4* tile = lds.tile;
   auto& masks = lds.masks;
   unsigned* tbase = lds.tbase;
@@ -409,8 +410,8 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
     pr2 = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(press) + (jj << 3));
   };
   auto eval = [&](unsigned j, const float4& ps2, const float4& vr2, const float2& pr2) {
-    pair_eval<TVISCO, TDDT, VELGRAD>((j >> 31) != 0, j & 0x7fffffffu, ps2, vr2, pr2, q, tau,
-                                     viscob, viscof, c, a);
+    pair_eval<TVISCO, TDDT, VELGRAD, MDBC2>((j >> 31) != 0, j & 0x7fffffffu, ps2, vr2, pr2, q,
+                                            tau, viscob, viscof, c, a, bmode, tangenvel);
   };
   // k wave-uniform iterations; each lane pops up to k of its queued pairs.
   // Two register sets alternate so the loads of the next pair are in flight
@@ -530,19 +531,21 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
   if (valid) store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
 }
 
-template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing>
+template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false>
 __global__ void __launch_bounds__(64) k_forces_fluid_tile(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c, float ks2scan,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ posf, const float4* __restrict__ velrho,
     const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
     float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
-    float* __restrict__ velgrad) {
+    float* __restrict__ velgrad, const uint8_t* __restrict__ bmode = nullptr,
+    const float* __restrict__ tangenvel = nullptr) {
   __shared__ TileLds<kRing> lds;
   const unsigned p = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-  fluid_tile_body<TVISCO, TDDT, VELGRAD, kRing>(p, p < n, pinit, viscob, viscof, dvd, c, ks2scan,
-                                                dcell, poscell, posf, velrho, press, tau, viscdt,
-                                                ar, ace, strain, velgrad, lds);
+  fluid_tile_body<TVISCO, TDDT, VELGRAD, kRing, MDBC2>(p, p < n, pinit, viscob, viscof, dvd, c,
+                                                       ks2scan, dcell, poscell, posf, velrho,
+                                                       press, tau, viscdt, ar, ace, strain,
+                                                       velgrad, lds, bmode, tangenvel);
 }
 
 // Boundary particle p1 against fluid neighbours (KerInteractionForcesBound,
@@ -599,8 +602,16 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
                   const float2* press, const float4* posf) {
   KTimer kt(ctx, "k_forces_fluid");
   const unsigned npf = t->np - t->npb;
-  if (t->d_boundmode) {
-    // mDBC2 boundaries: the one-lane-per-particle kernel.
+  if (t->d_boundmode && ctx->forces_kernel == 0 && !VELGRAD) {
+    // mDBC2 boundaries in the tile kernel.
+    k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, true><<<grid_for(npf, 64), 64, 0,
+                                                                 ctx->stream>>>(
+        npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
+        (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press, t->d_spstaurho2,
+        t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, nullptr, t->d_boundmode, t->d_tangenvel);
+  } else if (t->d_boundmode) {
+    // mDBC2 with the velocity gradient (learned closure) or the rows kernel
+    // requested: the one-lane-per-particle kernel.
     k_forces_fluid_rows<TVISCO, TDDT, VELGRAD, true><<<grid_for(npf, kBsForces), kBsForces, 0,
                                                        ctx->stream>>>(
         npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,

@@ -196,3 +196,38 @@ class SphGpuSingle:
         self._check(self.lib.dsph_sim_filter_probes(self._h, len(probes), _ptr(probes), hf,
                                                     _ptr(out)))
         return out
+
+
+def run_case(directory, name: str, out_dir, device: int = 0, timemax: float | None = None,
+             timeout: float | None = None, sync_dt: bool = False, log=None) -> list:
+    """Runs the case <directory>/<name>.xml + .bi4 (GenCase output) to its
+    TimeMax, saving Part_0000 and then a PART every TimeOut into
+    <out_dir>/data, as JSphGpuSingle::Run's main loop (JSphGpuSingle.cpp:
+    1000-1040: a PART after the step that reaches the next output time,
+    which is then the next multiple of TimeOut after the current time).
+    Returns the paths of the PART files."""
+    import math
+    from pathlib import Path
+    from . import bi4
+    c = bi4.read_case(directory, name)
+    tmax = c.params.timemax if timemax is None else timemax
+    tout = c.params.timeout if timeout is None else timeout
+    data = Path(out_dir) / "data"
+    s = SphGpuSingle(c, device)
+    try:
+        parts = [s.save_part(data, 0)]
+        if tout <= 0:
+            tout = tmax
+        tnext = tout
+        while True:
+            s.run_until(tnext)
+            t = s.stats()["timestep"]
+            parts.append(s.save_part(data, len(parts)))
+            if log:
+                log(f"{name}: t={t:.6f} s, {s.stats()['nstep']} steps, part {len(parts) - 1}")
+            if t >= tmax:
+                break
+            tnext = (math.floor(t / tout + 1e-9) + 1) * tout
+    finally:
+        s.close()
+    return parts

@@ -31,8 +31,11 @@ def test_case_round_trip(tmp_path, k):
     bi4.write_case(c, tmp_path, "case")
     r = bi4.read_case(tmp_path, "case")
     for f in ("dp", "h", "rhop0", "gamma", "b", "massfluid", "massbound", "gravity", "nbound",
-              "nfluid", "nmoving", "maprealposmin", "maprealposmax", "params"):
+              "nfluid", "nmoving", "maprealposmin", "maprealposmax"):
         assert getattr(r, f) == getattr(c, f), f
+    import dataclasses
+    assert r.params.timeout == c.params.timemax   # TimeOut 0 is written as TimeMax
+    assert dataclasses.replace(r.params, timeout=c.params.timeout) == c.params
     assert abs(r.cs0 - c.cs0) <= 1e-12 * c.cs0
     for f in ("idp", "pos", "velrho"):
         np.testing.assert_array_equal(getattr(r, f), getattr(c, f), err_msg=f)

@@ -72,10 +72,12 @@ def test_mdbc2_correction_matches_oracle(lib, ctx, name):
     np.testing.assert_array_equal(g_vr[:npb, :3], s["d"]["velrho"][:npb, :3])
 
 
+@pytest.mark.parametrize("kernel", [N.FORCES_TILE, N.FORCES_ROWS], ids=["tile", "rows"])
 @pytest.mark.parametrize("tvisco,tddt", [(N.VISCO_LAMINAR, N.DDT_DDT2FULL),
                                          (N.VISCO_ARTIFICIAL, N.DDT_DDT2FULL),
                                          (N.VISCO_LAMINARSPS, N.DDT_DDT2)])
-def test_interaction_with_boundary_modes_matches_oracle(lib, ctx, tvisco, tddt):
+def test_interaction_with_boundary_modes_matches_oracle(lib, ctx, tvisco, tddt, kernel):
+    assert lib.dsph_ctx_set_option(ctx, N.OPT_FORCES_KERNEL, kernel) == 0
     c = _cases()["perturbed_dam_break"]
     s = mdbc2_state(c)
     n, npb, npbok = s["n"], s["npb"], s["npbok"]
@@ -130,6 +132,7 @@ def test_interaction_with_boundary_modes_matches_oracle(lib, ctx, tvisco, tddt):
     # The modes change the result well beyond the tolerance (the test sees them).
     d_modes = rel_err(res[False]["ace"][fl], o["ace"][fl])
     print(f"tvisco {tvisco} tddt {tddt}: {e}, without modes the oracle differs by {d_modes:.2e}")
+    assert lib.dsph_ctx_set_option(ctx, N.OPT_FORCES_KERNEL, N.FORCES_TILE) == 0
     for k, v in e.items():
         assert v < 1e-5, (k, v)
     assert d_modes > 20 * 1e-5

@@ -39,3 +39,26 @@ def test_resume_equals_uninterrupted_run(tmp_path, name, sync_dt):
         np.testing.assert_array_equal(ga[k], gr[k], err_msg=k)
     a.close()
     r.close()
+
+
+def test_run_case_from_files(tmp_path):
+    """A case given as XML + .bi4 runs to TimeMax with a PART every TimeOut;
+    the last PART equals a direct run of the same steps."""
+    from dualsphysics_amd import bi4
+    from dualsphysics_amd.sim import run_case
+    c = casemod.dam_break(dp=0.04)
+    bi4.write_case(c, tmp_path / "in", "case", timemax=0.006, timeout=0.002)
+    parts = run_case(tmp_path / "in", "case", tmp_path / "out")
+    assert len(parts) == 4
+    times = [bi4.read_part(p)["time"] for p in parts]
+    assert times[0] == 0.0
+    for k, t in enumerate(times[1:], 1):
+        assert t >= 0.002 * k - 1e-12 and t < 0.002 * k + 0.001
+    last = bi4.read_part(parts[-1], sort_by_id=False)
+    s = SphGpuSingle(c)
+    s.run(int(last["step"]))
+    g = s.download()
+    np.testing.assert_array_equal(g["idp"], last["idp"])
+    np.testing.assert_array_equal(g["pos"], last["pos"])
+    np.testing.assert_array_equal(g["velrho"][:, 3], last["rho"])
+    s.close()
