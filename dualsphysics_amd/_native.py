@@ -164,6 +164,11 @@ SIGNATURES = {
     "dsph_sim_create": [_i, C.POINTER(DsphCase), _u, _vp, _vp, _vp, _vp, C.POINTER(_vp)],
     "dsph_sim_destroy": [_vp],
     "dsph_sim_set_time": [_vp, _d, _d, _u],
+    "dsph_sim_gauge_points": [_vp, _u, _vp, _vp, _vp],
+    "dsph_sim_gauge_maxz": [_vp, _u, _vp, _vp, _vp],
+    "dsph_gauge_points": [_vp, _u, _vp, C.POINTER(DsphDivData), C.POINTER(_d), C.POINTER(_d)]
+    + [_vp] * 6,
+    "dsph_gauge_maxz": [_vp, _u, _vp, _vp, C.POINTER(DsphDivData), _d, _vp, _vp, _vp, _vp],
     "dsph_sim_last_error": [_vp],
     "dsph_sim_run": [_vp, _u, _i],
     "dsph_sim_stats": [_vp, C.POINTER(DsphStats)],

@@ -193,7 +193,7 @@ def read_file(path) -> Item:
 
 # ---------------------------------------------------------------- case files
 def write_case(case, directory, name: str, timemax: float | None = None,
-               timeout: float | None = None):
+               timeout: float | None = None, gauges=None):
     """Writes <name>.xml and <name>.bi4 for the reference solver: the
     <execution> section GenCase would produce (constants, parameters,
     particle blocks mk 0 = fixed walls, mk 1 = fluid) and the particle file
@@ -323,10 +323,48 @@ def write_case(case, directory, name: str, timemax: float | None = None,
                 <posmax x="default" y="default" z="default + 50%" />
             </simulationdomain>
         </parameters>
-    </execution>
+{_gauges_xml(gauges)}    </execution>
 </case>
 """
     (d / f"{name}.xml").write_text(xml)
+
+
+def _gauges_xml(gs) -> str:
+    """<special><gauges> of a dualsphysics_amd.gauges.GaugeSystem (the keys
+    JGaugeSystem::ReadXml reads, JDsGaugeSystem.cpp:244-310); CSV output at
+    every computation."""
+    if gs is None or not gs.gauges:
+        return ""
+    from . import gauges as G
+    r = repr
+    v3 = lambda tag, x: f'<{tag} x="{r(float(x[0]))}" y="{r(float(x[1]))}" z="{r(float(x[2]))}" />'
+    ind = " " * 16
+    out = ["        <special>", "            <gauges>", "                <default>",
+           f'                    <computedt value="{r(float(gs.computedt))}" />',
+           '                    <output value="true" />',
+           f'                    <outputdt value="{r(float(gs.computedt))}" />',
+           "                </default>"]
+    for g in gs.gauges:
+        cd = ([f'{ind}    <computedt value="{r(float(g.computedt))}" />',
+               f'{ind}    <outputdt value="{r(float(g.computedt))}" />']
+              if g.computedt is not None else [])
+        if isinstance(g, G.Velocity):
+            out += [f'{ind}<velocity name="{g.name}">', f"{ind}    {v3('point', g.point)}"] + cd + \
+                   [f"{ind}</velocity>"]
+        elif isinstance(g, G.Swl):
+            ml = ([f'{ind}    <masslimit value="{r(float(g.masslimit))}" />']
+                  if g.masslimit is not None else [])
+            out += [f'{ind}<swl name="{g.name}">', f"{ind}    {v3('point0', g.point0)}",
+                    f"{ind}    {v3('point2', g.point2)}",
+                    f'{ind}    <pointdp value="{r(float(g.pointdp))}" />'] + ml + cd + \
+                   [f"{ind}</swl>"]
+        else:
+            out += [f'{ind}<maxz name="{g.name}">', f"{ind}    {v3('point0', g.point0)}",
+                    f'{ind}    <height value="{r(float(g.height))}" />',
+                    f'{ind}    <distlimit value="{r(float(g.distlimit))}" />'] + cd + \
+                   [f"{ind}</maxz>"]
+    out += ["            </gauges>", "        </special>", ""]
+    return "\n".join(out)
 
 
 def read_part(path, sort_by_id: bool = True) -> dict:

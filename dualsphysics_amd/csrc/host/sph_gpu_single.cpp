@@ -168,6 +168,62 @@ int dsph_sim_filter_probes(dsph_sim* sim, uint32_t nprobe, const double* h_probe
   return DSPH_OK;
 }
 
+int dsph_sim_gauge_points(dsph_sim* sim, uint32_t n, const double* h_points, float* h_vel,
+                          float* h_mass) {
+  if (!sim || !h_points || (!h_vel && !h_mass)) return DSPH_E_ARG;
+  if (!n) return DSPH_OK;
+  double* dp = nullptr;
+  float *dv = nullptr, *dm = nullptr;
+  int rc = DSPH_OK;
+  try {
+    dp = sim->alloc<double>(3 * size_t(n));
+    if (h_vel) dv = sim->alloc<float>(3 * size_t(n));
+    if (h_mass) dm = sim->alloc<float>(n);
+    sim->chk(dsph_memcpy_h2d(sim->ctx, dp, h_points, 24 * size_t(n)), "gauge points");
+    const dsph_divdata d = sim->divdata();
+    sim->chk(dsph_gauge_points(sim->ctx, n, dp, &d, sim->MapRealPosMin, sim->MapRealPosMax,
+                               sim->posxy, sim->posz, sim->velrho, sim->code, dv, dm),
+             "gauge points");
+    if (h_vel) sim->chk(dsph_memcpy_d2h(sim->ctx, h_vel, dv, 12 * size_t(n)), "gauge vel");
+    if (h_mass) sim->chk(dsph_memcpy_d2h(sim->ctx, h_mass, dm, 4 * size_t(n)), "gauge mass");
+  } catch (const Err& e) {
+    sim->err = e.what();
+    rc = e.code;
+  }
+  dsph_free(sim->ctx, dp);
+  dsph_free(sim->ctx, dv);
+  dsph_free(sim->ctx, dm);
+  return rc;
+}
+
+int dsph_sim_gauge_maxz(dsph_sim* sim, uint32_t n, const double* h_points,
+                        const float* h_height_dist, float* h_zmax) {
+  if (!sim || !h_points || !h_height_dist || !h_zmax) return DSPH_E_ARG;
+  if (!n) return DSPH_OK;
+  double* dp = nullptr;
+  float *dh = nullptr, *dz = nullptr;
+  int rc = DSPH_OK;
+  try {
+    dp = sim->alloc<double>(3 * size_t(n));
+    dh = sim->alloc<float>(2 * size_t(n));
+    dz = sim->alloc<float>(n);
+    sim->chk(dsph_memcpy_h2d(sim->ctx, dp, h_points, 24 * size_t(n)), "maxz points");
+    sim->chk(dsph_memcpy_h2d(sim->ctx, dh, h_height_dist, 8 * size_t(n)), "maxz sizes");
+    const dsph_divdata d = sim->divdata();
+    sim->chk(dsph_gauge_maxz(sim->ctx, n, dp, dh, &d, sim->MapRealPosMin[2], sim->posxy,
+                             sim->posz, sim->code, dz),
+             "gauge maxz");
+    sim->chk(dsph_memcpy_d2h(sim->ctx, h_zmax, dz, 4 * size_t(n)), "gauge zmax");
+  } catch (const Err& e) {
+    sim->err = e.what();
+    rc = e.code;
+  }
+  dsph_free(sim->ctx, dp);
+  dsph_free(sim->ctx, dh);
+  dsph_free(sim->ctx, dz);
+  return rc;
+}
+
 int dsph_sim_pips(dsph_sim* sim, uint64_t h_res[5]) {
   if (!sim || !h_res) return DSPH_E_ARG;
   const dsph_divdata d = sim->divdata();

@@ -303,6 +303,108 @@ __global__ void __launch_bounds__(kBsForces) k_mdbc2(unsigned n, DivData dvd, Ct
   tv[0] = t.x; tv[1] = t.y; tv[2] = t.z;
 }
 
+// ---------------------------------------------------------------- gauges
+// Velocity and mass at measurement points (KerInteractionGaugeVel and
+// KerCalculeMass, JDsGauge_ker.cu:50-164): one lane per point, FP64 sums in
+// the reference's row order over the fluid particles within 2h of
+// wab*V*v (velocity, no Shepard normalisation) and wab*V*m (mass), with
+// V = m/rho; distances float(FP64 difference); points outside the map give 0.
+__device__ inline float gauge_wab(float rr2, const Cte& c) {
+  const float qq = sqrtf(rr2) / c.h;
+  const float wqq = qq + qq + 1.f;
+  const float wqq1 = 1.f - 0.5f * qq;
+  const float wqq2 = wqq1 * wqq1;
+  return c.awen * wqq * wqq2 * wqq2;
+}
+
+__global__ void __launch_bounds__(kBs) k_gauge_points(unsigned n,
+    const double* __restrict__ pts, DivData dvd, Cte c, double3 mapmin, double3 mapmax,
+    const double2* __restrict__ posxy, const double* __restrict__ posz,
+    const float4* __restrict__ velrho, const typecode* __restrict__ code,
+    float* __restrict__ outvel, float* __restrict__ outmass) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double px = pts[3 * size_t(i)], py = pts[3 * size_t(i) + 1], pz = pts[3 * size_t(i) + 2];
+  double sx = 0, sy = 0, sz = 0, sm = 0;
+  const bool out = !(px >= mapmin.x && py >= mapmin.y && pz >= mapmin.z && px < mapmax.x &&
+                     py < mapmax.y && pz < mapmax.z);
+  if (!out) {
+    const int cx = int((px - dvd.domposminx) / dvd.scell) - dvd.czx;
+    const int cy = int((py - dvd.domposminy) / dvd.scell) - dvd.czy;
+    const int cz = int((pz - dvd.domposminz) / dvd.scell) - dvd.czz;
+    const NgLimits l = ng_limits(cx, cy, cz, dvd);
+    for (int z = l.z0; z < l.z1; z++)
+      for (int y = l.y0; y < l.y1; y++) {
+        unsigned pini, pfin;
+        row_range(dvd.begincell, int(dvd.cellfluid) + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1,
+                  pini, pfin);
+        for (unsigned p2 = pini; p2 < pfin; p2++) {
+          const double2 xy = posxy[p2];
+          const float drx = float(px - xy.x), dry = float(py - xy.y), drz = float(pz - posz[p2]);
+          const float rr2 = drx * drx + dry * dry + drz * drz;
+          if (rr2 <= c.ks2 && rr2 >= kAlmostZero && code_is_fluid(code[p2])) {
+            const float4 v = velrho[p2];
+            const float wab = gauge_wab(rr2, c) * (c.massf / v.w);
+            sx += wab * v.x; sy += wab * v.y; sz += wab * v.z;
+            sm += wab * c.massf;
+          }
+        }
+      }
+  }
+  if (outvel) {
+    outvel[3 * size_t(i)] = float(sx);
+    outvel[3 * size_t(i) + 1] = float(sy);
+    outvel[3 * size_t(i) + 2] = float(sz);
+  }
+  if (outmass) outmass[i] = float(sm);
+}
+
+// Highest fluid particle near a vertical line (KerInteractionGaugeMaxz,
+// JDsGauge_ker.cu:252-312; cell range JGaugeMaxZ::GetInteractionCellsMaxZ,
+// JDsGaugeItem.cpp:1023-1040): one lane per gauge, cell layers from the top
+// of [z0, z0+height] (widened by the distance limit) down to the first layer
+// with a fluid particle within distlimit horizontally; result
+// max(zmax, zfloor).
+__global__ void __launch_bounds__(kBs) k_gauge_maxz(unsigned n, const double* __restrict__ pts,
+    const float* __restrict__ hd, DivData dvd, double zfloor,
+    const double2* __restrict__ posxy, const double* __restrict__ posz,
+    const typecode* __restrict__ code, float* __restrict__ out) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double px = pts[3 * size_t(i)], py = pts[3 * size_t(i) + 1], pz = pts[3 * size_t(i) + 2];
+  const float height = hd[2 * size_t(i)], dist = hd[2 * size_t(i) + 1];
+  const float maxdist2 = dist * dist;
+  const int cx = int((px - dvd.domposminx) / dvd.scell) - dvd.czx;
+  const int cy = int((py - dvd.domposminy) / dvd.scell) - dvd.czy;
+  const int cz = int((pz - dvd.domposminz) / dvd.scell) - dvd.czz;
+  const int cz2 = int((pz + height - dvd.domposminz) / dvd.scell) - dvd.czz;
+  const int ncel = int(ceilf(dist / dvd.scell));
+  const int x0 = max(cx - ncel, 0), x1 = min(cx + ncel + 1, dvd.ncx);
+  const int y0 = max(cy - ncel, 0), y1 = min(cy + ncel + 1, dvd.ncy);
+  const int z0 = max(cz - ncel, 0), z1 = min(cz2 + ncel + 1, dvd.ncz);
+  float zmax = -FLT_MAX;
+  bool found = false;
+  if (x0 < x1)
+    for (int z = z1 - 1; z >= z0 && !found; z--)
+      for (int y = y0; y < y1; y++) {
+        unsigned pini, pfin;
+        row_range(dvd.begincell, int(dvd.cellfluid) + z * dvd.ncw + y * dvd.ncx, x0, x1, pini,
+                  pfin);
+        for (unsigned p2 = pini; p2 < pfin; p2++) {
+          const double z2 = posz[p2];
+          if (z2 > zmax) {
+            const double2 xy = posxy[p2];
+            const float drx = float(px - xy.x), dry = float(py - xy.y);
+            if (drx * drx + dry * dry <= maxdist2 && code_is_fluid(code[p2])) {
+              zmax = float(z2);
+              found = true;
+            }
+          }
+        }
+      }
+  out[i] = fmaxf(zmax, float(zfloor));
+}
+
 // ---------------------------------------------------------------- PIPS
 // Real and checked neighbour counts (KerInteractionNg, JDsPips_ker.cu:54-124):
 // boundary p1 in [pinitb,pinitb+nb) count fluid neighbours; fluid p1 in
@@ -411,6 +513,38 @@ int dsph_mdbc2_correction(dsph_ctx* ctx, uint32_t n, const float gravity[3],
       (const float4*)d_poscell, d_code, d_boundnor, d_motionvel, d_motionace, (float4*)d_velrho,
       d_boundmode, d_tangenvel);
   DSPH_LAUNCHED(ctx, "k_mdbc2");
+  return DSPH_OK;
+}
+
+int dsph_gauge_points(dsph_ctx* ctx, uint32_t n, const double* d_points,
+                      const dsph_divdata* dvdp, const double mapmin[3], const double mapmax[3],
+                      const double* d_posxy, const double* d_posz, const float* d_velrho,
+                      const uint16_t* d_code, float* d_vel, float* d_mass) {
+  if (!ctx || !dvdp || !mapmin || !mapmax) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  if (!n) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_points && d_posxy && d_posz && d_velrho && d_code && (d_vel || d_mass),
+               "dsph_gauge_points: null array");
+  k_gauge_points<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(
+      n, d_points, make_divdata(*dvdp), make_cte(ctx->cte),
+      make_double3(mapmin[0], mapmin[1], mapmin[2]), make_double3(mapmax[0], mapmax[1], mapmax[2]),
+      (const double2*)d_posxy, d_posz, (const float4*)d_velrho, d_code, d_vel, d_mass);
+  DSPH_LAUNCHED(ctx, "k_gauge_points");
+  return DSPH_OK;
+}
+
+int dsph_gauge_maxz(dsph_ctx* ctx, uint32_t n, const double* d_points,
+                    const float* d_height_dist, const dsph_divdata* dvdp, double zfloor,
+                    const double* d_posxy, const double* d_posz, const uint16_t* d_code,
+                    float* d_zmax) {
+  if (!ctx || !dvdp) return DSPH_E_ARG;
+  if (!n) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_points && d_height_dist && d_posxy && d_posz && d_code && d_zmax,
+               "dsph_gauge_maxz: null array");
+  k_gauge_maxz<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(
+      n, d_points, d_height_dist, make_divdata(*dvdp), zfloor, (const double2*)d_posxy, d_posz,
+      d_code, d_zmax);
+  DSPH_LAUNCHED(ctx, "k_gauge_maxz");
   return DSPH_OK;
 }
 

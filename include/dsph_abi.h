@@ -361,6 +361,26 @@ int dsph_mdbc2_correction(dsph_ctx* ctx, uint32_t n, const float gravity[3],
                           const float* d_motionace, float* d_velrho,
                           uint8_t* d_boundmode, float* d_tangenvel);
 
+/* Replaces cusph::Interaction_GaugeVel and the mass sums of
+ * Interaction_GaugeSwl (JDsGauge_ker.cu:50-245) for n points at once
+ * (double3[n]): d_vel (float3[n]) = sum over fluid within 2h of W*V*v
+ * (no Shepard normalisation, as the reference), d_mass (float[n]) =
+ * sum W*V*massf, FP64 sums; points outside [mapmin, mapmax) give 0. Either
+ * output may be NULL. The SWL level follows from the masses along the
+ * gauge line (JGaugeSwl, JDsGaugeItem.cpp:758-790). */
+int dsph_gauge_points(dsph_ctx* ctx, uint32_t n, const double* d_points,
+                      const dsph_divdata* dvd, const double mapmin[3],
+                      const double mapmax[3], const double* d_posxy,
+                      const double* d_posz, const float* d_velrho,
+                      const uint16_t* d_code, float* d_vel, float* d_mass);
+/* Replaces cusph::Interaction_GaugeMaxz (JDsGauge_ker.cu:252-312) for n
+ * gauges: point0 (double3[n]) and {height, distlimit} (float2[n]); d_zmax =
+ * max(highest fluid z within distlimit of the vertical line, zfloor). */
+int dsph_gauge_maxz(dsph_ctx* ctx, uint32_t n, const double* d_points,
+                    const float* d_height_dist, const dsph_divdata* dvd,
+                    double zfloor, const double* d_posxy, const double* d_posz,
+                    const uint16_t* d_code, float* d_zmax);
+
 /* --------------------------------------------------- reductions and dt */
 
 /* Replaces cusph::ComputeVelMod + ReduMaxFloat (JSphGpu_ker.h:281,290):
@@ -605,6 +625,14 @@ int dsph_sim_filter_probes(dsph_sim* sim, uint32_t nprobe, const double* h_probe
 /* Evaluates PreInteraction_Forces + Interaction_Forces on the current state
  * without advancing it (forces then readable with dsph_sim_download_forces). */
 int dsph_sim_interact(dsph_sim* sim);
+/* Gauges on the current state (after the last step's divide): h_points
+ * double3[n] -> h_vel float3[n] and/or h_mass float[n] (dsph_gauge_points);
+ * maxz gauges: h_points double3[n], h_height_dist float2[n] -> h_zmax
+ * (dsph_gauge_maxz, floor = the map's minimum z). Synchronous. */
+int dsph_sim_gauge_points(dsph_sim* sim, uint32_t n, const double* h_points,
+                          float* h_vel, float* h_mass);
+int dsph_sim_gauge_maxz(dsph_sim* sim, uint32_t n, const double* h_points,
+                        const float* h_height_dist, float* h_zmax);
 /* Access to the context of a simulation (streams, events). */
 dsph_ctx* dsph_sim_ctx(dsph_sim* sim);
 

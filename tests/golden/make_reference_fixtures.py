@@ -19,7 +19,7 @@ ref_observables_dp0.01.npz (--observables, ~1 h on 8 cores): front position
 and probe pressure every 0.01 s over 1 s of the dp=0.01 dam break.
 
 Run in the development container (the reference is not available on the
-GPU hosts):  python tests/golden/make_reference_fixtures.py [--observables] [names]
+GPU hosts):  python tests/golden/make_reference_fixtures.py [--observables | --gauges] [names]
 """
 from __future__ import annotations
 
@@ -104,11 +104,51 @@ def observables_fixture(dp: float = 0.01, tmax: float = 1.0, tout: float = 0.01)
     print(f"wrote {name}.npz ({len(times)} samples)")
 
 
+GAUGE_DP = 0.03
+GAUGE_TMAX = 0.02
+
+
+def gauge_system(case):
+    """The gauges of ref_gauges_dambreak_dp0.03.npz: two velocity points, two
+    SWL lines (one in the column, one ahead of it) and a maxz line."""
+    from dualsphysics_amd import gauges as G
+    return G.GaugeSystem(case, [
+        G.Velocity("v1", (0.3, 0.335, 0.1)), G.Velocity("v2", (0.05, 0.2, 0.25)),
+        G.Swl("s1", (0.2, 0.335, 0.0), (0.2, 0.335, 0.4), 0.015),
+        G.Swl("s2", (0.38, 0.335, 0.0), (0.38, 0.335, 0.4), 0.015),
+        G.MaxZ("m1", (0.35, 0.335, 0.0), 0.4, 0.06)], computedt=0.001)
+
+
+def gauges_fixture():
+    """ref_gauges_dambreak_dp0.03.npz: the reference's gauge CSV series
+    (GaugesVel/SWL/MaxZ) over 0.02 s of the dp=0.03 dam break."""
+    from dualsphysics_amd import gauges as G
+    case = cm.dam_break(dp=GAUGE_DP)
+    gs = gauge_system(case)
+    with tempfile.TemporaryDirectory() as td:
+        w = Path(td)
+        bi4.write_case(case, w / "in", "case", timemax=GAUGE_TMAX, timeout=0.005, gauges=gs)
+        cmd = [str(REF_EXE), str(w / "in" / "case"), str(w / "out"), "-cpu", "-ompthreads:8",
+               "-saveposdouble:1", "-stable:1", "-svres:0"]
+        r = subprocess.run(cmd, cwd=w, capture_output=True, text=True, timeout=3600)
+        if r.returncode != 0:
+            raise RuntimeError(r.stdout[-4000:] + r.stderr[-2000:])
+        kinds = {G.Velocity: "Vel", G.Swl: "SWL", G.MaxZ: "MaxZ"}
+        arrays = {g.name: G.read_csv(w / "out" / f"Gauges{kinds[type(g)]}_{g.name}.csv")
+                  for g in gs.gauges}
+    np.savez_compressed(OUT / "ref_gauges_dambreak_dp0.03.npz", dp=GAUGE_DP, seed=42,
+                        tmax=GAUGE_TMAX, **arrays)
+    print("wrote ref_gauges_dambreak_dp0.03.npz", {k: v.shape for k, v in arrays.items()})
+
+
 def main():
     if not REF_EXE.exists():
         sys.exit(f"{REF_EXE} missing: make -f oracle/ref.mk (needs /root/reference)")
     if "--observables" in sys.argv:
         observables_fixture()
+        return
+    if "--gauges" in sys.argv:
+        gauges_fixture()
         return
     only = [a for a in sys.argv[1:] if not a.startswith("--")]
     for name, (gen, dp, nsteps, saved, over) in CASES.items():
