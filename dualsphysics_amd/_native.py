@@ -166,6 +166,8 @@ SIGNATURES = {
     "dsph_sim_set_time": [_vp, _d, _d, _u],
     "dsph_sim_gauge_points": [_vp, _u, _vp, _vp, _vp],
     "dsph_sim_gauge_maxz": [_vp, _u, _vp, _vp, _vp],
+    "dsph_sim_gauge_force": [_vp, _u, _u, _vp],
+    "dsph_gauge_force": [_vp, _u, _u, _u, C.POINTER(DsphDivData)] + [_vp] * 6,
     "dsph_gauge_points": [_vp, _u, _vp, C.POINTER(DsphDivData), C.POINTER(_d), C.POINTER(_d)]
     + [_vp] * 6,
     "dsph_gauge_maxz": [_vp, _u, _vp, _vp, C.POINTER(DsphDivData), _d, _vp, _vp, _vp, _vp],

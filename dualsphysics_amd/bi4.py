@@ -358,6 +358,9 @@ def _gauges_xml(gs) -> str:
                     f"{ind}    {v3('point2', g.point2)}",
                     f'{ind}    <pointdp value="{r(float(g.pointdp))}" />'] + ml + cd + \
                    [f"{ind}</swl>"]
+        elif isinstance(g, G.Force):
+            out += [f'{ind}<force name="{g.name}">', f'{ind}    <target mkbound="{g.mkbound}" />'] + \
+                   cd + [f"{ind}</force>"]
         else:
             out += [f'{ind}<maxz name="{g.name}">', f"{ind}    {v3('point0', g.point0)}",
                     f'{ind}    <height value="{r(float(g.height))}" />',

@@ -224,6 +224,26 @@ int dsph_sim_gauge_maxz(dsph_sim* sim, uint32_t n, const double* h_points,
   return rc;
 }
 
+int dsph_sim_gauge_force(dsph_sim* sim, uint32_t idbegin, uint32_t count, float* h_ace) {
+  if (!sim || !h_ace) return DSPH_E_ARG;
+  if (!count) return DSPH_OK;
+  float* da = nullptr;
+  int rc = DSPH_OK;
+  try {
+    da = sim->alloc<float>(3 * size_t(count));
+    const dsph_divdata d = sim->divdata();
+    sim->chk(dsph_gauge_force(sim->ctx, sim->Npb, idbegin, count, &d, sim->posxy, sim->posz,
+                              sim->code, sim->idp, sim->velrho, da),
+             "gauge force");
+    sim->chk(dsph_memcpy_d2h(sim->ctx, h_ace, da, 12 * size_t(count)), "gauge force ace");
+  } catch (const Err& e) {
+    sim->err = e.what();
+    rc = e.code;
+  }
+  dsph_free(sim->ctx, da);
+  return rc;
+}
+
 int dsph_sim_pips(dsph_sim* sim, uint64_t h_res[5]) {
   if (!sim || !h_res) return DSPH_E_ARG;
   const dsph_divdata d = sim->divdata();

@@ -405,6 +405,53 @@ __global__ void __launch_bounds__(kBs) k_gauge_maxz(unsigned n, const double* __
   out[i] = fmaxf(zmax, float(zfloor));
 }
 
+// Pressure force of the fluid on boundary particles with id in
+// [idbegin, idbegin+count) (KerInteractionGaugeForce, JDsGauge_ker.cu:497-545):
+// per particle sum over fluid neighbours of -(P1+P2)/(rho1 rho2) m2 gradW,
+// P = B((rho/rho0)^gamma - 1); ace[id-idbegin] (the caller sums and scales by
+// the boundary mass).
+__global__ void __launch_bounds__(kBs) k_gauge_force(unsigned npb, unsigned idbegin,
+    unsigned count, DivData dvd, Cte c, const double2* __restrict__ posxy,
+    const double* __restrict__ posz, const typecode* __restrict__ code,
+    const unsigned* __restrict__ idp, const float4* __restrict__ velrho, float* __restrict__ ace) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npb) return;
+  const unsigned id = idp[p];
+  if (id - idbegin >= count || !code_is_normal(code[p])) return;
+  const double2 xy = posxy[p];
+  const double px = xy.x, py = xy.y, pz = posz[p];
+  const float rho1 = velrho[p].w;
+  const float press1 = c.cteb * (powf(rho1 / c.rhopzero, c.gamma) - 1.f);
+  float ax = 0, ay = 0, az = 0;
+  const int cx = int((px - dvd.domposminx) / dvd.scell) - dvd.czx;
+  const int cy = int((py - dvd.domposminy) / dvd.scell) - dvd.czy;
+  const int cz = int((pz - dvd.domposminz) / dvd.scell) - dvd.czz;
+  const NgLimits l = ng_limits(cx, cy, cz, dvd);
+  for (int z = l.z0; z < l.z1; z++)
+    for (int y = l.y0; y < l.y1; y++) {
+      unsigned pini, pfin;
+      row_range(dvd.begincell, int(dvd.cellfluid) + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1, pini,
+                pfin);
+      for (unsigned p2 = pini; p2 < pfin; p2++) {
+        const double2 xy2 = posxy[p2];
+        const float drx = float(px - xy2.x), dry = float(py - xy2.y), drz = float(pz - posz[p2]);
+        const float rr2 = drx * drx + dry * dry + drz * drz;
+        if (rr2 <= c.ks2 && rr2 >= kAlmostZero && code_is_fluid(code[p2])) {
+          const float qq = sqrtf(rr2) / c.h;
+          const float w1 = 1.f - 0.5f * qq;
+          const float fac = c.bwenh * w1 * w1 * w1;
+          const float rho2 = velrho[p2].w;
+          const float press2 = c.cteb * (powf(rho2 / c.rhopzero, c.gamma) - 1.f);
+          const float prs = (press1 + press2) / (rho1 * rho2);
+          const float p_vpm1 = -prs * c.massf;
+          ax += p_vpm1 * fac * drx; ay += p_vpm1 * fac * dry; az += p_vpm1 * fac * drz;
+        }
+      }
+    }
+  float* a = ace + 3 * size_t(id - idbegin);
+  a[0] = ax; a[1] = ay; a[2] = az;
+}
+
 // ---------------------------------------------------------------- PIPS
 // Real and checked neighbour counts (KerInteractionNg, JDsPips_ker.cu:54-124):
 // boundary p1 in [pinitb,pinitb+nb) count fluid neighbours; fluid p1 in
@@ -545,6 +592,23 @@ int dsph_gauge_maxz(dsph_ctx* ctx, uint32_t n, const double* d_points,
       n, d_points, d_height_dist, make_divdata(*dvdp), zfloor, (const double2*)d_posxy, d_posz,
       d_code, d_zmax);
   DSPH_LAUNCHED(ctx, "k_gauge_maxz");
+  return DSPH_OK;
+}
+
+int dsph_gauge_force(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin, uint32_t count,
+                     const dsph_divdata* dvdp, const double* d_posxy, const double* d_posz,
+                     const uint16_t* d_code, const uint32_t* d_idp, const float* d_velrho,
+                     float* d_ace) {
+  if (!ctx || !dvdp) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  if (!npb || !count) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_posxy && d_posz && d_code && d_idp && d_velrho && d_ace,
+               "dsph_gauge_force: null array");
+  DSPH_HIP(ctx, hipMemsetAsync(d_ace, 0, 12 * size_t(count), ctx->stream));
+  k_gauge_force<<<grid_for(npb, kBs), kBs, 0, ctx->stream>>>(
+      npb, idbegin, count, make_divdata(*dvdp), make_cte(ctx->cte), (const double2*)d_posxy,
+      d_posz, d_code, d_idp, (const float4*)d_velrho, d_ace);
+  DSPH_LAUNCHED(ctx, "k_gauge_force");
   return DSPH_OK;
 }
 

@@ -79,6 +79,25 @@ class MaxZ:
     computedt: float | None = None
 
 
+@dataclasses.dataclass
+class Force:
+    """Pressure force of the fluid on the boundary block ``mkbound`` (0 = the
+    fixed walls, 1 = the moving object here; JGaugeForce,
+    JDsGaugeItem.cpp:1847-1905): massbound times the sum over its particles
+    of -(P1+P2)/(rho1 rho2) massf gradW."""
+    name: str
+    mkbound: int
+    computedt: float | None = None
+
+    def ids(self, case):
+        nfix = case.nbound - case.nmoving
+        if self.mkbound == 0:
+            return 0, nfix
+        if self.mkbound == 1 and case.nmoving:
+            return nfix, case.nmoving
+        raise ValueError(f"no boundary block mkbound={self.mkbound}")
+
+
 def swl_surface(masses: np.ndarray, point0, pointdir, masslimit: float) -> np.ndarray:
     """Surface point from the masses at point0 + k*pointdir, k = 0..np
     (JGaugeSwl::CalculeCpuT, JDsGaugeItem.cpp:770-784; FP32 interpolation)."""
@@ -159,8 +178,20 @@ class GaugeSystem:
             s._check(s.lib.dsph_sim_gauge_maxz(s._h, len(mz), P.ctypes.data_as(C.c_void_p),
                                                HD.ctypes.data_as(C.c_void_p),
                                                zmax.ctypes.data_as(C.c_void_p)))
+        forces = {}
         for g in gs:
-            if isinstance(g, Velocity):
+            if isinstance(g, Force):
+                i0, cnt = g.ids(self.case)
+                ace = np.empty((cnt, 3), np.float32)
+                s._check(s.lib.dsph_sim_gauge_force(s._h, i0, cnt, ace.ctypes.data_as(C.c_void_p)))
+                acc = np.zeros(3, np.float32)
+                for a in ace:            # FP32 sum in id order, as the reference
+                    acc = acc + a
+                forces[g.name] = acc * np.float32(self.case.massbound)
+        for g in gs:
+            if isinstance(g, Force):
+                self.results[g.name].append((t, forces[g.name]))
+            elif isinstance(g, Velocity):
                 i, _ = slices[g.name]
                 self.results[g.name].append((t, vel[i].copy()))
             elif isinstance(g, Swl):
@@ -198,6 +229,11 @@ class GaugeSystem:
                         "pos2x [m];pos2y [m];pos2z [m]")
                 lines = [f"{t:g};{g3(v)};{g3(g.point0)};{g3(g.point2)}" for t, v in rows]
                 f = d / f"GaugesSWL_{g.name}.csv"
+            elif isinstance(g, Force):
+                head = "time [s];force [N];forcex [N];forcey [N];forcez [N]"
+                lines = [f"{t:g};{float(np.float32(np.sqrt(np.float32(np.dot(v, v))))):g};{g3(v)}"
+                         for t, v in rows]
+                f = d / f"GaugesForce_{g.name}.csv"
             else:
                 head = "time [s];zmax [m];posx [m];posy [m];posz [m]"
                 lines = [f"{t:g};{float(np.float32(z)):g};{g3(g.point0)}" for t, z in rows]
@@ -248,6 +284,8 @@ def read_xml(case, xml_path) -> GaugeSystem:
                 mlv = (float(ml.get("value")) if ml.get("value") is not None
                        else case.massfluid * float(ml.get("coef")))
             out.append(Swl(name, v3(e.find("point0")), v3(e.find("point2")), pdp, mlv, cd))
+        elif e.tag == "force":
+            out.append(Force(name, int(e.find("target").get("mkbound")), cd))
         elif e.tag == "maxz":
             dl = e.find("distlimit")
             if dl.get("value") is not None:

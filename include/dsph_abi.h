@@ -381,6 +381,16 @@ int dsph_gauge_maxz(dsph_ctx* ctx, uint32_t n, const double* d_points,
                     double zfloor, const double* d_posxy, const double* d_posz,
                     const uint16_t* d_code, float* d_zmax);
 
+/* Replaces cusph::Interaction_GaugeForce (JDsGauge_ker.cu:497-570): for
+ * the boundary particles [0,npb) with id in [idbegin, idbegin+count),
+ * d_ace[id-idbegin] (float3[count], cleared first) = pressure acceleration
+ * from the fluid, sum of -(P1+P2)/(rho1 rho2) massf gradW; the gauge's force
+ * is massbound times the sum over the object (JDsGaugeItem.cpp:1895-1900). */
+int dsph_gauge_force(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin, uint32_t count,
+                     const dsph_divdata* dvd, const double* d_posxy,
+                     const double* d_posz, const uint16_t* d_code,
+                     const uint32_t* d_idp, const float* d_velrho, float* d_ace);
+
 /* --------------------------------------------------- reductions and dt */
 
 /* Replaces cusph::ComputeVelMod + ReduMaxFloat (JSphGpu_ker.h:281,290):
@@ -633,6 +643,9 @@ int dsph_sim_gauge_points(dsph_sim* sim, uint32_t n, const double* h_points,
                           float* h_vel, float* h_mass);
 int dsph_sim_gauge_maxz(dsph_sim* sim, uint32_t n, const double* h_points,
                         const float* h_height_dist, float* h_zmax);
+/* Force gauge of the boundary object with ids [idbegin, idbegin+count):
+ * h_ace float3[count] per particle (dsph_gauge_force). Synchronous. */
+int dsph_sim_gauge_force(dsph_sim* sim, uint32_t idbegin, uint32_t count, float* h_ace);
 /* Access to the context of a simulation (streams, events). */
 dsph_ctx* dsph_sim_ctx(dsph_sim* sim);
 

@@ -110,13 +110,13 @@ GAUGE_TMAX = 0.02
 
 def gauge_system(case):
     """The gauges of ref_gauges_dambreak_dp0.03.npz: two velocity points, two
-    SWL lines (one in the column, one ahead of it) and a maxz line."""
+    SWL lines, a maxz line and the force on the tank walls."""
     from dualsphysics_amd import gauges as G
     return G.GaugeSystem(case, [
         G.Velocity("v1", (0.3, 0.335, 0.1)), G.Velocity("v2", (0.05, 0.2, 0.25)),
         G.Swl("s1", (0.2, 0.335, 0.0), (0.2, 0.335, 0.4), 0.015),
         G.Swl("s2", (0.38, 0.335, 0.0), (0.38, 0.335, 0.4), 0.015),
-        G.MaxZ("m1", (0.35, 0.335, 0.0), 0.4, 0.06)], computedt=0.001)
+        G.MaxZ("m1", (0.35, 0.335, 0.0), 0.4, 0.06), G.Force("f1", 0)], computedt=0.001)
 
 
 def gauges_fixture():
@@ -133,7 +133,7 @@ def gauges_fixture():
         r = subprocess.run(cmd, cwd=w, capture_output=True, text=True, timeout=3600)
         if r.returncode != 0:
             raise RuntimeError(r.stdout[-4000:] + r.stderr[-2000:])
-        kinds = {G.Velocity: "Vel", G.Swl: "SWL", G.MaxZ: "MaxZ"}
+        kinds = {G.Velocity: "Vel", G.Swl: "SWL", G.MaxZ: "MaxZ", G.Force: "Force"}
         arrays = {g.name: G.read_csv(w / "out" / f"Gauges{kinds[type(g)]}_{g.name}.csv")
                   for g in gs.gauges}
     np.savez_compressed(OUT / "ref_gauges_dambreak_dp0.03.npz", dp=GAUGE_DP, seed=42,

@@ -92,7 +92,7 @@ def test_gauges_follow_the_reference_cpu_build(tmp_path):
     G.run(s, gs, float(fx["tmax"]))
     s.close()
     gs.save_csv(tmp_path)
-    kinds = {G.Velocity: "Vel", G.Swl: "SWL", G.MaxZ: "MaxZ"}
+    kinds = {G.Velocity: "Vel", G.Swl: "SWL", G.MaxZ: "MaxZ", G.Force: "Force"}
     for g in gs.gauges:
         ref = fx[g.name]
         got = G.read_csv(tmp_path / f"Gauges{kinds[type(g)]}_{g.name}.csv")
@@ -103,6 +103,9 @@ def test_gauges_follow_the_reference_cpu_build(tmp_path):
             assert err < 1e-3, (g.name, err)
         elif isinstance(g, G.Swl):
             assert np.abs(got[:, 1:4] - ref[:, 1:4]).max() < 2e-5, g.name
+        elif isinstance(g, G.Force):
+            err = np.abs(got[:, 1:5] - ref[:, 1:5]).max() / np.abs(ref[:, 1]).max()
+            assert err < 1e-4, (g.name, err)
         else:
             assert np.abs(got[:, 1] - ref[:, 1]).max() < 2e-5, g.name
         print(g.name, "ok")
