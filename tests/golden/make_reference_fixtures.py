@@ -54,6 +54,13 @@ CASES = {
     "ref_flume_dp0.01_flap_rot_mdbc": ("flume", 0.01, 10, (1, 5, 10),
                                        dict(wavemaker="flap_rot", tboundary=2, tvisco=2,
                                             visco=1e-6)),
+    # More option combinations: Molteni-Colagrossi DDT; no DDT with
+    # laminar+SPS; Verlet with mDBC and the piston.
+    "ref_dambreak_dp0.03_ddt1": ("dam_break", 0.03, 10, (1, 5, 10), dict(tdensity=1)),
+    "ref_dambreak_dp0.03_noddt_sps": ("dam_break", 0.03, 10, (1, 5, 10),
+                                      dict(tdensity=0, tvisco=2, visco=1e-6)),
+    "ref_flume_dp0.01_piston_mdbc_verlet": ("flume", 0.01, 10, (1, 5, 10),
+                                            dict(tboundary=2, tstep=1, tvisco=3, visco=1e-4)),
     # mDBC2 (SlipMode no-slip) on the dam break's fixed walls.
     "ref_dambreak_dp0.03_mdbc2": ("dam_break", 0.03, 10, (1, 5, 10),
                                   dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4)),

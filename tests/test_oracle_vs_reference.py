@@ -12,7 +12,9 @@ viscosity + DDT2; wave flume with a piston wavemaker (moving boundary,
 rectilinear sinusoid) with DBC, and with mDBC + laminar+SPS viscosity; dam
 break with mDBC2 (SlipMode no-slip) + laminar viscosity; flume with a flap
 wavemaker (matrix motion: rotation sinusoid with DBC, constant rotation with
-mDBC and the normals turned with the flap).
+mDBC and the normals turned with the flap); dam break with the
+Molteni-Colagrossi DDT and with laminar+SPS without DDT; the piston flume
+with Verlet and mDBC.
 """
 import numpy as np
 import pytest
@@ -23,7 +25,8 @@ from tests.helpers import compare_with_reference, load_reference_fixture
 FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_flume_dp0.01_piston", "ref_flume_dp0.01_piston_mdbc_sps",
             "ref_dambreak_dp0.03_mdbc2", "ref_flume_dp0.01_flap",
-            "ref_flume_dp0.01_flap_rot_mdbc"]
+            "ref_flume_dp0.01_flap_rot_mdbc", "ref_dambreak_dp0.03_ddt1",
+            "ref_dambreak_dp0.03_noddt_sps", "ref_flume_dp0.01_piston_mdbc_verlet"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
