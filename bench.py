@@ -169,6 +169,44 @@ def ensemble_bench(n: int, tend: float, sample: int):
                     "(size-stratified sample, mean time x cases)"}
 
 
+def flume_bench(steps: int, warmup: int, dp: float = 0.005):
+    """The full-size wave-flume configuration: 12 x 1 x 1.2 m flume, water
+    depth 0.6 m, piston wavemaker (amplitude 0.1 m, period 1.6 s), dp = dp,
+    mDBC + laminar+SPS; ``steps`` timed Symplectic steps after ``warmup``."""
+    import torch
+    from dualsphysics_amd import case as casemod
+    from dualsphysics_amd import _native as N
+    from dualsphysics_amd.sim import SphGpuSingle
+    t0 = time.perf_counter()
+    log(f"flume: generating the dp={dp:g} case ...")
+    prm = casemod.Parameters(tboundary=N.BC_MDBC, tvisco=N.VISCO_LAMINARSPS, visco=1e-6)
+    c = casemod.flume(dp=dp, length=12.0, width=1.0, height=1.2, depth=0.6, ampl=0.1,
+                      period=1.6, params=prm)
+    log(f"flume: {c.nfluid} fluid + {c.nbound} boundary particles "
+        f"({time.perf_counter() - t0:.1f}s to generate)")
+    sim = SphGpuSingle(c)
+    del c
+    sim.run(warmup)
+    torch.cuda.synchronize()
+    p0 = sim.pips()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        n = min(10, steps - done)
+        sim.run(n)
+        done += n
+        log(f"  flume {done}/{steps} steps")
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    st = sim.stats()
+    sim.close()
+    pairs = 2.0 * (p0[0] + p0[1])
+    return {"workload": f"wave_flume_12x1x1.2m_dp{dp:g}_mdbc_laminar_sps_piston",
+            "particles": st["np"], "steps": steps, "ms_per_step": 1e3 * el / steps,
+            "pairs_per_step": pairs, "pair_interactions_per_s": pairs * steps / el,
+            "sim_time_s": st["timestep"], "dt_last": st["dt_last"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -185,6 +223,8 @@ def main():
                     help="physical end time of every ensemble case [s]")
     ap.add_argument("--ens-plain-sample", type=int, default=32,
                     help="cases timed on the plain single-case path")
+    ap.add_argument("--flume-steps", type=int, default=0, metavar="N",
+                    help="also time N steps of the full-size wave flume (~58M fluid particles)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a device-to-device copy of BYTES (3 times): the known-size "
                          "kernel that calibrates the HBM byte counters of a profiler")
@@ -268,6 +308,7 @@ def main():
                         "copy calibration (profiles/)"}
     sim.close()
     del sim
+    flume = flume_bench(a.flume_steps, 3) if a.flume_steps > 0 else None
     ens = None
     if a.ensemble > 0:
         ens = ensemble_bench(a.ensemble, a.ens_tend, a.ens_plain_sample)
@@ -293,6 +334,7 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "ensemble": ens,
+        "flume": flume,
         "kernels_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3),
                            "avg_ms": round(v[1] / max(v[0], 1), 5)} for k, v in kt.items()},
     }
