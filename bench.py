@@ -217,6 +217,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
     ap.add_argument("--forces-kernel", choices=["tile", "rows"], default="tile")
+    ap.add_argument("--cellmode", choices=["full", "half"], default="full",
+                    help="cell size 2h (the reference's default) or h (-cellmode:half)")
     ap.add_argument("--ensemble", type=int, default=512, metavar="N",
                     help="cases of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--ens-tend", type=float, default=0.01,
@@ -252,7 +254,8 @@ def main():
     from dualsphysics_amd.sim import SphGpuSingle
 
     t0 = time.perf_counter()
-    c = casemod.dam_break(dp=a.dp)
+    c = casemod.dam_break(dp=a.dp, params=casemod.Parameters(
+        cellmode=2 if a.cellmode == "half" else 1))
     log(f"case: {c.nfluid} fluid + {c.nbound} boundary particles ({time.perf_counter()-t0:.1f}s)")
     sim = SphGpuSingle(c)
     lib = N.load()
@@ -328,6 +331,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"dam_break_3d_dp{a.dp:g}", "n_fluid": c.nfluid,
                    "n_bound": c.nbound, "dp": a.dp, "kernel": "wendland", "forces_kernel": a.forces_kernel,
+                   "cellmode": a.cellmode,
                    "visco": "artificial 0.01", "ddt": "fourtakas_full 0.1",
                    "step": "symplectic", "cfl": 0.2, "pairs_per_step": pairs_step,
                    "sim_time_s": st["timestep"]},

@@ -61,6 +61,8 @@ CASES = {
                                       dict(tdensity=0, tvisco=2, visco=1e-6)),
     "ref_flume_dp0.01_piston_mdbc_verlet": ("flume", 0.01, 10, (1, 5, 10),
                                             dict(tboundary=2, tstep=1, tvisco=3, visco=1e-4)),
+    # Half cells (cells of size h, -cellmode:half).
+    "ref_dambreak_dp0.03_halfcells": ("dam_break", 0.03, 10, (1, 5, 10), dict(cellmode=2)),
     # mDBC2 (SlipMode no-slip) on the dam break's fixed walls.
     "ref_dambreak_dp0.03_mdbc2": ("dam_break", 0.03, 10, (1, 5, 10),
                                   dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4)),
@@ -75,7 +77,7 @@ def run_reference(case, nsteps: int, workdir: Path) -> list[dict]:
     bi4.write_case(case, workdir / "in", "case", timemax=10.0, timeout=10.0)
     cmd = [str(REF_EXE), str(workdir / "in" / "case"), str(workdir / "out"), "-cpu",
            "-ompthreads:8", f"-nsteps:{nsteps}", "-svsteps:1", "-saveposdouble:1", "-stable:1",
-           "-svres:0"]
+           "-svres:0"] + (["-cellmode:half"] if case.params.cellmode == 2 else [])
     r = subprocess.run(cmd, cwd=workdir, capture_output=True, text=True, timeout=3600)
     if r.returncode != 0:
         raise RuntimeError(r.stdout[-4000:] + r.stderr[-2000:])

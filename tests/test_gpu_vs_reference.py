@@ -17,7 +17,8 @@ pytestmark = pytest.mark.gpu
                                   "ref_dambreak_dp0.03_mdbc2", "ref_flume_dp0.01_flap",
                                   "ref_flume_dp0.01_flap_rot_mdbc", "ref_dambreak_dp0.03_ddt1",
                                   "ref_dambreak_dp0.03_noddt_sps",
-                                  "ref_flume_dp0.01_piston_mdbc_verlet"])
+                                  "ref_flume_dp0.01_piston_mdbc_verlet",
+                                  "ref_dambreak_dp0.03_halfcells"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)

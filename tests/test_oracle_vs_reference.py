@@ -26,7 +26,8 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_flume_dp0.01_piston", "ref_flume_dp0.01_piston_mdbc_sps",
             "ref_dambreak_dp0.03_mdbc2", "ref_flume_dp0.01_flap",
             "ref_flume_dp0.01_flap_rot_mdbc", "ref_dambreak_dp0.03_ddt1",
-            "ref_dambreak_dp0.03_noddt_sps", "ref_flume_dp0.01_piston_mdbc_verlet"]
+            "ref_dambreak_dp0.03_noddt_sps", "ref_flume_dp0.01_piston_mdbc_verlet",
+            "ref_dambreak_dp0.03_halfcells"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
