@@ -150,22 +150,24 @@ int dsph_sim_filter_probes(dsph_sim* sim, uint32_t nprobe, const double* h_probe
                            double* h_out) {
   if (!sim || !h_probes || !h_out) return DSPH_E_ARG;
   if (!nprobe) return DSPH_OK;
+  double *dp = nullptr, *dout = nullptr;
+  int rc = DSPH_OK;
   try {
-    double* dp = sim->alloc<double>(3 * size_t(nprobe));
-    double* dout = sim->alloc<double>(23 * size_t(nprobe));
+    dp = sim->alloc<double>(3 * size_t(nprobe));
+    dout = sim->alloc<double>(23 * size_t(nprobe));
     sim->chk(dsph_memcpy_h2d(sim->ctx, dp, h_probes, 24 * size_t(nprobe)), "probes");
     const dsph_divdata d = sim->divdata();
     sim->chk(dsph_filter_probes(sim->ctx, nprobe, dp, hf, &d, sim->MassFluid, sim->posxy,
                                 sim->posz, sim->velrho, sim->code, dout),
              "filter probes");
     sim->chk(dsph_memcpy_d2h(sim->ctx, h_out, dout, 8 * 23 * size_t(nprobe)), "probe moments");
-    dsph_free(sim->ctx, dp);
-    dsph_free(sim->ctx, dout);
   } catch (const Err& e) {
     sim->err = e.what();
-    return e.code;
+    rc = e.code;
   }
-  return DSPH_OK;
+  dsph_free(sim->ctx, dp);
+  dsph_free(sim->ctx, dout);
+  return rc;
 }
 
 int dsph_sim_gauge_points(dsph_sim* sim, uint32_t n, const double* h_points, float* h_vel,
