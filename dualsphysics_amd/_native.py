@@ -18,6 +18,7 @@ VISCO_ARTIFICIAL, VISCO_LAMINARSPS, VISCO_LAMINAR = 1, 2, 3
 DDT_NONE, DDT_DDT, DDT_DDT2, DDT_DDT2FULL = 0, 1, 2, 3
 BC_DBC, BC_MDBC = 1, 2
 SLIP_VEL0, SLIP_NOSLIP = 1, 2
+SHIFT_NONE, SHIFT_NOBOUND, SHIFT_NOFIXED, SHIFT_FULL = 0, 1, 2, 3
 BMODE_DBC, BMODE_MDBC2, BMODE_MDBC2OFF = 0, 1, 2
 OPT_FORCES_KERNEL = 1
 FORCES_TILE, FORCES_ROWS = 0, 1
@@ -54,7 +55,8 @@ class DsphInterParms(C.Structure):
                 ("d_code", C.c_void_p), ("d_spstaurho2", C.c_void_p), ("d_viscdt", C.c_void_p),
                 ("d_ar", C.c_void_p), ("d_ace", C.c_void_p), ("d_sps2strain", C.c_void_p),
                 ("d_velgrad", C.c_void_p), ("d_boundmode", C.c_void_p),
-                ("d_tangenvel", C.c_void_p)]
+                ("d_tangenvel", C.c_void_p), ("shiftmode", C.c_int32),
+                ("d_shiftposfs", C.c_void_p)]
 
 
 class DsphMotion(C.Structure):
@@ -77,7 +79,8 @@ class DsphCase(C.Structure):
         ("maprealposmin", C.c_double * 3), ("maprealposmax", C.c_double * 3)] + [
         (n, C.c_uint32) for n in ("casenp", "casenbound", "casenfixed", "casenmoving",
                                   "casenfloat", "casenfluid")] + [("motion", DsphMotion),
-                                                                  ("slipmode", C.c_int32)]
+                                                                  ("slipmode", C.c_int32),
+        ("shiftmode", C.c_int32), ("shiftcoef", C.c_double), ("shifttfs", C.c_double)]
 
 
 class DsphMlp(C.Structure):
@@ -141,11 +144,11 @@ SIGNATURES = {
     "dsph_redu_max_f32_sync": [_vp, _u, _u, _vp, C.POINTER(_f)],
     "dsph_dt_variable": [_vp, _vp, _d, _d, _d, _d, _vp, _u, _vp],
     "dsph_dt_advance": [_vp, _vp],
-    "dsph_step_symplectic_pre": [_vp, _u, _u, _vp, _vp, _vp, _vp, _d, _vp, _u, _f, _f, _f,
+    "dsph_step_symplectic_pre": [_vp, _u, _u, _vp, _vp, _vp, _vp, _vp, _d, _vp, _u, _f, _f, _f,
                                  C.POINTER(_f), _vp, _vp, _vp, _vp],
-    "dsph_step_symplectic_cor": [_vp, _u, _u, _vp, _vp, _vp, _vp, _d, _d, _vp, _u, _f, _f, _f,
+    "dsph_step_symplectic_cor": [_vp, _u, _u, _vp, _vp, _vp, _vp, _vp, _d, _d, _vp, _u, _f, _f, _f,
                                  C.POINTER(_f), _vp, _vp, _vp, _vp],
-    "dsph_step_verlet": [_vp, _u, _u, _vp, _vp, _vp, _vp, _vp, _d, _d, _f, _f, _f, C.POINTER(_f),
+    "dsph_step_verlet": [_vp, _u, _u, _vp, _vp, _vp, _vp, _vp, _vp, _d, _d, _f, _f, _f, C.POINTER(_f),
                          _vp, _vp, _vp, _vp],
     "dsph_step_pos2": [_vp, _u, _u] + [_vp] * 8,
     "dsph_step_pos": [_vp, _u, _u] + [_vp] * 6,
@@ -164,6 +167,7 @@ SIGNATURES = {
     "dsph_sim_create": [_i, C.POINTER(DsphCase), _u, _vp, _vp, _vp, _vp, C.POINTER(_vp)],
     "dsph_sim_destroy": [_vp],
     "dsph_sim_set_time": [_vp, _d, _d, _u],
+    "dsph_run_shifting": [_vp, _u, _u, _d, _vp, _u, _d, _d, _d, _f, _d, _vp, _vp],
     "dsph_sim_gauge_points": [_vp, _u, _vp, _vp, _vp],
     "dsph_sim_gauge_maxz": [_vp, _u, _vp, _vp, _vp],
     "dsph_sim_gauge_force": [_vp, _u, _u, _vp],

@@ -248,7 +248,9 @@ def write_case(case, directory, name: str, timemax: float | None = None,
         "SavePosDouble": 1, "StepAlgorithm": p.tstep, "VerletSteps": p.verletsteps,
         "Kernel": 2, "ViscoTreatment": p.tvisco, "Visco": p.visco,
         "ViscoBoundFactor": p.viscoboundfactor, "DensityDT": p.tdensity,
-        "DensityDTvalue": p.ddtvalue, "Shifting": 0, "RigidAlgorithm": 1, "FtPause": 0.0,
+        "DensityDTvalue": p.ddtvalue, "Shifting": getattr(p, "shifting", 0),
+        "ShiftCoef": getattr(p, "shiftcoef", -2.0), "ShiftTFS": getattr(p, "shifttfs", 0.0),
+        "RigidAlgorithm": 1, "FtPause": 0.0,
         "CoefDtMin": 0.05, "DtIni": p.dtini, "DtMin": p.dtmin, "DtFixed": 0,
         "DtAllParticles": 0, "TimeMax": timemax, "TimeOut": timeout,
         "RhopOutMin": p.rhopoutmin, "RhopOutMax": p.rhopoutmax,
@@ -489,7 +491,7 @@ def write_part_head(path, case, app: str = "dualsphysics_amd"):
 
 # ---------------------------------------------------------------- case XML reader
 _NOT_BUILT_PARAMS = {
-    "Shifting": "shifting", "RigidAlgorithm": None, "FtPause": None,
+    "RigidAlgorithm": None, "FtPause": None,
     "DtFixed": "fixed dt", "PeriXinc": "periodic boundaries", "PeriYinc": "periodic boundaries",
     "PeriZinc": "periodic boundaries", "PeriX": "periodic boundaries", "PeriY": "periodic boundaries",
     "PeriZ": "periodic boundaries",
@@ -553,7 +555,9 @@ def read_case(directory, name: str):
                       if ct.find("cflnumber") is not None else gf("CFLnumber", 0.2),
                       dtini=gf("DtIni", 0.0), dtmin=gf("DtMin", 0.0),
                       rhopoutmin=gf("RhopOutMin", 700.0), rhopoutmax=gf("RhopOutMax", 1300.0),
-                      timemax=gf("TimeMax", 1.0), timeout=gf("TimeOut", 0.0))
+                      timemax=gf("TimeMax", 1.0), timeout=gf("TimeOut", 0.0),
+                      shifting=gi("Shifting", 0), shiftcoef=gf("ShiftCoef", -2.0),
+                      shifttfs=gf("ShiftTFS", 0.0))
     parts = ex.find("particles")
     blocks = list(parts)
     if any(bk.tag == "floating" for bk in blocks):

@@ -60,6 +60,9 @@ class Parameters:
     ddtvalue: float = 0.1
     tboundary: int = N.BC_DBC
     slipmode: int = N.SLIP_VEL0     # mDBC SlipMode 1 vel=0, 2 no-slip (mDBC2)
+    shifting: int = N.SHIFT_NONE    # Shifting 0 none, 1 no boundaries, 2 no fixed, 3 full
+    shiftcoef: float = -2.0
+    shifttfs: float = 0.0
     cflnumber: float = 0.2
     dtini: float = 0.0
     dtmin: float = 0.0
@@ -117,6 +120,7 @@ class Case:
         c.rhopoutmin, c.rhopoutmax = p.rhopoutmin, p.rhopoutmax
         c.cellmode, c.celldomfixed = p.cellmode, p.celldomfixed
         c.slipmode = p.slipmode
+        c.shiftmode, c.shiftcoef, c.shifttfs = p.shifting, p.shiftcoef, p.shifttfs
         c.casenp, c.casenbound = self.np, self.nbound
         c.casenfixed = self.nbound - self.nmoving
         c.casenmoving, c.casenfloat, c.casenfluid = self.nmoving, 0, self.nfluid

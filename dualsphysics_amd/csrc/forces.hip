@@ -72,10 +72,12 @@ struct Acc {
   float ax, ay, az, ar, visc, delta;
   float s[6];   // two_strain {xx,xy,xz,yy,yz,zz}
   float g[9];   // full velocity gradient (learned closure features)
+  float4 sh;    // shifting: sum m/rho gradW (x = FLT_MAX: cancelled), w = -sum m/rho r.gradW
 };
 
 __device__ inline void acc_init(Acc& a) {
   a.ax = a.ay = a.az = a.ar = a.visc = a.delta = 0;
+  a.sh = make_float4(0, 0, 0, 0);
 #pragma unroll
   for (int k = 0; k < 6; k++) a.s[k] = 0;
 #pragma unroll
@@ -97,11 +99,15 @@ struct P1 {
 // no mass when their ghost node is dry, viscous terms against the tangential
 // velocity (JSphGpu_ker.cu:585-588, 725-730; velocities kept for continuity,
 // MDBC2_KEEPVEL).
-template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false>
+// SHIFT: standard shifting (JSphGpu_ker.cu:677-684): accumulates m2/rho2 gradW
+// and -m2/rho2 r.gradW; a boundary neighbour cancels it (x = FLT_MAX) for
+// shiftmode 1 (no boundaries) and, for shiftmode 2, when it is fixed.
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false>
 __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float4& ps2,
     const float4& vr2, float2 pr2, const P1& q, const float* __restrict__ tau, float viscob,
     float viscof, const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
-    const float* __restrict__ tangenvel = nullptr) {
+    const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
+    const typecode* __restrict__ code = nullptr) {
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
@@ -134,6 +140,15 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
     const float visc_densi = c.ddtkh * cbar * ((vr2.w - vr1.w) - drho) * rr2eta;
     const float delta = visc_densi * dot3 * massp2 * rrho2;
     a.delta = (boundp2 ? FLT_MAX : a.delta - delta);
+  }
+  if (SHIFT && a.sh.x != FLT_MAX) {
+    const float massrho = massp2 * rrho2;
+    const bool noshift = boundp2 && (shiftmode == 1 ||
+                                     (shiftmode == 2 && code_type(code[p2]) == kCodeTypeFixed));
+    a.sh.x = noshift ? FLT_MAX : a.sh.x + massrho * frx;
+    a.sh.y += massrho * fry;
+    a.sh.z += massrho * frz;
+    a.sh.w -= massrho * dot3;
   }
   if (MDBC2 && boundp2) {
     const float* tv = tangenvel + 3 * size_t(p2);
@@ -182,18 +197,20 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
 
 // Reference-structured pair: loads PosCell first and the rest only for a
 // real neighbour.
-template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false>
 __device__ __forceinline__ void pair_body(bool boundp2, unsigned p2, const P1& q,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
     const float2* __restrict__ press, const float* __restrict__ tau, float viscob, float viscof,
     const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
-    const float* __restrict__ tangenvel = nullptr) {
+    const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
+    const typecode* __restrict__ code = nullptr) {
   const float4 ps2 = poscell[p2];
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
-  pair_eval<TVISCO, TDDT, VELGRAD, MDBC2>(boundp2, p2, ps2, velrho[p2], press[p2], q, tau,
-                                          viscob, viscof, c, a, bmode, tangenvel);
+  pair_eval<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT>(boundp2, p2, ps2, velrho[p2], press[p2], q, tau,
+                                                 viscob, viscof, c, a, bmode, tangenvel,
+                                                 shiftmode, code);
 }
 
 template <int TVISCO>
@@ -244,16 +261,18 @@ __device__ inline NgLimits p1_limits(unsigned p1, const unsigned* __restrict__ d
 }
 
 // ------------------------------------------------------------ reference-order rows kernel
-template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false>
 __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float viscof,
     const DivData& dvd, const Cte& c, const unsigned* __restrict__ dcell,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
     const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
     float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
     float* __restrict__ velgrad, const uint8_t* __restrict__ bmode = nullptr,
-    const float* __restrict__ tangenvel = nullptr) {
+    const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
+    const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr) {
   Acc a;
   acc_init(a);
+  if (SHIFT) a.sh = shiftposfs[p1];
   P1 q;
   load_p1<TVISCO>(p1, poscell, velrho, press, tau, q);
   const NgLimits l = p1_limits(p1, dcell, dvd);
@@ -264,26 +283,30 @@ __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float
         unsigned pini, pfin;
         row_range(dvd.begincell, base + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1, pini, pfin);
         for (unsigned p2 = pini; p2 < pfin; p2++)
-          pair_body<TVISCO, TDDT, VELGRAD, MDBC2>(kind == 1, p2, q, poscell, velrho, press, tau,
-                                                  viscob, viscof, c, a, bmode, tangenvel);
+          pair_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT>(kind == 1, p2, q, poscell, velrho,
+                                                         press, tau, viscob, viscof, c, a, bmode,
+                                                         tangenvel, shiftmode, code);
       }
   }
   store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
+  if (SHIFT) shiftposfs[p1] = a.sh;
 }
 
-template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2, bool SHIFT = false>
 __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ velrho, const float2* __restrict__ press,
     const float* __restrict__ tau, float* __restrict__ viscdt, float* __restrict__ ar,
     float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad,
-    const uint8_t* __restrict__ bmode, const float* __restrict__ tangenvel) {
+    const uint8_t* __restrict__ bmode, const float* __restrict__ tangenvel, int shiftmode = 0,
+    const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr) {
   const unsigned p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  fluid_rows_body<TVISCO, TDDT, VELGRAD, MDBC2>(p + pinit, viscob, viscof, dvd, c, dcell,
-                                                poscell, velrho, press, tau, viscdt, ar, ace,
-                                                strain, velgrad, bmode, tangenvel);
+  fluid_rows_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT>(p + pinit, viscob, viscof, dvd, c, dcell,
+                                                       poscell, velrho, press, tau, viscdt, ar,
+                                                       ace, strain, velgrad, bmode, tangenvel,
+                                                       shiftmode, code, shiftposfs);
 }
 
 // ------------------------------------------------------------ cell-row tile kernel
@@ -602,7 +625,16 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
                   const float2* press, const float4* posf) {
   KTimer kt(ctx, "k_forces_fluid");
   const unsigned npf = t->np - t->npb;
-  if (t->d_boundmode && ctx->forces_kernel == 0 && !VELGRAD) {
+  if (t->d_shiftposfs) {
+    // Shifting: the one-lane-per-particle kernel (mDBC2 and the velocity
+    // gradient are refused with shifting by dsph_interaction_forces).
+    k_forces_fluid_rows<TVISCO, TDDT, false, false, true><<<grid_for(npf, kBsForces), kBsForces,
+                                                           0, ctx->stream>>>(
+        npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
+        (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
+        t->d_sps2strain, nullptr, nullptr, nullptr, t->shiftmode, t->d_code,
+        (float4*)t->d_shiftposfs);
+  } else if (t->d_boundmode && ctx->forces_kernel == 0 && !VELGRAD) {
     // mDBC2 boundaries in the tile kernel.
     k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, true><<<grid_for(npf, 64), 64, 0,
                                                                  ctx->stream>>>(
@@ -821,6 +853,10 @@ int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
                         t->d_ace && t->divdata.d_beginendcell,
                "dsph_interaction_forces: null array");
   DSPH_REQUIRE(ctx, !t->d_boundmode || t->d_tangenvel, "mDBC2 needs boundmode and tangenvel");
+  DSPH_REQUIRE(ctx, !t->d_shiftposfs || (t->shiftmode >= 1 && t->shiftmode <= 3 && t->d_code),
+               "shifting needs shiftmode 1..3 and the particle codes");
+  DSPH_REQUIRE(ctx, !t->d_shiftposfs || (!t->d_boundmode && !t->d_velgrad),
+               "shifting with mDBC2 or the velocity gradient is not built");
   const bool sps = t->tvisco == DSPH_VISCO_LAMINARSPS;
   DSPH_REQUIRE(ctx, !sps || (t->d_spstaurho2 && t->d_sps2strain),
                "Laminar+SPS needs spstaurho2 and sps2strain");

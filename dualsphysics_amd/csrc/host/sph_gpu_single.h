@@ -75,6 +75,8 @@ struct dsph_sim {
   double Cs0 = 0, DtIni = 0, DtMin = 0, CFLnumber = 0.2;
   int TStep = 2, VerletSteps = 40, TVisco = 1, TDensity = 0, TBoundary = 1, ScellDiv = 1;
   int SlipMode = DSPH_SLIP_VEL0;
+  int ShiftMode = DSPH_SHIFT_NONE;
+  double ShiftCoef = -2.0, ShiftTFS = 0.0;
   bool CellDomFixed = false;
   double MapRealPosMin[3], MapRealPosMax[3], MapRealSize[3];
   double Map_PosMin[3], Map_PosMax[3], Map_Size[3];
@@ -99,6 +101,9 @@ struct dsph_sim {
   // rewritten by every correction (JSphGpuSingle.cpp:700-707).
   uint8_t* boundmode = nullptr;
   float* tangenvel = nullptr;
+  // Shifting: float4 per particle (sums, then the displacement), fluid part
+  // cleared before every interaction (JSphShifting::InitGpu without zones).
+  float* shiftposfs = nullptr;
   bool pre_active = false;
   // device scalars: red[0..2] (velmax^2, viscdtmax, acemax^2), dt slots, flags
   float* d_red = nullptr;
@@ -145,7 +150,7 @@ struct dsph_sim {
     void* ptrs[] = {idp, dcell, idp2, dcell2, code, code2, posxy, posz, posxy2, posz2, velrho,
                     velrho2, poscell, posxypre, poszpre, posxypre2, poszpre2, velrhopre,
                     velrhopre2, velrhom1, velrhom1b, ace, ar, viscdt, movxy, movz, sps2strain,
-                    spstau, spstau2, boundnor, boundnor2, boundmode, tangenvel, d_red, d_dt,
+                    spstau, spstau2, boundnor, boundnor2, boundmode, tangenvel, shiftposfs, d_red, d_dt,
                     d_flags, cellpart,
                     sortpart, begincell, velgrad, closure_nu};
     for (void* p : ptrs) dsph_free(ctx, p);
@@ -175,6 +180,13 @@ struct dsph_sim {
       throw Err(DSPH_E_ARG, "SlipMode: only 1 (vel=0) and 2 (no-slip) are built");
     if (TBoundary == DSPH_BC_MDBC && SlipMode == DSPH_SLIP_NOSLIP && c.casenmoving)
       throw Err(DSPH_E_ARG, "mDBC no-slip is built for fixed boundaries only");
+    ShiftMode = c.shiftmode;
+    ShiftCoef = c.shiftcoef ? c.shiftcoef : -2.0;
+    ShiftTFS = c.shifttfs;
+    if (ShiftMode < DSPH_SHIFT_NONE || ShiftMode > DSPH_SHIFT_FULL)
+      throw Err(DSPH_E_ARG, "Shifting: mode 0..3");
+    if (ShiftMode && TBoundary == DSPH_BC_MDBC && SlipMode == DSPH_SLIP_NOSLIP)
+      throw Err(DSPH_E_ARG, "shifting with mDBC no-slip is not built");
     Visco = float(c.visco);
     ViscoBoundFactor = float(c.viscoboundfactor);
     RhopOutMin = float(c.rhopoutmin);
@@ -295,6 +307,7 @@ struct dsph_sim {
       sps2strain = alloc<float>(6 * n);
       spstau = alloc<float>(6 * n); spstau2 = alloc<float>(6 * n);
     }
+    if (ShiftMode) shiftposfs = alloc<float>(4 * n);
     if (TBoundary == DSPH_BC_MDBC) {
       boundnor = alloc<float>(3 * n); boundnor2 = alloc<float>(3 * n);
       if (SlipMode == DSPH_SLIP_NOSLIP) {
@@ -506,6 +519,9 @@ struct dsph_sim {
     chk(dsph_memset_async(ctx, ace, 0, 12 * size_t(Np)), "memset ace");
     if (sps2strain)
       chk(dsph_memset_async(ctx, sps2strain + 6 * size_t(Npb), 0, 24 * size_t(Np - Npb)), "memset strain");
+    if (shiftposfs)
+      chk(dsph_memset_async(ctx, shiftposfs + 4 * size_t(Npb), 0, 16 * size_t(Np - Npb)),
+          "memset shiftposfs");
     chk(dsph_velmax2(ctx, Np - Npb, Npb, velrho, d_red + 0), "velmax");
   }
 
@@ -543,6 +559,8 @@ struct dsph_sim {
     t.d_velgrad = UseClosure ? velgrad : nullptr;
     t.d_boundmode = boundmode;
     t.d_tangenvel = tangenvel;
+    t.shiftmode = ShiftMode;
+    t.d_shiftposfs = shiftposfs;
     chk(dsph_interaction_forces(ctx, &t), "interaction");
     if (TVisco == DSPH_VISCO_LAMINARSPS) {
       if (UseClosure)
@@ -580,11 +598,21 @@ struct dsph_sim {
     chk(dsph_memcpy_h2d(ctx, d_dt + slot, &v, 8), "dt upload");
   }
 
+  // ---- JSphGpu::RunShifting (JSphGpu.cpp:1153-1157) over [t, t + dt*scale]:
+  // the shifting displacement of the fluid from the interaction's sums.
+  void run_shifting(double dt, double scale) {
+    if (!shiftposfs) return;
+    chk(dsph_run_shifting(ctx, Np - Npb, Npb, dt * scale, device_dt ? d_dt : nullptr, 2, scale,
+                          double(float(ShiftCoef)), double(KernelH), float(ShiftTFS), cs.dp, velrho,
+                          shiftposfs),
+        "shifting");
+  }
+
   // ---- JSphGpu::ComputeSymplecticPre / Corr (JSphGpu.cpp:996-1097)
   void symplectic_pre(double dt) {
     swp(posxypre, posxy); swp(poszpre, posz); swp(velrhopre, velrho);
     pre_active = true;
-    chk(dsph_step_symplectic_pre(ctx, Np, Npb, velrhopre, boundmode, ar, ace, dt * .5,
+    chk(dsph_step_symplectic_pre(ctx, Np, Npb, velrhopre, boundmode, ar, ace, shiftposfs, dt * .5,
                                  device_dt ? d_dt : nullptr, 2, RhopZero, RhopOutMin, RhopOutMax,
                                  Gravity, code, movxy, movz, velrho),
         "symplectic pre");
@@ -593,7 +621,7 @@ struct dsph_sim {
     chk(dsph_memcpy_d2d_async(ctx, posz, poszpre, 8 * size_t(Npb)), "copy bound z");
   }
   void symplectic_cor(double dt) {
-    chk(dsph_step_symplectic_cor(ctx, Np, Npb, velrhopre, boundmode, ar, ace, dt * .5, dt,
+    chk(dsph_step_symplectic_cor(ctx, Np, Npb, velrhopre, boundmode, ar, ace, shiftposfs, dt * .5, dt,
                                  device_dt ? d_dt : nullptr, 2, RhopZero, RhopOutMin, RhopOutMax,
                                  Gravity, code, movxy, movz, velrho),
         "symplectic cor");
@@ -640,12 +668,14 @@ struct dsph_sim {
     pre_interaction();
     interaction();
     const double dt_p = dt_variable(0);
+    run_shifting(dt, 0.5);
     symplectic_pre(dt);
     run_cell_divide();
     mdbc(true);
     pre_interaction();
     interaction();
     const double dt_c = dt_variable(1);
+    run_shifting(dt, 1.0);
     symplectic_cor(dt);
     run_motion(dt);
     if (device_dt) {
@@ -665,13 +695,14 @@ struct dsph_sim {
     interaction();
     device_dt = false;
     const double dt = dt_variable(0);
+    run_shifting(dt, 1.0);
     VerletStep++;
     if (VerletStep < unsigned(VerletSteps)) {
-      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrhom1, boundmode, ar, ace, dt, dt + dt, RhopZero,
+      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrhom1, boundmode, ar, ace, shiftposfs, dt, dt + dt, RhopZero,
                            RhopOutMin, RhopOutMax, Gravity, code, movxy, movz, velrhom1),
           "verlet");
     } else {
-      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrho, boundmode, ar, ace, dt, dt, RhopZero, RhopOutMin,
+      chk(dsph_step_verlet(ctx, Np, Npb, velrho, velrho, boundmode, ar, ace, shiftposfs, dt, dt, RhopZero, RhopOutMin,
                            RhopOutMax, Gravity, code, movxy, movz, velrhom1),
           "verlet euler");
       VerletStep = 0;

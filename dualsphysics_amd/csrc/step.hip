@@ -158,9 +158,19 @@ __device__ inline bool mdbc2_keep(unsigned p, unsigned npb, const uint8_t* __res
   return boundmode && p < npb && boundmode[p] >= DSPH_BMODE_MDBC2;
 }
 
+// Shifting displacement added to a fluid particle's displacement
+// (JSphGpuSimple_ker.cu:164-168, 303-308, 460-465).
+__device__ inline void add_shift(unsigned p, const float4* __restrict__ shiftposfs,
+                                 double2* __restrict__ movxy, double* __restrict__ movz) {
+  const float4 s = shiftposfs[p];
+  const double2 m = movxy[p];
+  movxy[p] = make_double2(m.x + double(s.x), m.y + double(s.y));
+  movz[p] = movz[p] + double(s.z);
+}
+
 __global__ void __launch_bounds__(kBs) k_sym_pre(unsigned n, unsigned npb,
     const float4* __restrict__ velrhopre, const uint8_t* __restrict__ boundmode,
-    const float* __restrict__ ar,
+    const float4* __restrict__ shiftposfs, const float* __restrict__ ar,
     const float* __restrict__ ace, double dtm_h, const double* __restrict__ d_dt, unsigned slot,
     float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
     typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
@@ -173,6 +183,7 @@ __global__ void __launch_bounds__(kBs) k_sym_pre(unsigned n, unsigned npb,
   }
   sym_pre_one(p, p < npb, dev_dt(dtm_h, d_dt, slot, 0.5), velrhopre, ar, ace, rhopzero,
               rhopoutmin, rhopoutmax, gravity, code, movxy, movz, velrho);
+  if (shiftposfs && p >= npb) add_shift(p, shiftposfs, movxy, movz);
 }
 
 __device__ inline void sym_cor_one(unsigned p, bool bound, double dt, double dtm,
@@ -208,7 +219,7 @@ __device__ inline void sym_cor_one(unsigned p, bool bound, double dt, double dtm
 
 __global__ void __launch_bounds__(kBs) k_sym_cor(unsigned n, unsigned npb,
     const float4* __restrict__ velrhopre, const uint8_t* __restrict__ boundmode,
-    const float* __restrict__ ar,
+    const float4* __restrict__ shiftposfs, const float* __restrict__ ar,
     const float* __restrict__ ace, double dtm_h, double dt_h, const double* __restrict__ d_dt,
     unsigned slot, float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
     typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
@@ -222,11 +233,12 @@ __global__ void __launch_bounds__(kBs) k_sym_cor(unsigned n, unsigned npb,
   sym_cor_one(p, p < npb, dev_dt(dt_h, d_dt, slot, 1.0), dev_dt(dtm_h, d_dt, slot, 0.5),
               velrhopre, ar, ace, rhopzero, rhopoutmin, rhopoutmax, gravity, code, movxy, movz,
               velrho);
+  if (shiftposfs && p >= npb) add_shift(p, shiftposfs, movxy, movz);
 }
 
 __global__ void __launch_bounds__(kBs) k_verlet(unsigned n, unsigned npb,
     const float4* __restrict__ velrho1, const float4* __restrict__ velrho2,
-    const uint8_t* __restrict__ boundmode,
+    const uint8_t* __restrict__ boundmode, const float4* __restrict__ shiftposfs,
     const float* __restrict__ ar, const float* __restrict__ ace, double dt, double dt205,
     double dt2, float rhopzero, float rhopoutmin, float rhopoutmax, float3 gravity,
     typecode* __restrict__ code, double2* __restrict__ movxy, double* __restrict__ movz,
@@ -250,9 +262,13 @@ __global__ void __launch_bounds__(kBs) k_verlet(unsigned n, unsigned npb,
   const double ax = double(ace[3 * size_t(p)]) + gravity.x;
   const double ay = double(ace[3 * size_t(p) + 1]) + gravity.y;
   const double az = double(ace[3 * size_t(p) + 2]) + gravity.z;
-  const double dx = double(v1.x) * dt + ax * dt205;
-  const double dy = double(v1.y) * dt + ay * dt205;
-  const double dz = double(v1.z) * dt + az * dt205;
+  double dx = double(v1.x) * dt + ax * dt205;
+  double dy = double(v1.y) * dt + ay * dt205;
+  double dz = double(v1.z) * dt + az * dt205;
+  if (shiftposfs) {
+    const float4 s = shiftposfs[p];
+    dx += double(s.x); dy += double(s.y); dz += double(s.z);
+  }
   const bool outrho = (rhonew < rhopoutmin || rhonew > rhopoutmax);
   const float4 nw = make_float4(float(double(v2.x) + ax * dt2), float(double(v2.y) + ay * dt2),
                                 float(double(v2.z) + az * dt2), rhonew);
@@ -408,6 +424,39 @@ __global__ void __launch_bounds__(kBs) k_move_mat_bound(unsigned npb, unsigned i
     const double gz2 = gx * m.a[8] + gy * m.a[9] + gz * m.a[10] + m.a[11];
     bn[0] = float(gx2 - x2); bn[1] = float(gy2 - y2); bn[2] = float(gz2 - z2);
   }
+}
+
+// Shifting displacement of fluid particles (KerRunShifting,
+// JSphShifting_ker.cu:123-151): umagn = dt*coef*h*|v| (FP64), scaled by the
+// free-surface factor (w - tfs)/(3 - tfs) above the threshold and 0 below,
+// shift = sum*umagn clamped to +-maxdist; a cancelled particle gets 0.
+__global__ void __launch_bounds__(kBs) k_run_shifting(unsigned n, unsigned pini, double dt_h,
+    const double* __restrict__ d_dt, unsigned slot, double scale, double shiftcoef,
+    double kernelh, float tfs, float coeftfs, float maxdist, const float4* __restrict__ velrho,
+    float4* __restrict__ shiftposfs) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned p = i + pini;
+  const double dt = d_dt ? d_dt[slot] * scale : dt_h;
+  const double coefumagn = dt * shiftcoef * kernelh;
+  float4 rs = shiftposfs[p];
+  if (rs.x != FLT_MAX) {
+    const float4 v = velrho[p];
+    const double vx = v.x, vy = v.y, vz = v.z;
+    double umagn = coefumagn * sqrt(vx * vx + vy * vy + vz * vz);
+    if (tfs) {
+      if (rs.w < tfs) umagn = 0;
+      else umagn *= (double(rs.w) - tfs) / coeftfs;
+    }
+    const float sx = float(double(rs.x) * umagn), sy = float(double(rs.y) * umagn),
+                sz = float(double(rs.z) * umagn);
+    rs.x = (fabsf(sx) < maxdist ? sx : (sx >= 0 ? maxdist : -maxdist));
+    rs.y = (fabsf(sy) < maxdist ? sy : (sy >= 0 ? maxdist : -maxdist));
+    rs.z = (fabsf(sz) < maxdist ? sz : (sz >= 0 ? maxdist : -maxdist));
+  } else {
+    rs = make_float4(0, 0, 0, 0);
+  }
+  shiftposfs[p] = rs;
 }
 
 __global__ void __launch_bounds__(kBs) k_step_pos2(unsigned n, unsigned pini, PosCte c,
@@ -695,7 +744,8 @@ int dsph_dt_advance(dsph_ctx* ctx, double* d_dt) {
 }
 
 int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_velrhopre,
-                             const uint8_t* d_boundmode, const float* d_ar, const float* d_ace, double dtm,
+                             const uint8_t* d_boundmode, const float* d_ar, const float* d_ace,
+                             const float* d_shiftposfs, double dtm,
                              const double* d_dt, uint32_t dt_slot, float rhopzero,
                              float rhopoutmin, float rhopoutmax, const float gravity[3],
                              uint16_t* d_code, double* d_movxy, double* d_movz,
@@ -706,7 +756,8 @@ int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb, const flo
                "dsph_step_symplectic_pre: null array");
   KTimer kt(ctx, "k_sym_pre");
   k_sym_pre<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
-      np, npb, (const float4*)d_velrhopre, d_boundmode, d_ar, d_ace, dtm, d_dt, dt_slot,
+      np, npb, (const float4*)d_velrhopre, d_boundmode, (const float4*)d_shiftposfs, d_ar, d_ace,
+      dtm, d_dt, dt_slot,
       rhopzero, rhopoutmin,
       rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code, (double2*)d_movxy,
       d_movz, (float4*)d_velrho);
@@ -715,7 +766,8 @@ int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb, const flo
 }
 
 int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_velrhopre,
-                             const uint8_t* d_boundmode, const float* d_ar, const float* d_ace, double dtm, double dt,
+                             const uint8_t* d_boundmode, const float* d_ar, const float* d_ace,
+                             const float* d_shiftposfs, double dtm, double dt,
                              const double* d_dt, uint32_t dt_slot, float rhopzero,
                              float rhopoutmin, float rhopoutmax, const float gravity[3],
                              uint16_t* d_code, double* d_movxy, double* d_movz,
@@ -726,7 +778,8 @@ int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb, const flo
                "dsph_step_symplectic_cor: null array");
   KTimer kt(ctx, "k_sym_cor");
   k_sym_cor<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
-      np, npb, (const float4*)d_velrhopre, d_boundmode, d_ar, d_ace, dtm, dt, d_dt, dt_slot,
+      np, npb, (const float4*)d_velrhopre, d_boundmode, (const float4*)d_shiftposfs, d_ar, d_ace,
+      dtm, dt, d_dt, dt_slot,
       rhopzero,
       rhopoutmin, rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code,
       (double2*)d_movxy, d_movz, (float4*)d_velrho);
@@ -735,7 +788,8 @@ int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb, const flo
 }
 
 int dsph_step_verlet(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_velrho1,
-                     const float* d_velrho2, const uint8_t* d_boundmode, const float* d_ar, const float* d_ace, double dt,
+                     const float* d_velrho2, const uint8_t* d_boundmode, const float* d_ar,
+                     const float* d_ace, const float* d_shiftposfs, double dt,
                      double dt2, float rhopzero, float rhopoutmin, float rhopoutmax,
                      const float gravity[3], uint16_t* d_code, double* d_movxy, double* d_movz,
                      float* d_velrhonew) {
@@ -747,11 +801,26 @@ int dsph_step_verlet(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_ve
   const double dt205 = 0.5 * dt * dt;
   KTimer kt(ctx, "k_verlet");
   k_verlet<<<grid_for(np, kBs), kBs, 0, ctx->stream>>>(
-      np, npb, (const float4*)d_velrho1, (const float4*)d_velrho2, d_boundmode, d_ar, d_ace, dt,
+      np, npb, (const float4*)d_velrho1, (const float4*)d_velrho2, d_boundmode,
+      (const float4*)d_shiftposfs, d_ar, d_ace, dt,
       dt205, dt2,
       rhopzero, rhopoutmin, rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code,
       (double2*)d_movxy, d_movz, (float4*)d_velrhonew);
   DSPH_LAUNCHED(ctx, "k_verlet");
+  return DSPH_OK;
+}
+
+int dsph_run_shifting(dsph_ctx* ctx, uint32_t n, uint32_t pini, double dt, const double* d_dt,
+                      uint32_t dt_slot, double dt_scale, double shiftcoef, double kernelh,
+                      float shifttfs, double dp, const float* d_velrho, float* d_shiftposfs) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!n) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_velrho && d_shiftposfs, "dsph_run_shifting: null array");
+  KTimer kt(ctx, "k_run_shifting");
+  k_run_shifting<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(
+      n, pini, dt, d_dt, dt_slot, dt_scale, shiftcoef, kernelh, shifttfs, 3.0f - shifttfs,
+      float(dp * 0.1), (const float4*)d_velrho, (float4*)d_shiftposfs);
+  DSPH_LAUNCHED(ctx, "k_run_shifting");
   return DSPH_OK;
 }
 

@@ -64,6 +64,10 @@ extern "C" {
 #define DSPH_BC_MDBC          2
 #define DSPH_SLIP_VEL0        1   /* TpSlipMode: mDBC, boundary velocity 0 */
 #define DSPH_SLIP_NOSLIP      2   /* mDBC2 no-slip (MDBC2_Std) */
+#define DSPH_SHIFT_NONE       0   /* TpShifting */
+#define DSPH_SHIFT_NOBOUND    1   /* no shifting near any boundary */
+#define DSPH_SHIFT_NOFIXED    2   /* no shifting near fixed boundaries */
+#define DSPH_SHIFT_FULL       3
 #define DSPH_BMODE_DBC        0   /* boundmode of a boundary particle */
 #define DSPH_BMODE_MDBC2      1
 #define DSPH_BMODE_MDBC2OFF   2   /* ghost node not submerged: mass off */
@@ -141,6 +145,9 @@ typedef struct dsph_inter_parms {
                                     closure features) or NULL */
   const uint8_t* d_boundmode;    /* mDBC2: DSPH_BMODE_* of [0,npb), else NULL */
   const float*   d_tangenvel;    /* mDBC2: float3[npb] tangential velocity */
+  int32_t shiftmode;             /* DSPH_SHIFT_* (with d_shiftposfs) */
+  float* d_shiftposfs;           /* shifting: float4[np] (in/out; fluid part
+                                    cleared before the interaction), else NULL */
 } dsph_inter_parms;
 
 /* ---------------------------------------------------------------- context */
@@ -391,6 +398,17 @@ int dsph_gauge_force(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin, uint32_t cou
                      const double* d_posz, const uint16_t* d_code,
                      const uint32_t* d_idp, const float* d_velrho, float* d_ace);
 
+/* Replaces cushift::RunShifting (JSphShifting_ker.cu:123-165; host
+ * JSphShifting::RunGpu JSphShifting.cpp:463-470) for fluid [pini,pini+n):
+ * umagn = dt*shiftcoef*kernelh*|v| (times (w-tfs)/(3-tfs) above the
+ * free-surface threshold tfs, 0 below; tfs = 0 disables it), shift =
+ * shiftposfs.xyz*umagn clamped to +-0.1 dp; a cancelled particle (x =
+ * FLT_MAX) gets 0. dt = d_dt[dt_slot]*dt_scale when d_dt is not NULL. */
+int dsph_run_shifting(dsph_ctx* ctx, uint32_t n, uint32_t pini, double dt,
+                      const double* d_dt, uint32_t dt_slot, double dt_scale,
+                      double shiftcoef, double kernelh, float shifttfs, double dp,
+                      const float* d_velrho, float* d_shiftposfs);
+
 /* --------------------------------------------------- reductions and dt */
 
 /* Replaces cusph::ComputeVelMod + ReduMaxFloat (JSphGpu_ker.h:281,290):
@@ -425,7 +443,10 @@ int dsph_dt_advance(dsph_ctx* ctx, double* d_dt);
 
 /* ------------------------------------------------- integration (cusphs) */
 
-/* d_boundmode (mDBC2, else NULL): boundary particles with mode >=
+/* d_shiftposfs (shifting, else NULL): the shifting displacement
+ * (dsph_run_shifting) is added to the fluid displacement
+ * (JSphGpuSimple_ker.cu:164-168, 303-308, 460-465).
+ * d_boundmode (mDBC2, else NULL): boundary particles with mode >=
  * DSPH_BMODE_MDBC2 keep velrhopre (Verlet: velrho2) instead of the DBC
  * density update (JSphGpuSimple_ker.cu:142,276,406-412). */
 /* Replaces cusphs::ComputeStepSymplecticPre (JSphGpuSimple_ker.h:51) for
@@ -434,7 +455,7 @@ int dsph_dt_advance(dsph_ctx* ctx, double* d_dt);
 int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb,
                              const float* d_velrhopre, const uint8_t* d_boundmode,
                              const float* d_ar,
-                             const float* d_ace, double dtm,
+                             const float* d_ace, const float* d_shiftposfs, double dtm,
                              const double* d_dt, uint32_t dt_slot,
                              float rhopzero, float rhopoutmin, float rhopoutmax,
                              const float gravity[3], uint16_t* d_code,
@@ -444,7 +465,8 @@ int dsph_step_symplectic_pre(dsph_ctx* ctx, uint32_t np, uint32_t npb,
 int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb,
                              const float* d_velrhopre, const uint8_t* d_boundmode,
                              const float* d_ar,
-                             const float* d_ace, double dtm, double dt,
+                             const float* d_ace, const float* d_shiftposfs, double dtm,
+                             double dt,
                              const double* d_dt, uint32_t dt_slot,
                              float rhopzero, float rhopoutmin, float rhopoutmax,
                              const float gravity[3], uint16_t* d_code,
@@ -453,7 +475,8 @@ int dsph_step_symplectic_cor(dsph_ctx* ctx, uint32_t np, uint32_t npb,
 int dsph_step_verlet(dsph_ctx* ctx, uint32_t np, uint32_t npb,
                      const float* d_velrho1, const float* d_velrho2,
                      const uint8_t* d_boundmode,
-                     const float* d_ar, const float* d_ace, double dt, double dt2,
+                     const float* d_ar, const float* d_ace, const float* d_shiftposfs,
+                     double dt, double dt2,
                      float rhopzero, float rhopoutmin, float rhopoutmax,
                      const float gravity[3], uint16_t* d_code,
                      double* d_movxy, double* d_movz, float* d_velrhonew);
@@ -576,6 +599,8 @@ typedef struct dsph_case {
   dsph_motion motion;            /* the moving object (ids casenfixed.. +casenmoving) */
   int32_t slipmode;              /* mDBC SlipMode (JSph.cpp:706-716): DSPH_SLIP_VEL0 or
                                     DSPH_SLIP_NOSLIP (fixed boundaries only) */
+  int32_t shiftmode;             /* Shifting (JSph.cpp:744-760): DSPH_SHIFT_* */
+  double shiftcoef, shifttfs;    /* ShiftCoef (-2 default), ShiftTFS (0: off) */
 } dsph_case;
 
 /* Per-step statistics, mirror of the Run.out/Run.csv key numbers. */

@@ -217,6 +217,10 @@ struct Opts {
   // 2 mDBC2 switched off) and its tangential velocity; NULL without mDBC2.
   const unsigned char* bmode = nullptr;
   const float* tangenvel = nullptr;
+  // Shifting (JSphCpu.cpp:943-950): mode 1 no boundaries, 2 no fixed
+  // boundaries, 3 full; codes of the neighbours; 0 = off.
+  int shiftmode = 0;
+  const typecode* code = nullptr;
 };
 
 // Relative position p1-p2 in the chosen semantics.
@@ -278,6 +282,7 @@ struct FAcc {
   R s[6] = {0, 0, 0, 0, 0, 0};
   R g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   bool delta_off = false;
+  float sh[4] = {0, 0, 0, 0};  // shifting sums in FP32 (x = FLT_MAX: cancelled)
 };
 
 template <typename R>
@@ -326,6 +331,16 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
         const R delta = visc_densi * dot3 * massp2 / rho2;
         if (boundp2) a.delta_off = true;
         else a.delta -= delta;
+      }
+      if (o.shiftmode && a.sh[0] != FLT_MAX) {
+        const float massrho = float(massp2) / v2[3];
+        const bool noshift = boundp2 && (o.shiftmode == 1 ||
+                                         (o.shiftmode == 2 && (o.code[p2] & kTypeMask) == 0));
+        const float fx = float(frx), fy = float(fry), fz = float(frz);
+        a.sh[0] = noshift ? FLT_MAX : a.sh[0] + massrho * fx;
+        a.sh[1] += massrho * fy;
+        a.sh[2] += massrho * fz;
+        a.sh[3] -= massrho * float(dot3);
       }
       if (mdbc2) {
         // Viscous terms against the no-slip tangential velocity
@@ -386,7 +401,7 @@ template <typename R>
 void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np, unsigned npb,
                  unsigned npbok, const unsigned* dcell, const float* ps, const double* pos,
                  const float* velrho, const float* tau, float* ace, float* ar, float* viscdt,
-                 float* strain, float* velgrad) {
+                 float* strain, float* velgrad, float* shiftposfs = nullptr) {
   std::vector<R> press(np);
 #pragma omp parallel for schedule(static)
   for (long p = 0; p < long(np); p++) press[p] = eos<R>(o.sem, c, velrho[4 * size_t(p) + 3]);
@@ -410,8 +425,13 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
       // partial terms are summed and added to ar after both passes
       // (AddDelta, JSphCpuSingle.cpp:726).
       FAcc<R> a1, a2;
+      if (shiftposfs) for (int k = 0; k < 4; k++) a1.sh[k] = shiftposfs[4 * size_t(p1) + k];
       fluid_rows<R>(o, c, false, rf, nf, p1, ps, pos, velrho, press.data(), tau, a1);
+      // The CPU build stores the shifting sums after the fluid pass and
+      // continues from them in the boundary pass (JSphCpu.cpp:769, 1101).
+      for (int k = 0; k < 4; k++) a2.sh[k] = a1.sh[k];
       fluid_rows<R>(o, c, true, rb, nb, p1, ps, pos, velrho, press.data(), tau, a2);
+      if (shiftposfs) for (int k = 0; k < 4; k++) shiftposfs[4 * size_t(p1) + k] = a2.sh[k];
       oax = R(float(a1.ax)) + a2.ax; oay = R(float(a1.ay)) + a2.ay; oaz = R(float(a1.az)) + a2.az;
       oar = R(float(R(float(a1.ar)) + a2.ar));
       if (o.tddt && !a1.delta_off && !a2.delta_off)
@@ -421,8 +441,10 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
       for (int k = 0; k < 9; k++) og[k] = a1.g[k] + a2.g[k];
     } else {
       FAcc<R> a;
+      if (shiftposfs) for (int k = 0; k < 4; k++) a.sh[k] = shiftposfs[4 * size_t(p1) + k];
       fluid_rows<R>(o, c, false, rf, nf, p1, ps, pos, velrho, press.data(), tau, a);
       fluid_rows<R>(o, c, true, rb, nb, p1, ps, pos, velrho, press.data(), tau, a);
+      if (shiftposfs) for (int k = 0; k < 4; k++) shiftposfs[4 * size_t(p1) + k] = a.sh[k];
       oax = a.ax; oay = a.ay; oaz = a.az;
       oar = a.ar + (o.tddt && !a.delta_off ? a.delta : R(0));
       ovisc = a.visc;
@@ -480,6 +502,12 @@ void orc_interaction_mdbc2(int, int, int, int, float, float, const orc_cte*, con
                            const double*, const float*, const float*, float*, float*, float*,
                            float*, float*, const unsigned char*, const float*);
 
+void orc_interaction_ext(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                         unsigned, unsigned, unsigned, const unsigned*, const float*,
+                         const double*, const float*, const float*, float*, float*, float*,
+                         float*, float*, const unsigned char*, const float*, int,
+                         const typecode*, float*);
+
 // Force interaction (see header). prec: 0 float, 1 double. sem: SEM_*.
 // ace/ar/viscdt (and strain) accumulate as in the reference.
 void orc_interaction(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
@@ -500,13 +528,29 @@ void orc_interaction_mdbc2(int prec, int sem, int tvisco, int tddt, float viscob
                            const double* pos, const float* velrho, const float* tau, float* ace,
                            float* ar, float* viscdt, float* strain, float* velgrad,
                            const unsigned char* bmode, const float* tangenvel) {
-  Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof, bmode, tangenvel};
+  orc_interaction_ext(prec, sem, tvisco, tddt, viscob, viscof, c, d, np, npb, npbok, dcell,
+                      poscell, pos, velrho, tau, ace, ar, viscdt, strain, velgrad, bmode,
+                      tangenvel, 0, nullptr, nullptr);
+}
+
+// Force interaction with every option of this path: mDBC2 modes (bmode,
+// tangenvel or NULL) and shifting (shiftmode 1..3, the particle codes and
+// the float4 sums per particle, fluid part cleared by the caller; 0 = off).
+void orc_interaction_ext(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
+                         const orc_cte* c, const orc_div* d, unsigned np, unsigned npb,
+                         unsigned npbok, const unsigned* dcell, const float* poscell,
+                         const double* pos, const float* velrho, const float* tau, float* ace,
+                         float* ar, float* viscdt, float* strain, float* velgrad,
+                         const unsigned char* bmode, const float* tangenvel, int shiftmode,
+                         const typecode* code, float* shiftposfs) {
+  Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof, bmode, tangenvel,
+         shiftposfs ? shiftmode : 0, code};
   if (prec == 0)
     interaction<float>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
-                       viscdt, strain, velgrad);
+                       viscdt, strain, velgrad, o.shiftmode ? shiftposfs : nullptr);
   else
     interaction<double>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
-                        viscdt, strain, velgrad);
+                        viscdt, strain, velgrad, o.shiftmode ? shiftposfs : nullptr);
 }
 
 }  // extern "C"
@@ -1070,6 +1114,40 @@ void orc_mdbc2(int prec, int sem, const orc_cte* c, const orc_div* d, unsigned n
   else
     mdbc2_correction<float, float>(sem, *c, *d, n, pos, poscell, code, boundnor, gravity,
                                    motionvel, motionace, cs0, dp, velrho, bmode, tangenvel);
+}
+// Shifting displacement of fluid [pini, pini+n) (JSphShifting::RunCpu,
+// JSphShifting.cpp:390-420; GPU KerRunShifting JSphShifting_ker.cu:123-151):
+// umagn = dt*coef*h*|v| in FP64, free-surface factor (w - tfs)/(3 - tfs)
+// above tfs (0 below; tfs = 0: off), shift = sums*umagn clamped to
+// +-float(dp*0.1); cancelled particles (x = FLT_MAX) get 0.
+void orc_run_shifting(unsigned n, unsigned pini, double dt, float shiftcoef, float kernelh,
+                      float tfs, double dp, const float* velrho, float* shiftposfs) {
+  const double coefumagn = dt * shiftcoef * kernelh;
+  const double coeftfs = 3.0 - tfs;
+  const float maxdist = float(dp * 0.1);
+  for (unsigned p = pini; p < pini + n; p++) {
+    float* rs = shiftposfs + 4 * size_t(p);
+    if (rs[0] != FLT_MAX) {
+      const float* v = velrho + 4 * size_t(p);
+      const double vx = v[0], vy = v[1], vz = v[2];
+      double umagn = coefumagn * std::sqrt(vx * vx + vy * vy + vz * vz);
+      if (tfs) {
+        if (rs[3] < tfs) umagn = 0;
+        else umagn *= (double(rs[3]) - tfs) / coeftfs;
+      }
+      for (int k = 0; k < 3; k++) {
+        const float sd = float(double(rs[k]) * umagn);
+        rs[k] = (std::fabs(sd) < maxdist ? sd : (sd >= 0 ? maxdist : -maxdist));
+      }
+    } else {
+      rs[0] = rs[1] = rs[2] = rs[3] = 0;
+    }
+  }
+}
+// Adds the shifting displacement to the fluid displacement mov (FP64).
+void orc_add_shift(unsigned np, unsigned npb, const float* shiftposfs, double* mov) {
+  for (unsigned p = npb; p < np; p++)
+    for (int k = 0; k < 3; k++) mov[3 * size_t(p) + k] += double(shiftposfs[4 * size_t(p) + k]);
 }
 // Integrator rule of mDBC2 boundaries (mode >= 1 keeps the state it starts
 // from: KerComputeStepSymplecticPre/Cor JSphGpuSimple_ker.cu:276,406-412,

@@ -64,6 +64,14 @@ void orc_mdbc2(int, int, const orc_cte*, const orc_div*, unsigned, const double*
                const typecode*, const float*, const float*, const float*, const float*, double,
                double, float*, unsigned char*, float*);
 void orc_mdbc2_keep(unsigned, const unsigned char*, const float*, float*);
+void orc_interaction_ext(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                         unsigned, unsigned, unsigned, const unsigned*, const float*,
+                         const double*, const float*, const float*, float*, float*, float*,
+                         float*, float*, const unsigned char*, const float*, int,
+                         const typecode*, float*);
+void orc_run_shifting(unsigned, unsigned, double, float, float, float, double, const float*,
+                      float*);
+void orc_add_shift(unsigned, unsigned, const float*, double*);
 void orc_pips(const orc_cte*, const orc_div*, unsigned, unsigned, unsigned, const unsigned*,
               const float*, unsigned long long*);
 }
@@ -89,6 +97,9 @@ struct Sim {
   std::vector<float> velrho, velrhopre, velrhom1, poscell, ace, ar, viscdt, strain, tau;
   std::vector<float> boundnor;  // doubled normals (ConfigBoundNormals), mDBC only
   int TBoundary = 1, SlipMode = 1;
+  int ShiftMode = 0;
+  float ShiftCoef = -2.f, ShiftTFS = 0.f;
+  std::vector<float> shiftposfs;     // shifting float4 per particle
   std::vector<unsigned char> bmode;  // mDBC2 boundary modes of [0,Npb)
   std::vector<float> tangenvel;      // mDBC2 tangential velocities of [0,Npb)
   bool pre_active = false;
@@ -123,6 +134,9 @@ struct Sim {
     c.ddtgz = float(double(c.rhopzero) * double(std::fabs(g[2])) / double(c.cteb));
     TStep = s.tstep; TVisco = s.tvisco; TDDT = s.tdensity; TBoundary = s.tboundary;
     SlipMode = s.slipmode ? s.slipmode : 1;
+    ShiftMode = s.shiftmode;
+    ShiftCoef = float(s.shiftcoef ? s.shiftcoef : -2.0);
+    ShiftTFS = float(s.shifttfs);
     if (TBoundary == 2 && SlipMode == 2 && s.casenmoving)
       throw std::runtime_error("mDBC no-slip is built for fixed boundaries only");
     VerletSteps = s.verletsteps ? s.verletsteps : 40;
@@ -284,12 +298,14 @@ struct Sim {
     const orc_div d = divdata();
     // prec 0: FP32 GPU semantics, 1: FP64 yardstick, 2: FP32 CPU semantics.
     const bool m2 = mdbc2_on();
-    orc_interaction_mdbc2(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT,
-                          ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
-                          dcell.data(), poscell.data(), pos.data(), velrho.data(),
-                          TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(),
-                          viscdt.data(), TVisco == 2 ? strain.data() : nullptr, nullptr,
-                          m2 ? bmode.data() : nullptr, m2 ? tangenvel.data() : nullptr);
+    if (ShiftMode) shiftposfs.assign(4 * size_t(Np), 0.f);  // JSphShifting::InitCpu, no zones
+    orc_interaction_ext(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT,
+                        ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
+                        dcell.data(), poscell.data(), pos.data(), velrho.data(),
+                        TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(),
+                        viscdt.data(), TVisco == 2 ? strain.data() : nullptr, nullptr,
+                        m2 ? bmode.data() : nullptr, m2 ? tangenvel.data() : nullptr, ShiftMode,
+                        code.data(), ShiftMode ? shiftposfs.data() : nullptr);
     if (TVisco == 2) orc_sps_tau(Np, Npb, SpsSmag, SpsBlin, velrho.data(), strain.data(), tau.data());
     float vd = 0, a2 = 0;
     for (unsigned p = 0; p < Np; p++) vd = std::max(vd, viscdt[p]);
@@ -443,24 +459,35 @@ struct Sim {
              code.data(), boundnor.data(), velrho.data());
   }
 
+  // JSphCpu::RunShifting: shifting displacement from the interaction's sums.
+  void shifting(double dt) {
+    if (!ShiftMode) return;
+    orc_run_shifting(Np - Npb, Npb, dt, ShiftCoef, KernelH, ShiftTFS, cs.dp, velrho.data(),
+                     shiftposfs.data());
+  }
+
   void step_sym() {
     const double dt = SymplecticDtPre;
     mdbc();
     interaction();
     const double dt_p = dt_variable();
+    shifting(dt * .5);
     pospre = pos; velrhopre = velrho; pre_active = true;
     mov.assign(3 * size_t(Np), 0.0);
     orc_sym_pre(Np, Npb, velrhopre.data(), ar.data(), ace.data(), dt * .5, c.rhopzero, RhopOutMin,
                 RhopOutMax, g, code.data(), mov.data(), velrho.data());
+    if (ShiftMode) orc_add_shift(Np, Npb, shiftposfs.data(), mov.data());
     if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrhopre.data(), velrho.data());
     orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
     cell_divide();
     mdbc(true);
     interaction();
     const double dt_c = dt_variable();
+    shifting(dt);
     mov.assign(3 * size_t(Np), 0.0);
     orc_sym_cor(Np, Npb, velrhopre.data(), ar.data(), ace.data(), dt * .5, dt, c.rhopzero,
                 RhopOutMin, RhopOutMax, g, code.data(), mov.data(), velrho.data());
+    if (ShiftMode) orc_add_shift(Np, Npb, shiftposfs.data(), mov.data());
     if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrhopre.data(), velrho.data());
     orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
     pre_active = false;
@@ -474,6 +501,7 @@ struct Sim {
     mdbc();
     interaction();
     const double dt = dt_variable();
+    shifting(dt);
     VerletStep++;
     std::vector<float> nw(velrho.size());
     mov.assign(3 * size_t(Np), 0.0);
@@ -487,6 +515,7 @@ struct Sim {
       if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrho.data(), nw.data());
       VerletStep = 0;
     }
+    if (ShiftMode) orc_add_shift(Np, Npb, shiftposfs.data(), mov.data());
     std::vector<double> p0 = pos;
     orc_step_pos(Np, Npb, &c, p0.data(), mov.data(), pos.data(), dcell.data(), code.data());
     velrhom1 = velrho;

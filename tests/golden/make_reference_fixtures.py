@@ -61,6 +61,11 @@ CASES = {
                                       dict(tdensity=0, tvisco=2, visco=1e-6)),
     "ref_flume_dp0.01_piston_mdbc_verlet": ("flume", 0.01, 10, (1, 5, 10),
                                             dict(tboundary=2, tstep=1, tvisco=3, visco=1e-4)),
+    # Standard shifting: full, and without boundaries with the free-surface
+    # threshold.
+    "ref_dambreak_dp0.03_shift_full": ("dam_break", 0.03, 10, (1, 5, 10), dict(shifting=3)),
+    "ref_dambreak_dp0.03_shift_nobound_tfs": ("dam_break", 0.03, 10, (1, 5, 10),
+                                              dict(shifting=1, shifttfs=1.5)),
     # Half cells (cells of size h, -cellmode:half).
     "ref_dambreak_dp0.03_halfcells": ("dam_break", 0.03, 10, (1, 5, 10), dict(cellmode=2)),
     # mDBC2 (SlipMode no-slip) on the dam break's fixed walls.

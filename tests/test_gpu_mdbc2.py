@@ -166,14 +166,14 @@ def test_integrators_keep_mdbc2_boundaries(lib, ctx):
                 d_new = Dev.zeros(lib, ctx, (n, 4), np.float32)
                 rc = lib.dsph_step_verlet(ctx, n, npb, Dev(lib, ctx, vnow).ptr,
                                           Dev(lib, ctx, vpre).ptr, d_bm, Dev(lib, ctx, ar).ptr,
-                                          Dev(lib, ctx, ace).ptr, dt, 2 * dt, 1000., 700., 1300.,
+                                          Dev(lib, ctx, ace).ptr, None, dt, 2 * dt, 1000., 700., 1300.,
                                           G, d_code.ptr, d_mxy.ptr, d_mz.ptr, d_new.ptr)
             else:
                 d_new = Dev(lib, ctx, vnow)
                 fn = lib.dsph_step_symplectic_pre if mode == "pre" else lib.dsph_step_symplectic_cor
                 extra = (dt * .5,) if mode == "pre" else (dt * .5, dt)
                 rc = fn(ctx, n, npb, Dev(lib, ctx, vpre).ptr, d_bm, Dev(lib, ctx, ar).ptr,
-                        Dev(lib, ctx, ace).ptr, *extra, None, 0, 1000., 700., 1300., G,
+                        Dev(lib, ctx, ace).ptr, None, *extra, None, 0, 1000., 700., 1300., G,
                         d_code.ptr, d_mxy.ptr, d_mz.ptr, d_new.ptr)
             assert rc == 0, lib.dsph_last_error(ctx)
             out[(mode, with_bm)] = d_new.get()

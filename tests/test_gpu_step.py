@@ -42,11 +42,11 @@ def test_symplectic(lib, ctx, kind):
     d_movz = Dev.zeros(lib, ctx, n, np.float64)
     d_vr = Dev(lib, ctx, vr)
     if kind == "pre":
-        rc = lib.dsph_step_symplectic_pre(ctx, n, npb, d_vpre.ptr, None, d_ar.ptr, d_ace.ptr, dt * .5,
+        rc = lib.dsph_step_symplectic_pre(ctx, n, npb, d_vpre.ptr, None, d_ar.ptr, d_ace.ptr, None, dt * .5,
                                           None, 0, 1000., 700., 1300., G, d_code.ptr,
                                           d_movxy.ptr, d_movz.ptr, d_vr.ptr)
     else:
-        rc = lib.dsph_step_symplectic_cor(ctx, n, npb, d_vpre.ptr, None, d_ar.ptr, d_ace.ptr, dt * .5,
+        rc = lib.dsph_step_symplectic_cor(ctx, n, npb, d_vpre.ptr, None, d_ar.ptr, d_ace.ptr, None, dt * .5,
                                           dt, None, 0, 1000., 700., 1300., G, d_code.ptr,
                                           d_movxy.ptr, d_movz.ptr, d_vr.ptr)
     assert rc == 0
@@ -82,7 +82,7 @@ def test_verlet(lib, ctx):
     d_mxy = Dev.zeros(lib, ctx, (n, 2), np.float64)
     d_mz = Dev.zeros(lib, ctx, n, np.float64)
     rc = lib.dsph_step_verlet(ctx, n, npb, Dev(lib, ctx, vr).ptr, Dev(lib, ctx, vm1).ptr, None,
-                              Dev(lib, ctx, ar).ptr, Dev(lib, ctx, ace).ptr, dt, 2 * dt, 1000.,
+                              Dev(lib, ctx, ar).ptr, Dev(lib, ctx, ace).ptr, None, dt, 2 * dt, 1000.,
                               700., 1300., G, d_code.ptr, d_mxy.ptr, d_mz.ptr, d_new.ptr)
     assert rc == 0
     onew = np.zeros((n, 4), np.float32)

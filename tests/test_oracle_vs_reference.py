@@ -14,7 +14,8 @@ break with mDBC2 (SlipMode no-slip) + laminar viscosity; flume with a flap
 wavemaker (matrix motion: rotation sinusoid with DBC, constant rotation with
 mDBC and the normals turned with the flap); dam break with the
 Molteni-Colagrossi DDT and with laminar+SPS without DDT; the piston flume
-with Verlet and mDBC.
+with Verlet and mDBC; half cells; standard shifting (no boundaries with the
+free-surface threshold, and full).
 """
 import numpy as np
 import pytest
@@ -27,7 +28,8 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_dambreak_dp0.03_mdbc2", "ref_flume_dp0.01_flap",
             "ref_flume_dp0.01_flap_rot_mdbc", "ref_dambreak_dp0.03_ddt1",
             "ref_dambreak_dp0.03_noddt_sps", "ref_flume_dp0.01_piston_mdbc_verlet",
-            "ref_dambreak_dp0.03_halfcells"]
+            "ref_dambreak_dp0.03_halfcells", "ref_dambreak_dp0.03_shift_nobound_tfs",
+            "ref_dambreak_dp0.03_shift_full"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
