@@ -132,7 +132,8 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
     const float rhop1over2 = vr1.w * rrho2;
     const float visc_densi = c.ddtkh * cbar * (rhop1over2 - 1.f) * rr2eta;
     const float delta = visc_densi * dot3 * massp2;
-    a.delta = (boundp2 ? FLT_MAX : a.delta + delta);
+    // A boundary neighbour cancels the term with DBC only (JSphGpu_ker.cu:665).
+    a.delta = ((boundp2 && c.tboundary == DSPH_BC_DBC) ? FLT_MAX : a.delta + delta);
   }
   if ((TDDT == DSPH_DDT_DDT2 && a.delta != FLT_MAX) || (TDDT == DSPH_DDT_DDT2FULL && !boundp2)) {
     const float rh = 1.f + c.ddtgz * dr.z;

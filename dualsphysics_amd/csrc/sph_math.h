@@ -10,6 +10,7 @@ struct Cte {
   float massb, massf, h, invh, ks2, pcs, awen, bwenh, cs0, eta2, ddtkh, ddtgz, rhopzero, invgamma,
       cteb, gamma, ovrhopzero, dp;
   unsigned cellcode;
+  unsigned tboundary;  // DSPH_BC_*: the Molteni-Colagrossi term skips boundaries only with DBC
   // Binomial-series coefficients of (1+u)^(1/gamma), k = 1..5.
   float pc1, pc2, pc3, pc4, pc5;
 };
@@ -21,6 +22,7 @@ inline Cte make_cte(const dsph_cte& c) {
   r.ddtkh = c.ddtkh; r.ddtgz = c.ddtgz; r.rhopzero = c.rhopzero; r.invgamma = 1.f / c.gamma;
   r.cteb = c.cteb; r.gamma = c.gamma; r.ovrhopzero = c.ovrhopzero; r.dp = c.dp;
   r.cellcode = c.cellcode;
+  r.tboundary = c.tboundary;
   // binom(a,k) = binom(a,k-1) * (a-k+1)/k with a = invgamma (as the reference's 1.f/gamma).
   const double a = double(r.invgamma);
   double b = 1.0;

@@ -62,6 +62,7 @@ struct orc_cte {
   unsigned cellcode;
   double domposmin[3], maprealposmin[3], maprealsize[3];
   double mapposmin[3];
+  unsigned tboundary;  // 2 = mDBC; otherwise DBC
 };
 
 // Neighbour-search data (StDivDataCpu / StDivDataGpu).
@@ -321,7 +322,9 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
       const R dot3 = dx * frx + dy * fry + dz * frz;
       if (o.tddt == 1 && !a.delta_off) {
         const R visc_densi = ddtkh * cbar * (rho1 / rho2 - R(1)) / (rr2 + eta2);
-        if (boundp2) a.delta_off = true;
+        // A boundary neighbour cancels the term with DBC only
+        // (JSphCpu.cpp:930; JSphGpu_ker.cu:665).
+        if (boundp2 && c.tboundary != 2) a.delta_off = true;
         else a.delta += visc_densi * dot3 * massp2;
       }
       if ((o.tddt == 2 || (o.tddt == 3 && !boundp2)) && !a.delta_off) {

@@ -29,6 +29,7 @@ struct orc_cte {
   unsigned cellcode;
   double domposmin[3], maprealposmin[3], maprealsize[3];
   double mapposmin[3];
+  unsigned tboundary;
 };
 struct orc_div {
   int scelldiv, ncx, ncy, ncz, ncw;
@@ -171,6 +172,7 @@ struct Sim {
     Scell = ks / ScellDiv;
     c.scell = Scell;
     c.movlimit = Scell * 0.9f;
+    c.tboundary = unsigned(TBoundary);
     for (int k = 0; k < 3; k++) DomCells[k] = unsigned(std::ceil(msize[k] / Scell));
     c.cellcode = orc_cell_code(DomCells[0] + 1, DomCells[1] + 1, DomCells[2] + 1);
     CaseNbound = s.casenbound;

@@ -20,7 +20,8 @@ class OrcCte(C.Structure):
         "massb", "massf", "kernelh", "kernelsize2", "poscellsize", "awen", "bwenh", "cs0",
         "eta2", "ddtkh", "ddtgz", "scell", "kernelsize", "dp", "cteb", "gamma", "rhopzero",
         "ovrhopzero", "movlimit")] + [("cellcode", C.c_uint32)] + [
-        (n, C.c_double * 3) for n in ("domposmin", "maprealposmin", "maprealsize", "mapposmin")]
+        (n, C.c_double * 3) for n in ("domposmin", "maprealposmin", "maprealsize", "mapposmin")] + [
+        ("tboundary", C.c_uint32)]
 
 
 class OrcDiv(C.Structure):

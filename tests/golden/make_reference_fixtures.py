@@ -66,6 +66,9 @@ CASES = {
     "ref_dambreak_dp0.03_shift_full": ("dam_break", 0.03, 10, (1, 5, 10), dict(shifting=3)),
     "ref_dambreak_dp0.03_shift_nobound_tfs": ("dam_break", 0.03, 10, (1, 5, 10),
                                               dict(shifting=1, shifttfs=1.5)),
+    # Molteni-Colagrossi DDT with mDBC (boundaries inside the term).
+    "ref_flume_dp0.01_piston_mdbc_ddt1": ("flume", 0.01, 10, (1, 5, 10),
+                                          dict(tboundary=2, tdensity=1)),
     # Half cells (cells of size h, -cellmode:half).
     "ref_dambreak_dp0.03_halfcells": ("dam_break", 0.03, 10, (1, 5, 10), dict(cellmode=2)),
     # mDBC2 (SlipMode no-slip) on the dam break's fixed walls.

@@ -70,11 +70,11 @@ def test_options_outside_the_path_are_refused(tmp_path):
     c = cm.dam_break(dp=0.05)
     bi4.write_case(c, tmp_path, "case")
     xml = (tmp_path / "case.xml").read_text()
-    (tmp_path / "shift.xml").write_text(xml.replace('key="Shifting" value="0"',
-                                                    'key="Shifting" value="1"'))
-    shutil.copy(tmp_path / "case.bi4", tmp_path / "shift.bi4")
-    with pytest.raises(ValueError, match="shifting"):
-        bi4.read_case(tmp_path, "shift")
+    (tmp_path / "dtfix.xml").write_text(xml.replace('key="DtFixed" value="0"',
+                                                    'key="DtFixed" value="0.0001"'))
+    shutil.copy(tmp_path / "case.bi4", tmp_path / "dtfix.bi4")
+    with pytest.raises(ValueError, match="fixed dt"):
+        bi4.read_case(tmp_path, "dtfix")
     (tmp_path / "cubic.xml").write_text(xml.replace('key="Kernel" value="2"',
                                                     'key="Kernel" value="1"'))
     shutil.copy(tmp_path / "case.bi4", tmp_path / "cubic.bi4")

@@ -20,7 +20,8 @@ pytestmark = pytest.mark.gpu
                                   "ref_flume_dp0.01_piston_mdbc_verlet",
                                   "ref_dambreak_dp0.03_halfcells",
                                   "ref_dambreak_dp0.03_shift_nobound_tfs",
-                                  "ref_dambreak_dp0.03_shift_full"])
+                                  "ref_dambreak_dp0.03_shift_full",
+                                  "ref_flume_dp0.01_piston_mdbc_ddt1"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)
