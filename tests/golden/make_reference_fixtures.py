@@ -69,6 +69,16 @@ CASES = {
     # Molteni-Colagrossi DDT with mDBC (boundaries inside the term).
     "ref_flume_dp0.01_piston_mdbc_ddt1": ("flume", 0.01, 10, (1, 5, 10),
                                           dict(tboundary=2, tdensity=1)),
+    # More combinations: mDBC2 with Verlet and the Molteni-Colagrossi DDT;
+    # shifting that spares the moving piston (NoFixed) with mDBC; a boundary
+    # viscosity factor.
+    "ref_dambreak_dp0.03_mdbc2_verlet_ddt1": ("dam_break", 0.03, 10, (1, 5, 10),
+                                              dict(tboundary=2, slipmode=2, tstep=1, tdensity=1,
+                                                   tvisco=3, visco=1e-4)),
+    "ref_flume_dp0.01_piston_mdbc_shift_nofixed": ("flume", 0.01, 10, (1, 5, 10),
+                                                   dict(tboundary=2, shifting=2, shifttfs=1.5)),
+    "ref_dambreak_dp0.03_viscbound2": ("dam_break", 0.03, 10, (1, 5, 10),
+                                       dict(viscoboundfactor=2.0)),
     # Half cells (cells of size h, -cellmode:half).
     "ref_dambreak_dp0.03_halfcells": ("dam_break", 0.03, 10, (1, 5, 10), dict(cellmode=2)),
     # mDBC2 (SlipMode no-slip) on the dam break's fixed walls.

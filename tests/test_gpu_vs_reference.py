@@ -21,7 +21,10 @@ pytestmark = pytest.mark.gpu
                                   "ref_dambreak_dp0.03_halfcells",
                                   "ref_dambreak_dp0.03_shift_nobound_tfs",
                                   "ref_dambreak_dp0.03_shift_full",
-                                  "ref_flume_dp0.01_piston_mdbc_ddt1"])
+                                  "ref_flume_dp0.01_piston_mdbc_ddt1",
+                                  "ref_dambreak_dp0.03_mdbc2_verlet_ddt1",
+                                  "ref_flume_dp0.01_piston_mdbc_shift_nofixed",
+                                  "ref_dambreak_dp0.03_viscbound2"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)

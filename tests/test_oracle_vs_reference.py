@@ -29,7 +29,9 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_flume_dp0.01_flap_rot_mdbc", "ref_dambreak_dp0.03_ddt1",
             "ref_dambreak_dp0.03_noddt_sps", "ref_flume_dp0.01_piston_mdbc_verlet",
             "ref_dambreak_dp0.03_halfcells", "ref_dambreak_dp0.03_shift_nobound_tfs",
-            "ref_dambreak_dp0.03_shift_full", "ref_flume_dp0.01_piston_mdbc_ddt1"]
+            "ref_dambreak_dp0.03_shift_full", "ref_flume_dp0.01_piston_mdbc_ddt1",
+            "ref_dambreak_dp0.03_mdbc2_verlet_ddt1", "ref_flume_dp0.01_piston_mdbc_shift_nofixed",
+            "ref_dambreak_dp0.03_viscbound2"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
