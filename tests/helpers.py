@@ -172,7 +172,10 @@ REF_TOL_MDBC = {"time": 1e-5, "disp": 3e-3, "vel": 3e-3, "rho": 1e-2}
 # displacement reaches its clamp (0.1 dp per step) and the oracle (all three
 # semantics) and the HIP path, within 4e-5 of each other, stay 1.4e-4
 # (displacement), 2.6e-4 (velocity) and 1.2e-3 (density) from the reference
-# after 10 steps: an unresolved difference, bounded here at 2x.
+# after 10 steps. The reference's displacement recovered from its PART files
+# has the same direction (to 3e-10) and a magnitude within 2e-4 tracking |v|:
+# shifting (proportional to |v|) amplifies the FP32 velocity differences at
+# the free surface. Bounded here at 2x the measured spread.
 REF_TOL_SHIFT_NOTFS = {"time": 2e-6, "disp": 3e-4, "vel": 6e-4, "rho": 2.5e-3}
 
 
