@@ -372,7 +372,8 @@ struct TileLds {
   unsigned tbase[kRing];
 };
 
-template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false>
+template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false,
+          bool SHIFT = false>
 __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned pinit,
     float viscob, float viscof, const DivData& dvd, const Cte& c, float ks2scan,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
@@ -380,7 +381,8 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
     const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
     float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
     float* __restrict__ velgrad, TileLds<kRing>& lds, const uint8_t* __restrict__ bmode = nullptr,
-    const float* __restrict__ tangenvel = nullptr) {
+    const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
+    const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr) {
   float4* tile = lds.tile;
   auto& masks = lds.masks;
   unsigned* tbase = lds.tbase;
@@ -390,6 +392,7 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
   acc_init(a);
   P1 q;
   load_p1<TVISCO>(p1, poscell, velrho, press, tau, q);
+  if (SHIFT) a.sh = shiftposfs[p1];
   const float4 pf1 = posf[p1];
   int cx, cy, cz;
   {
@@ -434,8 +437,9 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
     pr2 = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(press) + (jj << 3));
   };
   auto eval = [&](unsigned j, const float4& ps2, const float4& vr2, const float2& pr2) {
-    pair_eval<TVISCO, TDDT, VELGRAD, MDBC2>((j >> 31) != 0, j & 0x7fffffffu, ps2, vr2, pr2, q,
-                                            tau, viscob, viscof, c, a, bmode, tangenvel);
+    pair_eval<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT>((j >> 31) != 0, j & 0x7fffffffu, ps2, vr2,
+                                                   pr2, q, tau, viscob, viscof, c, a, bmode,
+                                                   tangenvel, shiftmode, code);
   };
   // k wave-uniform iterations; each lane pops up to k of its queued pairs.
   // Two register sets alternate so the loads of the next pair are in flight
@@ -552,10 +556,14 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
     }
   }
   process(wave_max_u32(len), Some());
-  if (valid) store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
+  if (valid) {
+    store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
+    if (SHIFT) shiftposfs[p1] = a.sh;
+  }
 }
 
-template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false>
+template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false,
+          bool SHIFT = false>
 __global__ void __launch_bounds__(64) k_forces_fluid_tile(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c, float ks2scan,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
@@ -563,13 +571,13 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
     const float2* __restrict__ press, const float* __restrict__ tau, float* __restrict__ viscdt,
     float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
     float* __restrict__ velgrad, const uint8_t* __restrict__ bmode = nullptr,
-    const float* __restrict__ tangenvel = nullptr) {
+    const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
+    const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr) {
   __shared__ TileLds<kRing> lds;
   const unsigned p = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-  fluid_tile_body<TVISCO, TDDT, VELGRAD, kRing, MDBC2>(p, p < n, pinit, viscob, viscof, dvd, c,
-                                                       ks2scan, dcell, poscell, posf, velrho,
-                                                       press, tau, viscdt, ar, ace, strain,
-                                                       velgrad, lds, bmode, tangenvel);
+  fluid_tile_body<TVISCO, TDDT, VELGRAD, kRing, MDBC2, SHIFT>(
+      p, p < n, pinit, viscob, viscof, dvd, c, ks2scan, dcell, poscell, posf, velrho, press, tau,
+      viscdt, ar, ace, strain, velgrad, lds, bmode, tangenvel, shiftmode, code, shiftposfs);
 }
 
 // Boundary particle p1 against fluid neighbours (KerInteractionForcesBound,
@@ -626,9 +634,16 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
                   const float2* press, const float4* posf) {
   KTimer kt(ctx, "k_forces_fluid");
   const unsigned npf = t->np - t->npb;
-  if (t->d_shiftposfs) {
-    // Shifting: the one-lane-per-particle kernel (mDBC2 and the velocity
-    // gradient are refused with shifting by dsph_interaction_forces).
+  if (t->d_shiftposfs && ctx->forces_kernel == 0) {
+    // Shifting in the tile kernel (mDBC2 and the velocity gradient are
+    // refused with shifting by dsph_interaction_forces).
+    k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, false, true><<<grid_for(npf, 64), 64, 0,
+                                                                       ctx->stream>>>(
+        npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
+        (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press, t->d_spstaurho2,
+        t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, nullptr, nullptr, nullptr, t->shiftmode,
+        t->d_code, (float4*)t->d_shiftposfs);
+  } else if (t->d_shiftposfs) {
     k_forces_fluid_rows<TVISCO, TDDT, false, false, true><<<grid_for(npf, kBsForces), kBsForces,
                                                            0, ctx->stream>>>(
         npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
