@@ -34,7 +34,9 @@ class DsphCte(C.Structure):
         ("tboundary", C.c_uint32), ("cellcode", C.c_uint32)] + [
         (n, C.c_double) for n in (
             "domposminx", "domposminy", "domposminz", "maprealposminx", "maprealposminy",
-            "maprealposminz", "maprealsizex", "maprealsizey", "maprealsizez")]
+            "maprealposminz", "maprealsizex", "maprealsizey", "maprealsizez")] + [
+        ("periactive", C.c_uint32), ("pad_peri", C.c_uint32),
+        ("xperinc", C.c_double * 3), ("yperinc", C.c_double * 3), ("zperinc", C.c_double * 3)]
 
 
 class DsphDivData(C.Structure):
@@ -80,7 +82,8 @@ class DsphCase(C.Structure):
         (n, C.c_uint32) for n in ("casenp", "casenbound", "casenfixed", "casenmoving",
                                   "casenfloat", "casenfluid")] + [("motion", DsphMotion),
                                                                   ("slipmode", C.c_int32),
-        ("shiftmode", C.c_int32), ("shiftcoef", C.c_double), ("shifttfs", C.c_double)]
+        ("shiftmode", C.c_int32), ("shiftcoef", C.c_double), ("shifttfs", C.c_double),
+        ("periactive", C.c_int32), ("perinc", (C.c_double * 3) * 3)]
 
 
 class DsphMlp(C.Structure):
@@ -140,6 +143,11 @@ SIGNATURES = {
                               C.POINTER(_d)] + [_vp] * 10,
     "dsph_velmax2": [_vp, _u, _u, _vp, _vp],
     "dsph_acemax2": [_vp, _u, _u, _vp, _vp],
+    "dsph_acemax2_normal": [_vp, _u, _u, _vp, _vp, _vp, _vp],
+    "dsph_periodic_ignore": [_vp, _u, _vp],
+    "dsph_periodic_make_list": [_vp, _u, _u, _u, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "dsph_periodic_duplicate": [_vp, _u, _u, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp,
+                                _vp, _vp, _vp, _vp, _vp],
     "dsph_redu_max_f32": [_vp, _u, _u, _vp, _vp],
     "dsph_redu_max_f32_sync": [_vp, _u, _u, _vp, C.POINTER(_f)],
     "dsph_dt_variable": [_vp, _vp, _d, _d, _d, _d, _vp, _u, _vp],

@@ -258,6 +258,13 @@ def write_case(case, directory, name: str, timemax: float | None = None,
     }
     if p.tboundary == 2:
         params["SlipMode"] = getattr(p, "slipmode", 1)
+    # Periodic axes (JSph.cpp:810-826): a key of the axis' increments turns it on.
+    peri = getattr(p, "periodic", 0)
+    for a, keys in enumerate((("XPeriodicIncY", "XPeriodicIncZ"), ("YPeriodicIncX", "YPeriodicIncZ"),
+                              ("ZPeriodicIncX", "ZPeriodicIncY"))):
+        if peri & (1 << a):
+            for key in keys:
+                params[key] = 0.0
     moving, motion = "", ""
     if nmov:
         m = case.motion
@@ -557,7 +564,9 @@ def read_case(directory, name: str):
                       rhopoutmin=gf("RhopOutMin", 700.0), rhopoutmax=gf("RhopOutMax", 1300.0),
                       timemax=gf("TimeMax", 1.0), timeout=gf("TimeOut", 0.0),
                       shifting=gi("Shifting", 0), shiftcoef=gf("ShiftCoef", -2.0),
-                      shifttfs=gf("ShiftTFS", 0.0))
+                      shifttfs=gf("ShiftTFS", 0.0),
+                      periodic=sum(1 << a for a, ax in enumerate("XYZ")
+                                   if any(k.startswith(ax + "Periodic") for k in prm)))
     parts = ex.find("particles")
     blocks = list(parts)
     if any(bk.tag == "floating" for bk in blocks):
@@ -616,7 +625,8 @@ def read_case(directory, name: str):
             raise ValueError(f"motion <{mv.tag}> is not built")
     # Domain: the particles' box widened by h*0.05 ("default"), then the
     # expressions of <simulationdomain> ("default + X%" of that box's size).
-    bmin, bmax = (np.array(v) for v in cm.map_limits(pos, h, zextra=0.0))
+    bmin, bmax = (np.array(v) for v in cm.map_limits(pos, h, zextra=0.0, periodic=p.periodic,
+                                                     dp=dp))
     size = bmax - bmin
     mn, mx = bmin.copy(), bmax.copy()
     sd = ex.find("parameters").find("simulationdomain")

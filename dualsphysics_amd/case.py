@@ -63,6 +63,7 @@ class Parameters:
     shifting: int = N.SHIFT_NONE    # Shifting 0 none, 1 no boundaries, 2 no fixed, 3 full
     shiftcoef: float = -2.0
     shifttfs: float = 0.0
+    periodic: int = 0               # periodic axes, bits 1 X, 2 Y, 4 Z (XPeriodicIncY ...)
     cflnumber: float = 0.2
     dtini: float = 0.0
     dtmin: float = 0.0
@@ -121,6 +122,7 @@ class Case:
         c.cellmode, c.celldomfixed = p.cellmode, p.celldomfixed
         c.slipmode = p.slipmode
         c.shiftmode, c.shiftcoef, c.shifttfs = p.shifting, p.shiftcoef, p.shifttfs
+        c.periactive = int(p.periodic)
         c.casenp, c.casenbound = self.np, self.nbound
         c.casenfixed = self.nbound - self.nmoving
         c.casenmoving, c.casenfloat, c.casenfluid = self.nmoving, 0, self.nfluid
@@ -178,9 +180,15 @@ def _n(length, dp):
     return int(math.floor(length / dp + 0.5))
 
 
-def map_limits(pos: np.ndarray, h: float, zextra: float = 0.5):
-    """Map limits of the simulation domain from the particles (see module doc)."""
-    border = float(np.float32(h)) * 0.05
+def map_limits(pos: np.ndarray, h: float, zextra: float = 0.5, periodic: int = 0,
+               dp: float = 0.0):
+    """Map limits of the simulation domain from the particles (see module doc);
+    a periodic axis takes a border of dp/2 (JPartsLoad4::CalculeLimitsPos,
+    JPartsLoad4.cpp:389-399)."""
+    border = np.full(3, float(np.float32(h)) * 0.05)
+    for a in range(3):
+        if periodic & (1 << a):
+            border[a] = dp / 2.0
     pmin = pos.min(0) - border
     pmax = pos.max(0) + border
     pmax = pmax.copy()
@@ -224,6 +232,49 @@ def dam_break(dp: float = 0.01, tank=(1.6, 0.67, 0.4), column=(0.4, 0.67, 0.3), 
     return Case(name=name or f"dambreak_dp{dp:g}", dp=dp, h=h, rhop0=rhop0, gamma=gamma,
                 cs0=cs0, b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g),
                 hswl=hswl, params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb,
+                nfluid=nf, maprealposmin=mn, maprealposmax=mx, boundnor=boundnor)
+
+
+def periodic_channel(dp: float = 0.02, length: float = 0.4, width: float = 0.2,
+                     depth: float = 0.2, layers: int = 2, u0: float = 2.0, coefh: float = 1.2,
+                     coefsound: float = 20.0, rhop0: float = 1000.0, gamma: float = 7.0,
+                     g: float = 9.81, seed: int = 42, jitter: float = 1e-4,
+                     params: Parameters | None = None, name: str | None = None) -> Case:
+    """Channel periodic in x: a floor and two side walls (``layers`` thick)
+    under a fluid layer of ``depth`` moving at ``u0`` along x, hydrostatic
+    density. The lattice spans [0, length) in x, so that it continues across
+    the periodic faces (map limits dp/2 beyond the outer particles)."""
+    params = params or Parameters(periodic=1)
+    if not params.periodic & 1:
+        raise ValueError("periodic_channel needs periodic=1 (x)")
+    nx, ny, nz = _n(length, dp), _n(width, dp), _n(depth, dp)
+    k, j, i = np.meshgrid(np.arange(-layers, nz), np.arange(-layers, ny + layers),
+                          np.arange(0, nx), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+    wall = (k < 0) | (j < 0) | (j >= ny)
+    wpts = np.stack([(i[wall] + 0.5) * dp, (j[wall] + 0.5) * dp, (k[wall] + 0.5) * dp], 1)
+    fpts, _ = _lattice(0, nx, 0, ny, 0, nz, dp)
+    nb, nf = wpts.shape[0], fpts.shape[0]
+    pos = np.concatenate([wpts, fpts], 0)
+    u = uniform01(seed, 3 * pos.shape[0]).reshape(-1, 3)
+    pos = pos + (2.0 * u - 1.0) * (jitter * dp)
+    h = coefh * math.sqrt(3.0 * dp * dp)
+    cs0 = coefsound * math.sqrt(g * depth)
+    b = cs0 * cs0 * rhop0 / gamma
+    mass = rhop0 * dp ** 3
+    velrho = np.zeros((pos.shape[0], 4), np.float32)
+    velrho[:nb, 3] = rhop0
+    velrho[nb:, 0] = u0
+    zf = pos[nb:, 2]
+    velrho[nb:, 3] = (rhop0 * (1.0 + rhop0 * g * (depth - zf) / b) ** (1.0 / gamma)).astype(np.float32)
+    bp = pos[:nb]
+    q = np.stack([bp[:, 0], np.clip(bp[:, 1], 0.0, ny * dp), np.maximum(bp[:, 2], 0.0)], 1)
+    boundnor = (q - bp).astype(np.float32)
+    idp = np.arange(pos.shape[0], dtype=np.uint32)
+    mn, mx = map_limits(pos, h, periodic=params.periodic, dp=dp)
+    return Case(name=name or f"channel_dp{dp:g}", dp=dp, h=h, rhop0=rhop0, gamma=gamma,
+                cs0=cs0, b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g),
+                hswl=depth, params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb,
                 nfluid=nf, maprealposmin=mn, maprealposmax=mx, boundnor=boundnor)
 
 

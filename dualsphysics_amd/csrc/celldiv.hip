@@ -422,6 +422,130 @@ __global__ void __launch_bounds__(kBs) k_ens_sort_basic(const EnsCase* __restric
                           ec.c.pcs);
 }
 
+
+// ---------------------------------------------------------------- periodic boundaries
+// KerPeriodicIgnore (JSphGpu_ker.cu:2113-2122).
+__global__ void __launch_bounds__(kBs) k_periodic_ignore(unsigned n, uint16_t* __restrict__ code) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  const unsigned c = code[p];
+  if (code_special(c) == kCodePeriodic) code[p] = uint16_t(code_set(c, kCodeOutIgnore));
+}
+
+// Entries a particle adds to the periodic list (KerPeriodicMakeList,
+// JSphGpu_ker.cu:2142-2186): bit 0 plus, bit 1 minus.
+__device__ inline unsigned periodic_flags(unsigned p2, double3 mn, double3 mx, double3 inc,
+                                          const double2* posxy, const double* posz,
+                                          const uint16_t* code) {
+  if (code_special(code[p2]) > kCodePeriodic) return 0;
+  const double2 rxy = posxy[p2];
+  const double rz = posz[p2];
+  unsigned f = 0;
+  double x = rxy.x + inc.x, y = rxy.y + inc.y, z = rz + inc.z;
+  if (mn.x <= x && mn.y <= y && mn.z <= z && x < mx.x && y < mx.y && z < mx.z) f |= 1;
+  x = rxy.x - inc.x; y = rxy.y - inc.y; z = rz - inc.z;
+  if (mn.x <= x && mn.y <= y && mn.z <= z && x < mx.x && y < mx.y && z < mx.z) f |= 2;
+  return f;
+}
+
+// Inclusive prefix sum over the block of kBs lanes (wave scan + LDS).
+__device__ inline unsigned block_incl_scan(unsigned v, unsigned* lds) {
+  const unsigned lane = threadIdx.x & 63u, wv = threadIdx.x >> 6;
+  for (unsigned o = 1; o < 64; o <<= 1) {
+    const unsigned t = __shfl_up(v, o, 64);
+    if (lane >= o) v += t;
+  }
+  if (lane == 63) lds[wv] = v;
+  __syncthreads();
+  unsigned add = 0;
+  for (unsigned w = 0; w < wv; w++) add += lds[w];
+  __syncthreads();
+  return v + add;
+}
+
+// Pass 1: entries per block of kBs particles.
+__global__ void __launch_bounds__(kBs) k_periodic_count(unsigned n, unsigned pini, double3 mn,
+    double3 mx, double3 inc, const double2* __restrict__ posxy, const double* __restrict__ posz,
+    const uint16_t* __restrict__ code, unsigned* __restrict__ blockcount) {
+  __shared__ unsigned lds[kBs / 64];
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned f = (i < n) ? periodic_flags(i + pini, mn, mx, inc, posxy, posz, code) : 0u;
+  const unsigned s = block_incl_scan((f & 1) + (f >> 1), lds);
+  if (threadIdx.x == kBs - 1) blockcount[blockIdx.x] = s;
+}
+
+// Pass 2: exclusive scan of the block counts (one block; total at [nb]).
+__global__ void __launch_bounds__(kBs) k_periodic_scan(unsigned nb, unsigned* __restrict__ blockcount) {
+  __shared__ unsigned lds[kBs / 64];
+  __shared__ unsigned carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (unsigned b0 = 0; b0 < nb; b0 += kBs) {
+    const unsigned b = b0 + threadIdx.x;
+    const unsigned v = (b < nb) ? blockcount[b] : 0u;
+    const unsigned s = block_incl_scan(v, lds);
+    const unsigned c0 = carry;
+    if (b < nb) blockcount[b] = c0 + s - v;
+    __syncthreads();
+    if (threadIdx.x == kBs - 1) carry = c0 + s;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) blockcount[nb] = carry;
+}
+
+// Pass 3: write the entries in particle order, plus before minus.
+__global__ void __launch_bounds__(kBs) k_periodic_write(unsigned n, unsigned pini, unsigned nmax,
+    double3 mn, double3 mx, double3 inc, const double2* __restrict__ posxy,
+    const double* __restrict__ posz, const uint16_t* __restrict__ code,
+    const unsigned* __restrict__ blockoff, unsigned* __restrict__ listp) {
+  __shared__ unsigned lds[kBs / 64];
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned p2 = i + pini;
+  const unsigned f = (i < n) ? periodic_flags(p2, mn, mx, inc, posxy, posz, code) : 0u;
+  const unsigned cnt = (f & 1) + (f >> 1);
+  unsigned o = blockoff[blockIdx.x] + block_incl_scan(cnt, lds) - cnt;
+  if (f & 1) { if (o < nmax) listp[o] = p2; o++; }
+  if (f & 2) { if (o < nmax) listp[o] = p2 | 0x80000000u; }
+}
+
+// KerPeriodicDuplicatePos + KerPeriodicDuplicateSymplectic / Verlet / Normals
+// (JSphGpu_ker.cu:2229-2400).
+__global__ void __launch_bounds__(kBs) k_periodic_duplicate(unsigned n, unsigned pini,
+    uint3 cellmax, double3 inc, double3 dmin, double scell, unsigned cellcode,
+    const unsigned* __restrict__ listp, unsigned* idp, uint16_t* code, unsigned* dcell,
+    double2* posxy, double* posz, float4* velrho, float* spstau, double2* posxypre,
+    double* poszpre, float4* velrhopre, float4* velrhom1, float* boundnor) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned pnew = i + pini;
+  const unsigned rp = listp[i];
+  const unsigned pc = rp & 0x7fffffffu;
+  const bool inverse = rp >= 0x80000000u;
+  double2 rxy = posxy[pc];
+  double rz = posz[pc];
+  rxy.x += (inverse ? -inc.x : inc.x);
+  rxy.y += (inverse ? -inc.y : inc.y);
+  rz += (inverse ? -inc.z : inc.z);
+  unsigned cx = unsigned((rxy.x - dmin.x) / scell);
+  unsigned cy = unsigned((rxy.y - dmin.y) / scell);
+  unsigned cz = unsigned((rz - dmin.z) / scell);
+  cx = (cx <= cellmax.x ? cx : cellmax.x);
+  cy = (cy <= cellmax.y ? cy : cellmax.y);
+  cz = (cz <= cellmax.z ? cz : cellmax.z);
+  posxy[pnew] = rxy;
+  posz[pnew] = rz;
+  dcell[pnew] = dcel_cell(cellcode, cx, cy, cz);
+  idp[pnew] = idp[pc];
+  code[pnew] = uint16_t((code[pc] & ~kCodeMaskSpecial) | kCodePeriodic);
+  velrho[pnew] = velrho[pc];
+  if (posxypre) { posxypre[pnew] = posxypre[pc]; poszpre[pnew] = poszpre[pc]; }
+  if (velrhopre) velrhopre[pnew] = velrhopre[pc];
+  if (velrhom1) velrhom1[pnew] = velrhom1[pc];
+  if (spstau)
+    for (int k = 0; k < 6; k++) spstau[6 * size_t(pnew) + k] = spstau[6 * size_t(pc) + k];
+  if (boundnor)
+    for (int k = 0; k < 3; k++) boundnor[3 * size_t(pnew) + k] = boundnor[3 * size_t(pc) + k];
+}
 }  // namespace
 
 namespace dsph {
@@ -484,6 +608,67 @@ int dsph_cell_limits_sync(dsph_ctx* ctx, uint32_t np, uint32_t pini, uint32_t ce
   DSPH_HIP(ctx, hipMemcpyAsync(h, res, 6 * sizeof(unsigned), hipMemcpyDeviceToHost, ctx->stream));
   DSPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (int k = 0; k < 3; k++) { h_celmin[k] = h[k]; h_celmax[k] = h[3 + k]; }
+  return DSPH_OK;
+}
+
+int dsph_periodic_ignore(dsph_ctx* ctx, uint32_t n, uint16_t* d_code) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!n) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_code, "dsph_periodic_ignore: null argument");
+  k_periodic_ignore<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(n, d_code);
+  DSPH_LAUNCHED(ctx, "k_periodic_ignore");
+  return DSPH_OK;
+}
+
+int dsph_periodic_make_list(dsph_ctx* ctx, uint32_t n, uint32_t pini, uint32_t nmax,
+                            const double mapposmin[3], const double mapposmax[3],
+                            const double perinc[3], const double* d_posxy, const double* d_posz,
+                            const uint16_t* d_code, uint32_t* d_work, uint32_t* d_listp,
+                            uint32_t* h_count) {
+  if (!ctx || !h_count) return DSPH_E_ARG;
+  *h_count = 0;
+  if (!n) return DSPH_OK;
+  DSPH_REQUIRE(ctx, mapposmin && mapposmax && perinc && d_posxy && d_posz && d_code && d_work &&
+                        d_listp, "dsph_periodic_make_list: null argument");
+  const double3 mn = make_double3(mapposmin[0], mapposmin[1], mapposmin[2]);
+  const double3 mx = make_double3(mapposmax[0], mapposmax[1], mapposmax[2]);
+  const double3 inc = make_double3(perinc[0], perinc[1], perinc[2]);
+  const unsigned nb = grid_for(n, kBs);
+  const double2* pxy = reinterpret_cast<const double2*>(d_posxy);
+  k_periodic_count<<<nb, kBs, 0, ctx->stream>>>(n, pini, mn, mx, inc, pxy, d_posz, d_code, d_work);
+  k_periodic_scan<<<1, kBs, 0, ctx->stream>>>(nb, d_work);
+  k_periodic_write<<<nb, kBs, 0, ctx->stream>>>(n, pini, nmax, mn, mx, inc, pxy, d_posz, d_code,
+                                                d_work, d_listp);
+  DSPH_LAUNCHED(ctx, "k_periodic_list");
+  DSPH_HIP(ctx, hipMemcpyAsync(ctx->pinned, d_work + nb, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                               ctx->stream));
+  DSPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  *h_count = *static_cast<uint32_t*>(ctx->pinned);
+  return DSPH_OK;
+}
+
+int dsph_periodic_duplicate(dsph_ctx* ctx, uint32_t n, uint32_t pini, const uint32_t domcells[3],
+                            const double perinc[3], const uint32_t* d_listp, uint32_t* d_idp,
+                            uint16_t* d_code, uint32_t* d_dcell, double* d_posxy, double* d_posz,
+                            float* d_velrho, float* d_spstau, double* d_posxypre,
+                            double* d_poszpre, float* d_velrhopre, float* d_velrhom1,
+                            float* d_boundnor) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!n) return DSPH_OK;
+  DSPH_REQUIRE(ctx, ctx->cte_set, "dsph_periodic_duplicate: constants not set");
+  DSPH_REQUIRE(ctx, domcells && perinc && d_listp && d_idp && d_code && d_dcell && d_posxy &&
+                        d_posz && d_velrho && (!d_posxypre == !d_poszpre),
+               "dsph_periodic_duplicate: null argument");
+  const dsph_cte& c = ctx->cte;
+  k_periodic_duplicate<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(
+      n, pini, make_uint3(domcells[0] - 1, domcells[1] - 1, domcells[2] - 1),
+      make_double3(perinc[0], perinc[1], perinc[2]),
+      make_double3(c.domposminx, c.domposminy, c.domposminz), double(c.scell), c.cellcode,
+      d_listp, d_idp, d_code, d_dcell, reinterpret_cast<double2*>(d_posxy), d_posz,
+      reinterpret_cast<float4*>(d_velrho), d_spstau, reinterpret_cast<double2*>(d_posxypre),
+      d_poszpre, reinterpret_cast<float4*>(d_velrhopre), reinterpret_cast<float4*>(d_velrhom1),
+      d_boundnor);
+  DSPH_LAUNCHED(ctx, "k_periodic_duplicate");
   return DSPH_OK;
 }
 

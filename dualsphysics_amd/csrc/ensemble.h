@@ -26,6 +26,9 @@ struct PosCte {
   double mapx, mapy, mapz, sizex, sizey, sizez;
   float scell;
   unsigned cellcode;
+  unsigned periactive;              // periodic axes, bits 1 X, 2 Y, 4 Z (0 in ensembles)
+  double perinc[3][3];              // PeriXinc, PeriYinc, PeriZinc
+  double domx, domy, domz;          // DomPosMin (cells of periodic runs)
 };
 
 // Constant record of one case of an ensemble.

@@ -108,6 +108,7 @@ struct dsph_ens {
       if (cs.tstep != 2) throw Err(DSPH_E_ARG, "ensembles run the Symplectic integrator");
       if (cs.tboundary != DSPH_BC_DBC) throw Err(DSPH_E_ARG, "ensembles run DBC boundaries");
       if (cs.shiftmode != DSPH_SHIFT_NONE) throw Err(DSPH_E_ARG, "ensembles run without shifting");
+      if (cs.periactive) throw Err(DSPH_E_ARG, "ensembles run without periodic boundaries");
       if (cs.casenmoving || cs.casenfloat) throw Err(DSPH_E_ARG, "ensembles have fixed boundaries");
       if (k == 0) { tvisco = cs.tvisco; tddt = cs.tdensity; }
       if (cs.tvisco != tvisco || cs.tdensity != tddt)

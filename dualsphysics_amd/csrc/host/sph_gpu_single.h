@@ -84,6 +84,13 @@ struct dsph_sim {
   double DomPosMin[3];
   unsigned DomCellCode = 0;
   unsigned CaseNbound = 0, CaseNpb = 0;
+  // Periodic boundaries (JSph.cpp:810-826, 2183-2194).
+  unsigned PeriActive = 0;
+  double PeriInc[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  unsigned NpbPer = 0, NpfPer = 0;
+  unsigned* perilist = nullptr;   // NpCap entries
+  unsigned* periwork = nullptr;   // block offsets of the list
+  float* acemod = nullptr;        // |ace|^2 of normal particles
 
   // ---- particle state
   unsigned Np = 0, Npb = 0, NpbOk = 0, NpCap = 0;
@@ -212,14 +219,28 @@ struct dsph_sim {
     DtMin = c.dtmin ? c.dtmin : (double(KernelH) / Cs0) * double(0.05f);
     CaseNbound = c.casenbound;
     CaseNpb = c.casenfixed + c.casenmoving;
+    PeriActive = unsigned(c.periactive) & 7u;
+    if (PeriActive && c.casenmoving)
+      throw Err(DSPH_E_ARG, "periodic boundaries with moving boundaries are not built");
     for (int k = 0; k < 3; k++) {
       MapRealPosMin[k] = c.maprealposmin[k];
       MapRealPosMax[k] = c.maprealposmax[k];
       MapRealSize[k] = MapRealPosMax[k] - MapRealPosMin[k];
       Map_PosMin[k] = MapRealPosMin[k];
       Map_PosMax[k] = MapRealPosMax[k];
-      Map_Size[k] = Map_PosMax[k] - Map_PosMin[k];
     }
+    // LoadCaseParticles (JSph.cpp:2183-2195): the own component of each
+    // periodic increment spans the real map; the map grows by KernelSize on
+    // both sides of a periodic axis.
+    for (int a = 0; a < 3; a++) {
+      for (int k = 0; k < 3; k++) PeriInc[a][k] = (PeriActive & (1u << a)) ? c.perinc[a][k] : 0.0;
+      if (PeriActive & (1u << a)) {
+        PeriInc[a][a] = -MapRealSize[a];
+        Map_PosMin[a] = Map_PosMin[a] - KernelSize;
+        Map_PosMax[a] = Map_PosMax[a] + KernelSize;
+      }
+    }
+    for (int k = 0; k < 3; k++) Map_Size[k] = Map_PosMax[k] - Map_PosMin[k];
     // ConfigPosCellGpu (JSph.cpp:1949-2014): 13/10/9 bits for PosCell.
     PosCellSize = KernelSize;
     {
@@ -259,6 +280,10 @@ struct dsph_sim {
     c.maprealposminx = MapRealPosMin[0]; c.maprealposminy = MapRealPosMin[1];
     c.maprealposminz = MapRealPosMin[2];
     c.maprealsizex = MapRealSize[0]; c.maprealsizey = MapRealSize[1]; c.maprealsizez = MapRealSize[2];
+    c.periactive = PeriActive;
+    for (int k = 0; k < 3; k++) {
+      c.xperinc[k] = PeriInc[0][k]; c.yperinc[k] = PeriInc[1][k]; c.zperinc[k] = PeriInc[2][k];
+    }
     return c;
   }
 
@@ -267,6 +292,18 @@ struct dsph_sim {
             const float* hbn) {
     Np = np; Npb = CaseNpb; NpbOk = Npb;
     NpCap = np;
+    if (PeriActive) {
+      // Room for the periodic copies: twice the particles within KernelSize
+      // of a periodic face at load, plus a margin.
+      size_t near = 0;
+      for (unsigned p = 0; p < np; p++)
+        for (int a = 0; a < 3; a++)
+          if (PeriActive & (1u << a)) {
+            const double x = hpos[3 * size_t(p) + a];
+            if (x < MapRealPosMin[a] + KernelSize || x >= MapRealPosMax[a] - KernelSize) near++;
+          }
+      NpCap = unsigned(std::min<size_t>(size_t(np) + 2 * near + 4096, 0x7fffffffu));
+    }
     std::vector<uint16_t> hcode(np);
     std::vector<unsigned> hdcell(np);
     std::vector<double> hxy(2 * size_t(np)), hz(np);
@@ -286,6 +323,11 @@ struct dsph_sim {
                             unsigned((y - DomPosMin[1]) / Scell), unsigned((z - DomPosMin[2]) / Scell));
     }
     const size_t n = NpCap;
+    if (PeriActive) {
+      perilist = alloc<unsigned>(n);
+      periwork = alloc<unsigned>((n + 255) / 256 + 1);
+      acemod = alloc<float>(n);
+    }
     idp = alloc<unsigned>(n); idp2 = alloc<unsigned>(n);
     dcell = alloc<unsigned>(n); dcell2 = alloc<unsigned>(n);
     code = alloc<uint16_t>(n); code2 = alloc<uint16_t>(n);
@@ -321,25 +363,26 @@ struct dsph_sim {
     begincell_cap = 2ull * size_t(DomCells.x) * DomCells.y * DomCells.z + 5;
     begincell = alloc<int>(2 * begincell_cap);
     chk(dsph_ctx_reserve(ctx, NpCap, unsigned(begincell_cap)), "dsph_ctx_reserve");
-    chk(dsph_memcpy_h2d(ctx, idp, hidp, 4 * n), "upload idp");
-    chk(dsph_memcpy_h2d(ctx, code, hcode.data(), 2 * n), "upload code");
-    chk(dsph_memcpy_h2d(ctx, dcell, hdcell.data(), 4 * n), "upload dcell");
-    chk(dsph_memcpy_h2d(ctx, posxy, hxy.data(), 16 * n), "upload posxy");
-    chk(dsph_memcpy_h2d(ctx, posz, hz.data(), 8 * n), "upload posz");
-    chk(dsph_memcpy_h2d(ctx, velrho, hvr, 16 * n), "upload velrho");
-    if (velrhom1) chk(dsph_memcpy_h2d(ctx, velrhom1, hvr, 16 * n), "upload velrhom1");
+    const size_t n0 = np;
+    chk(dsph_memcpy_h2d(ctx, idp, hidp, 4 * n0), "upload idp");
+    chk(dsph_memcpy_h2d(ctx, code, hcode.data(), 2 * n0), "upload code");
+    chk(dsph_memcpy_h2d(ctx, dcell, hdcell.data(), 4 * n0), "upload dcell");
+    chk(dsph_memcpy_h2d(ctx, posxy, hxy.data(), 16 * n0), "upload posxy");
+    chk(dsph_memcpy_h2d(ctx, posz, hz.data(), 8 * n0), "upload posz");
+    chk(dsph_memcpy_h2d(ctx, velrho, hvr, 16 * n0), "upload velrho");
+    if (velrhom1) chk(dsph_memcpy_h2d(ctx, velrhom1, hvr, 16 * n0), "upload velrhom1");
     if (spstau) chk(dsph_memset_async(ctx, spstau, 0, 24 * n), "memset tau");
     if (boundnor) {
       // ConfigBoundNormals: normals are stored doubled (JSph.cpp:1385-1434).
       // hbn is indexed by id (boundary ids 0..CaseNbound-1); the particles
       // may come in any order with the boundary block first (a resumed PART
       // keeps the solver's order).
-      std::vector<float> bn(3 * n, 0.f);
+      std::vector<float> bn(3 * n0, 0.f);
       if (hbn)
         for (unsigned p = 0; p < np; p++)
           if (hidp[p] < CaseNbound)
             for (int k = 0; k < 3; k++) bn[3 * size_t(p) + k] = hbn[3 * size_t(hidp[p]) + k] * 2.f;
-      chk(dsph_memcpy_h2d(ctx, boundnor, bn.data(), 12 * n), "upload boundnor");
+      chk(dsph_memcpy_h2d(ctx, boundnor, bn.data(), 12 * n0), "upload boundnor");
     }
     std::vector<double> dt0(8, 0.0);
     dt0[2] = DtIni;
@@ -360,7 +403,23 @@ struct dsph_sim {
     cmin = u3{mn[0], mn[1], mn[2]};
     cmax = u3{mx[0], mx[1], mx[2]};
   }
-  void calc_cell_domain(unsigned npb1, unsigned npf1) {
+  // Bound limits over [0,npb1) and the bound copies [npb1+npf1, +npb2);
+  // fluid over [npb1, +npf1) and the fluid copies after them
+  // (JCellDivGpu::CalcCellDomainBound/Fluid, JCellDivGpu.cpp:250-300).
+  void limits2(unsigned n, unsigned pini, unsigned n2, unsigned pini2, u3& cmin, u3& cmax) {
+    u3 mn, mx;
+    limits(n, pini, mn, mx);
+    if (mn.x > mx.x) { mn = DomCells; mx = u3{0, 0, 0}; }
+    if (n2) {
+      u3 mn2, mx2;
+      limits(n2, pini2, mn2, mx2);
+      if (mn2.x > mx2.x) { mn2 = DomCells; mx2 = u3{0, 0, 0}; }
+      mn = u3{std::min(mn.x, mn2.x), std::min(mn.y, mn2.y), std::min(mn.z, mn2.z)};
+      mx = u3{std::max(mx.x, mx2.x), std::max(mx.y, mx2.y), std::max(mx.z, mx2.z)};
+    }
+    cmin = mn; cmax = mx;
+  }
+  void calc_cell_domain(unsigned npb1, unsigned npf1, unsigned npb2 = 0, unsigned npf2 = 0) {
     if (CellDomFixed) {
       CellDomainMin = u3{0, 0, 0};
       CellDomainMax = u3{DomCells.x - 1, DomCells.y - 1, DomCells.z - 1};
@@ -369,7 +428,7 @@ struct dsph_sim {
     u3 cbmin, cbmax;
     if (!BoundLimitOk) {
       u3 mn, mx;
-      limits(npb1, 0, mn, mx);
+      limits2(npb1, 0, npb2, npb1 + npf1, mn, mx);
       const bool none = mn.x > mx.x;
       cbmin = none ? DomCells : mn;
       cbmax = none ? u3{0, 0, 0} : mx;
@@ -380,7 +439,7 @@ struct dsph_sim {
     u3 cfmin, cfmax;
     {
       u3 mn, mx;
-      limits(npf1, npb1, mn, mx);
+      limits2(npf1, npb1, npf2, npb1 + npf1 + npb2, mn, mx);
       const bool none = mn.x > mx.x;
       cfmin = none ? DomCells : mn;
       cfmax = none ? u3{0, 0, 0} : mx;
@@ -405,14 +464,14 @@ struct dsph_sim {
   }
 
   // ---- JCellDivGpuSingle::Divide (JCellDivGpuSingle.cpp:152-236)
-  void divide(unsigned npb1, unsigned npf1, bool boundchanged) {
-    const unsigned nptot = npb1 + npf1;
-    if (boundchanged) {
+  void divide(unsigned npb1, unsigned npf1, unsigned npb2, unsigned npf2, bool boundchanged) {
+    const unsigned nptot = npb1 + npf1 + npb2 + npf2;
+    if (boundchanged || PeriActive) {
       BoundLimitOk = BoundDivideOk = false;
       BoundLimitCellMin = BoundLimitCellMax = u3{0, 0, 0};
       BoundDivideCellMin = BoundDivideCellMax = u3{0, 0, 0};
     }
-    calc_cell_domain(npb1, npf1);
+    calc_cell_domain(npb1, npf1, npb2, npf2);
     Ncx = CellDomainMax.x - CellDomainMin.x + 1;
     Ncy = CellDomainMax.y - CellDomainMin.y + 1;
     Ncz = CellDomainMax.z - CellDomainMin.z + 1;
@@ -453,7 +512,7 @@ struct dsph_sim {
     const unsigned npbOutIgnore = unsigned(be[5] - be[4]);
     const unsigned npfOutIgnore = unsigned(be[7] - be[6]);
     NpFinal = nptot - NpbOut - NpfOut - npbOutIgnore - npfOutIgnore;
-    NpbFinal = npb1 - npbOutIgnore;
+    NpbFinal = npb1 + npb2 - npbOutIgnore;
     if (NpbOut != 0 && DivideFull) NpbFinal = UINT_MAX;
     Ndiv++;
   }
@@ -477,9 +536,49 @@ struct dsph_sim {
   static void swp(T*& a, T*& b) { std::swap(a, b); }
 
   // ---- JSphGpuSingle::RunCellDivide (JSphGpuSingle.cpp:372-499)
+  // ---- JSphGpuSingle::RunPeriodic (JSphGpuSingle.cpp:276-366): the copies
+  // of the last divide are marked to be ignored and new ones are appended
+  // at Np, boundary copies first; for each periodic axis the copies made
+  // for earlier axes are examined before the original particles, so that
+  // corners get copies too.
+  void run_periodic() {
+    chk(dsph_periodic_ignore(ctx, Np, code), "periodic ignore");
+    const unsigned npb0 = Npb, npf0 = Np - Npb;
+    NpbPer = NpfPer = 0;
+    BoundChanged = true;
+    for (unsigned ctype = 0; ctype < 2; ctype++) {
+      const unsigned pini = ctype ? npb0 : 0u, num = ctype ? npf0 : npb0;
+      for (unsigned a = 0; a < 3; a++) {
+        if (!(PeriActive & (1u << a))) continue;
+        for (unsigned cblock = 0; cblock < 2; cblock++) {
+          const unsigned nper = ctype ? NpfPer : NpbPer;
+          const unsigned pini2 = cblock ? pini : Np - nper, num2 = cblock ? num : nper;
+          if (!num2) continue;
+          const unsigned nmax = NpCap - Np;
+          unsigned count = 0;
+          chk(dsph_periodic_make_list(ctx, num2, pini2, nmax, Map_PosMin, Map_PosMax, PeriInc[a],
+                                      posxy, posz, code, periwork, perilist, &count),
+              "periodic list");
+          if (count > nmax)
+            throw Err(DSPH_E_NOMEM, "periodic copies exceed the particle capacity");
+          chk(dsph_periodic_duplicate(ctx, count, Np, &DomCells.x, PeriInc[a], perilist, idp, code,
+                                      dcell, posxy, posz, velrho, spstau,
+                                      pre_active ? posxypre : nullptr,
+                                      pre_active ? poszpre : nullptr,
+                                      pre_active ? velrhopre : nullptr,
+                                      TStep == 1 ? velrhom1 : nullptr, boundnor),
+              "periodic duplicate");
+          Np += count;
+          if (!ctype) NpbPer += count; else NpfPer += count;
+        }
+      }
+    }
+  }
+
   void run_cell_divide() {
+    if (PeriActive) run_periodic();
     const unsigned nptot = Np;
-    divide(Npb, Np - Npb, BoundChanged);
+    divide(Npb, Np - Npb - NpbPer - NpfPer, NpbPer, NpfPer, BoundChanged);
     const unsigned pini = DivideFull ? 0u : NpbFinal;
     chk(dsph_sort_data_basic_poscell(ctx, nptot, pini, sortpart, idp, code, dcell, posxy, posz,
                                      velrho, idp2, code2, dcell2, posxy2, posz2, velrho2,
@@ -571,7 +670,10 @@ struct dsph_sim {
         chk(dsph_sps_tau(ctx, Np, Npb, SpsSmag, SpsBlin, velrho, sps2strain, spstau), "sps tau");
     }
     chk(dsph_redu_max_f32(ctx, Np, 0, viscdt, d_red + 1), "viscdtmax");
-    chk(dsph_acemax2(ctx, Np - Npb, Npb, ace, d_red + 2), "acemax");
+    if (PeriActive)
+      chk(dsph_acemax2_normal(ctx, Np - Npb, Npb, ace, code, acemod, d_red + 2), "acemax");
+    else
+      chk(dsph_acemax2(ctx, Np - Npb, Npb, ace, d_red + 2), "acemax");
   }
 
   // ---- JSphGpu::DtVariable (JSphGpu.cpp:1103-1147)

@@ -280,19 +280,40 @@ __global__ void __launch_bounds__(kBs) k_verlet(unsigned n, unsigned npb,
 
 // ---------------------------------------------------------------- positions
 
-// KerUpdatePos (JSphGpu_ker.cu:334-392) without periodic boundaries.
+// KerUpdatePos (JSphGpu_ker.cu:334-392). With periodic axes a particle
+// that leaves the real map through a periodic face re-enters by the axis'
+// increment, and its cell is measured from DomPosMin.
 __device__ inline void update_pos(double2 rxy, double rz, double mx, double my, double mz,
                                   bool outrho, unsigned p, const PosCte& c,
                                   double2* __restrict__ posxy, double* __restrict__ posz,
                                   unsigned* __restrict__ dcell, typecode* __restrict__ code) {
   const bool outmov =
       (fmaxf(fabsf(float(mx)), fmaxf(fabsf(float(my)), fabsf(float(mz)))) > c.movlimit);
-  const double x = rxy.x + mx, y = rxy.y + my, z = rz + mz;
-  const double dx = x - c.mapx, dy = y - c.mapy, dz = z - c.mapz;
-  const bool out = (dx != dx || dy != dy || dz != dz || dx < 0 || dy < 0 || dz < 0 ||
-                    dx >= c.sizex || dy >= c.sizey || dz >= c.sizez);
+  double x = rxy.x + mx, y = rxy.y + my, z = rz + mz;
+  double dx = x - c.mapx, dy = y - c.mapy, dz = z - c.mapz;
+  bool out = (dx != dx || dy != dy || dz != dz || dx < 0 || dy < 0 || dz < 0 ||
+              dx >= c.sizex || dy >= c.sizey || dz >= c.sizez);
+  if (c.periactive && out) {
+    const bool px = (c.periactive & 1), py = (c.periactive & 2), pz = (c.periactive & 4);
+    if (px) {
+      if (dx < 0) { dx -= c.perinc[0][0]; dy -= c.perinc[0][1]; dz -= c.perinc[0][2]; }
+      if (dx >= c.sizex) { dx += c.perinc[0][0]; dy += c.perinc[0][1]; dz += c.perinc[0][2]; }
+    }
+    if (py) {
+      if (dy < 0) { dx -= c.perinc[1][0]; dy -= c.perinc[1][1]; dz -= c.perinc[1][2]; }
+      if (dy >= c.sizey) { dx += c.perinc[1][0]; dy += c.perinc[1][1]; dz += c.perinc[1][2]; }
+    }
+    if (pz) {
+      if (dz < 0) { dx -= c.perinc[2][0]; dy -= c.perinc[2][1]; dz -= c.perinc[2][2]; }
+      if (dz >= c.sizez) { dx += c.perinc[2][0]; dy += c.perinc[2][1]; dz += c.perinc[2][2]; }
+    }
+    out = (!px && (dx < 0 || dx >= c.sizex)) || (!py && (dy < 0 || dy >= c.sizey)) ||
+          (!pz && (dz < 0 || dz >= c.sizez));
+    x = dx + c.mapx; y = dy + c.mapy; z = dz + c.mapz;
+  }
   posxy[p] = make_double2(x, y);
   posz[p] = z;
+  if (c.periactive) { dx = x - c.domx; dy = y - c.domy; dz = z - c.domz; }
   if (outrho || outmov || out) {
     unsigned rc = code[p];
     if (out) rc = code_set(rc, kCodeOutPos);
@@ -467,7 +488,9 @@ __global__ void __launch_bounds__(kBs) k_step_pos2(unsigned n, unsigned pini, Po
   const unsigned pt = blockIdx.x * blockDim.x + threadIdx.x;
   if (pt >= n) return;
   const unsigned p = pt + pini;
-  const bool outrho = code_is_outrho(code[p]);
+  const unsigned rcode = code[p];
+  const bool outrho = code_is_outrho(rcode);
+  if (c.periactive && !outrho && !code_is_normal(rcode)) return;  // periodic copies stay
   const double2 m = movxy[p];
   update_pos(posxypre[p], poszpre[p], m.x, m.y, movz[p], outrho, p, c, posxy, posz, dcell, code);
 }
@@ -479,19 +502,37 @@ __global__ void __launch_bounds__(kBs) k_step_pos(unsigned n, unsigned pini, Pos
   const unsigned pt = blockIdx.x * blockDim.x + threadIdx.x;
   if (pt >= n) return;
   const unsigned p = pt + pini;
-  const bool outrho = code_is_outrho(code[p]);
+  const unsigned rcode = code[p];
+  const bool outrho = code_is_outrho(rcode);
+  if (c.periactive && !outrho && !code_is_normal(rcode)) return;  // periodic copies stay
   const double2 m = movxy[p];
   update_pos(posxy[p], posz[p], m.x, m.y, movz[p], outrho, p, c, posxy, posz, dcell, code);
 }
 
 PosCte make_posc(const dsph_cte& c) {
-  PosCte r;
+  PosCte r{};
   r.movlimit = c.movlimit;
   r.mapx = c.maprealposminx; r.mapy = c.maprealposminy; r.mapz = c.maprealposminz;
   r.sizex = c.maprealsizex; r.sizey = c.maprealsizey; r.sizez = c.maprealsizez;
   r.scell = c.scell;
   r.cellcode = c.cellcode;
+  r.periactive = c.periactive;
+  for (int k = 0; k < 3; k++) {
+    r.perinc[0][k] = c.xperinc[k]; r.perinc[1][k] = c.yperinc[k]; r.perinc[2][k] = c.zperinc[k];
+  }
+  r.domx = c.domposminx; r.domy = c.domposminy; r.domz = c.domposminz;
   return r;
+}
+
+// ComputeAceMod with codes (JSphGpu_ker.cu:260-275): |ace|^2 of normal
+// particles, 0 for periodic copies.
+__global__ void __launch_bounds__(kBs) k_acemod_normal(unsigned n, unsigned pini,
+    const float* __restrict__ ace, const typecode* __restrict__ code, float* __restrict__ out) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned p = i + pini;
+  const float ax = ace[3 * size_t(p)], ay = ace[3 * size_t(p) + 1], az = ace[3 * size_t(p) + 2];
+  out[i] = code_is_normal(code[p]) ? ax * ax + ay * ay + az * az : 0.f;
 }
 
 // ---------------------------------------------------------------- ensembles
@@ -707,6 +748,16 @@ int dsph_velmax2(dsph_ctx* ctx, uint32_t n, uint32_t pini, const float* d_velrho
 int dsph_acemax2(dsph_ctx* ctx, uint32_t n, uint32_t pini, const float* d_ace, float* d_out) {
   if (!ctx || !d_out) return DSPH_E_ARG;
   return redu_max<2>(ctx, n, pini, d_ace, d_out);
+}
+
+int dsph_acemax2_normal(dsph_ctx* ctx, uint32_t n, uint32_t pini, const float* d_ace,
+                        const uint16_t* d_code, float* d_work, float* d_out) {
+  if (!ctx || !d_out) return DSPH_E_ARG;
+  if (!n) return redu_max<0>(ctx, 0, 0, d_work, d_out);
+  DSPH_REQUIRE(ctx, d_ace && d_code && d_work, "dsph_acemax2_normal: null argument");
+  k_acemod_normal<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(n, pini, d_ace, d_code, d_work);
+  DSPH_LAUNCHED(ctx, "k_acemod_normal");
+  return redu_max<0>(ctx, n, 0, d_work, d_out);
 }
 
 int dsph_redu_max_f32(dsph_ctx* ctx, uint32_t n, uint32_t ini, const float* d_data,

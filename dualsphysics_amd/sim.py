@@ -138,7 +138,8 @@ class SphGpuSingle:
         return {f: getattr(st, f) for f, _ in N.DsphStats._fields_}
 
     def download(self):
-        """Current particles in sorted order: dict of idp, pos, velrho, code."""
+        """Current particles in sorted order: dict of idp, pos, velrho, code
+        (the periodic copies left out, as the reference's PART files)."""
         n = self.stats()["np"]
         idp = np.empty(n, np.uint32)
         pos = np.empty((n, 3), np.float64)
@@ -147,6 +148,9 @@ class SphGpuSingle:
         cnt = C.c_uint32()
         self._check(self.lib.dsph_sim_download(self._h, C.byref(cnt), idp.ctypes.data_as(N.u32p),
                                                _ptr(pos), _ptr(vr), _ptr(code)))
+        keep = (code & 0xe000) != 0x2000
+        if not keep.all():
+            idp, pos, vr, code = idp[keep], pos[keep], vr[keep], code[keep]
         return {"idp": idp, "pos": pos, "velrho": vr, "code": code}
 
     def forces(self):

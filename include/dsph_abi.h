@@ -101,6 +101,11 @@ typedef struct dsph_cte {
   double domposminx, domposminy, domposminz;
   double maprealposminx, maprealposminy, maprealposminz;
   double maprealsizex, maprealsizey, maprealsizez;
+  /* Periodic boundaries (JSph.cpp:810-826, 2183-2194): bits 1 X, 2 Y, 4 Z;
+     the increment applied to a particle leaving through the upper face of
+     each periodic axis (PeriXinc, PeriYinc, PeriZinc). */
+  uint32_t periactive, pad_peri;
+  double xperinc[3], yperinc[3], zperinc[3];
 } dsph_cte;
 
 /*
@@ -217,6 +222,36 @@ int dsph_presort_fluid(dsph_ctx* ctx, uint32_t npf, uint32_t pini,
                        const uint16_t* d_code, const uint32_t cellmin[3],
                        const uint32_t ncells[3], uint32_t* d_cellpart,
                        uint32_t* d_sortpart);
+
+/* ---- Periodic boundaries (cusph, JSphGpu_ker.cu:2106-2400) ---- */
+/* Replaces cusph::PeriodicIgnore (JSphGpu_ker.cu:2113-2132): marks the
+ * periodic copies of [0,n) CODE_OUTIGNORE. */
+int dsph_periodic_ignore(dsph_ctx* ctx, uint32_t n, uint16_t* d_code);
+/* Replaces cusph::PeriodicMakeList (JSphGpu_ker.cu:2142-2213): list of the
+ * particles of [pini,pini+n) (normal or periodic codes) whose position plus
+ * perinc (entry p) or minus perinc (entry p|0x80000000) falls inside
+ * [mapposmin,mapposmax). The list is in particle order, the plus entry of a
+ * particle before its minus entry (the order of the CPU build,
+ * JSphCpuSingle.cpp:210-240). d_work holds ceil(n/256)+1 words; at most nmax
+ * entries are written and the full count is returned in *h_count (host
+ * memory; the call synchronises). */
+int dsph_periodic_make_list(dsph_ctx* ctx, uint32_t n, uint32_t pini, uint32_t nmax,
+                            const double mapposmin[3], const double mapposmax[3],
+                            const double perinc[3], const double* d_posxy, const double* d_posz,
+                            const uint16_t* d_code, uint32_t* d_work, uint32_t* d_listp,
+                            uint32_t* h_count);
+/* Replaces cusph::PeriodicDuplicateSymplectic / PeriodicDuplicateVerlet /
+ * PeriodicDuplicateNormals (JSphGpu_ker.cu:2229-2400): creates particles
+ * [pini,pini+n) from the list: position +-perinc, cell from the constants'
+ * DomPosMin clamped to domcells-1, the same id, a periodic code, velrho and,
+ * when not null, the Symplectic pre state, the Verlet velrhom1, the SPS tau
+ * and the mDBC normals. */
+int dsph_periodic_duplicate(dsph_ctx* ctx, uint32_t n, uint32_t pini, const uint32_t domcells[3],
+                            const double perinc[3], const uint32_t* d_listp, uint32_t* d_idp,
+                            uint16_t* d_code, uint32_t* d_dcell, double* d_posxy, double* d_posz,
+                            float* d_velrho, float* d_spstau, double* d_posxypre,
+                            double* d_poszpre, float* d_velrhopre, float* d_velrhom1,
+                            float* d_boundnor);
 
 /* Replaces cudiv::Sort (JCellDivGpu_ker.h:40): in-place sort of (keys,values)
  * by key, keys < key_limit. Always stable (the reference's -stable mode):
@@ -419,6 +454,11 @@ int dsph_velmax2(dsph_ctx* ctx, uint32_t n, uint32_t pini,
  * |ace|^2 over [pini,pini+n). */
 int dsph_acemax2(dsph_ctx* ctx, uint32_t n, uint32_t pini,
                  const float* d_ace, float* d_out);
+/* Replaces cusph::ComputeAceMod with codes + ReduMaxFloat (JSphGpu_ker.cu:260-275):
+ * max of |ace|^2 over the normal particles of [pini,pini+n) (periodic copies
+ * left out); d_work holds n floats. */
+int dsph_acemax2_normal(dsph_ctx* ctx, uint32_t n, uint32_t pini, const float* d_ace,
+                        const uint16_t* d_code, float* d_work, float* d_out);
 /* Replaces cusph::ReduMaxFloat (JSphGpu_ker.h:281): max of data[ini..ini+n)
  * (0 when n==0) to *d_out. */
 int dsph_redu_max_f32(dsph_ctx* ctx, uint32_t n, uint32_t ini,
@@ -601,6 +641,10 @@ typedef struct dsph_case {
                                     DSPH_SLIP_NOSLIP (fixed boundaries only) */
   int32_t shiftmode;             /* Shifting (JSph.cpp:744-760): DSPH_SHIFT_* */
   double shiftcoef, shifttfs;    /* ShiftCoef (-2 default), ShiftTFS (0: off) */
+  int32_t periactive;            /* periodic axes, bits 1 X, 2 Y, 4 Z (JSph.cpp:810-826) */
+  double perinc[3][3];           /* XPeriodicIncY/Z, YPeriodicIncX/Z, ZPeriodicIncX/Y: the
+                                    increments [axis][component]; each axis' own
+                                    component is set to -MapRealSize (JSph.cpp:2186-2188) */
 } dsph_case;
 
 /* Per-step statistics, mirror of the Run.out/Run.csv key numbers. */

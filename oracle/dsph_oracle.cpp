@@ -63,6 +63,8 @@ struct orc_cte {
   double domposmin[3], maprealposmin[3], maprealsize[3];
   double mapposmin[3];
   unsigned tboundary;  // 2 = mDBC; otherwise DBC
+  unsigned periactive;        // periodic axes, bits 1 X, 2 Y, 4 Z
+  double perinc[3][3];         // PeriXinc, PeriYinc, PeriZinc
 };
 
 // Neighbour-search data (StDivDataCpu / StDivDataGpu).
@@ -943,6 +945,8 @@ void orc_step_pos(unsigned np, unsigned npb, const orc_cte* c, const double* pos
                   const double* mov, double* pos, unsigned* dcell, typecode* code) {
   for (unsigned p = npb; p < np; p++) {
     const bool outrho = (code[p] & kSpecial) == kOutRho;
+    // Periodic copies keep their position (KerComputeStepPos2 'normal').
+    if (c->periactive && !outrho && (code[p] & kSpecial) != 0) continue;
     const double* m = mov + 3 * size_t(p);
     const bool outmov = (std::fabs(float(m[0])) > c->movlimit || std::fabs(float(m[1])) > c->movlimit ||
                          std::fabs(float(m[2])) > c->movlimit);
@@ -954,6 +958,21 @@ void orc_step_pos(unsigned np, unsigned npb, const orc_cte* c, const double* pos
       dd[k] = x[k] - c->maprealposmin[k];
       if (dd[k] != dd[k] || dd[k] < 0 || dd[k] >= c->maprealsize[k]) out = true;
     }
+    // UpdatePeriodicPos (JSph.cpp:2076-2101): re-enter through the opposite
+    // face of a periodic axis; the cell is then taken from DomPosMin.
+    if (c->periactive && out) {
+      for (int a = 0; a < 3; a++) {
+        if (!(c->periactive & (1u << a))) continue;
+        if (dd[a] < 0) for (int k = 0; k < 3; k++) dd[k] -= c->perinc[a][k];
+        if (dd[a] >= c->maprealsize[a]) for (int k = 0; k < 3; k++) dd[k] += c->perinc[a][k];
+      }
+      out = false;
+      for (int a = 0; a < 3; a++)
+        if (!(c->periactive & (1u << a)) && (dd[a] < 0 || dd[a] >= c->maprealsize[a])) out = true;
+      for (int k = 0; k < 3; k++) x[k] = dd[k] + c->maprealposmin[k];
+    }
+    if (c->periactive)
+      for (int k = 0; k < 3; k++) dd[k] = x[k] - c->domposmin[k];
     for (int k = 0; k < 3; k++) pos[3 * size_t(p) + k] = x[k];
     if (outrho || outmov || out) {
       unsigned rc = code[p] & ~kSpecial;

@@ -30,6 +30,8 @@ struct orc_cte {
   double domposmin[3], maprealposmin[3], maprealsize[3];
   double mapposmin[3];
   unsigned tboundary;
+  unsigned periactive;        // periodic axes, bits 1 X, 2 Y, 4 Z
+  double perinc[3][3];         // PeriXinc, PeriYinc, PeriZinc
 };
 struct orc_div {
   int scelldiv, ncx, ncy, ncz, ncw;
@@ -113,6 +115,8 @@ struct Sim {
   unsigned NpfOutTotal = 0, Nstep = 0, VerletStep = 0;
   double SymplecticDtPre = 0, TimeStep = 0, DtLast = 0, VelMax = 0, AceMax = 0;
   float ViscDtMax = 0;
+  unsigned NpbPer = 0, NpfPer = 0;
+  double MapPosMax[3] = {0, 0, 0};
 
   void config(const dsph_case& s) {
     cs = s;
@@ -153,12 +157,29 @@ struct Sim {
     }
     ScellDiv = s.cellmode == 2 ? 2 : 1;
     double msize[3];
+    c.periactive = unsigned(s.periactive) & 7u;
+    if (c.periactive && s.casenmoving)
+      throw std::runtime_error("periodic boundaries with moving boundaries are not built");
     for (int k = 0; k < 3; k++) {
       c.maprealposmin[k] = s.maprealposmin[k];
       c.maprealsize[k] = s.maprealposmax[k] - s.maprealposmin[k];
       c.mapposmin[k] = s.maprealposmin[k];
-      c.domposmin[k] = s.maprealposmin[k];
-      msize[k] = c.maprealsize[k];
+      MapPosMax[k] = s.maprealposmax[k];
+    }
+    // JSph.cpp:2183-2195: periodic increments and the map grown by
+    // KernelSize on both sides of each periodic axis.
+    for (int a = 0; a < 3; a++) {
+      const bool on = (c.periactive >> a) & 1u;
+      for (int k = 0; k < 3; k++) c.perinc[a][k] = on ? s.perinc[a][k] : 0.0;
+      if (on) {
+        c.perinc[a][a] = -c.maprealsize[a];
+        c.mapposmin[a] = c.mapposmin[a] - double(ks);
+        MapPosMax[a] = MapPosMax[a] + double(ks);
+      }
+    }
+    for (int k = 0; k < 3; k++) {
+      c.domposmin[k] = c.mapposmin[k];
+      msize[k] = MapPosMax[k] - c.mapposmin[k];
     }
     c.poscellsize = ks;
     {
@@ -222,10 +243,89 @@ struct Sim {
     a.swap(b);
   }
 
+  // JSphCpuSingle::RunPeriodic (JSphCpuSingle.cpp:420-500): the copies of
+  // the last divide are ignored; new ones are appended at Np (boundary
+  // copies first), for each periodic axis from the copies made so far and
+  // then from the original particles; the list is in particle order, the
+  // plus copy of a particle before its minus copy (PeriodicMakeList,
+  // JSphCpuSingle.cpp:210-240); a copy takes the shifted position, the cell
+  // from DomPosMin clamped to the domain, a periodic code and the particle's
+  // state (PeriodicDuplicate*, JSphCpuSingle.cpp:245-330).
+  template <typename T>
+  void grow(std::vector<T>& a, unsigned w, unsigned n) { if (!a.empty()) a.resize(size_t(n) * w); }
+  void copy_row(std::vector<float>& a, unsigned w, unsigned dst, unsigned src) {
+    if (a.empty()) return;
+    for (unsigned k = 0; k < w; k++) a[size_t(dst) * w + k] = a[size_t(src) * w + k];
+  }
+  void run_periodic() {
+    for (unsigned p = 0; p < Np; p++)
+      if ((code[p] & 0xe000u) == 0x2000u) code[p] = typecode((code[p] & ~0xe000u) | 0x4000u);
+    const unsigned npb0 = Npb, npf0 = Np - Npb;
+    NpbPer = NpfPer = 0;
+    BoundChanged = true;
+    for (unsigned ctype = 0; ctype < 2; ctype++) {
+      const unsigned pini = ctype ? npb0 : 0u, num = ctype ? npf0 : npb0;
+      for (unsigned a = 0; a < 3; a++) {
+        if (!((c.periactive >> a) & 1u)) continue;
+        const double* inc = c.perinc[a];
+        for (unsigned cblock = 0; cblock < 2; cblock++) {
+          const unsigned nper = ctype ? NpfPer : NpbPer;
+          const unsigned pini2 = cblock ? pini : Np - nper, num2 = cblock ? num : nper;
+          std::vector<unsigned> list;
+          for (unsigned p2 = pini2; p2 < pini2 + num2; p2++) {
+            if ((code[p2] & 0xe000u) > 0x2000u) continue;
+            const double* x = &pos[3 * size_t(p2)];
+            for (int sg = 0; sg < 2; sg++) {
+              const double q[3] = {sg ? x[0] - inc[0] : x[0] + inc[0],
+                                   sg ? x[1] - inc[1] : x[1] + inc[1],
+                                   sg ? x[2] - inc[2] : x[2] + inc[2]};
+              bool in = true;
+              for (int k = 0; k < 3; k++) in = in && c.mapposmin[k] <= q[k] && q[k] < MapPosMax[k];
+              if (in) list.push_back(sg ? (p2 | 0x80000000u) : p2);
+            }
+          }
+          const unsigned count = unsigned(list.size());
+          if (!count) continue;
+          const unsigned n1 = Np + count;
+          idp.resize(n1); code.resize(n1); dcell.resize(n1);
+          pos.resize(3 * size_t(n1)); velrho.resize(4 * size_t(n1));
+          grow(velrhom1, 4, n1); grow(tau, 6, n1); grow(strain, 6, n1); grow(boundnor, 3, n1);
+          if (pre_active) { grow(pospre, 3, n1); grow(velrhopre, 4, n1); }
+          for (unsigned i = 0; i < count; i++) {
+            const unsigned pn = Np + i, pc = list[i] & 0x7fffffffu;
+            const bool inv = list[i] >= 0x80000000u;
+            unsigned cl[3];
+            for (int k = 0; k < 3; k++) {
+              const double v = pos[3 * size_t(pc) + k] + (inv ? -inc[k] : inc[k]);
+              pos[3 * size_t(pn) + k] = v;
+              const unsigned cc = unsigned((v - c.domposmin[k]) / Scell);
+              cl[k] = std::min(cc, DomCells[k] - 1);
+            }
+            dcell[pn] = mkcell_host(cl[0], cl[1], cl[2]);
+            idp[pn] = idp[pc];
+            code[pn] = typecode((code[pc] & ~0xe000u) | 0x2000u);
+            copy_row(velrho, 4, pn, pc);
+            copy_row(velrhom1, 4, pn, pc);
+            copy_row(tau, 6, pn, pc);
+            copy_row(boundnor, 3, pn, pc);
+            if (pre_active) {
+              for (int k = 0; k < 3; k++) pospre[3 * size_t(pn) + k] = pospre[3 * size_t(pc) + k];
+              copy_row(velrhopre, 4, pn, pc);
+            }
+          }
+          Np = n1;
+          if (!ctype) NpbPer += count; else NpfPer += count;
+        }
+      }
+    }
+  }
+
   // JCellDivCpuSingle::Divide + JSphCpuSingle::RunCellDivide.
   void cell_divide() {
-    const unsigned npb1 = Npb, npf1 = Np - Npb, nptot = Np;
-    if (BoundChanged) { BoundLimitOk = BoundDivideOk = false; }
+    if (c.periactive) run_periodic();
+    const unsigned npb2 = NpbPer, npf2 = NpfPer;
+    const unsigned npb1 = Npb, npf1 = Np - Npb - npb2 - npf2, nptot = Np;
+    if (BoundChanged || c.periactive) { BoundLimitOk = BoundDivideOk = false; }
     if (cs.celldomfixed) {
       for (int k = 0; k < 3; k++) { cmin[k] = 0; cmax[k] = DomCells[k] - 1; }
     } else {
@@ -233,11 +333,13 @@ struct Sim {
       if (!BoundLimitOk) {
         orc_limits(npb1, 0, c.cellcode, dcell.data(), code.data(), bmin, bmax);
         if (bmin[0] > bmax[0]) for (int k = 0; k < 3; k++) { bmin[k] = DomCells[k]; bmax[k] = 0; }
+        merge_limits(npb2, npb1 + npf1, bmin, bmax);
         BoundLimitOk = true;
         std::memcpy(blmin, bmin, 12); std::memcpy(blmax, bmax, 12);
       } else { std::memcpy(bmin, blmin, 12); std::memcpy(bmax, blmax, 12); }
       orc_limits(npf1, npb1, c.cellcode, dcell.data(), code.data(), fmin, fmax);
       if (fmin[0] > fmax[0]) for (int k = 0; k < 3; k++) { fmin[k] = DomCells[k]; fmax[k] = 0; }
+      merge_limits(npf2, npb1 + npf1 + npb2, fmin, fmax);
       const unsigned s = unsigned(ScellDiv);
       for (int k = 0; k < 3; k++) {
         cmin[k] = std::max(std::min(bmin[k], fmin[k]), fmin[k] >= s ? fmin[k] - s : 0u);
@@ -269,7 +371,7 @@ struct Sim {
     const unsigned npbout = csize(bbo), npfout = csize(bbo + 1), npboi = csize(bbo + 2),
                    npfoi = csize(bbo + 3);
     const unsigned npfinal = nptot - npbout - npfout - npboi - npfoi;
-    unsigned npbfinal = npb1 - npboi;
+    unsigned npbfinal = npb1 + npb2 - npboi;
     if (npbout && DivideFull) npbfinal = UINT_MAX;
     const unsigned gpini = DivideFull ? 0 : npbfinal;
     gather(idp, 1, nptot, gpini); gather(code, 1, nptot, gpini); gather(dcell, 1, nptot, gpini);
@@ -284,6 +386,15 @@ struct Sim {
     poscell.resize(4 * size_t(Np));
     orc_poscell(Np, pos.data(), c.mapposmin, c.poscellsize, poscell.data());
     BoundChanged = false;
+  }
+
+  // Second range of CalcCellDomainBound/Fluid (JCellDivGpu.cpp:250-300).
+  void merge_limits(unsigned n2, unsigned pini2, unsigned mn[3], unsigned mx[3]) {
+    if (!n2) return;
+    unsigned a[3], b[3];
+    orc_limits(n2, pini2, c.cellcode, dcell.data(), code.data(), a, b);
+    if (a[0] > b[0]) for (int k = 0; k < 3; k++) { a[k] = DomCells[k]; b[k] = 0; }
+    for (int k = 0; k < 3; k++) { mn[k] = std::min(mn[k], a[k]); mx[k] = std::max(mx[k], b[k]); }
   }
 
   void interaction() {
@@ -312,6 +423,7 @@ struct Sim {
     float vd = 0, a2 = 0;
     for (unsigned p = 0; p < Np; p++) vd = std::max(vd, viscdt[p]);
     for (unsigned p = Npb; p < Np; p++) {
+      if (code[p] & 0xe000u) continue;  // ComputeAceMod: normal particles only
       const float* a = &ace[3 * size_t(p)];
       a2 = std::max(a2, a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
     }

@@ -21,7 +21,7 @@ class OrcCte(C.Structure):
         "eta2", "ddtkh", "ddtgz", "scell", "kernelsize", "dp", "cteb", "gamma", "rhopzero",
         "ovrhopzero", "movlimit")] + [("cellcode", C.c_uint32)] + [
         (n, C.c_double * 3) for n in ("domposmin", "maprealposmin", "maprealsize", "mapposmin")] + [
-        ("tboundary", C.c_uint32)]
+        ("tboundary", C.c_uint32), ("periactive", C.c_uint32), ("perinc", (C.c_double * 3) * 3)]
 
 
 class OrcDiv(C.Structure):
@@ -145,6 +145,9 @@ class OracleSim:
         vr = np.empty((n, 4), np.float32)
         code = np.empty(n, np.uint16)
         self.lib.orc_sim_download(self._h, _p(idp), _p(pos), _p(vr), _p(code))
+        keep = (code & 0xe000) != 0x2000   # periodic copies left out, order kept
+        if not keep.all():
+            idp, pos, vr, code = idp[keep], pos[keep], vr[keep], code[keep]
         return {"idp": idp, "pos": pos, "velrho": vr, "code": code}
 
     def forces(self):

@@ -84,6 +84,13 @@ CASES = {
     # mDBC2 (SlipMode no-slip) on the dam break's fixed walls.
     "ref_dambreak_dp0.03_mdbc2": ("dam_break", 0.03, 10, (1, 5, 10),
                                   dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4)),
+    # Channel periodic in x with the fluid moving at 2 m/s: particles leave
+    # through one face and re-enter through the other (Symplectic + DBC, and
+    # Verlet + mDBC + laminar viscosity).
+    "ref_channel_dp0.02_periodic": ("periodic_channel", 0.02, 60, (1, 30, 60), dict(periodic=1)),
+    "ref_channel_dp0.02_periodic_mdbc_verlet": ("periodic_channel", 0.02, 60, (1, 30, 60),
+                                                dict(periodic=1, tboundary=2, tstep=1, tvisco=3,
+                                                     visco=1e-4)),
 }
 
 

@@ -1,0 +1,91 @@
+"""Periodic boundaries on the GPU (RunPeriodic, JSphGpuSingle.cpp:276-366;
+KerPeriodicMakeList / KerPeriodicDuplicate*, JSphGpu_ker.cu:2106-2400;
+KerUpdatePos with periodic axes, JSphGpu_ker.cu:334-392) against the oracle,
+on a channel periodic in x whose fluid moves at 2 m/s so that particles leave
+through one face and re-enter through the other:
+- particle order and copy count: after every step the GPU's particles, with
+  the periodic copies left out, are in the order the oracle gives the same
+  state (the oracle makes its own copies and sorts them), and both hold the
+  same number of copies;
+- trajectories over 60 steps (host dt and device dt), compared by id, with
+  the particles that crossed a face among them;
+- a channel periodic in x and y (corner copies from copies) with mDBC.
+The comparison with the reference CPU build on the same channel is in
+test_gpu_vs_reference.py.
+"""
+import dataclasses
+
+import numpy as np
+import pytest
+
+from dualsphysics_amd import case as casemod
+
+pytestmark = pytest.mark.gpu
+
+
+def _state_case(c, state):
+    return dataclasses.replace(c, idp=state["idp"].copy(), pos=state["pos"].copy(),
+                               velrho=state["velrho"].copy())
+
+
+def test_order_and_copies_follow_oracle():
+    from dualsphysics_amd.sim import SphGpuSingle
+    from oracle.oracle import OracleSim
+    c = casemod.periodic_channel(dp=0.02)
+    s = SphGpuSingle(c)
+    for step in range(12):
+        s.run(1, sync_dt=True)
+        g = s.download()
+        o = OracleSim(_state_case(c, g))
+        od = o.download()
+        np.testing.assert_array_equal(od["idp"], g["idp"])
+        assert o.stats()["np"] == s.stats()["np"] > c.np   # the copies are there
+        assert o.stats()["npb"] == s.stats()["npb"]
+    s.close()
+
+
+@pytest.mark.parametrize("sync_dt", [True, False], ids=["host_dt", "device_dt"])
+def test_trajectory_matches_oracle(sync_dt):
+    from dualsphysics_amd.sim import SphGpuSingle
+    from oracle.oracle import OracleSim
+    c = casemod.periodic_channel(dp=0.02)
+    s = SphGpuSingle(c)
+    o = OracleSim(c)
+    s.run(60, sync_dt=sync_dt)
+    o.run(60)
+    gs, os_ = s.stats(), o.stats()
+    assert gs["np"] == os_["np"] and gs["npf_out_total"] == os_["npf_out_total"] == 0
+    assert abs(gs["timestep"] - os_["timestep"]) <= 1e-6 * os_["timestep"]
+    g, d = s.download(), o.download()
+    assert len(g["idp"]) == len(d["idp"]) == c.np
+    gi, oi = np.argsort(g["idp"]), np.argsort(d["idp"])
+    crossed = (d["pos"][oi, 0] - c.pos[:, 0]) < -0.1
+    assert crossed.sum() > 100
+    dpos = np.abs(g["pos"][gi] - d["pos"][oi]).max()
+    dvel = np.abs(g["velrho"][gi, :3] - d["velrho"][oi, :3]).max()
+    drho = np.abs(g["velrho"][gi, 3] - d["velrho"][oi, 3]).max()
+    print(f"60 steps ({int(crossed.sum())} crossed): max|dpos| {dpos:.2e} max|dv| {dvel:.2e} "
+          f"max|drho| {drho:.2e}")
+    assert dpos < 1e-4 * c.dp
+    assert dvel < 1e-4 * 2.0
+    assert drho < 1e-2
+
+
+def test_xy_periodic_mdbc_matches_oracle():
+    from dualsphysics_amd.sim import SphGpuSingle
+    from oracle.oracle import OracleSim
+    p = casemod.Parameters(periodic=3, tboundary=2, tvisco=3, visco=1e-4)
+    c = casemod.periodic_channel(dp=0.02, params=p)
+    s = SphGpuSingle(c)
+    o = OracleSim(c)
+    for _ in range(3):
+        s.run(10, sync_dt=True)
+        o.run(10)
+        g, d = s.download(), o.download()
+        np.testing.assert_array_equal(g["idp"], d["idp"])
+        assert s.stats()["np"] == o.stats()["np"]
+    dpos = np.abs(g["pos"] - d["pos"]).max()
+    drho = np.abs(g["velrho"][:, 3] - d["velrho"][:, 3]).max()
+    print(f"xy periodic + mDBC, 30 steps: max|dpos| {dpos:.2e} max|drho| {drho:.2e}")
+    assert dpos < 1e-4 * c.dp and drho < 1e-2
+    s.close()
