@@ -22,10 +22,12 @@ def _cases():
             cm.dam_break(dp=0.03, params=cm.Parameters(tboundary=2, slipmode=2, tvisco=3,
                                                        visco=1e-4, tstep=1, tdensity=2)),
             cm.flume(dp=0.01), cm.flume(dp=0.01, wavemaker="flap", params=mdbc),
-            cm.flume(dp=0.01, wavemaker="flap_rot")]
+            cm.flume(dp=0.01, wavemaker="flap_rot"),
+            cm.periodic_channel(dp=0.02),
+            cm.periodic_channel(dp=0.02, params=cm.Parameters(periodic=3, tboundary=2))]
 
 
-@pytest.mark.parametrize("k", range(5))
+@pytest.mark.parametrize("k", range(7))
 def test_case_round_trip(tmp_path, k):
     c = _cases()[k]
     bi4.write_case(c, tmp_path, "case")
