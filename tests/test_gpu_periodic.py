@@ -9,7 +9,9 @@ through one face and re-enter through the other:
   same number of copies;
 - trajectories over 60 steps (host dt and device dt), compared by id, with
   the particles that crossed a face among them;
-- a channel periodic in x and y (corner copies from copies) with mDBC.
+- a channel periodic in x and y (corner copies from copies) with mDBC;
+- the list kernel alone through the C ABI: entries in particle order over
+  partial and many blocks, skipped codes, truncation at nmax.
 The comparison with the reference CPU build on the same channel is in
 test_gpu_vs_reference.py.
 """
@@ -89,3 +91,50 @@ def test_xy_periodic_mdbc_matches_oracle():
     print(f"xy periodic + mDBC, 30 steps: max|dpos| {dpos:.2e} max|drho| {drho:.2e}")
     assert dpos < 1e-4 * c.dp and drho < 1e-2
     s.close()
+
+
+def _list_expected(pos, code, pini, n, mn, mx, inc):
+    """Entries of the periodic list in particle order, plus before minus."""
+    out = []
+    for p in range(pini, pini + n):
+        if (int(code[p]) & 0xe000) > 0x2000:
+            continue
+        for sg, flag in ((1.0, 0), (-1.0, 0x80000000)):
+            q = pos[p] + sg * inc
+            if np.all(mn <= q) and np.all(q < mx):
+                out.append(p | flag)
+    return np.array(out, np.uint32)
+
+
+@pytest.mark.parametrize("n,pini,nmax", [(0, 0, 16), (1, 0, 16), (255, 3, 4096), (256, 0, 4096),
+                                         (257, 1, 4096), (100003, 7, 1 << 20), (5000, 0, 37)])
+def test_make_list_matches_rule(lib, ctx, n, pini, nmax):
+    """dsph_periodic_make_list over partial and many blocks, with codes that
+    are copied (normal, periodic) and skipped (ignore, out), both faces, and
+    a list longer than nmax (count still complete, entries truncated)."""
+    import ctypes as C
+    from tests.helpers import Dev
+    rng = np.random.default_rng(n + pini)
+    tot = pini + n
+    mn = np.array([-0.1, 0.0, 0.0]); mx = np.array([1.1, 0.5, 0.5])
+    inc = np.array([-1.0, 0.0, 0.0])
+    pos = np.stack([rng.uniform(0.0, 1.0, tot), rng.uniform(0.0, 0.5, tot),
+                    rng.uniform(0.0, 0.5, tot)], 1)
+    code = rng.choice(np.array([0x1800, 0x3800, 0x5800, 0x9800, 0x0000], np.uint16), tot)
+    posxy = np.ascontiguousarray(pos[:, :2]).ravel() if tot else np.zeros(2)
+    posz = np.ascontiguousarray(pos[:, 2]) if tot else np.zeros(1)
+    d_xy, d_z = Dev(lib, ctx, posxy), Dev(lib, ctx, posz)
+    d_code = Dev(lib, ctx, code if tot else np.zeros(1, np.uint16))
+    d_work = Dev(lib, ctx, np.zeros((n + 255) // 256 + 1, np.uint32))
+    d_list = Dev(lib, ctx, np.full(nmax, 0xdeadbeef, np.uint32))
+    cnt = C.c_uint32(12345)
+    rc = lib.dsph_periodic_make_list(ctx, n, pini, nmax, mn.ctypes.data, mx.ctypes.data,
+                                     inc.ctypes.data, d_xy.ptr, d_z.ptr, d_code.ptr, d_work.ptr,
+                                     d_list.ptr, C.byref(cnt))
+    assert rc == 0, lib.dsph_last_error(ctx)
+    exp = _list_expected(pos, code, pini, n, mn, mx, inc)
+    assert cnt.value == exp.size
+    got = d_list.get()
+    k = min(exp.size, nmax)
+    np.testing.assert_array_equal(got[:k], exp[:k])
+    assert np.all(got[k:] == 0xdeadbeef)
