@@ -293,8 +293,10 @@ struct dsph_sim {
     Np = np; Npb = CaseNpb; NpbOk = Npb;
     NpCap = np;
     if (PeriActive) {
-      // Room for the periodic copies: twice the particles within KernelSize
-      // of a periodic face at load, plus a margin.
+      // Room for the periodic copies: until the divide drops them, the
+      // copies of the last divide stay beside the new ones, so twice the
+      // copies are held; each is a particle within KernelSize of a periodic
+      // face (counted at load), given twice that again as headroom.
       size_t near = 0;
       for (unsigned p = 0; p < np; p++)
         for (int a = 0; a < 3; a++)
@@ -302,7 +304,7 @@ struct dsph_sim {
             const double x = hpos[3 * size_t(p) + a];
             if (x < MapRealPosMin[a] + KernelSize || x >= MapRealPosMax[a] - KernelSize) near++;
           }
-      NpCap = unsigned(std::min<size_t>(size_t(np) + 2 * near + 4096, 0x7fffffffu));
+      NpCap = unsigned(std::min<size_t>(size_t(np) + 4 * near + 4096, 0x7fffffffu));
     }
     std::vector<uint16_t> hcode(np);
     std::vector<unsigned> hdcell(np);
