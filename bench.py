@@ -155,18 +155,26 @@ def ensemble_bench(n: int, tend: float, sample: int):
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t1)
         sim.close()
-    plain_total = float(np.mean(times)) * n
-    log(f"  plain: {len(pick)} sampled cases, mean {np.mean(times)*1e3:.1f} ms/case, "
-        f"estimated {plain_total:.1f}s for all {n}")
+    if len(pick) == n:
+        plain_total = float(np.sum(times))
+        plain_se = 0.0
+        how = f"all {n} cases timed"
+    else:
+        # Size-stratified sample: mean time x cases, with the standard error
+        # of that estimate.
+        plain_total = float(np.mean(times)) * n
+        plain_se = float(np.std(times, ddof=1) / np.sqrt(len(times))) * n if len(times) > 1 else None
+        how = f"{len(pick)} sampled cases, estimated for all {n}"
+    log(f"  plain: {how}: {plain_total:.1f}s, mean {np.mean(times)*1e3:.1f} ms/case")
     return {"cases": n, "particles": int(sizes.sum()), "np_min": int(sizes.min()),
             "np_max": int(sizes.max()), "tend_s": tend, "steps": steps, "seconds": el,
             "cases_per_s": n / el, "plain_cases_per_s": n / plain_total,
             "speedup": plain_total / el, "particle_steps_per_s": part_steps / el,
             "cases_completed": stopped, "plain_sample": len(pick),
-            "plain_seconds_estimated": plain_total, "autotune": chosen,
-            "steps_per_graph": info["steps_per_graph"],
-            "note": "plain = one SphGpuSingle run per case to the same end time "
-                    "(size-stratified sample, mean time x cases)"}
+            "plain_seconds": plain_total, "plain_seconds_stderr": plain_se,
+            "autotune": chosen, "steps_per_graph": info["steps_per_graph"],
+            "note": "plain = one SphGpuSingle run per case to the same end time, set-up "
+                    "untimed (" + how + ")"}
 
 
 def flume_bench(steps: int, warmup: int, dp: float = 0.005):
@@ -215,7 +223,10 @@ def main():
     ap.add_argument("--dp", type=float, default=0.0025)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-timing", action="store_true", help="disable per-kernel HIP events")
+    ap.add_argument("--no-timing", action="store_true",
+                    help="skip the instrumented window (per-kernel HIP events)")
+    ap.add_argument("--timed-steps", type=int, default=200,
+                    help="steps of the instrumented window after the headline window")
     ap.add_argument("--forces-kernel", choices=["tile", "rows"], default="tile")
     ap.add_argument("--cellmode", choices=["full", "half"], default="full",
                     help="cell size 2h (the reference's default) or h (-cellmode:half)")
@@ -223,8 +234,8 @@ def main():
                     help="cases of the batched-ensemble measurement (0: skip)")
     ap.add_argument("--ens-tend", type=float, default=0.01,
                     help="physical end time of every ensemble case [s]")
-    ap.add_argument("--ens-plain-sample", type=int, default=32,
-                    help="cases timed on the plain single-case path")
+    ap.add_argument("--ens-plain-sample", type=int, default=512,
+                    help="cases timed on the plain single-case path (all by default)")
     ap.add_argument("--flume-steps", type=int, default=0, metavar="N",
                     help="also time N steps of the full-size wave flume (~58M fluid particles)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
@@ -267,9 +278,7 @@ def main():
     sim.run(a.warmup)
     torch.cuda.synchronize()
     p0 = sim.pips()
-    if not a.no_timing:
-        lib.dsph_ctx_timing_reset(ctx)
-        lib.dsph_ctx_set_timing(ctx, 1)
+    # Headline window: kernel timing off (no HIP events between launches).
     log(f"timing {a.steps} steps ...")
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -282,33 +291,50 @@ def main():
         log(f"  {done}/{a.steps} steps")
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
-    kt = kernel_times(lib, ctx) if not a.no_timing else {}
-    lib.dsph_ctx_set_timing(ctx, 0)
     p1 = sim.pips()
     st = sim.stats()
     pairs_step = 2 * ((p0[0] + p0[1]) + (p1[0] + p1[1])) / 2
     value = pairs_step * a.steps / elapsed
     ms_step = 1e3 * elapsed / a.steps
 
+    # Second window with per-kernel HIP events on the context stream: kernel
+    # averages and the roofline of the fluid interaction kernel.
+    kt, kt_steps, ms_step_timed = {}, 0, None
     roof = None
-    if "k_forces_fluid" in kt:
-        cnt, ms = kt["k_forces_fluid"]
-        ff = ((p0[0] - p0[4]) + (p1[0] - p1[4])) / 2
-        fb = (p0[4] + p1[4]) / 2
-        flops = ff * F_FF + fb * F_FB
-        avg_s = ms / cnt / 1e3
-        ach = flops / avg_s / 1e12
-        traffic = None
-        tfile = ROOT / "profiles" / f"k_forces_fluid_traffic_dam_break_dp{a.dp:g}.json"
-        if tfile.exists() and a.forces_kernel == "tile":
-            traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_launch")
-        roof = {"bound": "valu", "kernel": "k_forces_fluid", "achieved": round(ach, 3),
-                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(ach / PEAK_FP32_TFLOPS, 4),
-                "traffic": traffic, "avg_launch_ms": round(ms / cnt, 4), "launches": cnt,
-                "flop_per_launch": flops,
-                "note": "VALU-bound FP32 kernel; peak = MI355X FP32 vector peak (packed FP32); "
-                        "traffic = rocprofv3 FETCH_SIZE/WRITE_SIZE per launch scaled by a 1 GiB "
-                        "copy calibration (profiles/)"}
+    if not a.no_timing:
+        kt_steps = max(1, min(a.steps, a.timed_steps))
+        lib.dsph_ctx_timing_reset(ctx)
+        lib.dsph_ctx_set_timing(ctx, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        sim.run(kt_steps)
+        torch.cuda.synchronize()
+        ms_step_timed = 1e3 * (time.perf_counter() - t0) / kt_steps
+        kt = kernel_times(lib, ctx)
+        lib.dsph_ctx_set_timing(ctx, 0)
+        p2 = sim.pips()
+        log(f"instrumented window: {kt_steps} steps, {ms_step_timed:.3f} ms/step")
+        if "k_forces_fluid" in kt:
+            cnt, ms = kt["k_forces_fluid"]
+            ff = ((p1[0] - p1[4]) + (p2[0] - p2[4])) / 2
+            fb = (p1[4] + p2[4]) / 2
+            flops = ff * F_FF + fb * F_FB
+            avg_s = ms / cnt / 1e3
+            ach = flops / avg_s / 1e12
+            traffic = None
+            tfile = ROOT / "profiles" / f"k_forces_fluid_traffic_dam_break_dp{a.dp:g}.json"
+            if tfile.exists() and a.forces_kernel == "tile":
+                traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_launch")
+            roof = {"bound": "valu", "kernel": "k_forces_fluid", "achieved": round(ach, 3),
+                    "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                    "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
+                    "avg_launch_ms": round(ms / cnt, 4), "launches": cnt,
+                    "flop_per_launch": flops,
+                    "note": "VALU-bound FP32 kernel; peak = MI355X FP32 vector peak (packed "
+                            "FP32); kernel time from HIP events on the context stream over a "
+                            "second window of the same run (the headline window has no events); "
+                            "traffic = rocprofv3 FETCH_SIZE/WRITE_SIZE per launch scaled by a "
+                            "1 GiB copy calibration (profiles/)"}
     sim.close()
     del sim
     flume = flume_bench(a.flume_steps, 3) if a.flume_steps > 0 else None
@@ -341,6 +367,7 @@ def main():
         "flume": flume,
         "kernels_ms": {k: {"launches": v[0], "total_ms": round(v[1], 3),
                            "avg_ms": round(v[1] / max(v[0], 1), 5)} for k, v in kt.items()},
+        "instrumented_window": {"steps": kt_steps, "ms_per_step": ms_step_timed},
     }
     print(json.dumps(out), flush=True)
 

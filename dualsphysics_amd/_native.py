@@ -22,6 +22,7 @@ SHIFT_NONE, SHIFT_NOBOUND, SHIFT_NOFIXED, SHIFT_FULL = 0, 1, 2, 3
 BMODE_DBC, BMODE_MDBC2, BMODE_MDBC2OFF = 0, 1, 2
 OPT_FORCES_KERNEL = 1
 FORCES_TILE, FORCES_ROWS = 0, 1
+NP_LIMIT = 1 << 28              # DSPH_NP_LIMIT: particles per simulation / interaction
 
 u32p = C.POINTER(C.c_uint32)
 

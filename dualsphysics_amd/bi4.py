@@ -260,11 +260,10 @@ def write_case(case, directory, name: str, timemax: float | None = None,
         params["SlipMode"] = getattr(p, "slipmode", 1)
     # Periodic axes (JSph.cpp:810-826): a key of the axis' increments turns it on.
     peri = getattr(p, "periodic", 0)
-    for a, keys in enumerate((("XPeriodicIncY", "XPeriodicIncZ"), ("YPeriodicIncX", "YPeriodicIncZ"),
-                              ("ZPeriodicIncX", "ZPeriodicIncY"))):
+    for a, keys in enumerate(_PERI_KEYS):
         if peri & (1 << a):
-            for key in keys:
-                params[key] = 0.0
+            for key, comp in keys:
+                params[key] = float(p.perinc[a][comp])
     moving, motion = "", ""
     if nmov:
         m = case.motion
@@ -397,6 +396,40 @@ def read_part(path, sort_by_id: bool = True) -> dict:
             "cpart": part.get("Cpart"), "symplecticdtpre": part.get("SymplecticDtPre")}
 
 
+# Case-XML keys of the periodic increments per axis: (key, component).
+_PERI_KEYS = ((("XPeriodicIncY", 1), ("XPeriodicIncZ", 2)),
+              (("YPeriodicIncX", 0), ("YPeriodicIncZ", 2)),
+              (("ZPeriodicIncX", 0), ("ZPeriodicIncY", 1)))
+
+
+def periodic_increments(case) -> list:
+    """PeriXinc, PeriYinc, PeriZinc of a case as the solver holds them
+    (JSph.cpp:2183-2188): the case's increments with each periodic axis' own
+    component set to -MapRealSize; zero for axes that are not periodic."""
+    p = case.params
+    peri = int(getattr(p, "periodic", 0))
+    size = [float(case.maprealposmax[k]) - float(case.maprealposmin[k]) for k in range(3)]
+    out = []
+    for a in range(3):
+        if peri & (1 << a):
+            inc = [float(p.perinc[a][k]) for k in range(3)]
+            inc[a] = -size[a]
+        else:
+            inc = [0.0, 0.0, 0.0]
+        out.append(tuple(inc))
+    return out
+
+
+def _set_periodic(root: "Item", case) -> None:
+    """PeriMode (TpPeri = the axis bits of PeriActive, JPeriodicDef.h:33-42)
+    and the increments of a PART or Part_Head header (JPartDataHead::
+    ConfigSimPeri, JSph.cpp:2743): the reference's JPartsLoad4::CheckConfig
+    refuses a resume whose periodic mode differs from the case's."""
+    root.set("PeriMode", T_INT, int(getattr(case.params, "periodic", 0)) & 7)
+    for k, inc in zip(("PeriXinc", "PeriYinc", "PeriZinc"), periodic_increments(case)):
+        root.set(k, T_DOUBLE3, inc)
+
+
 def _root_header(case, name: str, app: str = "dualsphysics_amd") -> "Item":
     """Root values of a particle file (JPartDataBi4::ConfigBasic/ConfigParticles,
     JPartDataBi4.cpp:105-190)."""
@@ -410,9 +443,7 @@ def _root_header(case, name: str, app: str = "dualsphysics_amd") -> "Item":
     root.set("Data2d", T_BOOL, False).set("Data2dPosY", T_DOUBLE, 0.0)
     root.set("MapPosMin", T_DOUBLE3, tuple(map(float, case.maprealposmin)))
     root.set("MapPosMax", T_DOUBLE3, tuple(map(float, case.maprealposmax)))
-    root.set("PeriMode", T_INT, 0)
-    for k in ("PeriXinc", "PeriYinc", "PeriZinc"):
-        root.set(k, T_DOUBLE3, (0.0, 0.0, 0.0))
+    _set_periodic(root, case)
     root.set("AxisDiv", T_INT, 0)
     root.set("CaseNp", T_ULLONG, n).set("CaseNfixed", T_ULLONG, nb - nmov)
     root.set("CaseNmoving", T_ULLONG, nmov).set("CaseNfloat", T_ULLONG, 0)
@@ -472,9 +503,7 @@ def write_part_head(path, case, app: str = "dualsphysics_amd"):
     root.set("NpDynamic", T_BOOL, False).set("ReuseIds", T_BOOL, False)
     root.set("MapPosMin", T_DOUBLE3, tuple(map(float, case.maprealposmin)))
     root.set("MapPosMax", T_DOUBLE3, tuple(map(float, case.maprealposmax)))
-    root.set("PeriMode", T_INT, 0)
-    for k in ("PeriXinc", "PeriYinc", "PeriZinc"):
-        root.set(k, T_DOUBLE3, (0.0, 0.0, 0.0))
+    _set_periodic(root, case)
     root.set("ViscoType", T_UINT, p.tvisco).set("ViscoValue", T_FLOAT, p.visco)
     root.set("ViscoBoundFactor", T_FLOAT, p.viscoboundfactor)
     root.set("Symmetry", T_BOOL, False).set("Splitting", T_BOOL, False)
@@ -545,7 +574,10 @@ def read_case(directory, name: str):
     dp, h, b = val("dp"), val("h"), val("b")
     rhop0, gamma = val("rhop0"), val("gamma")
     massb, massf = val("massbound"), val("massfluid")
-    prm = {e.get("key"): e.get("value") for e in ex.find("parameters").iter("parameter")}
+    # Parameters marked active="false" are skipped, as the solver does
+    # (JCaseEParms::ReadXml with onlyactivated, JCaseEParms.cpp:454-466).
+    prm = {e.get("key"): e.get("value") for e in ex.find("parameters").iter("parameter")
+           if e.get("active", "true").strip().lower() not in ("false", "0")}
     for k, what in _NOT_BUILT_PARAMS.items():
         if what and k in prm and prm[k] not in ("0", "false", "0.0", "0:0:0"):
             raise ValueError(f"parameter {k}={prm[k]}: {what} not built")
@@ -564,9 +596,26 @@ def read_case(directory, name: str):
                       rhopoutmin=gf("RhopOutMin", 700.0), rhopoutmax=gf("RhopOutMax", 1300.0),
                       timemax=gf("TimeMax", 1.0), timeout=gf("TimeOut", 0.0),
                       shifting=gi("Shifting", 0), shiftcoef=gf("ShiftCoef", -2.0),
-                      shifttfs=gf("ShiftTFS", 0.0),
-                      periodic=sum(1 << a for a, ax in enumerate("XYZ")
-                                   if any(k.startswith(ax + "Periodic") for k in prm)))
+                      shifttfs=gf("ShiftTFS", 0.0))
+    # Periodic boundaries (JSph.cpp:812-826): an increment key turns its axis
+    # on with that component; XYPeriodic / XZPeriodic / YZPeriodic set the
+    # two axes with zero increments (and turn the third off).
+    peri, perinc = 0, [[0.0, 0.0, 0.0] for _ in range(3)]
+    for a, keys in enumerate(_PERI_KEYS):
+        for key, comp in keys:
+            if key in prm:
+                perinc[a][comp] = float(prm[key])
+                peri |= 1 << a
+    for key, bits in (("XYPeriodic", 3), ("XZPeriodic", 5), ("YZPeriodic", 6)):
+        if key in prm:
+            peri, perinc = bits, [[0.0, 0.0, 0.0] for _ in range(3)]
+    unknown = [k for k in prm if "Periodic" in k and k not in
+               {key for keys in _PERI_KEYS for key, _ in keys} | {"XYPeriodic", "XZPeriodic",
+                                                                   "YZPeriodic"}]
+    if unknown:
+        raise ValueError(f"periodic parameter(s) {unknown} not recognised")
+    p.periodic = peri
+    p.perinc = tuple(tuple(v) for v in perinc)
     parts = ex.find("particles")
     blocks = list(parts)
     if any(bk.tag == "floating" for bk in blocks):

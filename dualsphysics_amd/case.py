@@ -64,6 +64,11 @@ class Parameters:
     shiftcoef: float = -2.0
     shifttfs: float = 0.0
     periodic: int = 0               # periodic axes, bits 1 X, 2 Y, 4 Z (XPeriodicIncY ...)
+    # Increments of the periodic axes [axis][component] (XPeriodicIncY/Z,
+    # YPeriodicIncX/Z, ZPeriodicIncX/Y; JSph.cpp:813-819): an inclined
+    # periodic axis. Each axis' own component is ignored (the solver sets it
+    # to -MapRealSize, JSph.cpp:2186-2188).
+    perinc: tuple = ((0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))
     cflnumber: float = 0.2
     dtini: float = 0.0
     dtmin: float = 0.0
@@ -123,6 +128,9 @@ class Case:
         c.slipmode = p.slipmode
         c.shiftmode, c.shiftcoef, c.shifttfs = p.shifting, p.shiftcoef, p.shifttfs
         c.periactive = int(p.periodic)
+        for a in range(3):
+            for k in range(3):
+                c.perinc[a][k] = float(p.perinc[a][k]) if a != k else 0.0
         c.casenp, c.casenbound = self.np, self.nbound
         c.casenfixed = self.nbound - self.nmoving
         c.casenmoving, c.casenfloat, c.casenfluid = self.nmoving, 0, self.nfluid

@@ -9,14 +9,8 @@ int set_error(dsph_ctx* ctx, int code, const std::string& msg) {
   return code;
 }
 
-// Digit histogram size for the radix sort: 256 digits x tiles + totals.
-static size_t hist_words(uint32_t n) {
-  const size_t tiles = (size_t(n) + 4095) / 4096 + 1;
-  return 256 * tiles + 512;
-}
-
 int ensure_sort_scratch(dsph_ctx* ctx, uint32_t n) {
-  if (size_t(n) > ctx->sort_cap || hist_words(n) > ctx->hist_cap)
+  if (size_t(n) > ctx->sort_cap)
     return set_error(ctx, DSPH_E_NOMEM,
                      "sort scratch too small: call dsph_ctx_reserve with np_max >= " +
                          std::to_string(n));
@@ -78,9 +72,7 @@ int dsph_ctx_destroy(dsph_ctx* ctx) {
     hipEventDestroy(it.second.second);
   }
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
-  hipFree(ctx->sort_keys_alt);
-  hipFree(ctx->sort_vals_alt);
-  hipFree(ctx->sort_hist);
+  hipFree(ctx->sort_buf);
   hipFree(ctx->redu);
   hipFree(ctx->prep_press);
   hipFree(ctx->prep_posf);
@@ -97,18 +89,10 @@ int dsph_ctx_reserve(dsph_ctx* ctx, uint32_t np_max, uint32_t nctt_max) {
   DSPH_HIP(ctx, hipSetDevice(ctx->device));
   DSPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   if (np_max > ctx->sort_cap) {
-    hipFree(ctx->sort_keys_alt); ctx->sort_keys_alt = nullptr;
-    hipFree(ctx->sort_vals_alt); ctx->sort_vals_alt = nullptr;
+    hipFree(ctx->sort_buf); ctx->sort_buf = nullptr;
     ctx->sort_cap = 0;
-    DSPH_HIP(ctx, hipMalloc(&ctx->sort_keys_alt, sizeof(uint32_t) * size_t(np_max)));
-    DSPH_HIP(ctx, hipMalloc(&ctx->sort_vals_alt, sizeof(uint32_t) * size_t(np_max)));
+    DSPH_HIP(ctx, hipMalloc(&ctx->sort_buf, sizeof(uint32_t) * sort_scratch_words(np_max)));
     ctx->sort_cap = np_max;
-  }
-  const size_t hw = hist_words(np_max);
-  if (hw > ctx->hist_cap) {
-    hipFree(ctx->sort_hist); ctx->sort_hist = nullptr; ctx->hist_cap = 0;
-    DSPH_HIP(ctx, hipMalloc(&ctx->sort_hist, sizeof(uint32_t) * hw));
-    ctx->hist_cap = hw;
   }
   if (np_max > ctx->prep_cap) {
     hipFree(ctx->prep_press); ctx->prep_press = nullptr;

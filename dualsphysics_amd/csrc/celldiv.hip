@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <climits>
 #include <cstring>
+#include <string>
 #include "dsph_ctx.h"
 #include "ensemble.h"
 
@@ -191,17 +192,28 @@ __global__ void __launch_bounds__(kBs) k_radix_scan_digit(unsigned* __restrict__
   if (threadIdx.x == kBs - 1) totals[blockIdx.x] = wbase + incl;
 }
 
+// Scatter of one pass. Each tile ranks its keys stably (per wave: lanes with
+// the same digit found by one ballot per digit bit, per-wave digit counters
+// in LDS), stages the pairs in LDS in digit order (tile-local positions), and
+// then writes every digit run with consecutive threads on consecutive
+// addresses: one coalesced store per run instead of one store per key at its
+// own global slot.
 __global__ void __launch_bounds__(kBs) k_radix_scatter(const unsigned* __restrict__ kin,
     const unsigned* __restrict__ vin, unsigned* __restrict__ kout, unsigned* __restrict__ vout,
     unsigned n, unsigned shift, unsigned bits, unsigned ntiles,
     const unsigned* __restrict__ hist, const unsigned* __restrict__ totals) {
-  __shared__ unsigned dbase[256];
-  __shared__ unsigned wcnt[kBs / 64][256];
+  __shared__ unsigned dbase[256];           // global slot of the tile's first key of digit d
+  __shared__ unsigned lstart[256];          // tile-local position of the first key of digit d
+  __shared__ unsigned wcnt[kBs / 64][256];  // per-wave digit counters, then wave offsets
+  __shared__ unsigned ws[kBs / 64];
+  __shared__ unsigned skey[kTile], sval[kTile];
   const unsigned mask = (1u << bits) - 1u;
   const unsigned tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
   const unsigned tile = blockIdx.x;
-  // Global base of every digit for this tile: exclusive scan of totals over
-  // digits plus the tile offset within the digit.
+  const unsigned tbase = tile * kTile;
+  const unsigned tcount = min(kTile, n - tbase);
+  // Global base of every digit for this tile: exclusive scan of the digit
+  // totals plus the tile's offset within the digit.
   {
     const unsigned t = (tid <= mask) ? totals[tid] : 0u;
     unsigned incl = t;
@@ -209,27 +221,25 @@ __global__ void __launch_bounds__(kBs) k_radix_scatter(const unsigned* __restric
       const unsigned v = __shfl_up((int)incl, off);
       if (lane >= (unsigned)off) incl += v;
     }
-    __shared__ unsigned ws[kBs / 64];
     if (lane == 63) ws[w] = incl;
-    for (unsigned d = tid; d < 256; d += kBs) wcnt[0][d] = 0;
+#pragma unroll
+    for (unsigned k = 0; k < kBs / 64; k++) wcnt[k][tid] = 0;
     __syncthreads();
     unsigned wb = 0;
     for (unsigned i = 0; i < w; i++) wb += ws[i];
     if (tid <= mask) dbase[tid] = wb + incl - t + hist[tid * ntiles + tile];
-#pragma unroll
-    for (unsigned k = 1; k < kBs / 64; k++) wcnt[k][tid] = 0;
   }
   __syncthreads();
   volatile unsigned* cnt = wcnt[w];
-  const unsigned seg = tile * kTile + w * (kTile / (kBs / 64));
+  const unsigned seg = w * (kTile / (kBs / 64));   // wave segment, tile-local
   unsigned key[kTileItems], val[kTileItems], rank[kTileItems];
   const unsigned long long lanelt = (1ull << lane) - 1ull;
 #pragma unroll
   for (unsigned r = 0; r < kTileItems; r++) {
-    const unsigned idx = seg + r * 64 + lane;
-    const bool valid = idx < n;
+    const unsigned li = seg + r * 64 + lane;
+    const bool valid = li < tcount;
     unsigned k = 0, v = 0;
-    if (valid) { k = kin[idx]; v = vin[idx]; }
+    if (valid) { k = kin[tbase + li]; v = vin[tbase + li]; }
     const unsigned d = (k >> shift) & mask;
     unsigned long long peers = __ballot(valid);
     for (unsigned b = 0; b < bits; b++) {
@@ -246,8 +256,11 @@ __global__ void __launch_bounds__(kBs) k_radix_scatter(const unsigned* __restric
     key[r] = k; val[r] = v; rank[r] = valid ? rk : 0xffffffffu;
   }
   __syncthreads();
-  // Exclusive scan of the per-wave digit counts across waves.
-  for (unsigned d = tid; d <= mask; d += kBs) {
+  // Per digit: offsets of the waves (exclusive scan over waves) and the
+  // tile's count of the digit.
+  unsigned dcount = 0;
+  {
+    const unsigned d = tid;   // kBs == 256 digits
     unsigned s = 0;
 #pragma unroll
     for (unsigned k = 0; k < kBs / 64; k++) {
@@ -255,16 +268,39 @@ __global__ void __launch_bounds__(kBs) k_radix_scatter(const unsigned* __restric
       wcnt[k][d] = s;
       s += c;
     }
+    dcount = (d <= mask) ? s : 0u;
+  }
+  // Exclusive scan of the digit counts over digits: tile-local digit starts.
+  {
+    unsigned incl = dcount;
+    for (int off = 1; off < 64; off <<= 1) {
+      const unsigned v = __shfl_up((int)incl, off);
+      if (lane >= (unsigned)off) incl += v;
+    }
+    __syncthreads();
+    if (lane == 63) ws[w] = incl;
+    __syncthreads();
+    unsigned wb = 0;
+    for (unsigned i = 0; i < w; i++) wb += ws[i];
+    lstart[tid] = wb + incl - dcount;
   }
   __syncthreads();
 #pragma unroll
   for (unsigned r = 0; r < kTileItems; r++) {
     if (rank[r] != 0xffffffffu) {
       const unsigned d = (key[r] >> shift) & mask;
-      const unsigned pos = dbase[d] + wcnt[w][d] + rank[r];
-      kout[pos] = key[r];
-      vout[pos] = val[r];
+      const unsigned lp = lstart[d] + wcnt[w][d] + rank[r];
+      skey[lp] = key[r];
+      sval[lp] = val[r];
     }
+  }
+  __syncthreads();
+  for (unsigned i = tid; i < tcount; i += kBs) {
+    const unsigned k = skey[i];
+    const unsigned d = (k >> shift) & mask;
+    const unsigned pos = dbase[d] + (i - lstart[d]);
+    kout[pos] = k;
+    vout[pos] = sval[i];
   }
 }
 
@@ -548,6 +584,7 @@ __global__ void __launch_bounds__(kBs) k_periodic_duplicate(unsigned n, unsigned
 }
 }  // namespace
 
+
 namespace dsph {
 
 int ens_presort(dsph_ctx* ctx, const EnsCase* cases, const EnsBlock* pmap, unsigned npblocks,
@@ -580,6 +617,55 @@ int ens_sort_basic_poscell(dsph_ctx* ctx, const EnsCase* cases, const EnsBlock* 
                                                       posxy, posz, velrho, idp2, code2, dcell2,
                                                       posxy2, posz2, velrho2, poscell);
   DSPH_LAUNCHED(ctx, "k_ens_sort_basic");
+  return DSPH_OK;
+}
+
+// Radix sort of (keys, values) in place with the scratch `buf` laid out for
+// `cap` keys (sort_scratch_words): two alternate key/value buffers A and B
+// and the digit histograms. The passes alternate between the caller's arrays
+// and A (two or four passes) or go d -> A -> B -> d (three passes), so the
+// result always ends in the caller's arrays without a copy; a key range of
+// one digit is sorted twice (the second pass leaves the order unchanged).
+// Nothing in the scratch needs clearing: every pass writes the histograms
+// of its tiles before reading them.
+int radix_sort(dsph_ctx* ctx, uint32_t* d_keys, uint32_t* d_values, uint32_t n,
+               uint32_t key_limit, uint32_t* buf, size_t cap) {
+  unsigned bits_total = 0;
+  while (bits_total < 32 && (key_limit - 1u) >> bits_total) bits_total++;
+  if (bits_total == 0) return DSPH_OK;  // all keys equal (key_limit<=1)
+  unsigned *ka_alt = buf, *va_alt = buf + cap, *kb_alt = buf + 2 * cap, *vb_alt = buf + 3 * cap;
+  unsigned* hist = buf + 4 * cap;
+  const unsigned ntiles = (n + kTile - 1) / kTile;
+  unsigned* totals = hist + size_t(256) * ntiles;
+  unsigned npass = (bits_total + 7) / 8;
+  // (shift, bits) of every pass.
+  unsigned sh[4], bw[4];
+  for (unsigned p = 0; p < npass; p++) {
+    sh[p] = 8 * p;
+    bw[p] = min(8u, bits_total - 8 * p);
+  }
+  if (npass == 1) {
+    sh[1] = sh[0];
+    bw[1] = bw[0];
+    npass = 2;
+  }
+  unsigned* kbuf[5];
+  unsigned* vbuf[5];
+  kbuf[0] = d_keys; vbuf[0] = d_values;
+  for (unsigned p = 1; p <= npass; p++) {
+    const bool to_caller = (p == npass) || (npass != 3 && p % 2 == 0);
+    const bool to_b = npass == 3 && p == 2;
+    kbuf[p] = to_caller ? d_keys : (to_b ? kb_alt : ka_alt);
+    vbuf[p] = to_caller ? d_values : (to_b ? vb_alt : va_alt);
+  }
+  for (unsigned p = 0; p < npass; p++) {
+    const unsigned mask = (1u << bw[p]) - 1u;
+    k_radix_hist<<<ntiles, kBs, 0, ctx->stream>>>(kbuf[p], n, sh[p], mask, ntiles, hist);
+    k_radix_scan_digit<<<mask + 1, kBs, 0, ctx->stream>>>(hist, ntiles, totals);
+    k_radix_scatter<<<ntiles, kBs, 0, ctx->stream>>>(kbuf[p], vbuf[p], kbuf[p + 1], vbuf[p + 1],
+                                                     n, sh[p], bw[p], ntiles, hist, totals);
+    DSPH_LAUNCHED(ctx, "radix pass");
+  }
   return DSPH_OK;
 }
 
@@ -710,34 +796,8 @@ int dsph_sort_by_cell(dsph_ctx* ctx, uint32_t* d_keys, uint32_t* d_values, uint3
   DSPH_REQUIRE(ctx, d_keys && d_values, "dsph_sort_by_cell: null array");
   int rc = ensure_sort_scratch(ctx, n);
   if (rc) return rc;
-  unsigned bits_total = 0;
-  while (bits_total < 32 && (key_limit - 1u) >> bits_total) bits_total++;
-  if (bits_total == 0) return DSPH_OK;  // all keys equal (key_limit<=1)
   KTimer kt(ctx, "radix_sort");
-  const unsigned ntiles = (n + kTile - 1) / kTile;
-  unsigned* totals = ctx->sort_hist + size_t(256) * ntiles;
-  unsigned *ka = d_keys, *va = d_values, *kb = ctx->sort_keys_alt, *vb = ctx->sort_vals_alt;
-  unsigned shift = 0;
-  while (shift < bits_total) {
-    const unsigned bits = min(8u, bits_total - shift);
-    const unsigned mask = (1u << bits) - 1u;
-    k_radix_hist<<<ntiles, kBs, 0, ctx->stream>>>(ka, n, shift, mask, ntiles, ctx->sort_hist);
-    k_radix_scan_digit<<<mask + 1, kBs, 0, ctx->stream>>>(ctx->sort_hist, ntiles, totals);
-    k_radix_scatter<<<ntiles, kBs, 0, ctx->stream>>>(ka, va, kb, vb, n, shift, bits, ntiles,
-                                                     ctx->sort_hist, totals);
-    DSPH_LAUNCHED(ctx, "radix pass");
-    unsigned* t;
-    t = ka; ka = kb; kb = t;
-    t = va; va = vb; vb = t;
-    shift += bits;
-  }
-  if (ka != d_keys) {
-    DSPH_HIP(ctx, hipMemcpyAsync(d_keys, ka, sizeof(unsigned) * n, hipMemcpyDeviceToDevice,
-                                 ctx->stream));
-    DSPH_HIP(ctx, hipMemcpyAsync(d_values, va, sizeof(unsigned) * n, hipMemcpyDeviceToDevice,
-                                 ctx->stream));
-  }
-  return DSPH_OK;
+  return radix_sort(ctx, d_keys, d_values, n, key_limit, ctx->sort_buf, ctx->sort_cap);
 }
 
 int dsph_begin_end_cell(dsph_ctx* ctx, int full, uint32_t np, uint32_t npb,

@@ -24,12 +24,10 @@ struct dsph_ctx {
   std::string err;
   dsph_cte cte{};
   bool cte_set = false;
-  // Radix-sort scratch: alternate key/value buffers and digit histograms.
-  uint32_t* sort_keys_alt = nullptr;
-  uint32_t* sort_vals_alt = nullptr;
+  // Radix-sort scratch for up to sort_cap keys (layout: sort_scratch_words):
+  // two alternate key/value buffers and the digit histograms.
+  uint32_t* sort_buf = nullptr;
   size_t sort_cap = 0;
-  uint32_t* sort_hist = nullptr;   // [256 digits][ntiles] + [256] totals
-  size_t hist_cap = 0;
   // Reduction partials (float) and a small device result area.
   float* redu = nullptr;
   size_t redu_cap = 0;
@@ -100,8 +98,18 @@ struct KTimer {
   }
 };
 
+// Words of radix-sort scratch for up to cap keys: alternate buffers A and B
+// (keys, values: 4 cap words) and the digit histograms of 4096-key tiles
+// (256 per tile, plus the digit totals).
+constexpr size_t sort_scratch_words(size_t cap) {
+  return 4 * cap + 256 * ((cap + 4095) / 4096 + 1) + 512;
+}
 // Ensures ctx scratch can hold a sort of n keys with key_limit.
 int ensure_sort_scratch(dsph_ctx* ctx, uint32_t n);
+// Stable LSD radix sort of (keys, values) in place with scratch buf laid out
+// for cap >= n keys (celldiv.hip).
+int radix_sort(dsph_ctx* ctx, uint32_t* d_keys, uint32_t* d_values, uint32_t n,
+               uint32_t key_limit, uint32_t* buf, size_t cap);
 int ensure_redu_scratch(dsph_ctx* ctx, uint32_t n);
 int ensure_prep_scratch(dsph_ctx* ctx, uint32_t n);
 

@@ -865,6 +865,8 @@ int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
   if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
   DSPH_REQUIRE(ctx, t->tkernel == DSPH_KERNEL_WENDLAND, "only the Wendland kernel is built");
   DSPH_REQUIRE(ctx, t->np >= t->npb && t->npb >= t->npbok, "invalid particle counts");
+  DSPH_REQUIRE(ctx, t->np < DSPH_NP_LIMIT,
+               "dsph_interaction_forces: np >= DSPH_NP_LIMIT (2^28: 32-bit byte offsets)");
   DSPH_REQUIRE(ctx, t->d_dcell && t->d_poscell && t->d_velrho && t->d_viscdt && t->d_ar &&
                         t->d_ace && t->divdata.d_beginendcell,
                "dsph_interaction_forces: null array");

@@ -442,9 +442,12 @@ extern "C" {
 int dsph_ens_create(int device, uint32_t ncases, const dsph_case* cases, const uint32_t* np,
                     const uint32_t* h_idp, const double* h_pos, const float* h_velrho,
                     const double* tend, dsph_ens** out) {
-  if (!out || !ncases || !cases || !np || !h_idp || !h_pos || !h_velrho || !tend)
-    return DSPH_E_ARG;
+  if (!out) return DSPH_E_ARG;
   *out = nullptr;
+  if (!ncases || !cases || !np || !h_idp || !h_pos || !h_velrho || !tend) return DSPH_E_ARG;
+  uint64_t ntot = 0;
+  for (uint32_t k = 0; k < ncases; k++) ntot += np[k];
+  if (ntot >= DSPH_NP_LIMIT) return DSPH_E_ARG;
   dsph_ens* e = new dsph_ens();
   int rc = dsph_ctx_create(device, &e->ctx);
   if (rc) {

@@ -16,8 +16,9 @@ extern "C" {
 int dsph_sim_create(int device, const dsph_case* cs, uint32_t np, const uint32_t* h_idp,
                     const double* h_pos, const float* h_velrho, const float* h_boundnor,
                     dsph_sim** out) {
-  if (!cs || !out || !h_idp || !h_pos || !h_velrho || !np) return DSPH_E_ARG;
+  if (!out) return DSPH_E_ARG;
   *out = nullptr;
+  if (!cs || !h_idp || !h_pos || !h_velrho || !np || np >= DSPH_NP_LIMIT) return DSPH_E_ARG;
   dsph_sim* s = new dsph_sim();
   int rc = dsph_ctx_create(device, &s->ctx);
   if (rc) {

@@ -10,6 +10,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 #include "dsph_abi.h"
 
@@ -222,6 +223,10 @@ struct dsph_sim {
     PeriActive = unsigned(c.periactive) & 7u;
     if (PeriActive && c.casenmoving)
       throw Err(DSPH_E_ARG, "periodic boundaries with moving boundaries are not built");
+    if (PeriActive && TBoundary == DSPH_BC_MDBC && SlipMode == DSPH_SLIP_NOSLIP)
+      throw Err(DSPH_E_ARG, "periodic boundaries with mDBC no-slip are not built");
+    if (PeriActive && ShiftMode)
+      throw Err(DSPH_E_ARG, "periodic boundaries with shifting are not built");
     for (int k = 0; k < 3; k++) {
       MapRealPosMin[k] = c.maprealposmin[k];
       MapRealPosMax[k] = c.maprealposmax[k];
@@ -304,7 +309,7 @@ struct dsph_sim {
             const double x = hpos[3 * size_t(p) + a];
             if (x < MapRealPosMin[a] + KernelSize || x >= MapRealPosMax[a] - KernelSize) near++;
           }
-      NpCap = unsigned(std::min<size_t>(size_t(np) + 4 * near + 4096, 0x7fffffffu));
+      NpCap = unsigned(std::min<size_t>(size_t(np) + 4 * near + 4096, DSPH_NP_LIMIT - 1u));
     }
     std::vector<uint16_t> hcode(np);
     std::vector<unsigned> hdcell(np);
@@ -556,13 +561,22 @@ struct dsph_sim {
           const unsigned nper = ctype ? NpfPer : NpbPer;
           const unsigned pini2 = cblock ? pini : Np - nper, num2 = cblock ? num : nper;
           if (!num2) continue;
-          const unsigned nmax = NpCap - Np;
+          unsigned nmax = NpCap - Np;
           unsigned count = 0;
           chk(dsph_periodic_make_list(ctx, num2, pini2, nmax, Map_PosMin, Map_PosMax, PeriInc[a],
                                       posxy, posz, code, periwork, perilist, &count),
               "periodic list");
-          if (count > nmax)
-            throw Err(DSPH_E_NOMEM, "periodic copies exceed the particle capacity");
+          if (count > nmax) {
+            // More copies than room: grow every particle array (contents of
+            // [0,Np) kept, as JSphGpuSingle::ResizeParticlesSize) and make
+            // the list again; nothing has changed the state of this block yet.
+            grow_particles(size_t(Np) + count + (size_t(Np) + count) / 4 + 4096);
+            nmax = NpCap - Np;
+            chk(dsph_periodic_make_list(ctx, num2, pini2, nmax, Map_PosMin, Map_PosMax,
+                                        PeriInc[a], posxy, posz, code, periwork, perilist, &count),
+                "periodic list");
+            if (count > nmax) throw Err(DSPH_E_STATE, "periodic list changed after growing");
+          }
           chk(dsph_periodic_duplicate(ctx, count, Np, &DomCells.x, PeriInc[a], perilist, idp, code,
                                       dcell, posxy, posz, velrho, spstau,
                                       pre_active ? posxypre : nullptr,
@@ -575,6 +589,43 @@ struct dsph_sim {
         }
       }
     }
+  }
+
+  // Reallocates every per-particle array for `cap` particles, keeping the
+  // contents of [0,Np) (JSphGpuSingle::ResizeParticlesSize,
+  // JSphGpuSingle.cpp:240-262); the context scratch follows.
+  void grow_particles(size_t cap) {
+    cap = std::min<size_t>(cap, DSPH_NP_LIMIT - 1u);
+    if (cap <= NpCap) throw Err(DSPH_E_NOMEM, "particle arrays at DSPH_NP_LIMIT");
+    chk(dsph_synchronize(ctx), "synchronize");
+    const size_t keep = Np;
+    auto regrow = [&](auto*& ptr, size_t per) {
+      using T = std::remove_reference_t<decltype(*ptr)>;
+      if (!ptr) return;
+      T* q = alloc<T>(per * cap);
+      chk(dsph_memcpy_d2d_async(ctx, q, ptr, sizeof(T) * per * keep), "grow copy");
+      chk(dsph_synchronize(ctx), "synchronize");
+      chk(dsph_free(ctx, ptr), "grow free");
+      ptr = q;
+    };
+    regrow(perilist, 1); regrow(acemod, 1);
+    regrow(idp, 1); regrow(idp2, 1); regrow(dcell, 1); regrow(dcell2, 1);
+    regrow(code, 1); regrow(code2, 1);
+    regrow(posxy, 2); regrow(posxy2, 2); regrow(posz, 1); regrow(posz2, 1);
+    regrow(velrho, 4); regrow(velrho2, 4); regrow(poscell, 4);
+    regrow(ace, 3); regrow(ar, 1); regrow(viscdt, 1); regrow(movxy, 2); regrow(movz, 1);
+    regrow(cellpart, 1); regrow(sortpart, 1);
+    regrow(posxypre, 2); regrow(posxypre2, 2); regrow(poszpre, 1); regrow(poszpre2, 1);
+    regrow(velrhopre, 4); regrow(velrhopre2, 4); regrow(velrhom1, 4); regrow(velrhom1b, 4);
+    regrow(sps2strain, 6); regrow(spstau, 6); regrow(spstau2, 6); regrow(shiftposfs, 4);
+    regrow(boundnor, 3); regrow(boundnor2, 3); regrow(boundmode, 1); regrow(tangenvel, 3);
+    regrow(velgrad, 9); regrow(closure_nu, 1);
+    if (periwork) {
+      chk(dsph_free(ctx, periwork), "grow free");
+      periwork = alloc<unsigned>((cap + 255) / 256 + 1);
+    }
+    NpCap = unsigned(cap);
+    chk(dsph_ctx_reserve(ctx, NpCap, unsigned(begincell_cap)), "dsph_ctx_reserve");
   }
 
   void run_cell_divide() {

@@ -153,12 +153,26 @@ class SphGpuSingle:
             idp, pos, vr, code = idp[keep], pos[keep], vr[keep], code[keep]
         return {"idp": idp, "pos": pos, "velrho": vr, "code": code}
 
-    def forces(self):
-        """ace (float3) and ar of the last interaction, sorted order."""
+    def _keep(self, n: int):
+        """Mask of the rows download() returns (periodic copies left out), or
+        None when every row is kept."""
+        code = np.empty(n, np.uint16)
+        cnt = C.c_uint32()
+        self._check(self.lib.dsph_sim_download(self._h, C.byref(cnt), None, None, None,
+                                               _ptr(code)))
+        keep = (code & 0xe000) != 0x2000
+        return None if keep.all() else keep
+
+    def forces(self, with_copies: bool = False):
+        """ace (float3) and ar of the last interaction, sorted order, row for
+        row as download() (periodic copies left out unless ``with_copies``)."""
         n = self.stats()["np"]
         ace = np.empty((n, 3), np.float32)
         ar = np.empty(n, np.float32)
         self._check(self.lib.dsph_sim_download_forces(self._h, _ptr(ace), _ptr(ar)))
+        keep = None if with_copies else self._keep(n)
+        if keep is not None:
+            ace, ar = ace[keep], ar[keep]
         return ace, ar
 
     def interact(self):
@@ -183,13 +197,17 @@ class SphGpuSingle:
         self._check(self.lib.dsph_sim_set_closure(self._h, None if mlp is None else C.byref(mlp.m)))
 
     def closure_nu(self, with_velgrad: bool = False):
-        """Eddy viscosity of the last interaction (sorted order; 0 for
-        boundary) and, with ``with_velgrad``, the velocity gradient [np,9] it
-        was evaluated on."""
+        """Eddy viscosity of the last interaction (sorted order, row for row
+        as download(); 0 for boundary) and, with ``with_velgrad``, the
+        velocity gradient [np,9] it was evaluated on."""
         n = self.stats()["np"]
         nu = np.empty(n, np.float32)
         g = np.empty((n, 9), np.float32) if with_velgrad else None
         self._check(self.lib.dsph_sim_closure_nu(self._h, _ptr(nu), None if g is None else _ptr(g)))
+        keep = self._keep(n)
+        if keep is not None:
+            nu = nu[keep]
+            g = None if g is None else g[keep]
         return (nu, g) if with_velgrad else nu
 
     def filter_probes(self, probes: np.ndarray, hf: float) -> np.ndarray:

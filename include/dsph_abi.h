@@ -72,6 +72,12 @@ extern "C" {
 #define DSPH_BMODE_MDBC2      1
 #define DSPH_BMODE_MDBC2OFF   2   /* ghost node not submerged: mass off */
 
+/* Particles per interaction, simulation or ensemble: the interaction kernels
+ * address the float4 particle arrays with 32-bit byte offsets (16 bytes per
+ * particle), so np must stay below 2^28; larger counts are refused with
+ * DSPH_E_ARG. */
+#define DSPH_NP_LIMIT 268435456u
+
 typedef struct dsph_ctx dsph_ctx;
 
 /*

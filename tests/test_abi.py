@@ -113,3 +113,26 @@ def test_product_path_fails_loudly_without_library(monkeypatch, tmp_path):
     c = casemod.dam_break(dp=0.05)
     with pytest.raises(RuntimeError):
         sim.SphGpuSingle(c)
+
+
+def test_particle_counts_beyond_the_32bit_offset_limit_are_refused():
+    """The interaction kernels address float4 arrays with 32-bit byte
+    offsets: simulations and ensembles of 2^28 particles or more are refused
+    with DSPH_E_ARG before any device call (so this runs without a GPU)."""
+    from dualsphysics_amd import case as casemod
+    lib = N.load()
+    c = casemod.dam_break(dp=0.05)
+    nc = c.to_native()
+    dummy = (C.c_double * 16)()
+    h = C.c_void_p()
+    for np_ in (N.NP_LIMIT, N.NP_LIMIT + 5, 0xFFFFFFFF):
+        rc = lib.dsph_sim_create(0, C.byref(nc), np_, dummy, dummy, dummy, None, C.byref(h))
+        assert rc == N.DSPH_E_ARG and not h.value
+    cases = (N.DsphCase * 2)(nc, nc)
+    npk = (C.c_uint32 * 2)(N.NP_LIMIT // 2, N.NP_LIMIT // 2)
+    tend = (C.c_double * 2)(1.0, 1.0)
+    rc = lib.dsph_ens_create(0, 2, cases, npk, dummy, dummy, dummy, tend, C.byref(h))
+    assert rc == N.DSPH_E_ARG and not h.value
+    with open(HEADER) as f:
+        text = f.read()
+    assert re.search(r"#define DSPH_NP_LIMIT 268435456u", text)

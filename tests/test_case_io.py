@@ -24,10 +24,12 @@ def _cases():
             cm.flume(dp=0.01), cm.flume(dp=0.01, wavemaker="flap", params=mdbc),
             cm.flume(dp=0.01, wavemaker="flap_rot"),
             cm.periodic_channel(dp=0.02),
-            cm.periodic_channel(dp=0.02, params=cm.Parameters(periodic=3, tboundary=2))]
+            cm.periodic_channel(dp=0.02, params=cm.Parameters(periodic=3, tboundary=2)),
+            cm.periodic_channel(dp=0.02, params=cm.Parameters(
+                periodic=1, perinc=((0.0, 0.0, 0.01), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))))]
 
 
-@pytest.mark.parametrize("k", range(7))
+@pytest.mark.parametrize("k", range(8))
 def test_case_round_trip(tmp_path, k):
     c = _cases()[k]
     bi4.write_case(c, tmp_path, "case")
@@ -84,15 +86,58 @@ def test_options_outside_the_path_are_refused(tmp_path):
         bi4.read_case(tmp_path, "cubic")
 
 
+def test_periodic_keys_of_the_case_xml(tmp_path):
+    """Periodic parameters as the solver reads them (JSph.cpp:812-826,
+    JCaseEParms.cpp:454-466): increments of an inclined axis, the two-axis
+    keys, parameters switched off with active="false", unknown keys refused;
+    PART and Part_Head headers carry PeriMode and the increments (own
+    component -MapRealSize) that the reference's resume checks."""
+    c = cm.periodic_channel(dp=0.02)
+    bi4.write_case(c, tmp_path, "case")
+    xml = (tmp_path / "case.xml").read_text()
+    assert 'key="XPeriodicIncY"' in xml
+
+    def variant(name, text):
+        (tmp_path / f"{name}.xml").write_text(text)
+        shutil.copy(tmp_path / "case.bi4", tmp_path / f"{name}.bi4")
+        return bi4.read_case(tmp_path, name)
+
+    r = variant("incl", xml.replace('key="XPeriodicIncZ" value="0.0"',
+                                    'key="XPeriodicIncZ" value="0.25"'))
+    assert r.params.periodic == 1 and r.params.perinc[0] == (0.0, 0.0, 0.25)
+    nat = r.to_native()
+    assert list(nat.perinc[0]) == [0.0, 0.0, 0.25]
+    r = variant("off", xml.replace('key="XPeriodicIncY" value="0.0"',
+                                   'key="XPeriodicIncY" value="0.0" active="false"')
+                .replace('key="XPeriodicIncZ" value="0.0"',
+                         'key="XPeriodicIncZ" value="0.0" active="false"'))
+    assert r.params.periodic == 0
+    r = variant("xy", xml.replace('<parameter key="XPeriodicIncY" value="0.0" />',
+                                  '<parameter key="XYPeriodic" value="1" />'))
+    assert r.params.periodic == 3 and r.params.perinc == ((0.0,) * 3,) * 3
+    with pytest.raises(ValueError, match="not recognised"):
+        variant("bad", xml.replace('key="XPeriodicIncY"', 'key="XPeriodicIncW"'))
+    bi4.write_part(tmp_path / "Part_0001.bi4", c, c.idp, c.pos, c.velrho, 0.0, 0, 1)
+    bi4.write_part_head(tmp_path / "Part_Head.ibi4", c)
+    size = c.maprealposmax[0] - c.maprealposmin[0]
+    for f in ("Part_0001.bi4", "Part_Head.ibi4"):
+        root = bi4.read_file(tmp_path / f)
+        assert root.get("PeriMode") == 1
+        assert tuple(root.get("PeriXinc")) == (-size, 0.0, 0.0)
+        assert tuple(root.get("PeriYinc")) == (0.0, 0.0, 0.0)
+
+
 @pytest.mark.ref
-def test_reference_resumes_from_a_part_written_here(tmp_path):
+@pytest.mark.parametrize("kind", ["dam_break", "periodic_channel"])
+def test_reference_resumes_from_a_part_written_here(tmp_path, kind):
     """The reference CPU build continued (-partbegin) from a PART + head
     written by write_part / write_part_head (from the data of its own
     Part_0005) gives its own continued run bit for bit, which equals its
-    uninterrupted run."""
+    uninterrupted run. The periodic channel checks the periodic header
+    (the reference refuses a PART whose PeriMode differs from the case)."""
     if not REF_EXE.exists():
         pytest.skip("reference CPU build absent")
-    c = cm.dam_break(dp=0.04)
+    c = cm.dam_break(dp=0.04) if kind == "dam_break" else cm.periodic_channel(dp=0.02)
     bi4.write_case(c, tmp_path / "in", "case", timemax=10.0, timeout=10.0)
 
     def run(out, extra):

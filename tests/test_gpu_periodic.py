@@ -73,24 +73,39 @@ def test_trajectory_matches_oracle(sync_dt):
     assert drho < 1e-2
 
 
-def test_xy_periodic_mdbc_matches_oracle():
+@pytest.mark.parametrize("tvisco,sync_dt,cellmode", [(3, True, 1), (2, False, 1), (3, True, 2)],
+                         ids=["laminar_host_dt", "laminarsps_device_dt", "laminar_half_cells"])
+def test_xy_periodic_mdbc_matches_oracle(tvisco, sync_dt, cellmode):
+    """Channel periodic in x and y with mDBC: laminar viscosity (host dt),
+    laminar+SPS with dt on the device, and half cells (cells of size h)."""
     from dualsphysics_amd.sim import SphGpuSingle
     from oracle.oracle import OracleSim
-    p = casemod.Parameters(periodic=3, tboundary=2, tvisco=3, visco=1e-4)
+    p = casemod.Parameters(periodic=3, tboundary=2, tvisco=tvisco,
+                           visco=1e-4 if tvisco == 3 else 1e-6, cellmode=cellmode)
     c = casemod.periodic_channel(dp=0.02, params=p)
     s = SphGpuSingle(c)
     o = OracleSim(c)
     for _ in range(3):
-        s.run(10, sync_dt=True)
+        s.run(10, sync_dt=sync_dt)
         o.run(10)
         g, d = s.download(), o.download()
         np.testing.assert_array_equal(g["idp"], d["idp"])
         assert s.stats()["np"] == o.stats()["np"]
     dpos = np.abs(g["pos"] - d["pos"]).max()
     drho = np.abs(g["velrho"][:, 3] - d["velrho"][:, 3]).max()
-    print(f"xy periodic + mDBC, 30 steps: max|dpos| {dpos:.2e} max|drho| {drho:.2e}")
+    print(f"xy periodic + mDBC (visco {tvisco}, cellmode {cellmode}), 30 steps: "
+          f"max|dpos| {dpos:.2e} max|drho| {drho:.2e}")
     assert dpos < 1e-4 * c.dp and drho < 1e-2
     s.close()
+
+
+@pytest.mark.parametrize("prm", [dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4),
+                                 dict(shifting=3)], ids=["mdbc2", "shifting"])
+def test_unbuilt_periodic_combinations_are_refused(prm):
+    from dualsphysics_amd.sim import DsphError, SphGpuSingle
+    c = casemod.periodic_channel(dp=0.04, params=casemod.Parameters(periodic=1, **prm))
+    with pytest.raises(DsphError, match="periodic boundaries with"):
+        SphGpuSingle(c)
 
 
 def _list_expected(pos, code, pini, n, mn, mx, inc):
@@ -138,3 +153,59 @@ def test_make_list_matches_rule(lib, ctx, n, pini, nmax):
     k = min(exp.size, nmax)
     np.testing.assert_array_equal(got[:k], exp[:k])
     assert np.all(got[k:] == 0xdeadbeef)
+
+
+def _free_block_channel(dp=0.02, length=1.0, u0=20.0):
+    """A fluid block 0.4 x 0.6 x 0.6 m at x in [0.45, 0.85] of a channel
+    periodic in x, without walls or gravity, moving at u0 along x: at load no
+    particle is near a periodic face (the arrays get the minimum headroom for
+    copies, 4096); when the block crosses the face, about 7000 copies are
+    made while the previous ones are still held."""
+    import math
+    from dualsphysics_amd import _native as N
+    pts, _ = casemod._lattice(round(0.45 / dp), round(0.85 / dp), 0, round(0.6 / dp), 0,
+                              round(0.6 / dp), dp)
+    u = casemod.uniform01(42, 3 * pts.shape[0]).reshape(-1, 3)
+    pos = pts + (2.0 * u - 1.0) * (1e-4 * dp)
+    h = 1.2 * math.sqrt(3.0) * dp
+    cs0 = 20.0 * math.sqrt(9.81 * 0.2)
+    rho0, gamma = 1000.0, 7.0
+    vr = np.zeros((pos.shape[0], 4), np.float32)
+    vr[:, 0], vr[:, 3] = u0, rho0
+    prm = casemod.Parameters(periodic=1, tvisco=N.VISCO_ARTIFICIAL, tdensity=N.DDT_NONE)
+    mn, mx = casemod.map_limits(pos, h, periodic=1, dp=dp)
+    mn = (0.0 - dp / 2, mn[1], mn[2])
+    mx = (length - dp / 2, mx[1], mx[2] + 0.2)
+    return casemod.Case(name="free_block", dp=dp, h=h, rhop0=rho0, gamma=gamma, cs0=cs0,
+                        b=cs0 * cs0 * rho0 / gamma, massfluid=rho0 * dp ** 3,
+                        massbound=rho0 * dp ** 3, gravity=(0.0, 0.0, 0.0), hswl=0.2, params=prm,
+                        idp=np.arange(pos.shape[0], dtype=np.uint32), pos=pos, velrho=vr,
+                        nbound=0, nfluid=pos.shape[0], maprealposmin=mn, maprealposmax=mx)
+
+
+def test_copies_beyond_the_initial_capacity_grow_the_arrays():
+    """The arrays are sized at load for the copies of that moment; a block
+    that later crosses a periodic face needs more: the arrays grow (contents
+    kept) and the run follows the oracle, order and copies included."""
+    from dualsphysics_amd.sim import SphGpuSingle
+    from oracle.oracle import OracleSim
+    c = _free_block_channel()
+    s = SphGpuSingle(c)
+    o = OracleSim(c)
+    assert s.stats()["np"] == c.np        # nothing near a face at load
+    most = 0
+    for _ in range(10):
+        s.run(20, sync_dt=True)
+        o.run(20)
+        most = max(most, s.stats()["np"] - c.np)
+        assert s.stats()["np"] == o.stats()["np"]
+        if most > 4096:
+            break
+    print(f"{c.np} particles, up to {most} copies after {s.stats()['nstep']} steps")
+    # Old and new copies coexist during the copy step: twice the copies
+    # exceed the 4096 of headroom, so the arrays must have grown.
+    assert 2 * most > 2 * 4096, "the block never needed more copies than the initial headroom"
+    g, d = s.download(), o.download()
+    np.testing.assert_array_equal(g["idp"], d["idp"])
+    assert np.abs(g["pos"] - d["pos"]).max() < 1e-4 * c.dp
+    s.close()
