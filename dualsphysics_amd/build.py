@@ -82,10 +82,10 @@ def build_oracle() -> Path:
     srcs = [ROOT / "oracle" / "dsph_oracle.cpp", ROOT / "oracle" / "dsph_oracle_sim.cpp"]
     tmp = ORACLE_LIB.with_suffix(".so.tmp")
     cxx = shutil.which("g++") or "g++"
-    # FMA contraction on, as in the reference's own builds (nvcc's default
-    # --fmad=true for the GPU path, -ffast-math for the CPU path); no other
-    # fast-math relaxation.
-    _run([cxx, "-O3", "-march=x86-64-v3", "-ffp-contract=fast", "-fopenmp", "-fPIC", "-shared",
+    # No FMA contraction: the reference CPU build the fixtures come from
+    # (oracle/ref.mk, IEEE flags, baseline x86-64) has none, so the oracle's
+    # CPU semantics reproduce its forces bit for bit; no fast-math relaxation.
+    _run([cxx, "-O3", "-march=x86-64-v3", "-ffp-contract=off", "-fopenmp", "-fPIC", "-shared",
           "-std=c++17", f"-I{INCLUDE}", *map(str, srcs), "-o", str(tmp)])
     os.replace(tmp, ORACLE_LIB)
     return ORACLE_LIB

@@ -394,11 +394,19 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
     }
 }
 
-// Tait EOS (FunSphEos.h:38-66 / FunSphEos_iker.h:37-59).
+// Tait EOS (FunSphEos.h:38-66 / FunSphEos_iker.h:37-59). The CPU build's
+// ComputePressMonaghan calls the C library's pow(double, double) on the FP32
+// ratio rho/rho0, so its pressure is B*(x^gamma - 1) evaluated in FP64 and
+// rounded once to FP32 (this reproduces every pressure of the reference CPU
+// build bit for bit); the GPU build evaluates the same in FP32.
 template <typename R>
 R eos(int sem, const orc_cte& c, float rho) {
   if (sem == SEM_GPU)
     return R(c.cteb) * (crpow<R>(R(rho) * R(c.ovrhopzero), R(c.gamma)) - R(1));
+  if (sem == SEM_CPU && sizeof(R) == sizeof(float)) {
+    const float x = rho / c.rhopzero;
+    return R(float(double(c.cteb) * (std::pow(double(x), double(c.gamma)) - 1.0)));
+  }
   return R(c.cteb) * (crpow<R>(R(rho) / R(c.rhopzero), R(c.gamma)) - R(1));
 }
 

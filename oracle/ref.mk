@@ -8,7 +8,10 @@
 #  - DualSPHysics5.4CPU_fast: the reference's own release flags (Makefile_cpu:
 #    -O3 -fopenmp -ffast-math) for timing, with a portable AVX2/FMA target
 #    instead of -march=native (the timing host is not this machine).
-#   make -f oracle/ref.mk -j8 [fast]
+#  - ref_forces: oracle/ref_forces.cpp (a driver of this repository) linked
+#    with the IEEE objects minus the reference's main: the solver's own
+#    interaction results on a PART state, for the fixtures.
+#   make -f oracle/ref.mk -j8 [fast] [forces]
 REFSRC ?= /root/reference/src/source
 OUT    ?= oracle/_ref
 OBJDIR := $(OUT)/obj
@@ -45,8 +48,15 @@ CXXFLAGS_FAST := -c -O3 -fopenmp -ffast-math -march=x86-64-v3 -std=c++0x -D_WITH
             -DDISABLE_CHRONO -DDISABLE_WAVEGEN -DDISABLE_MOORDYNPLUS -w
 OBJFILES_FAST := $(addprefix $(OBJDIR_FAST)/,$(addsuffix .o,$(OBJS)))
 
+EXE_FORCES := $(OUT)/ref_forces
+OBJFILES_NOMAIN := $(filter-out $(OBJDIR)/main.o,$(OBJFILES))
+
 all: $(EXE)
 fast: $(EXE_FAST)
+forces: $(EXE_FORCES)
+$(EXE_FORCES): $(OBJFILES_NOMAIN) oracle/ref_forces.cpp | $(OBJDIR)
+	$(CXX) -O2 -fopenmp -std=c++0x -D_WITHMR -DDISABLE_CHRONO -DDISABLE_WAVEGEN \
+	  -DDISABLE_MOORDYNPLUS -w -I$(REFSRC) oracle/ref_forces.cpp $(OBJFILES_NOMAIN) -fopenmp -lgomp -o $@
 $(EXE): $(OBJFILES)
 	$(CXX) $(OBJFILES) -fopenmp -lgomp -o $@
 $(EXE_FAST): $(OBJFILES_FAST)
@@ -59,4 +69,4 @@ $(OBJDIR):
 	mkdir -p $(OBJDIR)
 $(OBJDIR_FAST):
 	mkdir -p $(OBJDIR_FAST)
-.PHONY: all fast
+.PHONY: all fast forces

@@ -18,8 +18,21 @@ Other arrays are in id order.
 ref_observables_dp0.01.npz (--observables, ~1 h on 8 cores): front position
 and probe pressure every 0.01 s over 1 s of the dp=0.01 dam break.
 
+ref_forces_dambreak_dp0.03.npz (--forces): the reference solver's own
+interaction results over the first 10 steps of the dp=0.03 dam break
+(DensityDT=3). The reference runs 10 steps saving a PART every step; for
+each saved state, oracle/_ref/ref_forces (a driver of this repository
+linked with the reference's objects) resumes the reference from that PART
+and writes its per-particle ace and ar after Interaction_Forces (the
+predictor interaction of the next step). Arrays per step k = 1..10:
+  order     ids in the reference's particle order [k, np];
+  posf      fluid positions, FP64 [k, nf, 3]; vel [k, nf, 3]; rho [k, np]
+            (boundaries are fixed at the case's positions);
+  ace       [k, nf, 3] (fluid) and ar [k, np], in id order.
+
 Run in the development container (the reference is not available on the
-GPU hosts):  python tests/golden/make_reference_fixtures.py [--observables | --gauges] [names]
+GPU hosts):  python tests/golden/make_reference_fixtures.py [--observables | --gauges |
+             --forces] [names]
 """
 from __future__ import annotations
 
@@ -175,7 +188,64 @@ def gauges_fixture():
     print("wrote ref_gauges_dambreak_dp0.03.npz", {k: v.shape for k, v in arrays.items()})
 
 
+REF_FORCES = ROOT / "oracle" / "_ref" / "ref_forces"
+FORCES_DP, FORCES_STEPS = 0.03, 10
+
+
+def read_forces(path: Path):
+    """(idp, ace [np,3], ar [np]) in the solver's order from a ref_forces file."""
+    b = np.fromfile(path, dtype=np.uint8)
+    n = int(b[:4].view(np.uint32)[0])
+    rec = b[4:].reshape(n, 20)
+    idp = rec[:, :4].copy().view(np.uint32).ravel()
+    f = rec[:, 4:].copy().view(np.float32).reshape(n, 4)
+    return idp, f[:, :3].copy(), f[:, 3].copy()
+
+
+def forces_fixture():
+    """ref_forces_dambreak_dp0.03.npz (see the module doc)."""
+    if not REF_FORCES.exists():
+        sys.exit(f"{REF_FORCES} missing: make -f oracle/ref.mk forces")
+    case = cm.dam_break(dp=FORCES_DP)
+    nb = case.nbound
+    with tempfile.TemporaryDirectory() as td:
+        w = Path(td)
+        parts = run_reference(case, FORCES_STEPS, w)
+        assert np.array_equal(parts[0]["pos"], case.pos)
+        steps = list(range(1, FORCES_STEPS + 1))
+        out = {k: [] for k in ("order", "posf", "vel", "rho", "ace", "ar", "time")}
+        for k in steps:
+            fout = w / f"forces_{k}.bin"
+            cmd = [str(REF_FORCES), str(w / "in" / "case"), str(w / f"f{k}"), "-cpu",
+                   "-ompthreads:8", f"-partbegin:{k}", str(w / "out" / "data"), "-stable:1",
+                   "-svres:0", str(fout)]
+            r = subprocess.run(cmd, cwd=w, capture_output=True, text=True, timeout=3600)
+            if r.returncode != 0:
+                raise RuntimeError(r.stdout[-4000:] + r.stderr[-2000:])
+            idp, ace, ar = read_forces(fout)
+            pt = parts[k]
+            # The resumed solver keeps the PART's order (already sorted by cell).
+            assert np.array_equal(idp, pt["idp_saved"]), f"step {k}: order changed on resume"
+            assert np.array_equal(pt["pos"][:nb], case.pos[:nb]), "fixed boundaries moved"
+            o = np.argsort(idp, kind="stable")
+            out["order"].append(idp)
+            out["posf"].append(pt["pos"][nb:])
+            out["vel"].append(pt["vel"][nb:])
+            out["rho"].append(pt["rho"])
+            out["ace"].append(ace[o][nb:])
+            out["ar"].append(ar[o])
+            out["time"].append(pt["time"])
+    np.savez_compressed(OUT / "ref_forces_dambreak_dp0.03.npz", generator=np.array("dam_break"),
+                        dp=FORCES_DP, seed=42, params=np.array(repr([])), nbound=nb,
+                        nfluid=case.nfluid, steps=np.array(steps),
+                        **{k: np.stack(v) for k, v in out.items()})
+    print(f"wrote ref_forces_dambreak_dp0.03.npz ({case.np} particles, steps {steps})")
+
+
 def main():
+    if "--forces" in sys.argv:
+        forces_fixture()
+        return
     if not REF_EXE.exists():
         sys.exit(f"{REF_EXE} missing: make -f oracle/ref.mk (needs /root/reference)")
     if "--observables" in sys.argv:

@@ -271,3 +271,19 @@ def oracle_mdbc2(c, s, prec, sem, velrho=None):
                          None, float(c.cs0), float(c.dp), vr.ctypes.data, bm.ctypes.data,
                          tv.ctypes.data)
     return vr, bm, tv
+
+
+def reference_forces_state(fx, case, k):
+    """The state of step fx["steps"][k] of a reference-forces fixture as a
+    case (particles in the reference's order, positions FP64): fixed
+    boundaries at the case's positions, fluid at the saved positions."""
+    import dataclasses
+    nb = int(fx["nbound"])
+    pos = case.pos.copy()
+    pos[nb:] = fx["posf"][k]
+    vr = case.velrho.copy()
+    vr[nb:, :3] = fx["vel"][k]
+    vr[:, 3] = fx["rho"][k]
+    order = fx["order"][k]
+    return dataclasses.replace(case, idp=order.astype(np.uint32), pos=pos[order],
+                               velrho=vr[order])

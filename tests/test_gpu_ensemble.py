@@ -124,3 +124,60 @@ def test_unsupported_options_are_errors():
     c2 = casemod.dam_break(dp=0.05, params=casemod.Parameters(tdensity=N.DDT_DDT2))
     with pytest.raises(EnsembleError, match="DDT2Full"):
         Ensemble([c2], tend=1.0)
+
+
+@pytest.mark.timeout(1500)
+def test_512_case_ensemble_at_its_configuration():
+    """The benchmark ensemble itself (512 seeded dam breaks, 7.2k-247k
+    particles, 32.9M in all): after 12 steps (4 graph replays of 3 steps) a
+    size-stratified sample of 16 cases equals single-case runs bit for bit,
+    and the smallest and the largest case have forces within 1e-5 (norm-wise)
+    of the FP32 oracle on their state, in the oracle's own particle order."""
+    import dataclasses
+    import sys
+    from dualsphysics_amd.ensemble import dam_break_ensemble
+    from oracle.oracle import OracleSim
+    from tests.helpers import rel_err
+    cases = dam_break_ensemble(512)
+    sizes = np.array([c.np for c in cases])
+    nsteps = 12
+    ens = Ensemble(cases, tend=10.0)
+    try:
+        ens.autotune(reps=1)
+        assert ens.run(max_steps=nsteps) == nsteps
+        order = np.argsort(sizes, kind="stable")
+        pick = sorted(set(order[np.linspace(0, len(cases) - 1, 16).round().astype(int)].tolist()))
+        assert len(pick) == 16 and order[0] in pick and order[-1] in pick
+        for k in pick:
+            c = cases[k]
+            c1 = dataclasses.replace(c, params=dataclasses.replace(c.params, celldomfixed=1))
+            s = SphGpuSingle(c1)
+            try:
+                s.run(nsteps)
+                ref, st = s.download(), s.stats()
+                got = ens.download(k)
+                _assert_same(got, ref)
+                es = ens.state(k)
+                assert es["nstep"] == nsteps and es["time"] == st["timestep"]
+                print(f"case {k}: {c.np} particles, t={st['timestep']:.4e} s: bitwise equal",
+                      flush=True)
+                if k in (order[0], order[-1]):
+                    s.interact()
+                    gace, gar = s.forces()
+                    o = OracleSim(dataclasses.replace(c1, idp=ref["idp"].copy(),
+                                                      pos=ref["pos"].copy(),
+                                                      velrho=ref["velrho"].copy()))
+                    np.testing.assert_array_equal(o.download()["idp"], ref["idp"])
+                    o.interact()
+                    oace, oar = o.forces()
+                    npb = st["npb"]
+                    e_ace = rel_err(gace[npb:], oace[npb:])
+                    e_ar = rel_err(gar, oar)
+                    print(f"case {k}: forces vs oracle ace {e_ace:.2e} ar {e_ar:.2e}", flush=True)
+                    assert e_ace < 1e-5 and e_ar < 1e-5
+                    o.close()
+            finally:
+                s.close()
+            sys.stdout.flush()
+    finally:
+        ens.close()
