@@ -23,6 +23,7 @@ BMODE_DBC, BMODE_MDBC2, BMODE_MDBC2OFF = 0, 1, 2
 OPT_FORCES_KERNEL = 1
 FORCES_TILE, FORCES_ROWS = 0, 1
 NP_LIMIT = 1 << 28              # DSPH_NP_LIMIT: particles per simulation / interaction
+NOALLOC_NP_MAX = 1 << 20        # DSPH_NOALLOC_NP_MAX: particles per no-alloc interaction
 
 u32p = C.POINTER(C.c_uint32)
 
@@ -138,6 +139,7 @@ SIGNATURES = {
     "dsph_sort_data_d2_d_f4": [_vp, _u, _u, _vp] + [_vp] * 6,
     "dsph_update_poscell": [_vp, _u, C.POINTER(_d), _f, _vp, _vp, _vp],
     "dsph_interaction_forces": [_vp, C.POINTER(DsphInterParms)],
+    "dsph_interaction_forces_noalloc": [_vp, C.POINTER(DsphInterParms)],
     "dsph_sps_tau": [_vp, _u, _u, _f, _f, _vp, _vp, _vp],
     "dsph_mdbc_correction": [_vp, _u, C.POINTER(DsphDivData), C.POINTER(_d)] + [_vp] * 6,
     "dsph_mdbc2_correction": [_vp, _u, C.POINTER(_f), C.POINTER(DsphDivData),

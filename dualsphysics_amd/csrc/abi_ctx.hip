@@ -59,6 +59,10 @@ int dsph_ctx_create(int device, dsph_ctx** out) {
     dsph_ctx_destroy(c);
     return DSPH_E_NOMEM;
   }
+  if (noalloc_scratch_init(c) != DSPH_OK) {
+    dsph_ctx_destroy(c);
+    return DSPH_E_LAUNCH;
+  }
   *out = c;
   return DSPH_OK;
 }

@@ -38,6 +38,9 @@ struct dsph_ctx {
   float* prep_posf = nullptr;      // float4 per particle
   size_t prep_cap = 0;
   int forces_kernel = 0;           // DSPH_OPT_FORCES_KERNEL: 0 queue, 1 rows
+  // Device address of the static scratch of dsph_interaction_forces_noalloc
+  // on this context's device (set by dsph_ctx_create).
+  void* noalloc_scratch = nullptr;
 };
 
 namespace dsph {
@@ -112,5 +115,8 @@ int radix_sort(dsph_ctx* ctx, uint32_t* d_keys, uint32_t* d_values, uint32_t n,
                uint32_t key_limit, uint32_t* buf, size_t cap);
 int ensure_redu_scratch(dsph_ctx* ctx, uint32_t n);
 int ensure_prep_scratch(dsph_ctx* ctx, uint32_t n);
+// Looks up the device address of the no-alloc scratch on the context's
+// device (current at the call) and keeps it in the context (forces.hip).
+int noalloc_scratch_init(dsph_ctx* ctx);
 
 }  // namespace dsph

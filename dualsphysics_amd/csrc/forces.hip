@@ -21,6 +21,8 @@
 //    order is unchanged.
 #include <cfloat>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <type_traits>
 #include "ensemble.h"
 #include "sph_math.h"
@@ -860,8 +862,12 @@ int dsph_ctx_set_option(dsph_ctx* ctx, int key, int value) {
   }
 }
 
-int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
-  if (!ctx || !t) return DSPH_E_ARG;
+}  // extern "C"
+
+namespace {
+
+// Argument checks shared by the interaction entry points.
+int check_inter_parms(dsph_ctx* ctx, const dsph_inter_parms* t) {
   if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
   DSPH_REQUIRE(ctx, t->tkernel == DSPH_KERNEL_WENDLAND, "only the Wendland kernel is built");
   DSPH_REQUIRE(ctx, t->np >= t->npb && t->npb >= t->npbok, "invalid particle counts");
@@ -878,14 +884,17 @@ int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
   const bool sps = t->tvisco == DSPH_VISCO_LAMINARSPS;
   DSPH_REQUIRE(ctx, !sps || (t->d_spstaurho2 && t->d_sps2strain),
                "Laminar+SPS needs spstaurho2 and sps2strain");
+  return DSPH_OK;
+}
+
+// The interaction with the per-particle preparation in press/posf (np
+// entries each): k_prep, the fluid kernel, the boundary kernel.
+int interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t, float2* press, float4* posf) {
   const Cte c = make_cte(ctx->cte);
   const DivData dvd = make_divdata(t->divdata);
   const unsigned npf = t->np - t->npb;
+  int rc = DSPH_OK;
   if (npf) {
-    int rc = ensure_prep_scratch(ctx, t->np);
-    if (rc) return rc;
-    float2* press = reinterpret_cast<float2*>(ctx->prep_press);
-    float4* posf = reinterpret_cast<float4*>(ctx->prep_posf);
     {
       KTimer kt(ctx, "k_prep");
       k_prep<<<grid_for(t->np, kBs), kBs, 0, ctx->stream>>>(
@@ -919,6 +928,70 @@ int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
     DSPH_LAUNCHED(ctx, "k_forces_bound");
   }
   return DSPH_OK;
+}
+
+// Scratch of dsph_interaction_forces_noalloc: {press, 1/rho} (float2) for
+// DSPH_NOALLOC_NP_MAX particles, then their map-frame positions (float4).
+// Every GPU holds its own copy for the life of the process; host code uses
+// the device address that hipGetSymbolAddress gives on each device, never
+// the array's name.
+constexpr size_t kNoallocPressBytes = size_t(DSPH_NOALLOC_NP_MAX) * sizeof(float2);
+constexpr size_t kNoallocBytes = kNoallocPressBytes + size_t(DSPH_NOALLOC_NP_MAX) * sizeof(float4);
+static __device__ __align__(256) unsigned char scratch[kNoallocBytes];
+
+}  // namespace
+
+namespace dsph {
+
+// Device address of the no-alloc scratch on the current device (the
+// context's device), looked up once per device.
+int noalloc_scratch_init(dsph_ctx* ctx) {
+  static std::mutex mu;
+  static std::map<int, void*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = addr.find(ctx->device);
+  if (it == addr.end()) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return set_error(ctx, DSPH_E_LAUNCH,
+                       std::string("hipGetSymbolAddress(no-alloc scratch): ") +
+                           hipGetErrorString(e));
+    it = addr.emplace(ctx->device, ptr).first;
+  }
+  ctx->noalloc_scratch = it->second;
+  return DSPH_OK;
+}
+
+}  // namespace dsph
+
+extern "C" {
+
+int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
+  if (!ctx || !t) return DSPH_E_ARG;
+  int rc = check_inter_parms(ctx, t);
+  if (rc) return rc;
+  if (t->np - t->npb) {
+    rc = ensure_prep_scratch(ctx, t->np);
+    if (rc) return rc;
+  }
+  return interaction_forces(ctx, t, reinterpret_cast<float2*>(ctx->prep_press),
+                            reinterpret_cast<float4*>(ctx->prep_posf));
+}
+
+int dsph_interaction_forces_noalloc(dsph_ctx* ctx, const dsph_inter_parms* t) {
+  if (!ctx || !t) return DSPH_E_ARG;
+  DSPH_REQUIRE(ctx, t->np <= DSPH_NOALLOC_NP_MAX,
+               "dsph_interaction_forces_noalloc: np " + std::to_string(t->np) +
+                   " > DSPH_NOALLOC_NP_MAX (" + std::to_string(DSPH_NOALLOC_NP_MAX) +
+                   "); use dsph_interaction_forces");
+  int rc = check_inter_parms(ctx, t);
+  if (rc) return rc;
+  if (!ctx->noalloc_scratch)
+    return set_error(ctx, DSPH_E_STATE, "no-alloc scratch address not set up for this device");
+  unsigned char* base = static_cast<unsigned char*>(ctx->noalloc_scratch);
+  return interaction_forces(ctx, t, reinterpret_cast<float2*>(base),
+                            reinterpret_cast<float4*>(base + kNoallocPressBytes));
 }
 
 }  // extern "C"

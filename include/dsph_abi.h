@@ -325,6 +325,23 @@ int dsph_update_poscell(dsph_ctx* ctx, uint32_t np, const double posmin[3],
  * KerInteractionForcesFluid / KerInteractionForcesBound. */
 int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t);
 
+/* Particles per call of dsph_interaction_forces_noalloc: its scratch (24
+ * bytes per particle, 24 MiB) is a static device array that every GPU the
+ * library runs on holds until the process ends. */
+#define DSPH_NOALLOC_NP_MAX 1048576u
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): dsph_interaction_forces with the same arguments and the
+ * same kernels and results, without dsph_ctx_reserve and without any
+ * allocation; the per-particle preparation (pressure, 1/rho, map-frame
+ * positions) lives in the library's static device scratch. np above
+ * DSPH_NOALLOC_NP_MAX is refused with DSPH_E_ARG. All calls on one GPU share
+ * that GPU's scratch: calls that overlap on one GPU (other streams, other
+ * contexts, other threads) race. Kernel timing (dsph_ctx_set_timing) creates
+ * HIP events and must stay off for a call that may not allocate. */
+int dsph_interaction_forces_noalloc(dsph_ctx* ctx, const dsph_inter_parms* t);
+
 /* Replaces cusph::ComputeSpsTau (JSphGpu_ker.h:304). */
 int dsph_sps_tau(dsph_ctx* ctx, uint32_t np, uint32_t npb, float smag,
                  float blin, const float* d_velrho, const float* d_sps2strain,

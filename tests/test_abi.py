@@ -136,3 +136,12 @@ def test_particle_counts_beyond_the_32bit_offset_limit_are_refused():
     with open(HEADER) as f:
         text = f.read()
     assert re.search(r"#define DSPH_NP_LIMIT 268435456u", text)
+
+
+def test_noalloc_limit_matches_header():
+    """The particle limit of the no-alloc interaction entry point is the one
+    the header states, and its scratch (24 bytes per particle) is 24 MiB."""
+    text = HEADER.read_text()
+    m = re.search(r"#define DSPH_NOALLOC_NP_MAX (\d+)u", text)
+    assert m and int(m.group(1)) == N.NOALLOC_NP_MAX
+    assert N.NOALLOC_NP_MAX * (8 + 16) == 24 << 20
