@@ -81,11 +81,13 @@ int ens_prep(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st, const EnsB
 // wmap: one entry per wave of 64 fluid particles (.y = first fluid index of
 // the wave, relative to the case's first fluid particle); every variant runs
 // one 64-lane workgroup per entry.
-int ens_forces_fluid(dsph_ctx* ctx, int variant, int tvisco, int tddt, const EnsCase* cases,
-                     const EnsState* st, const EnsBlock* wmap, unsigned nblocks,
-                     const unsigned* dcell, const float4* poscell, const float4* posf,
-                     const float4* velrho, const float2* press, float* viscdt, float* ar,
-                     float* ace);
+// eos7: every case has gamma = 7 (the kernels evaluate the neighbour
+// pressure from its density instead of gathering it).
+int ens_forces_fluid(dsph_ctx* ctx, int variant, int tvisco, int tddt, bool eos7,
+                     const EnsCase* cases, const EnsState* st, const EnsBlock* wmap,
+                     unsigned nblocks, const unsigned* dcell, const float4* poscell,
+                     const float4* posf, const float4* velrho, const float2* press,
+                     float* viscdt, float* ar, float* ace);
 int ens_forces_bound(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st,
                      const EnsBlock* bmap, unsigned nblocks, const unsigned* dcell,
                      const float4* poscell, const float4* velrho, float* viscdt, float* ar);

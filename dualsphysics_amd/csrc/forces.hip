@@ -315,10 +315,7 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
 // ------------------------------------------------------------ cell-row tile kernel
 constexpr unsigned kTile = 64;   // candidates staged in LDS per tile (one bit each)
 constexpr unsigned kMinBatch = 16;
-#ifndef DSPH_SCAN_UNROLL
-#define DSPH_SCAN_UNROLL 8
-#endif
-constexpr unsigned kScanUnroll = DSPH_SCAN_UNROLL;  // staged candidates tested per loop trip
+constexpr unsigned kScanBlock = 8;  // staged candidates tested per loop trip
 #ifndef DSPH_RING_TILES
 #define DSPH_RING_TILES 12
 #endif
@@ -345,6 +342,44 @@ __device__ inline int uniform_i(int v) { return __builtin_amdgcn_readfirstlane(v
 __device__ inline float uniform_f(float v) {
   return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
 }
+// Shifts the result of d <= thr into bit 0 of bits (bits = 2*bits + (thr >= d)):
+// a compare into VCC and an add with carry, two instructions per candidate.
+__device__ __forceinline__ void shift_in_le(unsigned& bits, float d, float thr) {
+  asm("v_cmp_ge_f32_e32 vcc, %1, %2\n\tv_addc_co_u32_e32 %0, vcc, %0, %0, vcc"
+      : "+v"(bits)
+      : "v"(thr), "v"(d)
+      : "vcc");
+}
+
+// Conservative distance test of one lane against the first nblk blocks of 8
+// staged candidates (wave-uniform nblk <= 8): bit k of the result is set when
+// candidate k passes d = |q2|^2 - 2 q1.q2 <= thr. Bits of candidates beyond
+// the staged ones are undefined (the caller masks them with its window).
+__device__ __forceinline__ unsigned long long scan_tile(const float4* tile, unsigned nblk,
+                                                        float m2x, float m2y, float m2z,
+                                                        float thr) {
+  unsigned w[2];
+#pragma unroll
+  for (unsigned h = 0; h < 2; h++) {
+    const unsigned nb = min(nblk - min(nblk, 4u * h), 4u);
+    unsigned b = 0;
+#pragma unroll 1
+    for (unsigned i = 0; i < nb; i++) {
+#pragma unroll
+      for (unsigned k = 0; k < kScanBlock; k++) {
+        const float4 f = tile[32 * h + kScanBlock * i + k];
+        float d = fmaf(m2x, f.x, f.w);
+        d = fmaf(m2y, f.y, d);
+        d = fmaf(m2z, f.z, d);
+        shift_in_le(b, d, thr);
+      }
+    }
+    // Candidate 32h + j is at bit 8nb-1-j: left-align it to bit 31-j, reverse.
+    w[h] = nb ? __builtin_bitreverse32(b << (32u - kScanBlock * nb)) : 0u;
+  }
+  return ((unsigned long long)w[1] << 32) | w[0];
+}
+
 // Bits [k0, k1) of a 64-bit mask, 0 <= k0 <= k1 <= 64.
 __device__ inline unsigned long long bit_range(unsigned k0, unsigned k1) {
   const unsigned long long hi = k1 >= 64 ? ~0ull : ((1ull << k1) - 1ull);
@@ -375,7 +410,7 @@ struct TileLds {
 };
 
 template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false,
-          bool SHIFT = false>
+          bool SHIFT = false, bool EOS7 = false>
 __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned pinit,
     float viscob, float viscof, const DivData& dvd, const Cte& c, float ks2scan,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
@@ -431,16 +466,24 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
   // k wave-uniform iterations; each lane pops up to k of its queued pairs.
   // Pops the next queued pair and issues its loads (32-bit offsets from the
   // array bases).
+  // EOS7 (gamma = 7): the neighbour's {press, 1/rho} are evaluated from its
+  // density with the functions k_prep applies (so the same bits; the power is
+  // four FP64 products) instead of being gathered: two gathers per pair
+  // instead of three. Otherwise they are loaded.
   auto issue = [&](unsigned& j, float4& ps2, float4& vr2, float2& pr2) {
     pop(j);
     const unsigned jj = j & 0x7fffffffu;
     ps2 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(poscell) + (jj << 4));
     vr2 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(velrho) + (jj << 4));
-    pr2 = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(press) + (jj << 3));
+    if (!EOS7)
+      pr2 = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(press) + (jj << 3));
   };
   auto eval = [&](unsigned j, const float4& ps2, const float4& vr2, const float2& pr2) {
+    const float2 pr =
+        EOS7 ? make_float2(c.cteb * (pow7_f64(vr2.w * c.ovrhopzero) - 1.0f), fast_rcp(vr2.w))
+             : pr2;
     pair_eval<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT>((j >> 31) != 0, j & 0x7fffffffu, ps2, vr2,
-                                                   pr2, q, tau, viscob, viscof, c, a, bmode,
+                                                   pr, q, tau, viscob, viscof, c, a, bmode,
                                                    tangenvel, shiftmode, code);
   };
   // k wave-uniform iterations; each lane pops up to k of its queued pairs.
@@ -514,27 +557,16 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
             }
             if (lane == 0) tbase[wt % kRing] = t | kbit;
             __syncthreads();
-            // Hits of this lane among the staged candidates.
+            // Hits of this lane among the staged candidates (blocks of 8 up
+            // to the last staged one; the window drops the rest).
             unsigned long long hm = 0;
             if (member) {
               const unsigned k0 = lo > t ? min(lo - t, kTile) : 0u;
               const unsigned k1 = hi > t ? min(hi - t, kTile) : 0u;
               const unsigned long long win = k1 > k0 ? bit_range(k0, k1) : 0ull;
               if (win) {
-#pragma unroll 1
-                for (unsigned k0 = 0; k0 < kTile; k0 += kScanUnroll) {
-                  unsigned bits = 0;
-#pragma unroll
-                  for (unsigned k = 0; k < kScanUnroll; k++) {
-                    const float4 f = tile[k0 + k];
-                    float d = fmaf(m2x, f.x, f.w);
-                    d = fmaf(m2y, f.y, d);
-                    d = fmaf(m2z, f.z, d);
-                    bits |= (d <= thr) ? (1u << k) : 0u;
-                  }
-                  hm |= (unsigned long long)bits << k0;
-                }
-                hm &= win;
+                const unsigned nblk = (min(kTile, gfin - t) + kScanBlock - 1) / kScanBlock;
+                hm = scan_tile(tile, nblk, m2x, m2y, m2z, thr) & win;
               }
             }
             masks[wt % kRing][lane] = hm;
@@ -564,9 +596,22 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
   }
 }
 
+// Waves per SIMD the register allocation of the tile kernel must allow. The
+// kernel's speed is set by the latency of its neighbour gathers, so by the
+// waves that cover it: 5 (96 registers, a few bytes spilled outside the pair
+// loop) for the artificial and laminar accumulator sets, measured 10% faster
+// than the 4 waves of the unconstrained allocation; 4 (128 registers) for
+// the larger sets, which would otherwise get 3.
+template <int TVISCO, bool VELGRAD, bool MDBC2, bool SHIFT>
+constexpr int tile_min_waves() {
+  return (TVISCO != DSPH_VISCO_LAMINARSPS && !VELGRAD && !MDBC2 && !SHIFT) ? 5 : 4;
+}
+
 template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false,
-          bool SHIFT = false>
-__global__ void __launch_bounds__(64) k_forces_fluid_tile(
+          bool SHIFT = false, bool EOS7 = false>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(tile_min_waves<TVISCO, VELGRAD, MDBC2, SHIFT>(), 8)))
+k_forces_fluid_tile(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c, float ks2scan,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ posf, const float4* __restrict__ velrho,
@@ -577,7 +622,7 @@ __global__ void __launch_bounds__(64) k_forces_fluid_tile(
     const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr) {
   __shared__ TileLds<kRing> lds;
   const unsigned p = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
-  fluid_tile_body<TVISCO, TDDT, VELGRAD, kRing, MDBC2, SHIFT>(
+  fluid_tile_body<TVISCO, TDDT, VELGRAD, kRing, MDBC2, SHIFT, EOS7>(
       p, p < n, pinit, viscob, viscof, dvd, c, ks2scan, dcell, poscell, posf, velrho, press, tau,
       viscdt, ar, ace, strain, velgrad, lds, bmode, tangenvel, shiftmode, code, shiftposfs);
 }
@@ -668,10 +713,21 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
         (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
         t->d_sps2strain, t->d_velgrad, t->d_boundmode, t->d_tangenvel);
   } else if (ctx->forces_kernel == 0) {
-    k_forces_fluid_tile<TVISCO, TDDT, VELGRAD, kRingTiles><<<grid_for(npf, 64), 64, 0, ctx->stream>>>(
-        npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
-        (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press, t->d_spstaurho2,
-        t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, t->d_velgrad);
+    // gamma = 7 without the SPS or gradient outputs: the neighbour pressure is
+    // evaluated in the kernel instead of being gathered.
+    constexpr bool kEos7Built = !VELGRAD && TVISCO != DSPH_VISCO_LAMINARSPS;
+    if (kEos7Built && c.gamma == 7.f)
+      k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, false, false, kEos7Built>
+          <<<grid_for(npf, 64), 64, 0, ctx->stream>>>(
+              npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
+              (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press,
+              t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, nullptr);
+    else
+      k_forces_fluid_tile<TVISCO, TDDT, VELGRAD, kRingTiles>
+          <<<grid_for(npf, 64), 64, 0, ctx->stream>>>(
+              npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
+              (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press,
+              t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, t->d_velgrad);
   } else {
     k_forces_fluid_rows<TVISCO, TDDT, VELGRAD, false><<<grid_for(npf, kBsForces), kBsForces, 0,
                                                         ctx->stream>>>(
@@ -711,8 +767,10 @@ __global__ void __launch_bounds__(kBs) k_ens_prep(const EnsCase* __restrict__ ca
   prep_one(ec.p0 + l, ec.c, poscell, velrho, press, posf);
 }
 
-template <int TVISCO, int TDDT, unsigned kRing>
-__global__ void __launch_bounds__(64) k_ens_fluid_tile(const EnsCase* __restrict__ cases,
+template <int TVISCO, int TDDT, unsigned kRing, bool EOS7>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(tile_min_waves<TVISCO, false, false, false>(), 8)))
+k_ens_fluid_tile(const EnsCase* __restrict__ cases,
     const EnsState* __restrict__ st, const EnsBlock* __restrict__ wmap,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ posf, const float4* __restrict__ velrho,
@@ -725,7 +783,8 @@ __global__ void __launch_bounds__(64) k_ens_fluid_tile(const EnsCase* __restrict
   const unsigned npf = st[e.x].np - ec.npb;
   if (e.y >= npf) return;
   const unsigned p = e.y + threadIdx.x;
-  fluid_tile_body<TVISCO, TDDT, false, kRing>(p, p < npf, ec.p0 + ec.npb, ec.viscob, ec.viscof,
+  fluid_tile_body<TVISCO, TDDT, false, kRing, false, false, EOS7>(
+      p, p < npf, ec.p0 + ec.npb, ec.viscob, ec.viscof,
                                               ec.dvd, ec.c, ec.ks2scan, dcell, poscell, posf,
                                               velrho, press, nullptr, viscdt, ar, ace, nullptr,
                                               nullptr, lds);
@@ -759,22 +818,22 @@ __global__ void __launch_bounds__(kBsForces) k_ens_bound(const EnsCase* __restri
   bound_body(ec.p0 + l, ec.dvd, ec.c, dcell, poscell, velrho, viscdt, ar);
 }
 
-template <int TVISCO, int TDDT>
+template <int TVISCO, int TDDT, bool EOS7>
 void launch_ens_fluid(dsph_ctx* ctx, int variant, const EnsCase* cases, const EnsState* st,
                       const EnsBlock* wmap, unsigned nb, const unsigned* dcell,
                       const float4* poscell, const float4* posf, const float4* velrho,
                       const float2* press, float* viscdt, float* ar, float* ace) {
   switch (variant) {
     case kEnsTileRing8:
-      k_ens_fluid_tile<TVISCO, TDDT, 8><<<nb, 64, 0, ctx->stream>>>(
+      k_ens_fluid_tile<TVISCO, TDDT, 8, EOS7><<<nb, 64, 0, ctx->stream>>>(
           cases, st, wmap, dcell, poscell, posf, velrho, press, viscdt, ar, ace);
       break;
     case kEnsTileRing12:
-      k_ens_fluid_tile<TVISCO, TDDT, 12><<<nb, 64, 0, ctx->stream>>>(
+      k_ens_fluid_tile<TVISCO, TDDT, 12, EOS7><<<nb, 64, 0, ctx->stream>>>(
           cases, st, wmap, dcell, poscell, posf, velrho, press, viscdt, ar, ace);
       break;
     case kEnsTileRing16:
-      k_ens_fluid_tile<TVISCO, TDDT, 16><<<nb, 64, 0, ctx->stream>>>(
+      k_ens_fluid_tile<TVISCO, TDDT, 16, EOS7><<<nb, 64, 0, ctx->stream>>>(
           cases, st, wmap, dcell, poscell, posf, velrho, press, viscdt, ar, ace);
       break;
     default:
@@ -798,17 +857,19 @@ int ens_prep(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st, const EnsB
   return DSPH_OK;
 }
 
-int ens_forces_fluid(dsph_ctx* ctx, int variant, int tvisco, int tddt, const EnsCase* cases,
-                     const EnsState* st, const EnsBlock* wmap, unsigned nblocks,
-                     const unsigned* dcell, const float4* poscell, const float4* posf,
-                     const float4* velrho, const float2* press, float* viscdt, float* ar,
-                     float* ace) {
+int ens_forces_fluid(dsph_ctx* ctx, int variant, int tvisco, int tddt, bool eos7,
+                     const EnsCase* cases, const EnsState* st, const EnsBlock* wmap,
+                     unsigned nblocks, const unsigned* dcell, const float4* poscell,
+                     const float4* posf, const float4* velrho, const float2* press,
+                     float* viscdt, float* ar, float* ace) {
   if (!nblocks) return DSPH_OK;
   DSPH_REQUIRE(ctx, variant >= 0 && variant < kEnsNumVariants, "ensemble: invalid kernel variant");
   KTimer kt(ctx, "k_forces_fluid");
 #define DSPH_ENS_FLUID(V, D)                                                              \
-  launch_ens_fluid<V, D>(ctx, variant, cases, st, wmap, nblocks, dcell, poscell, posf, velrho, \
-                         press, viscdt, ar, ace)
+  (eos7 ? launch_ens_fluid<V, D, true>(ctx, variant, cases, st, wmap, nblocks, dcell, poscell, \
+                                       posf, velrho, press, viscdt, ar, ace)                  \
+        : launch_ens_fluid<V, D, false>(ctx, variant, cases, st, wmap, nblocks, dcell,        \
+                                        poscell, posf, velrho, press, viscdt, ar, ace))
   if (tvisco == DSPH_VISCO_ARTIFICIAL && tddt == DSPH_DDT_DDT2FULL)
     DSPH_ENS_FLUID(DSPH_VISCO_ARTIFICIAL, DSPH_DDT_DDT2FULL);
   else if (tvisco == DSPH_VISCO_ARTIFICIAL && tddt == DSPH_DDT_NONE)

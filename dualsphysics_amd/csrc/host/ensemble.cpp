@@ -35,6 +35,7 @@ struct dsph_ens {
   std::string err;
   unsigned ncases = 0, ntot = 0, nkeys = 0;
   int tvisco = 0, tddt = 0;
+  bool eos7 = true;  // every case has gamma = 7
   std::vector<EnsCase> hc;
   EnsCase* d_cases = nullptr;
   EnsState* d_st = nullptr;
@@ -111,6 +112,7 @@ struct dsph_ens {
       if (cs.periactive) throw Err(DSPH_E_ARG, "ensembles run without periodic boundaries");
       if (cs.casenmoving || cs.casenfloat) throw Err(DSPH_E_ARG, "ensembles have fixed boundaries");
       if (k == 0) { tvisco = cs.tvisco; tddt = cs.tdensity; }
+      eos7 = eos7 && float(cs.gamma) == 7.f;
       if (cs.tvisco != tvisco || cs.tdensity != tddt)
         throw Err(DSPH_E_ARG, "all cases of an ensemble share the viscosity and density options");
       if (!(tvisco == DSPH_VISCO_ARTIFICIAL || tvisco == DSPH_VISCO_LAMINAR) ||
@@ -300,7 +302,8 @@ struct dsph_ens {
 
   void fluid(unsigned b, int variant) {
     const Bucket& bk = buckets[b];
-    chk(dsph::ens_forces_fluid(ctx, variant, tvisco, tddt, d_cases, d_st, bk.d_wmap, bk.nblocks,
+    chk(dsph::ens_forces_fluid(ctx, variant, tvisco, tddt, eos7, d_cases, d_st, bk.d_wmap,
+                               bk.nblocks,
                                dcell, (const float4*)poscell, (const float4*)posf,
                                (const float4*)velrho, (const float2*)press, viscdt, ar, ace),
         "fluid interaction");

@@ -34,13 +34,25 @@ inline Cte make_cte(const dsph_cte& c) {
   return r;
 }
 
+// x^7 in FP64 rounded once to FP32: x^2 is exact in FP64 and x^3, x^4, x^7
+// round once each, so the FP64 value is within 3 ulp(FP64) of x^7 and its
+// FP32 rounding is the correctly rounded power except when x^7 lies within
+// that distance of an FP32 rounding boundary (about 6e-9 of the ratios).
+__device__ inline float pow7_f64(float x) {
+  const double xd = double(x);
+  const double x2 = xd * xd;
+  return float((x2 * x2) * (x2 * xd));
+}
+
 // Tait EOS (FunSphEos_iker.h:37-59): B*((rho*(1/rho0))^gamma - 1). The power
 // is evaluated in FP64 and rounded once, i.e. the correctly rounded FP32
 // power of the FP32 ratio (the reference's powf is platform dependent; its
-// correctly rounded value is the common target of the kernels and the oracle).
+// correctly rounded value is the common target of the kernels and the oracle);
+// for gamma = 7 by four FP64 products (pow7_f64), cheap enough for the fluid
+// kernel to evaluate per pair instead of loading the neighbour's pressure.
 __device__ inline float eos_press(float rho, const Cte& c) {
   const float x = rho * c.ovrhopzero;
-  const float pw = float(pow(double(x), double(c.gamma)));
+  const float pw = c.gamma == 7.f ? pow7_f64(x) : float(pow(double(x), double(c.gamma)));
   return c.cteb * (pw - 1.0f);
 }
 
