@@ -330,8 +330,9 @@ def main():
                     "frac": round(ach / PEAK_FP32_TFLOPS, 4), "traffic": traffic,
                     "avg_launch_ms": round(ms / cnt, 4), "launches": cnt,
                     "flop_per_launch": flops,
-                    "note": "VALU-bound FP32 kernel; peak = MI355X FP32 vector peak (packed "
-                            "FP32); kernel time from HIP events on the context stream over a "
+                    "note": "FP32 kernel bound by the latency of its neighbour gathers "
+                            "(rocprof counters in profiles/); peak = MI355X FP32 vector peak "
+                            "(packed FP32); kernel time from HIP events on the context stream over a "
                             "second window of the same run (the headline window has no events); "
                             "traffic = rocprofv3 FETCH_SIZE/WRITE_SIZE per launch scaled by a "
                             "1 GiB copy calibration (profiles/)"}
