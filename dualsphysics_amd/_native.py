@@ -60,7 +60,7 @@ class DsphInterParms(C.Structure):
                 ("d_ar", C.c_void_p), ("d_ace", C.c_void_p), ("d_sps2strain", C.c_void_p),
                 ("d_velgrad", C.c_void_p), ("d_boundmode", C.c_void_p),
                 ("d_tangenvel", C.c_void_p), ("shiftmode", C.c_int32),
-                ("d_shiftposfs", C.c_void_p)]
+                ("d_shiftposfs", C.c_void_p), ("d_ftomassp", C.c_void_p)]
 
 
 class DsphMotion(C.Structure):
@@ -69,6 +69,18 @@ class DsphMotion(C.Structure):
                 ("start", C.c_double), ("finish", C.c_double), ("axisp1", C.c_double * 3),
                 ("axisp2", C.c_double * 3), ("angfreq", C.c_double), ("angampl", C.c_double),
                 ("angphase", C.c_double), ("angvel", C.c_double)]
+
+
+class DsphFloating(C.Structure):
+    _fields_ = [("begin", C.c_uint32), ("count", C.c_uint32), ("massbody", C.c_double),
+                ("masspart", C.c_double), ("center", C.c_double * 3),
+                ("inertia", C.c_double * 9), ("velini", C.c_double * 3),
+                ("omegaini", C.c_double * 3)]
+
+
+class DsphFloatingState(C.Structure):
+    _fields_ = [("center", C.c_double * 3)] + [
+        (n, C.c_float * 3) for n in ("angles", "fvel", "fomega", "facelin", "faceang")]
 
 
 class DsphCase(C.Structure):
@@ -85,7 +97,9 @@ class DsphCase(C.Structure):
                                   "casenfloat", "casenfluid")] + [("motion", DsphMotion),
                                                                   ("slipmode", C.c_int32),
         ("shiftmode", C.c_int32), ("shiftcoef", C.c_double), ("shifttfs", C.c_double),
-        ("periactive", C.c_int32), ("perinc", (C.c_double * 3) * 3)]
+        ("periactive", C.c_int32), ("perinc", (C.c_double * 3) * 3),
+        ("ftcount", C.c_uint32), ("pad_ft", C.c_uint32),
+        ("floatings", C.POINTER(DsphFloating))]
 
 
 class DsphMlp(C.Structure):
@@ -164,6 +178,10 @@ SIGNATURES = {
     "dsph_step_pos2": [_vp, _u, _u] + [_vp] * 8,
     "dsph_step_pos": [_vp, _u, _u] + [_vp] * 6,
     "dsph_move_lin_bound": [_vp, _u, _u, _u, C.POINTER(_d), C.POINTER(_f)] + [_vp] * 6,
+    "dsph_ft_calc_ridp": [_vp, _u, _u, _u, _u, _vp, _vp],
+    "dsph_ft_parts_sum_ace": [_vp, _u] + [_vp] * 7,
+    "dsph_ft_parts_update": [_vp, _d, _u, _u, C.POINTER(_f), C.POINTER(_f), C.POINTER(_d)] +
+                            [_vp] * 6,
     "dsph_move_motion_device": [_vp, _u, C.POINTER(DsphMotion), _d] + [_vp] * 8,
     "dsph_move_mat_bound": [_vp, _u, _u, _u, C.POINTER(_d), _d] + [_vp] * 7,
     "dsph_motion_matrix": [C.POINTER(DsphMotion), _d, _d, C.POINTER(_d)],
@@ -192,6 +210,7 @@ SIGNATURES = {
     "dsph_sim_download": [_vp, u32p, _vp, _vp, _vp, _vp],
     "dsph_sim_download_forces": [_vp, _vp, _vp],
     "dsph_sim_pips": [_vp, C.POINTER(C.c_uint64)],
+    "dsph_sim_floating": [_vp, _u, C.POINTER(DsphFloatingState)],
     "dsph_sim_interact": [_vp],
     "dsph_sim_ctx": [_vp],
     "dsph_mlp_param_count": [],

@@ -216,9 +216,10 @@ def write_case(case, directory, name: str, timemax: float | None = None,
         root.set(k, T_DOUBLE3, (0.0, 0.0, 0.0))
     root.set("AxisDiv", T_INT, 0)
     nmov = getattr(case, "nmoving", 0)
-    nfix = nb - nmov
+    nflt = case.nfloat
+    nfix = nb - nflt - nmov
     root.set("CaseNp", T_ULLONG, n).set("CaseNfixed", T_ULLONG, nfix)
-    root.set("CaseNmoving", T_ULLONG, nmov).set("CaseNfloat", T_ULLONG, 0)
+    root.set("CaseNmoving", T_ULLONG, nmov).set("CaseNfloat", T_ULLONG, nflt)
     root.set("CaseNfluid", T_ULLONG, nf)
     root.set("CasePosMin", T_DOUBLE3, tuple(map(float, pmin)))
     root.set("CasePosMax", T_DOUBLE3, tuple(map(float, pmax)))
@@ -265,6 +266,22 @@ def write_case(case, directory, name: str, timemax: float | None = None,
             for key, comp in keys:
                 params[key] = float(p.perinc[a][comp])
     moving, motion = "", ""
+    # Floating bodies (JCasePartBlock_Floating::ReadXml, JCaseParts.cpp:274-297):
+    # mkbound 2.. after the fixed (0) and moving (1) blocks.
+    floating = ""
+    for k, f in enumerate(case.floatings or []):
+        iv = " ".join(f'v{i + 1}{j + 1}="{r(float(f.inertia[3 * i + j]))}"'
+                      for i in range(3) for j in range(3))
+        c3 = lambda tag, x: f'<{tag} x="{r(float(x[0]))}" y="{r(float(x[1]))}" z="{r(float(x[2]))}" />'
+        floating += (f'            <floating mkbound="{2 + k}" mk="{13 + k}" begin="{f.begin}" '
+                     f'count="{f.count}">\n'
+                     f'                <massbody value="{r(float(f.massbody))}" />\n'
+                     f'                <masspart value="{r(float(f.masspart))}" />\n'
+                     f'                {c3("center", f.center)}\n'
+                     f'                <inertia {iv} />\n'
+                     f'                {c3("linearvelini", f.velini)}\n'
+                     f'                {c3("angularvelini", f.omegaini)}\n'
+                     f'            </floating>\n')
     if nmov:
         m = case.motion
         moving = (f'            <moving mkbound="1" mk="12" begin="{nfix}" count="{nmov}" '
@@ -319,9 +336,9 @@ def write_case(case, directory, name: str, timemax: float | None = None,
             <massbound value="{r(case.massbound)}" />
             <massfluid value="{r(case.massfluid)}" />
         </constants>
-        <particles np="{n}" nb="{nb}" nbf="{nfix}" mkboundfirst="11" mkfluidfirst="1">
+        <particles np="{n}" nb="{nb}" nbf="{nfix + nmov}" mkboundfirst="11" mkfluidfirst="1">
             <fixed mkbound="0" mk="11" begin="0" count="{nfix}" />
-{moving}            <fluid mkfluid="0" mk="1" begin="{nb}" count="{nf}" />
+{moving}{floating}            <fluid mkfluid="0" mk="1" begin="{nb}" count="{nf}" />
         </particles>
 {motion}
         <parameters>
@@ -435,6 +452,7 @@ def _root_header(case, name: str, app: str = "dualsphysics_amd") -> "Item":
     JPartDataBi4.cpp:105-190)."""
     n, nb, nf = case.np, case.nbound, case.nfluid
     nmov = getattr(case, "nmoving", 0)
+    nflt = case.nfloat
     pmin, pmax = case.pos.min(0), case.pos.max(0)
     root = Item("JPartDataBi4")
     root.set("Piece", T_UINT, 0).set("Npiece", T_UINT, 1)
@@ -445,8 +463,8 @@ def _root_header(case, name: str, app: str = "dualsphysics_amd") -> "Item":
     root.set("MapPosMax", T_DOUBLE3, tuple(map(float, case.maprealposmax)))
     _set_periodic(root, case)
     root.set("AxisDiv", T_INT, 0)
-    root.set("CaseNp", T_ULLONG, n).set("CaseNfixed", T_ULLONG, nb - nmov)
-    root.set("CaseNmoving", T_ULLONG, nmov).set("CaseNfloat", T_ULLONG, 0)
+    root.set("CaseNp", T_ULLONG, n).set("CaseNfixed", T_ULLONG, nb - nflt - nmov)
+    root.set("CaseNmoving", T_ULLONG, nmov).set("CaseNfloat", T_ULLONG, nflt)
     root.set("CaseNfluid", T_ULLONG, nf)
     root.set("CasePosMin", T_DOUBLE3, tuple(map(float, pmin)))
     root.set("CasePosMax", T_DOUBLE3, tuple(map(float, pmax)))
@@ -491,6 +509,7 @@ def write_part_head(path, case, app: str = "dualsphysics_amd"):
     run resumed from one of its PARTs reads first (JPartsLoad4.cpp:150-190)."""
     p = case.params
     nmov = getattr(case, "nmoving", 0)
+    nflt = case.nfloat
     nb = case.nbound
     pmin, pmax = case.pos.min(0), case.pos.max(0)
     root = Item("JPartDataHead")
@@ -511,10 +530,11 @@ def write_part_head(path, case, app: str = "dualsphysics_amd"):
     root.set("Gamma", T_DOUBLE, case.gamma).set("RhopZero", T_DOUBLE, case.rhop0)
     root.set("MassBound", T_DOUBLE, case.massbound).set("MassFluid", T_DOUBLE, case.massfluid)
     root.set("Gravity", T_FLOAT3, tuple(map(float, case.gravity)))
-    root.set("CaseNp", T_ULLONG, case.np).set("CaseNfixed", T_ULLONG, nb - nmov)
-    root.set("CaseNmoving", T_ULLONG, nmov).set("CaseNfloat", T_ULLONG, 0)
+    root.set("CaseNp", T_ULLONG, case.np).set("CaseNfixed", T_ULLONG, nb - nflt - nmov)
+    root.set("CaseNmoving", T_ULLONG, nmov).set("CaseNfloat", T_ULLONG, nflt)
     root.set("CaseNfluid", T_ULLONG, case.nfluid)
-    blocks = [("Fixed", 11, 0, nb - nmov)] + ([("Moving", 12, 1, nmov)] if nmov else []) + \
+    blocks = [("Fixed", 11, 0, nb - nflt - nmov)] + ([("Moving", 12, 1, nmov)] if nmov else []) + \
+             [("Floating", 13 + k, 2 + k, f.count) for k, f in enumerate(case.floatings or [])] + \
              [("Fluid", 1, 0, case.nfluid)]
     mk = root.item("MkBlocks")
     mk.set("Count", T_UINT, len(blocks))
@@ -618,8 +638,31 @@ def read_case(directory, name: str):
     p.perinc = tuple(tuple(v) for v in perinc)
     parts = ex.find("particles")
     blocks = list(parts)
-    if any(bk.tag == "floating" for bk in blocks):
-        raise ValueError("floating bodies are not built")
+    floatings = []
+    v3 = lambda e, d=(0.0, 0.0, 0.0): d if e is None else \
+        (float(e.get("x")), float(e.get("y")), float(e.get("z")))
+    for bk in blocks:
+        if bk.tag != "floating":
+            continue
+        if any(bk.find(t) is not None for t in ("translation", "translationDOF", "rotation",
+                                                 "rotationDOF", "linearvel", "angularvel",
+                                                 "linearforce", "angularforce")):
+            raise ValueError("floating constraints, imposed velocities or forces are not built")
+        ine = bk.find("inertia")
+        if ine.get("x") is not None:
+            d3 = v3(ine)
+            inertia = (d3[0], 0.0, 0.0, 0.0, d3[1], 0.0, 0.0, 0.0, d3[2])
+        else:
+            inertia = tuple(float(ine.get(f"v{i}{j}", 0.0)) for i in (1, 2, 3) for j in (1, 2, 3))
+        vel = bk.find("linearvelini")
+        vel = bk.find("velini") if vel is None else vel
+        om = bk.find("angularvelini")
+        om = bk.find("omegaini") if om is None else om
+        floatings.append(cm.Floating(
+            begin=int(bk.get("begin")), count=int(bk.get("count")),
+            massbody=float(bk.find("massbody").get("value")),
+            masspart=float(bk.find("masspart").get("value")), center=v3(bk.find("center")),
+            inertia=inertia, velini=v3(vel), omegaini=v3(om)))
     nmov = sum(int(bk.get("count")) for bk in blocks if bk.tag == "moving")
     if sum(1 for bk in blocks if bk.tag == "moving") > 1:
         raise ValueError("several moving objects are not built")
@@ -633,8 +676,10 @@ def read_case(directory, name: str):
     o = np.argsort(idp, kind="stable")
     idp, pos, vel, rho = idp[o], pos[o], vel[o], rho[o]
     nb = int(root.get("CaseNfixed", 0)) + int(root.get("CaseNmoving", 0))
-    if int(root.get("CaseNfloat", 0)):
-        raise ValueError("floating bodies are not built")
+    nflt = int(root.get("CaseNfloat", 0))
+    if nflt != sum(f.count for f in floatings):
+        raise ValueError("CaseNfloat does not match the floating blocks")
+    nb += nflt
     velrho = np.concatenate([vel, rho[:, None]], 1).astype(np.float32)
     bn = part.get_array("BoundNor")
     if bn is not None:   # per particle (file order) or per boundary particle (id order)
@@ -691,4 +736,4 @@ def read_case(directory, name: str):
                    gravity=gravity, hswl=0.0, params=p, idp=idp, pos=pos, velrho=velrho,
                    nbound=nb, nfluid=int(idp.shape[0]) - nb, maprealposmin=tuple(map(float, mn)),
                    maprealposmax=tuple(map(float, mx)), boundnor=boundnor, nmoving=nmov,
-                   motion=motion)
+                   motion=motion, floatings=floatings or None)

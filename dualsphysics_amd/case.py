@@ -21,6 +21,7 @@ JPartsLoad4.cpp:378-399) with ``posmax.z = default + 50%``
 """
 from __future__ import annotations
 
+import ctypes as C
 import dataclasses
 import math
 
@@ -81,6 +82,21 @@ class Parameters:
 
 
 @dataclasses.dataclass
+class Floating:
+    """A floating body: the <floating> block of the case particles
+    (JCaseParts.cpp:274-297). Its particles are the ids [begin, begin+count),
+    inside the boundary id block after the fixed and moving ones."""
+    begin: int
+    count: int
+    massbody: float
+    masspart: float
+    center: tuple
+    inertia: tuple                 # 3x3 about the centre, row-major
+    velini: tuple = (0.0, 0.0, 0.0)
+    omegaini: tuple = (0.0, 0.0, 0.0)
+
+
+@dataclasses.dataclass
 class Case:
     name: str
     dp: float
@@ -102,12 +118,24 @@ class Case:
     maprealposmin: tuple
     maprealposmax: tuple
     boundnor: np.ndarray | None = None   # float32 [nbound,3] (mDBC), not doubled
-    nmoving: int = 0                     # moving boundary ids [nbound-nmoving, nbound)
+    nmoving: int = 0                     # moving boundary ids [npb-nmoving, npb)
     motion: dict | None = None           # rectilinear sinusoid: freq, ampl, phase (3 each), start, finish
+    floatings: list | None = None        # Floating bodies: ids [npb, nbound)
 
     @property
     def np(self) -> int:
         return int(self.idp.shape[0])
+
+    @property
+    def nfloat(self) -> int:
+        """Floating particles (CaseNfloat)."""
+        return sum(f.count for f in self.floatings) if self.floatings else 0
+
+    @property
+    def npb(self) -> int:
+        """Fixed and moving boundary particles (CaseNpb); the floating ones
+        follow them in the boundary id block."""
+        return self.nbound - self.nfloat
 
     def to_native(self) -> N.DsphCase:
         c = N.DsphCase()
@@ -132,13 +160,28 @@ class Case:
             for k in range(3):
                 c.perinc[a][k] = float(p.perinc[a][k]) if a != k else 0.0
         c.casenp, c.casenbound = self.np, self.nbound
-        c.casenfixed = self.nbound - self.nmoving
-        c.casenmoving, c.casenfloat, c.casenfluid = self.nmoving, 0, self.nfluid
+        c.casenfixed = self.npb - self.nmoving
+        c.casenmoving, c.casenfloat, c.casenfluid = self.nmoving, self.nfloat, self.nfluid
+        if self.floatings:
+            arr = (N.DsphFloating * len(self.floatings))()
+            for k, f in enumerate(self.floatings):
+                a = arr[k]
+                a.begin, a.count = int(f.begin), int(f.count)
+                a.massbody, a.masspart = float(f.massbody), float(f.masspart)
+                for i in range(3):
+                    a.center[i] = float(f.center[i])
+                    a.velini[i] = float(f.velini[i])
+                    a.omegaini[i] = float(f.omegaini[i])
+                for i in range(9):
+                    a.inertia[i] = float(f.inertia[i])
+            c.ftcount = len(self.floatings)
+            c.floatings = C.cast(arr, C.POINTER(N.DsphFloating))
+            c._floatings_keep = arr   # the array lives as long as the struct
         if self.nmoving and self.motion:
             m = self.motion
             kind = m.get("type", "rectsinu")
             c.motion.type = {"rectsinu": 1, "rotsinu": 2, "rot": 3}[kind]
-            c.motion.idbegin, c.motion.count = self.nbound - self.nmoving, self.nmoving
+            c.motion.idbegin, c.motion.count = self.npb - self.nmoving, self.nmoving
             if kind == "rectsinu":
                 for k in range(3):
                     c.motion.freq[k] = m["freq"][k]
@@ -241,6 +284,62 @@ def dam_break(dp: float = 0.01, tank=(1.6, 0.67, 0.4), column=(0.4, 0.67, 0.3), 
                 cs0=cs0, b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g),
                 hswl=hswl, params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb,
                 nfluid=nf, maprealposmin=mn, maprealposmax=mx, boundnor=boundnor)
+
+
+def rigid_body(pos: np.ndarray, masspart: float):
+    """Centre of mass and inertia tensor about it (row-major) of equal point
+    masses at ``pos``: the body data GenCase derives from a floating body's
+    particles."""
+    p = np.asarray(pos, np.float64)
+    cen = p.mean(0)
+    r = p - cen
+    r2 = (r * r).sum(1)
+    inertia = masspart * (np.eye(3) * r2.sum() - r.T @ r)
+    return tuple(float(v) for v in cen), tuple(float(v) for v in inertia.ravel())
+
+
+def dam_break_floating(dp: float = 0.03, box=(0.12, 0.12, 0.12), box_center=(0.16, 0.335, 0.3),
+                       rhobody: float = 500.0, velini=(0.0, 0.0, 0.0),
+                       omegaini=(0.0, 0.0, 0.0), params: Parameters | None = None,
+                       name: str | None = None, **kw) -> Case:
+    """The dam break (``dam_break``) with a floating box of density
+    ``rhobody``: the lattice points inside the box ``box`` centred at
+    ``box_center`` become the body's particles (ids after the walls) and
+    the fluid points there are removed. The default box sits half-immersed
+    at the column's free surface, as a box of half the water density floats
+    at rest. Mass per particle rhobody*dp^3; centre and inertia from the
+    (jittered) particles."""
+    base = dam_break(dp=dp, params=params, name=name or f"dambreak_float_dp{dp:g}",
+                     jitter=0.0, **{k: v for k, v in kw.items() if k != "jitter"})
+    nb = base.nbound
+    lo = np.array(box_center) - 0.5 * np.array(box)
+    hi = np.array(box_center) + 0.5 * np.array(box)
+    # Lattice points of the tank interior inside the box.
+    i0 = np.ceil(lo / dp - 0.5).astype(int)
+    i1 = np.floor(hi / dp - 0.5).astype(int) + 1
+    bpts, _ = _lattice(i0[0], i1[0], i0[1], i1[1], i0[2], i1[2], dp)
+    fl = base.pos[nb:]
+    inside = np.all((fl > lo - 0.5 * dp) & (fl < hi + 0.5 * dp), axis=1)
+    fpts = fl[~inside]
+    pos = np.concatenate([base.pos[:nb], bpts, fpts], 0)
+    seed, jitter = kw.get("seed", 42), kw.get("jitter", 1e-4)
+    u = uniform01(seed, 3 * pos.shape[0]).reshape(-1, 3)
+    pos = pos + (2.0 * u - 1.0) * (jitter * dp)
+    nfl, nf = bpts.shape[0], fpts.shape[0]
+    velrho = np.zeros((pos.shape[0], 4), np.float32)
+    velrho[:nb + nfl, 3] = base.rhop0
+    zf = pos[nb + nfl:, 2]
+    g = -base.gravity[2]
+    velrho[nb + nfl:, 3] = (base.rhop0 * (1.0 + base.rhop0 * g * (base.hswl - zf) / base.b)
+                            ** (1.0 / base.gamma)).astype(np.float32)
+    masspart = rhobody * dp ** 3
+    cen, inertia = rigid_body(pos[nb:nb + nfl], masspart)
+    ft = Floating(begin=nb, count=nfl, massbody=masspart * nfl, masspart=masspart, center=cen,
+                  inertia=inertia, velini=tuple(velini), omegaini=tuple(omegaini))
+    mn, mx = map_limits(pos, base.h)
+    return dataclasses.replace(base, idp=np.arange(pos.shape[0], dtype=np.uint32), pos=pos,
+                               velrho=velrho, nbound=nb + nfl, nfluid=nf, maprealposmin=mn,
+                               maprealposmax=mx, boundnor=None, floatings=[ft])
 
 
 def periodic_channel(dp: float = 0.02, length: float = 0.4, width: float = 0.2,

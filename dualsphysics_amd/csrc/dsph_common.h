@@ -26,6 +26,7 @@ constexpr unsigned kCodeMaskType = 0x1800u;
 constexpr unsigned kCodeTypeFixed = 0x0000u;
 constexpr unsigned kCodeTypeMoving = 0x0800u;
 constexpr unsigned kCodeTypeFloating = 0x1000u;
+constexpr unsigned kCodeMaskValue = 0x07ffu;   // CODE_MASKVALUE: the object of its type
 constexpr unsigned kCodeTypeFluid = 0x1800u;
 
 __host__ __device__ inline unsigned code_special(unsigned c) { return c & kCodeMaskSpecial; }

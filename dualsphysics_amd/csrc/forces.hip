@@ -90,6 +90,7 @@ struct P1 {
   float4 ps, vr;
   float press, rrho;
   float tau[6];
+  bool ftp1;   // floating particle (FT kernels only)
 };
 
 // Exact pair body for fluid p1 and neighbour p2 (KerInteractionForcesFluidBox,
@@ -104,17 +105,34 @@ struct P1 {
 // SHIFT: standard shifting (JSphGpu_ker.cu:677-684): accumulates m2/rho2 gradW
 // and -m2/rho2 r.gradW; a boundary neighbour cancels it (x = FLT_MAX) for
 // shiftmode 1 (no boundaries) and, for shiftmode 2, when it is fixed.
-template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false>
+// FT: floating bodies. A floating neighbour (code type floating, in the
+// fluid range) has its body's particle mass ftomassp[type value], stops the
+// Molteni-Colagrossi term when it is not heavier than 1.2 fluid particles,
+// takes no part in the Fourtakas term and gives no SPS tau; a floating p1
+// has no density diffusion and no strain (JSphGpu_ker.cu:570-583, 668,
+// 755-766, 838-843).
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false,
+          bool FT = false>
 __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float4& ps2,
     const float4& vr2, float2 pr2, const P1& q, const float* __restrict__ tau, float viscob,
     float viscof, const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
-    const typecode* __restrict__ code = nullptr) {
+    const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr) {
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
   float massp2 = boundp2 ? c.massb : c.massf;
   if (MDBC2 && boundp2 && bmode[p2] == DSPH_BMODE_MDBC2OFF) massp2 = 0.f;
+  bool ftp2 = false;
+  if (FT && !boundp2) {
+    const unsigned cd = code[p2];
+    ftp2 = code_type(cd) == kCodeTypeFloating;
+    if (ftp2) {
+      const float ftm = ftomassp[cd & kCodeMaskValue];
+      if (TDDT == DSPH_DDT_DDT && ftm <= massp2 * 1.2f) a.delta = FLT_MAX;
+      massp2 = ftm;
+    }
+  }
   const float visco = boundp2 ? viscob : viscof;
   const float fac = wendland_fac(rr2, c);
   const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
@@ -137,7 +155,8 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
     // A boundary neighbour cancels the term with DBC only (JSphGpu_ker.cu:665).
     a.delta = ((boundp2 && c.tboundary == DSPH_BC_DBC) ? FLT_MAX : a.delta + delta);
   }
-  if ((TDDT == DSPH_DDT_DDT2 && a.delta != FLT_MAX) || (TDDT == DSPH_DDT_DDT2FULL && !boundp2)) {
+  if ((TDDT == DSPH_DDT_DDT2 && a.delta != FLT_MAX && !(FT && ftp2)) ||
+      (TDDT == DSPH_DDT_DDT2FULL && !boundp2 && !(FT && (ftp2 || a.delta == FLT_MAX)))) {
     const float rh = 1.f + c.ddtgz * dr.z;
     const float drho = c.rhopzero * pow_invgamma(rh, c) - c.rhopzero;
     const float visc_densi = c.ddtkh * cbar * ((vr2.w - vr1.w) - drho) * rr2eta;
@@ -176,17 +195,19 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
   if (TVISCO == DSPH_VISCO_LAMINARSPS) {
     float sxx = q.tau[0], sxy = q.tau[1], sxz = q.tau[2], syy = q.tau[3], syz = q.tau[4],
           szz = q.tau[5];
-    if (!boundp2) {
+    if (!boundp2 && !(FT && ftp2)) {
       const float* t2 = tau + 6 * size_t(p2);
       sxx += t2[0]; sxy += t2[1]; sxz += t2[2]; syy += t2[3]; syz += t2[4]; szz += t2[5];
     }
     a.ax += massp2 * (sxx * frx + sxy * fry + sxz * frz);
     a.ay += massp2 * (sxy * frx + syy * fry + syz * frz);
     a.az += massp2 * (sxz * frx + syz * fry + szz * frz);
-    const float volp2 = -massp2 * rrho2;
-    float dv = dvx * volp2; a.s[0] += dv * frx; a.s[1] += dv * fry; a.s[2] += dv * frz;
-    dv = dvy * volp2;       a.s[1] += dv * frx; a.s[3] += dv * fry; a.s[4] += dv * frz;
-    dv = dvz * volp2;       a.s[2] += dv * frx; a.s[4] += dv * fry; a.s[5] += dv * frz;
+    if (!(FT && q.ftp1)) {
+      const float volp2 = -massp2 * rrho2;
+      float dv = dvx * volp2; a.s[0] += dv * frx; a.s[1] += dv * fry; a.s[2] += dv * frz;
+      dv = dvy * volp2;       a.s[1] += dv * frx; a.s[3] += dv * fry; a.s[4] += dv * frz;
+      dv = dvz * volp2;       a.s[2] += dv * frx; a.s[4] += dv * fry; a.s[5] += dv * frz;
+    }
   }
   if (VELGRAD) {
     // Full gradient G_ij = sum_b -m_b/rho_b (v_a-v_b)_i dW/dx_j.
@@ -200,20 +221,21 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
 
 // Reference-structured pair: loads PosCell first and the rest only for a
 // real neighbour.
-template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false,
+          bool FT = false>
 __device__ __forceinline__ void pair_body(bool boundp2, unsigned p2, const P1& q,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
     const float2* __restrict__ press, const float* __restrict__ tau, float viscob, float viscof,
     const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
-    const typecode* __restrict__ code = nullptr) {
+    const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr) {
   const float4 ps2 = poscell[p2];
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
-  pair_eval<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT>(boundp2, p2, ps2, velrho[p2], press[p2], q, tau,
-                                                 viscob, viscof, c, a, bmode, tangenvel,
-                                                 shiftmode, code);
+  pair_eval<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT>(boundp2, p2, ps2, velrho[p2], press[p2], q,
+                                                     tau, viscob, viscof, c, a, bmode, tangenvel,
+                                                     shiftmode, code, ftomassp);
 }
 
 template <int TVISCO>
@@ -225,6 +247,7 @@ __device__ inline void load_p1(unsigned p1, const float4* __restrict__ poscell,
   const float2 pr1 = press[p1];
   q.press = pr1.x;
   q.rrho = pr1.y;
+  q.ftp1 = false;
 #pragma unroll
   for (int k = 0; k < 6; k++)
     q.tau[k] = (TVISCO == DSPH_VISCO_LAMINARSPS) ? tau[6 * size_t(p1) + k] : 0.f;
@@ -264,7 +287,8 @@ __device__ inline NgLimits p1_limits(unsigned p1, const unsigned* __restrict__ d
 }
 
 // ------------------------------------------------------------ reference-order rows kernel
-template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false,
+          bool FT = false>
 __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float viscof,
     const DivData& dvd, const Cte& c, const unsigned* __restrict__ dcell,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
@@ -272,12 +296,17 @@ __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float
     float* __restrict__ ar, float* __restrict__ ace, float* __restrict__ strain,
     float* __restrict__ velgrad, const uint8_t* __restrict__ bmode = nullptr,
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
-    const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr) {
+    const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr,
+    const float* __restrict__ ftomassp = nullptr) {
   Acc a;
   acc_init(a);
   if (SHIFT) a.sh = shiftposfs[p1];
   P1 q;
   load_p1<TVISCO>(p1, poscell, velrho, press, tau, q);
+  if (FT) {
+    q.ftp1 = code_type(code[p1]) == kCodeTypeFloating;
+    if (q.ftp1 && TDDT != DSPH_DDT_NONE) a.delta = FLT_MAX;
+  }
   const NgLimits l = p1_limits(p1, dcell, dvd);
   for (int kind = 0; kind < 2; kind++) {
     const int base = kind == 0 ? int(dvd.cellfluid) : 0;
@@ -286,16 +315,17 @@ __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float
         unsigned pini, pfin;
         row_range(dvd.begincell, base + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1, pini, pfin);
         for (unsigned p2 = pini; p2 < pfin; p2++)
-          pair_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT>(kind == 1, p2, q, poscell, velrho,
-                                                         press, tau, viscob, viscof, c, a, bmode,
-                                                         tangenvel, shiftmode, code);
+          pair_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT>(kind == 1, p2, q, poscell, velrho,
+                                                             press, tau, viscob, viscof, c, a,
+                                                             bmode, tangenvel, shiftmode, code,
+                                                             ftomassp);
       }
   }
   store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
   if (SHIFT) shiftposfs[p1] = a.sh;
 }
 
-template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2, bool SHIFT = false>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2, bool SHIFT = false, bool FT = false>
 __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
@@ -303,13 +333,13 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
     const float* __restrict__ tau, float* __restrict__ viscdt, float* __restrict__ ar,
     float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad,
     const uint8_t* __restrict__ bmode, const float* __restrict__ tangenvel, int shiftmode = 0,
-    const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr) {
+    const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr,
+    const float* __restrict__ ftomassp = nullptr) {
   const unsigned p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  fluid_rows_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT>(p + pinit, viscob, viscof, dvd, c, dcell,
-                                                       poscell, velrho, press, tau, viscdt, ar,
-                                                       ace, strain, velgrad, bmode, tangenvel,
-                                                       shiftmode, code, shiftposfs);
+  fluid_rows_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT>(
+      p + pinit, viscob, viscof, dvd, c, dcell, poscell, velrho, press, tau, viscdt, ar, ace,
+      strain, velgrad, bmode, tangenvel, shiftmode, code, shiftposfs, ftomassp);
 }
 
 // ------------------------------------------------------------ cell-row tile kernel
@@ -629,9 +659,12 @@ k_forces_fluid_tile(
 
 // Boundary particle p1 against fluid neighbours (KerInteractionForcesBound,
 // JSphGpu_ker.cu:450-533).
+// FT: a floating neighbour has its body's particle mass (JSphGpu_ker.cu:470-483).
+template <bool FT = false>
 __device__ __forceinline__ void bound_body(unsigned p1, const DivData& dvd, const Cte& c,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
-    const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar) {
+    const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar,
+    const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr) {
   float visc = 0, arp1 = 0;
   const float4 ps1 = poscell[p1];
   const float4 vr1 = velrho[p1];
@@ -650,7 +683,12 @@ __device__ __forceinline__ void bound_body(unsigned p1, const DivData& dvd, cons
           const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
           const float4 vr2 = velrho[p2];
           const float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
-          arp1 += c.massf * (dvx * frx + dvy * fry + dvz * frz) * fast_div(vr1.w, vr2.w);
+          float massp2 = c.massf;
+          if (FT) {
+            const unsigned cd = code[p2];
+            if (code_type(cd) == kCodeTypeFloating) massp2 = ftomassp[cd & kCodeMaskValue];
+          }
+          arp1 += massp2 * (dvx * frx + dvy * fry + dvz * frz) * fast_div(vr1.w, vr2.w);
           const float dot = dr.x * dvx + dr.y * dvy + dr.z * dvz;
           const float dot_rr2 = fast_div(dot, rr2 + c.eta2);
           visc = fmaxf(dot_rr2, visc);
@@ -663,12 +701,14 @@ __device__ __forceinline__ void bound_body(unsigned p1, const DivData& dvd, cons
   }
 }
 
+template <bool FT = false>
 __global__ void __launch_bounds__(kBsForces) k_forces_bound(unsigned n, DivData dvd, Cte c,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
-    const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar) {
+    const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar,
+    const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr) {
   const unsigned p1 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p1 >= n) return;
-  bound_body(p1, dvd, c, dcell, poscell, velrho, viscdt, ar);
+  bound_body<FT>(p1, dvd, c, dcell, poscell, velrho, viscdt, ar, code, ftomassp);
 }
 
 // Radius^2 of the conservative scan test: (2h + slack)^2 where the slack
@@ -681,7 +721,17 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
                   const float2* press, const float4* posf) {
   KTimer kt(ctx, "k_forces_fluid");
   const unsigned npf = t->np - t->npb;
-  if (t->d_shiftposfs && ctx->forces_kernel == 0) {
+  if (t->d_ftomassp) {
+    // Floating bodies: the one-lane-per-particle kernel with the floating
+    // rules (shifting, mDBC2 and the velocity gradient are refused with them
+    // by dsph_interaction_forces).
+    if constexpr (!VELGRAD)
+      k_forces_fluid_rows<TVISCO, TDDT, false, false, false, true>
+          <<<grid_for(npf, kBsForces), kBsForces, 0, ctx->stream>>>(
+              npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
+              (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
+              t->d_sps2strain, nullptr, nullptr, nullptr, 0, t->d_code, nullptr, t->d_ftomassp);
+  } else if (t->d_shiftposfs && ctx->forces_kernel == 0) {
     // Shifting in the tile kernel (mDBC2 and the velocity gradient are
     // refused with shifting by dsph_interaction_forces).
     k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, false, true><<<grid_for(npf, 64), 64, 0,
@@ -815,7 +865,7 @@ __global__ void __launch_bounds__(kBsForces) k_ens_bound(const EnsCase* __restri
   const unsigned l = e.y + threadIdx.x;
   if (l >= st[e.x].npbok) return;
   const EnsCase& ec = cases[e.x];
-  bound_body(ec.p0 + l, ec.dvd, ec.c, dcell, poscell, velrho, viscdt, ar);
+  bound_body<false>(ec.p0 + l, ec.dvd, ec.c, dcell, poscell, velrho, viscdt, ar);
 }
 
 template <int TVISCO, int TDDT, bool EOS7>
@@ -942,6 +992,9 @@ int check_inter_parms(dsph_ctx* ctx, const dsph_inter_parms* t) {
                "shifting needs shiftmode 1..3 and the particle codes");
   DSPH_REQUIRE(ctx, !t->d_shiftposfs || (!t->d_boundmode && !t->d_velgrad),
                "shifting with mDBC2 or the velocity gradient is not built");
+  DSPH_REQUIRE(ctx, !t->d_ftomassp || t->d_code, "floating bodies need the particle codes");
+  DSPH_REQUIRE(ctx, !t->d_ftomassp || (!t->d_shiftposfs && !t->d_boundmode && !t->d_velgrad),
+               "floating bodies with shifting, mDBC2 or the velocity gradient are not built");
   const bool sps = t->tvisco == DSPH_VISCO_LAMINARSPS;
   DSPH_REQUIRE(ctx, !sps || (t->d_spstaurho2 && t->d_sps2strain),
                "Laminar+SPS needs spstaurho2 and sps2strain");
@@ -983,9 +1036,14 @@ int interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t, float2* press, 
   }
   if (t->npbok) {
     KTimer kt(ctx, "k_forces_bound");
-    k_forces_bound<<<grid_for(t->npbok, kBsForces), kBsForces, 0, ctx->stream>>>(
-        t->npbok, dvd, c, t->d_dcell, (const float4*)t->d_poscell, (const float4*)t->d_velrho,
-        t->d_viscdt, t->d_ar);
+    if (t->d_ftomassp)
+      k_forces_bound<true><<<grid_for(t->npbok, kBsForces), kBsForces, 0, ctx->stream>>>(
+          t->npbok, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
+          (const float4*)t->d_velrho, t->d_viscdt, t->d_ar, t->d_code, t->d_ftomassp);
+    else
+      k_forces_bound<false><<<grid_for(t->npbok, kBsForces), kBsForces, 0, ctx->stream>>>(
+          t->npbok, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
+          (const float4*)t->d_velrho, t->d_viscdt, t->d_ar);
     DSPH_LAUNCHED(ctx, "k_forces_bound");
   }
   return DSPH_OK;

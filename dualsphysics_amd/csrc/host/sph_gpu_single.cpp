@@ -256,6 +256,21 @@ int dsph_sim_pips(dsph_sim* sim, uint64_t h_res[5]) {
   return rc;
 }
 
+int dsph_sim_floating(dsph_sim* sim, uint32_t cf, dsph_floating_state* out) {
+  if (!sim || !out) return DSPH_E_ARG;
+  if (cf >= sim->ft.size()) {
+    sim->err = "dsph_sim_floating: no floating body " + std::to_string(cf);
+    return DSPH_E_ARG;
+  }
+  const auto& f = sim->ft[cf];
+  for (int k = 0; k < 3; k++) {
+    out->center[k] = f.center[k];
+    out->angles[k] = f.angles[k]; out->fvel[k] = f.fvel[k]; out->fomega[k] = f.fomega[k];
+    out->facelin[k] = f.facelin[k]; out->faceang[k] = f.faceang[k];
+  }
+  return DSPH_OK;
+}
+
 int dsph_sim_interact(dsph_sim* sim) {
   if (!sim) return DSPH_E_ARG;
   try {

@@ -135,6 +135,22 @@ struct dsph_sim {
   float* velgrad = nullptr;
   float* closure_nu = nullptr;
 
+  // ---- floating bodies (StFloatingData, JSph::FtObjs; JSph.cpp:1123-1167)
+  struct FtObj {
+    unsigned begin = 0, count = 0;
+    float mass = 0, massp = 0;
+    double center[3] = {0, 0, 0};
+    float angles[3] = {0, 0, 0}, fvel[3] = {0, 0, 0}, fomega[3] = {0, 0, 0};
+    float facelin[3] = {0, 0, 0}, faceang[3] = {0, 0, 0};
+    float inertia[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  };
+  std::vector<FtObj> ft;
+  float* d_ftomassp = nullptr;    // FtoMasspg: particle mass per body
+  unsigned* d_ftbody = nullptr;   // {first index into d_ridp, count} per body
+  double* d_ftcenter = nullptr;   // FtoCenterg
+  float* d_ftace = nullptr;       // FtoAceg: 6 sums per body
+  unsigned* d_ridp = nullptr;     // RidpMotg for the floating ids
+
   // ---- run state
   double SymplecticDtPre = 0, TimeStep = 0, DtLast = 0;
   double VelMax = 0, AceMax = 0;
@@ -160,7 +176,8 @@ struct dsph_sim {
                     velrhopre2, velrhom1, velrhom1b, ace, ar, viscdt, movxy, movz, sps2strain,
                     spstau, spstau2, boundnor, boundnor2, boundmode, tangenvel, shiftposfs, d_red, d_dt,
                     d_flags, cellpart,
-                    sortpart, begincell, velgrad, closure_nu};
+                    sortpart, begincell, velgrad, closure_nu, d_ftomassp, d_ftbody,
+                    d_ftcenter, d_ftace, d_ridp};
     for (void* p : ptrs) dsph_free(ctx, p);
     dsph_ctx_destroy(ctx);
     ctx = nullptr;
@@ -227,6 +244,33 @@ struct dsph_sim {
       throw Err(DSPH_E_ARG, "periodic boundaries with mDBC no-slip are not built");
     if (PeriActive && ShiftMode)
       throw Err(DSPH_E_ARG, "periodic boundaries with shifting are not built");
+    ft.clear();
+    if (c.casenfloat) {
+      if (!c.ftcount || !c.floatings) throw Err(DSPH_E_ARG, "floating particles without bodies");
+      if (TStep != 2 || TBoundary != DSPH_BC_DBC || ShiftMode || PeriActive)
+        throw Err(DSPH_E_ARG, "floating bodies are built for the Symplectic integrator with DBC "
+                              "boundaries, without shifting or periodic axes");
+      unsigned id = CaseNpb;
+      for (unsigned cf = 0; cf < c.ftcount; cf++) {
+        const dsph_floating& f = c.floatings[cf];
+        if (f.begin != id || !f.count)
+          throw Err(DSPH_E_ARG, "floating bodies must tile the floating ids in order");
+        id += f.count;
+        FtObj o;
+        o.begin = f.begin; o.count = f.count;
+        o.mass = float(f.massbody); o.massp = float(f.masspart);
+        for (int k = 0; k < 3; k++) {
+          o.center[k] = f.center[k];
+          o.fvel[k] = float(f.velini[k]);
+          o.fomega[k] = float(f.omegaini[k]);
+        }
+        for (int k = 0; k < 9; k++) o.inertia[k] = float(f.inertia[k]);
+        ft.push_back(o);
+      }
+      if (id != CaseNbound) throw Err(DSPH_E_ARG, "floating bodies must tile the floating ids");
+      if (ft.size() > 2048) throw Err(DSPH_E_ARG, "more than 2048 floating bodies");
+    }
+    cs.floatings = nullptr;   // the caller's array is read here only
     for (int k = 0; k < 3; k++) {
       MapRealPosMin[k] = c.maprealposmin[k];
       MapRealPosMax[k] = c.maprealposmax[k];
@@ -316,10 +360,16 @@ struct dsph_sim {
     std::vector<double> hxy(2 * size_t(np)), hz(np);
     for (unsigned p = 0; p < np; p++) {
       const unsigned id = hidp[p];
-      // LoadCodeParticles: fixed boundary (mk 0) and fluid (mk 0).
-      // Fixed boundary 0x0000, moving boundary CODE_TYPE_MOVING 0x0800 (mk 1),
-      // fluid 0x1800.
-      hcode[p] = uint16_t(id < cs.casenfixed ? 0x0000 : (id < CaseNbound ? 0x0801 : 0x1800));
+      // LoadCodeParticles: fixed boundary 0x0000, moving boundary
+      // CODE_TYPE_MOVING|1 (mk 1), floating CODE_TYPE_FLOATING|body, fluid
+      // CODE_TYPE_FLUID 0x1800.
+      uint16_t cd = 0x1800;
+      if (id < cs.casenfixed) cd = 0x0000;
+      else if (id < CaseNpb) cd = 0x0801;
+      else if (id < CaseNbound)
+        for (unsigned cf = 0; cf < ft.size(); cf++)
+          if (id - ft[cf].begin < ft[cf].count) cd = uint16_t(0x1000u | cf);
+      hcode[p] = cd;
       const double x = hpos[3 * size_t(p)], y = hpos[3 * size_t(p) + 1], z = hpos[3 * size_t(p) + 2];
       hxy[2 * size_t(p)] = x; hxy[2 * size_t(p) + 1] = y; hz[p] = z;
       // LoadDcellParticles (JSph.cpp:1794-1816).
@@ -367,6 +417,23 @@ struct dsph_sim {
     d_red = alloc<float>(4);
     d_dt = alloc<double>(8);
     d_flags = alloc<unsigned>(4);
+    if (!ft.empty()) {
+      const unsigned nft = unsigned(ft.size());
+      d_ftomassp = alloc<float>(nft);
+      d_ftbody = alloc<unsigned>(2 * nft);
+      d_ftcenter = alloc<double>(3 * nft);
+      d_ftace = alloc<float>(6 * nft);
+      d_ridp = alloc<unsigned>(cs.casenfloat);
+      std::vector<float> mp(nft);
+      std::vector<unsigned> body(2 * nft);
+      for (unsigned cf = 0; cf < nft; cf++) {
+        mp[cf] = ft[cf].massp;
+        body[2 * cf] = ft[cf].begin - CaseNpb;
+        body[2 * cf + 1] = ft[cf].count;
+      }
+      chk(dsph_memcpy_h2d(ctx, d_ftomassp, mp.data(), 4 * nft), "upload ftomassp");
+      chk(dsph_memcpy_h2d(ctx, d_ftbody, body.data(), 8 * nft), "upload ftbody");
+    }
     begincell_cap = 2ull * size_t(DomCells.x) * DomCells.y * DomCells.z + 5;
     begincell = alloc<int>(2 * begincell_cap);
     chk(dsph_ctx_reserve(ctx, NpCap, unsigned(begincell_cap)), "dsph_ctx_reserve");
@@ -713,6 +780,7 @@ struct dsph_sim {
     t.d_tangenvel = tangenvel;
     t.shiftmode = ShiftMode;
     t.d_shiftposfs = shiftposfs;
+    t.d_ftomassp = d_ftomassp;
     chk(dsph_interaction_forces(ctx, &t), "interaction");
     if (TVisco == DSPH_VISCO_LAMINARSPS) {
       if (UseClosure)
@@ -816,6 +884,94 @@ struct dsph_sim {
     BoundChanged = true;
   }
 
+  // ---- JSphGpuSingle::RunFloating (JSphGpuSingle.cpp:827-903) without
+  // Chrono, moorings, external forces, imposed velocities, constraints,
+  // periodic axes or mDBC normals, FtPause 0: the particle sums on the
+  // device, the rigid-body update of each body on the host (JSph::
+  // FtComputeAceVel, JSph.cpp:2565-2660), the particles moved on the device,
+  // and after the corrector JSph::FtUpdateFloatings (JSph.cpp:2689-2704).
+  void ft_compute_ace_vel(unsigned cf, double dt, const float* s6, float fvel[3],
+                          float fomega[3], double fcenter[3]) const {
+    const FtObj& f = ft[cf];
+    float fforcelin[3], fforceang[3], acelin[3], aceang[3];
+    for (int k = 0; k < 3; k++) {
+      fforcelin[k] = s6[k] * f.massp;
+      fforceang[k] = s6[3 + k] * f.massp;
+      acelin[k] = (fforcelin[k] + Gravity[k] * f.mass) / f.mass;
+    }
+    // fmath::RotMatrix3x3 (FunctionsMath.h:343-352); I = (R I0) R^T; inverse
+    // (FunctionsMath.h:132-150).
+    const float cx = std::cos(f.angles[0]), cy = std::cos(f.angles[1]), cz = std::cos(f.angles[2]);
+    const float sx = std::sin(f.angles[0]), sy = std::sin(f.angles[1]), sz = std::sin(f.angles[2]);
+    const float r[9] = {cy * cz, -cy * sz, sy,
+                        sx * sy * cz + cx * sz, -sx * sy * sz + cx * cz, -sx * cy,
+                        -cx * sy * cz + sx * sz, cx * sy * sz + sx * cz, cx * cy};
+    auto mul = [](const float* a, const float* b, float* o) {
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          o[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+    };
+    float ri[9], rt[9], in[9];
+    mul(r, f.inertia, ri);
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) rt[3 * i + j] = r[3 * j + i];
+    mul(ri, rt, in);
+    const float* d = in;
+    const float det = d[0] * d[4] * d[8] + d[1] * d[5] * d[6] + d[2] * d[3] * d[7] -
+                      d[6] * d[4] * d[2] - d[7] * d[5] * d[0] - d[8] * d[3] * d[1];
+    float inv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (det) {
+      inv[0] = (d[4] * d[8] - d[5] * d[7]) / det;
+      inv[1] = -(d[1] * d[8] - d[2] * d[7]) / det;
+      inv[2] = (d[1] * d[5] - d[2] * d[4]) / det;
+      inv[3] = -(d[3] * d[8] - d[5] * d[6]) / det;
+      inv[4] = (d[0] * d[8] - d[2] * d[6]) / det;
+      inv[5] = -(d[0] * d[5] - d[2] * d[3]) / det;
+      inv[6] = (d[3] * d[7] - d[4] * d[6]) / det;
+      inv[7] = -(d[0] * d[7] - d[1] * d[6]) / det;
+      inv[8] = (d[0] * d[4] - d[1] * d[3]) / det;
+    }
+    for (int i = 0; i < 3; i++)
+      aceang[i] = fforceang[0] * inv[3 * i] + fforceang[1] * inv[3 * i + 1] + fforceang[2] * inv[3 * i + 2];
+    for (int k = 0; k < 3; k++) {
+      fcenter[k] = f.center[k] + dt * f.fvel[k];
+      fvel[k] = float(dt * acelin[k] + f.fvel[k]);
+      fomega[k] = float(dt * aceang[k] + f.fomega[k]);
+    }
+  }
+  void run_floating(double dt, bool predictor) {
+    if (ft.empty()) return;
+    const unsigned nft = unsigned(ft.size());
+    chk(dsph_memset_async(ctx, d_ridp, 0xff, 4 * size_t(cs.casenfloat)), "clear ft ridp");
+    chk(dsph_ft_calc_ridp(ctx, Np - Npb, Npb, CaseNpb, CaseNbound, idp, d_ridp), "ft ridp");
+    std::vector<double> cen(3 * nft);
+    for (unsigned cf = 0; cf < nft; cf++)
+      for (int k = 0; k < 3; k++) cen[3 * cf + k] = ft[cf].center[k];
+    chk(dsph_memcpy_h2d(ctx, d_ftcenter, cen.data(), 24 * nft), "upload ft centre");
+    chk(dsph_ft_parts_sum_ace(ctx, nft, d_ftbody, d_ftcenter, d_ridp, posxy, posz, ace, d_ftace),
+        "ft sum ace");
+    std::vector<float> s6(6 * nft);
+    chk(dsph_memcpy_d2h(ctx, s6.data(), d_ftace, 24 * nft), "ft sums readback");
+    for (unsigned cf = 0; cf < nft; cf++) {
+      float fvel[3], fomega[3];
+      double fcenter[3];
+      ft_compute_ace_vel(cf, dt, &s6[6 * cf], fvel, fomega, fcenter);
+      chk(dsph_ft_parts_update(ctx, dt, ft[cf].count, ft[cf].begin - CaseNpb, fvel, fomega,
+                               fcenter, d_ridp, posxy, posz, velrho, dcell, code),
+          "ft update");
+      if (!predictor) {
+        FtObj& f = ft[cf];
+        for (int k = 0; k < 3; k++) {
+          f.center[k] = fcenter[k];
+          f.angles[k] = float(double(f.angles[k]) + double(fomega[k]) * dt);
+          f.facelin[k] = (fvel[k] - f.fvel[k]) / float(dt);
+          f.faceang[k] = (fomega[k] - f.fomega[k]) / float(dt);
+          f.fvel[k] = fvel[k];
+          f.fomega[k] = fomega[k];
+        }
+      }
+    }
+  }
+
   // ---- JSphGpuSingle::ComputeStep_Sym (JSphGpuSingle.cpp:785-821)
   void step_sym() {
     const double dt = SymplecticDtPre;
@@ -825,6 +981,7 @@ struct dsph_sim {
     const double dt_p = dt_variable(0);
     run_shifting(dt, 0.5);
     symplectic_pre(dt);
+    run_floating(dt * .5, true);
     run_cell_divide();
     mdbc(true);
     pre_interaction();
@@ -832,6 +989,7 @@ struct dsph_sim {
     const double dt_c = dt_variable(1);
     run_shifting(dt, 1.0);
     symplectic_cor(dt);
+    run_floating(dt, false);
     run_motion(dt);
     if (device_dt) {
       chk(dsph_dt_variable_min_advance(), "dt advance");
@@ -870,7 +1028,8 @@ struct dsph_sim {
   }
 
   void run(unsigned nsteps, bool sync_dt) {
-    device_dt = (!sync_dt && TStep == 2);
+    // Floating bodies need dt on the host (their update runs there).
+    device_dt = (!sync_dt && TStep == 2 && ft.empty());
     if (device_dt) {
       // The device carries the absolute time so that it is summed step by
       // step as on the host (time += dt), whatever the batching of runs.

@@ -8,6 +8,7 @@
 // scope (floatings, shifting, inlet/outlet, mDBC2, periodic) are not built.
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include "dsph_ctx.h"
 #include "ensemble.h"
 
@@ -140,6 +141,15 @@ __device__ inline void sym_pre_one(unsigned p, bool bound, double dtm,
   const float4 pre = velrhopre[p];
   float4 nw = pre;
   nw.w = float(double(pre.w) + dtm * ar[p]);
+  if (code_type(rcode) == kCodeTypeFloating) {
+    // Floating particle: its velocity and position stay (RunFloating moves
+    // it); density clamped at rho0 (JSphGpuSimple_ker.cu:341-344).
+    nw.w = (nw.w < rhopzero ? rhopzero : nw.w);
+    movxy[p] = make_double2(0.0, 0.0);
+    movz[p] = 0.0;
+    velrho[p] = nw;
+    return;
+  }
   const double dx = double(pre.x) * dtm, dy = double(pre.y) * dtm, dz = double(pre.z) * dtm;
   const bool outrho = (nw.w < rhopoutmin || nw.w > rhopoutmax);
   const float ax = ace[3 * size_t(p)], ay = ace[3 * size_t(p) + 1], az = ace[3 * size_t(p) + 2];
@@ -203,6 +213,13 @@ __device__ inline void sym_cor_one(unsigned p, bool bound, double dt, double dtm
   const float4 pre = velrhopre[p];
   float4 nw = pre;
   nw.w = float(double(pre.w) * (2. - eps) / (2. + eps));
+  if (code_type(rcode) == kCodeTypeFloating) {  // JSphGpuSimple_ker.cu:501-504
+    nw.w = (nw.w < rhopzero ? rhopzero : nw.w);
+    movxy[p] = make_double2(0.0, 0.0);
+    movz[p] = 0.0;
+    velrho[p] = nw;
+    return;
+  }
   const float ax = ace[3 * size_t(p)], ay = ace[3 * size_t(p) + 1], az = ace[3 * size_t(p) + 2];
   nw.x = float(double(pre.x) + (double(ax) + gravity.x) * dt);
   nw.y = float(double(pre.y) + (double(ay) + gravity.y) * dt);
@@ -522,6 +539,81 @@ PosCte make_posc(const dsph_cte& c) {
   }
   r.domx = c.domposminx; r.domy = c.domposminy; r.domz = c.domposminz;
   return r;
+}
+
+// ---------------------------------------------------------------- floating bodies
+// Index of each floating particle by id (KerCalcRidp, JSphGpu_ker.cu:1582-1628):
+// ridp[id - idini] = p for the particles p of [pini, pini+n) with ids in
+// [idini, idfin).
+__global__ void __launch_bounds__(kBs) k_ft_calc_ridp(unsigned n, unsigned pini, unsigned idini,
+    unsigned idfin, const unsigned* __restrict__ idp, unsigned* __restrict__ ridp) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const unsigned p = i + pini;
+  const unsigned id = idp[p];
+  if (id >= idini && id < idfin) ridp[id - idini] = p;
+}
+
+// Sums over the particles of body blockIdx.x of ace and of (pos - centre) x ace
+// (KerFtPartsSumAce, JSphGpu_ker.cu:1938-2012): thread t accumulates the
+// body's particles t, t+256, ... in order, thread 0 then adds the 256
+// partial sums in order, all in FP32 without contraction.
+constexpr unsigned kFtBs = 256;
+__global__ void __launch_bounds__(kFtBs) k_ft_sum_ace(const unsigned* __restrict__ ftbody,
+    const double* __restrict__ center, const unsigned* __restrict__ ridp,
+    const double2* __restrict__ posxy, const double* __restrict__ posz,
+    const float* __restrict__ ace, float* __restrict__ acelinang) {
+#pragma clang fp contract(off)
+  __shared__ float part[6][kFtBs];
+  const unsigned cf = blockIdx.x, tid = threadIdx.x;
+  const unsigned fpini = ftbody[2 * cf], fnp = ftbody[2 * cf + 1];
+  const double cx = center[3 * cf], cy = center[3 * cf + 1], cz = center[3 * cf + 2];
+  float l[3] = {0.f, 0.f, 0.f}, g[3] = {0.f, 0.f, 0.f};
+  for (unsigned fp = tid; fp < fnp; fp += kFtBs) {
+    const unsigned p = ridp[fpini + fp];
+    if (p == UINT_MAX) continue;   // particle not present (excluded)
+    const float ax = ace[3 * size_t(p)], ay = ace[3 * size_t(p) + 1], az = ace[3 * size_t(p) + 2];
+    l[0] += ax; l[1] += ay; l[2] += az;
+    const double2 rxy = posxy[p];
+    const float dx = float(rxy.x - cx), dy = float(rxy.y - cy), dz = float(posz[p] - cz);
+    g[0] += (az * dy - ay * dz);
+    g[1] += (ax * dz - az * dx);
+    g[2] += (ay * dx - ax * dy);
+  }
+  for (int k = 0; k < 3; k++) { part[k][tid] = l[k]; part[3 + k][tid] = g[k]; }
+  __syncthreads();
+  if (tid == 0) {
+    const unsigned ntid = min(fnp, kFtBs);
+    float r[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    for (unsigned t = 0; t < ntid; t++)
+      for (int k = 0; k < 6; k++) r[k] += part[k][t];
+    for (int k = 0; k < 6; k++) acelinang[6 * cf + k] = r[k];
+  }
+}
+
+// Moves the particles of one body (KerFtPartsUpdate, JSphGpu_ker.cu:2027-2075,
+// without periodic axes or mDBC normals): displacement dt times the
+// particle's velocity (KerUpdatePos), then the rigid velocity
+// fvel + fomega x (pos - fcenter), in FP32 without contraction.
+__global__ void __launch_bounds__(kBs) k_ft_parts_update(unsigned n, unsigned fpini, double dt,
+    PosCte c, float3 fvel, float3 fomega, double3 fcenter, const unsigned* __restrict__ ridp,
+    double2* __restrict__ posxy, double* __restrict__ posz, float4* __restrict__ velrho,
+    unsigned* __restrict__ dcell, typecode* __restrict__ code) {
+#pragma clang fp contract(off)
+  const unsigned fp = blockIdx.x * blockDim.x + threadIdx.x;
+  if (fp >= n) return;
+  const unsigned p = ridp[fpini + fp];
+  if (p == UINT_MAX) return;   // particle not present (excluded)
+  float4 vr = velrho[p];
+  const double dx = dt * double(vr.x), dy = dt * double(vr.y), dz = dt * double(vr.z);
+  update_pos(posxy[p], posz[p], dx, dy, dz, false, p, c, posxy, posz, dcell, code);
+  const double2 rxy = posxy[p];
+  const float distx = float(rxy.x - fcenter.x), disty = float(rxy.y - fcenter.y),
+              distz = float(posz[p] - fcenter.z);
+  vr.x = fvel.x + (fomega.y * distz - fomega.z * disty);
+  vr.y = fvel.y + (fomega.z * distx - fomega.x * distz);
+  vr.z = fvel.z + (fomega.x * disty - fomega.y * distx);
+  velrho[p] = vr;
 }
 
 // ComputeAceMod with codes (JSphGpu_ker.cu:260-275): |ace|^2 of normal
@@ -901,6 +993,53 @@ int dsph_step_pos(dsph_ctx* ctx, uint32_t np, uint32_t npb, const double* d_movx
       n, npb, make_posc(ctx->cte), (const double2*)d_movxy, d_movz, (double2*)d_posxy, d_posz,
       d_dcell, d_code);
   DSPH_LAUNCHED(ctx, "k_step_pos");
+  return DSPH_OK;
+}
+
+int dsph_ft_calc_ridp(dsph_ctx* ctx, uint32_t n, uint32_t pini, uint32_t idini,
+                      uint32_t idfin, const uint32_t* d_idp, uint32_t* d_ridp) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!n) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_idp && d_ridp && idfin >= idini, "dsph_ft_calc_ridp: invalid argument");
+  KTimer kt(ctx, "k_ft_calc_ridp");
+  k_ft_calc_ridp<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(n, pini, idini, idfin, d_idp, d_ridp);
+  DSPH_LAUNCHED(ctx, "k_ft_calc_ridp");
+  return DSPH_OK;
+}
+
+int dsph_ft_parts_sum_ace(dsph_ctx* ctx, uint32_t ftcount, const uint32_t* d_ftbody,
+                          const double* d_center, const uint32_t* d_ridp, const double* d_posxy,
+                          const double* d_posz, const float* d_ace, float* d_acelinang) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!ftcount) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_ftbody && d_center && d_ridp && d_posxy && d_posz && d_ace && d_acelinang,
+               "dsph_ft_parts_sum_ace: null argument");
+  KTimer kt(ctx, "k_ft_sum_ace");
+  k_ft_sum_ace<<<ftcount, kFtBs, 0, ctx->stream>>>(d_ftbody, d_center, d_ridp,
+                                                   (const double2*)d_posxy, d_posz, d_ace,
+                                                   d_acelinang);
+  DSPH_LAUNCHED(ctx, "k_ft_sum_ace");
+  return DSPH_OK;
+}
+
+int dsph_ft_parts_update(dsph_ctx* ctx, double dt, uint32_t count, uint32_t fpini,
+                         const float fvel[3], const float fomega[3], const double fcenter[3],
+                         const uint32_t* d_ridp, double* d_posxy, double* d_posz,
+                         float* d_velrho, uint32_t* d_dcell, uint16_t* d_code) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
+  if (!count) return DSPH_OK;
+  DSPH_REQUIRE(ctx, fvel && fomega && fcenter && d_ridp && d_posxy && d_posz && d_velrho &&
+                        d_dcell && d_code,
+               "dsph_ft_parts_update: null argument");
+  DSPH_REQUIRE(ctx, !ctx->cte.periactive, "dsph_ft_parts_update: periodic axes are not built");
+  KTimer kt(ctx, "k_ft_parts_update");
+  k_ft_parts_update<<<grid_for(count, kBs), kBs, 0, ctx->stream>>>(
+      count, fpini, dt, make_posc(ctx->cte), make_float3(fvel[0], fvel[1], fvel[2]),
+      make_float3(fomega[0], fomega[1], fomega[2]),
+      make_double3(fcenter[0], fcenter[1], fcenter[2]), d_ridp, (double2*)d_posxy, d_posz,
+      (float4*)d_velrho, d_dcell, d_code);
+  DSPH_LAUNCHED(ctx, "k_ft_parts_update");
   return DSPH_OK;
 }
 

@@ -186,6 +186,17 @@ class SphGpuSingle:
         self._check(self.lib.dsph_sim_pips(self._h, res))
         return [int(v) for v in res]
 
+    def floating(self, cf: int = 0) -> dict:
+        """State of floating body cf after the last step: centre, angles,
+        linear and angular velocity and acceleration (StFloatingData)."""
+        st = N.DsphFloatingState()
+        self._check(self.lib.dsph_sim_floating(self._h, cf, C.byref(st)))
+        return {"center": np.array(st.center[:]), "angles": np.array(st.angles[:], np.float32),
+                "fvel": np.array(st.fvel[:], np.float32),
+                "fomega": np.array(st.fomega[:], np.float32),
+                "facelin": np.array(st.facelin[:], np.float32),
+                "faceang": np.array(st.faceang[:], np.float32)}
+
     def set_closure(self, mlp):
         """Uses the learned eddy viscosity of ``mlp`` (a
         :class:`dualsphysics_amd.closure.Mlp` on this simulation's device) in

@@ -159,6 +159,9 @@ typedef struct dsph_inter_parms {
   int32_t shiftmode;             /* DSPH_SHIFT_* (with d_shiftposfs) */
   float* d_shiftposfs;           /* shifting: float4[np] (in/out; fluid part
                                     cleared before the interaction), else NULL */
+  const float* d_ftomassp;       /* floating bodies: mass of the particles of each
+                                    body, indexed by the code's type value
+                                    (FtoMasspg of StInterParmsg), else NULL */
 } dsph_inter_parms;
 
 /* ---------------------------------------------------------------- context */
@@ -581,6 +584,31 @@ typedef struct dsph_motion {
   double angvel;                 /* type 3: degrees/s */
 } dsph_motion;
 
+/* ------------------------------------------------------ floating bodies */
+
+/* Replaces cusph::CalcRidp (JSphGpu_ker.h, KerCalcRidp JSphGpu_ker.cu:1582-1628)
+ * for the floating ids: d_ridp[id - idini] = p for the particles p of
+ * [pini, pini+n) with ids in [idini, idfin) (the others are not written). */
+int dsph_ft_calc_ridp(dsph_ctx* ctx, uint32_t n, uint32_t pini, uint32_t idini,
+                      uint32_t idfin, const uint32_t* d_idp, uint32_t* d_ridp);
+/* Replaces cusph::FtPartsSumAce (JSphGpu_ker.cu:1938-2012): for each body cf
+ * (d_ftbody = {first index into d_ridp, count} per body, d_center double3 per
+ * body), d_acelinang[6cf..6cf+5] = the sum of ace and of (pos - centre) x ace
+ * over its particles, in the reference kernel's order (256 strided partial
+ * sums, then summed in order). */
+int dsph_ft_parts_sum_ace(dsph_ctx* ctx, uint32_t ftcount, const uint32_t* d_ftbody,
+                          const double* d_center, const uint32_t* d_ridp,
+                          const double* d_posxy, const double* d_posz, const float* d_ace,
+                          float* d_acelinang);
+/* Replaces cusph::FtPartsUpdate (JSphGpu_ker.cu:2027-2092) without periodic
+ * axes or mDBC normals: the count particles d_ridp[fpini..] of one body move
+ * by dt times their velocity (KerUpdatePos: position, cell, out-of-domain
+ * code) and take the rigid velocity fvel + fomega x (pos - fcenter). */
+int dsph_ft_parts_update(dsph_ctx* ctx, double dt, uint32_t count, uint32_t fpini,
+                         const float fvel[3], const float fomega[3], const double fcenter[3],
+                         const uint32_t* d_ridp, double* d_posxy, double* d_posz,
+                         float* d_velrho, uint32_t* d_dcell, uint16_t* d_code);
+
 /* Replaces cusph::MoveLinBound (JSphGpu_ker.h:326): displaces the moving
  * boundary particles with id in [idbegin, idbegin+count) by mvpos (FP64,
  * KerUpdatePos rules) and sets their velocity to mvvel. The particles are
@@ -642,6 +670,18 @@ int dsph_pips_count_split_sync(dsph_ctx* ctx, uint32_t nb, uint32_t pinitb,
  */
 typedef struct dsph_sim dsph_sim;
 
+/* A floating body of the case: mirror of the <floating> block of the case
+ * particles (JCaseParts.cpp:274-297) as JSph::LoadCaseConfig keeps it in
+ * StFloatingData (JSph.cpp:1123-1167). Its particles are the ids
+ * [begin, begin+count) within [casenfixed+casenmoving, casenbound). */
+typedef struct dsph_floating {
+  uint32_t begin, count;         /* first id and number of particles */
+  double massbody, masspart;     /* body mass and mass of each of its particles [kg] */
+  double center[3];              /* centre of mass [m] */
+  double inertia[9];             /* inertia tensor about the centre, row-major [kg m^2] */
+  double velini[3], omegaini[3]; /* initial linear [m/s] and angular [rad/s] velocity */
+} dsph_floating;
+
 /* Case description: mirror of the <execution><constants> and <parameters>
  * blocks (JCaseCtes.cpp:208-222, JSph.cpp:663-854) plus the map limits. */
 typedef struct dsph_case {
@@ -668,7 +708,17 @@ typedef struct dsph_case {
   double perinc[3][3];           /* XPeriodicIncY/Z, YPeriodicIncX/Z, ZPeriodicIncX/Y: the
                                     increments [axis][component]; each axis' own
                                     component is set to -MapRealSize (JSph.cpp:2186-2188) */
+  uint32_t ftcount;              /* floating bodies (casenfloat particles in all) */
+  uint32_t pad_ft;
+  const dsph_floating* floatings; /* ftcount entries, in id order; read at creation */
 } dsph_case;
+
+/* State of a floating body (StFloatingData, DualSphDef.h): centre, rotation
+ * angles, linear and angular velocity and acceleration. */
+typedef struct dsph_floating_state {
+  double center[3];
+  float angles[3], fvel[3], fomega[3], facelin[3], faceang[3];
+} dsph_floating_state;
 
 /* Per-step statistics, mirror of the Run.out/Run.csv key numbers. */
 typedef struct dsph_stats {
@@ -711,6 +761,8 @@ int dsph_sim_download_forces(dsph_sim* sim, float* h_ace, float* h_ar);
  * dsph_pips_count_split_sync: {real fluid, real bound, checked fluid,
  * checked bound, real fluid-boundary}. */
 int dsph_sim_pips(dsph_sim* sim, uint64_t h_res[5]);
+/* State of floating body cf after the last step (StFloatingData). */
+int dsph_sim_floating(dsph_sim* sim, uint32_t cf, dsph_floating_state* out);
 /* Enables the learned SPS closure (Laminar+SPS only; m = NULL disables):
  * the interaction also computes the velocity gradient and the SPS tau uses
  * the network's eddy viscosity (dsph_sps_tau_learned). The network buffers

@@ -224,7 +224,17 @@ struct Opts {
   // boundaries, 3 full; codes of the neighbours; 0 = off.
   int shiftmode = 0;
   const typecode* code = nullptr;
+  // Floating bodies: mass of the particles of each body (FtObjs[].massp,
+  // indexed by the code's type value) and the particle codes; NULL = none.
+  const float* ftmassp = nullptr;
 };
+
+// Floating particle test and its body (CODE_IsFloating, CODE_GetTypeValue,
+// DualSphDef.h:164-200).
+inline bool is_floating(const Opts& o, unsigned p) {
+  return o.ftmassp && (o.code[p] & kTypeMask) == kTypeFloating;
+}
+inline float ft_massp(const Opts& o, unsigned p) { return o.ftmassp[o.code[p] & 0x7ffu]; }
 
 // Relative position p1-p2 in the chosen semantics.
 template <typename R>
@@ -295,6 +305,7 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
   const R massp2_kind = R(boundp2 ? c.massb : c.massf);
   const R visco = R(boundp2 ? o.viscob : o.viscof);
   const bool mdbc2 = boundp2 && o.bmode;
+  const bool ftp1 = is_floating(o, p1);
   const float* v1 = velrho + 4 * size_t(p1);
   const R v1x = R(v1[0]), v1y = R(v1[1]), v1z = R(v1[2]), rho1 = R(v1[3]);
   const R press1 = press[p1];
@@ -312,8 +323,16 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
       const float* v2 = velrho + 4 * size_t(p2);
       const R rho2 = R(v2[3]);
       // mDBC2: a boundary particle whose ghost node is dry has no mass
-      // (JSphCpu.cpp:851-854; JSphGpu_ker.cu:585-588).
-      const R massp2 = (mdbc2 && o.bmode[p2] == 2) ? R(0) : massp2_kind;
+      // (JSphCpu.cpp:851-854; JSphGpu_ker.cu:585-588). A floating neighbour
+      // has its body's particle mass and stops the Molteni-Colagrossi term
+      // when it is not heavier than 1.2 fluid particles (DELTA_HEAVYFLOATING,
+      // JSphCpu.cpp:838-846; JSphGpu_ker.cu:570-580).
+      const bool ftp2 = !boundp2 && is_floating(o, p2);
+      R massp2 = (mdbc2 && o.bmode[p2] == 2) ? R(0) : massp2_kind;
+      if (ftp2) {
+        massp2 = R(ft_massp(o, p2));
+        if (o.tddt == 1 && float(massp2) <= c.massf * 1.2f) a.delta_off = true;
+      }
       {
         const R prs = (press1 + press[p2]) / (rho1 * rho2);
         const R p_vpm = -prs * massp2;
@@ -329,7 +348,7 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
         if (boundp2 && c.tboundary != 2) a.delta_off = true;
         else a.delta += visc_densi * dot3 * massp2;
       }
-      if ((o.tddt == 2 || (o.tddt == 3 && !boundp2)) && !a.delta_off) {
+      if ((o.tddt == 2 || (o.tddt == 3 && !boundp2)) && !a.delta_off && !ftp2) {
         const R rh = R(1) + ddtgz * dz;
         const R drho = rho0 * crpow<R>(rh, invgamma) - rho0;
         const R visc_densi = ddtkh * cbar * ((rho2 - rho1) - drho) / (rr2 + eta2);
@@ -372,17 +391,21 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
       if (o.tvisco == 2) {
         const float* t1 = tau + 6 * size_t(p1);
         R t[6] = {R(t1[0]), R(t1[1]), R(t1[2]), R(t1[3]), R(t1[4]), R(t1[5])};
-        if (!boundp2) {
+        // tau of the neighbour when it is fluid; the strain of p1 when p1 is
+        // fluid (JSphCpu.cpp:1019-1033).
+        if (!boundp2 && !ftp2) {
           const float* t2 = tau + 6 * size_t(p2);
           for (int k = 0; k < 6; k++) t[k] += R(t2[k]);
         }
         a.ax += massp2 * (t[0] * frx + t[1] * fry + t[2] * frz);
         a.ay += massp2 * (t[1] * frx + t[3] * fry + t[4] * frz);
         a.az += massp2 * (t[2] * frx + t[4] * fry + t[5] * frz);
-        const R volp2 = -massp2 / rho2;
-        R dv = dvx * volp2; a.s[0] += dv * frx; a.s[1] += dv * fry; a.s[2] += dv * frz;
-        dv = dvy * volp2;   a.s[1] += dv * frx; a.s[3] += dv * fry; a.s[4] += dv * frz;
-        dv = dvz * volp2;   a.s[2] += dv * frx; a.s[4] += dv * fry; a.s[5] += dv * frz;
+        if (!ftp1) {
+          const R volp2 = -massp2 / rho2;
+          R dv = dvx * volp2; a.s[0] += dv * frx; a.s[1] += dv * fry; a.s[2] += dv * frz;
+          dv = dvy * volp2;   a.s[1] += dv * frx; a.s[3] += dv * fry; a.s[4] += dv * frz;
+          dv = dvz * volp2;   a.s[2] += dv * frx; a.s[4] += dv * fry; a.s[5] += dv * frz;
+        }
       }
       if (o.velgrad) {
         const R volp2 = -massp2 / rho2;
@@ -438,6 +461,8 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
       // partial terms are summed and added to ar after both passes
       // (AddDelta, JSphCpuSingle.cpp:726).
       FAcc<R> a1, a2;
+      // No density diffusion for floating particles (JSphCpu.cpp:774-778).
+      a1.delta_off = a2.delta_off = is_floating(o, p1) && o.tddt != 0;
       if (shiftposfs) for (int k = 0; k < 4; k++) a1.sh[k] = shiftposfs[4 * size_t(p1) + k];
       fluid_rows<R>(o, c, false, rf, nf, p1, ps, pos, velrho, press.data(), tau, a1);
       // The CPU build stores the shifting sums after the fluid pass and
@@ -454,6 +479,7 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
       for (int k = 0; k < 9; k++) og[k] = a1.g[k] + a2.g[k];
     } else {
       FAcc<R> a;
+      a.delta_off = is_floating(o, p1) && o.tddt != 0;  // JSphGpu_ker.cu:838-843
       if (shiftposfs) for (int k = 0; k < 4; k++) a.sh[k] = shiftposfs[4 * size_t(p1) + k];
       fluid_rows<R>(o, c, false, rf, nf, p1, ps, pos, velrho, press.data(), tau, a);
       fluid_rows<R>(o, c, true, rb, nb, p1, ps, pos, velrho, press.data(), tau, a);
@@ -497,7 +523,9 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
         const R frx = fac * dx, fry = fac * dy, frz = fac * dz;
         const float* v2 = velrho + 4 * size_t(p2);
         const R dvx = R(v1[0]) - R(v2[0]), dvy = R(v1[1]) - R(v2[1]), dvz = R(v1[2]) - R(v2[2]);
-        arp += massf * (dvx * frx + dvy * fry + dvz * frz) * (R(v1[3]) / R(v2[3]));
+        // A floating neighbour has its body's particle mass (JSphCpu.cpp:673-676).
+        const R massp2 = is_floating(o, p2) ? R(ft_massp(o, p2)) : massf;
+        arp += massp2 * (dvx * frx + dvy * fry + dvz * frz) * (R(v1[3]) / R(v2[3]));
         const R dot = dx * dvx + dy * dvy + dz * dvz;
         visc = std::max(dot / (rr2 + eta2), visc);
       }
@@ -549,6 +577,12 @@ void orc_interaction_mdbc2(int prec, int sem, int tvisco, int tddt, float viscob
 // Force interaction with every option of this path: mDBC2 modes (bmode,
 // tangenvel or NULL) and shifting (shiftmode 1..3, the particle codes and
 // the float4 sums per particle, fluid part cleared by the caller; 0 = off).
+void orc_interaction_ft(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                        unsigned, unsigned, unsigned, const unsigned*, const float*,
+                        const double*, const float*, const float*, float*, float*, float*,
+                        float*, float*, const unsigned char*, const float*, int,
+                        const typecode*, float*, const float*);
+
 void orc_interaction_ext(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
                          const orc_cte* c, const orc_div* d, unsigned np, unsigned npb,
                          unsigned npbok, const unsigned* dcell, const float* poscell,
@@ -556,8 +590,22 @@ void orc_interaction_ext(int prec, int sem, int tvisco, int tddt, float viscob, 
                          float* ar, float* viscdt, float* strain, float* velgrad,
                          const unsigned char* bmode, const float* tangenvel, int shiftmode,
                          const typecode* code, float* shiftposfs) {
+  orc_interaction_ft(prec, sem, tvisco, tddt, viscob, viscof, c, d, np, npb, npbok, dcell,
+                     poscell, pos, velrho, tau, ace, ar, viscdt, strain, velgrad, bmode,
+                     tangenvel, shiftmode, code, shiftposfs, nullptr);
+}
+
+// The same with floating bodies: ftmassp = mass of the particles of each body
+// (indexed by the code's type value), with the particle codes; NULL = none.
+void orc_interaction_ft(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
+                        const orc_cte* c, const orc_div* d, unsigned np, unsigned npb,
+                        unsigned npbok, const unsigned* dcell, const float* poscell,
+                        const double* pos, const float* velrho, const float* tau, float* ace,
+                        float* ar, float* viscdt, float* strain, float* velgrad,
+                        const unsigned char* bmode, const float* tangenvel, int shiftmode,
+                        const typecode* code, float* shiftposfs, const float* ftmassp) {
   Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof, bmode, tangenvel,
-         shiftposfs ? shiftmode : 0, code};
+         shiftposfs ? shiftmode : 0, code, ftmassp && code ? ftmassp : nullptr};
   if (prec == 0)
     interaction<float>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
                        viscdt, strain, velgrad, o.shiftmode ? shiftposfs : nullptr);
@@ -871,7 +919,7 @@ void orc_sps_tau(unsigned np, unsigned npb, float smag, float blin, const float*
 }
 
 // Symplectic predictor (JSphCpu.cpp:1687-1844; KerComputeStepSymplecticPre,
-// JSphGpuSimple_ker.cu:265-353) for DBC, no floatings/shifting.
+// JSphGpuSimple_ker.cu:265-353) for DBC, floating particles included.
 void orc_sym_pre(unsigned np, unsigned npb, const float* vrpre, const float* ar, const float* ace,
                  double dtm, float rho0, float rhomin, float rhomax, const float g[3],
                  typecode* code, double* mov, float* velrho) {
@@ -885,6 +933,13 @@ void orc_sym_pre(unsigned np, unsigned npb, const float* vrpre, const float* ar,
       continue;
     }
     const float rhonew = float(double(pre[3]) + dtm * ar[p]);
+    if ((code[p] & kTypeMask) == kTypeFloating) {
+      // Floating particles keep their velocity and position (moved by
+      // RunFloating); density clamped at rho0 (JSphCpu.cpp:1795-1799).
+      nw[0] = pre[0]; nw[1] = pre[1]; nw[2] = pre[2];
+      nw[3] = (rhonew < rho0 ? rho0 : rhonew);
+      continue;
+    }
     mov[3 * size_t(p)] = double(pre[0]) * dtm;
     mov[3 * size_t(p) + 1] = double(pre[1]) * dtm;
     mov[3 * size_t(p) + 2] = double(pre[2]) * dtm;
@@ -907,6 +962,11 @@ void orc_sym_cor(unsigned np, unsigned npb, const float* vrpre, const float* ar,
     const float rhonew = float(double(pre[3]) * (2. - eps) / (2. + eps));
     if (p < npb) {
       nw[0] = nw[1] = nw[2] = 0;
+      nw[3] = (rhonew < rho0 ? rho0 : rhonew);
+      continue;
+    }
+    if ((code[p] & kTypeMask) == kTypeFloating) {  // JSphCpu.cpp:1979-1983
+      nw[0] = pre[0]; nw[1] = pre[1]; nw[2] = pre[2];
       nw[3] = (rhonew < rho0 ? rho0 : rhonew);
       continue;
     }

@@ -75,6 +75,11 @@ void orc_interaction_ext(int, int, int, int, float, float, const orc_cte*, const
 void orc_run_shifting(unsigned, unsigned, double, float, float, float, double, const float*,
                       float*);
 void orc_add_shift(unsigned, unsigned, const float*, double*);
+void orc_interaction_ft(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                        unsigned, unsigned, unsigned, const unsigned*, const float*,
+                        const double*, const float*, const float*, float*, float*, float*,
+                        float*, float*, const unsigned char*, const float*, int,
+                        const typecode*, float*, const float*);
 void orc_pips(const orc_cte*, const orc_div*, unsigned, unsigned, unsigned, const unsigned*,
               const float*, unsigned long long*);
 }
@@ -117,6 +122,21 @@ struct Sim {
   float ViscDtMax = 0;
   unsigned NpbPer = 0, NpfPer = 0;
   double MapPosMax[3] = {0, 0, 0};
+  // Floating bodies (StFloatingData, DualSphDef.h; JSph::LoadCaseConfig,
+  // JSph.cpp:1123-1167): body data, particle mass per body and the index of
+  // each floating particle by id (RidpMot).
+  struct FtObj {
+    unsigned begin = 0, count = 0;
+    float mass = 0, massp = 0;
+    double center[3] = {0, 0, 0};
+    float angles[3] = {0, 0, 0}, fvel[3] = {0, 0, 0}, fomega[3] = {0, 0, 0};
+    float facelin[3] = {0, 0, 0}, faceang[3] = {0, 0, 0};
+    float inertia[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  };
+  std::vector<FtObj> ft;
+  std::vector<float> ftmassp;
+  std::vector<unsigned> ridp;
+  unsigned CaseNpb = 0;
 
   void config(const dsph_case& s) {
     cs = s;
@@ -197,6 +217,33 @@ struct Sim {
     for (int k = 0; k < 3; k++) DomCells[k] = unsigned(std::ceil(msize[k] / Scell));
     c.cellcode = orc_cell_code(DomCells[0] + 1, DomCells[1] + 1, DomCells[2] + 1);
     CaseNbound = s.casenbound;
+    CaseNpb = s.casenfixed + s.casenmoving;
+    ft.clear();
+    ftmassp.clear();
+    if (s.casenfloat) {
+      if (!s.ftcount || !s.floatings) throw std::runtime_error("floating particles without bodies");
+      if (TStep != 2 || TBoundary != 1 || ShiftMode || c.periactive)
+        throw std::runtime_error("floating bodies are built for the Symplectic integrator with DBC "
+                                 "boundaries, without shifting or periodic axes");
+      unsigned id = CaseNpb;
+      for (unsigned cf = 0; cf < s.ftcount; cf++) {
+        const dsph_floating& f = s.floatings[cf];
+        if (f.begin != id || !f.count) throw std::runtime_error("floating bodies must tile the floating ids");
+        id += f.count;
+        FtObj o;
+        o.begin = f.begin; o.count = f.count;
+        o.mass = float(f.massbody); o.massp = float(f.masspart);
+        for (int k = 0; k < 3; k++) {
+          o.center[k] = f.center[k];
+          o.fvel[k] = float(f.velini[k]);
+          o.fomega[k] = float(f.omegaini[k]);
+        }
+        for (int k = 0; k < 9; k++) o.inertia[k] = float(f.inertia[k]);
+        ft.push_back(o);
+        ftmassp.push_back(o.massp);
+      }
+      if (id != CaseNbound) throw std::runtime_error("floating bodies must tile the floating ids");
+    }
   }
 
   void load(unsigned np, const unsigned* hidp, const double* hpos, const float* hvr) {
@@ -207,7 +254,16 @@ struct Sim {
     code.resize(np);
     dcell.resize(np);
     for (unsigned p = 0; p < np; p++) {
-      code[p] = typecode(idp[p] < cs.casenfixed ? 0x0000 : (idp[p] < CaseNbound ? 0x0801 : 0x1800));
+      // LoadCodeParticles: fixed 0x0000, moving CODE_TYPE_MOVING|1, floating
+      // CODE_TYPE_FLOATING|body, fluid CODE_TYPE_FLUID.
+      const unsigned id = idp[p];
+      typecode cd = 0x1800;
+      if (id < cs.casenfixed) cd = 0x0000;
+      else if (id < CaseNpb) cd = 0x0801;
+      else if (id < CaseNbound)
+        for (unsigned cf = 0; cf < ft.size(); cf++)
+          if (id - ft[cf].begin < ft[cf].count) cd = typecode(0x1000u | cf);
+      code[p] = cd;
       const double* x = &pos[3 * size_t(p)];
       const unsigned cx = unsigned((x[0] - c.domposmin[0]) / Scell);
       const unsigned cy = unsigned((x[1] - c.domposmin[1]) / Scell);
@@ -412,13 +468,14 @@ struct Sim {
     // prec 0: FP32 GPU semantics, 1: FP64 yardstick, 2: FP32 CPU semantics.
     const bool m2 = mdbc2_on();
     if (ShiftMode) shiftposfs.assign(4 * size_t(Np), 0.f);  // JSphShifting::InitCpu, no zones
-    orc_interaction_ext(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT,
-                        ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
-                        dcell.data(), poscell.data(), pos.data(), velrho.data(),
-                        TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(),
-                        viscdt.data(), TVisco == 2 ? strain.data() : nullptr, nullptr,
-                        m2 ? bmode.data() : nullptr, m2 ? tangenvel.data() : nullptr, ShiftMode,
-                        code.data(), ShiftMode ? shiftposfs.data() : nullptr);
+    orc_interaction_ft(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT,
+                       ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
+                       dcell.data(), poscell.data(), pos.data(), velrho.data(),
+                       TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(),
+                       viscdt.data(), TVisco == 2 ? strain.data() : nullptr, nullptr,
+                       m2 ? bmode.data() : nullptr, m2 ? tangenvel.data() : nullptr, ShiftMode,
+                       code.data(), ShiftMode ? shiftposfs.data() : nullptr,
+                       ft.empty() ? nullptr : ftmassp.data());
     if (TVisco == 2) orc_sps_tau(Np, Npb, SpsSmag, SpsBlin, velrho.data(), strain.data(), tau.data());
     float vd = 0, a2 = 0;
     for (unsigned p = 0; p < Np; p++) vd = std::max(vd, viscdt[p]);
@@ -488,7 +545,8 @@ struct Sim {
     mat_mul(t, rx, t); mat_mul(t, ry, t); mat_mul(t, rz, t);
     mat_mul(t, rym1, t); mat_mul(t, rxm1, t); mat_mul(t, tm1, t);
   }
-  void move_particle(unsigned p, const double d[3]) {
+  void move_particle(unsigned p, const double d[3],
+                     const char* what = "moving boundary particle left the domain") {
     double* x = &pos[3 * size_t(p)];
     for (int k = 0; k < 3; k++) x[k] += d[k];
     double dd[3];
@@ -497,7 +555,7 @@ struct Sim {
       dd[k] = x[k] - c.maprealposmin[k];
       if (dd[k] != dd[k] || dd[k] < 0 || dd[k] >= c.maprealsize[k]) out = true;
     }
-    if (out) throw std::runtime_error("moving boundary particle left the domain");
+    if (out) throw std::runtime_error(what);
     dcell[p] = mkcell_host(unsigned(dd[0] / c.scell), unsigned(dd[1] / c.scell),
                            unsigned(dd[2] / c.scell));
   }
@@ -580,6 +638,154 @@ struct Sim {
                      shiftposfs.data());
   }
 
+  // ---- floating bodies: JSphCpuSingle::RunFloating (JSphCpuSingle.cpp:938-993)
+  // without Chrono, moorings, external forces, imposed velocities or
+  // constraints, and FtPause 0.
+  void calc_ridp() {
+    ridp.assign(cs.casenfloat, UINT_MAX);
+    for (unsigned p = Npb; p < Np; p++) {
+      const unsigned id = idp[p];
+      if (id >= CaseNpb && id < CaseNbound) ridp[id - CaseNpb] = p;
+    }
+    for (unsigned k = 0; k < cs.casenfloat; k++)
+      if (ridp[k] == UINT_MAX) throw std::runtime_error("There are floating particles not found.");
+  }
+  // FtPartsSumAce: sum of ace and of (pos-center) x ace over the particles of
+  // body cf. CPU build (prec 2): in id order (JSphCpuSingle.cpp:1014-1040);
+  // GPU build (prec 0): KerFtPartsSumAce's order (JSphGpu_ker.cu:1938-2012),
+  // 256 partial sums (partial t takes ids t, t+256, ...) then summed in
+  // order; FP64 yardstick (prec 1): in id order in FP64.
+  void ft_sum_ace(unsigned cf, float acelin[3], float aceang[3]) const {
+    const FtObj& f = ft[cf];
+    const unsigned fpini = f.begin - CaseNpb;
+    auto term = [&](unsigned fp, float a[3], float dist[3]) {
+      const unsigned p = ridp[fpini + fp];
+      for (int k = 0; k < 3; k++) {
+        a[k] = ace[3 * size_t(p) + k];
+        dist[k] = float(pos[3 * size_t(p) + k] - f.center[k]);
+      }
+    };
+    if (prec == 1) {
+      double l[3] = {0, 0, 0}, g[3] = {0, 0, 0};
+      for (unsigned fp = 0; fp < f.count; fp++) {
+        float a[3], d[3];
+        term(fp, a, d);
+        for (int k = 0; k < 3; k++) l[k] += a[k];
+        g[0] += double(a[2]) * d[1] - double(a[1]) * d[2];
+        g[1] += double(a[0]) * d[2] - double(a[2]) * d[0];
+        g[2] += double(a[1]) * d[0] - double(a[0]) * d[1];
+      }
+      for (int k = 0; k < 3; k++) { acelin[k] = float(l[k]); aceang[k] = float(g[k]); }
+      return;
+    }
+    const unsigned nt = prec == 0 ? 256u : 1u;
+    const unsigned ntid = std::min(f.count, nt);
+    std::vector<float> pl(3 * size_t(ntid), 0.f), pg(3 * size_t(ntid), 0.f);
+    for (unsigned fp = 0; fp < f.count; fp++) {
+      const unsigned t = fp % nt;
+      float a[3], d[3];
+      term(fp, a, d);
+      for (int k = 0; k < 3; k++) pl[3 * t + k] += a[k];
+      pg[3 * t] += (a[2] * d[1] - a[1] * d[2]);
+      pg[3 * t + 1] += (a[0] * d[2] - a[2] * d[0]);
+      pg[3 * t + 2] += (a[1] * d[0] - a[0] * d[1]);
+    }
+    for (int k = 0; k < 3; k++) acelin[k] = aceang[k] = 0.f;
+    for (unsigned t = 0; t < ntid; t++)
+      for (int k = 0; k < 3; k++) { acelin[k] += pl[3 * t + k]; aceang[k] += pg[3 * t + k]; }
+  }
+  // FtComputeAceVel (JSph.cpp:2565-2660) for one body: accelerations from the
+  // particle sums, gravity and the inertia tensor rotated to the body's
+  // angles; new centre, linear and angular velocity over dt.
+  void ft_compute_ace_vel(unsigned cf, double dt, const float sl[3], const float sa[3],
+                          float fvel[3], float fomega[3], double fcenter[3]) const {
+    const FtObj& f = ft[cf];
+    float fforcelin[3], fforceang[3], acelin[3], aceang[3];
+    for (int k = 0; k < 3; k++) {
+      fforcelin[k] = sl[k] * f.massp;
+      fforceang[k] = sa[k] * f.massp;
+      acelin[k] = (fforcelin[k] + g[k] * f.mass) / f.mass;
+    }
+    // fmath::RotMatrix3x3 (FunctionsMath.h:343-352), I = (R I0) R^T, inverse.
+    const float cx = std::cos(f.angles[0]), cy = std::cos(f.angles[1]), cz = std::cos(f.angles[2]);
+    const float sx = std::sin(f.angles[0]), sy = std::sin(f.angles[1]), sz = std::sin(f.angles[2]);
+    const float r[9] = {cy * cz, -cy * sz, sy,
+                        sx * sy * cz + cx * sz, -sx * sy * sz + cx * cz, -sx * cy,
+                        -cx * sy * cz + sx * sz, cx * sy * sz + sx * cz, cx * cy};
+    auto mul = [](const float* a, const float* b, float* o) {
+      for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+          o[3 * i + j] = a[3 * i] * b[j] + a[3 * i + 1] * b[3 + j] + a[3 * i + 2] * b[6 + j];
+    };
+    float ri[9], rt[9], in[9];
+    mul(r, f.inertia, ri);
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) rt[3 * i + j] = r[3 * j + i];
+    mul(ri, rt, in);
+    const float* d = in;
+    const float det = d[0] * d[4] * d[8] + d[1] * d[5] * d[6] + d[2] * d[3] * d[7] -
+                      d[6] * d[4] * d[2] - d[7] * d[5] * d[0] - d[8] * d[3] * d[1];
+    float inv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (det) {  // fmath::InverseMatrix3x3 (FunctionsMath.h:132-150)
+      inv[0] = (d[4] * d[8] - d[5] * d[7]) / det;
+      inv[1] = -(d[1] * d[8] - d[2] * d[7]) / det;
+      inv[2] = (d[1] * d[5] - d[2] * d[4]) / det;
+      inv[3] = -(d[3] * d[8] - d[5] * d[6]) / det;
+      inv[4] = (d[0] * d[8] - d[2] * d[6]) / det;
+      inv[5] = -(d[0] * d[5] - d[2] * d[3]) / det;
+      inv[6] = (d[3] * d[7] - d[4] * d[6]) / det;
+      inv[7] = -(d[0] * d[7] - d[1] * d[6]) / det;
+      inv[8] = (d[0] * d[4] - d[1] * d[3]) / det;
+    }
+    for (int i = 0; i < 3; i++)
+      aceang[i] = fforceang[0] * inv[3 * i] + fforceang[1] * inv[3 * i + 1] + fforceang[2] * inv[3 * i + 2];
+    for (int k = 0; k < 3; k++) {
+      fcenter[k] = f.center[k] + dt * f.fvel[k];
+      fvel[k] = float(dt * acelin[k] + f.fvel[k]);
+      fomega[k] = float(dt * aceang[k] + f.fomega[k]);
+    }
+  }
+  // FtPartsUpdate (JSphCpuSingle.cpp:1045-1105) without periodic axes and
+  // mDBC normals: each particle moves by dt times its velocity (UpdatePos)
+  // and takes the rigid velocity about the new centre.
+  void ft_parts_update(unsigned cf, double dt, const float fvel[3], const float fomega[3],
+                       const double fcenter[3]) {
+    const FtObj& f = ft[cf];
+    const unsigned fpini = f.begin - CaseNpb;
+    for (unsigned fp = 0; fp < f.count; fp++) {
+      const unsigned p = ridp[fpini + fp];
+      float* vr = &velrho[4 * size_t(p)];
+      const double dd[3] = {dt * double(vr[0]), dt * double(vr[1]), dt * double(vr[2])};
+      move_particle(p, dd, "floating particle left the domain");
+      float dist[3];
+      for (int k = 0; k < 3; k++) dist[k] = float(pos[3 * size_t(p) + k] - fcenter[k]);
+      vr[0] = fvel[0] + (fomega[1] * dist[2] - fomega[2] * dist[1]);
+      vr[1] = fvel[1] + (fomega[2] * dist[0] - fomega[0] * dist[2]);
+      vr[2] = fvel[2] + (fomega[0] * dist[1] - fomega[1] * dist[0]);
+    }
+  }
+  void run_floating(double dt, bool predictor) {
+    if (ft.empty()) return;
+    calc_ridp();
+    for (unsigned cf = 0; cf < ft.size(); cf++) {
+      float sl[3], sa[3], fvel[3], fomega[3];
+      double fcenter[3];
+      ft_sum_ace(cf, sl, sa);
+      ft_compute_ace_vel(cf, dt, sl, sa, fvel, fomega, fcenter);
+      ft_parts_update(cf, dt, fvel, fomega, fcenter);
+      if (!predictor) {  // FtUpdateFloatings (JSph.cpp:2689-2704)
+        FtObj& f = ft[cf];
+        for (int k = 0; k < 3; k++) {
+          f.center[k] = fcenter[k];
+          f.angles[k] = float(double(f.angles[k]) + double(fomega[k]) * dt);
+          f.facelin[k] = (fvel[k] - f.fvel[k]) / float(dt);
+          f.faceang[k] = (fomega[k] - f.fomega[k]) / float(dt);
+          f.fvel[k] = fvel[k];
+          f.fomega[k] = fomega[k];
+        }
+      }
+    }
+  }
+
   void step_sym() {
     const double dt = SymplecticDtPre;
     mdbc();
@@ -593,6 +799,7 @@ struct Sim {
     if (ShiftMode) orc_add_shift(Np, Npb, shiftposfs.data(), mov.data());
     if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrhopre.data(), velrho.data());
     orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
+    run_floating(dt * .5, true);
     cell_divide();
     mdbc(true);
     interaction();
@@ -604,6 +811,7 @@ struct Sim {
     if (ShiftMode) orc_add_shift(Np, Npb, shiftposfs.data(), mov.data());
     if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrhopre.data(), velrho.data());
     orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
+    run_floating(dt, false);
     pre_active = false;
     run_motion(dt);
     SymplecticDtPre = std::min(dt_p, dt_c);
@@ -732,6 +940,19 @@ void orc_sim_pips(void* h, unsigned long long res[4]) {
 }
 // Re-runs the interaction on the current state (forces of the current state).
 void orc_sim_interact(void* h) { static_cast<Sim*>(h)->interaction(); }
+
+// State of floating body cf (-1 when there is none).
+int orc_sim_floating(void* h, unsigned cf, dsph_floating_state* out) {
+  const Sim* s = static_cast<Sim*>(h);
+  if (cf >= s->ft.size()) return -1;
+  const auto& f = s->ft[cf];
+  for (int k = 0; k < 3; k++) {
+    out->center[k] = f.center[k];
+    out->angles[k] = f.angles[k]; out->fvel[k] = f.fvel[k]; out->fomega[k] = f.fomega[k];
+    out->facelin[k] = f.facelin[k]; out->faceang[k] = f.faceang[k];
+  }
+  return 0;
+}
 // Neighbour-list arrays of the current state: dcell, poscell (float4), and
 // the constants used (orc_cte).
 void orc_sim_arrays(void* h, unsigned* dcell, float* poscell, orc_cte* cte) {

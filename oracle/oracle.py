@@ -81,6 +81,9 @@ def load() -> C.CDLL:
             "orc_sim_pips": ([vp, vp], None),
             "orc_sim_interact": ([vp], None),
             "orc_sim_arrays": ([vp, vp, vp, C.POINTER(OrcCte)], None),
+            "orc_sim_floating": ([vp, u, vp], i),
+            "orc_interaction_ft": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
+                                    u, u] + [vp] * 12 + [i, vp, vp, vp], None),
             "orc_kernel_eval": ([i, f, f, f, f, C.POINTER(d), C.POINTER(d)], None),
             "orc_eos": ([i, i, C.POINTER(OrcCte), f], d),
         }
@@ -175,6 +178,19 @@ class OracleSim:
         cte = OrcCte()
         self.lib.orc_sim_arrays(self._h, _p(dcell), _p(poscell), C.byref(cte))
         return dcell, poscell, cte
+
+    def floating(self, cf: int = 0) -> dict:
+        """State of floating body cf: centre, angles, linear and angular
+        velocity and acceleration (StFloatingData)."""
+        from dualsphysics_amd import _native as N
+        st = N.DsphFloatingState()
+        if self.lib.orc_sim_floating(self._h, cf, C.cast(C.pointer(st), C.c_void_p)):
+            raise IndexError(f"no floating body {cf}")
+        return {"center": np.array(st.center[:]), "angles": np.array(st.angles[:], np.float32),
+                "fvel": np.array(st.fvel[:], np.float32),
+                "fomega": np.array(st.fomega[:], np.float32),
+                "facelin": np.array(st.facelin[:], np.float32),
+                "faceang": np.array(st.faceang[:], np.float32)}
 
     def pips(self):
         res = (C.c_uint64 * 4)()

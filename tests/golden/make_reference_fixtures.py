@@ -13,7 +13,9 @@ Fixture ref_dambreak_dp0.03.npz (data only, no code of the reference):
   vel       fluid velocity [k, nf, 3]; rho: density of all particles [k, np];
   order     the ids in the reference's own particle order (sorted by cell)
             after each saved step [k, np].
-Other arrays are in id order.
+Other arrays are in id order. A case with a floating body
+(ref_dambreak_dp0.03_floating) also holds disp_float and vel_float, the
+displacement (FP64) and velocity of the body's particles [k, nfloat, 3].
 
 ref_observables_dp0.01.npz (--observables, ~1 h on 8 cores): front position
 and probe pressure every 0.01 s over 1 s of the dp=0.01 dam break.
@@ -100,6 +102,7 @@ CASES = {
     # Channel periodic in x with the fluid moving at 2 m/s: particles leave
     # through one face and re-enter through the other (Symplectic + DBC, and
     # Verlet + mDBC + laminar viscosity).
+    "ref_dambreak_dp0.03_floating": ("dam_break_floating", 0.03, 40, (1, 10, 20, 40), {}),
     "ref_channel_dp0.02_periodic": ("periodic_channel", 0.02, 60, (1, 30, 60), dict(periodic=1)),
     "ref_channel_dp0.02_periodic_mdbc_verlet": ("periodic_channel", 0.02, 60, (1, 30, 60),
                                                 dict(periodic=1, tboundary=2, tstep=1, tvisco=3,
@@ -269,18 +272,24 @@ def main():
         assert np.array_equal(p0["pos"], case.pos), "reference did not load the same particles"
         assert np.array_equal(p0["rho"], case.velrho[:, 3])
         fl = slice(case.nbound, None)
+        mv = slice(case.npb - case.nmoving, case.npb)
+        ft = slice(case.npb, case.nbound)
+        extra = {}
+        if case.nfloat:
+            # Floating particles (ids [npb, nbound)): displacement and velocity.
+            extra = dict(
+                disp_float=np.stack([(parts[k]["pos"] - p0["pos"])[ft] for k in saved]),
+                vel_float=np.stack([parts[k]["vel"][ft] for k in saved]))
         np.savez_compressed(
-            OUT / f"{name}.npz",
+            OUT / f"{name}.npz", **extra,
             generator=np.array(gen), dp=dp, seed=42, params=np.array(repr(sorted(over.items()))),
             genargs=np.array(repr(sorted(genargs.items()))),
             nmoving=case.nmoving,
             nbound=case.nbound, nfluid=case.nfluid, steps=np.array(saved),
             time=np.array([parts[k]["time"] for k in saved]),
             disp=np.stack([(parts[k]["pos"] - p0["pos"])[fl].astype(np.float32) for k in saved]),
-            disp_moving=np.stack([(parts[k]["pos"] - p0["pos"])[case.nbound - case.nmoving:case.nbound]
-                                  for k in saved]),
-            vel_moving=np.stack([parts[k]["vel"][case.nbound - case.nmoving:case.nbound]
-                                 for k in saved]),
+            disp_moving=np.stack([(parts[k]["pos"] - p0["pos"])[mv] for k in saved]),
+            vel_moving=np.stack([parts[k]["vel"][mv] for k in saved]),
             vel=np.stack([parts[k]["vel"][fl] for k in saved]),
             order=np.stack([parts[k]["idp_saved"] for k in saved]),
             rho=np.stack([parts[k]["rho"] for k in saved]))

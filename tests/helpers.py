@@ -195,8 +195,10 @@ def compare_with_reference(fx, k, time, pos0, pos, vel, rho, order=None):
     if "'shifting'" in prm and "shifttfs" not in prm:
         tol = REF_TOL_SHIFT_NOTFS
     nmov = int(fx["nmoving"]) if "nmoving" in fx else 0
+    nfl = int(fx["disp_float"].shape[1]) if "disp_float" in fx else 0
+    npb = nb - nfl
     if nmov:
-        mv = slice(nb - nmov, nb)
+        mv = slice(npb - nmov, npb)
         dm = float(np.abs((pos[mv] - pos0[mv]) - fx["disp_moving"][k]).max())
         vm = float(np.abs(vel[mv] - fx["vel_moving"][k]).max())
         assert dm < 1e-9 and vm < 1e-6, (int(fx["steps"][k]), "moving boundary", dm, vm)
@@ -208,9 +210,24 @@ def compare_with_reference(fx, k, time, pos0, pos, vel, rho, order=None):
         "vel": float(np.abs(vel[nb:] - fx["vel"][k]).max() / np.abs(fx["vel"][k]).max()),
         "rho": float(np.abs(rho.astype(np.float64) - fx["rho"][k]).max()),
     }
+    if nfl:
+        # Floating body particles (ids [npb, nbound)): displacement and
+        # velocity, relative to the largest of the run's (absolute while the
+        # body has not moved yet).
+        ft = slice(npb, nb)
+        dref, vref = fx["disp_float"][k], fx["vel_float"][k]
+        e["disp_float"] = float(np.abs((pos[ft] - pos0[ft]) - dref).max() /
+                                max(float(np.abs(dref).max()), 1e-3 * float(case_dp(fx))))
+        e["vel_float"] = float(np.abs(vel[ft] - vref).max() / max(float(np.abs(vref).max()), 1e-6))
+        assert e["disp_float"] < tol["disp"] and e["vel_float"] < tol["vel"], \
+            (int(fx["steps"][k]), "floating body", e)
     for key, t in tol.items():
         assert e[key] < t, (int(fx["steps"][k]), key, e)
     return e
+
+
+def case_dp(fx) -> float:
+    return float(fx["dp"])
 
 
 def hydrostatic_mdbc2_case(dp=0.02, depth=0.2, u=0.3, jitter=0.0):

@@ -71,7 +71,8 @@ def test_bindings_load_with_signatures():
 STRUCTS = {"dsph_cte": N.DsphCte, "dsph_divdata": N.DsphDivData,
            "dsph_inter_parms": N.DsphInterParms, "dsph_case": N.DsphCase,
            "dsph_stats": N.DsphStats, "dsph_motion": N.DsphMotion,
-           "dsph_mlp": N.DsphMlp, "dsph_ens_state": N.DsphEnsState}
+           "dsph_mlp": N.DsphMlp, "dsph_ens_state": N.DsphEnsState,
+           "dsph_floating": N.DsphFloating, "dsph_floating_state": N.DsphFloatingState}
 
 
 def test_struct_layouts_match_the_c_compiler(tmp_path):

@@ -17,7 +17,9 @@ Molteni-Colagrossi DDT and with laminar+SPS without DDT; the piston flume
 with Verlet and mDBC; half cells; standard shifting (no boundaries with the
 free-surface threshold, and full); a channel periodic in x (Symplectic +
 DBC, Verlet + mDBC + laminar viscosity) over 60 steps in which particles
-leave through one face and re-enter through the other.
+leave through one face and re-enter through the other; the dam break with
+a floating box of half the water density at the column's surface over 40
+steps (rigid-body motion from the fluid forces).
 """
 import numpy as np
 import pytest
@@ -34,7 +36,7 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_dambreak_dp0.03_shift_full", "ref_flume_dp0.01_piston_mdbc_ddt1",
             "ref_dambreak_dp0.03_mdbc2_verlet_ddt1", "ref_flume_dp0.01_piston_mdbc_shift_nofixed",
             "ref_dambreak_dp0.03_viscbound2", "ref_channel_dp0.02_periodic",
-            "ref_channel_dp0.02_periodic_mdbc_verlet"]
+            "ref_channel_dp0.02_periodic_mdbc_verlet", "ref_dambreak_dp0.03_floating"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
