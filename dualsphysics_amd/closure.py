@@ -258,3 +258,63 @@ def normalisation(feats: np.ndarray, targets: np.ndarray):
     scale = np.where(std > 0, 1.0 / np.where(std > 0, std, 1.0), 0.0)
     out_scale = float(max(targets.mean(), 1e-12))
     return mean.astype(np.float32), scale.astype(np.float32), out_scale
+
+
+# Algorithmic FLOP per sample of the network (11 -> 64 -> 64 -> 1): the
+# forward products, and forward + backward for the gradient (dW1, dW2, dH1,
+# dW3/dH2); element-wise activations not counted.
+FLOP_FORWARD = 2 * (N_FEAT * 64 + 64 * 64 + 64)
+FLOP_GRAD = FLOP_FORWARD + 2 * (64 + 64 + 64 * 64 + 64 * 64 + N_FEAT * 64)
+PEAK_BF16_TFLOPS = 2500.0   # MI355X dense BF16 matrix peak
+
+
+def bench(n: int = 1 << 22, reps: int = 5, device: int = 0) -> dict:
+    """Times the network's kernels on n random samples: inference
+    (dsph_mlp_forward) and one training gradient (dsph_mlp_grad over n
+    samples in 1024 workgroups); returns samples/s and algorithmic TFLOP/s
+    against the BF16 matrix peak."""
+    import time
+    lib = N.load()
+    ctx = C.c_void_p()
+    if lib.dsph_ctx_create(device, C.byref(ctx)):
+        raise RuntimeError("dsph_ctx_create failed")
+    rng = np.random.default_rng(0)
+    feats = rng.standard_normal((n, N_FEAT)).astype(np.float32)
+    targets = np.abs(rng.standard_normal(n)).astype(np.float32) * 1e-6
+    net = Mlp(lib, ctx, init_params(0), np.zeros(N_FEAT), np.ones(N_FEAT), 1e-6)
+    sb = 1024 * 4 * (net.nparam + 1)
+    bufs = [_alloc(lib, ctx, x) for x in (feats.nbytes, targets.nbytes, 4 * n, sb,
+                                          4 * net.nparam, 4)]
+    df, dt_, do, dsc, dgr, dlo = bufs
+    out = {}
+    try:
+        net._h2d(df.value, feats)
+        net._h2d(dt_.value, targets)
+        for name, call, flop in (
+                ("forward", lambda: lib.dsph_mlp_forward(ctx, C.byref(net.m), n, df, do),
+                 FLOP_FORWARD),
+                ("grad", lambda: lib.dsph_mlp_grad(ctx, C.byref(net.m), n, df, dt_, dsc, sb, dgr,
+                                                   dlo), FLOP_GRAD)):
+            net._check(call())
+            net.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                net._check(call())
+            net.synchronize()
+            s = (time.perf_counter() - t0) / reps
+            tf = n * flop / s / 1e12
+            out[name] = {"ms": 1e3 * s, "samples_per_s": n / s, "tflops": tf,
+                         "frac_bf16_peak": tf / PEAK_BF16_TFLOPS, "flop_per_sample": flop}
+    finally:
+        for b in bufs:
+            lib.dsph_free(ctx, b)
+        net.free()
+        lib.dsph_ctx_destroy(ctx)
+    return out
+
+
+if __name__ == "__main__":
+    import json
+    import sys
+    nn = int(sys.argv[sys.argv.index("--bench") + 1]) if "--bench" in sys.argv else 1 << 22
+    print(json.dumps({"samples": nn, **bench(nn)}))

@@ -99,6 +99,7 @@ def evaluate(fine_dp=0.004, coarse_dp=0.02, tmax=1.0, tout=0.01,
     import os
     import dualsphysics_amd._native as Nn
     times = [round(k * tout, 9) for k in range(int(round(tmax / tout)) + 1)]
+    train_times = [t for t in train_times if t < tmax]
     fine = casemod.dam_break(dp=fine_dp, params=_params())
     if fine_file and os.path.exists(fine_file):
         rf = dict(np.load(fine_file, allow_pickle=False))
