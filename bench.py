@@ -115,8 +115,9 @@ def cpu_baseline_reference(case, steps: int, pairs_step: float):
 
 def ensemble_bench(n: int, tend: float, sample: int):
     """Batched ensemble of ``n`` uneven dam breaks (seeded draws), every case
-    run from t=0 to ``tend``: cases/s of the ensemble (one graph launch per 3
-    steps for all cases, autotuned fluid kernel per size bucket) against the
+    run from t=0 to ``tend``: cases/s of the ensemble (one graph launch per 24
+    steps for all cases, fluid-kernel variant and launch shape (plain grid or
+    persistent) per size bucket and boundary block size from the autotuner) against the
     plain single-case path (one SphGpuSingle per case, default parameters)
     timed on a size-stratified sample of ``sample`` cases. Set-up (case
     generation, upload, first divide, graph capture) is not timed for either."""
@@ -131,7 +132,13 @@ def ensemble_bench(n: int, tend: float, sample: int):
         f"generated in {time.perf_counter() - t0:.1f}s")
     ens = Ensemble(cases, tend=tend)
     chosen, tuned = ens.autotune(reps=2)
-    log(f"  autotuner: {chosen}")
+    shapes = {}
+    for b, name in enumerate(("fluid<16384", "fluid<131072", "fluid>=131072")):
+        sh = ens.shape(b)
+        if sh["variant"] >= 0:
+            shapes[name] = {k: (round(v, 4) if isinstance(v, float) else v)
+                            for k, v in sh.items()}
+    log(f"  autotuner: {shapes}")
     ens.run(max_steps=0)  # graph capture
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -172,7 +179,7 @@ def ensemble_bench(n: int, tend: float, sample: int):
             "speedup": plain_total / el, "particle_steps_per_s": part_steps / el,
             "cases_completed": stopped, "plain_sample": len(pick),
             "plain_seconds": plain_total, "plain_seconds_stderr": plain_se,
-            "autotune": chosen, "steps_per_graph": info["steps_per_graph"],
+            "autotune": shapes, "steps_per_graph": info["steps_per_graph"],
             "note": "plain = one SphGpuSingle run per case to the same end time, set-up "
                     "untimed (" + how + ")"}
 

@@ -114,6 +114,12 @@ class DsphEnsState(C.Structure):
         (n, C.c_double) for n in ("time", "dt_last", "dt_next")]
 
 
+class DsphEnsShape(C.Structure):
+    _fields_ = [("variant", C.c_int32)] + [
+        (n, C.c_uint32) for n in ("persistent", "chunk", "waves_per_cu", "bound_block")] + [
+        (n, C.c_float) for n in ("ms_plain", "ms_persistent", "ms_bound")]
+
+
 class DsphStats(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("np", "npb", "npbok", "nct", "nstep",
                                           "npf_out_total")] + [
@@ -233,6 +239,8 @@ SIGNATURES = {
     "dsph_ens_download": [_vp, _u, _vp, _vp, _vp, _vp, _vp],
     "dsph_ens_autotune": [_vp, _u, _vp, _vp],
     "dsph_ens_set_variant": [_vp, _u, C.c_int32],
+    "dsph_ens_get_shape": [_vp, _u, C.POINTER(DsphEnsShape)],
+    "dsph_ens_set_shape": [_vp, _u, C.POINTER(DsphEnsShape)],
     "dsph_ens_info": [_vp, _vp],
     "dsph_ens_set_timing": [_vp, _i],
     "dsph_ens_context": [_vp],

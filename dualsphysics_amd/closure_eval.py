@@ -134,9 +134,12 @@ def evaluate(fine_dp=0.004, coarse_dp=0.02, tmax=1.0, tout=0.01,
     def metrics(r):
         fe = np.abs(r["front"] - rf["front"]) / rf["front"]
         pm = np.abs(r["pressure"][win].mean(0) - rf["pressure"][win].mean(0)) / scale_p
+        # Impact: the peak of each probe's pressure over the run.
+        pk = np.abs(r["pressure"].max(0) - rf["pressure"].max(0)) / rf["pressure"].max(0)
         return {"front_max_rel": float(fe.max()), "front_mean_rel": float(fe.mean()),
                 "front_mean_rel_before_impact": float(fe[before].mean()),
                 "pressure_mean_diff_over_rho_g_H": [float(v) for v in pm],
+                "pressure_peak_rel": [float(v) for v in pk],
                 "seconds": r["seconds"], "steps": int(r["nstep"])}
 
     res = {"fine_dp": fine_dp, "coarse_dp": coarse_dp, "tmax": tmax,

@@ -62,7 +62,8 @@ struct EnsState {
 typedef uint2 EnsBlock;
 
 constexpr unsigned kEnsBs = 256;      // particle kernels: blocks of 256 over [0, np0)
-constexpr unsigned kEnsBsBound = 128; // boundary interaction: blocks of 128 over [0, npb)
+constexpr unsigned kEnsBsBound = 128; // boundary interaction: default blocks of 128 over [0, npb)
+constexpr unsigned kEnsBsBoundMax = 256;  // largest boundary block the autotuner tries
 
 // Fluid interaction kernel variants the autotuner chooses from.
 enum EnsFluidVariant {
@@ -73,23 +74,35 @@ enum EnsFluidVariant {
   kEnsNumVariants = 4
 };
 
+// Persistent launch of the fluid tile kernel: `grid` resident waves pull
+// `chunk` work items at a time from ctr[0..8) (one counter per XCD segment of
+// the work list; the launcher zeroes them with a fill kernel before the
+// kernel, so the pair can sit in a captured graph).
+struct EnsPersist {
+  unsigned grid, chunk;
+  unsigned* ctr;
+};
+
 // ---- launchers (defined in forces.hip, step.hip, celldiv.hip); all
 // asynchronous on ctx's stream, no host synchronisation.
 int ens_prep(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st, const EnsBlock* pmap,
              unsigned npblocks, const float4* poscell, const float4* velrho, float2* press,
              float4* posf);
 // wmap: one entry per wave of 64 fluid particles (.y = first fluid index of
-// the wave, relative to the case's first fluid particle); every variant runs
-// one 64-lane workgroup per entry.
+// the wave, relative to the case's first fluid particle); without `pers`
+// every variant runs one 64-lane workgroup per entry, with it the tile
+// variants run persistent (see EnsPersist).
 // eos7: every case has gamma = 7 (the kernels evaluate the neighbour
 // pressure from its density instead of gathering it).
 int ens_forces_fluid(dsph_ctx* ctx, int variant, int tvisco, int tddt, bool eos7,
                      const EnsCase* cases, const EnsState* st, const EnsBlock* wmap,
-                     unsigned nblocks, const unsigned* dcell, const float4* poscell,
+                     unsigned nblocks, const EnsPersist* pers, const unsigned* dcell,
+                     const float4* poscell,
                      const float4* posf, const float4* velrho, const float2* press,
                      float* viscdt, float* ar, float* ace);
+// bmap: one entry per block of bs boundary particles (bs = 64, 128, 192 or 256).
 int ens_forces_bound(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st,
-                     const EnsBlock* bmap, unsigned nblocks, const unsigned* dcell,
+                     const EnsBlock* bmap, unsigned nblocks, unsigned bs, const unsigned* dcell,
                      const float4* poscell, const float4* velrho, float* viscdt, float* ar);
 
 // Radius^2 of the fluid tile kernel's conservative scan test for a case.

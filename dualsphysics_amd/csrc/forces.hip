@@ -805,6 +805,11 @@ int dispatch_ddt(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, c
 // ---------------------------------------------------------------- ensembles
 // The same bodies over the cases of an ensemble; the block map gives the case
 // and the first case-local index of each workgroup.
+__global__ void k_fill_u32(unsigned* __restrict__ a, unsigned n, unsigned v) {
+  const unsigned i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
 __global__ void __launch_bounds__(kBs) k_ens_prep(const EnsCase* __restrict__ cases,
     const EnsState* __restrict__ st, const EnsBlock* __restrict__ pmap,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
@@ -840,6 +845,52 @@ k_ens_fluid_tile(const EnsCase* __restrict__ cases,
                                               nullptr, lds);
 }
 
+// Persistent form of k_ens_fluid_tile: a grid sized to the resident waves
+// pulls work items (waves of 64 fluid particles, entries of wmap) in chunks of
+// `chunk` consecutive items from counters, until the list is exhausted. The
+// list is split into kNumXcd contiguous segments, one per XCD (block i runs
+// on XCD i % 8), so neighbouring particles stay on one L2; a wave drains its
+// own XCD's segment first, then helps the others. Every wave leaves once each
+// segment's counter has passed its end (counters only grow), so the grid
+// drains. ctr[0..kNumXcd) must be zero at launch. Items are independent:
+// the schedule changes no result.
+template <int TVISCO, int TDDT, unsigned kRing, bool EOS7>
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(tile_min_waves<TVISCO, false, false, false>(), 8)))
+k_ens_fluid_tile_persist(const EnsCase* __restrict__ cases,
+    const EnsState* __restrict__ st, const EnsBlock* __restrict__ wmap, unsigned nitems,
+    unsigned chunk, unsigned* __restrict__ ctr,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ posf, const float4* __restrict__ velrho,
+    const float2* __restrict__ press, float* __restrict__ viscdt, float* __restrict__ ar,
+    float* __restrict__ ace) {
+  __shared__ TileLds<kRing> lds;
+  const unsigned per = nitems / kNumXcd, rem = nitems % kNumXcd;
+  const unsigned x0 = blockIdx.x % kNumXcd;
+  for (unsigned s = 0; s < kNumXcd; s++) {
+    const unsigned x = (x0 + s) % kNumXcd;
+    const unsigned seg0 = x * per + min(x, rem), segn = per + (x < rem ? 1u : 0u);
+    for (;;) {
+      unsigned i0 = 0;
+      if (threadIdx.x == 0) i0 = atomicAdd(ctr + x, chunk);
+      i0 = __builtin_amdgcn_readfirstlane(i0);
+      if (i0 >= segn) break;
+      const unsigned i1 = min(i0 + chunk, segn);
+      for (unsigned i = i0; i < i1; i++) {
+        const EnsBlock e = wmap[seg0 + i];
+        if (!st[e.x].active) continue;
+        const EnsCase& ec = cases[e.x];
+        const unsigned npf = st[e.x].np - ec.npb;
+        if (e.y >= npf) continue;
+        const unsigned p = e.y + threadIdx.x;
+        fluid_tile_body<TVISCO, TDDT, false, kRing, false, false, EOS7>(
+            p, p < npf, ec.p0 + ec.npb, ec.viscob, ec.viscof, ec.dvd, ec.c, ec.ks2scan, dcell,
+            poscell, posf, velrho, press, nullptr, viscdt, ar, ace, nullptr, nullptr, lds);
+      }
+    }
+  }
+}
+
 template <int TVISCO, int TDDT>
 __global__ void __launch_bounds__(64) k_ens_fluid_rows(const EnsCase* __restrict__ cases,
     const EnsState* __restrict__ st, const EnsBlock* __restrict__ wmap,
@@ -856,7 +907,9 @@ __global__ void __launch_bounds__(64) k_ens_fluid_rows(const EnsCase* __restrict
                                        nullptr, nullptr);
 }
 
-__global__ void __launch_bounds__(kBsForces) k_ens_bound(const EnsCase* __restrict__ cases,
+// Any block size up to kEnsBsBoundMax; bmap has one entry per block of
+// blockDim.x boundary particles.
+__global__ void __launch_bounds__(kEnsBsBoundMax) k_ens_bound(const EnsCase* __restrict__ cases,
     const EnsState* __restrict__ st, const EnsBlock* __restrict__ bmap,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
     const float4* __restrict__ velrho, float* __restrict__ viscdt, float* __restrict__ ar) {
@@ -870,9 +923,31 @@ __global__ void __launch_bounds__(kBsForces) k_ens_bound(const EnsCase* __restri
 
 template <int TVISCO, int TDDT, bool EOS7>
 void launch_ens_fluid(dsph_ctx* ctx, int variant, const EnsCase* cases, const EnsState* st,
-                      const EnsBlock* wmap, unsigned nb, const unsigned* dcell,
+                      const EnsBlock* wmap, unsigned nb, const EnsPersist* pers,
+                      const unsigned* dcell,
                       const float4* poscell, const float4* posf, const float4* velrho,
                       const float2* press, float* viscdt, float* ar, float* ace) {
+  if (pers) {
+    const unsigned g = pers->grid;
+    switch (variant) {
+      case kEnsTileRing8:
+        k_ens_fluid_tile_persist<TVISCO, TDDT, 8, EOS7><<<g, 64, 0, ctx->stream>>>(
+            cases, st, wmap, nb, pers->chunk, pers->ctr, dcell, poscell, posf, velrho, press,
+            viscdt, ar, ace);
+        break;
+      case kEnsTileRing12:
+        k_ens_fluid_tile_persist<TVISCO, TDDT, 12, EOS7><<<g, 64, 0, ctx->stream>>>(
+            cases, st, wmap, nb, pers->chunk, pers->ctr, dcell, poscell, posf, velrho, press,
+            viscdt, ar, ace);
+        break;
+      default:
+        k_ens_fluid_tile_persist<TVISCO, TDDT, 16, EOS7><<<g, 64, 0, ctx->stream>>>(
+            cases, st, wmap, nb, pers->chunk, pers->ctr, dcell, poscell, posf, velrho, press,
+            viscdt, ar, ace);
+        break;
+    }
+    return;
+  }
   switch (variant) {
     case kEnsTileRing8:
       k_ens_fluid_tile<TVISCO, TDDT, 8, EOS7><<<nb, 64, 0, ctx->stream>>>(
@@ -909,16 +984,25 @@ int ens_prep(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st, const EnsB
 
 int ens_forces_fluid(dsph_ctx* ctx, int variant, int tvisco, int tddt, bool eos7,
                      const EnsCase* cases, const EnsState* st, const EnsBlock* wmap,
-                     unsigned nblocks, const unsigned* dcell, const float4* poscell,
+                     unsigned nblocks, const EnsPersist* pers, const unsigned* dcell,
+                     const float4* poscell,
                      const float4* posf, const float4* velrho, const float2* press,
                      float* viscdt, float* ar, float* ace) {
   if (!nblocks) return DSPH_OK;
   DSPH_REQUIRE(ctx, variant >= 0 && variant < kEnsNumVariants, "ensemble: invalid kernel variant");
+  if (pers) {
+    DSPH_REQUIRE(ctx, variant != kEnsRows64, "ensemble: the persistent form runs the tile kernel");
+    DSPH_REQUIRE(ctx, pers->ctr && pers->grid && pers->chunk,
+                 "ensemble: persistent launch without counters, grid or chunk");
+    // The counters start at zero (fill kernel: safe inside a captured graph).
+    k_fill_u32<<<1, kNumXcd, 0, ctx->stream>>>(pers->ctr, kNumXcd, 0u);
+    DSPH_LAUNCHED(ctx, "k_fill_u32");
+  }
   KTimer kt(ctx, "k_forces_fluid");
 #define DSPH_ENS_FLUID(V, D)                                                              \
-  (eos7 ? launch_ens_fluid<V, D, true>(ctx, variant, cases, st, wmap, nblocks, dcell, poscell, \
-                                       posf, velrho, press, viscdt, ar, ace)                  \
-        : launch_ens_fluid<V, D, false>(ctx, variant, cases, st, wmap, nblocks, dcell,        \
+  (eos7 ? launch_ens_fluid<V, D, true>(ctx, variant, cases, st, wmap, nblocks, pers, dcell,   \
+                                       poscell, posf, velrho, press, viscdt, ar, ace)         \
+        : launch_ens_fluid<V, D, false>(ctx, variant, cases, st, wmap, nblocks, pers, dcell,  \
                                         poscell, posf, velrho, press, viscdt, ar, ace))
   if (tvisco == DSPH_VISCO_ARTIFICIAL && tddt == DSPH_DDT_DDT2FULL)
     DSPH_ENS_FLUID(DSPH_VISCO_ARTIFICIAL, DSPH_DDT_DDT2FULL);
@@ -938,12 +1022,14 @@ int ens_forces_fluid(dsph_ctx* ctx, int variant, int tvisco, int tddt, bool eos7
 }
 
 int ens_forces_bound(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st,
-                     const EnsBlock* bmap, unsigned nblocks, const unsigned* dcell,
+                     const EnsBlock* bmap, unsigned nblocks, unsigned bs, const unsigned* dcell,
                      const float4* poscell, const float4* velrho, float* viscdt, float* ar) {
   if (!nblocks) return DSPH_OK;
+  DSPH_REQUIRE(ctx, bs >= 64 && bs <= kEnsBsBoundMax && bs % 64 == 0,
+               "ensemble: boundary block size must be a multiple of 64 up to 256");
   KTimer kt(ctx, "k_forces_bound");
-  k_ens_bound<<<nblocks, kBsForces, 0, ctx->stream>>>(cases, st, bmap, dcell, poscell, velrho,
-                                                     viscdt, ar);
+  k_ens_bound<<<nblocks, bs, 0, ctx->stream>>>(cases, st, bmap, dcell, poscell, velrho,
+                                              viscdt, ar);
   DSPH_LAUNCHED(ctx, "k_ens_bound");
   return DSPH_OK;
 }

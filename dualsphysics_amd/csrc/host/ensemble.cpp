@@ -11,7 +11,17 @@
 // neighbours in the same order.
 //
 // Pointer swaps: one Symplectic step swaps the state arrays so that their
-// assignment repeats every 3 steps; a graph therefore holds 3 steps.
+// assignment repeats every 3 steps; a graph therefore holds a multiple of 3
+// steps: kLongSteps for the bulk of a run (one graph launch per 24 steps,
+// the host only reads the cases' active flags between launches) and 3 for
+// the remainder.
+//
+// Autotuner: per fluid-size bucket it times the fluid kernel's variants (tile
+// kernel with a ring of 8, 12 or 16 LDS tiles, one lane per particle rows
+// kernel) on a one-workgroup-per-wave grid, then the best tile variant in its
+// persistent form over chunks of 1, 2 or 4 waves of particles per counter
+// fetch and grids of 10, 20 or 40 waves per CU; and the boundary kernel's
+// block size (64, 128, 256). All shapes give identical results.
 #include <hip/hip_runtime.h>
 #include <memory>
 #include "ensemble.h"
@@ -24,6 +34,12 @@ using dsph::EnsState;
 namespace {
 
 constexpr unsigned kGraphSteps = 3;
+constexpr unsigned kLongSteps = 8 * kGraphSteps;
+constexpr unsigned kNumXcd = 8;
+constexpr unsigned kBoundBs[] = {64u, 128u, 256u};
+constexpr unsigned kNumBoundBs = 3;
+constexpr unsigned kChunks[] = {1u, 2u, 4u};
+constexpr unsigned kWavesPerCu[] = {10u, 20u, 40u};
 // Fluid-size buckets (fluid particles per case) tuned separately.
 constexpr unsigned kBucketEdges[] = {16384u, 131072u};
 constexpr unsigned kNumBuckets = 3;
@@ -39,13 +55,25 @@ struct dsph_ens {
   std::vector<EnsCase> hc;
   EnsCase* d_cases = nullptr;
   EnsState* d_st = nullptr;
-  EnsBlock *d_pmap = nullptr, *d_bmap = nullptr;
-  unsigned npblocks = 0, nbblocks = 0;
+  EnsBlock* d_pmap = nullptr;
+  unsigned npblocks = 0;
+  // Boundary block maps, one per candidate block size.
+  EnsBlock* d_bmaps[kNumBoundBs] = {nullptr, nullptr, nullptr};
+  unsigned nbblocks[kNumBoundBs] = {0, 0, 0};
+  unsigned ibs = 1;  // kBoundBs[ibs] = 128 until tuned
+  float bound_ms[kNumBoundBs] = {0, 0, 0};
+  unsigned ncu = 256;
+  unsigned* d_ctr = nullptr;  // kNumXcd counters per bucket (persistent fluid kernel)
   struct Bucket {
     EnsBlock* d_wmap = nullptr;
     unsigned nblocks = 0, ncases = 0;
     int variant = dsph::kEnsTileRing12;
     float tuned_ms[dsph::kEnsNumVariants] = {0, 0, 0, 0};
+    // Persistent form (tile variants only): grid of wpc waves per CU, chunk
+    // work items per fetch; persist_ms = its tuned time.
+    bool persist = false;
+    unsigned chunk = 1, wpc = 20;
+    float persist_ms = 0;
   };
   Bucket buckets[kNumBuckets];
   unsigned *idp = nullptr, *idp2 = nullptr, *dcell = nullptr, *dcell2 = nullptr;
@@ -60,8 +88,9 @@ struct dsph_ens {
   int* begincell = nullptr;
   bool pre_active = false;
   bool use_graph = true;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  // [0]: kGraphSteps steps, [1]: kLongSteps steps.
+  hipGraph_t graph[2] = {nullptr, nullptr};
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
   std::vector<void*> allocs;
   unsigned long long steps_run = 0;
 
@@ -81,10 +110,12 @@ struct dsph_ens {
     return static_cast<T*>(p);
   }
   void drop_graph() {
-    if (gexec) hipGraphExecDestroy(gexec);
-    if (graph) hipGraphDestroy(graph);
-    gexec = nullptr;
-    graph = nullptr;
+    for (int g = 0; g < 2; g++) {
+      if (gexec[g]) hipGraphExecDestroy(gexec[g]);
+      if (graph[g]) hipGraphDestroy(graph[g]);
+      gexec[g] = nullptr;
+      graph[g] = nullptr;
+    }
   }
   void release() {
     if (!ctx) return;
@@ -226,6 +257,11 @@ struct dsph_ens {
     }
     d_st = alloc<EnsState>(ncases);
     chk(dsph_memcpy_h2d(ctx, d_st, st.data(), sizeof(EnsState) * ncases), "upload state");
+    d_ctr = alloc<unsigned>(kNumBuckets * kNumXcd);
+    int cus = 0;
+    hip(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device),
+        "compute units");
+    ncu = unsigned(std::max(cus, 1));
     build_maps();
     // First divide (JSphGpuSingle::ConfigDomain -> RunCellDivide).
     divide();
@@ -233,11 +269,12 @@ struct dsph_ens {
   }
 
   void build_maps() {
-    std::vector<EnsBlock> pm, bm, wm[kNumBuckets];
+    std::vector<EnsBlock> pm, bm[kNumBoundBs], wm[kNumBuckets];
     for (unsigned k = 0; k < ncases; k++) {
       const EnsCase& e = hc[k];
       for (unsigned i = 0; i < e.np0; i += dsph::kEnsBs) pm.push_back(make_uint2(k, i));
-      for (unsigned i = 0; i < e.npb; i += dsph::kEnsBsBound) bm.push_back(make_uint2(k, i));
+      for (unsigned s = 0; s < kNumBoundBs; s++)
+        for (unsigned i = 0; i < e.npb; i += kBoundBs[s]) bm[s].push_back(make_uint2(k, i));
       const unsigned npf = e.np0 - e.npb;
       unsigned b = 0;
       while (b < kNumBuckets - 1 && npf >= kBucketEdges[b]) b++;
@@ -245,12 +282,14 @@ struct dsph_ens {
       buckets[b].ncases++;
     }
     npblocks = unsigned(pm.size());
-    nbblocks = unsigned(bm.size());
     d_pmap = alloc<EnsBlock>(pm.size());
     chk(dsph_memcpy_h2d(ctx, d_pmap, pm.data(), sizeof(EnsBlock) * pm.size()), "pmap");
-    if (!bm.empty()) {
-      d_bmap = alloc<EnsBlock>(bm.size());
-      chk(dsph_memcpy_h2d(ctx, d_bmap, bm.data(), sizeof(EnsBlock) * bm.size()), "bmap");
+    for (unsigned s = 0; s < kNumBoundBs; s++) {
+      nbblocks[s] = unsigned(bm[s].size());
+      if (bm[s].empty()) continue;
+      d_bmaps[s] = alloc<EnsBlock>(bm[s].size());
+      chk(dsph_memcpy_h2d(ctx, d_bmaps[s], bm[s].data(), sizeof(EnsBlock) * bm[s].size()),
+          "bmap");
     }
     for (unsigned b = 0; b < kNumBuckets; b++) {
       buckets[b].nblocks = unsigned(wm[b].size());
@@ -300,22 +339,37 @@ struct dsph_ens {
         "prep");
   }
 
-  void fluid(unsigned b, int variant) {
+  // Fluid interaction of bucket b: the variant on a one-workgroup-per-wave
+  // grid, or persistent with (chunk, wpc) when chunk != 0.
+  void fluid(unsigned b, int variant, unsigned chunk = 0, unsigned wpc = 0) {
     const Bucket& bk = buckets[b];
+    dsph::EnsPersist pers{0, chunk, d_ctr + b * kNumXcd};
+    if (chunk) {
+      const unsigned need = (bk.nblocks + chunk - 1) / chunk;
+      pers.grid = std::max(1u, std::min(ncu * wpc, need));
+    }
     chk(dsph::ens_forces_fluid(ctx, variant, tvisco, tddt, eos7, d_cases, d_st, bk.d_wmap,
-                               bk.nblocks,
+                               bk.nblocks, chunk ? &pers : nullptr,
                                dcell, (const float4*)poscell, (const float4*)posf,
                                (const float4*)velrho, (const float2*)press, viscdt, ar, ace),
         "fluid interaction");
+  }
+  void fluid_tuned(unsigned b) {
+    const Bucket& bk = buckets[b];
+    if (bk.persist) fluid(b, bk.variant, bk.chunk, bk.wpc);
+    else fluid(b, bk.variant);
+  }
+  void bound(unsigned s) {
+    chk(dsph::ens_forces_bound(ctx, d_cases, d_st, d_bmaps[s], nbblocks[s], kBoundBs[s], dcell,
+                               (const float4*)poscell, (const float4*)velrho, viscdt, ar),
+        "bound interaction");
   }
 
   void interaction() {
     prep(true);
     for (unsigned b = 0; b < kNumBuckets; b++)
-      if (buckets[b].nblocks) fluid(b, buckets[b].variant);
-    chk(dsph::ens_forces_bound(ctx, d_cases, d_st, d_bmap, nbblocks, dcell,
-                               (const float4*)poscell, (const float4*)velrho, viscdt, ar),
-        "bound interaction");
+      if (buckets[b].nblocks) fluid_tuned(b);
+    bound(ibs);
     chk(dsph::ens_reduce(ctx, d_cases, d_st, d_pmap, npblocks, 1, (const float4*)velrho, viscdt,
                          ace),
         "viscdtmax");
@@ -355,42 +409,55 @@ struct dsph_ens {
             posxypre2, poszpre, poszpre2, velrho, velrho2, velrhopre, velrhopre2};
   }
 
-  void capture() {
-    drop_graph();
+  // Captures graph g (0: kGraphSteps steps, 1: kLongSteps steps).
+  void capture(int g) {
+    const unsigned nsteps = g ? kLongSteps : kGraphSteps;
     const bool timing = ctx->timing;
     ctx->timing = false;  // no event records inside the graph
     const std::vector<void*> before = layout();
     hip(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal), "begin capture");
     try {
-      for (unsigned s = 0; s < kGraphSteps; s++) step();
+      for (unsigned s = 0; s < nsteps; s++) step();
     } catch (...) {
-      hipGraph_t g = nullptr;
-      hipStreamEndCapture(ctx->stream, &g);
-      if (g) hipGraphDestroy(g);
+      hipGraph_t gr = nullptr;
+      hipStreamEndCapture(ctx->stream, &gr);
+      if (gr) hipGraphDestroy(gr);
       ctx->timing = timing;
       throw;
     }
-    hip(hipStreamEndCapture(ctx->stream, &graph), "end capture");
+    hip(hipStreamEndCapture(ctx->stream, &graph[g]), "end capture");
     ctx->timing = timing;
     if (layout() != before) throw Err(DSPH_E_STATE, "graph steps do not restore the array layout");
-    hip(hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0), "graph instantiate");
+    hip(hipGraphInstantiate(&gexec[g], graph[g], nullptr, nullptr, 0), "graph instantiate");
   }
 
-  // Runs groups of kGraphSteps steps until every case has stopped or
-  // max_steps steps ran; the active flags are read every `check` groups.
+  // Runs steps in graph launches of kLongSteps (kGraphSteps for the rest)
+  // until every case has stopped or max_steps steps ran (rounded down to a
+  // multiple of kGraphSteps); the active flags are read once at least
+  // check * kGraphSteps steps ran since the last reading.
   unsigned run(unsigned max_steps, unsigned check) {
-    if (use_graph && !gexec) capture();
-    unsigned done = 0, groups = 0;
+    if (use_graph) {
+      // Both graphs are captured on entry (run(0) captures without running).
+      if (!gexec[0]) capture(0);
+      if (!gexec[1]) capture(1);
+    }
+    unsigned done = 0, since = 0;
     std::vector<EnsState> st(ncases);
     while (done + kGraphSteps <= max_steps) {
+      const int g = done + kLongSteps <= max_steps ? 1 : 0;
+      const unsigned n = g ? kLongSteps : kGraphSteps;
       if (use_graph) {
-        hip(hipGraphLaunch(gexec, ctx->stream), "graph launch");
+        hip(hipGraphLaunch(gexec[g], ctx->stream), "graph launch");
       } else {
-        for (unsigned s = 0; s < kGraphSteps; s++) step();
+        for (unsigned s = 0; s < n; s++) step();
       }
-      done += kGraphSteps;
-      steps_run += kGraphSteps;
-      if (++groups % std::max(check, 1u) == 0 && nactive(st) == 0) break;
+      done += n;
+      steps_run += n;
+      since += n;
+      if (since >= std::max(check, 1u) * kGraphSteps) {
+        since = 0;
+        if (nactive(st) == 0) break;
+      }
     }
     chk(dsph_synchronize(ctx), "synchronize");
     return done;
@@ -403,36 +470,80 @@ struct dsph_ens {
     return n;
   }
 
-  // Times every fluid-kernel variant on each size bucket over the current
-  // state and keeps the fastest (all variants give identical results).
+  // Times the shapes of the fluid kernel on each size bucket and of the
+  // boundary kernel over the current state and keeps the fastest (every
+  // shape gives identical results): the variants on a one-workgroup-per-wave
+  // grid, then the best tile variant persistent over chunks x grids.
   void autotune(unsigned reps) {
     drop_graph();
     hipEvent_t e0, e1;
     hip(hipEventCreate(&e0), "event");
     hip(hipEventCreate(&e1), "event");
-    prep(false);
-    for (unsigned b = 0; b < kNumBuckets; b++) {
-      Bucket& bk = buckets[b];
-      if (!bk.nblocks) continue;
-      float best = 1e30f;
-      for (int v = 0; v < dsph::kEnsNumVariants; v++) {
-        fluid(b, v);  // warm-up
-        float tmin = 1e30f;
-        for (unsigned r = 0; r < std::max(reps, 1u); r++) {
-          hip(hipEventRecord(e0, ctx->stream), "record");
-          fluid(b, v);
-          hip(hipEventRecord(e1, ctx->stream), "record");
-          hip(hipEventSynchronize(e1), "sync");
-          float ms = 0;
-          hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
-          tmin = std::min(tmin, ms);
+    auto timed = [&](auto&& launch) {
+      launch();  // warm-up
+      float tmin = 1e30f;
+      for (unsigned r = 0; r < std::max(reps, 1u); r++) {
+        hip(hipEventRecord(e0, ctx->stream), "record");
+        launch();
+        hip(hipEventRecord(e1, ctx->stream), "record");
+        hip(hipEventSynchronize(e1), "sync");
+        float ms = 0;
+        hip(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+        tmin = std::min(tmin, ms);
+      }
+      return tmin;
+    };
+    try {
+      prep(false);
+      for (unsigned b = 0; b < kNumBuckets; b++) {
+        Bucket& bk = buckets[b];
+        if (!bk.nblocks) continue;
+        float best = 1e30f, best_tile = 1e30f;
+        int tile = dsph::kEnsTileRing12;
+        for (int v = 0; v < dsph::kEnsNumVariants; v++) {
+          const float ms = timed([&] { fluid(b, v); });
+          bk.tuned_ms[v] = ms;
+          if (ms < best) {
+            best = ms;
+            bk.variant = v;
+          }
+          if (v != dsph::kEnsRows64 && ms < best_tile) {
+            best_tile = ms;
+            tile = v;
+          }
         }
-        bk.tuned_ms[v] = tmin;
-        if (tmin < best) {
-          best = tmin;
-          bk.variant = v;
+        bk.persist = false;
+        bk.persist_ms = 1e30f;
+        unsigned pc = 1, pw = 20;
+        for (unsigned ch : kChunks)
+          for (unsigned w : kWavesPerCu) {
+            const float ms = timed([&] { fluid(b, tile, ch, w); });
+            if (ms < bk.persist_ms) {
+              bk.persist_ms = ms;
+              pc = ch;
+              pw = w;
+            }
+          }
+        bk.chunk = pc;
+        bk.wpc = pw;
+        if (bk.persist_ms < best) {
+          bk.persist = true;
+          bk.variant = tile;
         }
       }
+      float bbest = 1e30f;
+      for (unsigned s = 0; s < kNumBoundBs; s++) {
+        if (!nbblocks[s]) continue;
+        bound_ms[s] = timed([&] { bound(s); });
+        if (bound_ms[s] < bbest) {
+          bbest = bound_ms[s];
+          ibs = s;
+        }
+      }
+    } catch (...) {
+      hipEventDestroy(e0);
+      hipEventDestroy(e1);
+      throw;
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
@@ -565,6 +676,56 @@ int dsph_ens_set_variant(dsph_ens* e, uint32_t bucket, int32_t variant) {
   if (!e || bucket >= kNumBuckets || variant < 0 || variant >= dsph::kEnsNumVariants)
     return DSPH_E_ARG;
   e->buckets[bucket].variant = variant;
+  e->buckets[bucket].persist = false;
+  e->drop_graph();
+  return DSPH_OK;
+}
+
+int dsph_ens_get_shape(dsph_ens* e, uint32_t bucket, dsph_ens_shape* out) {
+  if (!e || !out || bucket >= kNumBuckets) return DSPH_E_ARG;
+  const auto& bk = e->buckets[bucket];
+  out->variant = bk.nblocks ? bk.variant : -1;
+  out->persistent = bk.persist ? 1u : 0u;
+  out->chunk = bk.chunk;
+  out->waves_per_cu = bk.wpc;
+  out->bound_block = kBoundBs[e->ibs];
+  float plain = 0;
+  for (int v = 0; v < dsph::kEnsNumVariants; v++)
+    if (bk.tuned_ms[v] > 0 && (plain == 0 || bk.tuned_ms[v] < plain)) plain = bk.tuned_ms[v];
+  out->ms_plain = plain;
+  out->ms_persistent = bk.persist_ms < 1e30f ? bk.persist_ms : 0.f;
+  out->ms_bound = e->bound_ms[e->ibs];
+  return DSPH_OK;
+}
+
+int dsph_ens_set_shape(dsph_ens* e, uint32_t bucket, const dsph_ens_shape* shape) {
+  if (!e || !shape || bucket >= kNumBuckets) return DSPH_E_ARG;
+  if (shape->variant < 0 || shape->variant >= dsph::kEnsNumVariants) {
+    e->err = "dsph_ens_set_shape: invalid variant";
+    return DSPH_E_ARG;
+  }
+  if (shape->persistent &&
+      (shape->variant == dsph::kEnsRows64 || shape->chunk < 1 || shape->chunk > 64 ||
+       shape->waves_per_cu < 1 || shape->waves_per_cu > 64)) {
+    e->err = "dsph_ens_set_shape: persistent launches run a tile variant with chunk and "
+             "waves_per_cu in 1..64";
+    return DSPH_E_ARG;
+  }
+  unsigned s = kNumBoundBs;
+  for (unsigned k = 0; k < kNumBoundBs; k++)
+    if (kBoundBs[k] == shape->bound_block) s = k;
+  if (s == kNumBoundBs) {
+    e->err = "dsph_ens_set_shape: bound_block must be 64, 128 or 256";
+    return DSPH_E_ARG;
+  }
+  auto& bk = e->buckets[bucket];
+  bk.variant = shape->variant;
+  bk.persist = shape->persistent != 0;
+  if (bk.persist) {
+    bk.chunk = shape->chunk;
+    bk.wpc = shape->waves_per_cu;
+  }
+  e->ibs = s;
   e->drop_graph();
   return DSPH_OK;
 }
@@ -575,7 +736,7 @@ int dsph_ens_info(dsph_ens* e, uint32_t* h_info) {
   h_info[1] = e->ntot;
   h_info[2] = e->nkeys;
   for (unsigned b = 0; b < kNumBuckets; b++) h_info[3 + b] = e->buckets[b].ncases;
-  h_info[6] = kGraphSteps;
+  h_info[6] = kLongSteps;
   h_info[7] = unsigned(e->steps_run);
   return DSPH_OK;
 }

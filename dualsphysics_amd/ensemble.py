@@ -2,7 +2,10 @@
 
 Every case runs the Symplectic step with DBC boundaries, its own constants,
 cell grid (fixed to its map) and adaptive dt; all cases advance together,
-each stage of the step one launch for all of them, 3 steps per HIP graph.
+each stage of the step one launch for all of them, 24 steps per HIP graph
+launch; after :meth:`Ensemble.autotune` the fluid interaction of a size
+bucket may be one persistent launch whose resident waves pull work from
+counters.
 A case stops by itself at its end time. Results of a case equal a single
 :class:`dualsphysics_amd.sim.SphGpuSingle` run of that case with
 ``celldomfixed=1`` and device-side dt, bit for bit.
@@ -110,7 +113,23 @@ class Ensemble:
         return chosen, ms.reshape(3, 4)
 
     def set_variant(self, bucket: int, variant: int):
+        """Plain grid (one workgroup per wave of particles) with ``variant``."""
         self._check(self.lib.dsph_ens_set_variant(self._h, bucket, variant))
+
+    def shape(self, bucket: int) -> dict:
+        """Launch shape of a bucket's fluid kernel and of the boundary kernel
+        (with the autotuner's times)."""
+        sh = N.DsphEnsShape()
+        self._check(self.lib.dsph_ens_get_shape(self._h, bucket, C.byref(sh)))
+        d = {f: getattr(sh, f) for f, _ in N.DsphEnsShape._fields_}
+        d["variant_name"] = VARIANTS[sh.variant] if sh.variant >= 0 else None
+        return d
+
+    def set_shape(self, bucket: int, variant: int, persistent: bool = False, chunk: int = 1,
+                  waves_per_cu: int = 20, bound_block: int = 128):
+        sh = N.DsphEnsShape(variant=variant, persistent=int(persistent), chunk=chunk,
+                            waves_per_cu=waves_per_cu, bound_block=bound_block)
+        self._check(self.lib.dsph_ens_set_shape(self._h, bucket, C.byref(sh)))
 
     def info(self) -> dict:
         a = np.empty(8, np.uint32)

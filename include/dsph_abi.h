@@ -796,7 +796,9 @@ dsph_ctx* dsph_sim_ctx(dsph_sim* sim);
 /*
  * Batched ensembles (no counterpart in the reference, which runs one case
  * per process): many independent cases advanced together on one GPU, each
- * stage of the step one launch for all cases, 3 steps per HIP graph. Every
+ * stage of the step one launch for all cases, 24 steps per HIP graph launch
+ * (3 for a remainder), the fluid interaction optionally one persistent
+ * launch per bucket of case sizes. Every
  * case runs JSphGpuSingle's Symplectic step with DBC (artificial or laminar
  * viscosity, DDT2Full or no density diffusion, the same options for all
  * cases), a cell domain fixed to its map and its own adaptive dt, and stops
@@ -824,23 +826,47 @@ int dsph_ens_create(int device, uint32_t ncases, const dsph_case* cases,
                     const float* h_velrho, const double* tend, dsph_ens** out);
 int dsph_ens_destroy(dsph_ens* e);
 const char* dsph_ens_last_error(dsph_ens* e);
-/* Runs groups of 3 steps (one graph launch each, or eager launches with the
- * graph disabled) until every case has stopped or max_steps steps ran;
- * the stop flags are read back every check_every groups. h_steps = steps run. */
+/* Runs steps in graph launches of 24 steps (of 3 for the last < 24 steps
+ * of max_steps; eager launches with the graph disabled) until every case has
+ * stopped or max_steps steps ran (rounded down to a multiple of 3); the stop
+ * flags are read back once at least 3 * check_every steps ran since the last
+ * reading. h_steps = steps run. */
 int dsph_ens_run(dsph_ens* e, uint32_t max_steps, uint32_t check_every, uint32_t* h_steps);
 int dsph_ens_set_graph(dsph_ens* e, int enable);
 int dsph_ens_case_state(dsph_ens* e, uint32_t icase, dsph_ens_state* out);
 /* Current particles of case icase in sorted order (as dsph_sim_download). */
 int dsph_ens_download(dsph_ens* e, uint32_t icase, uint32_t* h_count, uint32_t* h_idp,
                       double* h_pos, float* h_velrho, uint16_t* h_code);
-/* Autotuner: times the fluid-kernel variants (LDS tile kernel with 8, 12 or
- * 16 mask tiles per wave, one-lane-per-particle kernel) on each case-size
- * bucket (fluid particles < 16384, < 131072, larger) over the current state,
- * reps runs each, and keeps the fastest per bucket. h_variant[3] (-1 for an
- * empty bucket) and h_ms[3*4] (best time of each variant) may be NULL. */
+/* Autotuner: on each case-size bucket (fluid particles < 16384, < 131072,
+ * larger), over the current state, reps runs each, it times the fluid-kernel
+ * variants (LDS tile kernel with 8, 12 or 16 mask tiles per wave,
+ * one-lane-per-particle kernel) on a grid of one workgroup per wave of 64
+ * particles, then the best tile variant as a persistent launch (chunks of 1,
+ * 2 or 4 waves of particles per work-counter fetch, grids of 10, 20 or 40
+ * waves per CU), and the boundary kernel's block size (64, 128, 256); it
+ * keeps the fastest shapes. h_variant[3] (-1 for an empty bucket) and
+ * h_ms[3*4] (best time of each variant on the plain grid) may be NULL; the
+ * chosen shapes are read with dsph_ens_get_shape. */
 int dsph_ens_autotune(dsph_ens* e, uint32_t reps, int32_t* h_variant, float* h_ms);
+/* Plain grid with the given variant for a bucket (persistent form off). */
 int dsph_ens_set_variant(dsph_ens* e, uint32_t bucket, int32_t variant);
-/* {cases, particles, sort keys, cases per bucket [3], steps per graph, steps run} */
+
+/* Launch shape of a bucket's fluid interaction and of the boundary kernel. */
+typedef struct dsph_ens_shape {
+  int32_t variant;          /* 0..2 tile ring 8/12/16, 3 one lane per particle */
+  uint32_t persistent;      /* 1: persistent launch (tile variants only) */
+  uint32_t chunk;           /* persistent: waves of 64 particles per counter fetch (1..64) */
+  uint32_t waves_per_cu;    /* persistent: grid = waves_per_cu x compute units (1..64) */
+  uint32_t bound_block;     /* boundary kernel block size: 64, 128 or 256 (all buckets) */
+  float ms_plain;           /* autotuner: best plain-grid time of the bucket [ms] (0 untuned) */
+  float ms_persistent;      /* autotuner: best persistent time [ms] (0 untuned) */
+  float ms_bound;           /* autotuner: boundary kernel time with bound_block [ms] */
+} dsph_ens_shape;
+int dsph_ens_get_shape(dsph_ens* e, uint32_t bucket, dsph_ens_shape* out);
+/* Sets a bucket's shape (ms fields ignored); bound_block applies to the
+ * boundary kernel of all buckets. */
+int dsph_ens_set_shape(dsph_ens* e, uint32_t bucket, const dsph_ens_shape* shape);
+/* {cases, particles, sort keys, cases per bucket [3], steps per graph launch, steps run} */
 int dsph_ens_info(dsph_ens* e, uint32_t* h_info);
 int dsph_ens_set_timing(dsph_ens* e, int enable);
 dsph_ctx* dsph_ens_context(dsph_ens* e);

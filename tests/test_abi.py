@@ -72,6 +72,7 @@ STRUCTS = {"dsph_cte": N.DsphCte, "dsph_divdata": N.DsphDivData,
            "dsph_inter_parms": N.DsphInterParms, "dsph_case": N.DsphCase,
            "dsph_stats": N.DsphStats, "dsph_motion": N.DsphMotion,
            "dsph_mlp": N.DsphMlp, "dsph_ens_state": N.DsphEnsState,
+           "dsph_ens_shape": N.DsphEnsShape,
            "dsph_floating": N.DsphFloating, "dsph_floating_state": N.DsphFloatingState}
 
 

@@ -87,6 +87,13 @@ def test_autotuner_changes_speed_not_results():
         chosen, ms = a.autotune(reps=2)
         assert chosen and np.all(ms[ms > 0] < 1e4)
         for bucket in range(3):
+            sh = a.shape(bucket)
+            print(f"bucket {bucket}: {sh}", flush=True)
+            if sh["variant"] >= 0:
+                assert sh["ms_plain"] > 0 and sh["ms_persistent"] > 0 and sh["ms_bound"] > 0
+                best = min(sh["ms_plain"], sh["ms_persistent"])
+                assert (sh["ms_persistent"] if sh["persistent"] else sh["ms_plain"]) == best
+        for bucket in range(3):
             b.set_variant(bucket, 3)   # the one-lane-per-particle kernel everywhere
         a.run(max_steps=6)
         b.run(max_steps=6)
@@ -95,6 +102,49 @@ def test_autotuner_changes_speed_not_results():
     finally:
         a.close()
         b.close()
+
+
+@pytest.mark.parametrize("variant,chunk,wpc,bound_block", [
+    (0, 1, 10, 64),     # about one work item per wave
+    (1, 4, 40, 256),
+    (2, 64, 1, 128),    # a handful of waves draining every XCD segment
+])
+def test_persistent_launch_changes_speed_not_results(variant, chunk, wpc, bound_block):
+    """The fluid interaction as one persistent launch per bucket (resident
+    waves pulling chunks of work items from per-XCD counters, which the step
+    clears with a fill kernel before each launch) inside the 24-step graph,
+    against the plain one-workgroup-per-wave launches run eagerly."""
+    cases = _cases()
+    a = Ensemble(cases, tend=10.0)
+    b = Ensemble(cases, tend=10.0)
+    try:
+        for bucket in range(3):
+            a.set_shape(bucket, variant, persistent=True, chunk=chunk, waves_per_cu=wpc,
+                        bound_block=bound_block)
+            sh = a.shape(bucket)
+            assert sh["persistent"] == 1 and sh["chunk"] == chunk and sh["bound_block"] == bound_block
+        assert a.run(max_steps=27, graph=True) == 27   # one 24-step launch + one 3-step launch
+        assert b.run(max_steps=27, graph=False) == 27
+        assert a.info()["steps_per_graph"] == 24
+        for k in range(len(cases)):
+            _assert_same(a.download(k), b.download(k))
+            assert a.state(k)["time"] == b.state(k)["time"]
+    finally:
+        a.close()
+        b.close()
+
+
+def test_shape_arguments_are_checked():
+    ens = Ensemble(_cases()[:1], tend=1.0)
+    try:
+        with pytest.raises(EnsembleError, match="tile variant"):
+            ens.set_shape(0, 3, persistent=True)
+        with pytest.raises(EnsembleError, match="bound_block"):
+            ens.set_shape(0, 1, bound_block=96)
+        with pytest.raises(EnsembleError, match="tile variant"):
+            ens.set_shape(0, 1, persistent=True, chunk=0)
+    finally:
+        ens.close()
 
 
 def test_cases_stop_at_their_end_times():
