@@ -351,6 +351,24 @@ constexpr unsigned kScanBlock = 8;  // staged candidates tested per loop trip
 #endif
 // Tiles of hit masks kept per wave (8 bytes per lane and tile).
 constexpr unsigned kRingTiles = DSPH_RING_TILES;
+#ifndef DSPH_TILE_WAVES
+#define DSPH_TILE_WAVES 1
+#endif
+// Waves per workgroup of the single-case tile kernel: the waves of one
+// workgroup run on one CU and take consecutive groups of 64 particles, so
+// they share the L1 copy of their common neighbour rows.
+constexpr unsigned kTileWaves = DSPH_TILE_WAVES;
+
+// Orders the LDS accesses of one wave whose lanes exchange data through LDS.
+// The LDS executes the instructions of a wave in issue order, so fences at
+// wavefront scope (which keep the compiler from moving accesses across them)
+// are all that is needed; there is no workgroup barrier, as the waves of a
+// workgroup run loops of different lengths.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 __device__ inline unsigned wave_min_u32(unsigned v) {
   for (int off = 32; off > 0; off >>= 1) v = min(v, (unsigned)__shfl_xor((int)v, off));
@@ -453,7 +471,7 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
   float4* tile = lds.tile;
   auto& masks = lds.masks;
   unsigned* tbase = lds.tbase;
-  const unsigned lane = threadIdx.x;
+  const unsigned lane = threadIdx.x & 63u;
   const unsigned p1 = (valid ? p : 0) + pinit;
   Acc a;
   acc_init(a);
@@ -579,14 +597,14 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
           for (unsigned t = gini; t < gfin; t += kTile) {
             // The ring slot of tile wt must have been consumed by every lane.
             while (wt >= kRing && wave_min_u32(rt) <= wt - kRing) process(kMinBatch, Some());
-            __syncthreads();
+            wave_sync();
             if (t + lane < gfin) {
               const float4 f = posf[t + lane];
               const float x = f.x - ox, y2 = f.y - oy, z2 = f.z - oz;
               tile[lane] = make_float4(x, y2, z2, x * x + y2 * y2 + z2 * z2);
             }
             if (lane == 0) tbase[wt % kRing] = t | kbit;
-            __syncthreads();
+            wave_sync();
             // Hits of this lane among the staged candidates (blocks of 8 up
             // to the last staged one; the window drops the rest).
             unsigned long long hm = 0;
@@ -639,7 +657,7 @@ constexpr int tile_min_waves() {
 
 template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false,
           bool SHIFT = false, bool EOS7 = false>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64 * kTileWaves)
 __attribute__((amdgpu_waves_per_eu(tile_min_waves<TVISCO, VELGRAD, MDBC2, SHIFT>(), 8)))
 k_forces_fluid_tile(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c, float ks2scan,
@@ -650,11 +668,13 @@ k_forces_fluid_tile(
     float* __restrict__ velgrad, const uint8_t* __restrict__ bmode = nullptr,
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
     const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr) {
-  __shared__ TileLds<kRing> lds;
-  const unsigned p = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
+  __shared__ TileLds<kRing> lds[kTileWaves];
+  const unsigned w = threadIdx.x >> 6;
+  const unsigned p =
+      (xcd_block(blockIdx.x, gridDim.x) * kTileWaves + w) * 64 + (threadIdx.x & 63u);
   fluid_tile_body<TVISCO, TDDT, VELGRAD, kRing, MDBC2, SHIFT, EOS7>(
       p, p < n, pinit, viscob, viscof, dvd, c, ks2scan, dcell, poscell, posf, velrho, press, tau,
-      viscdt, ar, ace, strain, velgrad, lds, bmode, tangenvel, shiftmode, code, shiftposfs);
+      viscdt, ar, ace, strain, velgrad, lds[w], bmode, tangenvel, shiftmode, code, shiftposfs);
 }
 
 // Boundary particle p1 against fluid neighbours (KerInteractionForcesBound,
@@ -734,8 +754,8 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
   } else if (t->d_shiftposfs && ctx->forces_kernel == 0) {
     // Shifting in the tile kernel (mDBC2 and the velocity gradient are
     // refused with shifting by dsph_interaction_forces).
-    k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, false, true><<<grid_for(npf, 64), 64, 0,
-                                                                       ctx->stream>>>(
+    k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, false, true>
+        <<<grid_for(npf, 64 * kTileWaves), 64 * kTileWaves, 0, ctx->stream>>>(
         npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
         (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press, t->d_spstaurho2,
         t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, nullptr, nullptr, nullptr, t->shiftmode,
@@ -749,8 +769,8 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
         (float4*)t->d_shiftposfs);
   } else if (t->d_boundmode && ctx->forces_kernel == 0 && !VELGRAD) {
     // mDBC2 boundaries in the tile kernel.
-    k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, true><<<grid_for(npf, 64), 64, 0,
-                                                                 ctx->stream>>>(
+    k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, true>
+        <<<grid_for(npf, 64 * kTileWaves), 64 * kTileWaves, 0, ctx->stream>>>(
         npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
         (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press, t->d_spstaurho2,
         t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, nullptr, t->d_boundmode, t->d_tangenvel);
@@ -768,13 +788,13 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
     constexpr bool kEos7Built = !VELGRAD && TVISCO != DSPH_VISCO_LAMINARSPS;
     if (kEos7Built && c.gamma == 7.f)
       k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, false, false, kEos7Built>
-          <<<grid_for(npf, 64), 64, 0, ctx->stream>>>(
+          <<<grid_for(npf, 64 * kTileWaves), 64 * kTileWaves, 0, ctx->stream>>>(
               npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
               (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press,
               t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, nullptr);
     else
       k_forces_fluid_tile<TVISCO, TDDT, VELGRAD, kRingTiles>
-          <<<grid_for(npf, 64), 64, 0, ctx->stream>>>(
+          <<<grid_for(npf, 64 * kTileWaves), 64 * kTileWaves, 0, ctx->stream>>>(
               npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
               (const float4*)t->d_poscell, posf, (const float4*)t->d_velrho, press,
               t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace, t->d_sps2strain, t->d_velgrad);
