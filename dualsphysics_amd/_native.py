@@ -17,7 +17,7 @@ KERNEL_WENDLAND = 2
 VISCO_ARTIFICIAL, VISCO_LAMINARSPS, VISCO_LAMINAR = 1, 2, 3
 DDT_NONE, DDT_DDT, DDT_DDT2, DDT_DDT2FULL = 0, 1, 2, 3
 BC_DBC, BC_MDBC = 1, 2
-SLIP_VEL0, SLIP_NOSLIP = 1, 2
+SLIP_VEL0, SLIP_NOSLIP, SLIP_FREESLIP = 1, 2, 3
 SHIFT_NONE, SHIFT_NOBOUND, SHIFT_NOFIXED, SHIFT_FULL = 0, 1, 2, 3
 BMODE_DBC, BMODE_MDBC2, BMODE_MDBC2OFF = 0, 1, 2
 OPT_FORCES_KERNEL = 1
@@ -60,7 +60,8 @@ class DsphInterParms(C.Structure):
                 ("d_ar", C.c_void_p), ("d_ace", C.c_void_p), ("d_sps2strain", C.c_void_p),
                 ("d_velgrad", C.c_void_p), ("d_boundmode", C.c_void_p),
                 ("d_tangenvel", C.c_void_p), ("shiftmode", C.c_int32),
-                ("d_shiftposfs", C.c_void_p), ("d_ftomassp", C.c_void_p)]
+                ("d_shiftposfs", C.c_void_p), ("d_ftomassp", C.c_void_p),
+                ("d_boundnor", C.c_void_p), ("d_nopenshift", C.c_void_p)]
 
 
 class DsphMotion(C.Structure):
@@ -98,7 +99,7 @@ class DsphCase(C.Structure):
                                                                   ("slipmode", C.c_int32),
         ("shiftmode", C.c_int32), ("shiftcoef", C.c_double), ("shifttfs", C.c_double),
         ("periactive", C.c_int32), ("perinc", (C.c_double * 3) * 3),
-        ("ftcount", C.c_uint32), ("pad_ft", C.c_uint32),
+        ("ftcount", C.c_uint32), ("nopenetration", C.c_int32),
         ("floatings", C.POINTER(DsphFloating))]
 
 
@@ -164,6 +165,9 @@ SIGNATURES = {
     "dsph_mdbc_correction": [_vp, _u, C.POINTER(DsphDivData), C.POINTER(_d)] + [_vp] * 6,
     "dsph_mdbc2_correction": [_vp, _u, C.POINTER(_f), C.POINTER(DsphDivData),
                               C.POINTER(_d)] + [_vp] * 10,
+    "dsph_mdbc2_correction_slip": [_vp, _i, _u, C.POINTER(_f), C.POINTER(DsphDivData),
+                                   C.POINTER(_d)] + [_vp] * 10,
+    "dsph_mdbc2_nopen_step": [_vp, _u, _u, _vp, _vp, _d, _vp, _u, _vp, _vp, _vp],
     "dsph_velmax2": [_vp, _u, _u, _vp, _vp],
     "dsph_acemax2": [_vp, _u, _u, _vp, _vp],
     "dsph_acemax2_normal": [_vp, _u, _u, _vp, _vp, _vp, _vp],

@@ -259,6 +259,8 @@ def write_case(case, directory, name: str, timemax: float | None = None,
     }
     if p.tboundary == 2:
         params["SlipMode"] = getattr(p, "slipmode", 1)
+        if getattr(p, "slipmode", 1) >= 2 and getattr(p, "nopenetration", 0):
+            params["NoPenetration"] = 1
     # Periodic axes (JSph.cpp:810-826): a key of the axis' increments turns it on.
     peri = getattr(p, "periodic", 0)
     for a, keys in enumerate(_PERI_KEYS):
@@ -610,6 +612,8 @@ def read_case(directory, name: str):
                       viscoboundfactor=gf("ViscoBoundFactor", 1.0),
                       tdensity=gi("DensityDT", 0), ddtvalue=gf("DensityDTvalue", 0.1),
                       tboundary=gi("Boundary", 1), slipmode=gi("SlipMode", 1),
+                      nopenetration=int(str(prm.get("NoPenetration", "0")).strip().upper()
+                                        in ("1", "TRUE")),
                       cflnumber=float(ct.find("cflnumber").get("value"))
                       if ct.find("cflnumber") is not None else gf("CFLnumber", 0.2),
                       dtini=gf("DtIni", 0.0), dtmin=gf("DtMin", 0.0),

@@ -60,7 +60,8 @@ class Parameters:
     tdensity: int = N.DDT_DDT2FULL
     ddtvalue: float = 0.1
     tboundary: int = N.BC_DBC
-    slipmode: int = N.SLIP_VEL0     # mDBC SlipMode 1 vel=0, 2 no-slip (mDBC2)
+    slipmode: int = N.SLIP_VEL0     # mDBC SlipMode 1 vel=0, 2 no-slip, 3 free slip (mDBC2)
+    nopenetration: int = 0          # mDBC2 NoPenetration (SlipMode 2 or 3 only)
     shifting: int = N.SHIFT_NONE    # Shifting 0 none, 1 no boundaries, 2 no fixed, 3 full
     shiftcoef: float = -2.0
     shifttfs: float = 0.0
@@ -154,6 +155,7 @@ class Case:
         c.rhopoutmin, c.rhopoutmax = p.rhopoutmin, p.rhopoutmax
         c.cellmode, c.celldomfixed = p.cellmode, p.celldomfixed
         c.slipmode = p.slipmode
+        c.nopenetration = int(getattr(p, "nopenetration", 0))
         c.shiftmode, c.shiftcoef, c.shifttfs = p.shifting, p.shiftcoef, p.shifttfs
         c.periactive = int(p.periodic)
         for a in range(3):
@@ -250,9 +252,11 @@ def map_limits(pos: np.ndarray, h: float, zextra: float = 0.5, periodic: int = 0
 def dam_break(dp: float = 0.01, tank=(1.6, 0.67, 0.4), column=(0.4, 0.67, 0.3), layers: int = 2,
               coefh: float = 1.2, coefsound: float = 20.0, rhop0: float = 1000.0,
               gamma: float = 7.0, g: float = 9.81, seed: int = 42, jitter: float = 1e-4,
-              params: Parameters | None = None, name: str | None = None) -> Case:
+              params: Parameters | None = None, name: str | None = None,
+              vel0=(0.0, 0.0, 0.0)) -> Case:
     """3-D dam break in a box tank (the reference's CaseDambreak geometry without
-    the obstacle): column ``column`` at the x=0 end of tank ``tank``.
+    the obstacle): column ``column`` at the x=0 end of tank ``tank``; the fluid
+    starts with velocity ``vel0`` (at rest by default).
     ``boundnor`` (mDBC normals) is the vector from each wall particle to the
     nearest point of the tank interior [0,L]x[0,W]x[0,inf)."""
     params = params or Parameters()
@@ -274,6 +278,7 @@ def dam_break(dp: float = 0.01, tank=(1.6, 0.67, 0.4), column=(0.4, 0.67, 0.3), 
     velrho[:nb, 3] = rhop0
     zf = pos[nb:, 2]
     velrho[nb:, 3] = (rhop0 * (1.0 + rhop0 * g * (hswl - zf) / b) ** (1.0 / gamma)).astype(np.float32)
+    velrho[nb:, :3] = np.asarray(vel0, np.float32)
     bp = pos[:nb]
     q = np.stack([np.clip(bp[:, 0], 0.0, nxt * dp), np.clip(bp[:, 1], 0.0, nyt * dp),
                   np.maximum(bp[:, 2], 0.0)], 1)

@@ -75,11 +75,14 @@ struct Acc {
   float s[6];   // two_strain {xx,xy,xz,yy,yz,zz}
   float g[9];   // full velocity gradient (learned closure features)
   float4 sh;    // shifting: sum m/rho gradW (x = FLT_MAX: cancelled), w = -sum m/rho r.gradW
+  float npc[3], nps[3];  // no-penetration: boundary neighbours counted and corrections, per axis
 };
 
 __device__ inline void acc_init(Acc& a) {
   a.ax = a.ay = a.az = a.ar = a.visc = a.delta = 0;
   a.sh = make_float4(0, 0, 0, 0);
+#pragma unroll
+  for (int k = 0; k < 3; k++) a.npc[k] = a.nps[k] = 0;
 #pragma unroll
   for (int k = 0; k < 6; k++) a.s[k] = 0;
 #pragma unroll
@@ -111,13 +114,16 @@ struct P1 {
 // takes no part in the Fourtakas term and gives no SPS tau; a floating p1
 // has no density diffusion and no strain (JSphGpu_ker.cu:570-583, 668,
 // 755-766, 838-843).
+// NOPEN: mDBC2 no-penetration sums against fixed boundary neighbours
+// (MDBC2_NoPen, JSphGpu_ker.cu:686-707 with ComputeNoPenVel, :428-440).
 template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false,
-          bool FT = false>
+          bool FT = false, bool NOPEN = false>
 __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float4& ps2,
     const float4& vr2, float2 pr2, const P1& q, const float* __restrict__ tau, float viscob,
     float viscof, const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
-    const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr) {
+    const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr,
+    const float* __restrict__ boundnor = nullptr) {
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
@@ -172,6 +178,30 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
     a.sh.z += massrho * frz;
     a.sh.w -= massrho * dot3;
   }
+  if (NOPEN && boundp2) {
+    // A boundary neighbour of the first layer (|normal| < 1.75 dp) within
+    // 1.25 dp, with the particle less than 0.75 |normal| away along the
+    // normal: per axis, a particle moving towards the wall (fixed walls:
+    // motion velocity 0) gets the correction -(3 - 4 max(|dr/n|, 1/4)) v n^2.
+    if (sqrtf(rr2) < 1.25f * c.dp) {
+      const float* bn = boundnor + 3 * size_t(p2);
+      const float norm = sqrtf(bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]);
+      const float nrm[3] = {fast_div(bn[0], norm), fast_div(bn[1], norm), fast_div(bn[2], norm)};
+      const float d[3] = {dr.x, dr.y, dr.z};
+      const float v[3] = {q.vr.x, q.vr.y, q.vr.z};
+      if (nrm[0] * dr.x + nrm[1] * dr.y + nrm[2] * dr.z < 0.75f * norm && norm < 1.75f * c.dp) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          if (d[k] * nrm[k] < 0.75f && fabsf(nrm[k]) > 0.001f * c.dp && v[k] * nrm[k] < 0.f) {
+            const float ratio = fmaxf(fabsf(fast_div(d[k], nrm[k])), 0.25f);
+            const float factor = -4.f * ratio + 3.f;
+            a.npc[k] += 1.f;
+            a.nps[k] -= factor * v[k] * nrm[k] * nrm[k];
+          }
+        }
+      }
+    }
+  }
   if (MDBC2 && boundp2) {
     const float* tv = tangenvel + 3 * size_t(p2);
     dvx = vr1.x - tv[0]; dvy = vr1.y - tv[1]; dvz = vr1.z - tv[2];
@@ -222,20 +252,21 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
 // Reference-structured pair: loads PosCell first and the rest only for a
 // real neighbour.
 template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false,
-          bool FT = false>
+          bool FT = false, bool NOPEN = false>
 __device__ __forceinline__ void pair_body(bool boundp2, unsigned p2, const P1& q,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
     const float2* __restrict__ press, const float* __restrict__ tau, float viscob, float viscof,
     const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
-    const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr) {
+    const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr,
+    const float* __restrict__ boundnor = nullptr) {
   const float4 ps2 = poscell[p2];
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
-  pair_eval<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT>(boundp2, p2, ps2, velrho[p2], press[p2], q,
-                                                     tau, viscob, viscof, c, a, bmode, tangenvel,
-                                                     shiftmode, code, ftomassp);
+  pair_eval<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT, NOPEN>(
+      boundp2, p2, ps2, velrho[p2], press[p2], q, tau, viscob, viscof, c, a, bmode, tangenvel,
+      shiftmode, code, ftomassp, boundnor);
 }
 
 template <int TVISCO>
@@ -288,7 +319,7 @@ __device__ inline NgLimits p1_limits(unsigned p1, const unsigned* __restrict__ d
 
 // ------------------------------------------------------------ reference-order rows kernel
 template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false,
-          bool FT = false>
+          bool FT = false, bool NOPEN = false>
 __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float viscof,
     const DivData& dvd, const Cte& c, const unsigned* __restrict__ dcell,
     const float4* __restrict__ poscell, const float4* __restrict__ velrho,
@@ -297,7 +328,8 @@ __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float
     float* __restrict__ velgrad, const uint8_t* __restrict__ bmode = nullptr,
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
     const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr,
-    const float* __restrict__ ftomassp = nullptr) {
+    const float* __restrict__ ftomassp = nullptr, const float* __restrict__ boundnor = nullptr,
+    float4* __restrict__ nopenshift = nullptr) {
   Acc a;
   acc_init(a);
   if (SHIFT) a.sh = shiftposfs[p1];
@@ -315,17 +347,26 @@ __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float
         unsigned pini, pfin;
         row_range(dvd.begincell, base + z * dvd.ncw + y * dvd.ncx, l.x0, l.x1, pini, pfin);
         for (unsigned p2 = pini; p2 < pfin; p2++)
-          pair_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT>(kind == 1, p2, q, poscell, velrho,
-                                                             press, tau, viscob, viscof, c, a,
-                                                             bmode, tangenvel, shiftmode, code,
-                                                             ftomassp);
+          pair_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT, NOPEN>(
+              kind == 1, p2, q, poscell, velrho, press, tau, viscob, viscof, c, a, bmode,
+              tangenvel, shiftmode, code, ftomassp, boundnor);
       }
   }
   store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
   if (SHIFT) shiftposfs[p1] = a.sh;
+  if (NOPEN) {
+    // Mean correction per axis over the boundary neighbours that asked for
+    // one; w = 10 marks a particle to correct (JSphGpu_ker.cu:958-976).
+    float4 ns = make_float4(0, 0, 0, 0);
+    if (a.npc[0] > 0.f) { ns.x = fast_div(a.nps[0], a.npc[0]); ns.w = 10.f; }
+    if (a.npc[1] > 0.f) { ns.y = fast_div(a.nps[1], a.npc[1]); ns.w = 10.f; }
+    if (a.npc[2] > 0.f) { ns.z = fast_div(a.nps[2], a.npc[2]); ns.w = 10.f; }
+    nopenshift[p1] = ns;
+  }
 }
 
-template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2, bool SHIFT = false, bool FT = false>
+template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2, bool SHIFT = false, bool FT = false,
+          bool NOPEN = false>
 __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
     unsigned n, unsigned pinit, float viscob, float viscof, DivData dvd, Cte c,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
@@ -334,12 +375,14 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
     float* __restrict__ ace, float* __restrict__ strain, float* __restrict__ velgrad,
     const uint8_t* __restrict__ bmode, const float* __restrict__ tangenvel, int shiftmode = 0,
     const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr,
-    const float* __restrict__ ftomassp = nullptr) {
+    const float* __restrict__ ftomassp = nullptr, const float* __restrict__ boundnor = nullptr,
+    float4* __restrict__ nopenshift = nullptr) {
   const unsigned p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
-  fluid_rows_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT>(
+  fluid_rows_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT, NOPEN>(
       p + pinit, viscob, viscof, dvd, c, dcell, poscell, velrho, press, tau, viscdt, ar, ace,
-      strain, velgrad, bmode, tangenvel, shiftmode, code, shiftposfs, ftomassp);
+      strain, velgrad, bmode, tangenvel, shiftmode, code, shiftposfs, ftomassp, boundnor,
+      nopenshift);
 }
 
 // ------------------------------------------------------------ cell-row tile kernel
@@ -741,7 +784,17 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
                   const float2* press, const float4* posf) {
   KTimer kt(ctx, "k_forces_fluid");
   const unsigned npf = t->np - t->npb;
-  if (t->d_ftomassp) {
+  if (t->d_nopenshift) {
+    // mDBC2 with no penetration: the one-lane-per-particle kernel with the
+    // no-penetration sums (shifting and floating bodies are refused with it
+    // by dsph_interaction_forces).
+    k_forces_fluid_rows<TVISCO, TDDT, VELGRAD, true, false, false, true>
+        <<<grid_for(npf, kBsForces), kBsForces, 0, ctx->stream>>>(
+            npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
+            (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
+            t->d_sps2strain, t->d_velgrad, t->d_boundmode, t->d_tangenvel, 0, nullptr, nullptr,
+            nullptr, t->d_boundnor, (float4*)t->d_nopenshift);
+  } else if (t->d_ftomassp) {
     // Floating bodies: the one-lane-per-particle kernel with the floating
     // rules (shifting, mDBC2 and the velocity gradient are refused with them
     // by dsph_interaction_forces).
@@ -1101,6 +1154,8 @@ int check_inter_parms(dsph_ctx* ctx, const dsph_inter_parms* t) {
   DSPH_REQUIRE(ctx, !t->d_ftomassp || t->d_code, "floating bodies need the particle codes");
   DSPH_REQUIRE(ctx, !t->d_ftomassp || (!t->d_shiftposfs && !t->d_boundmode && !t->d_velgrad),
                "floating bodies with shifting, mDBC2 or the velocity gradient are not built");
+  DSPH_REQUIRE(ctx, !t->d_nopenshift || (t->d_boundmode && t->d_boundnor),
+               "no penetration needs mDBC2 (boundmode, tangenvel) and the boundary normals");
   const bool sps = t->tvisco == DSPH_VISCO_LAMINARSPS;
   DSPH_REQUIRE(ctx, !sps || (t->d_spstaurho2 && t->d_sps2strain),
                "Laminar+SPS needs spstaurho2 and sps2strain");

@@ -76,6 +76,9 @@ struct dsph_sim {
   double Cs0 = 0, DtIni = 0, DtMin = 0, CFLnumber = 0.2;
   int TStep = 2, VerletSteps = 40, TVisco = 1, TDensity = 0, TBoundary = 1, ScellDiv = 1;
   int SlipMode = DSPH_SLIP_VEL0;
+  bool NoPen = false;  // mDBC2 NoPenetration (TMdbc2 == MDBC2_NoPen)
+  // mDBC2: no-slip or free slip (MdbcCorrector, JSph.cpp:897).
+  bool mdbc2_on() const { return TBoundary == DSPH_BC_MDBC && SlipMode >= DSPH_SLIP_NOSLIP; }
   int ShiftMode = DSPH_SHIFT_NONE;
   double ShiftCoef = -2.0, ShiftTFS = 0.0;
   bool CellDomFixed = false;
@@ -109,6 +112,9 @@ struct dsph_sim {
   // rewritten by every correction (JSphGpuSingle.cpp:700-707).
   uint8_t* boundmode = nullptr;
   float* tangenvel = nullptr;
+  // mDBC2 no penetration (MDBC2_NoPen): float4 per particle written by every
+  // interaction for the fluid, used by the corrector / Verlet step.
+  float* nopenshift = nullptr;
   // Shifting: float4 per particle (sums, then the displacement), fluid part
   // cleared before every interaction (JSphShifting::InitGpu without zones).
   float* shiftposfs = nullptr;
@@ -174,7 +180,8 @@ struct dsph_sim {
     void* ptrs[] = {idp, dcell, idp2, dcell2, code, code2, posxy, posz, posxy2, posz2, velrho,
                     velrho2, poscell, posxypre, poszpre, posxypre2, poszpre2, velrhopre,
                     velrhopre2, velrhom1, velrhom1b, ace, ar, viscdt, movxy, movz, sps2strain,
-                    spstau, spstau2, boundnor, boundnor2, boundmode, tangenvel, shiftposfs, d_red, d_dt,
+                    spstau, spstau2, boundnor, boundnor2, boundmode, tangenvel, nopenshift, shiftposfs,
+                    d_red, d_dt,
                     d_flags, cellpart,
                     sortpart, begincell, velgrad, closure_nu, d_ftomassp, d_ftbody,
                     d_ftcenter, d_ftace, d_ridp};
@@ -201,17 +208,19 @@ struct dsph_sim {
     TDensity = c.tdensity;
     TBoundary = c.tboundary;
     SlipMode = c.slipmode ? c.slipmode : DSPH_SLIP_VEL0;
-    if (TBoundary == DSPH_BC_MDBC && SlipMode != DSPH_SLIP_VEL0 && SlipMode != DSPH_SLIP_NOSLIP)
-      throw Err(DSPH_E_ARG, "SlipMode: only 1 (vel=0) and 2 (no-slip) are built");
-    if (TBoundary == DSPH_BC_MDBC && SlipMode == DSPH_SLIP_NOSLIP && c.casenmoving)
-      throw Err(DSPH_E_ARG, "mDBC no-slip is built for fixed boundaries only");
+    if (TBoundary == DSPH_BC_MDBC && (SlipMode < DSPH_SLIP_VEL0 || SlipMode > DSPH_SLIP_FREESLIP))
+      throw Err(DSPH_E_ARG, "SlipMode: 1 (vel=0), 2 (no-slip) or 3 (free slip)");
+    if (mdbc2_on() && c.casenmoving)
+      throw Err(DSPH_E_ARG, "mDBC no-slip and free slip are built for fixed boundaries only");
+    // NoPenetration is read for SlipMode >= no-slip only (JSph.cpp:715-716).
+    NoPen = mdbc2_on() && c.nopenetration != 0;
     ShiftMode = c.shiftmode;
     ShiftCoef = c.shiftcoef ? c.shiftcoef : -2.0;
     ShiftTFS = c.shifttfs;
     if (ShiftMode < DSPH_SHIFT_NONE || ShiftMode > DSPH_SHIFT_FULL)
       throw Err(DSPH_E_ARG, "Shifting: mode 0..3");
-    if (ShiftMode && TBoundary == DSPH_BC_MDBC && SlipMode == DSPH_SLIP_NOSLIP)
-      throw Err(DSPH_E_ARG, "shifting with mDBC no-slip is not built");
+    if (ShiftMode && mdbc2_on())
+      throw Err(DSPH_E_ARG, "shifting with mDBC no-slip or free slip is not built");
     Visco = float(c.visco);
     ViscoBoundFactor = float(c.viscoboundfactor);
     RhopOutMin = float(c.rhopoutmin);
@@ -240,8 +249,8 @@ struct dsph_sim {
     PeriActive = unsigned(c.periactive) & 7u;
     if (PeriActive && c.casenmoving)
       throw Err(DSPH_E_ARG, "periodic boundaries with moving boundaries are not built");
-    if (PeriActive && TBoundary == DSPH_BC_MDBC && SlipMode == DSPH_SLIP_NOSLIP)
-      throw Err(DSPH_E_ARG, "periodic boundaries with mDBC no-slip are not built");
+    if (PeriActive && mdbc2_on())
+      throw Err(DSPH_E_ARG, "periodic boundaries with mDBC no-slip or free slip are not built");
     if (PeriActive && ShiftMode)
       throw Err(DSPH_E_ARG, "periodic boundaries with shifting are not built");
     ft.clear();
@@ -409,9 +418,10 @@ struct dsph_sim {
     if (ShiftMode) shiftposfs = alloc<float>(4 * n);
     if (TBoundary == DSPH_BC_MDBC) {
       boundnor = alloc<float>(3 * n); boundnor2 = alloc<float>(3 * n);
-      if (SlipMode == DSPH_SLIP_NOSLIP) {
+      if (mdbc2_on()) {
         boundmode = alloc<uint8_t>(n);
         tangenvel = alloc<float>(3 * n);
+        if (NoPen) nopenshift = alloc<float>(4 * n);
       }
     }
     d_red = alloc<float>(4);
@@ -686,6 +696,7 @@ struct dsph_sim {
     regrow(velrhopre, 4); regrow(velrhopre2, 4); regrow(velrhom1, 4); regrow(velrhom1b, 4);
     regrow(sps2strain, 6); regrow(spstau, 6); regrow(spstau2, 6); regrow(shiftposfs, 4);
     regrow(boundnor, 3); regrow(boundnor2, 3); regrow(boundmode, 1); regrow(tangenvel, 3);
+    regrow(nopenshift, 4);
     regrow(velgrad, 9); regrow(closure_nu, 1);
     if (periwork) {
       chk(dsph_free(ctx, periwork), "grow free");
@@ -750,10 +761,11 @@ struct dsph_sim {
   void mdbc(bool corrector) {
     if (TBoundary != DSPH_BC_MDBC) return;
     const dsph_divdata d = divdata();
-    if (SlipMode == DSPH_SLIP_NOSLIP) {
+    if (mdbc2_on()) {
       chk(dsph_memset_async(ctx, boundmode, DSPH_BMODE_DBC, Npb), "memset boundmode");
-      chk(dsph_mdbc2_correction(ctx, Npb, Gravity, &d, Map_PosMin, posxy, posz, poscell, code,
-                                boundnor, nullptr, nullptr, velrho, boundmode, tangenvel),
+      chk(dsph_mdbc2_correction_slip(ctx, SlipMode, Npb, Gravity, &d, Map_PosMin, posxy, posz,
+                                     poscell, code, boundnor, nullptr, nullptr, velrho, boundmode,
+                                     tangenvel),
           "mdbc2");
       return;
     }
@@ -781,6 +793,8 @@ struct dsph_sim {
     t.shiftmode = ShiftMode;
     t.d_shiftposfs = shiftposfs;
     t.d_ftomassp = d_ftomassp;
+    t.d_boundnor = NoPen ? boundnor : nullptr;
+    t.d_nopenshift = nopenshift;
     chk(dsph_interaction_forces(ctx, &t), "interaction");
     if (TVisco == DSPH_VISCO_LAMINARSPS) {
       if (UseClosure)
@@ -848,6 +862,10 @@ struct dsph_sim {
                                  device_dt ? d_dt : nullptr, 2, RhopZero, RhopOutMin, RhopOutMax,
                                  Gravity, code, movxy, movz, velrho),
         "symplectic cor");
+    if (NoPen)
+      chk(dsph_mdbc2_nopen_step(ctx, Np, Npb, nopenshift, velrhopre, dt, device_dt ? d_dt : nullptr,
+                                2, velrho, movxy, movz),
+          "no penetration");
     chk(dsph_step_pos2(ctx, Np, Npb, posxypre, poszpre, movxy, movz, posxy, posz, dcell, code), "pos2");
     pre_active = false;
   }
@@ -1020,6 +1038,10 @@ struct dsph_sim {
           "verlet euler");
       VerletStep = 0;
     }
+    if (NoPen)
+      chk(dsph_mdbc2_nopen_step(ctx, Np, Npb, nopenshift, velrho, dt, nullptr, 0, velrhom1, movxy,
+                                movz),
+          "no penetration");
     chk(dsph_step_pos(ctx, Np, Npb, movxy, movz, posxy, posz, dcell, code), "pos");
     swp(velrho, velrhom1);
     run_motion(dt);

@@ -172,7 +172,9 @@ __global__ void __launch_bounds__(kBsForces) k_mdbc(unsigned n, DivData dvd, Cte
 // sums, inverse rounded to FP32), Shepard otherwise (clamped at rho0 when the
 // kernel sum is < 0.1); pressure cloned to the particle with the hydrostatic
 // gradient (KerMdbc2PressClone); tangential part of 2 v_motion - v_ghost
-// (KerMdbc2TangenVel). A dry ghost node switches the particle's mass off.
+// (KerMdbc2TangenVel), or of v_ghost for free slip (tslip =
+// DSPH_SLIP_FREESLIP, JSphGpu_mdbc_iker.cu:464-473). A dry ghost node
+// switches the particle's mass off.
 __device__ inline float3 mdbc2_tangential(float bx, float by, float bz, float3 v) {
   const float sn = sqrtf(bx * bx + by * by + bz * bz);
   const float nx = bx / sn, ny = by / sn, nz = bz / sn;
@@ -180,7 +182,7 @@ __device__ inline float3 mdbc2_tangential(float bx, float by, float bz, float3 v
   return make_float3(v.x - vdn * nx, v.y - vdn * ny, v.z - vdn * nz);
 }
 
-__global__ void __launch_bounds__(kBsForces) k_mdbc2(unsigned n, DivData dvd, Cte c,
+__global__ void __launch_bounds__(kBsForces) k_mdbc2(unsigned n, int tslip, DivData dvd, Cte c,
     float3 gravity, double3 mapposmin, const double2* __restrict__ posxy,
     const double* __restrict__ posz, const float4* __restrict__ poscell,
     const typecode* __restrict__ code, const float* __restrict__ boundnor,
@@ -297,8 +299,10 @@ __global__ void __launch_bounds__(kBsForces) k_mdbc2(unsigned n, DivData dvd, Ct
   }
   velrho[p1].w = rhofinal;
   const float ia = A[0][0];
-  const float3 v2 = make_float3(mv.x + mv.x - vpx / ia, mv.y + mv.y - vpy / ia,
-                                mv.z + mv.z - vpz / ia);
+  const float3 vg = make_float3(vpx / ia, vpy / ia, vpz / ia);
+  const float3 v2 = tslip == DSPH_SLIP_FREESLIP
+                        ? vg
+                        : make_float3(mv.x + mv.x - vg.x, mv.y + mv.y - vg.y, mv.z + mv.z - vg.z);
   const float3 t = mdbc2_tangential(bnx, bny, bnz, v2);
   tv[0] = t.x; tv[1] = t.y; tv[2] = t.z;
 }
@@ -547,7 +551,21 @@ int dsph_mdbc2_correction(dsph_ctx* ctx, uint32_t n, const float gravity[3],
                           const uint16_t* d_code, const float* d_boundnor,
                           const float* d_motionvel, const float* d_motionace, float* d_velrho,
                           uint8_t* d_boundmode, float* d_tangenvel) {
+  return dsph_mdbc2_correction_slip(ctx, DSPH_SLIP_NOSLIP, n, gravity, dvdp, mapposmin, d_posxy,
+                                    d_posz, d_poscell, d_code, d_boundnor, d_motionvel,
+                                    d_motionace, d_velrho, d_boundmode, d_tangenvel);
+}
+
+int dsph_mdbc2_correction_slip(dsph_ctx* ctx, int32_t slipmode, uint32_t n,
+                               const float gravity[3], const dsph_divdata* dvdp,
+                               const double mapposmin[3], const double* d_posxy,
+                               const double* d_posz, const float* d_poscell,
+                               const uint16_t* d_code, const float* d_boundnor,
+                               const float* d_motionvel, const float* d_motionace,
+                               float* d_velrho, uint8_t* d_boundmode, float* d_tangenvel) {
   if (!ctx || !dvdp || !mapposmin || !gravity) return DSPH_E_ARG;
+  if (slipmode != DSPH_SLIP_NOSLIP && slipmode != DSPH_SLIP_FREESLIP)
+    return set_error(ctx, DSPH_E_ARG, "dsph_mdbc2_correction: slipmode must be 2 or 3");
   if (!ctx->cte_set) return set_error(ctx, DSPH_E_STATE, "constants not set");
   if (!n) return DSPH_OK;
   DSPH_REQUIRE(ctx, d_posxy && d_posz && d_poscell && d_code && d_boundnor && d_velrho &&
@@ -555,7 +573,7 @@ int dsph_mdbc2_correction(dsph_ctx* ctx, uint32_t n, const float gravity[3],
                "dsph_mdbc2_correction: null array");
   const Cte c = make_cte(ctx->cte);
   k_mdbc2<<<grid_for(n, kBsForces), kBsForces, 0, ctx->stream>>>(
-      n, make_divdata(*dvdp), c, make_float3(gravity[0], gravity[1], gravity[2]),
+      n, slipmode, make_divdata(*dvdp), c, make_float3(gravity[0], gravity[1], gravity[2]),
       make_double3(mapposmin[0], mapposmin[1], mapposmin[2]), (const double2*)d_posxy, d_posz,
       (const float4*)d_poscell, d_code, d_boundnor, d_motionvel, d_motionace, (float4*)d_velrho,
       d_boundmode, d_tangenvel);

@@ -295,6 +295,30 @@ __global__ void __launch_bounds__(kBs) k_verlet(unsigned n, unsigned npb,
   velrhonew[p] = nw;
 }
 
+// mDBC2 no penetration (JSphGpuSimple_ker.cu:177-191, 447-461): after the
+// Symplectic corrector or the Verlet step, each axis of a flagged fluid
+// particle (w > 5) with a non-zero correction takes velocity
+// base + correction and displacement velocity * dt; other axes keep the
+// step's values.
+__global__ void __launch_bounds__(kBs) k_nopen_step(unsigned np, unsigned npb,
+    const float4* __restrict__ nopenshift, const float4* __restrict__ velbase, double dt_h,
+    const double* __restrict__ d_dt, unsigned slot, float4* __restrict__ velrho,
+    double2* __restrict__ movxy, double* __restrict__ movz) {
+  const unsigned p = npb + blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const float4 ns = nopenshift[p];
+  if (!(ns.w > 5.f)) return;
+  const double dt = dev_dt(dt_h, d_dt, slot, 1.0);
+  const float4 vb = velbase[p];
+  float4 v = velrho[p];
+  double2 m = movxy[p];
+  if (ns.x != 0) { v.x = vb.x + ns.x; m.x = double(v.x) * dt; }
+  if (ns.y != 0) { v.y = vb.y + ns.y; m.y = double(v.y) * dt; }
+  if (ns.z != 0) { v.z = vb.z + ns.z; movz[p] = double(v.z) * dt; }
+  velrho[p] = v;
+  movxy[p] = m;
+}
+
 // ---------------------------------------------------------------- positions
 
 // KerUpdatePos (JSphGpu_ker.cu:334-392). With periodic axes a particle
@@ -950,6 +974,22 @@ int dsph_step_verlet(dsph_ctx* ctx, uint32_t np, uint32_t npb, const float* d_ve
       rhopzero, rhopoutmin, rhopoutmax, make_float3(gravity[0], gravity[1], gravity[2]), d_code,
       (double2*)d_movxy, d_movz, (float4*)d_velrhonew);
   DSPH_LAUNCHED(ctx, "k_verlet");
+  return DSPH_OK;
+}
+
+int dsph_mdbc2_nopen_step(dsph_ctx* ctx, uint32_t np, uint32_t npb,
+                          const float* d_nopenshift, const float* d_velbase, double dt,
+                          const double* d_dt, uint32_t dt_slot, float* d_velrho,
+                          double* d_movxy, double* d_movz) {
+  if (!ctx) return DSPH_E_ARG;
+  if (np <= npb) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_nopenshift && d_velbase && d_velrho && d_movxy && d_movz,
+               "dsph_mdbc2_nopen_step: null array");
+  KTimer kt(ctx, "k_nopen_step");
+  k_nopen_step<<<grid_for(np - npb, kBs), kBs, 0, ctx->stream>>>(
+      np, npb, (const float4*)d_nopenshift, (const float4*)d_velbase, dt, d_dt, dt_slot,
+      (float4*)d_velrho, (double2*)d_movxy, d_movz);
+  DSPH_LAUNCHED(ctx, "k_nopen_step");
   return DSPH_OK;
 }
 

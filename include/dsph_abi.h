@@ -64,6 +64,7 @@ extern "C" {
 #define DSPH_BC_MDBC          2
 #define DSPH_SLIP_VEL0        1   /* TpSlipMode: mDBC, boundary velocity 0 */
 #define DSPH_SLIP_NOSLIP      2   /* mDBC2 no-slip (MDBC2_Std) */
+#define DSPH_SLIP_FREESLIP    3   /* mDBC2 free slip */
 #define DSPH_SHIFT_NONE       0   /* TpShifting */
 #define DSPH_SHIFT_NOBOUND    1   /* no shifting near any boundary */
 #define DSPH_SHIFT_NOFIXED    2   /* no shifting near fixed boundaries */
@@ -162,6 +163,12 @@ typedef struct dsph_inter_parms {
   const float* d_ftomassp;       /* floating bodies: mass of the particles of each
                                     body, indexed by the code's type value
                                     (FtoMasspg of StInterParmsg), else NULL */
+  const float* d_boundnor;       /* mDBC2 NoPenetration: float3[np] boundary normals
+                                    (BoundNor of StInterParmsg), else NULL */
+  float* d_nopenshift;           /* mDBC2 NoPenetration (MDBC2_NoPen): float4[np] out,
+                                    written for every fluid particle: velocity
+                                    correction per axis, w = 10 when any applies
+                                    (NoPenShift of StInterParmsg), else NULL */
 } dsph_inter_parms;
 
 /* ---------------------------------------------------------------- context */
@@ -428,6 +435,32 @@ int dsph_mdbc2_correction(dsph_ctx* ctx, uint32_t n, const float gravity[3],
                           const float* d_boundnor, const float* d_motionvel,
                           const float* d_motionace, float* d_velrho,
                           uint8_t* d_boundmode, float* d_tangenvel);
+/* The same with the slip mode of Interaction_Mdbc2Correction
+ * (JSphGpu_mdbc_iker.cu:289-487, 517-560): DSPH_SLIP_NOSLIP (as above) or
+ * DSPH_SLIP_FREESLIP, for which the tangential velocity is the tangential
+ * part of the ghost node's Shepard velocity. */
+int dsph_mdbc2_correction_slip(dsph_ctx* ctx, int32_t slipmode, uint32_t n,
+                               const float gravity[3], const dsph_divdata* dvd,
+                               const double mapposmin[3], const double* d_posxy,
+                               const double* d_posz, const float* d_poscell,
+                               const uint16_t* d_code, const float* d_boundnor,
+                               const float* d_motionvel, const float* d_motionace,
+                               float* d_velrho, uint8_t* d_boundmode, float* d_tangenvel);
+
+/* Replaces the no-penetration blocks of cusphs::ComputeStepSymplecticCor and
+ * cusphs::ComputeStepVerlet (JSphGpuSimple_ker.cu:177-191, 447-461; mDBC2
+ * with NoPenetration, MDBC2_NoPen): for fluid particles [npb,np) whose
+ * d_nopenshift (float4[np], from dsph_interaction_forces) has w > 5, each
+ * component c with a non-zero correction gets velocity d_velbase.c +
+ * nopenshift.c and displacement that velocity times dt. Run after the step
+ * kernel, on its outputs (d_velrho, d_movxy, d_movz); d_velbase = the
+ * velocity the step starts from (velrhopre for the Symplectic corrector,
+ * the current velocity for Verlet). dt = d_dt[dt_slot] when d_dt is not NULL
+ * (device-side dt), else dt. */
+int dsph_mdbc2_nopen_step(dsph_ctx* ctx, uint32_t np, uint32_t npb,
+                          const float* d_nopenshift, const float* d_velbase, double dt,
+                          const double* d_dt, uint32_t dt_slot, float* d_velrho,
+                          double* d_movxy, double* d_movz);
 
 /* Replaces cusph::Interaction_GaugeVel and the mass sums of
  * Interaction_GaugeSwl (JDsGauge_ker.cu:50-245) for n points at once
@@ -700,8 +733,9 @@ typedef struct dsph_case {
   double maprealposmin[3], maprealposmax[3];
   uint32_t casenp, casenbound, casenfixed, casenmoving, casenfloat, casenfluid;
   dsph_motion motion;            /* the moving object (ids casenfixed.. +casenmoving) */
-  int32_t slipmode;              /* mDBC SlipMode (JSph.cpp:706-716): DSPH_SLIP_VEL0 or
-                                    DSPH_SLIP_NOSLIP (fixed boundaries only) */
+  int32_t slipmode;              /* mDBC SlipMode (JSph.cpp:706-716): DSPH_SLIP_VEL0,
+                                    DSPH_SLIP_NOSLIP or DSPH_SLIP_FREESLIP (the last two:
+                                    fixed boundaries only) */
   int32_t shiftmode;             /* Shifting (JSph.cpp:744-760): DSPH_SHIFT_* */
   double shiftcoef, shifttfs;    /* ShiftCoef (-2 default), ShiftTFS (0: off) */
   int32_t periactive;            /* periodic axes, bits 1 X, 2 Y, 4 Z (JSph.cpp:810-826) */
@@ -709,7 +743,7 @@ typedef struct dsph_case {
                                     increments [axis][component]; each axis' own
                                     component is set to -MapRealSize (JSph.cpp:2186-2188) */
   uint32_t ftcount;              /* floating bodies (casenfloat particles in all) */
-  uint32_t pad_ft;
+  int32_t nopenetration;         /* mDBC2 NoPenetration (JSph.cpp:715-716): 1 = on */
   const dsph_floating* floatings; /* ftcount entries, in id order; read at creation */
 } dsph_case;
 

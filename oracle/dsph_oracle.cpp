@@ -227,6 +227,10 @@ struct Opts {
   // Floating bodies: mass of the particles of each body (FtObjs[].massp,
   // indexed by the code's type value) and the particle codes; NULL = none.
   const float* ftmassp = nullptr;
+  // mDBC2 no penetration (MDBC2_NoPen): boundary normals (float3 per
+  // particle) and the per-particle output (float4); NULL = off.
+  const float* boundnor = nullptr;
+  float* nopenshift = nullptr;
 };
 
 // Floating particle test and its body (CODE_IsFloating, CODE_GetTypeValue,
@@ -296,6 +300,7 @@ struct FAcc {
   R g[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   bool delta_off = false;
   float sh[4] = {0, 0, 0, 0};  // shifting sums in FP32 (x = FLT_MAX: cancelled)
+  float npc[3] = {0, 0, 0}, nps[3] = {0, 0, 0};  // no-penetration counts and sums (FP32)
 };
 
 template <typename R>
@@ -366,6 +371,36 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
         a.sh[2] += massrho * fz;
         a.sh[3] -= massrho * float(dot3);
       }
+      if (boundp2 && o.nopenshift && !ftp2) {
+        // No penetration (JSphCpu.cpp:952-974 with ComputeNoPenVel, :716-726;
+        // JSphGpu_ker.cu:686-707): a first-layer boundary neighbour
+        // (|normal| < 1.75 dp) within 1.25 dp with the particle less than
+        // 0.75 |normal| away along the normal; per axis, a particle moving
+        // towards the wall (fixed walls: motion velocity 0) counts it and
+        // gets -(3 - 4 max(|dr/n|, 1/4)) v n^2. FP32, as both builds.
+        const float fdx = float(dx), fdy = float(dy), fdz = float(dz);
+        const float rrmag = std::sqrt(fdx * fdx + fdy * fdy + fdz * fdz);
+        if (rrmag < 1.25f * c.dp) {
+          const float* bn = o.boundnor + 3 * size_t(p2);
+          const float norm = std::sqrt(bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]);
+          const float n[3] = {bn[0] / norm, bn[1] / norm, bn[2] / norm};
+          const float dr[3] = {fdx, fdy, fdz};
+          const float v[3] = {v1[0], v1[1], v1[2]};
+          const float normdist = n[0] * fdx + n[1] * fdy + n[2] * fdz;
+          if (normdist < 0.75f * norm && norm < 1.75f * c.dp) {
+            for (int k = 0; k < 3; k++) {
+              if (!(dr[k] * n[k] < 0.75f && std::fabs(n[k]) > 0.001f * c.dp)) continue;
+              const float vfc = v[k] * n[k];
+              if (vfc < 0.f) {
+                const float ratio = std::max(std::fabs(dr[k] / n[k]), 0.25f);
+                const float factor = -4.f * ratio + 3.f;
+                a.npc[k] += 1.f;
+                a.nps[k] -= factor * v[k] * n[k] * n[k];
+              }
+            }
+          }
+        }
+      }
       if (mdbc2) {
         // Viscous terms against the no-slip tangential velocity
         // (JSphCpu.cpp:988-995; JSphGpu_ker.cu:725-730).
@@ -433,6 +468,19 @@ R eos(int sem, const orc_cte& c, float rho) {
   return R(c.cteb) * (crpow<R>(R(rho) / R(c.rhopzero), R(c.gamma)) - R(1));
 }
 
+// Mean no-penetration correction per axis over the boundary neighbours that
+// asked for one; w = 10 marks a particle to correct, 0 none (the array is
+// cleared before each interaction: JSphCpu.cpp:519, 1103-1123).
+template <typename R>
+void store_nopen(const FAcc<R>& a, float* ns) {
+  ns[0] = ns[1] = ns[2] = ns[3] = 0.f;
+  for (int k = 0; k < 3; k++)
+    if (a.npc[k] > 0.f) {
+      ns[k] = a.nps[k] / a.npc[k];
+      ns[3] = 10.f;
+    }
+}
+
 template <typename R>
 void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np, unsigned npb,
                  unsigned npbok, const unsigned* dcell, const float* ps, const double* pos,
@@ -470,6 +518,7 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
       for (int k = 0; k < 4; k++) a2.sh[k] = a1.sh[k];
       fluid_rows<R>(o, c, true, rb, nb, p1, ps, pos, velrho, press.data(), tau, a2);
       if (shiftposfs) for (int k = 0; k < 4; k++) shiftposfs[4 * size_t(p1) + k] = a2.sh[k];
+      if (o.nopenshift) store_nopen(a2, o.nopenshift + 4 * size_t(p1));
       oax = R(float(a1.ax)) + a2.ax; oay = R(float(a1.ay)) + a2.ay; oaz = R(float(a1.az)) + a2.az;
       oar = R(float(R(float(a1.ar)) + a2.ar));
       if (o.tddt && !a1.delta_off && !a2.delta_off)
@@ -484,6 +533,7 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
       fluid_rows<R>(o, c, false, rf, nf, p1, ps, pos, velrho, press.data(), tau, a);
       fluid_rows<R>(o, c, true, rb, nb, p1, ps, pos, velrho, press.data(), tau, a);
       if (shiftposfs) for (int k = 0; k < 4; k++) shiftposfs[4 * size_t(p1) + k] = a.sh[k];
+      if (o.nopenshift) store_nopen(a, o.nopenshift + 4 * size_t(p1));
       oax = a.ax; oay = a.ay; oaz = a.az;
       oar = a.ar + (o.tddt && !a.delta_off ? a.delta : R(0));
       ovisc = a.visc;
@@ -597,6 +647,12 @@ void orc_interaction_ext(int prec, int sem, int tvisco, int tddt, float viscob, 
 
 // The same with floating bodies: ftmassp = mass of the particles of each body
 // (indexed by the code's type value), with the particle codes; NULL = none.
+void orc_interaction_np(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                        unsigned, unsigned, unsigned, const unsigned*, const float*,
+                        const double*, const float*, const float*, float*, float*, float*,
+                        float*, float*, const unsigned char*, const float*, int,
+                        const typecode*, float*, const float*, const float*, float*);
+
 void orc_interaction_ft(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
                         const orc_cte* c, const orc_div* d, unsigned np, unsigned npb,
                         unsigned npbok, const unsigned* dcell, const float* poscell,
@@ -604,8 +660,25 @@ void orc_interaction_ft(int prec, int sem, int tvisco, int tddt, float viscob, f
                         float* ar, float* viscdt, float* strain, float* velgrad,
                         const unsigned char* bmode, const float* tangenvel, int shiftmode,
                         const typecode* code, float* shiftposfs, const float* ftmassp) {
+  orc_interaction_np(prec, sem, tvisco, tddt, viscob, viscof, c, d, np, npb, npbok, dcell,
+                     poscell, pos, velrho, tau, ace, ar, viscdt, strain, velgrad, bmode,
+                     tangenvel, shiftmode, code, shiftposfs, ftmassp, nullptr, nullptr);
+}
+
+// The same with mDBC2 no penetration: boundnor = float3 boundary normals per
+// particle and nopenshift = float4 output per particle (written for the fluid;
+// needs bmode), or NULL.
+void orc_interaction_np(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
+                        const orc_cte* c, const orc_div* d, unsigned np, unsigned npb,
+                        unsigned npbok, const unsigned* dcell, const float* poscell,
+                        const double* pos, const float* velrho, const float* tau, float* ace,
+                        float* ar, float* viscdt, float* strain, float* velgrad,
+                        const unsigned char* bmode, const float* tangenvel, int shiftmode,
+                        const typecode* code, float* shiftposfs, const float* ftmassp,
+                        const float* boundnor, float* nopenshift) {
   Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof, bmode, tangenvel,
-         shiftposfs ? shiftmode : 0, code, ftmassp && code ? ftmassp : nullptr};
+         shiftposfs ? shiftmode : 0, code, ftmassp && code ? ftmassp : nullptr,
+         nopenshift ? boundnor : nullptr, boundnor ? nopenshift : nullptr};
   if (prec == 0)
     interaction<float>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
                        viscdt, strain, velgrad, o.shiftmode ? shiftposfs : nullptr);
@@ -739,7 +812,8 @@ void mdbc_correction(int sem, const orc_cte& c, const orc_div& d, unsigned n, co
 // matrix is invertible and its scaled infinity-norm condition number
 // dp^2 |A| |A^-1| <= 50 (else Shepard; Shepard clamped at rho0 when the
 // kernel sum is < 0.1), pressure cloned to the particle with the hydrostatic
-// gradient, and the tangential velocity of 2 v_motion - v_ghost.
+// gradient, and the tangential velocity of 2 v_motion - v_ghost (no-slip) or
+// of v_ghost (free slip, JSphCpu_mdbc.cpp:412-420; JSphGpu_mdbc_iker.cu:464-473).
 // CPU build: JSphCpu_mdbc.cpp:246-455 (dr from FP64 positions, matrix and
 // velocity sums in FP64, FP64 Cs0 and Dp); GPU build:
 // KerInteractionMdbc2Correction_Fast, JSphGpu_mdbc_iker.cu:289-487 (dr from
@@ -751,7 +825,7 @@ void mdbc2_correction(int sem, const orc_cte& c, const orc_div& d, unsigned n, c
                       const float* ps, const typecode* code, const float* boundnor,
                       const float* gravity, const float* motionvel, const float* motionace,
                       double cs0d, double dpd, float* velrho, unsigned char* bmode,
-                      float* tangenvel) {
+                      float* tangenvel, bool freeslip) {
   const R ks2 = R(c.kernelsize2), h = R(c.kernelh), awen = R(c.awen), bwenh = R(c.bwenh);
   const R massf = R(c.massf);
   const bool f64 = (sem == SEM_F64);
@@ -884,7 +958,7 @@ void mdbc2_correction(int sem, const orc_cte& c, const orc_div& d, unsigned n, c
     vr1[3] = rhofinal;
     float vg[3], v2[3];
     for (int k = 0; k < 3; k++) vg[k] = float(velp[k] / a[0][0]);
-    for (int k = 0; k < 3; k++) v2[k] = mv[k] + mv[k] - vg[k];
+    for (int k = 0; k < 3; k++) v2[k] = freeslip ? vg[k] : mv[k] + mv[k] - vg[k];
     tangential(bn, v2, tv);
   }
 }
@@ -1190,20 +1264,54 @@ void orc_mdbc(int prec, int sem, const orc_cte* c, const orc_div* d, unsigned n,
 // mDBC2 correction of boundary particles [0,n) (see mdbc2_correction).
 // prec: 0 FP32, 1 FP64; sem: SEM_GPU or SEM_CPU. motionvel/motionace may be
 // NULL (fixed boundaries). cs0 and dp are the FP64 case constants.
+void orc_mdbc2_slip(int, int, int, const orc_cte*, const orc_div*, unsigned, const double*,
+                    const float*, const typecode*, const float*, const float*, const float*,
+                    const float*, double, double, float*, unsigned char*, float*);
+
 void orc_mdbc2(int prec, int sem, const orc_cte* c, const orc_div* d, unsigned n,
                const double* pos, const float* poscell, const typecode* code,
                const float* boundnor, const float* gravity, const float* motionvel,
                const float* motionace, double cs0, double dp, float* velrho,
                unsigned char* bmode, float* tangenvel) {
+  orc_mdbc2_slip(prec, sem, 2, c, d, n, pos, poscell, code, boundnor, gravity, motionvel,
+                 motionace, cs0, dp, velrho, bmode, tangenvel);
+}
+
+// The same with the slip mode: 2 no-slip, 3 free slip.
+void orc_mdbc2_slip(int prec, int sem, int slipmode, const orc_cte* c, const orc_div* d,
+                    unsigned n, const double* pos, const float* poscell, const typecode* code,
+                    const float* boundnor, const float* gravity, const float* motionvel,
+                    const float* motionace, double cs0, double dp, float* velrho,
+                    unsigned char* bmode, float* tangenvel) {
+  const bool fs = slipmode == 3;
   if (prec == 1)
     mdbc2_correction<double, double>(SEM_F64, *c, *d, n, pos, poscell, code, boundnor, gravity,
-                                     motionvel, motionace, cs0, dp, velrho, bmode, tangenvel);
+                                     motionvel, motionace, cs0, dp, velrho, bmode, tangenvel, fs);
   else if (sem == SEM_CPU)
     mdbc2_correction<float, double>(sem, *c, *d, n, pos, poscell, code, boundnor, gravity,
-                                    motionvel, motionace, cs0, dp, velrho, bmode, tangenvel);
+                                    motionvel, motionace, cs0, dp, velrho, bmode, tangenvel, fs);
   else
     mdbc2_correction<float, float>(sem, *c, *d, n, pos, poscell, code, boundnor, gravity,
-                                   motionvel, motionace, cs0, dp, velrho, bmode, tangenvel);
+                                   motionvel, motionace, cs0, dp, velrho, bmode, tangenvel, fs);
+}
+
+// No-penetration step correction (JSphCpu.cpp:1555-1570, 1920-1935;
+// JSphGpuSimple_ker.cu:177-191, 447-461) on the outputs of the Symplectic
+// corrector or the Verlet step: a flagged fluid particle (w > 5) takes, on
+// each axis with a non-zero correction, velocity base + correction and
+// displacement velocity * dt. velbase: velrhopre (corrector) or the current
+// velocity (Verlet); mov: double3 per particle.
+void orc_nopen_step(unsigned np, unsigned npb, const float* nopenshift, const float* velbase,
+                    double dt, float* velrho, double* mov) {
+  for (unsigned p = npb; p < np; p++) {
+    const float* ns = nopenshift + 4 * size_t(p);
+    if (!(ns[3] > 5.f)) continue;
+    for (int k = 0; k < 3; k++)
+      if (ns[k] != 0) {
+        velrho[4 * size_t(p) + k] = velbase[4 * size_t(p) + k] + ns[k];
+        mov[3 * size_t(p) + k] = double(velrho[4 * size_t(p) + k]) * dt;
+      }
+  }
 }
 // Shifting displacement of fluid [pini, pini+n) (JSphShifting::RunCpu,
 // JSphShifting.cpp:390-420; GPU KerRunShifting JSphShifting_ker.cu:123-151):

@@ -67,6 +67,15 @@ void orc_mdbc2(int, int, const orc_cte*, const orc_div*, unsigned, const double*
                const typecode*, const float*, const float*, const float*, const float*, double,
                double, float*, unsigned char*, float*);
 void orc_mdbc2_keep(unsigned, const unsigned char*, const float*, float*);
+void orc_mdbc2_slip(int, int, int, const orc_cte*, const orc_div*, unsigned, const double*,
+                    const float*, const typecode*, const float*, const float*, const float*,
+                    const float*, double, double, float*, unsigned char*, float*);
+void orc_interaction_np(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                        unsigned, unsigned, unsigned, const unsigned*, const float*,
+                        const double*, const float*, const float*, float*, float*, float*,
+                        float*, float*, const unsigned char*, const float*, int,
+                        const typecode*, float*, const float*, const float*, float*);
+void orc_nopen_step(unsigned, unsigned, const float*, const float*, double, float*, double*);
 void orc_interaction_ext(int, int, int, int, float, float, const orc_cte*, const orc_div*,
                          unsigned, unsigned, unsigned, const unsigned*, const float*,
                          const double*, const float*, const float*, float*, float*, float*,
@@ -105,6 +114,8 @@ struct Sim {
   std::vector<float> velrho, velrhopre, velrhom1, poscell, ace, ar, viscdt, strain, tau;
   std::vector<float> boundnor;  // doubled normals (ConfigBoundNormals), mDBC only
   int TBoundary = 1, SlipMode = 1;
+  bool NoPen = false;                // mDBC2 NoPenetration (MDBC2_NoPen)
+  std::vector<float> nopenshift;     // float4 per particle, rewritten by every interaction
   int ShiftMode = 0;
   float ShiftCoef = -2.f, ShiftTFS = 0.f;
   std::vector<float> shiftposfs;     // shifting float4 per particle
@@ -159,11 +170,13 @@ struct Sim {
     c.ddtgz = float(double(c.rhopzero) * double(std::fabs(g[2])) / double(c.cteb));
     TStep = s.tstep; TVisco = s.tvisco; TDDT = s.tdensity; TBoundary = s.tboundary;
     SlipMode = s.slipmode ? s.slipmode : 1;
+    // NoPenetration is read for SlipMode >= no-slip only (JSph.cpp:715-716).
+    NoPen = TBoundary == 2 && SlipMode >= 2 && s.nopenetration != 0;
     ShiftMode = s.shiftmode;
     ShiftCoef = float(s.shiftcoef ? s.shiftcoef : -2.0);
     ShiftTFS = float(s.shifttfs);
-    if (TBoundary == 2 && SlipMode == 2 && s.casenmoving)
-      throw std::runtime_error("mDBC no-slip is built for fixed boundaries only");
+    if (TBoundary == 2 && SlipMode >= 2 && s.casenmoving)
+      throw std::runtime_error("mDBC no-slip and free slip are built for fixed boundaries only");
     VerletSteps = s.verletsteps ? s.verletsteps : 40;
     Visco = float(s.visco); ViscoB = Visco * float(s.viscoboundfactor);
     RhopOutMin = float(s.rhopoutmin); RhopOutMax = float(s.rhopoutmax);
@@ -468,14 +481,16 @@ struct Sim {
     // prec 0: FP32 GPU semantics, 1: FP64 yardstick, 2: FP32 CPU semantics.
     const bool m2 = mdbc2_on();
     if (ShiftMode) shiftposfs.assign(4 * size_t(Np), 0.f);  // JSphShifting::InitCpu, no zones
-    orc_interaction_ft(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT,
+    if (NoPen) nopenshift.assign(4 * size_t(Np), 0.f);  // JSphCpu.cpp:519
+    orc_interaction_np(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT,
                        ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
                        dcell.data(), poscell.data(), pos.data(), velrho.data(),
                        TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(),
                        viscdt.data(), TVisco == 2 ? strain.data() : nullptr, nullptr,
                        m2 ? bmode.data() : nullptr, m2 ? tangenvel.data() : nullptr, ShiftMode,
                        code.data(), ShiftMode ? shiftposfs.data() : nullptr,
-                       ft.empty() ? nullptr : ftmassp.data());
+                       ft.empty() ? nullptr : ftmassp.data(), NoPen ? boundnor.data() : nullptr,
+                       NoPen ? nopenshift.data() : nullptr);
     if (TVisco == 2) orc_sps_tau(Np, Npb, SpsSmag, SpsBlin, velrho.data(), strain.data(), tau.data());
     float vd = 0, a2 = 0;
     for (unsigned p = 0; p < Np; p++) vd = std::max(vd, viscdt[p]);
@@ -614,16 +629,16 @@ struct Sim {
   // (SLIP_Vel0) before the predictor (Symplectic) or the Verlet interaction,
   // not before the corrector (MdbcCorrector=false); no-slip (mDBC2) before
   // every interaction (MdbcCorrector=true), modes cleared to DBC first.
-  bool mdbc2_on() const { return TBoundary == 2 && SlipMode == 2; }
+  bool mdbc2_on() const { return TBoundary == 2 && SlipMode >= 2; }
   void mdbc(bool corrector = false) {
     if (TBoundary != 2) return;
     const orc_div d = divdata();
     if (mdbc2_on()) {
       bmode.assign(Npb, 0);
       tangenvel.assign(3 * size_t(Npb), 0.f);
-      orc_mdbc2(prec == 1 ? 1 : 0, prec == 2 ? 1 : 0, &c, &d, Npb, pos.data(), poscell.data(),
-                code.data(), boundnor.data(), g, nullptr, nullptr, Cs0, cs.dp, velrho.data(),
-                bmode.data(), tangenvel.data());
+      orc_mdbc2_slip(prec == 1 ? 1 : 0, prec == 2 ? 1 : 0, SlipMode, &c, &d, Npb, pos.data(),
+                     poscell.data(), code.data(), boundnor.data(), g, nullptr, nullptr, Cs0,
+                     cs.dp, velrho.data(), bmode.data(), tangenvel.data());
       return;
     }
     if (corrector) return;
@@ -808,6 +823,8 @@ struct Sim {
     mov.assign(3 * size_t(Np), 0.0);
     orc_sym_cor(Np, Npb, velrhopre.data(), ar.data(), ace.data(), dt * .5, dt, c.rhopzero,
                 RhopOutMin, RhopOutMax, g, code.data(), mov.data(), velrho.data());
+    if (NoPen) orc_nopen_step(Np, Npb, nopenshift.data(), velrhopre.data(), dt, velrho.data(),
+                              mov.data());
     if (ShiftMode) orc_add_shift(Np, Npb, shiftposfs.data(), mov.data());
     if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrhopre.data(), velrho.data());
     orc_step_pos(Np, Npb, &c, pospre.data(), mov.data(), pos.data(), dcell.data(), code.data());
@@ -837,6 +854,7 @@ struct Sim {
       if (mdbc2_on()) orc_mdbc2_keep(Npb, bmode.data(), velrho.data(), nw.data());
       VerletStep = 0;
     }
+    if (NoPen) orc_nopen_step(Np, Npb, nopenshift.data(), velrho.data(), dt, nw.data(), mov.data());
     if (ShiftMode) orc_add_shift(Np, Npb, shiftposfs.data(), mov.data());
     std::vector<double> p0 = pos;
     orc_step_pos(Np, Npb, &c, p0.data(), mov.data(), pos.data(), dcell.data(), code.data());

@@ -63,6 +63,11 @@ def load() -> C.CDLL:
             "orc_mdbc2": ([i, i, C.POINTER(OrcCte), C.POINTER(OrcDiv), u, vp, vp, vp, vp, vp, vp,
                            vp, d, d, vp, vp, vp], None),
             "orc_mdbc2_keep": ([u, vp, vp, vp], None),
+            "orc_mdbc2_slip": ([i, i, i, C.POINTER(OrcCte), C.POINTER(OrcDiv), u, vp, vp, vp,
+                                vp, vp, vp, vp, d, d, vp, vp, vp], None),
+            "orc_nopen_step": ([u, u, vp, vp, d, vp, vp], None),
+            "orc_interaction_np": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
+                                    u, u] + [vp] * 12 + [i, vp, vp, vp, vp, vp], None),
             "orc_matrix_rot": ([d, vp, vp, vp], None),
             "orc_interaction_ext": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
                                      u, u] + [vp] * 12 + [i, vp, vp], None),

@@ -99,6 +99,26 @@ CASES = {
     # mDBC2 (SlipMode no-slip) on the dam break's fixed walls.
     "ref_dambreak_dp0.03_mdbc2": ("dam_break", 0.03, 10, (1, 5, 10),
                                   dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4)),
+    # mDBC2 free slip (SlipMode 3); no-slip with no penetration
+    # (NoPenetration, MDBC2_NoPen) under the Symplectic corrector; free slip
+    # with no penetration under Verlet. The column starts moving at 1 m/s
+    # towards the floor and the back wall, so that fluid particles come
+    # within 0.75 of a normal length of the first wall layer (where the
+    # no-penetration correction acts) within the saved steps.
+    "ref_dambreak_dp0.03_mdbc2_freeslip": ("dam_break", 0.03, 40, (1, 10, 20, 40),
+                                           dict(tboundary=2, slipmode=3, tvisco=3, visco=1e-4,
+                                                vel0=(-1.0, 0.0, -1.0))),
+    "ref_dambreak_dp0.03_mdbc2_nopen": ("dam_break", 0.03, 40, (1, 10, 20, 40),
+                                        dict(tboundary=2, slipmode=2, nopenetration=1, tvisco=3,
+                                             visco=1e-4, vel0=(-1.0, 0.0, -1.0))),
+    "ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet": (
+        "dam_break", 0.03, 40, (1, 10, 20, 40),
+        dict(tboundary=2, slipmode=3, nopenetration=1, tstep=1, tvisco=3, visco=1e-4,
+             vel0=(-1.0, 0.0, -1.0))),
+    # The same start without no penetration (the correction's effect).
+    "ref_dambreak_dp0.03_mdbc2_moving": ("dam_break", 0.03, 40, (1, 10, 20, 40),
+                                         dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4,
+                                              vel0=(-1.0, 0.0, -1.0))),
     # Channel periodic in x with the fluid moving at 2 m/s: particles leave
     # through one face and re-enter through the other (Symplectic + DBC, and
     # Verlet + mDBC + laminar viscosity).
@@ -111,7 +131,7 @@ CASES = {
 
 
 # Keys of a CASES entry that are arguments of the case generator, not Parameters.
-GENERATOR_ARGS = ("wavemaker",)
+GENERATOR_ARGS = ("wavemaker", "vel0")
 
 
 def run_reference(case, nsteps: int, workdir: Path) -> list[dict]:

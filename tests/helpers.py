@@ -273,8 +273,9 @@ def mdbc2_state(c):
             "pos": np.ascontiguousarray(d["pos"])}
 
 
-def oracle_mdbc2(c, s, prec, sem, velrho=None):
-    """orc_mdbc2 on state s: (velrho after the correction, boundmode, tangenvel)."""
+def oracle_mdbc2(c, s, prec, sem, velrho=None, slipmode=2):
+    """orc_mdbc2_slip on state s (slipmode 2 no-slip, 3 free slip): (velrho
+    after the correction, boundmode, tangenvel)."""
     import ctypes as C
     import oracle.oracle as orc
     vr = (s["d"]["velrho"] if velrho is None else velrho).copy()
@@ -282,11 +283,11 @@ def oracle_mdbc2(c, s, prec, sem, velrho=None):
     bm = np.zeros(npb, np.uint8)
     tv = np.zeros((npb, 3), np.float32)
     g = np.array(c.gravity, np.float32)
-    orc.load().orc_mdbc2(prec, sem, C.byref(s["cte"]), C.byref(s["odiv"]), npb,
-                         s["pos"].ctypes.data, s["poscell"].ctypes.data,
-                         s["d"]["code"].ctypes.data, s["bn2"].ctypes.data, g.ctypes.data, None,
-                         None, float(c.cs0), float(c.dp), vr.ctypes.data, bm.ctypes.data,
-                         tv.ctypes.data)
+    orc.load().orc_mdbc2_slip(prec, sem, slipmode, C.byref(s["cte"]), C.byref(s["odiv"]), npb,
+                              s["pos"].ctypes.data, s["poscell"].ctypes.data,
+                              s["d"]["code"].ctypes.data, s["bn2"].ctypes.data, g.ctypes.data,
+                              None, None, float(c.cs0), float(c.dp), vr.ctypes.data,
+                              bm.ctypes.data, tv.ctypes.data)
     return vr, bm, tv
 
 

@@ -1,8 +1,10 @@
-"""mDBC2 (SlipMode no-slip) on the GPU against the oracle: the correction
-kernel (density, boundary mode, tangential velocity), the fluid interaction
-with boundary modes, and the integrators' rule for mDBC2 boundaries. The
-whole step against the reference CPU build is in test_gpu_vs_reference.py
-(fixture ref_dambreak_dp0.03_mdbc2)."""
+"""mDBC2 (SlipMode no-slip and free slip) on the GPU against the oracle: the
+correction kernel (density, boundary mode, tangential velocity), the fluid
+interaction with boundary modes and with the no-penetration sums, the
+no-penetration step correction, and the integrators' rule for mDBC2
+boundaries. The whole step against the reference CPU build is in
+test_gpu_vs_reference.py (fixtures ref_dambreak_dp0.03_mdbc2*)."""
+import dataclasses
 import ctypes as C
 
 import numpy as np
@@ -17,7 +19,7 @@ from tests.helpers import (Dev, divdata, hydrostatic_mdbc2_case, mdbc2_state, or
 pytestmark = pytest.mark.gpu
 
 
-def _gpu_mdbc2(lib, ctx, c, s, velrho):
+def _gpu_mdbc2(lib, ctx, c, s, velrho, slipmode=N.SLIP_NOSLIP):
     acte = to_cte(s["cte"], tboundary=N.BC_MDBC)
     assert lib.dsph_set_constants(ctx, C.byref(acte)) == 0
     bc = Dev(lib, ctx, s["be"])
@@ -34,8 +36,14 @@ def _gpu_mdbc2(lib, ctx, c, s, velrho):
     d_tv = Dev.zeros(lib, ctx, (npb, 3), np.float32)
     g = (C.c_float * 3)(*c.gravity)
     mp = (C.c_double * 3)(*s["cte"].mapposmin)
-    rc = lib.dsph_mdbc2_correction(ctx, npb, g, C.byref(dv), mp, d_xy.ptr, d_z.ptr, d_ps.ptr,
-                                   d_code.ptr, d_bn.ptr, None, None, d_vr.ptr, d_bm.ptr, d_tv.ptr)
+    if slipmode == N.SLIP_NOSLIP:
+        rc = lib.dsph_mdbc2_correction(ctx, npb, g, C.byref(dv), mp, d_xy.ptr, d_z.ptr, d_ps.ptr,
+                                       d_code.ptr, d_bn.ptr, None, None, d_vr.ptr, d_bm.ptr,
+                                       d_tv.ptr)
+    else:
+        rc = lib.dsph_mdbc2_correction_slip(ctx, slipmode, npb, g, C.byref(dv), mp, d_xy.ptr,
+                                            d_z.ptr, d_ps.ptr, d_code.ptr, d_bn.ptr, None, None,
+                                            d_vr.ptr, d_bm.ptr, d_tv.ptr)
     assert rc == 0, lib.dsph_last_error(ctx)
     return d_vr.get(), d_bm.get(), d_tv.get(), dv, bc
 
@@ -186,3 +194,128 @@ def test_integrators_keep_mdbc2_boundaries(lib, ctx):
         ref = b.copy()
         orc.load().orc_mdbc2_keep(npb, bm.ctypes.data, vpre.ctypes.data, ref.ctypes.data)
         np.testing.assert_array_equal(a, ref)
+
+
+def test_mdbc2_freeslip_correction_matches_oracle(lib, ctx):
+    """Free slip: the same density and modes as no-slip, the tangential
+    velocity of the ghost node's Shepard velocity instead of its mirror."""
+    c = _cases()["perturbed_dam_break"]
+    s = mdbc2_state(c)
+    npb = s["npb"]
+    g_vr, g_bm, g_tv, _, _ = _gpu_mdbc2(lib, ctx, c, s, s["d"]["velrho"], N.SLIP_FREESLIP)
+    o_vr, o_bm, o_tv = oracle_mdbc2(c, s, 0, orc.SEM_GPU, slipmode=N.SLIP_FREESLIP)
+    n_vr, n_bm, n_tv = oracle_mdbc2(c, s, 0, orc.SEM_GPU, slipmode=N.SLIP_NOSLIP)
+    same = g_bm == o_bm
+    assert same.mean() > 0.999, (np.count_nonzero(~same), npb)
+    e_rho = np.abs(g_vr[:npb, 3] - o_vr[:npb, 3])[same].max()
+    on = same & (g_bm == 1)
+    e_tv = rel_err(g_tv[on], o_tv[on])
+    d_ns = rel_err(n_tv[on], o_tv[on])
+    print(f"free slip: rho |gpu-oracle32| {e_rho:.2e} tangenvel {e_tv:.2e}; no-slip's "
+          f"tangential velocity differs by {d_ns:.2e}")
+    assert e_rho < 5e-3 and e_tv < 1e-4
+    assert d_ns > 0.5
+    assert np.abs(o_vr[:npb, 3] - n_vr[:npb, 3]).max() == 0.0   # density: slip mode plays no part
+
+
+def _nopen_case():
+    """The perturbed mDBC2 dam break with the fluid moved 0.3 dp towards the
+    floor and the back wall, so that the first fluid layers sit closer than
+    0.75 of a normal length to the first wall layer (where the
+    no-penetration correction acts), about half of them moving towards it."""
+    c = _cases()["perturbed_dam_break"]
+    pos = c.pos.copy()
+    pos[c.nbound:, 0] -= 0.3 * c.dp
+    pos[c.nbound:, 2] -= 0.3 * c.dp
+    return dataclasses.replace(c, pos=pos)
+
+
+@pytest.mark.parametrize("tvisco,tddt", [(N.VISCO_LAMINAR, N.DDT_DDT2FULL),
+                                         (N.VISCO_LAMINARSPS, N.DDT_DDT2)])
+def test_interaction_no_penetration_matches_oracle(lib, ctx, tvisco, tddt):
+    c = _nopen_case()
+    s = mdbc2_state(c)
+    n, npb, npbok = s["n"], s["npb"], s["npbok"]
+    o_vr, o_bm, o_tv = oracle_mdbc2(c, s, 0, orc.SEM_GPU)
+    rng = np.random.default_rng(9)
+    tau = (rng.standard_normal((n, 6)) * 1e-3).astype(np.float32)
+    visco = 1e-3
+    acte = to_cte(s["cte"], tboundary=N.BC_MDBC)
+    assert lib.dsph_set_constants(ctx, C.byref(acte)) == 0
+    bc = Dev(lib, ctx, s["be"])
+    dv = divdata(s["celldiv"], bc, s["cte"])
+    bn2 = np.ascontiguousarray(s["bn2"], np.float32)
+    gd = {k: Dev(lib, ctx, a) for k, a in (("dcell", s["dcell"]), ("poscell", s["poscell"]),
+                                          ("velrho", o_vr), ("code", s["d"]["code"]),
+                                          ("bm", o_bm), ("tv", o_tv), ("tau", tau),
+                                          ("bn", bn2))}
+    ace = Dev.zeros(lib, ctx, (n, 3), np.float32)
+    ar = Dev.zeros(lib, ctx, n, np.float32)
+    vdt = Dev.zeros(lib, ctx, n, np.float32)
+    strain = Dev.zeros(lib, ctx, (n, 6), np.float32)
+    ns = Dev(lib, ctx, np.full((n, 4), 7.0, np.float32))   # fluid rows are all rewritten
+    t = N.DsphInterParms()
+    t.tkernel, t.tvisco, t.tdensity = N.KERNEL_WENDLAND, tvisco, tddt
+    t.viscob, t.viscof = visco, visco
+    t.np, t.npb, t.npbok = n, npb, npbok
+    t.divdata = dv
+    t.d_dcell, t.d_poscell, t.d_velrho, t.d_code = (gd["dcell"].ptr, gd["poscell"].ptr,
+                                                     gd["velrho"].ptr, gd["code"].ptr)
+    t.d_spstaurho2 = gd["tau"].ptr if tvisco == N.VISCO_LAMINARSPS else None
+    t.d_viscdt, t.d_ar, t.d_ace, t.d_sps2strain = vdt.ptr, ar.ptr, ace.ptr, strain.ptr
+    t.d_boundmode, t.d_tangenvel = gd["bm"].ptr, gd["tv"].ptr
+    t.d_boundnor, t.d_nopenshift = gd["bn"].ptr, ns.ptr
+    assert lib.dsph_interaction_forces(ctx, C.byref(t)) == 0, lib.dsph_last_error(ctx)
+    g = {"ace": ace.get(), "ar": ar.get(), "viscdt": vdt.get(), "ns": ns.get()}
+    oa = np.zeros((n, 3), np.float32)
+    oar = np.zeros(n, np.float32)
+    ov = np.zeros(n, np.float32)
+    os_ = np.zeros((n, 6), np.float32)
+    ons = np.zeros((n, 4), np.float32)
+    orc.load().orc_interaction_np(
+        0, orc.SEM_GPU, tvisco, tddt, visco, visco, C.byref(s["cte"]), C.byref(s["odiv"]), n,
+        npb, npbok, s["dcell"].ctypes.data, s["poscell"].ctypes.data, s["pos"].ctypes.data,
+        o_vr.ctypes.data, tau.ctypes.data if tvisco == N.VISCO_LAMINARSPS else None,
+        oa.ctypes.data, oar.ctypes.data, ov.ctypes.data, os_.ctypes.data, None,
+        o_bm.ctypes.data, o_tv.ctypes.data, 0, None, None, None, bn2.ctypes.data,
+        ons.ctypes.data)
+    fl = slice(npb, None)
+    e = {"ace": rel_err(g["ace"][fl], oa[fl]), "ar": rel_err(g["ar"][fl], oar[fl]),
+         "viscdt": rel_err(g["viscdt"], ov)}
+    gw, ow = g["ns"][fl, 3], ons[fl, 3]
+    same = gw == ow
+    flagged = same & (ow > 5)
+    e["nopenshift"] = rel_err(g["ns"][fl][flagged, :3], ons[fl][flagged, :3])
+    print(f"tvisco {tvisco} tddt {tddt}: {int(flagged.sum())} particles corrected, "
+          f"{int((~same).sum())} flags differ; {e}", flush=True)
+    assert flagged.sum() > 100
+    assert same.mean() > 0.999
+    assert np.all(g["ns"][fl][~(gw > 5)] == 0)      # unflagged: cleared, w = 0
+    for k, v in e.items():
+        assert v < 1e-5, (k, v)
+
+
+def test_no_penetration_step_matches_oracle(lib, ctx):
+    """dsph_mdbc2_nopen_step on random flags and corrections (some axes 0)
+    equals the oracle's restatement bit for bit."""
+    n, npb, dt = 5000, 700, 1.3e-4
+    rng = np.random.default_rng(3)
+    ns = (rng.standard_normal((n, 4)) * 0.2).astype(np.float32)
+    ns[:, 3] = np.where(rng.random(n) < 0.6, 10.0, 0.0)
+    ns[rng.random((n, 4)) < 0.3] = 0.0
+    base = rng.standard_normal((n, 4)).astype(np.float32)
+    vr = rng.standard_normal((n, 4)).astype(np.float32)
+    mov = rng.standard_normal((n, 3))
+    ovr, omov = vr.copy(), mov.copy()
+    orc.load().orc_nopen_step(n, npb, ns.ctypes.data, base.ctypes.data, dt, ovr.ctypes.data,
+                              omov.ctypes.data)
+    d_vr = Dev(lib, ctx, vr)
+    d_xy = Dev(lib, ctx, np.ascontiguousarray(mov[:, :2]))
+    d_z = Dev(lib, ctx, np.ascontiguousarray(mov[:, 2]))
+    rc = lib.dsph_mdbc2_nopen_step(ctx, n, npb, Dev(lib, ctx, ns).ptr, Dev(lib, ctx, base).ptr,
+                                   dt, None, 0, d_vr.ptr, d_xy.ptr, d_z.ptr)
+    assert rc == 0, lib.dsph_last_error(ctx)
+    np.testing.assert_array_equal(d_vr.get(), ovr)
+    np.testing.assert_array_equal(d_xy.get(), omov[:, :2])
+    np.testing.assert_array_equal(d_z.get(), omov[:, 2])
+    assert not np.array_equal(ovr, vr)

@@ -26,7 +26,11 @@ pytestmark = pytest.mark.gpu
                                   "ref_flume_dp0.01_piston_mdbc_shift_nofixed",
                                   "ref_dambreak_dp0.03_viscbound2",
                                   "ref_channel_dp0.02_periodic",
-                                  "ref_channel_dp0.02_periodic_mdbc_verlet"])
+                                  "ref_channel_dp0.02_periodic_mdbc_verlet",
+                                  "ref_dambreak_dp0.03_mdbc2_moving",
+                                  "ref_dambreak_dp0.03_mdbc2_freeslip",
+                                  "ref_dambreak_dp0.03_mdbc2_nopen",
+                                  "ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)

@@ -19,7 +19,9 @@ free-surface threshold, and full); a channel periodic in x (Symplectic +
 DBC, Verlet + mDBC + laminar viscosity) over 60 steps in which particles
 leave through one face and re-enter through the other; the dam break with
 a floating box of half the water density at the column's surface over 40
-steps (rigid-body motion from the fluid forces).
+steps (rigid-body motion from the fluid forces); mDBC2 free slip, and
+no penetration (Symplectic with no-slip, Verlet with free slip), with the
+column thrown at 1 m/s against the floor and the back wall over 40 steps.
 """
 import numpy as np
 import pytest
@@ -36,7 +38,9 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_dambreak_dp0.03_shift_full", "ref_flume_dp0.01_piston_mdbc_ddt1",
             "ref_dambreak_dp0.03_mdbc2_verlet_ddt1", "ref_flume_dp0.01_piston_mdbc_shift_nofixed",
             "ref_dambreak_dp0.03_viscbound2", "ref_channel_dp0.02_periodic",
-            "ref_channel_dp0.02_periodic_mdbc_verlet", "ref_dambreak_dp0.03_floating"]
+            "ref_channel_dp0.02_periodic_mdbc_verlet", "ref_dambreak_dp0.03_floating",
+            "ref_dambreak_dp0.03_mdbc2_moving", "ref_dambreak_dp0.03_mdbc2_freeslip",
+            "ref_dambreak_dp0.03_mdbc2_nopen", "ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
@@ -70,3 +74,27 @@ def test_fixture_detects_a_planted_error():
     with pytest.raises(AssertionError):
         compare_with_reference(fx, len(fx["steps"]) - 1, o.stats()["timestep"], case.pos,
                                d["pos"][idx], d["velrho"][idx, :3], d["velrho"][idx, 3])
+
+
+@pytest.mark.parametrize("name,wrong", [
+    ("ref_dambreak_dp0.03_mdbc2_nopen", dict(nopenetration=0)),
+    ("ref_dambreak_dp0.03_mdbc2_freeslip", dict(slipmode=2)),
+    ("ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet", dict(nopenetration=0)),
+])
+def test_mdbc2_options_are_pinned_by_their_fixtures(name, wrong):
+    """The oracle run without the option a fixture was made with (no
+    penetration off, no-slip instead of free slip) leaves the bounds by the
+    last saved step: each of these fixtures detects its option."""
+    import dataclasses
+    fx, case = load_reference_fixture(name)
+    bad = dataclasses.replace(case, params=dataclasses.replace(case.params, **wrong))
+    o = OracleSim(bad, prec=2)
+    try:
+        o.run(int(fx["steps"][-1]))
+        d, t = o.download(), o.stats()["timestep"]
+    finally:
+        o.close()
+    idx = np.argsort(d["idp"], kind="stable")
+    with pytest.raises(AssertionError):
+        compare_with_reference(fx, len(fx["steps"]) - 1, t, case.pos, d["pos"][idx],
+                               d["velrho"][idx, :3], d["velrho"][idx, 3])
