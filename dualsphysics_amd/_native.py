@@ -117,7 +117,8 @@ class DsphEnsState(C.Structure):
 
 class DsphEnsShape(C.Structure):
     _fields_ = [("variant", C.c_int32)] + [
-        (n, C.c_uint32) for n in ("persistent", "chunk", "waves_per_cu", "bound_block")] + [
+        (n, C.c_uint32) for n in ("persistent", "chunk", "waves_per_cu", "bound_block",
+                                  "bound_tile")] + [
         (n, C.c_float) for n in ("ms_plain", "ms_persistent", "ms_bound")]
 
 

@@ -100,6 +100,12 @@ int ens_forces_fluid(dsph_ctx* ctx, int variant, int tvisco, int tddt, bool eos7
                      const float4* poscell,
                      const float4* posf, const float4* velrho, const float2* press,
                      float* viscdt, float* ar, float* ace);
+// Tile form of the boundary interaction (one wave per 64 boundary particles:
+// bmap64 has one entry per 64; posf/press from ens_prep).
+int ens_forces_bound_tile(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st,
+                          const EnsBlock* bmap64, unsigned nblocks, const unsigned* dcell,
+                          const float4* poscell, const float4* posf, const float4* velrho,
+                          const float2* press, float* viscdt, float* ar);
 // bmap: one entry per block of bs boundary particles (bs = 64, 128, 192 or 256).
 int ens_forces_bound(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st,
                      const EnsBlock* bmap, unsigned nblocks, unsigned bs, const unsigned* dcell,

@@ -385,6 +385,22 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
       nopenshift);
 }
 
+// Boundary particle against one fluid neighbour that passed the neighbour
+// test (KerInteractionForcesBound, JSphGpu_ker.cu:450-493): density rate and
+// the viscous dt term. Shared by the one-lane-per-particle and the tile
+// boundary kernels (same expressions, same bits).
+__device__ __forceinline__ void bound_pair(const float4& vr1, const float3& dr, float rr2,
+                                           const float4& vr2, float massp2, const Cte& c,
+                                           float& arp1, float& visc) {
+  const float fac = wendland_fac(rr2, c);
+  const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
+  const float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
+  arp1 += massp2 * (dvx * frx + dvy * fry + dvz * frz) * fast_div(vr1.w, vr2.w);
+  const float dot = dr.x * dvx + dr.y * dvy + dr.z * dvz;
+  const float dot_rr2 = fast_div(dot, rr2 + c.eta2);
+  visc = fmaxf(dot_rr2, visc);
+}
+
 // ------------------------------------------------------------ cell-row tile kernel
 constexpr unsigned kTile = 64;   // candidates staged in LDS per tile (one bit each)
 constexpr unsigned kMinBatch = 16;
@@ -500,8 +516,11 @@ struct TileLds {
   unsigned tbase[kRing];
 };
 
+// BP1: the boundary kernel's form (KerInteractionForcesBound): p1 is a
+// boundary particle, only fluid cells are scanned, bound_pair is the pair
+// body and ar/viscdt the outputs (no neighbour pressure is needed).
 template <int TVISCO, int TDDT, bool VELGRAD, unsigned kRing, bool MDBC2 = false,
-          bool SHIFT = false, bool EOS7 = false>
+          bool SHIFT = false, bool EOS7 = false, bool BP1 = false>
 __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned pinit,
     float viscob, float viscof, const DivData& dvd, const Cte& c, float ks2scan,
     const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
@@ -566,10 +585,16 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
     const unsigned jj = j & 0x7fffffffu;
     ps2 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(poscell) + (jj << 4));
     vr2 = *reinterpret_cast<const float4*>(reinterpret_cast<const char*>(velrho) + (jj << 4));
-    if (!EOS7)
+    if (!EOS7 && !BP1)
       pr2 = *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(press) + (jj << 3));
   };
   auto eval = [&](unsigned j, const float4& ps2, const float4& vr2, const float2& pr2) {
+    if constexpr (BP1) {
+      const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
+      const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
+      if (rr2 <= c.ks2 && rr2 >= kAlmostZero) bound_pair(q.vr, dr, rr2, vr2, c.massf, c, a.ar, a.visc);
+      return;
+    }
     const float2 pr =
         EOS7 ? make_float2(c.cteb * (pow7_f64(vr2.w * c.ovrhopzero) - 1.0f), fast_rcp(vr2.w))
              : pr2;
@@ -624,7 +649,7 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
     const float qm = sqrtf(uniform_f(wave_max_f32(member ? s1 : 0.f))) + 1.01f * sqrtf(ks2scan);
     const float thr = ks2scan - s1 + 2.0e-6f * qm * qm;
     const float m2x = -2.f * q1x, m2y = -2.f * q1y, m2z = -2.f * q1z;
-    for (int kind = 0; kind < 2; kind++) {
+    for (int kind = 0; kind < (BP1 ? 1 : 2); kind++) {
       const int base = kind == 0 ? int(dvd.cellfluid) : 0;
       const unsigned kbit = unsigned(kind) << 31;
       for (int z = Z0; z < Z1; z++)
@@ -681,7 +706,12 @@ __device__ __forceinline__ void fluid_tile_body(unsigned p, bool valid, unsigned
     }
   }
   process(wave_max_u32(len), Some());
-  if (valid) {
+  if (BP1) {
+    if (valid && (a.ar != 0 || a.visc != 0)) {
+      ar[p1] += a.ar;
+      if (a.visc > viscdt[p1]) viscdt[p1] = a.visc;
+    }
+  } else if (valid) {
     store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
     if (SHIFT) shiftposfs[p1] = a.sh;
   }
@@ -742,19 +772,13 @@ __device__ __forceinline__ void bound_body(unsigned p1, const DivData& dvd, cons
         const float3 dr = pscel_dr(ps1, ps2, c.pcs);
         const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
         if (rr2 <= c.ks2 && rr2 >= kAlmostZero) {
-          const float fac = wendland_fac(rr2, c);
-          const float frx = fac * dr.x, fry = fac * dr.y, frz = fac * dr.z;
           const float4 vr2 = velrho[p2];
-          const float dvx = vr1.x - vr2.x, dvy = vr1.y - vr2.y, dvz = vr1.z - vr2.z;
           float massp2 = c.massf;
           if (FT) {
             const unsigned cd = code[p2];
             if (code_type(cd) == kCodeTypeFloating) massp2 = ftomassp[cd & kCodeMaskValue];
           }
-          arp1 += massp2 * (dvx * frx + dvy * fry + dvz * frz) * fast_div(vr1.w, vr2.w);
-          const float dot = dr.x * dvx + dr.y * dvy + dr.z * dvz;
-          const float dot_rr2 = fast_div(dot, rr2 + c.eta2);
-          visc = fmaxf(dot_rr2, visc);
+          bound_pair(vr1, dr, rr2, vr2, massp2, c, arp1, visc);
         }
       }
     }
@@ -772,6 +796,22 @@ __global__ void __launch_bounds__(kBsForces) k_forces_bound(unsigned n, DivData 
   const unsigned p1 = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p1 >= n) return;
   bound_body<FT>(p1, dvd, c, dcell, poscell, velrho, viscdt, ar, code, ftomassp);
+}
+
+// Boundary interaction in the tile form (same pairs, same order, same bits as
+// k_forces_bound): a wave of 64 consecutive boundary particles stages the
+// fluid rows around them in LDS and each lane tests the staged candidates.
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(5, 8)))
+k_forces_bound_tile(unsigned n, DivData dvd, Cte c, float ks2scan,
+    const unsigned* __restrict__ dcell, const float4* __restrict__ poscell,
+    const float4* __restrict__ posf, const float4* __restrict__ velrho,
+    const float2* __restrict__ press, float* __restrict__ viscdt, float* __restrict__ ar) {
+  __shared__ TileLds<kRingTiles> lds;
+  const unsigned p = xcd_block(blockIdx.x, gridDim.x) * 64 + threadIdx.x;
+  fluid_tile_body<DSPH_VISCO_ARTIFICIAL, DSPH_DDT_NONE, false, kRingTiles, false, false, true,
+                  true>(p, p < n, 0, 0.f, 0.f, dvd, c, ks2scan, dcell, poscell, posf, velrho,
+                        press, nullptr, viscdt, ar, nullptr, nullptr, nullptr, lds);
 }
 
 // Radius^2 of the conservative scan test: (2h + slack)^2 where the slack
@@ -994,6 +1034,27 @@ __global__ void __launch_bounds__(kEnsBsBoundMax) k_ens_bound(const EnsCase* __r
   bound_body<false>(ec.p0 + l, ec.dvd, ec.c, dcell, poscell, velrho, viscdt, ar);
 }
 
+// The boundary interaction of an ensemble in the tile form (k_forces_bound_tile
+// per case): bmap has one entry per 64 boundary particles.
+__global__ void __launch_bounds__(64)
+__attribute__((amdgpu_waves_per_eu(5, 8)))
+k_ens_bound_tile(const EnsCase* __restrict__ cases, const EnsState* __restrict__ st,
+    const EnsBlock* __restrict__ bmap, const unsigned* __restrict__ dcell,
+    const float4* __restrict__ poscell, const float4* __restrict__ posf,
+    const float4* __restrict__ velrho, const float2* __restrict__ press,
+    float* __restrict__ viscdt, float* __restrict__ ar) {
+  __shared__ TileLds<kRingTiles> lds;
+  const EnsBlock e = bmap[xcd_block(blockIdx.x, gridDim.x)];
+  if (!st[e.x].active) return;
+  const unsigned npbok = st[e.x].npbok;
+  if (e.y >= npbok) return;
+  const EnsCase& ec = cases[e.x];
+  const unsigned l = e.y + threadIdx.x;
+  fluid_tile_body<DSPH_VISCO_ARTIFICIAL, DSPH_DDT_NONE, false, kRingTiles, false, false, true,
+                  true>(l, l < npbok, ec.p0, 0.f, 0.f, ec.dvd, ec.c, ec.ks2scan, dcell, poscell,
+                        posf, velrho, press, nullptr, viscdt, ar, nullptr, nullptr, nullptr, lds);
+}
+
 template <int TVISCO, int TDDT, bool EOS7>
 void launch_ens_fluid(dsph_ctx* ctx, int variant, const EnsCase* cases, const EnsState* st,
                       const EnsBlock* wmap, unsigned nb, const EnsPersist* pers,
@@ -1107,6 +1168,18 @@ int ens_forces_bound(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st,
   return DSPH_OK;
 }
 
+int ens_forces_bound_tile(dsph_ctx* ctx, const EnsCase* cases, const EnsState* st,
+                          const EnsBlock* bmap64, unsigned nblocks, const unsigned* dcell,
+                          const float4* poscell, const float4* posf, const float4* velrho,
+                          const float2* press, float* viscdt, float* ar) {
+  if (!nblocks) return DSPH_OK;
+  KTimer kt(ctx, "k_forces_bound");
+  k_ens_bound_tile<<<nblocks, 64, 0, ctx->stream>>>(cases, st, bmap64, dcell, poscell, posf,
+                                                    velrho, press, viscdt, ar);
+  DSPH_LAUNCHED(ctx, "k_ens_bound_tile");
+  return DSPH_OK;
+}
+
 // Radius^2 of the conservative scan test for a case (see scan_radius2).
 float ens_scan_radius2(const dsph_cte& cte) {
   const Cte c = make_cte(cte);
@@ -1197,7 +1270,11 @@ int interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t, float2* press, 
   }
   if (t->npbok) {
     KTimer kt(ctx, "k_forces_bound");
-    if (t->d_ftomassp)
+    if (ctx->forces_kernel == 0 && !t->d_ftomassp && npf)
+      k_forces_bound_tile<<<grid_for(t->npbok, 64), 64, 0, ctx->stream>>>(
+          t->npbok, dvd, c, scan_radius2(ctx, c), t->d_dcell, (const float4*)t->d_poscell, posf,
+          (const float4*)t->d_velrho, press, t->d_viscdt, t->d_ar);
+    else if (t->d_ftomassp)
       k_forces_bound<true><<<grid_for(t->npbok, kBsForces), kBsForces, 0, ctx->stream>>>(
           t->npbok, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
           (const float4*)t->d_velrho, t->d_viscdt, t->d_ar, t->d_code, t->d_ftomassp);

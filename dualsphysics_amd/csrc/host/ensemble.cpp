@@ -20,8 +20,9 @@
 // kernel with a ring of 8, 12 or 16 LDS tiles, one lane per particle rows
 // kernel) on a one-workgroup-per-wave grid, then the best tile variant in its
 // persistent form over chunks of 1, 2 or 4 waves of particles per counter
-// fetch and grids of 10, 20 or 40 waves per CU; and the boundary kernel's
-// block size (64, 128, 256). All shapes give identical results.
+// fetch and grids of 10, 20 or 40 waves per CU; and the boundary kernel:
+// one lane per particle in blocks of 64, 128 or 256, or the tile form (one
+// wave per 64 boundary particles). All shapes give identical results.
 #include <hip/hip_runtime.h>
 #include <memory>
 #include "ensemble.h"
@@ -60,8 +61,8 @@ struct dsph_ens {
   // Boundary block maps, one per candidate block size.
   EnsBlock* d_bmaps[kNumBoundBs] = {nullptr, nullptr, nullptr};
   unsigned nbblocks[kNumBoundBs] = {0, 0, 0};
-  unsigned ibs = 1;  // kBoundBs[ibs] = 128 until tuned
-  float bound_ms[kNumBoundBs] = {0, 0, 0};
+  unsigned ibs = 1;  // kBoundBs[ibs] = 128 until tuned; kNumBoundBs = the tile form
+  float bound_ms[kNumBoundBs + 1] = {0, 0, 0, 0};
   unsigned ncu = 256;
   unsigned* d_ctr = nullptr;  // kNumXcd counters per bucket (persistent fluid kernel)
   struct Bucket {
@@ -359,10 +360,19 @@ struct dsph_ens {
     if (bk.persist) fluid(b, bk.variant, bk.chunk, bk.wpc);
     else fluid(b, bk.variant);
   }
+  // Boundary interaction: s < kNumBoundBs one lane per particle in blocks
+  // of kBoundBs[s]; s == kNumBoundBs the tile form over the 64-blocks.
   void bound(unsigned s) {
-    chk(dsph::ens_forces_bound(ctx, d_cases, d_st, d_bmaps[s], nbblocks[s], kBoundBs[s], dcell,
-                               (const float4*)poscell, (const float4*)velrho, viscdt, ar),
-        "bound interaction");
+    if (s == kNumBoundBs)
+      chk(dsph::ens_forces_bound_tile(ctx, d_cases, d_st, d_bmaps[0], nbblocks[0], dcell,
+                                      (const float4*)poscell, (const float4*)posf,
+                                      (const float4*)velrho, (const float2*)press, viscdt, ar),
+          "bound interaction");
+    else
+      chk(dsph::ens_forces_bound(ctx, d_cases, d_st, d_bmaps[s], nbblocks[s], kBoundBs[s],
+                                 dcell, (const float4*)poscell, (const float4*)velrho, viscdt,
+                                 ar),
+          "bound interaction");
   }
 
   void interaction() {
@@ -532,8 +542,8 @@ struct dsph_ens {
         }
       }
       float bbest = 1e30f;
-      for (unsigned s = 0; s < kNumBoundBs; s++) {
-        if (!nbblocks[s]) continue;
+      for (unsigned s = 0; s <= kNumBoundBs; s++) {
+        if (!nbblocks[s < kNumBoundBs ? s : 0]) continue;
         bound_ms[s] = timed([&] { bound(s); });
         if (bound_ms[s] < bbest) {
           bbest = bound_ms[s];
@@ -688,7 +698,8 @@ int dsph_ens_get_shape(dsph_ens* e, uint32_t bucket, dsph_ens_shape* out) {
   out->persistent = bk.persist ? 1u : 0u;
   out->chunk = bk.chunk;
   out->waves_per_cu = bk.wpc;
-  out->bound_block = kBoundBs[e->ibs];
+  out->bound_block = e->ibs < kNumBoundBs ? kBoundBs[e->ibs] : 64;
+  out->bound_tile = e->ibs == kNumBoundBs ? 1u : 0u;
   float plain = 0;
   for (int v = 0; v < dsph::kEnsNumVariants; v++)
     if (bk.tuned_ms[v] > 0 && (plain == 0 || bk.tuned_ms[v] < plain)) plain = bk.tuned_ms[v];
@@ -711,11 +722,12 @@ int dsph_ens_set_shape(dsph_ens* e, uint32_t bucket, const dsph_ens_shape* shape
              "waves_per_cu in 1..64";
     return DSPH_E_ARG;
   }
-  unsigned s = kNumBoundBs;
+  unsigned s = kNumBoundBs + 1;
   for (unsigned k = 0; k < kNumBoundBs; k++)
     if (kBoundBs[k] == shape->bound_block) s = k;
-  if (s == kNumBoundBs) {
-    e->err = "dsph_ens_set_shape: bound_block must be 64, 128 or 256";
+  if (shape->bound_tile) s = (shape->bound_block == 64 ? kNumBoundBs : kNumBoundBs + 1);
+  if (s > kNumBoundBs) {
+    e->err = "dsph_ens_set_shape: bound_block must be 64, 128 or 256 (64 with bound_tile)";
     return DSPH_E_ARG;
   }
   auto& bk = e->buckets[bucket];

@@ -126,9 +126,10 @@ class Ensemble:
         return d
 
     def set_shape(self, bucket: int, variant: int, persistent: bool = False, chunk: int = 1,
-                  waves_per_cu: int = 20, bound_block: int = 128):
+                  waves_per_cu: int = 20, bound_block: int = 128, bound_tile: bool = False):
         sh = N.DsphEnsShape(variant=variant, persistent=int(persistent), chunk=chunk,
-                            waves_per_cu=waves_per_cu, bound_block=bound_block)
+                            waves_per_cu=waves_per_cu, bound_block=bound_block,
+                            bound_tile=int(bound_tile))
         self._check(self.lib.dsph_ens_set_shape(self._h, bucket, C.byref(sh)))
 
     def info(self) -> dict:

@@ -877,7 +877,8 @@ int dsph_ens_download(dsph_ens* e, uint32_t icase, uint32_t* h_count, uint32_t* 
  * one-lane-per-particle kernel) on a grid of one workgroup per wave of 64
  * particles, then the best tile variant as a persistent launch (chunks of 1,
  * 2 or 4 waves of particles per work-counter fetch, grids of 10, 20 or 40
- * waves per CU), and the boundary kernel's block size (64, 128, 256); it
+ * waves per CU), and the boundary kernel's shape (one lane per particle in
+ * blocks of 64, 128, 256, or the tile form); it
  * keeps the fastest shapes. h_variant[3] (-1 for an empty bucket) and
  * h_ms[3*4] (best time of each variant on the plain grid) may be NULL; the
  * chosen shapes are read with dsph_ens_get_shape. */
@@ -892,6 +893,7 @@ typedef struct dsph_ens_shape {
   uint32_t chunk;           /* persistent: waves of 64 particles per counter fetch (1..64) */
   uint32_t waves_per_cu;    /* persistent: grid = waves_per_cu x compute units (1..64) */
   uint32_t bound_block;     /* boundary kernel block size: 64, 128 or 256 (all buckets) */
+  uint32_t bound_tile;      /* 1: boundary kernel in the tile form (bound_block 64) */
   float ms_plain;           /* autotuner: best plain-grid time of the bucket [ms] (0 untuned) */
   float ms_persistent;      /* autotuner: best persistent time [ms] (0 untuned) */
   float ms_bound;           /* autotuner: boundary kernel time with bound_block [ms] */

@@ -104,12 +104,13 @@ def test_autotuner_changes_speed_not_results():
         b.close()
 
 
-@pytest.mark.parametrize("variant,chunk,wpc,bound_block", [
-    (0, 1, 10, 64),     # about one work item per wave
-    (1, 4, 40, 256),
-    (2, 64, 1, 128),    # a handful of waves draining every XCD segment
+@pytest.mark.parametrize("variant,chunk,wpc,bound_block,bound_tile", [
+    (0, 1, 10, 64, True),     # about one work item per wave; tile boundary kernel
+    (1, 4, 40, 256, False),
+    (2, 64, 1, 128, False),   # a handful of waves draining every XCD segment
 ])
-def test_persistent_launch_changes_speed_not_results(variant, chunk, wpc, bound_block):
+def test_persistent_launch_changes_speed_not_results(variant, chunk, wpc, bound_block,
+                                                     bound_tile):
     """The fluid interaction as one persistent launch per bucket (resident
     waves pulling chunks of work items from per-XCD counters, which the step
     clears with a fill kernel before each launch) inside the 24-step graph,
@@ -120,9 +121,10 @@ def test_persistent_launch_changes_speed_not_results(variant, chunk, wpc, bound_
     try:
         for bucket in range(3):
             a.set_shape(bucket, variant, persistent=True, chunk=chunk, waves_per_cu=wpc,
-                        bound_block=bound_block)
+                        bound_block=bound_block, bound_tile=bound_tile)
             sh = a.shape(bucket)
-            assert sh["persistent"] == 1 and sh["chunk"] == chunk and sh["bound_block"] == bound_block
+            assert sh["persistent"] == 1 and sh["chunk"] == chunk
+            assert sh["bound_block"] == bound_block and sh["bound_tile"] == int(bound_tile)
         assert a.run(max_steps=27, graph=True) == 27   # one 24-step launch + one 3-step launch
         assert b.run(max_steps=27, graph=False) == 27
         assert a.info()["steps_per_graph"] == 24
@@ -141,6 +143,8 @@ def test_shape_arguments_are_checked():
             ens.set_shape(0, 3, persistent=True)
         with pytest.raises(EnsembleError, match="bound_block"):
             ens.set_shape(0, 1, bound_block=96)
+        with pytest.raises(EnsembleError, match="bound_block"):
+            ens.set_shape(0, 1, bound_block=128, bound_tile=True)
         with pytest.raises(EnsembleError, match="tile variant"):
             ens.set_shape(0, 1, persistent=True, chunk=0)
     finally:
