@@ -231,6 +231,9 @@ struct Opts {
   // particle) and the per-particle output (float4); NULL = off.
   const float* boundnor = nullptr;
   float* nopenshift = nullptr;
+  // Particles whose forces are computed: fluid [fbeg, fend) and boundary
+  // [bbeg, bend), clipped to [npb, np) and [0, npbok); the default is all.
+  long fbeg = 0, fend = LONG_MAX, bbeg = 0, bend = LONG_MAX;
 };
 
 // Floating particle test and its body (CODE_IsFloating, CODE_GetTypeValue,
@@ -490,8 +493,10 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
 #pragma omp parallel for schedule(static)
   for (long p = 0; p < long(np); p++) press[p] = eos<R>(o.sem, c, velrho[4 * size_t(p) + 3]);
   // Fluid particles.
+  const long f0 = std::max(long(npb), o.fbeg), f1 = std::min(long(np), o.fend);
+  const long b0 = std::max(0L, o.bbeg), b1 = std::min(long(npbok), o.bend);
 #pragma omp parallel for schedule(guided)
-  for (long pp = long(npb); pp < long(np); pp++) {
+  for (long pp = f0; pp < f1; pp++) {
     const unsigned p1 = unsigned(pp);
     Row rf[25], rb[25];
     const unsigned cel = dcell[p1];
@@ -554,7 +559,7 @@ void interaction(const Opts& o, const orc_cte& c, const orc_div& d, unsigned np,
   const R ks2 = R(c.kernelsize2), eta2 = R(c.eta2), h = R(c.kernelh), bwenh = R(c.bwenh);
   const R massf = R(c.massf);
 #pragma omp parallel for schedule(guided)
-  for (long pp = 0; pp < long(npbok); pp++) {
+  for (long pp = b0; pp < b1; pp++) {
     const unsigned p1 = unsigned(pp);
     Row rf[25];
     const unsigned cel = dcell[p1];
@@ -665,6 +670,13 @@ void orc_interaction_ft(int prec, int sem, int tvisco, int tddt, float viscob, f
                      tangenvel, shiftmode, code, shiftposfs, ftmassp, nullptr, nullptr);
 }
 
+void orc_interaction_range(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                           unsigned, unsigned, unsigned, const unsigned*, const float*,
+                           const double*, const float*, const float*, float*, float*, float*,
+                           float*, float*, const unsigned char*, const float*, int,
+                           const typecode*, float*, const float*, const float*, float*, long,
+                           long, long, long);
+
 // The same with mDBC2 no penetration: boundnor = float3 boundary normals per
 // particle and nopenshift = float4 output per particle (written for the fluid;
 // needs bmode), or NULL.
@@ -676,9 +688,28 @@ void orc_interaction_np(int prec, int sem, int tvisco, int tddt, float viscob, f
                         const unsigned char* bmode, const float* tangenvel, int shiftmode,
                         const typecode* code, float* shiftposfs, const float* ftmassp,
                         const float* boundnor, float* nopenshift) {
+  orc_interaction_range(prec, sem, tvisco, tddt, viscob, viscof, c, d, np, npb, npbok, dcell,
+                        poscell, pos, velrho, tau, ace, ar, viscdt, strain, velgrad, bmode,
+                        tangenvel, shiftmode, code, shiftposfs, ftmassp, boundnor, nopenshift,
+                        0, LONG_MAX, 0, LONG_MAX);
+}
+
+// The same for a subset of the particles: fluid p1 in [fbeg, fend) and
+// boundary p1 in [bbeg, bend) (the rest of the outputs is left alone), so
+// that a large state can be checked on a sample of its particles.
+void orc_interaction_range(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
+                           const orc_cte* c, const orc_div* d, unsigned np, unsigned npb,
+                           unsigned npbok, const unsigned* dcell, const float* poscell,
+                           const double* pos, const float* velrho, const float* tau,
+                           float* ace, float* ar, float* viscdt, float* strain, float* velgrad,
+                           const unsigned char* bmode, const float* tangenvel, int shiftmode,
+                           const typecode* code, float* shiftposfs, const float* ftmassp,
+                           const float* boundnor, float* nopenshift, long fbeg, long fend,
+                           long bbeg, long bend) {
   Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof, bmode, tangenvel,
          shiftposfs ? shiftmode : 0, code, ftmassp && code ? ftmassp : nullptr,
-         nopenshift ? boundnor : nullptr, boundnor ? nopenshift : nullptr};
+         nopenshift ? boundnor : nullptr, boundnor ? nopenshift : nullptr,
+         fbeg, fend, bbeg, bend};
   if (prec == 0)
     interaction<float>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
                        viscdt, strain, velgrad, o.shiftmode ? shiftposfs : nullptr);

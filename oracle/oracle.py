@@ -68,6 +68,9 @@ def load() -> C.CDLL:
             "orc_nopen_step": ([u, u, vp, vp, d, vp, vp], None),
             "orc_interaction_np": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
                                     u, u] + [vp] * 12 + [i, vp, vp, vp, vp, vp], None),
+            "orc_interaction_range": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv),
+                                       u, u, u] + [vp] * 12 + [i, vp, vp, vp, vp, vp]
+                                      + [C.c_long] * 4, None),
             "orc_matrix_rot": ([d, vp, vp, vp], None),
             "orc_interaction_ext": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
                                      u, u] + [vp] * 12 + [i, vp, vp], None),

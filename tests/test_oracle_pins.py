@@ -358,3 +358,49 @@ def test_dt_variable_closed_form():
         got = lib.orc_dt_variable(velmax, acemax, vdt, h, cs0, cfl, 0.0)
         assert math.isclose(got, exp, rel_tol=1e-12)
     assert lib.orc_dt_variable(1e6, 1e12, 1e6, h, cs0, cfl, 1e-5) == 1e-5
+
+
+def test_interaction_range_equals_full_interaction_on_its_particles():
+    """orc_interaction_range (the oracle on a sample of a large state) gives
+    the full interaction's bits on the sampled fluid and boundary particles
+    and leaves the others alone."""
+    import ctypes as C
+    import numpy as np
+    import oracle.oracle as orc
+    from dualsphysics_amd import case as casemod
+    from oracle.oracle import OracleSim
+    c = casemod.dam_break(dp=0.04)
+    o = OracleSim(c)
+    try:
+        od = o.download()
+        dcell, poscell, cte = o.arrays()
+        celldiv, be = o.celldiv()
+        st = o.stats()
+    finally:
+        o.close()
+    n, npb, npbok = st["np"], st["npb"], st["npbok"]
+    c6 = celldiv
+    ncx, ncy, ncz = c6[3] - c6[0] + 1, c6[4] - c6[1] + 1, c6[5] - c6[2] + 1
+    odiv = orc.OrcDiv(1, int(ncx), int(ncy), int(ncz), int(ncx * ncy), int(ncx * ncy * ncz + 1),
+                      int(c6[0]), int(c6[1]), int(c6[2]), be.ctypes.data)
+    pos = np.ascontiguousarray(od["pos"])
+    out = {}
+    for rng in (None, (npb + 100, npb + 900, 50, 300)):
+        a = np.full((n, 3), 7.0, np.float32)
+        ar = np.full(n, 7.0, np.float32)
+        v = np.zeros(n, np.float32)
+        args = (0, orc.SEM_GPU, 1, 3, 0.01, 0.01, C.byref(cte), C.byref(odiv), n, npb, npbok,
+                dcell.ctypes.data, poscell.ctypes.data, pos.ctypes.data,
+                od["velrho"].ctypes.data, None, a.ctypes.data, ar.ctypes.data, v.ctypes.data,
+                None, None, None, None, 0, None, None, None, None, None)
+        r = (0, 2 ** 62, 0, 2 ** 62) if rng is None else rng
+        orc.load().orc_interaction_range(*args, *r)
+        out[rng is None] = (a, ar, v)
+    (fa, far, fv), (sa, sar, sv) = out[True], out[False]
+    fs, bs = slice(npb + 100, npb + 900), slice(50, 300)
+    np.testing.assert_array_equal(sa[fs], fa[fs])
+    np.testing.assert_array_equal(sar[fs], far[fs])
+    np.testing.assert_array_equal(sar[bs], far[bs])
+    np.testing.assert_array_equal(sv[bs], fv[bs])
+    assert np.all(sa[npb + 900:] == 7.0) and np.all(sar[300:npb] == 7.0)
+    assert np.all(sa[npb:npb + 100] == 7.0)
