@@ -22,6 +22,8 @@ SHIFT_NONE, SHIFT_NOBOUND, SHIFT_NOFIXED, SHIFT_FULL = 0, 1, 2, 3
 BMODE_DBC, BMODE_MDBC2, BMODE_MDBC2OFF = 0, 1, 2
 OPT_FORCES_KERNEL = 1
 FORCES_TILE, FORCES_ROWS = 0, 1
+OPT_BOUND_KERNEL = 2
+BOUND_AUTO, BOUND_TILE, BOUND_ROWS = 0, 1, 2
 NP_LIMIT = 1 << 28              # DSPH_NP_LIMIT: particles per simulation / interaction
 NOALLOC_NP_MAX = 1 << 20        # DSPH_NOALLOC_NP_MAX: particles per no-alloc interaction
 

@@ -76,6 +76,8 @@ int dsph_ctx_destroy(dsph_ctx* ctx) {
     hipEventDestroy(it.second.second);
   }
   for (hipEvent_t e : ctx->ev_pool) hipEventDestroy(e);
+  for (hipEvent_t e : ctx->bound_ev)
+    if (e) hipEventDestroy(e);
   hipFree(ctx->sort_buf);
   hipFree(ctx->redu);
   hipFree(ctx->prep_press);

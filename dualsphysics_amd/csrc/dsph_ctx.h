@@ -38,6 +38,14 @@ struct dsph_ctx {
   float* prep_posf = nullptr;      // float4 per particle
   size_t prep_cap = 0;
   int forces_kernel = 0;           // DSPH_OPT_FORCES_KERNEL: 0 queue, 1 rows
+  // DSPH_OPT_BOUND_KERNEL: 0 auto, 1 tile, 2 one lane per particle. Auto
+  // times both forms (identical bits) on interactions 1 and 3 of every
+  // kBoundRetrial (0, 2 are warm-ups) and keeps the faster until the next
+  // trial; bound_choice is the form in use.
+  int bound_kernel = 0;
+  int bound_choice = 1;
+  unsigned long long bound_count = 0;
+  hipEvent_t bound_ev[4] = {nullptr, nullptr, nullptr, nullptr};
   // Device address of the static scratch of dsph_interaction_forces_noalloc
   // on this context's device (set by dsph_ctx_create).
   void* noalloc_scratch = nullptr;

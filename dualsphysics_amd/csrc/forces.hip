@@ -876,9 +876,10 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
         (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
         t->d_sps2strain, t->d_velgrad, t->d_boundmode, t->d_tangenvel);
   } else if (ctx->forces_kernel == 0) {
-    // gamma = 7 without the SPS or gradient outputs: the neighbour pressure is
-    // evaluated in the kernel instead of being gathered.
-    constexpr bool kEos7Built = !VELGRAD && TVISCO != DSPH_VISCO_LAMINARSPS;
+    // gamma = 7 without the gradient output: the neighbour pressure is
+    // evaluated in the kernel instead of being gathered (Laminar+SPS too: 127
+    // registers, no spill, 9 % faster on the wave flume).
+    constexpr bool kEos7Built = !VELGRAD;
     if (kEos7Built && c.gamma == 7.f)
       k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, false, false, kEos7Built>
           <<<grid_for(npf, 64 * kTileWaves), 64 * kTileWaves, 0, ctx->stream>>>(
@@ -1201,6 +1202,13 @@ int dsph_ctx_set_option(dsph_ctx* ctx, int key, int value) {
       DSPH_REQUIRE(ctx, value == 0 || value == 1, "forces kernel must be 0 (tile) or 1 (rows)");
       ctx->forces_kernel = value;
       return DSPH_OK;
+    case DSPH_OPT_BOUND_KERNEL:
+      DSPH_REQUIRE(ctx, value >= 0 && value <= 2,
+                   "boundary kernel must be 0 (auto), 1 (tile) or 2 (rows)");
+      ctx->bound_kernel = value;
+      ctx->bound_count = 0;
+      if (value) ctx->bound_choice = value;
+      return DSPH_OK;
     default: return set_error(ctx, DSPH_E_ARG, "unknown option");
   }
 }
@@ -1235,9 +1243,15 @@ int check_inter_parms(dsph_ctx* ctx, const dsph_inter_parms* t) {
   return DSPH_OK;
 }
 
+// Interactions between two trials of the boundary kernel's forms (auto).
+constexpr unsigned long long kBoundRetrial = 4096;
+
 // The interaction with the per-particle preparation in press/posf (np
-// entries each): k_prep, the fluid kernel, the boundary kernel.
-int interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t, float2* press, float4* posf) {
+// entries each): k_prep, the fluid kernel, the boundary kernel. allow_trial:
+// the boundary kernel's auto choice may time its forms (events, one host
+// synchronisation per trial); without it the current choice runs.
+int interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t, float2* press, float4* posf,
+                       bool allow_trial) {
   const Cte c = make_cte(ctx->cte);
   const DivData dvd = make_divdata(t->divdata);
   const unsigned npf = t->np - t->npb;
@@ -1270,11 +1284,48 @@ int interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t, float2* press, 
   }
   if (t->npbok) {
     KTimer kt(ctx, "k_forces_bound");
-    if (ctx->forces_kernel == 0 && !t->d_ftomassp && npf)
-      k_forces_bound_tile<<<grid_for(t->npbok, 64), 64, 0, ctx->stream>>>(
-          t->npbok, dvd, c, scan_radius2(ctx, c), t->d_dcell, (const float4*)t->d_poscell, posf,
-          (const float4*)t->d_velrho, press, t->d_viscdt, t->d_ar);
-    else if (t->d_ftomassp)
+    if (ctx->forces_kernel == 0 && !t->d_ftomassp && npf) {
+      // Tile or one-lane form (same pairs, same order, same bits): which is
+      // faster depends on how many walls are wet, so by default it is timed.
+      hipEvent_t* ev = nullptr;
+      int form = ctx->bound_kernel ? ctx->bound_kernel : ctx->bound_choice;
+      hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+      if (!ctx->bound_kernel && allow_trial &&
+          hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess &&
+          cs == hipStreamCaptureStatusNone) {
+        // Not while a graph is being captured: the count only advances on
+        // eager interactions, so the events of a trial are always recorded.
+        const unsigned long long k = ctx->bound_count++ % kBoundRetrial;
+        if (k < 5) {
+          if (k < 4) {
+            form = k < 2 ? 1 : 2;
+            if (k == 1 || k == 3) {
+              hipEvent_t* e = ctx->bound_ev + (k == 1 ? 0 : 2);
+              for (int q = 0; q < 2; q++)
+                if (!e[q]) DSPH_HIP(ctx, hipEventCreate(&e[q]));
+              ev = e;
+            }
+          } else {
+            float ms_tile = 0, ms_rows = 0;
+            DSPH_HIP(ctx, hipEventSynchronize(ctx->bound_ev[3]));
+            DSPH_HIP(ctx, hipEventElapsedTime(&ms_tile, ctx->bound_ev[0], ctx->bound_ev[1]));
+            DSPH_HIP(ctx, hipEventElapsedTime(&ms_rows, ctx->bound_ev[2], ctx->bound_ev[3]));
+            ctx->bound_choice = ms_rows < ms_tile ? 2 : 1;
+            form = ctx->bound_choice;
+          }
+        }
+      }
+      if (ev) DSPH_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
+      if (form == 1)
+        k_forces_bound_tile<<<grid_for(t->npbok, 64), 64, 0, ctx->stream>>>(
+            t->npbok, dvd, c, scan_radius2(ctx, c), t->d_dcell, (const float4*)t->d_poscell,
+            posf, (const float4*)t->d_velrho, press, t->d_viscdt, t->d_ar);
+      else
+        k_forces_bound<false><<<grid_for(t->npbok, kBsForces), kBsForces, 0, ctx->stream>>>(
+            t->npbok, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
+            (const float4*)t->d_velrho, t->d_viscdt, t->d_ar);
+      if (ev) DSPH_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+    } else if (t->d_ftomassp)
       k_forces_bound<true><<<grid_for(t->npbok, kBsForces), kBsForces, 0, ctx->stream>>>(
           t->npbok, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
           (const float4*)t->d_velrho, t->d_viscdt, t->d_ar, t->d_code, t->d_ftomassp);
@@ -1333,7 +1384,7 @@ int dsph_interaction_forces(dsph_ctx* ctx, const dsph_inter_parms* t) {
     if (rc) return rc;
   }
   return interaction_forces(ctx, t, reinterpret_cast<float2*>(ctx->prep_press),
-                            reinterpret_cast<float4*>(ctx->prep_posf));
+                            reinterpret_cast<float4*>(ctx->prep_posf), true);
 }
 
 int dsph_interaction_forces_noalloc(dsph_ctx* ctx, const dsph_inter_parms* t) {
@@ -1348,7 +1399,7 @@ int dsph_interaction_forces_noalloc(dsph_ctx* ctx, const dsph_inter_parms* t) {
     return set_error(ctx, DSPH_E_STATE, "no-alloc scratch address not set up for this device");
   unsigned char* base = static_cast<unsigned char*>(ctx->noalloc_scratch);
   return interaction_forces(ctx, t, reinterpret_cast<float2*>(base),
-                            reinterpret_cast<float4*>(base + kNoallocPressBytes));
+                            reinterpret_cast<float4*>(base + kNoallocPressBytes), false);
 }
 
 }  // extern "C"

@@ -209,6 +209,13 @@ int dsph_ctx_timing_reset(dsph_ctx* ctx);
                                      staged in LDS per wave with per-lane pair
                                      queues (default), 1 = one lane per
                                      particle over its cell rows */
+#define DSPH_OPT_BOUND_KERNEL  2  /* boundary interaction kernel (with
+                                     DSPH_OPT_FORCES_KERNEL 0): 0 = auto (the
+                                     default: times both forms on interactions
+                                     1 and 3 of every 4096 and keeps the faster;
+                                     one host synchronisation per trial), 1 =
+                                     staged rows (tile), 2 = one lane per
+                                     particle. Identical bits either way. */
 int dsph_ctx_set_option(dsph_ctx* ctx, int key, int value);
 
 /* Replaces cusph::CteInteractionUp (JSphGpu_ker.h:284). Copies into ctx. */

@@ -168,3 +168,43 @@ def test_sps_tau(lib, ctx):
     ot = np.zeros((n, 6), np.float32)
     orc.load().orc_sps_tau(n, npb, smag, blin, vr.ctypes.data, s2.ctypes.data, ot.ctypes.data)
     assert rel_err(tau.get()[npb:], ot[npb:]) < 1e-6
+
+
+def test_boundary_kernel_forms_equal_bitwise(lib, ctx):
+    """The boundary interaction's tile form and its one-lane-per-particle
+    form give the same bits, and so does the automatic choice between them
+    over the interactions in which it times both (DSPH_OPT_BOUND_KERNEL)."""
+    c = perturbed_case(casemod.dam_break(dp=0.01), seed=4)
+    o, d, dcell, poscell, cte, celldiv, be = _prepared(c)
+    st = o.stats()
+    n, npb, npbok = st["np"], st["npb"], st["npbok"]
+    bc = Dev(lib, ctx, be)
+    dv = divdata(celldiv, bc, cte)
+    assert lib.dsph_set_constants(ctx, C.byref(to_cte(cte))) == 0
+    gd = {k: Dev(lib, ctx, a) for k, a in (("dcell", dcell), ("poscell", poscell),
+                                          ("velrho", d["velrho"]), ("code", d["code"]))}
+    outs = []
+    try:
+        for form, reps in ((N.BOUND_TILE, 1), (N.BOUND_ROWS, 1), (N.BOUND_AUTO, 7)):
+            assert lib.dsph_ctx_set_option(ctx, N.OPT_BOUND_KERNEL, form) == 0
+            for _ in range(reps):
+                ace = Dev.zeros(lib, ctx, (n, 3), np.float32)
+                ar = Dev.zeros(lib, ctx, n, np.float32)
+                vdt = Dev.zeros(lib, ctx, n, np.float32)
+                t = N.DsphInterParms()
+                t.tkernel, t.tvisco, t.tdensity = N.KERNEL_WENDLAND, 1, 3
+                t.viscob = t.viscof = 0.01
+                t.np, t.npb, t.npbok = n, npb, npbok
+                t.divdata = dv
+                t.d_dcell, t.d_poscell = gd["dcell"].ptr, gd["poscell"].ptr
+                t.d_velrho, t.d_code = gd["velrho"].ptr, gd["code"].ptr
+                t.d_viscdt, t.d_ar, t.d_ace = vdt.ptr, ar.ptr, ace.ptr
+                assert lib.dsph_interaction_forces(ctx, C.byref(t)) == 0, lib.dsph_last_error(ctx)
+                outs.append((ar.get(), vdt.get()))
+    finally:
+        lib.dsph_ctx_set_option(ctx, N.OPT_BOUND_KERNEL, N.BOUND_AUTO)
+    assert np.abs(outs[0][0][:npbok]).max() > 0
+    for a, v in outs[1:]:
+        np.testing.assert_array_equal(a.view(np.uint32), outs[0][0].view(np.uint32))
+        np.testing.assert_array_equal(v.view(np.uint32), outs[0][1].view(np.uint32))
+    assert lib.dsph_ctx_set_option(ctx, N.OPT_BOUND_KERNEL, 3) != 0
