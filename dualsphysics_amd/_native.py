@@ -39,7 +39,7 @@ class DsphCte(C.Structure):
         (n, C.c_double) for n in (
             "domposminx", "domposminy", "domposminz", "maprealposminx", "maprealposminy",
             "maprealposminz", "maprealsizex", "maprealsizey", "maprealsizez")] + [
-        ("periactive", C.c_uint32), ("pad_peri", C.c_uint32),
+        ("periactive", C.c_uint32), ("simulate2d", C.c_uint32),
         ("xperinc", C.c_double * 3), ("yperinc", C.c_double * 3), ("zperinc", C.c_double * 3)]
 
 
@@ -102,7 +102,8 @@ class DsphCase(C.Structure):
         ("shiftmode", C.c_int32), ("shiftcoef", C.c_double), ("shifttfs", C.c_double),
         ("periactive", C.c_int32), ("perinc", (C.c_double * 3) * 3),
         ("ftcount", C.c_uint32), ("nopenetration", C.c_int32),
-        ("floatings", C.POINTER(DsphFloating))]
+        ("floatings", C.POINTER(DsphFloating)),
+        ("simulate2d", C.c_int32), ("pad_2d", C.c_int32), ("simulate2dposy", C.c_double)]
 
 
 class DsphMlp(C.Structure):

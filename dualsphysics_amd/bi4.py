@@ -209,7 +209,8 @@ def write_case(case, directory, name: str, timemax: float | None = None,
     root.set("Piece", T_UINT, 0).set("Npiece", T_UINT, 1)
     root.set("RunCode", T_TEXT, "00000000").set("Date", T_TEXT, "???")
     root.set("AppName", T_TEXT, "dualsphysics_amd case writer").set("CaseName", T_TEXT, name)
-    root.set("Data2d", T_BOOL, False).set("Data2dPosY", T_DOUBLE, 0.0)
+    root.set("Data2d", T_BOOL, bool(case.data2d))
+    root.set("Data2dPosY", T_DOUBLE, float(case.data2dposy) if case.data2d else 0.0)
     root.set("MapPosMin", T_DOUBLE3, (0.0, 0.0, 0.0)).set("MapPosMax", T_DOUBLE3, (0.0, 0.0, 0.0))
     root.set("PeriMode", T_INT, 0)
     for k in ("PeriXinc", "PeriYinc", "PeriZinc"):
@@ -323,11 +324,13 @@ def write_case(case, directory, name: str, timemax: float | None = None,
 """
     plines = "\n".join(f'            <parameter key="{k}" value="{r(v)}" />'
                        for k, v in params.items())
+    d2 = (f'<data2d value="true" />\n            <data2dposy value="{r(case.data2dposy)}" />'
+          if case.data2d else '<data2d value="false" />')
     xml = f"""<?xml version="1.0" encoding="UTF-8" ?>
 <case app="dualsphysics_amd case writer" date="???">
     <execution>
         <constants>
-            <data2d value="false" />
+            {d2}
             <gravity x="{r(g[0])}" y="{r(g[1])}" z="{r(g[2])}" />
             <cflnumber value="{r(p.cflnumber)}" />
             <gamma value="{r(case.gamma)}" />
@@ -460,7 +463,8 @@ def _root_header(case, name: str, app: str = "dualsphysics_amd") -> "Item":
     root.set("Piece", T_UINT, 0).set("Npiece", T_UINT, 1)
     root.set("RunCode", T_TEXT, "00000000").set("Date", T_TEXT, "???")
     root.set("AppName", T_TEXT, app).set("CaseName", T_TEXT, name)
-    root.set("Data2d", T_BOOL, False).set("Data2dPosY", T_DOUBLE, 0.0)
+    root.set("Data2d", T_BOOL, bool(case.data2d))
+    root.set("Data2dPosY", T_DOUBLE, float(case.data2dposy) if case.data2d else 0.0)
     root.set("MapPosMin", T_DOUBLE3, tuple(map(float, case.maprealposmin)))
     root.set("MapPosMax", T_DOUBLE3, tuple(map(float, case.maprealposmax)))
     _set_periodic(root, case)
@@ -517,8 +521,9 @@ def write_part_head(path, case, app: str = "dualsphysics_amd"):
     root = Item("JPartDataHead")
     root.set("FmtVersion", T_UINT, 180324).set("AppName", T_TEXT, app)
     root.set("Date", T_TEXT, "???").set("RunCode", T_TEXT, "00000000")
-    root.set("CaseName", T_TEXT, case.name).set("Data2d", T_BOOL, False)
-    root.set("Data2dPosY", T_DOUBLE, 0.0).set("Npiece", T_UINT, 1).set("FirstPart", T_UINT, 0)
+    root.set("CaseName", T_TEXT, case.name).set("Data2d", T_BOOL, bool(case.data2d))
+    root.set("Data2dPosY", T_DOUBLE, float(case.data2dposy) if case.data2d else 0.0)
+    root.set("Npiece", T_UINT, 1).set("FirstPart", T_UINT, 0)
     root.set("CasePosMin", T_DOUBLE3, tuple(map(float, pmin)))
     root.set("CasePosMax", T_DOUBLE3, tuple(map(float, pmax)))
     root.set("NpDynamic", T_BOOL, False).set("ReuseIds", T_BOOL, False)
@@ -592,7 +597,9 @@ def read_case(directory, name: str):
     g = ct.find("gravity")
     gravity = (float(g.get("x")), float(g.get("y")), float(g.get("z")))
     if ct.find("data2d") is not None and ct.find("data2d").get("value") in ("true", "1"):
-        raise ValueError("2-D cases are not built")
+        data2d, posy = True, float(ct.find("data2dposy").get("value"))
+    else:
+        data2d, posy = False, 0.0
     dp, h, b = val("dp"), val("h"), val("b")
     rhop0, gamma = val("rhop0"), val("gamma")
     massb, massf = val("massbound"), val("massfluid")
@@ -740,4 +747,5 @@ def read_case(directory, name: str):
                    gravity=gravity, hswl=0.0, params=p, idp=idp, pos=pos, velrho=velrho,
                    nbound=nb, nfluid=int(idp.shape[0]) - nb, maprealposmin=tuple(map(float, mn)),
                    maprealposmax=tuple(map(float, mx)), boundnor=boundnor, nmoving=nmov,
-                   motion=motion, floatings=floatings or None)
+                   motion=motion, floatings=floatings or None, data2d=data2d,
+                   data2dposy=posy)

@@ -122,6 +122,8 @@ class Case:
     nmoving: int = 0                     # moving boundary ids [npb-nmoving, npb)
     motion: dict | None = None           # rectilinear sinusoid: freq, ampl, phase (3 each), start, finish
     floatings: list | None = None        # Floating bodies: ids [npb, nbound)
+    data2d: bool = False                 # 2-D case: every particle at y = data2dposy
+    data2dposy: float = 0.0
 
     @property
     def np(self) -> int:
@@ -161,6 +163,7 @@ class Case:
         for a in range(3):
             for k in range(3):
                 c.perinc[a][k] = float(p.perinc[a][k]) if a != k else 0.0
+        c.simulate2d, c.simulate2dposy = int(bool(self.data2d)), float(self.data2dposy)
         c.casenp, c.casenbound = self.np, self.nbound
         c.casenfixed = self.npb - self.nmoving
         c.casenmoving, c.casenfloat, c.casenfluid = self.nmoving, self.nfloat, self.nfluid
@@ -289,6 +292,55 @@ def dam_break(dp: float = 0.01, tank=(1.6, 0.67, 0.4), column=(0.4, 0.67, 0.3), 
                 cs0=cs0, b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g),
                 hswl=hswl, params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb,
                 nfluid=nf, maprealposmin=mn, maprealposmax=mx, boundnor=boundnor)
+
+
+def dam_break_2d(dp: float = 0.01, tank=(1.6, 0.4), column=(0.4, 0.3), layers: int = 2,
+                 coefh: float = 1.2, coefsound: float = 20.0, rhop0: float = 1000.0,
+                 gamma: float = 7.0, g: float = 9.81, seed: int = 42, jitter: float = 1e-4,
+                 posy: float = 0.0, params: Parameters | None = None, name: str | None = None,
+                 vel0=(0.0, 0.0)) -> Case:
+    """2-D dam break (the x-z section of :func:`dam_break`): every particle
+    in the plane y = ``posy``; walls of ``layers`` lattice rows around the
+    tank interior [0,L]x[0,inf), the column ``column`` at the x=0 end,
+    starting with velocity ``vel0`` (x, z). 2-D constants as GenCase sets
+    them for <data2d>: h = coefh*sqrt(2 dp^2) (JSph.cpp:1667), particle mass
+    rho0*dp^2. The jitter moves x and z only."""
+    params = params or Parameters()
+    nxt, nzt = _n(tank[0], dp), _n(tank[1], dp)
+    nxf, nzf = _n(column[0], dp), _n(column[1], dp)
+    k, i = np.meshgrid(np.arange(-layers, nzt), np.arange(-layers, nxt + layers), indexing="ij")
+    i, k = i.ravel(), k.ravel()
+    wall = ~((i >= 0) & (i < nxt) & (k >= 0))
+    wpts = np.stack([(i[wall] + 0.5) * dp, np.full(int(wall.sum()), posy),
+                     (k[wall] + 0.5) * dp], 1)
+    kf, jf = np.meshgrid(np.arange(nzf), np.arange(nxf), indexing="ij")
+    fpts = np.stack([(jf.ravel() + 0.5) * dp, np.full(kf.size, posy), (kf.ravel() + 0.5) * dp], 1)
+    nb, nf = wpts.shape[0], fpts.shape[0]
+    pos = np.concatenate([wpts, fpts], 0)
+    u = uniform01(seed, 2 * pos.shape[0]).reshape(-1, 2)
+    pos[:, 0] += (2.0 * u[:, 0] - 1.0) * (jitter * dp)
+    pos[:, 2] += (2.0 * u[:, 1] - 1.0) * (jitter * dp)
+    h = coefh * math.sqrt(2.0 * dp * dp)
+    hswl = column[1]
+    cs0 = coefsound * math.sqrt(g * hswl)
+    b = cs0 * cs0 * rhop0 / gamma
+    mass = rhop0 * dp ** 2
+    velrho = np.zeros((pos.shape[0], 4), np.float32)
+    velrho[:nb, 3] = rhop0
+    zf = pos[nb:, 2]
+    velrho[nb:, 3] = (rhop0 * (1.0 + rhop0 * g * (hswl - zf) / b) ** (1.0 / gamma)).astype(np.float32)
+    velrho[nb:, 0] = np.float32(vel0[0])
+    velrho[nb:, 2] = np.float32(vel0[1])
+    bp = pos[:nb]
+    q = np.stack([np.clip(bp[:, 0], 0.0, nxt * dp), bp[:, 1], np.maximum(bp[:, 2], 0.0)], 1)
+    boundnor = (q - bp).astype(np.float32)
+    idp = np.arange(pos.shape[0], dtype=np.uint32)
+    mn, mx = map_limits(pos, h)
+    return Case(name=name or f"dambreak2d_dp{dp:g}", dp=dp, h=h, rhop0=rhop0, gamma=gamma,
+                cs0=cs0, b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g),
+                hswl=hswl, params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb,
+                nfluid=nf, maprealposmin=mn, maprealposmax=mx, boundnor=boundnor,
+                data2d=True, data2dposy=posy)
 
 
 def rigid_body(pos: np.ndarray, masspart: float):

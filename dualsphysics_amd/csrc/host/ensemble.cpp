@@ -142,6 +142,7 @@ struct dsph_ens {
       if (cs.tboundary != DSPH_BC_DBC) throw Err(DSPH_E_ARG, "ensembles run DBC boundaries");
       if (cs.shiftmode != DSPH_SHIFT_NONE) throw Err(DSPH_E_ARG, "ensembles run without shifting");
       if (cs.periactive) throw Err(DSPH_E_ARG, "ensembles run without periodic boundaries");
+      if (cs.simulate2d) throw Err(DSPH_E_ARG, "ensembles run 3-D cases");
       if (cs.casenmoving || cs.casenfloat) throw Err(DSPH_E_ARG, "ensembles have fixed boundaries");
       if (k == 0) { tvisco = cs.tvisco; tddt = cs.tdensity; }
       eos7 = eos7 && float(cs.gamma) == 7.f;

@@ -77,6 +77,8 @@ struct dsph_sim {
   int TStep = 2, VerletSteps = 40, TVisco = 1, TDensity = 0, TBoundary = 1, ScellDiv = 1;
   int SlipMode = DSPH_SLIP_VEL0;
   bool NoPen = false;  // mDBC2 NoPenetration (TMdbc2 == MDBC2_NoPen)
+  bool Simulate2D = false;  // particles in the plane y = Simulate2DPosY (JSph.cpp:646-647)
+  double Simulate2DPosY = 0;
   // mDBC2: no-slip or free slip (MdbcCorrector, JSph.cpp:897).
   bool mdbc2_on() const { return TBoundary == DSPH_BC_MDBC && SlipMode >= DSPH_SLIP_NOSLIP; }
   int ShiftMode = DSPH_SHIFT_NONE;
@@ -228,15 +230,32 @@ struct dsph_sim {
     ScellDiv = (c.cellmode == 2 ? 2 : 1);
     CellDomFixed = c.celldomfixed != 0;
     if (c.tkernel != DSPH_KERNEL_WENDLAND) throw Err(DSPH_E_ARG, "only Wendland kernel is built");
+    // 2-D: every particle lies in the plane y = Simulate2DPosY, so every pair
+    // distance has dy = 0 and the y row of each pair term is a product with
+    // 0; the y component of ace therefore stays 0 (the reference zeroes it,
+    // JSphGpuSingle.cpp:646, and the fluid's y velocity stays 0 as gravity.y
+    // is the only other source). Linear motions lose their y part
+    // (JDsMotion.cpp:132).
+    Simulate2D = c.simulate2d != 0;
+    Simulate2DPosY = c.simulate2dposy;
+    if (Simulate2D) {
+      if (c.casenfloat || c.periactive || (c.shiftmode && c.shifttfs > 0) || c.motion.type >= 2)
+        throw Err(DSPH_E_ARG, "2-D runs are built without floating bodies, periodic axes, the "
+                              "shifting free-surface threshold and rotation motions");
+      cs.motion.ampl[1] = 0;
+    }
     const double h = KernelH;
     KernelSize = float(h * 2.0);
     KernelSize2 = KernelSize * KernelSize;
-    awen = float(0.41778 / (h * h * h));
-    bwenh = float(-2.08891 / (h * h * h * h * h));
+    // Wendland constants, 2-D or 3-D (FunSphKernel.h:198-209).
+    awen = Simulate2D ? float(0.557 / (h * h)) : float(0.41778 / (h * h * h));
+    bwenh = Simulate2D ? float(-2.7852 / (h * h * h * h)) : float(-2.08891 / (h * h * h * h * h));
     Cs0 = std::sqrt(double(Gamma) * double(CteB) / double(RhopZero));
     Eta2 = float((h * 0.1) * (h * 0.1));
     if (TVisco == DSPH_VISCO_LAMINARSPS) {
-      const double dp_sps = std::sqrt(double(Dp) * double(Dp) * 3.) / 3.;
+      // dp_sps (JSph.cpp:1529).
+      const double dp_sps = Simulate2D ? std::sqrt(double(Dp) * double(Dp) * 2.) / 2.
+                                       : std::sqrt(double(Dp) * double(Dp) * 3.) / 3.;
       SpsSmag = float((2. / 3.) * std::pow(0.12 * dp_sps, 2));
       SpsBlin = float((2. / 3.) * 0.0066 * dp_sps * dp_sps);
     }
@@ -339,6 +358,7 @@ struct dsph_sim {
     c.maprealposminz = MapRealPosMin[2];
     c.maprealsizex = MapRealSize[0]; c.maprealsizey = MapRealSize[1]; c.maprealsizez = MapRealSize[2];
     c.periactive = PeriActive;
+    c.simulate2d = Simulate2D ? 1u : 0u;
     for (int k = 0; k < 3; k++) {
       c.xperinc[k] = PeriInc[0][k]; c.yperinc[k] = PeriInc[1][k]; c.zperinc[k] = PeriInc[2][k];
     }

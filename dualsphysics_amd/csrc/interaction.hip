@@ -80,6 +80,45 @@ __device__ inline double det4_inv(const M4d& a, M4d& inv) {
   return det;
 }
 
+// 2-D: the y row and column of the moment sums vanish (all dy = 0) and the
+// system is the 3x3 one over (1, x, z) of the reference's 2-D branches
+// (Determinant3x3dbl by the rule of Sarrus and InverseMatrix3x3dbl by 2x2
+// cofactors, FunctionsMath_iker.h:111-114 and 150-165; FP64 from the FP32
+// sums); its inverse is returned in the 4x4 positions, the y row and column 0.
+__device__ inline double mdbc_det_inv(bool sim2d, const M4d& a, M4d& inv) {
+  if (!sim2d) return det4_inv(a, inv);
+  const int ix[3] = {0, 1, 3};
+  double m[3][3];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) m[i][j] = a.m[ix[i]][ix[j]];
+  const double det = m[0][0] * m[1][1] * m[2][2] + m[0][1] * m[1][2] * m[2][0] +
+                     m[0][2] * m[1][0] * m[2][1] - m[2][0] * m[1][1] * m[0][2] -
+                     m[2][1] * m[1][2] * m[0][0] - m[2][2] * m[1][0] * m[0][1];
+  double r[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  if (det != 0) {
+    r[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det;
+    r[0][1] = -(m[0][1] * m[2][2] - m[0][2] * m[2][1]) / det;
+    r[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
+    r[1][0] = -(m[1][0] * m[2][2] - m[1][2] * m[2][0]) / det;
+    r[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
+    r[1][2] = -(m[0][0] * m[1][2] - m[0][2] * m[1][0]) / det;
+    r[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det;
+    r[2][1] = -(m[0][0] * m[2][1] - m[0][1] * m[2][0]) / det;
+    r[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
+  }
+#pragma unroll
+  for (int i = 0; i < 4; i++)
+#pragma unroll
+    for (int j = 0; j < 4; j++) inv.m[i][j] = 0;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) inv.m[ix[i]][ix[j]] = r[i][j];
+  return det;
+}
+
 // mDBC, SlipMode SLIP_Vel0 (KerInteractionMdbcCorrection_Fast,
 // JSphGpu_mdbc_iker.cu:34-150): Shepard density and gradient at the ghost node
 // g = x_b + boundnor with a first-order (4x4) correction; mirrored back to
@@ -145,7 +184,7 @@ __global__ void __launch_bounds__(kBsForces) k_mdbc(unsigned n, DivData dvd, Cte
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int j = 0; j < 4; j++) Ad.m[i][j] = double(A[i][j]);
-  const double det = det4_inv(Ad, inv);
+  const double det = mdbc_det_inv(c.sim2d != 0, Ad, inv);
   if (fabs(det) >= 1e-3) {
     float fi[4][4];
 #pragma unroll
@@ -265,7 +304,7 @@ __global__ void __launch_bounds__(kBsForces) k_mdbc2(unsigned n, int tslip, DivD
     for (int i = 0; i < 4; i++)
 #pragma unroll
       for (int j = 0; j < 4; j++) Ad.m[i][j] = double(A[i][j]);
-    const double det = det4_inv(Ad, inv);
+    const double det = mdbc_det_inv(c.sim2d != 0, Ad, inv);
     if (fabs(det) >= 0.001) {
       float fi[4][4];
 #pragma unroll

@@ -11,6 +11,7 @@ struct Cte {
       cteb, gamma, ovrhopzero, dp;
   unsigned cellcode;
   unsigned tboundary;  // DSPH_BC_*: the Molteni-Colagrossi term skips boundaries only with DBC
+  unsigned sim2d;      // 2-D simulation: the mDBC systems are 3x3 over (1, x, z)
   // Binomial-series coefficients of (1+u)^(1/gamma), k = 1..5.
   float pc1, pc2, pc3, pc4, pc5;
 };
@@ -23,6 +24,7 @@ inline Cte make_cte(const dsph_cte& c) {
   r.cteb = c.cteb; r.gamma = c.gamma; r.ovrhopzero = c.ovrhopzero; r.dp = c.dp;
   r.cellcode = c.cellcode;
   r.tboundary = c.tboundary;
+  r.sim2d = c.simulate2d;
   // binom(a,k) = binom(a,k-1) * (a-k+1)/k with a = invgamma (as the reference's 1.f/gamma).
   const double a = double(r.invgamma);
   double b = 1.0;

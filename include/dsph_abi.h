@@ -111,7 +111,9 @@ typedef struct dsph_cte {
   /* Periodic boundaries (JSph.cpp:810-826, 2183-2194): bits 1 X, 2 Y, 4 Z;
      the increment applied to a particle leaving through the upper face of
      each periodic axis (PeriXinc, PeriYinc, PeriZinc). */
-  uint32_t periactive, pad_peri;
+  uint32_t periactive;
+  uint32_t simulate2d;           /* 2-D simulation (Simulate2D, JSph.cpp:646): the
+                                    mDBC systems drop the y row and column */
   double xperinc[3], yperinc[3], zperinc[3];
 } dsph_cte;
 
@@ -752,6 +754,11 @@ typedef struct dsph_case {
   uint32_t ftcount;              /* floating bodies (casenfloat particles in all) */
   int32_t nopenetration;         /* mDBC2 NoPenetration (JSph.cpp:715-716): 1 = on */
   const dsph_floating* floatings; /* ftcount entries, in id order; read at creation */
+  int32_t simulate2d;            /* <data2d> (JCaseCtes.cpp:209-211): particles in the
+                                    plane y = simulate2dposy, 2-D Wendland constants
+                                    (FunSphKernel.h:198-210) */
+  int32_t pad_2d;
+  double simulate2dposy;
 } dsph_case;
 
 /* State of a floating body (StFloatingData, DualSphDef.h): centre, rotation

@@ -65,6 +65,7 @@ struct orc_cte {
   unsigned tboundary;  // 2 = mDBC; otherwise DBC
   unsigned periactive;        // periodic axes, bits 1 X, 2 Y, 4 Z
   double perinc[3][3];         // PeriXinc, PeriYinc, PeriZinc
+  unsigned sim2d;              // 2-D simulation: mDBC systems over (1, x, z)
 };
 
 // Neighbour-search data (StDivDataCpu / StDivDataGpu).
@@ -748,6 +749,46 @@ inline double inverse4(const double a[4][4], double inv[4][4]) {
   return det;
 }
 
+// Determinant and inverse of a 3x3 matrix (FP64): the 2-D systems over
+// (1, x, z) of the reference's 2-D branches, with its term order (rule of
+// Sarrus for the determinant, 2x2 cofactors over the determinant for the
+// inverse; FunctionsMath.h:93-96 and 161-176, FunctionsMath_iker.h:111-114
+// and 150-165).
+inline double inverse3(const double a[3][3], double inv[3][3]) {
+  const double det = a[0][0] * a[1][1] * a[2][2] + a[0][1] * a[1][2] * a[2][0] +
+                     a[0][2] * a[1][0] * a[2][1] - a[2][0] * a[1][1] * a[0][2] -
+                     a[2][1] * a[1][2] * a[0][0] - a[2][2] * a[1][0] * a[0][1];
+  if (det != 0) {
+    inv[0][0] = (a[1][1] * a[2][2] - a[1][2] * a[2][1]) / det;
+    inv[0][1] = -(a[0][1] * a[2][2] - a[0][2] * a[2][1]) / det;
+    inv[0][2] = (a[0][1] * a[1][2] - a[0][2] * a[1][1]) / det;
+    inv[1][0] = -(a[1][0] * a[2][2] - a[1][2] * a[2][0]) / det;
+    inv[1][1] = (a[0][0] * a[2][2] - a[0][2] * a[2][0]) / det;
+    inv[1][2] = -(a[0][0] * a[1][2] - a[0][2] * a[1][0]) / det;
+    inv[2][0] = (a[1][0] * a[2][1] - a[1][1] * a[2][0]) / det;
+    inv[2][1] = -(a[0][0] * a[2][1] - a[0][1] * a[2][0]) / det;
+    inv[2][2] = (a[0][0] * a[1][1] - a[0][1] * a[1][0]) / det;
+  }
+  return det;
+}
+
+// Solve of the mDBC moment system: in 3-D the 4x4 matrix over (1, x, y, z);
+// in 2-D (all dy = 0, so the y row and column vanish) the 3x3 system over
+// (1, x, z) the reference's 2-D branches build. a: the 4x4 sums; returns the
+// determinant and writes the inverse into the 4x4 positions of the system's
+// indices (the y row and column stay 0 in 2-D).
+inline double mdbc_inverse(bool sim2d, const double a[4][4], double inv[4][4]) {
+  if (!sim2d) return inverse4(a, inv);
+  const int ix[3] = {0, 1, 3};
+  double a3[3][3], i3[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) a3[i][j] = a[ix[i]][ix[j]];
+  const double det = inverse3(a3, i3);
+  for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) inv[i][j] = 0;
+  if (det != 0)
+    for (int i = 0; i < 3; i++) for (int j = 0; j < 3; j++) inv[ix[i]][ix[j]] = i3[i][j];
+  return det;
+}
+
 // mDBC density of boundary particles, SlipMode SLIP_Vel0: first-order
 // (4x4) corrected Shepard density and gradient at the ghost node
 // g = x_b + boundnor (normals already doubled), extrapolated back to the
@@ -818,7 +859,7 @@ void mdbc_correction(int sem, const orc_cte& c, const orc_div& d, unsigned n, co
     float rhofinal = FLT_MAX;
     double ad[4][4], inv[4][4];
     for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) ad[i][j] = double(a[i][j]);
-    const double det = inverse4(ad, inv);
+    const double det = mdbc_inverse(c.sim2d != 0, ad, inv);
     if (std::fabs(det) >= 1e-3) {
       // The GPU build rounds the inverse to FP32 (tmatrix4f); the CPU build keeps FP64.
       M im[4][4];
@@ -946,7 +987,7 @@ void mdbc2_correction(int sem, const orc_cte& c, const orc_div& d, unsigned n, c
     } else {
       double ad[4][4], inv[4][4];
       for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) ad[i][j] = double(a[i][j]);
-      const double det = inverse4(ad, inv);
+      const double det = mdbc_inverse(c.sim2d != 0, ad, inv);
       if (std::fabs(det) >= 0.001) {
         M im[4][4];
         for (int i = 0; i < 4; i++) for (int j = 0; j < 4; j++) im[i][j] = M(inv[i][j]);

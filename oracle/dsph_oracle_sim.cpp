@@ -32,6 +32,7 @@ struct orc_cte {
   unsigned tboundary;
   unsigned periactive;        // periodic axes, bits 1 X, 2 Y, 4 Z
   double perinc[3][3];         // PeriXinc, PeriYinc, PeriZinc
+  unsigned sim2d;              // 2-D simulation
 };
 struct orc_div {
   int scelldiv, ncx, ncy, ncz, ncw;
@@ -157,8 +158,10 @@ struct Sim {
     c.kernelh = KernelH;
     c.kernelsize = ks;
     c.kernelsize2 = ks * ks;
-    c.awen = float(0.41778 / (h * h * h));
-    c.bwenh = float(-2.08891 / (h * h * h * h * h));
+    // Wendland constants (FunSphKernel.h:198-210), 2-D or 3-D.
+    c.sim2d = s.simulate2d ? 1u : 0u;
+    c.awen = c.sim2d ? float(0.557 / (h * h)) : float(0.41778 / (h * h * h));
+    c.bwenh = c.sim2d ? float(-2.7852 / (h * h * h * h)) : float(-2.08891 / (h * h * h * h * h));
     c.cteb = float(s.cteb); c.gamma = float(s.gamma); c.rhopzero = float(s.rhopzero);
     c.ovrhopzero = 1.f / c.rhopzero;
     Cs0 = std::sqrt(double(c.gamma) * double(c.cteb) / double(c.rhopzero));
@@ -183,8 +186,16 @@ struct Sim {
     CFL = s.cflnumber;
     DtIni = s.dtini ? s.dtini : double(KernelH) / Cs0;
     DtMin = s.dtmin ? s.dtmin : (double(KernelH) / Cs0) * double(0.05f);
+    // 2-D: linear motions lose their y part (JDsMotion.cpp:132).
+    if (c.sim2d && (s.casenfloat || s.periactive || (ShiftMode && ShiftTFS > 0) ||
+                    s.motion.type >= 2))
+      throw std::runtime_error("2-D runs are built without floating bodies, periodic axes, "
+                               "the shifting free-surface threshold and rotations");
+    if (c.sim2d) cs.motion.ampl[1] = 0;
     if (TVisco == 2) {
-      const double dps = std::sqrt(double(c.dp) * double(c.dp) * 3.) / 3.;
+      // dp_sps (JSph.cpp:1529).
+      const double dps = c.sim2d ? std::sqrt(double(c.dp) * double(c.dp) * 2.) / 2.
+                                 : std::sqrt(double(c.dp) * double(c.dp) * 3.) / 3.;
       SpsSmag = float((2. / 3.) * std::pow(0.12 * dps, 2));
       SpsBlin = float((2. / 3.) * 0.0066 * dps * dps);
     }

@@ -21,7 +21,8 @@ class OrcCte(C.Structure):
         "eta2", "ddtkh", "ddtgz", "scell", "kernelsize", "dp", "cteb", "gamma", "rhopzero",
         "ovrhopzero", "movlimit")] + [("cellcode", C.c_uint32)] + [
         (n, C.c_double * 3) for n in ("domposmin", "maprealposmin", "maprealsize", "mapposmin")] + [
-        ("tboundary", C.c_uint32), ("periactive", C.c_uint32), ("perinc", (C.c_double * 3) * 3)]
+        ("tboundary", C.c_uint32), ("periactive", C.c_uint32), ("perinc", (C.c_double * 3) * 3),
+        ("sim2d", C.c_uint32)]
 
 
 class OrcDiv(C.Structure):

@@ -119,6 +119,20 @@ CASES = {
     "ref_dambreak_dp0.03_mdbc2_moving": ("dam_break", 0.03, 40, (1, 10, 20, 40),
                                          dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4,
                                               vel0=(-1.0, 0.0, -1.0))),
+    # 2-D dam break (<data2d>, particles in the plane y = 0): DBC with the
+    # benchmark options; mDBC with laminar+SPS; mDBC2 no-slip under Verlet;
+    # mDBC2 free slip with no penetration, the column thrown at 1 m/s
+    # against the floor and the back wall.
+    "ref_dambreak2d_dp0.005": ("dam_break_2d", 0.005, 40, (1, 10, 20, 40), {}),
+    "ref_dambreak2d_dp0.005_mdbc_sps": ("dam_break_2d", 0.005, 40, (1, 10, 20, 40),
+                                        dict(tboundary=2, tvisco=2, visco=1e-6)),
+    "ref_dambreak2d_dp0.005_mdbc2_verlet": ("dam_break_2d", 0.005, 40, (1, 10, 20, 40),
+                                            dict(tboundary=2, slipmode=2, tstep=1, tvisco=3,
+                                                 visco=1e-4)),
+    "ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen": (
+        "dam_break_2d", 0.005, 40, (1, 10, 20, 40),
+        dict(tboundary=2, slipmode=3, nopenetration=1, tvisco=3, visco=1e-4,
+             vel0=(-2.0, -2.0))),
     # Channel periodic in x with the fluid moving at 2 m/s: particles leave
     # through one face and re-enter through the other (Symplectic + DBC, and
     # Verlet + mDBC + laminar viscosity).

@@ -74,12 +74,15 @@ def u3(vals):
 
 
 def perturbed_case(case, seed=7, vamp=0.5, ramp=0.01):
-    """A copy of ``case`` with random fluid velocities (|v|<~vamp m/s) and
-    density perturbations (relative ~ramp), so every interaction term is active."""
+    """A copy of ``case`` with random fluid velocities (|v|<~vamp m/s; in the
+    x-z plane for a 2-D case) and density perturbations (relative ~ramp), so
+    every interaction term is active."""
     rng = np.random.default_rng(seed)
     vr = case.velrho.copy()
     nb = case.nbound
     vr[nb:, :3] = (rng.standard_normal((case.nfluid, 3)) * vamp).astype(np.float32)
+    if case.data2d:
+        vr[nb:, 1] = 0
     vr[nb:, 3] *= (1 + ramp * rng.standard_normal(case.nfluid)).astype(np.float32)
     vr[:nb, 3] *= (1 + 0.5 * ramp * rng.random(nb)).astype(np.float32)
     import dataclasses
@@ -94,6 +97,7 @@ def to_cte(orc_cte, tboundary=N.BC_DBC) -> N.DsphCte:
               "ovrhopzero", "movlimit", "cellcode"):
         setattr(c, f, getattr(orc_cte, f))
     c.tboundary = tboundary
+    c.simulate2d = orc_cte.sim2d
     c.domposminx, c.domposminy, c.domposminz = orc_cte.domposmin
     c.maprealposminx, c.maprealposminy, c.maprealposminz = orc_cte.maprealposmin
     c.maprealsizex, c.maprealsizey, c.maprealsizez = orc_cte.maprealsize
@@ -230,7 +234,7 @@ def case_dp(fx) -> float:
     return float(fx["dp"])
 
 
-def hydrostatic_mdbc2_case(dp=0.02, depth=0.2, u=0.3, jitter=0.0):
+def hydrostatic_mdbc2_case(dp=0.02, depth=0.2, u=0.3, jitter=0.0, dim2=False):
     """Fluid layer of depth ``depth`` filling a 0.6 m x 0.6 m tank (mDBC
     no-slip), density linear in z with the hydrostatic gradient of the
     linearised EOS, rho = rho0 + rho0 g (depth - z) / cs0^2, and uniform
@@ -238,13 +242,18 @@ def hydrostatic_mdbc2_case(dp=0.02, depth=0.2, u=0.3, jitter=0.0):
     particle far from the side walls gets the same linear density at its own
     position (first-order correction reproduces linear fields; pressure
     cloning adds the hydrostatic gradient back) and tangential velocity
-    (-u, 0, 0) (no-slip mirror 2*0 - u)."""
+    (-u, 0, 0) (no-slip mirror 2*0 - u). ``dim2``: the 2-D section (0.6 m
+    wide, particles in the plane y = 0)."""
     from dualsphysics_amd import case as cm
     from dualsphysics_amd import _native as N
     p = cm.Parameters(tboundary=N.BC_MDBC, slipmode=N.SLIP_NOSLIP, tvisco=N.VISCO_LAMINAR,
                       visco=1e-4)
-    c = cm.dam_break(dp=dp, tank=(0.6, 0.6, 0.3), column=(0.6, 0.6, depth), jitter=jitter,
-                     params=p)
+    if dim2:
+        c = cm.dam_break_2d(dp=dp, tank=(0.6, 0.3), column=(0.6, depth), jitter=jitter,
+                            params=p)
+    else:
+        c = cm.dam_break(dp=dp, tank=(0.6, 0.6, 0.3), column=(0.6, 0.6, depth), jitter=jitter,
+                         params=p)
     nb = c.nbound
     z = c.pos[nb:, 2]
     c.velrho[nb:, 3] = (c.rhop0 + c.rhop0 * 9.81 * (depth - z) / c.cs0 ** 2).astype(np.float32)

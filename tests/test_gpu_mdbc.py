@@ -1,6 +1,7 @@
 """mDBC with SlipMode vel=0 on the GPU (k_mdbc, the boundary of the wave
 flume) against the oracle, kernel by kernel: the density of every boundary
-particle extrapolated from its ghost node (JSphGpu_mdbc_iker.cu:34-150).
+particle extrapolated from its ghost node (JSphGpu_mdbc_iker.cu:34-150), in
+3-D (4x4 systems) and 2-D (3x3 systems over (1, x, z)).
 
 Tolerance: densities of ~1000 kg/m^3 from the 4x4 first-order system, whose
 inverse is rounded to FP32 (the reference GPU build's tmatrix4f). The
@@ -50,6 +51,8 @@ def _cases():
         "flume_dp0.01": lambda: casemod.flume(dp=0.01, params=p),
         "perturbed_dam_break_dp0.02": lambda: perturbed_case(casemod.dam_break(dp=0.02, params=p)),
         "linear_layer_dp0.02": linear_layer,
+        "perturbed_dam_break_2d_dp0.005": lambda: perturbed_case(
+            casemod.dam_break_2d(dp=0.005, params=p)),
     }
 
 

@@ -53,10 +53,12 @@ def _cases():
     p = casemod.Parameters(tboundary=N.BC_MDBC, slipmode=N.SLIP_NOSLIP, tvisco=N.VISCO_LAMINAR,
                            visco=1e-4)
     db = perturbed_case(casemod.dam_break(dp=0.02, params=p))
-    return {"hydrostatic_layer": lay, "perturbed_dam_break": db}
+    db2 = perturbed_case(casemod.dam_break_2d(dp=0.005, params=p))
+    return {"hydrostatic_layer": lay, "perturbed_dam_break": db, "perturbed_dam_break_2d": db2}
 
 
-@pytest.mark.parametrize("name", ["hydrostatic_layer", "perturbed_dam_break"])
+@pytest.mark.parametrize("name", ["hydrostatic_layer", "perturbed_dam_break",
+                                  "perturbed_dam_break_2d"])
 def test_mdbc2_correction_matches_oracle(lib, ctx, name):
     c = _cases()[name]
     s = mdbc2_state(c)

@@ -22,11 +22,12 @@ from tests.helpers import hydrostatic_mdbc2_case, mdbc2_state
 DEPTH = 0.2
 
 
-def linear_layer(dp=0.02):
-    """Hydrostatic layer of depth DEPTH in a 0.6 m x 0.6 m tank with mDBC
-    SlipMode vel=0, density linear in z (see hydrostatic_mdbc2_case), at rest."""
+def linear_layer(dp=0.02, dim2=False):
+    """Hydrostatic layer of depth DEPTH in a 0.6 m x 0.6 m tank (or its 2-D
+    section) with mDBC SlipMode vel=0, density linear in z (see
+    hydrostatic_mdbc2_case), at rest."""
     import dataclasses
-    c = hydrostatic_mdbc2_case(dp=dp, depth=DEPTH, u=0.0)
+    c = hydrostatic_mdbc2_case(dp=dp, depth=DEPTH, u=0.0, dim2=dim2)
     prm = dataclasses.replace(c.params, slipmode=N.SLIP_VEL0)
     return dataclasses.replace(c, params=prm)
 
@@ -82,3 +83,27 @@ def test_dry_ghost_nodes_keep_rho0(layer):
     dry = pb[:, 2] > DEPTH + 2 * c.h
     assert dry.sum() > 0
     assert np.all(vr[:npb, 3][dry] == np.float32(c.rhop0))
+
+
+@pytest.mark.parametrize("prec,sem,tol", SEMS, ids=["fp64", "fp32_gpu", "fp32_cpu"])
+def test_2d_floor_density_is_the_linear_field_at_the_particle(prec, sem, tol):
+    """2-D (all dy = 0): the 3x3 system over (1, x, z) reproduces the linear
+    field as the 4x4 one does in 3-D; the singular 4x4 system (solved as 3-D)
+    would fall back to the Shepard sum, off by far more than tol."""
+    c = linear_layer(dp=0.01, dim2=True)
+    s = mdbc2_state(c)
+    assert s["cte"].sim2d == 1
+    vr = oracle_mdbc(s, prec, sem)
+    npb = s["npb"]
+    pb = s["pos"][:npb]
+    bn = s["bn2"][:npb]
+    floor = (bn[:, 0] == 0) & (bn[:, 2] > 0)
+    inner = floor & (pb[:, 0] > 0.15) & (pb[:, 0] < 0.45)
+    assert inner.sum() > 40 and np.all(bn[:, 1] == 0)
+    rho_exact = c.rhop0 + c.rhop0 * 9.81 * (DEPTH - pb[:, 2]) / c.cs0 ** 2
+    err = np.abs(vr[:npb, 3] - rho_exact)[inner]
+    print(f"2-D prec {prec} sem {sem}: max |rho_b - rho(x_b)| {err.max():.2e} over {inner.sum()}")
+    assert err.max() < tol
+    s["cte"].sim2d = 0
+    shep = oracle_mdbc(s, prec, sem)
+    assert np.abs(shep[:npb, 3] - rho_exact)[inner].min() > 10 * tol

@@ -43,6 +43,28 @@ def test_floor_density_and_tangential_velocity_closed_form(layer, prec, sem, tol
     np.testing.assert_array_equal(vr[:npb, :3], s["d"]["velrho"][:npb, :3])
 
 
+@pytest.mark.parametrize("prec,sem,tol", SEMS, ids=["fp64", "fp32_gpu", "fp32_cpu"])
+def test_2d_floor_density_and_tangential_velocity_closed_form(prec, sem, tol):
+    """2-D section of the layer (all dy = 0, 3x3 systems over (1, x, z), the
+    2-D Wendland constants): the same closed forms as in 3-D."""
+    c = hydrostatic_mdbc2_case(dp=0.01, depth=DEPTH, u=U, dim2=True)
+    s = mdbc2_state(c)
+    assert s["cte"].sim2d == 1
+    vr, bm, tv = oracle_mdbc2(c, s, prec, sem)
+    npb = s["npb"]
+    pb = s["pos"][:npb]
+    bn = s["bn2"][:npb]
+    inner = (bn[:, 0] == 0) & (bn[:, 2] > 0) & (pb[:, 0] > 0.15) & (pb[:, 0] < 0.45)
+    assert inner.sum() > 40
+    rho_exact = c.rhop0 + c.rhop0 * 9.81 * (DEPTH - pb[:, 2]) / c.cs0 ** 2
+    err = np.abs(vr[:npb, 3] - rho_exact)[inner].max()
+    print(f"2-D prec {prec} sem {sem}: max |rho_b - rho(x_b)| {err:.2e}")
+    assert err < tol
+    assert np.all(bm[inner] == 1)
+    np.testing.assert_allclose(tv[inner], np.tile([-U, 0.0, 0.0], (inner.sum(), 1)),
+                               atol=2e-6 if prec == 0 else 1e-12)
+
+
 def test_dry_ghost_nodes_switch_the_particle_off(layer):
     c, s = layer
     vr, bm, tv = oracle_mdbc2(c, s, 1, orc.SEM_F64)

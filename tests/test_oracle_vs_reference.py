@@ -21,7 +21,10 @@ leave through one face and re-enter through the other; the dam break with
 a floating box of half the water density at the column's surface over 40
 steps (rigid-body motion from the fluid forces); mDBC2 free slip, and
 no penetration (Symplectic with no-slip, Verlet with free slip), with the
-column thrown at 1 m/s against the floor and the back wall over 40 steps.
+column thrown at 1 m/s against the floor and the back wall over 40 steps;
+the 2-D dam break (<data2d>: 2-D Wendland constants, 3x3 mDBC systems over
+(1, x, z), 2-D SPS length) with DBC, mDBC + laminar+SPS, mDBC2 no-slip
+under Verlet and mDBC2 free slip with no penetration, over 40 steps.
 """
 import numpy as np
 import pytest
@@ -40,7 +43,9 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_dambreak_dp0.03_viscbound2", "ref_channel_dp0.02_periodic",
             "ref_channel_dp0.02_periodic_mdbc_verlet", "ref_dambreak_dp0.03_floating",
             "ref_dambreak_dp0.03_mdbc2_moving", "ref_dambreak_dp0.03_mdbc2_freeslip",
-            "ref_dambreak_dp0.03_mdbc2_nopen", "ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet"]
+            "ref_dambreak_dp0.03_mdbc2_nopen", "ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet",
+            "ref_dambreak2d_dp0.005", "ref_dambreak2d_dp0.005_mdbc_sps",
+            "ref_dambreak2d_dp0.005_mdbc2_verlet", "ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
@@ -80,6 +85,8 @@ def test_fixture_detects_a_planted_error():
     ("ref_dambreak_dp0.03_mdbc2_nopen", dict(nopenetration=0)),
     ("ref_dambreak_dp0.03_mdbc2_freeslip", dict(slipmode=2)),
     ("ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet", dict(nopenetration=0)),
+    ("ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen", dict(nopenetration=0)),
+    ("ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen", dict(slipmode=2)),
 ])
 def test_mdbc2_options_are_pinned_by_their_fixtures(name, wrong):
     """The oracle run without the option a fixture was made with (no
@@ -98,3 +105,25 @@ def test_mdbc2_options_are_pinned_by_their_fixtures(name, wrong):
     with pytest.raises(AssertionError):
         compare_with_reference(fx, len(fx["steps"]) - 1, t, case.pos, d["pos"][idx],
                                d["velrho"][idx, :3], d["velrho"][idx, 3])
+
+
+@pytest.mark.parametrize("name", ["ref_dambreak2d_dp0.005", "ref_dambreak2d_dp0.005_mdbc_sps"])
+def test_2d_mode_is_pinned_by_its_fixtures(name):
+    """The 2-D fixtures run as 3-D (3-D Wendland constants and SPS length,
+    singular 4x4 mDBC systems falling back to Shepard) leave the bounds at
+    the first saved step (particles excluded, or the fields off)."""
+    import dataclasses
+    fx, case = load_reference_fixture(name)
+    bad = dataclasses.replace(case, data2d=False)
+    o = OracleSim(bad, prec=2)
+    try:
+        o.run(int(fx["steps"][0]))
+        d, t = o.download(), o.stats()["timestep"]
+    finally:
+        o.close()
+    if d["idp"].shape[0] != case.np:
+        return
+    idx = np.argsort(d["idp"], kind="stable")
+    with pytest.raises(AssertionError):
+        compare_with_reference(fx, 0, t, case.pos, d["pos"][idx], d["velrho"][idx, :3],
+                               d["velrho"][idx, 3])
