@@ -442,6 +442,49 @@ def periodic_channel(dp: float = 0.02, length: float = 0.4, width: float = 0.2,
                 nfluid=nf, maprealposmin=mn, maprealposmax=mx, boundnor=boundnor)
 
 
+def periodic_channel_2d(dp: float = 0.01, length: float = 0.4, depth: float = 0.2,
+                        layers: int = 2, u0: float = 2.0, coefh: float = 1.2,
+                        coefsound: float = 20.0, rhop0: float = 1000.0, gamma: float = 7.0,
+                        g: float = 9.81, seed: int = 42, jitter: float = 1e-4, posy: float = 0.0,
+                        params: Parameters | None = None, name: str | None = None) -> Case:
+    """2-D section of :func:`periodic_channel` (particles in the plane
+    y = ``posy``): a floor ``layers`` thick under a fluid layer of ``depth``
+    moving at ``u0`` along x, periodic in x, 2-D constants (h from sqrt(2 dp^2),
+    mass rho0 dp^2); the jitter moves x and z only."""
+    params = params or Parameters(periodic=1)
+    if not params.periodic & 1:
+        raise ValueError("periodic_channel_2d needs periodic=1 (x)")
+    nx, nz = _n(length, dp), _n(depth, dp)
+    k, i = np.meshgrid(np.arange(-layers, nz), np.arange(0, nx), indexing="ij")
+    i, k = i.ravel(), k.ravel()
+    wall = k < 0
+    pts = np.stack([(i + 0.5) * dp, np.full(i.size, posy), (k + 0.5) * dp], 1)
+    pos = np.concatenate([pts[wall], pts[~wall]], 0)
+    nb, nf = int(wall.sum()), int((~wall).sum())
+    u = uniform01(seed, 2 * pos.shape[0]).reshape(-1, 2)
+    pos[:, 0] += (2.0 * u[:, 0] - 1.0) * (jitter * dp)
+    pos[:, 2] += (2.0 * u[:, 1] - 1.0) * (jitter * dp)
+    h = coefh * math.sqrt(2.0 * dp * dp)
+    cs0 = coefsound * math.sqrt(g * depth)
+    b = cs0 * cs0 * rhop0 / gamma
+    mass = rhop0 * dp ** 2
+    velrho = np.zeros((pos.shape[0], 4), np.float32)
+    velrho[:nb, 3] = rhop0
+    velrho[nb:, 0] = u0
+    zf = pos[nb:, 2]
+    velrho[nb:, 3] = (rhop0 * (1.0 + rhop0 * g * (depth - zf) / b) ** (1.0 / gamma)).astype(np.float32)
+    bp = pos[:nb]
+    q = np.stack([bp[:, 0], bp[:, 1], np.maximum(bp[:, 2], 0.0)], 1)
+    boundnor = (q - bp).astype(np.float32)
+    idp = np.arange(pos.shape[0], dtype=np.uint32)
+    mn, mx = map_limits(pos, h, periodic=params.periodic, dp=dp)
+    return Case(name=name or f"channel2d_dp{dp:g}", dp=dp, h=h, rhop0=rhop0, gamma=gamma,
+                cs0=cs0, b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g),
+                hswl=depth, params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb,
+                nfluid=nf, maprealposmin=mn, maprealposmax=mx, boundnor=boundnor,
+                data2d=True, data2dposy=posy)
+
+
 def dam_break_cfg0(**kw) -> Case:
     """3-D dam break at dp=0.01 m: 80,400 fluid + 60,248 wall particles."""
     return dam_break(dp=0.01, **kw)

@@ -239,9 +239,10 @@ struct dsph_sim {
     Simulate2D = c.simulate2d != 0;
     Simulate2DPosY = c.simulate2dposy;
     if (Simulate2D) {
-      if (c.casenfloat || c.periactive || (c.shiftmode && c.shifttfs > 0) || c.motion.type >= 2)
-        throw Err(DSPH_E_ARG, "2-D runs are built without floating bodies, periodic axes, the "
-                              "shifting free-surface threshold and rotation motions");
+      if (c.periactive & 2)
+        throw Err(DSPH_E_ARG, "Cannot use periodic conditions in Y with 2D simulations");
+      if (c.casenfloat || c.motion.type >= 2)
+        throw Err(DSPH_E_ARG, "2-D runs are built without floating bodies and rotation motions");
       cs.motion.ampl[1] = 0;
     }
     const double h = KernelH;

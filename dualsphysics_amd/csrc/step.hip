@@ -1001,7 +1001,8 @@ int dsph_run_shifting(dsph_ctx* ctx, uint32_t n, uint32_t pini, double dt, const
   DSPH_REQUIRE(ctx, d_velrho && d_shiftposfs, "dsph_run_shifting: null array");
   KTimer kt(ctx, "k_run_shifting");
   k_run_shifting<<<grid_for(n, kBs), kBs, 0, ctx->stream>>>(
-      n, pini, dt, d_dt, dt_slot, dt_scale, shiftcoef, kernelh, shifttfs, 3.0f - shifttfs,
+      n, pini, dt, d_dt, dt_slot, dt_scale, shiftcoef, kernelh, shifttfs,
+      ((ctx->cte_set && ctx->cte.simulate2d) ? 2.0f : 3.0f) - shifttfs,
       float(dp * 0.1), (const float4*)d_velrho, (float4*)d_shiftposfs);
   DSPH_LAUNCHED(ctx, "k_run_shifting");
   return DSPH_OK;

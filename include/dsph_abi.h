@@ -504,7 +504,8 @@ int dsph_gauge_force(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin, uint32_t cou
 /* Replaces cushift::RunShifting (JSphShifting_ker.cu:123-165; host
  * JSphShifting::RunGpu JSphShifting.cpp:463-470) for fluid [pini,pini+n):
  * umagn = dt*shiftcoef*kernelh*|v| (times (w-tfs)/(3-tfs) above the
- * free-surface threshold tfs, 0 below; tfs = 0 disables it), shift =
+ * free-surface threshold tfs, 0 below; tfs = 0 disables it; 2-tfs when the
+ * context's constants have simulate2d set), shift =
  * shiftposfs.xyz*umagn clamped to +-0.1 dp; a cancelled particle (x =
  * FLT_MAX) gets 0. dt = d_dt[dt_slot]*dt_scale when d_dt is not NULL. */
 int dsph_run_shifting(dsph_ctx* ctx, uint32_t n, uint32_t pini, double dt,

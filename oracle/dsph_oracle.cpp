@@ -1388,12 +1388,14 @@ void orc_nopen_step(unsigned np, unsigned npb, const float* nopenshift, const fl
 // Shifting displacement of fluid [pini, pini+n) (JSphShifting::RunCpu,
 // JSphShifting.cpp:390-420; GPU KerRunShifting JSphShifting_ker.cu:123-151):
 // umagn = dt*coef*h*|v| in FP64, free-surface factor (w - tfs)/(3 - tfs)
-// above tfs (0 below; tfs = 0: off), shift = sums*umagn clamped to
-// +-float(dp*0.1); cancelled particles (x = FLT_MAX) get 0.
+// (2 - tfs in 2-D, JSphShifting.cpp:394) above tfs (0 below; tfs = 0: off),
+// shift = sums*umagn clamped to +-float(dp*0.1); cancelled particles
+// (x = FLT_MAX) get 0.
 void orc_run_shifting(unsigned n, unsigned pini, double dt, float shiftcoef, float kernelh,
-                      float tfs, double dp, const float* velrho, float* shiftposfs) {
+                      float tfs, double dp, const float* velrho, float* shiftposfs,
+                      unsigned sim2d) {
   const double coefumagn = dt * shiftcoef * kernelh;
-  const double coeftfs = 3.0 - tfs;
+  const double coeftfs = (sim2d ? 2.0 : 3.0) - tfs;
   const float maxdist = float(dp * 0.1);
   for (unsigned p = pini; p < pini + n; p++) {
     float* rs = shiftposfs + 4 * size_t(p);

@@ -83,7 +83,7 @@ void orc_interaction_ext(int, int, int, int, float, float, const orc_cte*, const
                          float*, float*, const unsigned char*, const float*, int,
                          const typecode*, float*);
 void orc_run_shifting(unsigned, unsigned, double, float, float, float, double, const float*,
-                      float*);
+                      float*, unsigned);
 void orc_add_shift(unsigned, unsigned, const float*, double*);
 void orc_interaction_ft(int, int, int, int, float, float, const orc_cte*, const orc_div*,
                         unsigned, unsigned, unsigned, const unsigned*, const float*,
@@ -187,10 +187,10 @@ struct Sim {
     DtIni = s.dtini ? s.dtini : double(KernelH) / Cs0;
     DtMin = s.dtmin ? s.dtmin : (double(KernelH) / Cs0) * double(0.05f);
     // 2-D: linear motions lose their y part (JDsMotion.cpp:132).
-    if (c.sim2d && (s.casenfloat || s.periactive || (ShiftMode && ShiftTFS > 0) ||
-                    s.motion.type >= 2))
-      throw std::runtime_error("2-D runs are built without floating bodies, periodic axes, "
-                               "the shifting free-surface threshold and rotations");
+    // 2-D: no periodic y (JSph.cpp:827).
+    if (c.sim2d && (s.casenfloat || (s.periactive & 2) || s.motion.type >= 2))
+      throw std::runtime_error("2-D runs are built without floating bodies, a periodic y axis "
+                               "and rotations");
     if (c.sim2d) cs.motion.ampl[1] = 0;
     if (TVisco == 2) {
       // dp_sps (JSph.cpp:1529).
@@ -661,7 +661,7 @@ struct Sim {
   void shifting(double dt) {
     if (!ShiftMode) return;
     orc_run_shifting(Np - Npb, Npb, dt, ShiftCoef, KernelH, ShiftTFS, cs.dp, velrho.data(),
-                     shiftposfs.data());
+                     shiftposfs.data(), c.sim2d);
   }
 
   // ---- floating bodies: JSphCpuSingle::RunFloating (JSphCpuSingle.cpp:938-993)

@@ -75,7 +75,7 @@ def load() -> C.CDLL:
             "orc_matrix_rot": ([d, vp, vp, vp], None),
             "orc_interaction_ext": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
                                      u, u] + [vp] * 12 + [i, vp, vp], None),
-            "orc_run_shifting": ([u, u, d, f, f, f, d, vp, vp], None),
+            "orc_run_shifting": ([u, u, d, f, f, f, d, vp, vp, u], None),
             "orc_add_shift": ([u, u, vp, vp], None),
             "orc_interaction_mdbc2": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
                                        u, u, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp],

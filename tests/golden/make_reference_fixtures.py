@@ -133,6 +133,18 @@ CASES = {
         "dam_break_2d", 0.005, 40, (1, 10, 20, 40),
         dict(tboundary=2, slipmode=3, nopenetration=1, tvisco=3, visco=1e-4,
              vel0=(-2.0, -2.0))),
+    # 2-D shifting (free-surface factor (w - tfs)/(2 - tfs)): no boundaries
+    # with the threshold, and full; 2-D channel periodic in x (DBC, and mDBC
+    # with Verlet and laminar viscosity).
+    "ref_dambreak2d_dp0.005_shift_nobound_tfs": ("dam_break_2d", 0.005, 40, (1, 10, 20, 40),
+                                                 dict(shifting=1, shifttfs=1.5)),
+    "ref_dambreak2d_dp0.005_shift_full": ("dam_break_2d", 0.005, 40, (1, 10, 20, 40),
+                                          dict(shifting=3)),
+    "ref_channel2d_dp0.005_periodic": ("periodic_channel_2d", 0.005, 60, (1, 30, 60),
+                                       dict(periodic=1)),
+    "ref_channel2d_dp0.005_periodic_mdbc_verlet": ("periodic_channel_2d", 0.005, 60, (1, 30, 60),
+                                                   dict(periodic=1, tboundary=2, tstep=1,
+                                                        tvisco=3, visco=1e-4)),
     # Channel periodic in x with the fluid moving at 2 m/s: particles leave
     # through one face and re-enter through the other (Symplectic + DBC, and
     # Verlet + mDBC + laminar viscosity).

@@ -84,7 +84,14 @@ def test_shifting_sums_match_oracle_and_kernels_agree(lib, ctx, shiftmode):
     assert e < 1e-5 and ew < 1e-5
 
 
-def test_shifting_displacement_matches_oracle(lib, ctx):
+@pytest.mark.parametrize("sim2d", [0, 1], ids=["3d", "2d"])
+def test_shifting_displacement_matches_oracle(lib, ctx, sim2d):
+    """The displacement pass: free-surface factor (w - tfs)/(3 - tfs), or
+    (w - tfs)/(2 - tfs) in 2-D (JSphShifting.cpp:467), from the context's
+    constants."""
+    cte = N.DsphCte()
+    cte.simulate2d = sim2d
+    assert lib.dsph_set_constants(ctx, C.byref(cte)) == 0
     rng = np.random.default_rng(1)
     n, pini = 5000, 100
     vr = np.zeros((n + pini, 4), np.float32)
@@ -100,7 +107,7 @@ def test_shifting_displacement_matches_oracle(lib, ctx):
         got = d_sh.get()
         ref = sh.copy()
         orc.load().orc_run_shifting(n, pini, 1e-4, -2.0, float(np.float32(0.0249)), tfs, 0.01,
-                                    vr.ctypes.data, ref.ctypes.data)
+                                    vr.ctypes.data, ref.ctypes.data, sim2d)
         np.testing.assert_array_equal(got[:pini], sh[:pini])
         np.testing.assert_allclose(got[pini:, :3], ref[pini:, :3], rtol=2e-7, atol=1e-12)
         assert (np.abs(got[pini:, :3]) <= np.float32(0.001)).all()

@@ -3,8 +3,9 @@ particles: the whole step (divide, interaction, integration, dt) for the
 first 10 to 60 steps, from the reference-run fixtures of tests/golden/
 (bounds in tests/helpers.REF_TOL, the FP32 rounding spread between the
 reference and the oracle). The 2-D cases (<data2d>, particles in the plane
-y = 0) cover DBC, mDBC with laminar+SPS, mDBC2 no-slip under Verlet and
-mDBC2 free slip with no penetration."""
+y = 0) cover DBC, mDBC with laminar+SPS, mDBC2 no-slip under Verlet, mDBC2
+free slip with no penetration, shifting (with the free-surface threshold,
+and full) and a channel periodic in x."""
 import numpy as np
 import pytest
 
@@ -35,7 +36,11 @@ pytestmark = pytest.mark.gpu
                                   "ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet",
                                   "ref_dambreak2d_dp0.005", "ref_dambreak2d_dp0.005_mdbc_sps",
                                   "ref_dambreak2d_dp0.005_mdbc2_verlet",
-                                  "ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen"])
+                                  "ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen",
+                                  "ref_dambreak2d_dp0.005_shift_nobound_tfs",
+                                  "ref_dambreak2d_dp0.005_shift_full",
+                                  "ref_channel2d_dp0.005_periodic",
+                                  "ref_channel2d_dp0.005_periodic_mdbc_verlet"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)

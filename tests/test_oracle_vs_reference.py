@@ -24,7 +24,11 @@ no penetration (Symplectic with no-slip, Verlet with free slip), with the
 column thrown at 1 m/s against the floor and the back wall over 40 steps;
 the 2-D dam break (<data2d>: 2-D Wendland constants, 3x3 mDBC systems over
 (1, x, z), 2-D SPS length) with DBC, mDBC + laminar+SPS, mDBC2 no-slip
-under Verlet and mDBC2 free slip with no penetration, over 40 steps.
+under Verlet and mDBC2 free slip with no penetration, over 40 steps; 2-D
+shifting (no boundaries with the free-surface threshold, whose 2-D factor
+(w - tfs)/(2 - tfs) the fixture pins: the 3-D factor puts the displacement
+3 % off after one step; and full) over 40 steps; a 2-D channel periodic in
+x (DBC; Verlet + mDBC + laminar) over 60 steps.
 """
 import numpy as np
 import pytest
@@ -45,7 +49,9 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_dambreak_dp0.03_mdbc2_moving", "ref_dambreak_dp0.03_mdbc2_freeslip",
             "ref_dambreak_dp0.03_mdbc2_nopen", "ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet",
             "ref_dambreak2d_dp0.005", "ref_dambreak2d_dp0.005_mdbc_sps",
-            "ref_dambreak2d_dp0.005_mdbc2_verlet", "ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen"]
+            "ref_dambreak2d_dp0.005_mdbc2_verlet", "ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen",
+            "ref_dambreak2d_dp0.005_shift_nobound_tfs", "ref_dambreak2d_dp0.005_shift_full",
+            "ref_channel2d_dp0.005_periodic", "ref_channel2d_dp0.005_periodic_mdbc_verlet"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
