@@ -341,7 +341,7 @@ def write_case(case, directory, name: str, timemax: float | None = None,
             <massbound value="{r(case.massbound)}" />
             <massfluid value="{r(case.massfluid)}" />
         </constants>
-        <particles np="{n}" nb="{nb}" nbf="{nfix + nmov}" mkboundfirst="11" mkfluidfirst="1">
+        <particles np="{n}" nb="{nb}" nbf="{nfix}" mkboundfirst="11" mkfluidfirst="1">
             <fixed mkbound="0" mk="11" begin="0" count="{nfix}" />
 {moving}{floating}            <fluid mkfluid="0" mk="1" begin="{nb}" count="{nf}" />
         </particles>
@@ -649,6 +649,14 @@ def read_case(directory, name: str):
     p.perinc = tuple(tuple(v) for v in perinc)
     parts = ex.find("particles")
     blocks = list(parts)
+    # Header counts against the blocks (JCaseParts::ReadXml, JCaseParts.cpp:566-568):
+    # np all particles, nb all but fluid, nbf the fixed ones.
+    cnt = {t: sum(int(bk.get("count")) for bk in blocks if bk.tag == t)
+           for t in ("fixed", "moving", "floating", "fluid")}
+    if (int(parts.get("np")) != sum(cnt.values()) or
+            int(parts.get("nb")) != int(parts.get("np")) - cnt["fluid"] or
+            int(parts.get("nbf")) != cnt["fixed"]):
+        raise ValueError("The amount of particles does not match the header.")
     floatings = []
     v3 = lambda e, d=(0.0, 0.0, 0.0): d if e is None else \
         (float(e.get("x")), float(e.get("y")), float(e.get("z")))

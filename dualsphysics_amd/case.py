@@ -399,6 +399,50 @@ def dam_break_floating(dp: float = 0.03, box=(0.12, 0.12, 0.12), box_center=(0.1
                                maprealposmax=mx, boundnor=None, floatings=[ft])
 
 
+def dam_break_floating_2d(dp: float = 0.005, box=(0.12, 0.12), box_center=(0.16, 0.3),
+                          rhobody: float = 500.0, velini=(0.0, 0.0, 0.0),
+                          omegaini=(0.0, 0.0, 0.0), params: Parameters | None = None,
+                          name: str | None = None, **kw) -> Case:
+    """The 2-D dam break (``dam_break_2d``) with a floating box (x-z section
+    ``box`` centred at ``box_center``) of density ``rhobody``, half-immersed
+    at the column's surface by default; mass per particle rhobody*dp^2,
+    centre and inertia from the (jittered) particles."""
+    base = dam_break_2d(dp=dp, params=params, name=name or f"dambreak2d_float_dp{dp:g}",
+                        jitter=0.0, **{k: v for k, v in kw.items() if k != "jitter"})
+    nb = base.nbound
+    posy = base.data2dposy
+    lo = np.array(box_center) - 0.5 * np.array(box)
+    hi = np.array(box_center) + 0.5 * np.array(box)
+    i0 = np.ceil(lo / dp - 0.5).astype(int)
+    i1 = np.floor(hi / dp - 0.5).astype(int) + 1
+    k, i = np.meshgrid(np.arange(i0[1], i1[1]), np.arange(i0[0], i1[0]), indexing="ij")
+    bpts = np.stack([(i.ravel() + 0.5) * dp, np.full(i.size, posy), (k.ravel() + 0.5) * dp], 1)
+    fl = base.pos[nb:]
+    xz = fl[:, [0, 2]]
+    inside = np.all((xz > lo - 0.5 * dp) & (xz < hi + 0.5 * dp), axis=1)
+    fpts = fl[~inside]
+    pos = np.concatenate([base.pos[:nb], bpts, fpts], 0)
+    seed, jitter = kw.get("seed", 42), kw.get("jitter", 1e-4)
+    u = uniform01(seed, 2 * pos.shape[0]).reshape(-1, 2)
+    pos[:, 0] += (2.0 * u[:, 0] - 1.0) * (jitter * dp)
+    pos[:, 2] += (2.0 * u[:, 1] - 1.0) * (jitter * dp)
+    nfl, nf = bpts.shape[0], fpts.shape[0]
+    velrho = np.zeros((pos.shape[0], 4), np.float32)
+    velrho[:nb + nfl, 3] = base.rhop0
+    zf = pos[nb + nfl:, 2]
+    g = -base.gravity[2]
+    velrho[nb + nfl:, 3] = (base.rhop0 * (1.0 + base.rhop0 * g * (base.hswl - zf) / base.b)
+                            ** (1.0 / base.gamma)).astype(np.float32)
+    masspart = rhobody * dp ** 2
+    cen, inertia = rigid_body(pos[nb:nb + nfl], masspart)
+    ft = Floating(begin=nb, count=nfl, massbody=masspart * nfl, masspart=masspart, center=cen,
+                  inertia=inertia, velini=tuple(velini), omegaini=tuple(omegaini))
+    mn, mx = map_limits(pos, base.h)
+    return dataclasses.replace(base, idp=np.arange(pos.shape[0], dtype=np.uint32), pos=pos,
+                               velrho=velrho, nbound=nb + nfl, nfluid=nf, maprealposmin=mn,
+                               maprealposmax=mx, boundnor=None, floatings=[ft])
+
+
 def periodic_channel(dp: float = 0.02, length: float = 0.4, width: float = 0.2,
                      depth: float = 0.2, layers: int = 2, u0: float = 2.0, coefh: float = 1.2,
                      coefsound: float = 20.0, rhop0: float = 1000.0, gamma: float = 7.0,
@@ -561,3 +605,69 @@ def flume(dp: float = 0.01, length: float = 0.6, width: float = 0.1, height: flo
                 params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb, nfluid=nf,
                 maprealposmin=mn, maprealposmax=mx, boundnor=boundnor, nmoving=nmov,
                 motion=motion)
+
+
+def _lattice_xz(nx0, nx1, nz0, nz1, dp, posy):
+    """2-D lattice points ((i+1/2)dp, posy, (k+1/2)dp), x fastest then z."""
+    k, i = np.meshgrid(np.arange(nz0, nz1), np.arange(nx0, nx1), indexing="ij")
+    i, k = i.ravel(), k.ravel()
+    return np.stack([(i + 0.5) * dp, np.full(i.size, posy), (k + 0.5) * dp], 1), np.stack([i, k], 1)
+
+
+def flume_2d(dp: float = 0.005, length: float = 0.6, height: float = 0.3, depth: float = 0.15,
+             ampl: float = 0.02, period: float = 0.8, phase: float = 0.0, layers: int = 2,
+             coefh: float = 1.2, coefsound: float = 20.0, rhop0: float = 1000.0,
+             gamma: float = 7.0, g: float = 9.81, seed: int = 42, jitter: float = 1e-4,
+             posy: float = 0.0, params: Parameters | None = None, name: str | None = None,
+             wavemaker: str = "piston", flap_deg: float = 4.0, angvel: float = 10.0) -> Case:
+    """2-D section of :func:`flume` (particles in the plane y = ``posy``):
+    fluid [0,length]x[0,depth], fixed floor and far-end wall extended behind
+    the wavemaker's stroke, a piston (rectilinear sinusoid in x) or a flap
+    (rotation about the hinge line x=0, z=0, axis +y: sinusoid or constant
+    angular velocity) of ``layers`` layers. Ids: fixed walls, the wavemaker,
+    the fluid; 2-D constants (h from sqrt(2 dp^2), mass rho0 dp^2), jitter in
+    x and z only."""
+    params = params or Parameters()
+    nx, nz = _n(length, dp), _n(height, dp)
+    nback = int(math.ceil(ampl / dp)) + layers
+    wpts, widx = _lattice_xz(-nback - layers, nx + layers, -layers, nz, dp, posy)
+    inside = (widx[:, 0] >= -nback) & (widx[:, 0] < nx) & (widx[:, 1] >= 0)
+    fixed = wpts[~inside]
+    ppts, _ = _lattice_xz(-layers, 0, 0, nz, dp, posy)
+    fpts, _ = _lattice_xz(0, nx, 0, _n(depth, dp), dp, posy)
+    nfix, nmov, nf = fixed.shape[0], ppts.shape[0], fpts.shape[0]
+    pos = np.concatenate([fixed, ppts, fpts], 0)
+    u = uniform01(seed, 2 * pos.shape[0]).reshape(-1, 2)
+    pos[:, 0] += (2.0 * u[:, 0] - 1.0) * (jitter * dp)
+    pos[:, 2] += (2.0 * u[:, 1] - 1.0) * (jitter * dp)
+    h = coefh * math.sqrt(2.0 * dp * dp)
+    cs0 = coefsound * math.sqrt(g * depth)
+    b = cs0 * cs0 * rhop0 / gamma
+    mass = rhop0 * dp ** 2
+    velrho = np.zeros((pos.shape[0], 4), np.float32)
+    nb = nfix + nmov
+    velrho[:nb, 3] = rhop0
+    zf = pos[nb:, 2]
+    velrho[nb:, 3] = (rhop0 * (1.0 + rhop0 * g * (depth - zf) / b) ** (1.0 / gamma)).astype(np.float32)
+    bp = pos[:nb]
+    xlo = np.where(np.arange(nb) >= nfix, 0.0, -nback * dp)
+    q = np.stack([np.clip(bp[:, 0], xlo, length), bp[:, 1], np.maximum(bp[:, 2], 0.0)], 1)
+    boundnor = (q - bp).astype(np.float32)
+    idp = np.arange(pos.shape[0], dtype=np.uint32)
+    mn, mx = map_limits(pos, h)
+    motion = {"freq": (1.0 / period, 0.0, 0.0), "ampl": (ampl, 0.0, 0.0),
+              "phase": (phase, 0.0, 0.0), "start": 0.0, "finish": 1e30}
+    if wavemaker == "flap":
+        motion = {"type": "rotsinu", "axisp1": (0.0, posy, 0.0), "axisp2": (0.0, posy + 1.0, 0.0),
+                  "angfreq": 1.0 / period, "angampl": flap_deg, "angphase": phase,
+                  "start": 0.0, "finish": 1e30}
+    elif wavemaker == "flap_rot":
+        motion = {"type": "rot", "axisp1": (0.0, posy, 0.0), "axisp2": (0.0, posy + 1.0, 0.0),
+                  "angvel": angvel, "start": 0.0, "finish": 1e30}
+    elif wavemaker != "piston":
+        raise ValueError(f"unknown wavemaker {wavemaker!r}")
+    return Case(name=name or f"flume2d_dp{dp:g}", dp=dp, h=h, rhop0=rhop0, gamma=gamma, cs0=cs0,
+                b=b, massfluid=mass, massbound=mass, gravity=(0.0, 0.0, -g), hswl=depth,
+                params=params, idp=idp, pos=pos, velrho=velrho, nbound=nb, nfluid=nf,
+                maprealposmin=mn, maprealposmax=mx, boundnor=boundnor, nmoving=nmov,
+                motion=motion, data2d=True, data2dposy=posy)

@@ -29,6 +29,7 @@ struct PosCte {
   unsigned periactive;              // periodic axes, bits 1 X, 2 Y, 4 Z (0 in ensembles)
   double perinc[3][3];              // PeriXinc, PeriYinc, PeriZinc
   double domx, domy, domz;          // DomPosMin (cells of periodic runs)
+  unsigned sim2d;                   // 2-D: matrix motions keep y (0 in ensembles)
 };
 
 // Constant record of one case of an ensemble.

@@ -241,8 +241,6 @@ struct dsph_sim {
     if (Simulate2D) {
       if (c.periactive & 2)
         throw Err(DSPH_E_ARG, "Cannot use periodic conditions in Y with 2D simulations");
-      if (c.casenfloat || c.motion.type >= 2)
-        throw Err(DSPH_E_ARG, "2-D runs are built without floating bodies and rotation motions");
       cs.motion.ampl[1] = 0;
     }
     const double h = KernelH;
@@ -975,6 +973,11 @@ struct dsph_sim {
       fcenter[k] = f.center[k] + dt * f.fvel[k];
       fvel[k] = float(dt * acelin[k] + f.fvel[k]);
       fomega[k] = float(dt * aceang[k] + f.fomega[k]);
+    }
+    if (Simulate2D) {  // in-plane motion only (JSph.cpp:2628-2640)
+      fcenter[1] = f.center[1];
+      fvel[1] = 0;
+      fomega[0] = fomega[2] = 0;
     }
   }
   void run_floating(double dt, bool predictor) {

@@ -472,7 +472,8 @@ __global__ void __launch_bounds__(kBs) k_move_mat_bound(unsigned npb, unsigned i
   const double2 rxy = posxy[p];
   const double rx = rxy.x, ry = rxy.y, rz = posz[p];
   const double x2 = rx * m.a[0] + ry * m.a[1] + rz * m.a[2] + m.a[3];
-  const double y2 = rx * m.a[4] + ry * m.a[5] + rz * m.a[6] + m.a[7];
+  // 2-D: y kept (KerMoveMatBound<.., simulate2d>, JSphGpu_ker.cu:1721).
+  const double y2 = c.sim2d ? ry : rx * m.a[4] + ry * m.a[5] + rz * m.a[6] + m.a[7];
   const double z2 = rx * m.a[8] + ry * m.a[9] + rz * m.a[10] + m.a[11];
   const double dx = x2 - rx, dy = y2 - ry, dz = z2 - rz;
   update_pos(rxy, rz, dx, dy, dz, false, p, c, posxy, posz, dcell, code);
@@ -562,6 +563,7 @@ PosCte make_posc(const dsph_cte& c) {
     r.perinc[0][k] = c.xperinc[k]; r.perinc[1][k] = c.yperinc[k]; r.perinc[2][k] = c.zperinc[k];
   }
   r.domx = c.domposminx; r.domy = c.domposminy; r.domz = c.domposminz;
+  r.sim2d = c.simulate2d;
   return r;
 }
 

@@ -188,9 +188,8 @@ struct Sim {
     DtMin = s.dtmin ? s.dtmin : (double(KernelH) / Cs0) * double(0.05f);
     // 2-D: linear motions lose their y part (JDsMotion.cpp:132).
     // 2-D: no periodic y (JSph.cpp:827).
-    if (c.sim2d && (s.casenfloat || (s.periactive & 2) || s.motion.type >= 2))
-      throw std::runtime_error("2-D runs are built without floating bodies, a periodic y axis "
-                               "and rotations");
+    if (c.sim2d && (s.periactive & 2))
+      throw std::runtime_error("Cannot use periodic conditions in Y with 2D simulations");
     if (c.sim2d) cs.motion.ampl[1] = 0;
     if (TVisco == 2) {
       // dp_sps (JSph.cpp:1529).
@@ -620,6 +619,7 @@ struct Sim {
         const double x[3] = {pos[3 * size_t(p)], pos[3 * size_t(p) + 1], pos[3 * size_t(p) + 2]};
         double x2[3], d[3];
         apply(x, x2);
+        if (c.sim2d) x2[1] = x[1];  // 2-D: y kept (JSphCpu.cpp:2161)
         for (int k = 0; k < 3; k++) d[k] = x2[k] - x[k];
         move_particle(p, d);
         float* v = &velrho[4 * size_t(p)];
@@ -768,6 +768,11 @@ struct Sim {
       fcenter[k] = f.center[k] + dt * f.fvel[k];
       fvel[k] = float(dt * acelin[k] + f.fvel[k]);
       fomega[k] = float(dt * aceang[k] + f.fomega[k]);
+    }
+    if (c.sim2d) {  // in-plane motion only (JSph.cpp:2628-2640)
+      fcenter[1] = f.center[1];
+      fvel[1] = 0;
+      fomega[0] = fomega[2] = 0;
     }
   }
   // FtPartsUpdate (JSphCpuSingle.cpp:1045-1105) without periodic axes and

@@ -145,6 +145,26 @@ CASES = {
     "ref_channel2d_dp0.005_periodic_mdbc_verlet": ("periodic_channel_2d", 0.005, 60, (1, 30, 60),
                                                    dict(periodic=1, tboundary=2, tstep=1,
                                                         tvisco=3, visco=1e-4)),
+    # 2-D dam break with a floating box of half the water density at the
+    # column's surface (in-plane motion: x, z and rotation about y), and the
+    # same box thrown at 1 m/s spinning at 2 rad/s about y.
+    "ref_dambreak2d_dp0.005_floating": ("dam_break_floating_2d", 0.005, 40, (1, 10, 20, 40), {}),
+    "ref_dambreak2d_dp0.005_floating_spin": ("dam_break_floating_2d", 0.005, 40, (1, 10, 20, 40),
+                                             dict(velini=(1.0, 0.0, -0.5),
+                                                  omegaini=(0.0, 2.0, 0.0))),
+    # 2-D flume: piston with mDBC + laminar+SPS; flap (rotation sinusoid,
+    # DBC); constant rotation with mDBC (normals turned with the flap). The
+    # mDBC cases save from step 10: after one step the fluid has moved
+    # 1.6e-7 m (the residual of the hydrostatic start), and the few-ulp
+    # spread of the mDBC boundary densities between FP32 builds is 2e-3 of
+    # that, above the bound; from step 10 on it is below 1e-3.
+    "ref_flume2d_dp0.005_piston_mdbc_sps": ("flume_2d", 0.005, 40, (10, 20, 40),
+                                            dict(tboundary=2, tvisco=2, visco=1e-6)),
+    "ref_flume2d_dp0.005_flap": ("flume_2d", 0.005, 40, (1, 10, 20, 40),
+                                 dict(wavemaker="flap")),
+    "ref_flume2d_dp0.005_flap_rot_mdbc": ("flume_2d", 0.005, 40, (10, 20, 40),
+                                          dict(wavemaker="flap_rot", tboundary=2, tvisco=3,
+                                               visco=1e-4)),
     # Channel periodic in x with the fluid moving at 2 m/s: particles leave
     # through one face and re-enter through the other (Symplectic + DBC, and
     # Verlet + mDBC + laminar viscosity).
@@ -157,7 +177,7 @@ CASES = {
 
 
 # Keys of a CASES entry that are arguments of the case generator, not Parameters.
-GENERATOR_ARGS = ("wavemaker", "vel0")
+GENERATOR_ARGS = ("wavemaker", "vel0", "velini", "omegaini")
 
 
 def run_reference(case, nsteps: int, workdir: Path) -> list[dict]:

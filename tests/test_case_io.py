@@ -26,16 +26,18 @@ def _cases():
             cm.periodic_channel(dp=0.02),
             cm.periodic_channel(dp=0.02, params=cm.Parameters(periodic=3, tboundary=2)),
             cm.periodic_channel(dp=0.02, params=cm.Parameters(
-                periodic=1, perinc=((0.0, 0.0, 0.01), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0))))]
+                periodic=1, perinc=((0.0, 0.0, 0.01), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0)))),
+            cm.dam_break_2d(dp=0.02, posy=0.25),
+            cm.flume_2d(dp=0.01, wavemaker="flap", params=mdbc)]
 
 
-@pytest.mark.parametrize("k", range(8))
+@pytest.mark.parametrize("k", range(10))
 def test_case_round_trip(tmp_path, k):
     c = _cases()[k]
     bi4.write_case(c, tmp_path, "case")
     r = bi4.read_case(tmp_path, "case")
     for f in ("dp", "h", "rhop0", "gamma", "b", "massfluid", "massbound", "gravity", "nbound",
-              "nfluid", "nmoving", "maprealposmin", "maprealposmax"):
+              "nfluid", "nmoving", "maprealposmin", "maprealposmax", "data2d", "data2dposy"):
         assert getattr(r, f) == getattr(c, f), f
     import dataclasses
     assert r.params.timeout == c.params.timemax   # TimeOut 0 is written as TimeMax
@@ -128,16 +130,20 @@ def test_periodic_keys_of_the_case_xml(tmp_path):
 
 
 @pytest.mark.ref
-@pytest.mark.parametrize("kind", ["dam_break", "periodic_channel"])
+@pytest.mark.parametrize("kind", ["dam_break", "periodic_channel", "flume_2d"])
 def test_reference_resumes_from_a_part_written_here(tmp_path, kind):
     """The reference CPU build continued (-partbegin) from a PART + head
     written by write_part / write_part_head (from the data of its own
     Part_0005) gives its own continued run bit for bit, which equals its
     uninterrupted run. The periodic channel checks the periodic header
-    (the reference refuses a PART whose PeriMode differs from the case)."""
+    (the reference refuses a PART whose PeriMode differs from the case); the
+    2-D flume checks the moving block and the 2-D header (the reference
+    refuses a PART whose Data2d or Data2dPosY differs from the case)."""
     if not REF_EXE.exists():
         pytest.skip("reference CPU build absent")
-    c = cm.dam_break(dp=0.04) if kind == "dam_break" else cm.periodic_channel(dp=0.02)
+    c = {"dam_break": lambda: cm.dam_break(dp=0.04),
+         "periodic_channel": lambda: cm.periodic_channel(dp=0.02),
+         "flume_2d": lambda: cm.flume_2d(dp=0.01, posy=0.1)}[kind]()
     bi4.write_case(c, tmp_path / "in", "case", timemax=10.0, timeout=10.0)
 
     def run(out, extra):

@@ -5,7 +5,8 @@ first 10 to 60 steps, from the reference-run fixtures of tests/golden/
 reference and the oracle). The 2-D cases (<data2d>, particles in the plane
 y = 0) cover DBC, mDBC with laminar+SPS, mDBC2 no-slip under Verlet, mDBC2
 free slip with no penetration, shifting (with the free-surface threshold,
-and full) and a channel periodic in x."""
+and full), a channel periodic in x, a floating box (at rest; thrown and
+spinning) and the flume with a piston or a flap."""
 import numpy as np
 import pytest
 
@@ -40,7 +41,12 @@ pytestmark = pytest.mark.gpu
                                   "ref_dambreak2d_dp0.005_shift_nobound_tfs",
                                   "ref_dambreak2d_dp0.005_shift_full",
                                   "ref_channel2d_dp0.005_periodic",
-                                  "ref_channel2d_dp0.005_periodic_mdbc_verlet"])
+                                  "ref_channel2d_dp0.005_periodic_mdbc_verlet",
+                                  "ref_dambreak2d_dp0.005_floating",
+                                  "ref_dambreak2d_dp0.005_floating_spin",
+                                  "ref_flume2d_dp0.005_piston_mdbc_sps",
+                                  "ref_flume2d_dp0.005_flap",
+                                  "ref_flume2d_dp0.005_flap_rot_mdbc"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)

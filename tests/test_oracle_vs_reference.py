@@ -28,7 +28,11 @@ under Verlet and mDBC2 free slip with no penetration, over 40 steps; 2-D
 shifting (no boundaries with the free-surface threshold, whose 2-D factor
 (w - tfs)/(2 - tfs) the fixture pins: the 3-D factor puts the displacement
 3 % off after one step; and full) over 40 steps; a 2-D channel periodic in
-x (DBC; Verlet + mDBC + laminar) over 60 steps.
+x (DBC; Verlet + mDBC + laminar) over 60 steps; the 2-D dam break with a
+floating box at rest, and thrown at 1 m/s spinning at 2 rad/s about y, over
+40 steps (reproduced bit for bit in CPU semantics); the 2-D flume with the
+piston (mDBC + laminar+SPS), the flap (rotation sinusoid, DBC) and the flap
+at a constant rotation (mDBC) over 40 steps.
 """
 import numpy as np
 import pytest
@@ -51,7 +55,10 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_dambreak2d_dp0.005", "ref_dambreak2d_dp0.005_mdbc_sps",
             "ref_dambreak2d_dp0.005_mdbc2_verlet", "ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen",
             "ref_dambreak2d_dp0.005_shift_nobound_tfs", "ref_dambreak2d_dp0.005_shift_full",
-            "ref_channel2d_dp0.005_periodic", "ref_channel2d_dp0.005_periodic_mdbc_verlet"]
+            "ref_channel2d_dp0.005_periodic", "ref_channel2d_dp0.005_periodic_mdbc_verlet",
+            "ref_dambreak2d_dp0.005_floating", "ref_dambreak2d_dp0.005_floating_spin",
+            "ref_flume2d_dp0.005_piston_mdbc_sps", "ref_flume2d_dp0.005_flap",
+            "ref_flume2d_dp0.005_flap_rot_mdbc"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
