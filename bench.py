@@ -227,7 +227,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--dp", type=float, default=0.0025)
+    ap.add_argument("--dp", type=float, default=None,
+                    help="particle spacing (default 0.0025 m in 3-D, 0.0002 m in 2-D)")
+    ap.add_argument("--workload", choices=["dam_break_3d", "dam_break_2d"],
+                    default="dam_break_3d",
+                    help="dam_break_2d: the x-z section of the same tank and column (data2d)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true",
@@ -272,7 +276,10 @@ def main():
     from dualsphysics_amd.sim import SphGpuSingle
 
     t0 = time.perf_counter()
-    c = casemod.dam_break(dp=a.dp, params=casemod.Parameters(
+    if a.dp is None:
+        a.dp = 0.0025 if a.workload == "dam_break_3d" else 0.0002
+    gen = casemod.dam_break if a.workload == "dam_break_3d" else casemod.dam_break_2d
+    c = gen(dp=a.dp, params=casemod.Parameters(
         cellmode=2 if a.cellmode == "half" else 1))
     log(f"case: {c.nfluid} fluid + {c.nbound} boundary particles ({time.perf_counter()-t0:.1f}s)")
     sim = SphGpuSingle(c)
@@ -329,7 +336,9 @@ def main():
             avg_s = ms / cnt / 1e3
             ach = flops / avg_s / 1e12
             traffic = None
-            tfile = ROOT / "profiles" / f"k_forces_fluid_traffic_dam_break_dp{a.dp:g}.json"
+            tfile = ROOT / "profiles" / (f"k_forces_fluid_traffic_dam_break_dp{a.dp:g}.json"
+                                         if a.workload == "dam_break_3d" else
+                                         f"k_forces_fluid_traffic_{a.workload}_dp{a.dp:g}.json")
             if tfile.exists() and a.forces_kernel == "tile":
                 traffic = json.loads(tfile.read_text()).get("traffic_bytes_per_launch")
             roof = {"bound": "valu", "kernel": "k_forces_fluid", "achieved": round(ach, 3),
@@ -363,7 +372,7 @@ def main():
         "warmup": a.warmup, "ms_per_step": ms_step, "steps_per_s": 1e3 / ms_step,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"dam_break_3d_dp{a.dp:g}", "n_fluid": c.nfluid,
+        "config": {"workload": f"{a.workload}_dp{a.dp:g}", "n_fluid": c.nfluid,
                    "n_bound": c.nbound, "dp": a.dp, "kernel": "wendland", "forces_kernel": a.forces_kernel,
                    "cellmode": a.cellmode,
                    "visco": "artificial 0.01", "ddt": "fourtakas_full 0.1",
