@@ -63,7 +63,8 @@ class DsphInterParms(C.Structure):
                 ("d_velgrad", C.c_void_p), ("d_boundmode", C.c_void_p),
                 ("d_tangenvel", C.c_void_p), ("shiftmode", C.c_int32),
                 ("d_shiftposfs", C.c_void_p), ("d_ftomassp", C.c_void_p),
-                ("d_boundnor", C.c_void_p), ("d_nopenshift", C.c_void_p)]
+                ("d_boundnor", C.c_void_p), ("d_nopenshift", C.c_void_p),
+                ("d_motionvel", C.c_void_p)]
 
 
 class DsphMotion(C.Structure):
@@ -172,6 +173,7 @@ SIGNATURES = {
     "dsph_mdbc2_correction_slip": [_vp, _i, _u, C.POINTER(_f), C.POINTER(DsphDivData),
                                    C.POINTER(_d)] + [_vp] * 10,
     "dsph_mdbc2_nopen_step": [_vp, _u, _u, _vp, _vp, _d, _vp, _u, _vp, _vp, _vp],
+    "dsph_mdbc2_motion_vel_ace": [_vp, _u, _u, _u, _d, _vp, _u, _vp, _vp, _vp, _vp],
     "dsph_velmax2": [_vp, _u, _u, _vp, _vp],
     "dsph_acemax2": [_vp, _u, _u, _vp, _vp],
     "dsph_acemax2_normal": [_vp, _u, _u, _vp, _vp, _vp, _vp],

@@ -20,31 +20,16 @@ namespace {
 constexpr unsigned kBs = 256;
 
 // ------------------------------------------------------------- cell limits
-// Min/max cell (x,y,z) over non-excluded particles, combined with 32-bit
-// integer atomics (exact and order independent). res[0..2]=min, res[3..5]=max.
-__global__ void k_limits_init(unsigned* res) {
-  const unsigned t = threadIdx.x;
-  if (t < 3) res[t] = UINT_MAX;
-  else if (t < 6) res[t] = 0u;
-}
-
-__global__ void __launch_bounds__(kBs) k_limits(unsigned n, unsigned pini, unsigned cellcode,
-                                                const unsigned* __restrict__ dcell,
-                                                const typecode* __restrict__ code,
-                                                unsigned* __restrict__ res) {
+// Min/max cell (x,y,z) over non-excluded particles (exact and order
+// independent): each block writes its six extremes to part[6 block + k]
+// (k 0..2 min, 3..5 max), one block combines them into res[0..5]. Partials
+// instead of global atomics: thousands of atomics on six words serialise in
+// L2 and cost ten times the 6 bytes per particle that are read.
+// Block extremes: on return, threads 0..2 hold min (a) and max (b) of axis
+// threadIdx.x over the block.
+__device__ inline void limits_block_reduce(unsigned mn[3], unsigned mx[3], unsigned& a,
+                                           unsigned& b) {
   __shared__ unsigned smin[3][kBs / 64], smax[3][kBs / 64];
-  unsigned mn[3] = {UINT_MAX, UINT_MAX, UINT_MAX}, mx[3] = {0u, 0u, 0u};
-  // Grid-stride: a bounded grid keeps the number of global atomics small.
-  for (unsigned p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
-    const unsigned pp = p + pini;
-    if (code_special(code[pp]) < kCodeOutIgnore) {
-      const unsigned c = dcell[pp];
-      const unsigned cx = dcel_cellx(cellcode, c), cy = dcel_celly(cellcode, c),
-                     cz = dcel_cellz(cellcode, c);
-      mn[0] = min(mn[0], cx); mn[1] = min(mn[1], cy); mn[2] = min(mn[2], cz);
-      mx[0] = max(mx[0], cx); mx[1] = max(mx[1], cy); mx[2] = max(mx[2], cz);
-    }
-  }
   for (int k = 0; k < 3; k++) {
     for (int off = 32; off > 0; off >>= 1) {
       mn[k] = min(mn[k], (unsigned)__shfl_xor((int)mn[k], off));
@@ -55,14 +40,52 @@ __global__ void __launch_bounds__(kBs) k_limits(unsigned n, unsigned pini, unsig
   if (lane == 0)
     for (int k = 0; k < 3; k++) { smin[k][w] = mn[k]; smax[k][w] = mx[k]; }
   __syncthreads();
-  if (threadIdx.x < 3) {
-    unsigned a = UINT_MAX, b = 0;
+  a = UINT_MAX;
+  b = 0;
+  if (threadIdx.x < 3)
     for (unsigned i = 0; i < blockDim.x / 64; i++) {
       a = min(a, smin[threadIdx.x][i]);
       b = max(b, smax[threadIdx.x][i]);
     }
-    if (a != UINT_MAX) atomicMin(res + threadIdx.x, a);
-    if (b != 0u) atomicMax(res + 3 + threadIdx.x, b);
+}
+
+__global__ void __launch_bounds__(kBs) k_limits_final(unsigned nparts,
+                                                      const unsigned* __restrict__ part,
+                                                      unsigned* __restrict__ res) {
+  unsigned mn[3] = {UINT_MAX, UINT_MAX, UINT_MAX}, mx[3] = {0u, 0u, 0u};
+  for (unsigned b = threadIdx.x; b < nparts; b += blockDim.x)
+    for (int k = 0; k < 3; k++) {
+      mn[k] = min(mn[k], part[6 * b + k]);
+      mx[k] = max(mx[k], part[6 * b + 3 + k]);
+    }
+  unsigned a, b;
+  limits_block_reduce(mn, mx, a, b);
+  if (threadIdx.x < 3) {
+    res[threadIdx.x] = a;
+    res[3 + threadIdx.x] = b;
+  }
+}
+
+__global__ void __launch_bounds__(kBs) k_limits(unsigned n, unsigned pini, unsigned cellcode,
+                                                const unsigned* __restrict__ dcell,
+                                                const typecode* __restrict__ code,
+                                                unsigned* __restrict__ part) {
+  unsigned mn[3] = {UINT_MAX, UINT_MAX, UINT_MAX}, mx[3] = {0u, 0u, 0u};
+  for (unsigned p = blockIdx.x * blockDim.x + threadIdx.x; p < n; p += gridDim.x * blockDim.x) {
+    const unsigned pp = p + pini;
+    if (code_special(code[pp]) < kCodeOutIgnore) {
+      const unsigned c = dcell[pp];
+      const unsigned cx = dcel_cellx(cellcode, c), cy = dcel_celly(cellcode, c),
+                     cz = dcel_cellz(cellcode, c);
+      mn[0] = min(mn[0], cx); mn[1] = min(mn[1], cy); mn[2] = min(mn[2], cz);
+      mx[0] = max(mx[0], cx); mx[1] = max(mx[1], cy); mx[2] = max(mx[2], cz);
+    }
+  }
+  unsigned a, b;
+  limits_block_reduce(mn, mx, a, b);
+  if (threadIdx.x < 3) {
+    part[6 * blockIdx.x + threadIdx.x] = a;
+    part[6 * blockIdx.x + 3 + threadIdx.x] = b;
   }
 }
 
@@ -682,12 +705,17 @@ int dsph_cell_limits_sync(dsph_ctx* ctx, uint32_t np, uint32_t pini, uint32_t ce
     return DSPH_OK;
   }
   DSPH_REQUIRE(ctx, d_dcell && d_code, "dsph_cell_limits_sync: null array");
+  // Block partials in the reduction scratch (>= 256 words after dsph_ctx_reserve).
+  if (ctx->redu_cap < 6)
+    return set_error(ctx, DSPH_E_NOMEM, "dsph_cell_limits_sync: call dsph_ctx_reserve first");
   unsigned* res = ctx->small_dev;
+  unsigned* part = reinterpret_cast<unsigned*>(ctx->redu);
+  const unsigned nparts = std::min<unsigned>(
+      std::min(grid_for(np, kBs), 1024u), unsigned(std::min<size_t>(ctx->redu_cap / 6, 1024)));
   {
     KTimer kt(ctx, "k_limits");
-    k_limits_init<<<1, 64, 0, ctx->stream>>>(res);
-    k_limits<<<std::min(grid_for(np, kBs), 2048u), kBs, 0, ctx->stream>>>(np, pini, cellcode,
-                                                                         d_dcell, d_code, res);
+    k_limits<<<nparts, kBs, 0, ctx->stream>>>(np, pini, cellcode, d_dcell, d_code, part);
+    k_limits_final<<<1, kBs, 0, ctx->stream>>>(nparts, part, res);
     DSPH_LAUNCHED(ctx, "k_limits");
   }
   unsigned* h = static_cast<unsigned*>(ctx->pinned);

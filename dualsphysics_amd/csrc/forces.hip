@@ -114,8 +114,10 @@ struct P1 {
 // takes no part in the Fourtakas term and gives no SPS tau; a floating p1
 // has no density diffusion and no strain (JSphGpu_ker.cu:570-583, 668,
 // 755-766, 838-843).
-// NOPEN: mDBC2 no-penetration sums against fixed boundary neighbours
-// (MDBC2_NoPen, JSphGpu_ker.cu:686-707 with ComputeNoPenVel, :428-440).
+// NOPEN: mDBC2 no-penetration sums against boundary neighbours, with the
+// velocity relative to their motion velocity (motionvel, float3 per
+// particle; NULL = fixed boundaries) (MDBC2_NoPen, JSphGpu_ker.cu:686-707
+// with ComputeNoPenVel, :428-440).
 template <int TVISCO, int TDDT, bool VELGRAD, bool MDBC2 = false, bool SHIFT = false,
           bool FT = false, bool NOPEN = false>
 __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float4& ps2,
@@ -123,7 +125,7 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
     float viscof, const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
     const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr,
-    const float* __restrict__ boundnor = nullptr) {
+    const float* __restrict__ boundnor = nullptr, const float* __restrict__ motionvel = nullptr) {
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
@@ -181,14 +183,16 @@ __device__ __forceinline__ void pair_eval(bool boundp2, unsigned p2, const float
   if (NOPEN && boundp2) {
     // A boundary neighbour of the first layer (|normal| < 1.75 dp) within
     // 1.25 dp, with the particle less than 0.75 |normal| away along the
-    // normal: per axis, a particle moving towards the wall (fixed walls:
-    // motion velocity 0) gets the correction -(3 - 4 max(|dr/n|, 1/4)) v n^2.
+    // normal: per axis, a particle moving towards the wall (v relative to
+    // the wall's motion velocity) gets -(3 - 4 max(|dr/n|, 1/4)) v n^2.
     if (sqrtf(rr2) < 1.25f * c.dp) {
       const float* bn = boundnor + 3 * size_t(p2);
       const float norm = sqrtf(bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]);
       const float nrm[3] = {fast_div(bn[0], norm), fast_div(bn[1], norm), fast_div(bn[2], norm)};
       const float d[3] = {dr.x, dr.y, dr.z};
-      const float v[3] = {q.vr.x, q.vr.y, q.vr.z};
+      const float* mv = motionvel ? motionvel + 3 * size_t(p2) : nullptr;
+      const float v[3] = {mv ? q.vr.x - mv[0] : q.vr.x, mv ? q.vr.y - mv[1] : q.vr.y,
+                          mv ? q.vr.z - mv[2] : q.vr.z};
       if (nrm[0] * dr.x + nrm[1] * dr.y + nrm[2] * dr.z < 0.75f * norm && norm < 1.75f * c.dp) {
 #pragma unroll
         for (int k = 0; k < 3; k++) {
@@ -259,14 +263,14 @@ __device__ __forceinline__ void pair_body(bool boundp2, unsigned p2, const P1& q
     const Cte& c, Acc& a, const uint8_t* __restrict__ bmode = nullptr,
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
     const typecode* __restrict__ code = nullptr, const float* __restrict__ ftomassp = nullptr,
-    const float* __restrict__ boundnor = nullptr) {
+    const float* __restrict__ boundnor = nullptr, const float* __restrict__ motionvel = nullptr) {
   const float4 ps2 = poscell[p2];
   const float3 dr = pscel_dr(q.ps, ps2, c.pcs);
   const float rr2 = dr.x * dr.x + dr.y * dr.y + dr.z * dr.z;
   if (!(rr2 <= c.ks2 && rr2 >= kAlmostZero)) return;
   pair_eval<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT, NOPEN>(
       boundp2, p2, ps2, velrho[p2], press[p2], q, tau, viscob, viscof, c, a, bmode, tangenvel,
-      shiftmode, code, ftomassp, boundnor);
+      shiftmode, code, ftomassp, boundnor, motionvel);
 }
 
 template <int TVISCO>
@@ -329,7 +333,7 @@ __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float
     const float* __restrict__ tangenvel = nullptr, int shiftmode = 0,
     const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr,
     const float* __restrict__ ftomassp = nullptr, const float* __restrict__ boundnor = nullptr,
-    float4* __restrict__ nopenshift = nullptr) {
+    float4* __restrict__ nopenshift = nullptr, const float* __restrict__ motionvel = nullptr) {
   Acc a;
   acc_init(a);
   if (SHIFT) a.sh = shiftposfs[p1];
@@ -349,7 +353,7 @@ __device__ __forceinline__ void fluid_rows_body(unsigned p1, float viscob, float
         for (unsigned p2 = pini; p2 < pfin; p2++)
           pair_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT, NOPEN>(
               kind == 1, p2, q, poscell, velrho, press, tau, viscob, viscof, c, a, bmode,
-              tangenvel, shiftmode, code, ftomassp, boundnor);
+              tangenvel, shiftmode, code, ftomassp, boundnor, motionvel);
       }
   }
   store_results<TVISCO, TDDT, VELGRAD>(p1, a, viscdt, ar, ace, strain, velgrad);
@@ -376,13 +380,13 @@ __global__ void __launch_bounds__(kBsForces) k_forces_fluid_rows(
     const uint8_t* __restrict__ bmode, const float* __restrict__ tangenvel, int shiftmode = 0,
     const typecode* __restrict__ code = nullptr, float4* __restrict__ shiftposfs = nullptr,
     const float* __restrict__ ftomassp = nullptr, const float* __restrict__ boundnor = nullptr,
-    float4* __restrict__ nopenshift = nullptr) {
+    float4* __restrict__ nopenshift = nullptr, const float* __restrict__ motionvel = nullptr) {
   const unsigned p = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (p >= n) return;
   fluid_rows_body<TVISCO, TDDT, VELGRAD, MDBC2, SHIFT, FT, NOPEN>(
       p + pinit, viscob, viscof, dvd, c, dcell, poscell, velrho, press, tau, viscdt, ar, ace,
       strain, velgrad, bmode, tangenvel, shiftmode, code, shiftposfs, ftomassp, boundnor,
-      nopenshift);
+      nopenshift, motionvel);
 }
 
 // Boundary particle against one fluid neighbour that passed the neighbour
@@ -833,7 +837,7 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
             npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
             (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
             t->d_sps2strain, t->d_velgrad, t->d_boundmode, t->d_tangenvel, 0, nullptr, nullptr,
-            nullptr, t->d_boundnor, (float4*)t->d_nopenshift);
+            nullptr, t->d_boundnor, (float4*)t->d_nopenshift, t->d_motionvel);
   } else if (t->d_ftomassp) {
     // Floating bodies: the one-lane-per-particle kernel with the floating
     // rules (shifting, mDBC2 and the velocity gradient are refused with them

@@ -114,6 +114,9 @@ struct dsph_sim {
   // rewritten by every correction (JSphGpuSingle.cpp:700-707).
   uint8_t* boundmode = nullptr;
   float* tangenvel = nullptr;
+  // mDBC2 with moving boundaries: motion velocity and acceleration per
+  // particle (MotionVel_g, MotionAce_g; zero at the start, JSphGpu.cpp:834-835).
+  float *motionvel = nullptr, *motionvel2 = nullptr, *motionace = nullptr, *motionace2 = nullptr;
   // mDBC2 no penetration (MDBC2_NoPen): float4 per particle written by every
   // interaction for the fluid, used by the corrector / Verlet step.
   float* nopenshift = nullptr;
@@ -183,6 +186,7 @@ struct dsph_sim {
                     velrho2, poscell, posxypre, poszpre, posxypre2, poszpre2, velrhopre,
                     velrhopre2, velrhom1, velrhom1b, ace, ar, viscdt, movxy, movz, sps2strain,
                     spstau, spstau2, boundnor, boundnor2, boundmode, tangenvel, nopenshift, shiftposfs,
+                    motionvel, motionvel2, motionace, motionace2,
                     d_red, d_dt,
                     d_flags, cellpart,
                     sortpart, begincell, velgrad, closure_nu, d_ftomassp, d_ftbody,
@@ -212,8 +216,6 @@ struct dsph_sim {
     SlipMode = c.slipmode ? c.slipmode : DSPH_SLIP_VEL0;
     if (TBoundary == DSPH_BC_MDBC && (SlipMode < DSPH_SLIP_VEL0 || SlipMode > DSPH_SLIP_FREESLIP))
       throw Err(DSPH_E_ARG, "SlipMode: 1 (vel=0), 2 (no-slip) or 3 (free slip)");
-    if (mdbc2_on() && c.casenmoving)
-      throw Err(DSPH_E_ARG, "mDBC no-slip and free slip are built for fixed boundaries only");
     // NoPenetration is read for SlipMode >= no-slip only (JSph.cpp:715-716).
     NoPen = mdbc2_on() && c.nopenetration != 0;
     ShiftMode = c.shiftmode;
@@ -441,6 +443,10 @@ struct dsph_sim {
         boundmode = alloc<uint8_t>(n);
         tangenvel = alloc<float>(3 * n);
         if (NoPen) nopenshift = alloc<float>(4 * n);
+        if (cs.casenmoving) {
+          motionvel = alloc<float>(3 * n); motionvel2 = alloc<float>(3 * n);
+          motionace = alloc<float>(3 * n); motionace2 = alloc<float>(3 * n);
+        }
       }
     }
     d_red = alloc<float>(4);
@@ -475,6 +481,10 @@ struct dsph_sim {
     chk(dsph_memcpy_h2d(ctx, velrho, hvr, 16 * n0), "upload velrho");
     if (velrhom1) chk(dsph_memcpy_h2d(ctx, velrhom1, hvr, 16 * n0), "upload velrhom1");
     if (spstau) chk(dsph_memset_async(ctx, spstau, 0, 24 * n), "memset tau");
+    if (motionvel) {
+      chk(dsph_memset_async(ctx, motionvel, 0, 12 * n), "memset motionvel");
+      chk(dsph_memset_async(ctx, motionace, 0, 12 * n), "memset motionace");
+    }
     if (boundnor) {
       // ConfigBoundNormals: normals are stored doubled (JSph.cpp:1385-1434).
       // hbn is indexed by id (boundary ids 0..CaseNbound-1); the particles
@@ -716,6 +726,7 @@ struct dsph_sim {
     regrow(sps2strain, 6); regrow(spstau, 6); regrow(spstau2, 6); regrow(shiftposfs, 4);
     regrow(boundnor, 3); regrow(boundnor2, 3); regrow(boundmode, 1); regrow(tangenvel, 3);
     regrow(nopenshift, 4);
+    regrow(motionvel, 3); regrow(motionvel2, 3); regrow(motionace, 3); regrow(motionace2, 3);
     regrow(velgrad, 9); regrow(closure_nu, 1);
     if (periwork) {
       chk(dsph_free(ctx, periwork), "grow free");
@@ -753,6 +764,12 @@ struct dsph_sim {
       chk(dsph_sort_data_f3(ctx, nptot, pini, sortpart, boundnor, boundnor2), "sort boundnor");
       swp(boundnor, boundnor2);
     }
+    if (motionvel) {
+      chk(dsph_sort_data_f3(ctx, nptot, pini, sortpart, motionvel, motionvel2), "sort motionvel");
+      swp(motionvel, motionvel2);
+      chk(dsph_sort_data_f3(ctx, nptot, pini, sortpart, motionace, motionace2), "sort motionace");
+      swp(motionace, motionace2);
+    }
     if (NpbOut) throw Err(DSPH_E_NUMERIC, "boundary particles excluded (AbortBoundOut)");
     Np = NpFinal;
     Npb = NpbFinal;
@@ -783,8 +800,8 @@ struct dsph_sim {
     if (mdbc2_on()) {
       chk(dsph_memset_async(ctx, boundmode, DSPH_BMODE_DBC, Npb), "memset boundmode");
       chk(dsph_mdbc2_correction_slip(ctx, SlipMode, Npb, Gravity, &d, Map_PosMin, posxy, posz,
-                                     poscell, code, boundnor, nullptr, nullptr, velrho, boundmode,
-                                     tangenvel),
+                                     poscell, code, boundnor, motionvel, motionace, velrho,
+                                     boundmode, tangenvel),
           "mdbc2");
       return;
     }
@@ -813,6 +830,7 @@ struct dsph_sim {
     t.d_shiftposfs = shiftposfs;
     t.d_ftomassp = d_ftomassp;
     t.d_boundnor = NoPen ? boundnor : nullptr;
+    t.d_motionvel = NoPen ? motionvel : nullptr;
     t.d_nopenshift = nopenshift;
     chk(dsph_interaction_forces(ctx, &t), "interaction");
     if (TVisco == DSPH_VISCO_LAMINARSPS) {
@@ -897,6 +915,16 @@ struct dsph_sim {
   void run_motion(double dt) {
     const dsph_motion& m = cs.motion;
     if (!cs.casenmoving || m.type == 0) return;
+    move_boundaries(dt);
+    // mDBC2: motion velocity and acceleration of the moving particles,
+    // whether the motion is active or not (JSphGpu.cpp:1219-1223).
+    if (motionvel)
+      chk(dsph_mdbc2_motion_vel_ace(ctx, Npb, m.idbegin, m.count, dt, device_dt ? d_dt : nullptr,
+                                    2, idp, velrho, motionvel, motionace),
+          "motion vel ace");
+  }
+  void move_boundaries(double dt) {
+    const dsph_motion& m = cs.motion;
     if (device_dt) {
       chk(dsph_move_motion_device(ctx, Npb, &m, 0.0, d_dt, idp, posxy, posz, dcell, velrho,
                                   code, boundnor),

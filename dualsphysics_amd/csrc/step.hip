@@ -319,6 +319,26 @@ __global__ void __launch_bounds__(kBs) k_nopen_step(unsigned np, unsigned npb,
   movxy[p] = m;
 }
 
+// mDBC2 with moving boundaries (KerCopyMotionVelAce,
+// JSphGpu_mdbc_iker.cu:587-603): each moving particle (boundary p with
+// idp - idbegin < count) gets motion acceleration float((v - motionvel)/dt)
+// in FP64 and motion velocity v, after the step's motion.
+__global__ void __launch_bounds__(kBs) k_motion_vel_ace(unsigned npb, unsigned idbegin,
+    unsigned count, double dt_h, const double* __restrict__ d_dt, unsigned slot,
+    const unsigned* __restrict__ idp, const float4* __restrict__ velrho,
+    float* __restrict__ motionvel, float* __restrict__ motionace) {
+  const unsigned p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= npb || idp[p] - idbegin >= count) return;
+  const double dt = dev_dt(dt_h, d_dt, slot, 1.0);
+  const float4 v = velrho[p];
+  float* mv = motionvel + 3 * size_t(p);
+  float* ma = motionace + 3 * size_t(p);
+  ma[0] = float((double(v.x) - mv[0]) / dt);
+  ma[1] = float((double(v.y) - mv[1]) / dt);
+  ma[2] = float((double(v.z) - mv[2]) / dt);
+  mv[0] = v.x; mv[1] = v.y; mv[2] = v.z;
+}
+
 // ---------------------------------------------------------------- positions
 
 // KerUpdatePos (JSphGpu_ker.cu:334-392). With periodic axes a particle
@@ -992,6 +1012,23 @@ int dsph_mdbc2_nopen_step(dsph_ctx* ctx, uint32_t np, uint32_t npb,
       np, npb, (const float4*)d_nopenshift, (const float4*)d_velbase, dt, d_dt, dt_slot,
       (float4*)d_velrho, (double2*)d_movxy, d_movz);
   DSPH_LAUNCHED(ctx, "k_nopen_step");
+  return DSPH_OK;
+}
+
+int dsph_mdbc2_motion_vel_ace(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin, uint32_t count,
+                              double dt, const double* d_dt, uint32_t dt_slot,
+                              const uint32_t* d_idp, const float* d_velrho, float* d_motionvel,
+                              float* d_motionace) {
+  if (!ctx) return DSPH_E_ARG;
+  if (!npb || !count) return DSPH_OK;
+  DSPH_REQUIRE(ctx, d_idp && d_velrho && d_motionvel && d_motionace,
+               "dsph_mdbc2_motion_vel_ace: null array");
+  DSPH_REQUIRE(ctx, d_dt || dt > 0, "dsph_mdbc2_motion_vel_ace: dt must be > 0");
+  KTimer kt(ctx, "k_motion_vel_ace");
+  k_motion_vel_ace<<<grid_for(npb, kBs), kBs, 0, ctx->stream>>>(
+      npb, idbegin, count, dt, d_dt, dt_slot, d_idp, (const float4*)d_velrho, d_motionvel,
+      d_motionace);
+  DSPH_LAUNCHED(ctx, "k_motion_vel_ace");
   return DSPH_OK;
 }
 

@@ -171,6 +171,11 @@ typedef struct dsph_inter_parms {
                                     written for every fluid particle: velocity
                                     correction per axis, w = 10 when any applies
                                     (NoPenShift of StInterParmsg), else NULL */
+  const float* d_motionvel;      /* mDBC2 NoPenetration with moving boundaries:
+                                    float3[np] motion velocity of the boundary
+                                    particles (MotionVel of StInterParmsg), the
+                                    no-penetration test uses the velocity relative
+                                    to it; NULL = fixed boundaries */
 } dsph_inter_parms;
 
 /* ---------------------------------------------------------------- context */
@@ -470,6 +475,19 @@ int dsph_mdbc2_nopen_step(dsph_ctx* ctx, uint32_t np, uint32_t npb,
                           const float* d_nopenshift, const float* d_velbase, double dt,
                           const double* d_dt, uint32_t dt_slot, float* d_velrho,
                           double* d_movxy, double* d_movz);
+
+/* Replaces cusph::CopyMotionVelAce (JSphGpu_mdbc_iker.cu:587-613; called at
+ * the end of JSphGpu::RunMotion, JSphGpu.cpp:1219-1223) for mDBC2 with moving
+ * boundaries: the moving particles among the boundary particles [0,npb)
+ * (idp in [idbegin, idbegin+count)) get d_motionace = float((v -
+ * d_motionvel)/dt) in FP64 and d_motionvel = v (float3[npb] each, zero at
+ * the start, reordered with the particles). Run after the step's motion,
+ * whether the motion was active or not. dt = d_dt[dt_slot] when d_dt is not
+ * NULL, else dt. */
+int dsph_mdbc2_motion_vel_ace(dsph_ctx* ctx, uint32_t npb, uint32_t idbegin, uint32_t count,
+                              double dt, const double* d_dt, uint32_t dt_slot,
+                              const uint32_t* d_idp, const float* d_velrho, float* d_motionvel,
+                              float* d_motionace);
 
 /* Replaces cusph::Interaction_GaugeVel and the mass sums of
  * Interaction_GaugeSwl (JDsGauge_ker.cu:50-245) for n points at once

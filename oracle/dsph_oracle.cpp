@@ -232,6 +232,9 @@ struct Opts {
   // particle) and the per-particle output (float4); NULL = off.
   const float* boundnor = nullptr;
   float* nopenshift = nullptr;
+  // Motion velocity of the boundary particles (MotionVel, float3 per
+  // particle) for the no-penetration sums; NULL = fixed boundaries (0).
+  const float* motionvel = nullptr;
   // Particles whose forces are computed: fluid [fbeg, fend) and boundary
   // [bbeg, bend), clipped to [npb, np) and [0, npbok); the default is all.
   long fbeg = 0, fend = LONG_MAX, bbeg = 0, bend = LONG_MAX;
@@ -380,8 +383,9 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
         // JSphGpu_ker.cu:686-707): a first-layer boundary neighbour
         // (|normal| < 1.75 dp) within 1.25 dp with the particle less than
         // 0.75 |normal| away along the normal; per axis, a particle moving
-        // towards the wall (fixed walls: motion velocity 0) counts it and
-        // gets -(3 - 4 max(|dr/n|, 1/4)) v n^2. FP32, as both builds.
+        // towards the wall (velocity relative to the wall's motion velocity)
+        // counts it and gets -(3 - 4 max(|dr/n|, 1/4)) v n^2. FP32, as both
+        // builds.
         const float fdx = float(dx), fdy = float(dy), fdz = float(dz);
         const float rrmag = std::sqrt(fdx * fdx + fdy * fdy + fdz * fdz);
         if (rrmag < 1.25f * c.dp) {
@@ -389,7 +393,10 @@ void fluid_rows(const Opts& o, const orc_cte& c, bool boundp2, const Row* rows, 
           const float norm = std::sqrt(bn[0] * bn[0] + bn[1] * bn[1] + bn[2] * bn[2]);
           const float n[3] = {bn[0] / norm, bn[1] / norm, bn[2] / norm};
           const float dr[3] = {fdx, fdy, fdz};
-          const float v[3] = {v1[0], v1[1], v1[2]};
+          const float* mv = o.motionvel ? o.motionvel + 3 * size_t(p2) : nullptr;
+          const float v[3] = {mv ? float(v1[0]) - mv[0] : float(v1[0]),
+                              mv ? float(v1[1]) - mv[1] : float(v1[1]),
+                              mv ? float(v1[2]) - mv[2] : float(v1[2])};
           const float normdist = n[0] * fdx + n[1] * fdy + n[2] * fdz;
           if (normdist < 0.75f * norm && norm < 1.75f * c.dp) {
             for (int k = 0; k < 3; k++) {
@@ -698,6 +705,28 @@ void orc_interaction_np(int prec, int sem, int tvisco, int tddt, float viscob, f
 // The same for a subset of the particles: fluid p1 in [fbeg, fend) and
 // boundary p1 in [bbeg, bend) (the rest of the outputs is left alone), so
 // that a large state can be checked on a sample of its particles.
+void orc_interaction_range_mv(int prec, int sem, int tvisco, int tddt, float viscob,
+                              float viscof, const orc_cte* c, const orc_div* d, unsigned np,
+                              unsigned npb, unsigned npbok, const unsigned* dcell,
+                              const float* poscell, const double* pos, const float* velrho,
+                              const float* tau, float* ace, float* ar, float* viscdt,
+                              float* strain, float* velgrad, const unsigned char* bmode,
+                              const float* tangenvel, int shiftmode, const typecode* code,
+                              float* shiftposfs, const float* ftmassp, const float* boundnor,
+                              float* nopenshift, const float* motionvel, long fbeg, long fend,
+                              long bbeg, long bend) {
+  Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof, bmode, tangenvel,
+         shiftposfs ? shiftmode : 0, code, ftmassp && code ? ftmassp : nullptr,
+         nopenshift ? boundnor : nullptr, boundnor ? nopenshift : nullptr,
+         boundnor && nopenshift ? motionvel : nullptr, fbeg, fend, bbeg, bend};
+  if (prec == 0)
+    interaction<float>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
+                       viscdt, strain, velgrad, o.shiftmode ? shiftposfs : nullptr);
+  else
+    interaction<double>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
+                        viscdt, strain, velgrad, o.shiftmode ? shiftposfs : nullptr);
+}
+
 void orc_interaction_range(int prec, int sem, int tvisco, int tddt, float viscob, float viscof,
                            const orc_cte* c, const orc_div* d, unsigned np, unsigned npb,
                            unsigned npbok, const unsigned* dcell, const float* poscell,
@@ -707,16 +736,30 @@ void orc_interaction_range(int prec, int sem, int tvisco, int tddt, float viscob
                            const typecode* code, float* shiftposfs, const float* ftmassp,
                            const float* boundnor, float* nopenshift, long fbeg, long fend,
                            long bbeg, long bend) {
-  Opts o{sem, tvisco, tddt, velgrad != nullptr, viscob, viscof, bmode, tangenvel,
-         shiftposfs ? shiftmode : 0, code, ftmassp && code ? ftmassp : nullptr,
-         nopenshift ? boundnor : nullptr, boundnor ? nopenshift : nullptr,
-         fbeg, fend, bbeg, bend};
-  if (prec == 0)
-    interaction<float>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
-                       viscdt, strain, velgrad, o.shiftmode ? shiftposfs : nullptr);
-  else
-    interaction<double>(o, *c, *d, np, npb, npbok, dcell, poscell, pos, velrho, tau, ace, ar,
-                        viscdt, strain, velgrad, o.shiftmode ? shiftposfs : nullptr);
+  orc_interaction_range_mv(prec, sem, tvisco, tddt, viscob, viscof, c, d, np, npb, npbok, dcell,
+                           poscell, pos, velrho, tau, ace, ar, viscdt, strain, velgrad, bmode,
+                           tangenvel, shiftmode, code, shiftposfs, ftmassp, boundnor, nopenshift,
+                           nullptr, fbeg, fend, bbeg, bend);
+}
+
+// mDBC2 with moving boundaries (JSph*::RunMotion, CopyMotionVelAce,
+// JSphCpu_mdbc.cpp:472-490, JSphGpu_mdbc_iker.cu:587-603): for the boundary
+// particles [0, npb) with ids in [idbegin, idbegin+count), motion
+// acceleration float((v - motionvel)/dt) in FP64 and motion velocity v, after
+// the motion of the step.
+void orc_motion_vel_ace(unsigned npb, unsigned idbegin, unsigned count, double dt,
+                        const unsigned* idp, const float* velrho, float* motionvel,
+                        float* motionace) {
+  for (unsigned p = 0; p < npb; p++) {
+    if (idp[p] - idbegin >= count) continue;
+    const float* v = velrho + 4 * size_t(p);
+    float* mv = motionvel + 3 * size_t(p);
+    float* ma = motionace + 3 * size_t(p);
+    for (int k = 0; k < 3; k++) {
+      ma[k] = float((double(v[k]) - mv[k]) / dt);
+      mv[k] = v[k];
+    }
+  }
 }
 
 }  // extern "C"

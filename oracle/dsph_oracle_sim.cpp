@@ -77,6 +77,14 @@ void orc_interaction_np(int, int, int, int, float, float, const orc_cte*, const 
                         float*, float*, const unsigned char*, const float*, int,
                         const typecode*, float*, const float*, const float*, float*);
 void orc_nopen_step(unsigned, unsigned, const float*, const float*, double, float*, double*);
+void orc_interaction_range_mv(int, int, int, int, float, float, const orc_cte*, const orc_div*,
+                              unsigned, unsigned, unsigned, const unsigned*, const float*,
+                              const double*, const float*, const float*, float*, float*, float*,
+                              float*, float*, const unsigned char*, const float*, int,
+                              const typecode*, float*, const float*, const float*, float*,
+                              const float*, long, long, long, long);
+void orc_motion_vel_ace(unsigned, unsigned, unsigned, double, const unsigned*, const float*,
+                        float*, float*);
 void orc_interaction_ext(int, int, int, int, float, float, const orc_cte*, const orc_div*,
                          unsigned, unsigned, unsigned, const unsigned*, const float*,
                          const double*, const float*, const float*, float*, float*, float*,
@@ -122,6 +130,9 @@ struct Sim {
   std::vector<float> shiftposfs;     // shifting float4 per particle
   std::vector<unsigned char> bmode;  // mDBC2 boundary modes of [0,Npb)
   std::vector<float> tangenvel;      // mDBC2 tangential velocities of [0,Npb)
+  // mDBC2 with moving boundaries: motion velocity and acceleration per
+  // particle (MotionVel, MotionAce), zero at the start (JSphCpu.cpp:476).
+  std::vector<float> motionvel, motionace;
   bool pre_active = false;
   // cell division
   std::vector<unsigned> cellpart, sortpart;
@@ -178,8 +189,6 @@ struct Sim {
     ShiftMode = s.shiftmode;
     ShiftCoef = float(s.shiftcoef ? s.shiftcoef : -2.0);
     ShiftTFS = float(s.shifttfs);
-    if (TBoundary == 2 && SlipMode >= 2 && s.casenmoving)
-      throw std::runtime_error("mDBC no-slip and free slip are built for fixed boundaries only");
     VerletSteps = s.verletsteps ? s.verletsteps : 40;
     Visco = float(s.visco); ViscoB = Visco * float(s.viscoboundfactor);
     RhopOutMin = float(s.rhopoutmin); RhopOutMax = float(s.rhopoutmax);
@@ -274,6 +283,10 @@ struct Sim {
     idp.assign(hidp, hidp + np);
     pos.assign(hpos, hpos + 3 * size_t(np));
     velrho.assign(hvr, hvr + 4 * size_t(np));
+    if (mdbc2_on() && cs.casenmoving) {
+      motionvel.assign(3 * size_t(np), 0.f);
+      motionace.assign(3 * size_t(np), 0.f);
+    }
     code.resize(np);
     dcell.resize(np);
     for (unsigned p = 0; p < np; p++) {
@@ -459,6 +472,7 @@ struct Sim {
     else if (pre_active) { gather(pospre, 3, nptot, gpini); gather(velrhopre, 4, nptot, gpini); }
     if (TVisco == 2) gather(tau, 6, nptot, gpini);
     if (TBoundary == 2) gather(boundnor, 3, nptot, gpini);
+    if (!motionvel.empty()) { gather(motionvel, 3, nptot, gpini); gather(motionace, 3, nptot, gpini); }
     if (npbout) throw std::runtime_error("boundary particles excluded");
     Np = npfinal; Npb = npbfinal; NpbOk = Npb - npbignore;
     NpfOutTotal += npfout;
@@ -492,15 +506,18 @@ struct Sim {
     const bool m2 = mdbc2_on();
     if (ShiftMode) shiftposfs.assign(4 * size_t(Np), 0.f);  // JSphShifting::InitCpu, no zones
     if (NoPen) nopenshift.assign(4 * size_t(Np), 0.f);  // JSphCpu.cpp:519
-    orc_interaction_np(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco, TDDT,
-                       ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
-                       dcell.data(), poscell.data(), pos.data(), velrho.data(),
-                       TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(),
-                       viscdt.data(), TVisco == 2 ? strain.data() : nullptr, nullptr,
-                       m2 ? bmode.data() : nullptr, m2 ? tangenvel.data() : nullptr, ShiftMode,
-                       code.data(), ShiftMode ? shiftposfs.data() : nullptr,
-                       ft.empty() ? nullptr : ftmassp.data(), NoPen ? boundnor.data() : nullptr,
-                       NoPen ? nopenshift.data() : nullptr);
+    orc_interaction_range_mv(prec == 1 ? 1 : 0, prec == 1 ? 2 : (prec == 2 ? 1 : 0), TVisco,
+                             TDDT, ViscoB, Visco, &c, &d, Np, Npb, NpbOk,
+                             dcell.data(), poscell.data(), pos.data(), velrho.data(),
+                             TVisco == 2 ? tau.data() : nullptr, ace.data(), ar.data(),
+                             viscdt.data(), TVisco == 2 ? strain.data() : nullptr, nullptr,
+                             m2 ? bmode.data() : nullptr, m2 ? tangenvel.data() : nullptr,
+                             ShiftMode, code.data(), ShiftMode ? shiftposfs.data() : nullptr,
+                             ft.empty() ? nullptr : ftmassp.data(),
+                             NoPen ? boundnor.data() : nullptr,
+                             NoPen ? nopenshift.data() : nullptr,
+                             motionvel.empty() ? nullptr : motionvel.data(), 0, LONG_MAX, 0,
+                             LONG_MAX);
     if (TVisco == 2) orc_sps_tau(Np, Npb, SpsSmag, SpsBlin, velrho.data(), strain.data(), tau.data());
     float vd = 0, a2 = 0;
     for (unsigned p = 0; p < Np; p++) vd = std::max(vd, viscdt[p]);
@@ -584,9 +601,19 @@ struct Sim {
     dcell[p] = mkcell_host(unsigned(dd[0] / c.scell), unsigned(dd[1] / c.scell),
                            unsigned(dd[2] / c.scell));
   }
+  // JSphCpu::RunMotion (JSphCpu.cpp:2188-2234): the motion when active,
+  // then, with mDBC2, the motion velocity and acceleration of the moving
+  // particles (CopyMotionVelAce) whether the motion is active or not.
   void run_motion(double dt) {
     const dsph_motion& m = cs.motion;
     if (!cs.casenmoving || m.type == 0) return;
+    move_boundaries(dt);
+    if (!motionvel.empty())
+      orc_motion_vel_ace(Npb, m.idbegin, m.count, dt, idp.data(), velrho.data(),
+                         motionvel.data(), motionace.data());
+  }
+  void move_boundaries(double dt) {
+    const dsph_motion& m = cs.motion;
     if (!(TimeStep >= m.start && TimeStep < m.finish)) return;
     const double twopi = 6.283185307179586476925286766559;
     if (m.type == 1) {
@@ -647,8 +674,10 @@ struct Sim {
     if (mdbc2_on()) {
       bmode.assign(Npb, 0);
       tangenvel.assign(3 * size_t(Npb), 0.f);
+      const bool mv = !motionvel.empty();
       orc_mdbc2_slip(prec == 1 ? 1 : 0, prec == 2 ? 1 : 0, SlipMode, &c, &d, Npb, pos.data(),
-                     poscell.data(), code.data(), boundnor.data(), g, nullptr, nullptr, Cs0,
+                     poscell.data(), code.data(), boundnor.data(), g,
+                     mv ? motionvel.data() : nullptr, mv ? motionace.data() : nullptr, Cs0,
                      cs.dp, velrho.data(), bmode.data(), tangenvel.data());
       return;
     }

@@ -72,6 +72,10 @@ def load() -> C.CDLL:
             "orc_interaction_range": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv),
                                        u, u, u] + [vp] * 12 + [i, vp, vp, vp, vp, vp]
                                       + [C.c_long] * 4, None),
+            "orc_interaction_range_mv": ([i, i, i, i, f, f, C.POINTER(OrcCte),
+                                          C.POINTER(OrcDiv), u, u, u] + [vp] * 12
+                                         + [i, vp, vp, vp, vp, vp, vp] + [C.c_long] * 4, None),
+            "orc_motion_vel_ace": ([u, u, u, d, vp, vp, vp, vp], None),
             "orc_matrix_rot": ([d, vp, vp, vp], None),
             "orc_interaction_ext": ([i, i, i, i, f, f, C.POINTER(OrcCte), C.POINTER(OrcDiv), u,
                                      u, u] + [vp] * 12 + [i, vp, vp], None),

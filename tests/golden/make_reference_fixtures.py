@@ -165,6 +165,20 @@ CASES = {
     "ref_flume2d_dp0.005_flap_rot_mdbc": ("flume_2d", 0.005, 40, (10, 20, 40),
                                           dict(wavemaker="flap_rot", tboundary=2, tvisco=3,
                                                visco=1e-4)),
+    # mDBC2 with moving boundaries (motion velocity and acceleration of the
+    # wavemaker in the pressure clone and the no-penetration sums): the
+    # piston with no-slip; a piston of 0.1 m stroke (0.79 m/s at the start,
+    # so that it closes on the fluid within 0.75 of a normal length after
+    # about 20 steps) with no-slip + no penetration; the flap at a constant
+    # rotation with free slip.
+    "ref_flume_dp0.01_piston_mdbc2": ("flume", 0.01, 40, (1, 10, 20, 40),
+                                      dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4)),
+    "ref_flume_dp0.01_piston_mdbc2_nopen": ("flume", 0.01, 40, (1, 10, 20, 40),
+                                            dict(tboundary=2, slipmode=2, nopenetration=1,
+                                                 tvisco=3, visco=1e-4, ampl=0.1)),
+    "ref_flume_dp0.01_flap_rot_mdbc2_freeslip": ("flume", 0.01, 40, (1, 10, 20, 40),
+                                                 dict(wavemaker="flap_rot", tboundary=2,
+                                                      slipmode=3, tvisco=3, visco=1e-4)),
     # Channel periodic in x with the fluid moving at 2 m/s: particles leave
     # through one face and re-enter through the other (Symplectic + DBC, and
     # Verlet + mDBC + laminar viscosity).
@@ -177,7 +191,7 @@ CASES = {
 
 
 # Keys of a CASES entry that are arguments of the case generator, not Parameters.
-GENERATOR_ARGS = ("wavemaker", "vel0", "velini", "omegaini")
+GENERATOR_ARGS = ("wavemaker", "vel0", "velini", "omegaini", "ampl")
 
 
 def run_reference(case, nsteps: int, workdir: Path) -> list[dict]:

@@ -282,8 +282,9 @@ def mdbc2_state(c):
             "pos": np.ascontiguousarray(d["pos"])}
 
 
-def oracle_mdbc2(c, s, prec, sem, velrho=None, slipmode=2):
-    """orc_mdbc2_slip on state s (slipmode 2 no-slip, 3 free slip): (velrho
+def oracle_mdbc2(c, s, prec, sem, velrho=None, slipmode=2, motionvel=None, motionace=None):
+    """orc_mdbc2_slip on state s (slipmode 2 no-slip, 3 free slip; motion
+    velocity and acceleration float3[n] of moving walls or None): (velrho
     after the correction, boundmode, tangenvel)."""
     import ctypes as C
     import oracle.oracle as orc
@@ -295,7 +296,9 @@ def oracle_mdbc2(c, s, prec, sem, velrho=None, slipmode=2):
     orc.load().orc_mdbc2_slip(prec, sem, slipmode, C.byref(s["cte"]), C.byref(s["odiv"]), npb,
                               s["pos"].ctypes.data, s["poscell"].ctypes.data,
                               s["d"]["code"].ctypes.data, s["bn2"].ctypes.data, g.ctypes.data,
-                              None, None, float(c.cs0), float(c.dp), vr.ctypes.data,
+                              None if motionvel is None else motionvel.ctypes.data,
+                              None if motionace is None else motionace.ctypes.data,
+                              float(c.cs0), float(c.dp), vr.ctypes.data,
                               bm.ctypes.data, tv.ctypes.data)
     return vr, bm, tv
 

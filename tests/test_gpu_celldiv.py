@@ -84,6 +84,31 @@ def test_cell_limits(lib, ctx, cstate):
         assert list(mn) == list(omn) and list(mx) == list(omx)
 
 
+@pytest.mark.parametrize("n", [1, 257, 3_000_017])
+def test_cell_limits_many_blocks(lib, ctx, cstate, n):
+    """Limits over up to 3M particles (the grid-stride loop over 1024 block
+    partials and the final block): the cells of the dam break repeated with
+    random exclusions, against the oracle; all excluded gives min > max."""
+    import oracle.oracle as orc
+    c, o, d, dcell0, poscell, cte, celldiv, be = cstate
+    rng = np.random.default_rng(n)
+    pick = rng.integers(0, dcell0.size, n)
+    dcell = np.ascontiguousarray(dcell0[pick])
+    code = np.ascontiguousarray(d["code"][pick])
+    code[rng.random(n) < 0.3] |= 0x8000
+    olib = orc.load()
+    for cd in (code, code | np.uint16(0x8000)):
+        mn, mx = (C.c_uint32 * 3)(), (C.c_uint32 * 3)()
+        rc = lib.dsph_cell_limits_sync(ctx, n, 0, cte.cellcode, Dev(lib, ctx, dcell).ptr,
+                                       Dev(lib, ctx, cd).ptr, mn, mx)
+        assert rc == 0, lib.dsph_last_error(ctx)
+        omn, omx = np.zeros(3, np.uint32), np.zeros(3, np.uint32)
+        olib.orc_limits(n, 0, cte.cellcode, dcell.ctypes.data, cd.ctypes.data,
+                        omn.ctypes.data, omx.ctypes.data)
+        assert list(mn) == list(omn) and list(mx) == list(omx)
+    assert all(a > b for a, b in zip(mn, mx))   # every particle excluded
+
+
 @pytest.mark.parametrize("full", [1, 0])
 def test_presort_sort_beginend_match_oracle(lib, ctx, cstate, full):
     import oracle.oracle as orc

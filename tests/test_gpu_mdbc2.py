@@ -19,7 +19,7 @@ from tests.helpers import (Dev, divdata, hydrostatic_mdbc2_case, mdbc2_state, or
 pytestmark = pytest.mark.gpu
 
 
-def _gpu_mdbc2(lib, ctx, c, s, velrho, slipmode=N.SLIP_NOSLIP):
+def _gpu_mdbc2(lib, ctx, c, s, velrho, slipmode=N.SLIP_NOSLIP, mv=None, ma=None):
     acte = to_cte(s["cte"], tboundary=N.BC_MDBC)
     assert lib.dsph_set_constants(ctx, C.byref(acte)) == 0
     bc = Dev(lib, ctx, s["be"])
@@ -36,13 +36,17 @@ def _gpu_mdbc2(lib, ctx, c, s, velrho, slipmode=N.SLIP_NOSLIP):
     d_tv = Dev.zeros(lib, ctx, (npb, 3), np.float32)
     g = (C.c_float * 3)(*c.gravity)
     mp = (C.c_double * 3)(*s["cte"].mapposmin)
+    d_mv = Dev(lib, ctx, mv) if mv is not None else None
+    d_ma = Dev(lib, ctx, ma) if ma is not None else None
+    pmv = d_mv.ptr if d_mv else None
+    pma = d_ma.ptr if d_ma else None
     if slipmode == N.SLIP_NOSLIP:
         rc = lib.dsph_mdbc2_correction(ctx, npb, g, C.byref(dv), mp, d_xy.ptr, d_z.ptr, d_ps.ptr,
-                                       d_code.ptr, d_bn.ptr, None, None, d_vr.ptr, d_bm.ptr,
+                                       d_code.ptr, d_bn.ptr, pmv, pma, d_vr.ptr, d_bm.ptr,
                                        d_tv.ptr)
     else:
         rc = lib.dsph_mdbc2_correction_slip(ctx, slipmode, npb, g, C.byref(dv), mp, d_xy.ptr,
-                                            d_z.ptr, d_ps.ptr, d_code.ptr, d_bn.ptr, None, None,
+                                            d_z.ptr, d_ps.ptr, d_code.ptr, d_bn.ptr, pmv, pma,
                                             d_vr.ptr, d_bm.ptr, d_tv.ptr)
     assert rc == 0, lib.dsph_last_error(ctx)
     return d_vr.get(), d_bm.get(), d_tv.get(), dv, bc
@@ -220,6 +224,67 @@ def test_mdbc2_freeslip_correction_matches_oracle(lib, ctx):
     assert np.abs(o_vr[:npb, 3] - n_vr[:npb, 3]).max() == 0.0   # density: slip mode plays no part
 
 
+@pytest.mark.parametrize("slipmode", [N.SLIP_NOSLIP, N.SLIP_FREESLIP], ids=["noslip", "freeslip"])
+def test_mdbc2_correction_with_moving_walls_matches_oracle(lib, ctx, slipmode):
+    """Moving walls: the motion acceleration enters the pressure clone and
+    the motion velocity the no-slip mirror (JSphGpu_mdbc_iker.cu:411-470);
+    random motion velocity (0.3 m/s) and acceleration (5 m/s^2) per wall
+    particle against the oracle, and the result differs from fixed walls."""
+    c = _cases()["perturbed_dam_break"]
+    s = mdbc2_state(c)
+    npb, n = s["npb"], s["n"]
+    rng = np.random.default_rng(5)
+    mv = np.zeros((n, 3), np.float32)
+    ma = np.zeros((n, 3), np.float32)
+    mv[:npb] = (rng.standard_normal((npb, 3)) * 0.3).astype(np.float32)
+    ma[:npb] = (rng.standard_normal((npb, 3)) * 5.0).astype(np.float32)
+    g_vr, g_bm, g_tv, _, _ = _gpu_mdbc2(lib, ctx, c, s, s["d"]["velrho"], slipmode, mv, ma)
+    o_vr, o_bm, o_tv = oracle_mdbc2(c, s, 0, orc.SEM_GPU, slipmode=slipmode, motionvel=mv,
+                                    motionace=ma)
+    f_vr, _, f_tv = oracle_mdbc2(c, s, 0, orc.SEM_GPU, slipmode=slipmode)
+    same = g_bm == o_bm
+    assert same.mean() > 0.999, (np.count_nonzero(~same), npb)
+    on = same & (g_bm == 1)
+    e_rho = np.abs(g_vr[:npb, 3] - o_vr[:npb, 3])[same].max()
+    e_tv = rel_err(g_tv[same], o_tv[same])
+    d_rho = np.abs(o_vr[:npb, 3] - f_vr[:npb, 3])[on].max()
+    d_tv = rel_err(f_tv[same], o_tv[same])
+    print(f"slip {slipmode}: rho |gpu-oracle32| {e_rho:.2e} tangenvel {e_tv:.2e}; moving vs "
+          f"fixed walls: rho {d_rho:.2e} tangenvel {d_tv:.2e}")
+    assert e_rho < 5e-3 and e_tv < 1e-4
+    assert d_rho > 0.05
+    if slipmode == N.SLIP_NOSLIP:
+        assert d_tv > 0.1
+
+
+@pytest.mark.parametrize("device_dt", [False, True], ids=["host_dt", "device_dt"])
+def test_motion_vel_ace_matches_oracle(lib, ctx, device_dt):
+    """CopyMotionVelAce: the moving particles (ids in [idbegin, idbegin+count)
+    among the boundary particles, any order) get float((v - motionvel)/dt)
+    and motionvel = v, bit for bit; the others are untouched."""
+    rng = np.random.default_rng(3)
+    npb, idbegin, count = 5000, 1200, 1700
+    idp = rng.permutation(npb).astype(np.uint32)
+    vr = (rng.standard_normal((npb, 4))).astype(np.float32)
+    mv = (rng.standard_normal((npb, 3))).astype(np.float32)
+    ma = np.full((npb, 3), 7.0, np.float32)
+    dt = 1.7e-4
+    d_idp, d_vr = Dev(lib, ctx, idp), Dev(lib, ctx, vr)
+    d_mv, d_ma = Dev(lib, ctx, mv), Dev(lib, ctx, ma)
+    d_dt = Dev(lib, ctx, np.array([0.0, 0.0, dt, 0.0], np.float64)) if device_dt else None
+    rc = lib.dsph_mdbc2_motion_vel_ace(ctx, npb, idbegin, count, 0.0 if device_dt else dt,
+                                       d_dt.ptr if device_dt else None, 2, d_idp.ptr, d_vr.ptr,
+                                       d_mv.ptr, d_ma.ptr)
+    assert rc == 0, lib.dsph_last_error(ctx)
+    omv, oma = mv.copy(), ma.copy()
+    orc.load().orc_motion_vel_ace(npb, idbegin, count, dt, idp.ctypes.data, vr.ctypes.data,
+                                  omv.ctypes.data, oma.ctypes.data)
+    np.testing.assert_array_equal(d_mv.get(), omv)
+    np.testing.assert_array_equal(d_ma.get(), oma)
+    moving = (idp >= idbegin) & (idp < idbegin + count)
+    assert moving.sum() == count and np.all(oma[~moving] == 7.0)
+
+
 def _nopen_case():
     """The perturbed mDBC2 dam break with the fluid moved 0.3 dp towards the
     floor and the back wall, so that the first fluid layers sit closer than
@@ -232,9 +297,13 @@ def _nopen_case():
     return dataclasses.replace(c, pos=pos)
 
 
+@pytest.mark.parametrize("moving", [False, True], ids=["fixed", "moving"])
 @pytest.mark.parametrize("tvisco,tddt", [(N.VISCO_LAMINAR, N.DDT_DDT2FULL),
                                          (N.VISCO_LAMINARSPS, N.DDT_DDT2)])
-def test_interaction_no_penetration_matches_oracle(lib, ctx, tvisco, tddt):
+def test_interaction_no_penetration_matches_oracle(lib, ctx, tvisco, tddt, moving):
+    """The no-penetration sums against the oracle, with fixed walls and with
+    walls moving at random velocities (0.5 m/s, the test velocity relative to
+    the wall's motion velocity)."""
     c = _nopen_case()
     s = mdbc2_state(c)
     n, npb, npbok = s["n"], s["npb"], s["npbok"]
@@ -251,6 +320,10 @@ def test_interaction_no_penetration_matches_oracle(lib, ctx, tvisco, tddt):
                                           ("velrho", o_vr), ("code", s["d"]["code"]),
                                           ("bm", o_bm), ("tv", o_tv), ("tau", tau),
                                           ("bn", bn2))}
+    mv = np.zeros((n, 3), np.float32)
+    if moving:
+        mv[:npb] = (np.random.default_rng(4).standard_normal((npb, 3)) * 0.5).astype(np.float32)
+    d_mv = Dev(lib, ctx, mv)
     ace = Dev.zeros(lib, ctx, (n, 3), np.float32)
     ar = Dev.zeros(lib, ctx, n, np.float32)
     vdt = Dev.zeros(lib, ctx, n, np.float32)
@@ -267,6 +340,7 @@ def test_interaction_no_penetration_matches_oracle(lib, ctx, tvisco, tddt):
     t.d_viscdt, t.d_ar, t.d_ace, t.d_sps2strain = vdt.ptr, ar.ptr, ace.ptr, strain.ptr
     t.d_boundmode, t.d_tangenvel = gd["bm"].ptr, gd["tv"].ptr
     t.d_boundnor, t.d_nopenshift = gd["bn"].ptr, ns.ptr
+    t.d_motionvel = d_mv.ptr if moving else None
     assert lib.dsph_interaction_forces(ctx, C.byref(t)) == 0, lib.dsph_last_error(ctx)
     g = {"ace": ace.get(), "ar": ar.get(), "viscdt": vdt.get(), "ns": ns.get()}
     oa = np.zeros((n, 3), np.float32)
@@ -274,13 +348,13 @@ def test_interaction_no_penetration_matches_oracle(lib, ctx, tvisco, tddt):
     ov = np.zeros(n, np.float32)
     os_ = np.zeros((n, 6), np.float32)
     ons = np.zeros((n, 4), np.float32)
-    orc.load().orc_interaction_np(
+    orc.load().orc_interaction_range_mv(
         0, orc.SEM_GPU, tvisco, tddt, visco, visco, C.byref(s["cte"]), C.byref(s["odiv"]), n,
         npb, npbok, s["dcell"].ctypes.data, s["poscell"].ctypes.data, s["pos"].ctypes.data,
         o_vr.ctypes.data, tau.ctypes.data if tvisco == N.VISCO_LAMINARSPS else None,
         oa.ctypes.data, oar.ctypes.data, ov.ctypes.data, os_.ctypes.data, None,
         o_bm.ctypes.data, o_tv.ctypes.data, 0, None, None, None, bn2.ctypes.data,
-        ons.ctypes.data)
+        ons.ctypes.data, mv.ctypes.data if moving else None, 0, 2**62, 0, 2**62)
     fl = slice(npb, None)
     e = {"ace": rel_err(g["ace"][fl], oa[fl]), "ar": rel_err(g["ar"][fl], oar[fl]),
          "viscdt": rel_err(g["viscdt"], ov)}

@@ -6,7 +6,9 @@ reference and the oracle). The 2-D cases (<data2d>, particles in the plane
 y = 0) cover DBC, mDBC with laminar+SPS, mDBC2 no-slip under Verlet, mDBC2
 free slip with no penetration, shifting (with the free-surface threshold,
 and full), a channel periodic in x, a floating box (at rest; thrown and
-spinning) and the flume with a piston or a flap."""
+spinning) and the flume with a piston or a flap. mDBC2 with moving walls:
+the piston flume with no-slip, with no-slip and no penetration, and the flap
+at constant rotation with free slip."""
 import numpy as np
 import pytest
 
@@ -46,7 +48,10 @@ pytestmark = pytest.mark.gpu
                                   "ref_dambreak2d_dp0.005_floating_spin",
                                   "ref_flume2d_dp0.005_piston_mdbc_sps",
                                   "ref_flume2d_dp0.005_flap",
-                                  "ref_flume2d_dp0.005_flap_rot_mdbc"])
+                                  "ref_flume2d_dp0.005_flap_rot_mdbc",
+                                  "ref_flume_dp0.01_piston_mdbc2",
+                                  "ref_flume_dp0.01_piston_mdbc2_nopen",
+                                  "ref_flume_dp0.01_flap_rot_mdbc2_freeslip"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)

@@ -32,7 +32,11 @@ x (DBC; Verlet + mDBC + laminar) over 60 steps; the 2-D dam break with a
 floating box at rest, and thrown at 1 m/s spinning at 2 rad/s about y, over
 40 steps (reproduced bit for bit in CPU semantics); the 2-D flume with the
 piston (mDBC + laminar+SPS), the flap (rotation sinusoid, DBC) and the flap
-at a constant rotation (mDBC) over 40 steps.
+at a constant rotation (mDBC) over 40 steps; mDBC2 with moving walls (the
+motion acceleration in the pressure clone, the motion velocity in the
+no-penetration test): the piston with no-slip, a 0.1 m piston with no-slip
+and no penetration (acting from step ~25, when the piston has closed on the
+fluid), the flap at constant rotation with free slip, over 40 steps.
 """
 import numpy as np
 import pytest
@@ -58,7 +62,8 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_channel2d_dp0.005_periodic", "ref_channel2d_dp0.005_periodic_mdbc_verlet",
             "ref_dambreak2d_dp0.005_floating", "ref_dambreak2d_dp0.005_floating_spin",
             "ref_flume2d_dp0.005_piston_mdbc_sps", "ref_flume2d_dp0.005_flap",
-            "ref_flume2d_dp0.005_flap_rot_mdbc"]
+            "ref_flume2d_dp0.005_flap_rot_mdbc", "ref_flume_dp0.01_piston_mdbc2",
+            "ref_flume_dp0.01_piston_mdbc2_nopen", "ref_flume_dp0.01_flap_rot_mdbc2_freeslip"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
@@ -100,6 +105,7 @@ def test_fixture_detects_a_planted_error():
     ("ref_dambreak_dp0.03_mdbc2_freeslip_nopen_verlet", dict(nopenetration=0)),
     ("ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen", dict(nopenetration=0)),
     ("ref_dambreak2d_dp0.005_mdbc2_freeslip_nopen", dict(slipmode=2)),
+    ("ref_flume_dp0.01_piston_mdbc2_nopen", dict(nopenetration=0)),
 ])
 def test_mdbc2_options_are_pinned_by_their_fixtures(name, wrong):
     """The oracle run without the option a fixture was made with (no
