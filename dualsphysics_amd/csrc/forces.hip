@@ -848,9 +848,19 @@ void launch_fluid(dsph_ctx* ctx, const dsph_inter_parms* t, const DivData& dvd, 
               npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
               (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
               t->d_sps2strain, nullptr, nullptr, nullptr, 0, t->d_code, nullptr, t->d_ftomassp);
+  } else if (t->d_shiftposfs && t->d_boundmode) {
+    // Shifting with mDBC2 boundaries: the one-lane-per-particle kernel with
+    // both rules (a boundary switched off has mass 0 in the shifting sums
+    // too, JSphGpu_ker.cu:586-588, 733-740).
+    k_forces_fluid_rows<TVISCO, TDDT, false, true, true><<<grid_for(npf, kBsForces), kBsForces,
+                                                          0, ctx->stream>>>(
+        npf, t->npb, t->viscob, t->viscof, dvd, c, t->d_dcell, (const float4*)t->d_poscell,
+        (const float4*)t->d_velrho, press, t->d_spstaurho2, t->d_viscdt, t->d_ar, t->d_ace,
+        t->d_sps2strain, nullptr, t->d_boundmode, t->d_tangenvel, t->shiftmode, t->d_code,
+        (float4*)t->d_shiftposfs);
   } else if (t->d_shiftposfs && ctx->forces_kernel == 0) {
-    // Shifting in the tile kernel (mDBC2 and the velocity gradient are
-    // refused with shifting by dsph_interaction_forces).
+    // Shifting in the tile kernel (no penetration and the velocity gradient
+    // are refused with shifting by dsph_interaction_forces).
     k_forces_fluid_tile<TVISCO, TDDT, false, kRingTiles, false, true>
         <<<grid_for(npf, 64 * kTileWaves), 64 * kTileWaves, 0, ctx->stream>>>(
         npf, t->npb, t->viscob, t->viscof, dvd, c, scan_radius2(ctx, c), t->d_dcell,
@@ -1234,8 +1244,8 @@ int check_inter_parms(dsph_ctx* ctx, const dsph_inter_parms* t) {
   DSPH_REQUIRE(ctx, !t->d_boundmode || t->d_tangenvel, "mDBC2 needs boundmode and tangenvel");
   DSPH_REQUIRE(ctx, !t->d_shiftposfs || (t->shiftmode >= 1 && t->shiftmode <= 3 && t->d_code),
                "shifting needs shiftmode 1..3 and the particle codes");
-  DSPH_REQUIRE(ctx, !t->d_shiftposfs || (!t->d_boundmode && !t->d_velgrad),
-               "shifting with mDBC2 or the velocity gradient is not built");
+  DSPH_REQUIRE(ctx, !t->d_shiftposfs || (!t->d_nopenshift && !t->d_velgrad),
+               "shifting with mDBC2 no penetration or the velocity gradient is not built");
   DSPH_REQUIRE(ctx, !t->d_ftomassp || t->d_code, "floating bodies need the particle codes");
   DSPH_REQUIRE(ctx, !t->d_ftomassp || (!t->d_shiftposfs && !t->d_boundmode && !t->d_velgrad),
                "floating bodies with shifting, mDBC2 or the velocity gradient are not built");

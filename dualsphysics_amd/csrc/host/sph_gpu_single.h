@@ -223,8 +223,8 @@ struct dsph_sim {
     ShiftTFS = c.shifttfs;
     if (ShiftMode < DSPH_SHIFT_NONE || ShiftMode > DSPH_SHIFT_FULL)
       throw Err(DSPH_E_ARG, "Shifting: mode 0..3");
-    if (ShiftMode && mdbc2_on())
-      throw Err(DSPH_E_ARG, "shifting with mDBC no-slip or free slip is not built");
+    if (ShiftMode && NoPen)
+      throw Err(DSPH_E_ARG, "shifting with mDBC2 no penetration is not built");
     Visco = float(c.visco);
     ViscoBoundFactor = float(c.viscoboundfactor);
     RhopOutMin = float(c.rhopoutmin);

@@ -179,6 +179,15 @@ CASES = {
     "ref_flume_dp0.01_flap_rot_mdbc2_freeslip": ("flume", 0.01, 40, (1, 10, 20, 40),
                                                  dict(wavemaker="flap_rot", tboundary=2,
                                                       slipmode=3, tvisco=3, visco=1e-4)),
+    # Shifting with mDBC2 (boundaries switched off have mass 0 in the sums):
+    # full shifting on the dam break with no-slip; shifting that spares
+    # fixed walls, with the moving piston and free slip.
+    "ref_dambreak_dp0.03_mdbc2_shift_full": ("dam_break", 0.03, 10, (1, 5, 10),
+                                             dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4,
+                                                  shifting=3)),
+    "ref_flume_dp0.01_piston_mdbc2_freeslip_shift_nofixed": (
+        "flume", 0.01, 40, (1, 10, 20, 40),
+        dict(tboundary=2, slipmode=3, tvisco=3, visco=1e-4, shifting=2, shifttfs=1.5)),
     # Channel periodic in x with the fluid moving at 2 m/s: particles leave
     # through one face and re-enter through the other (Symplectic + DBC, and
     # Verlet + mDBC + laminar viscosity).

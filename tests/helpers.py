@@ -197,15 +197,21 @@ def compare_with_reference(fx, k, time, pos0, pos, vel, rho, order=None):
     prm = str(fx["params"])
     tol = REF_TOL_MDBC if "tboundary', 2" in prm else REF_TOL
     if "'shifting'" in prm and "shifttfs" not in prm:
-        tol = REF_TOL_SHIFT_NOTFS
+        # Shifting without the threshold (with mDBC: the larger of the two
+        # spreads for each quantity).
+        tol = {k: max(v, tol[k]) for k, v in REF_TOL_SHIFT_NOTFS.items()}
     nmov = int(fx["nmoving"]) if "nmoving" in fx else 0
     nfl = int(fx["disp_float"].shape[1]) if "disp_float" in fx else 0
     npb = nb - nfl
     if nmov:
+        # The motion is a function of time: its position may differ by the
+        # wall speed times the time difference (bounded above) beyond 1e-9 m.
         mv = slice(npb - nmov, npb)
         dm = float(np.abs((pos[mv] - pos0[mv]) - fx["disp_moving"][k]).max())
         vm = float(np.abs(vel[mv] - fx["vel_moving"][k]).max())
-        assert dm < 1e-9 and vm < 1e-6, (int(fx["steps"][k]), "moving boundary", dm, vm)
+        slack = float(np.abs(fx["vel_moving"][k]).max()) * abs(time - float(fx["time"][k]))
+        assert dm < 1e-9 + 1.5 * slack and vm < 1e-6, (int(fx["steps"][k]), "moving boundary",
+                                                       dm, vm)
     disp_ref = fx["disp"][k].astype(np.float64)
     disp = pos[nb:] - pos0[nb:]
     e = {

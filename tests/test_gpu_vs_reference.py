@@ -51,7 +51,9 @@ pytestmark = pytest.mark.gpu
                                   "ref_flume2d_dp0.005_flap_rot_mdbc",
                                   "ref_flume_dp0.01_piston_mdbc2",
                                   "ref_flume_dp0.01_piston_mdbc2_nopen",
-                                  "ref_flume_dp0.01_flap_rot_mdbc2_freeslip"])
+                                  "ref_flume_dp0.01_flap_rot_mdbc2_freeslip",
+                                  "ref_dambreak_dp0.03_mdbc2_shift_full",
+                                  "ref_flume_dp0.01_piston_mdbc2_freeslip_shift_nofixed"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)
