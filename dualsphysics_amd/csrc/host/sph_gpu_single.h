@@ -276,9 +276,9 @@ struct dsph_sim {
     ft.clear();
     if (c.casenfloat) {
       if (!c.ftcount || !c.floatings) throw Err(DSPH_E_ARG, "floating particles without bodies");
-      if (TStep != 2 || TBoundary != DSPH_BC_DBC || ShiftMode || PeriActive)
-        throw Err(DSPH_E_ARG, "floating bodies are built for the Symplectic integrator with DBC "
-                              "boundaries, without shifting or periodic axes");
+      if (TBoundary != DSPH_BC_DBC || ShiftMode || PeriActive)
+        throw Err(DSPH_E_ARG, "floating bodies are built with DBC boundaries, without shifting "
+                              "or periodic axes");
       unsigned id = CaseNpb;
       for (unsigned cf = 0; cf < c.ftcount; cf++) {
         const dsph_floating& f = c.floatings[cf];
@@ -1096,6 +1096,7 @@ struct dsph_sim {
           "no penetration");
     chk(dsph_step_pos(ctx, Np, Npb, movxy, movz, posxy, posz, dcell, code), "pos");
     swp(velrho, velrhom1);
+    run_floating(dt, false);
     run_motion(dt);
     TimeStep += dt;
     DtLast = dt;

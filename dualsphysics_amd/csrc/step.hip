@@ -276,6 +276,12 @@ __global__ void __launch_bounds__(kBs) k_verlet(unsigned n, unsigned npb,
   const float4 v2 = velrho2[p];
   const float rhonew = float(double(v2.w) + dt2 * ar[p]);
   const float4 v1 = velrho1[p];
+  if (code_type(rcode) == kCodeTypeFloating) {
+    // Floating particle (JSphGpuSimple_ker.cu:217-221): the current velocity
+    // and the new density clamped at rho0; RunFloating moves it.
+    velrhonew[p] = make_float4(v1.x, v1.y, v1.z, rhonew < rhopzero ? rhopzero : rhonew);
+    return;
+  }
   const double ax = double(ace[3 * size_t(p)]) + gravity.x;
   const double ay = double(ace[3 * size_t(p) + 1]) + gravity.y;
   const double az = double(ace[3 * size_t(p) + 2]) + gravity.z;
@@ -567,6 +573,7 @@ __global__ void __launch_bounds__(kBs) k_step_pos(unsigned n, unsigned pini, Pos
   const unsigned rcode = code[p];
   const bool outrho = code_is_outrho(rcode);
   if (c.periactive && !outrho && !code_is_normal(rcode)) return;  // periodic copies stay
+  if (code_type(rcode) == kCodeTypeFloating) return;  // RunFloating moves floating particles
   const double2 m = movxy[p];
   update_pos(posxy[p], posz[p], m.x, m.y, movz[p], outrho, p, c, posxy, posz, dcell, code);
 }

@@ -1183,6 +1183,13 @@ void orc_verlet(unsigned np, unsigned npb, const float* vr1, const float* vr2, c
       continue;
     }
     const float rhonew = float(double(vr2[4 * size_t(p) + 3]) + dt2 * ar[p]);
+    if ((code[p] & kTypeMask) == kTypeFloating) {
+      // Floating particle (JSphCpu.cpp:1592-1596): current velocity, density
+      // clamped at rho0, no displacement (RunFloating moves it).
+      for (int k = 0; k < 3; k++) nw[k] = vr1[4 * size_t(p) + k];
+      nw[3] = (rhonew < rho0 ? rho0 : rhonew);
+      continue;
+    }
     double acg[3];
     for (int k = 0; k < 3; k++) acg[k] = double(ace[3 * size_t(p) + k]) + g[k];
     float v[3];

@@ -254,9 +254,9 @@ struct Sim {
     ftmassp.clear();
     if (s.casenfloat) {
       if (!s.ftcount || !s.floatings) throw std::runtime_error("floating particles without bodies");
-      if (TStep != 2 || TBoundary != 1 || ShiftMode || c.periactive)
-        throw std::runtime_error("floating bodies are built for the Symplectic integrator with DBC "
-                                 "boundaries, without shifting or periodic axes");
+      if (TBoundary != 1 || ShiftMode || c.periactive)
+        throw std::runtime_error("floating bodies are built with DBC boundaries, without "
+                                 "shifting or periodic axes");
       unsigned id = CaseNpb;
       for (unsigned cf = 0; cf < s.ftcount; cf++) {
         const dsph_floating& f = s.floatings[cf];
@@ -905,6 +905,7 @@ struct Sim {
     orc_step_pos(Np, Npb, &c, p0.data(), mov.data(), pos.data(), dcell.data(), code.data());
     velrhom1 = velrho;
     velrho = nw;
+    run_floating(dt, false);
     run_motion(dt);
     TimeStep += dt;
     DtLast = dt;

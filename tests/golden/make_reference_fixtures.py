@@ -188,6 +188,17 @@ CASES = {
     "ref_flume_dp0.01_piston_mdbc2_freeslip_shift_nofixed": (
         "flume", 0.01, 40, (1, 10, 20, 40),
         dict(tboundary=2, slipmode=3, tvisco=3, visco=1e-4, shifting=2, shifttfs=1.5)),
+    # Floating box under the Verlet integrator (floating particles keep their
+    # velocity in the step, RunFloating moves them): at rest, and thrown
+    # spinning in 2-D; 10 steps, as the other Verlet cases (the FP32 spread
+    # of these Verlet runs grows past the bounds by step 20;
+    # the oracle in CPU semantics reproduces the reference bit for bit over
+    # 40 steps).
+    "ref_dambreak_dp0.03_floating_verlet": ("dam_break_floating", 0.03, 10, (1, 5, 10),
+                                            dict(tstep=1)),
+    "ref_dambreak2d_dp0.005_floating_spin_verlet": (
+        "dam_break_floating_2d", 0.005, 10, (1, 5, 10),
+        dict(tstep=1, velini=(1.0, 0.0, -0.5), omegaini=(0.0, 2.0, 0.0))),
     # Channel periodic in x with the fluid moving at 2 m/s: particles leave
     # through one face and re-enter through the other (Symplectic + DBC, and
     # Verlet + mDBC + laminar viscosity).

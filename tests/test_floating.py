@@ -116,7 +116,8 @@ def test_case_files_carry_the_floating_body(tmp_path):
         np.testing.assert_array_equal(np.array(getattr(a, k)), np.array(getattr(b, k)))
 
 
-def test_unsupported_floating_options_are_refused():
-    c = cm.dam_break_floating(dp=0.05, params=cm.Parameters(tstep=1))
+@pytest.mark.parametrize("prm", [dict(tboundary=2), dict(shifting=3)], ids=["mdbc", "shifting"])
+def test_unsupported_floating_options_are_refused(prm):
+    c = cm.dam_break_floating(dp=0.05, params=cm.Parameters(**prm))
     with pytest.raises(RuntimeError, match="floating"):
         OracleSim(c)

@@ -38,7 +38,8 @@ no-penetration test): the piston with no-slip, a 0.1 m piston with no-slip
 and no penetration (acting from step ~25, when the piston has closed on the
 fluid), the flap at constant rotation with free slip, over 40 steps;
 shifting with mDBC2 (full shifting with no-slip on the dam break; shifting
-that spares fixed walls with the moving piston and free slip).
+that spares fixed walls with the moving piston and free slip); floating
+bodies under Verlet (3-D at rest, 2-D thrown spinning), 10 steps.
 """
 import numpy as np
 import pytest
@@ -67,7 +68,8 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_flume2d_dp0.005_flap_rot_mdbc", "ref_flume_dp0.01_piston_mdbc2",
             "ref_flume_dp0.01_piston_mdbc2_nopen", "ref_flume_dp0.01_flap_rot_mdbc2_freeslip",
             "ref_dambreak_dp0.03_mdbc2_shift_full",
-            "ref_flume_dp0.01_piston_mdbc2_freeslip_shift_nofixed"]
+            "ref_flume_dp0.01_piston_mdbc2_freeslip_shift_nofixed",
+            "ref_dambreak_dp0.03_floating_verlet", "ref_dambreak2d_dp0.005_floating_spin_verlet"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
