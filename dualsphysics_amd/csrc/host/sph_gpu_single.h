@@ -269,8 +269,8 @@ struct dsph_sim {
     PeriActive = unsigned(c.periactive) & 7u;
     if (PeriActive && c.casenmoving)
       throw Err(DSPH_E_ARG, "periodic boundaries with moving boundaries are not built");
-    if (PeriActive && mdbc2_on())
-      throw Err(DSPH_E_ARG, "periodic boundaries with mDBC no-slip or free slip are not built");
+    if (PeriActive && mdbc2_on() && cs.casenmoving)
+      throw Err(DSPH_E_ARG, "periodic boundaries with moving boundaries are not built");
     if (PeriActive && ShiftMode)
       throw Err(DSPH_E_ARG, "periodic boundaries with shifting are not built");
     ft.clear();

@@ -199,6 +199,11 @@ CASES = {
     "ref_dambreak2d_dp0.005_floating_spin_verlet": (
         "dam_break_floating_2d", 0.005, 10, (1, 5, 10),
         dict(tstep=1, velini=(1.0, 0.0, -0.5), omegaini=(0.0, 2.0, 0.0))),
+    # Periodic channel with mDBC2 no-slip (the periodic copies of the walls
+    # get their own modes and tangential velocities).
+    "ref_channel_dp0.02_periodic_mdbc2": ("periodic_channel", 0.02, 60, (1, 30, 60),
+                                          dict(periodic=1, tboundary=2, slipmode=2, tvisco=3,
+                                               visco=1e-4)),
     # Channel periodic in x with the fluid moving at 2 m/s: particles leave
     # through one face and re-enter through the other (Symplectic + DBC, and
     # Verlet + mDBC + laminar viscosity).

@@ -99,8 +99,7 @@ def test_xy_periodic_mdbc_matches_oracle(tvisco, sync_dt, cellmode):
     s.close()
 
 
-@pytest.mark.parametrize("prm", [dict(tboundary=2, slipmode=2, tvisco=3, visco=1e-4),
-                                 dict(shifting=3)], ids=["mdbc2", "shifting"])
+@pytest.mark.parametrize("prm", [dict(shifting=3)], ids=["shifting"])
 def test_unbuilt_periodic_combinations_are_refused(prm):
     from dualsphysics_amd.sim import DsphError, SphGpuSingle
     c = casemod.periodic_channel(dp=0.04, params=casemod.Parameters(periodic=1, **prm))

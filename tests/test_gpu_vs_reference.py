@@ -55,7 +55,8 @@ pytestmark = pytest.mark.gpu
                                   "ref_dambreak_dp0.03_mdbc2_shift_full",
                                   "ref_flume_dp0.01_piston_mdbc2_freeslip_shift_nofixed",
                                   "ref_dambreak_dp0.03_floating_verlet",
-                                  "ref_dambreak2d_dp0.005_floating_spin_verlet"])
+                                  "ref_dambreak2d_dp0.005_floating_spin_verlet",
+                                  "ref_channel_dp0.02_periodic_mdbc2"])
 def test_gpu_steps_follow_reference_cpu_build(name):
     from dualsphysics_amd.sim import SphGpuSingle
     fx, case = load_reference_fixture(name)

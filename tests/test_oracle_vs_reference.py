@@ -69,7 +69,8 @@ FIXTURES = ["ref_dambreak_dp0.03", "ref_dambreak_dp0.03_verlet_laminar",
             "ref_flume_dp0.01_piston_mdbc2_nopen", "ref_flume_dp0.01_flap_rot_mdbc2_freeslip",
             "ref_dambreak_dp0.03_mdbc2_shift_full",
             "ref_flume_dp0.01_piston_mdbc2_freeslip_shift_nofixed",
-            "ref_dambreak_dp0.03_floating_verlet", "ref_dambreak2d_dp0.005_floating_spin_verlet"]
+            "ref_dambreak_dp0.03_floating_verlet", "ref_dambreak2d_dp0.005_floating_spin_verlet",
+            "ref_channel_dp0.02_periodic_mdbc2"]
 
 
 @pytest.mark.parametrize("prec", [2, 0, 1], ids=["fp32_cpu_semantics", "fp32_gpu_semantics",
